@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the matrix-free phase-field hot path on one MI355X.
+
+Headline (one JSON line on stdout): element-apply GDOF/s of the periodic hex8 voxel
+operator on the 64^3 pore cube (a centred spherical pore of radius 12 voxels, void
+stiffness 1e-6, E = 3 GPa, nu = 0.35, Gc = 0.5 J/m^2, l0 = 2 voxels, voxel 1 um).  One
+step = one apply y = A(g(d)) x with the degradation formed in-kernel from nodal damage,
+inputs resident in HBM.  GDOF = 3 n + 6 output dofs per apply.  ``roofline`` prices the
+apply kernel's algorithmic bytes (57 B per node: u 24 + d 8 + phase 1 + r 24) against
+8 TB/s; ``cpu_baseline`` times the oracle's numpy restatement of the same apply on the
+host cores.  ``secondary`` holds the Q4 apply, the fused explicit step and the larger
+voxel sizes measured in the same run.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+HEX8_BYTES_PER_NODE = 57.0      # u 24 + d 8 + phase 1 + r 24
+Q4_APPLY_BYTES_PER_NODE = 40.0  # u 16 + d 8 + f 16
+Q4_STEP_BYTES_PER_NODE = 192.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, H r+w 64
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--tiles", type=str, default="", help="fixed hex8 tile tx,ty,kz (skips the autotuner)")
+    return ap.parse_args()
+
+
+def events_time(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return s.elapsed_time(e) / 1e3 / steps, wall
+
+
+def pore_cube(n, r_vox, device):
+    import numpy as np
+    from phast_amd import geometry as G
+    from phast_amd import solid3d as S
+    h = (1e-3,) * 3
+    phase = G.sphere_pore_radius((n, n, n), h, r_vox * 1e-3)
+    phases = {"mi_matrix": S.PhaseMaterial("mi_matrix", 3000.0, 0.35, 5e-4),
+              "pore": S.PhaseMaterial("pore", 3000.0 * 1e-6, 0.35, 5e-4)}
+    mats = S.build_element_arrays(phase, phases)
+    load = S.LoadCase("xx", 0, tuple(5e-4 * k for k in range(1, 41)), (1, 2, 3, 4, 5), ())
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid((n, n, n), h), mats, load, kappa=1e-6, ell=2e-3,
+                                           device=device)
+    return op, mats, phase
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
+              "(each box has one GPU; see the sharding design note).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from phast_amd import autotune
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(0)
+
+    # ------------------------------------------------------------------ headline
+    n = 64
+    op, mats, phase = pore_cube(n, 12, dev)
+    x = torch.as_tensor(rng.uniform(-1e-6, 1e-6, op.n_mech), dtype=torch.float64, device=dev)
+    x[-6:] = torch.tensor([5e-4, -1e-4, -1e-4, 0, 0, 0], dtype=torch.float64)
+    d = torch.as_tensor(rng.uniform(0, 1, op.n_red), dtype=torch.float64, device=dev)
+    if a.tiles:
+        op.set_tiles(*(int(v) for v in a.tiles.split(",")))
+        tune = {"fixed": list(op.tiles)}
+    else:
+        best, timings = autotune.tune_hex8(op, x, d)
+        tune = {"best": list(best), "ms": {",".join(map(str, k)): round(v, 5) for k, v in timings.items()}}
+    y = torch.empty_like(x)
+    t_apply, wall = events_time(lambda: op.mech_action_d(x, d), a.steps, a.warmup)
+    gdofs = op.n_mech / t_apply / 1e9
+    achieved = HEX8_BYTES_PER_NODE * op.n_red / t_apply / 1e9
+    flops_per_voxel = 1238.0
+    tflops = flops_per_voxel * op.n_elements / t_apply / 1e12
+
+    # ------------------------------------------------------------------ CPU baseline (oracle port)
+    cpu = None
+    if not a.no_cpu:
+        from oracle import hex8 as OH
+        try:
+            from threadpoolctl import threadpool_info
+            threads = max([p.get("num_threads", 1) for p in threadpool_info()] + [1])
+        except Exception:
+            threads = 1
+        oop = OH.VoxelOperator((n, n, n), (1e-3,) * 3, mats.stiffness_bank, mats.stiffness_index, mats.Gc, mats.lam,
+                               mats.mu, mats.driver.astype(np.int64), kappa=1e-6, ell=2e-3)
+        xc, dc = x.cpu().numpy(), d.cpu().numpy()
+        reps, times = 0, []
+        t_end = time.perf_counter() + 15.0
+        while (time.perf_counter() < t_end and reps < 25) or reps < 3:
+            t0 = time.perf_counter()
+            yc = oop.mech_action(xc, oop.degradation(dc))
+            times.append(time.perf_counter() - t0)
+            reps += 1
+        t_cpu = float(np.median(times[1:] if len(times) > 1 else times))
+        err = float(np.abs(y.copy_(op.mech_action_d(x, d)).cpu().numpy() - yc).max() / np.abs(yc).max())
+        cpu = {"value": op.n_mech / t_cpu / 1e9, "unit": "GDOF/s", "cores": int(threads), "kind": "port",
+               "sample": f"{reps} numpy applies (degradation + B-matrix stress pass + scatter-add) of the same 64^3 "
+                         f"state, median of the last {max(reps - 1, 1)}; max rel. diff to GPU {err:.1e}",
+               "cpu_model": _cpu_model()}
+
+    # ------------------------------------------------------------------ secondary measurements
+    sec = {}
+    if not a.no_secondary:
+        sec = secondary(rng, dev, a)
+
+    out = {
+        "metric": "element-apply GDOF/s (periodic hex8 voxel mech_action, 64^3 pore cube)",
+        "value": gdofs,
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t_apply * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random displacement fluctuation, macro strain 5e-4, damage U(0,1))",
+        "config": {"workload": "hex8 periodic voxel element apply, 64^3 cube, centred pore r=12 voxels "
+                               "(configs[1])", "n_nodes": op.n_red, "n_dofs": op.n_mech,
+                   "tiles": list(op.tiles), "autotune": tune},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "fp64_tflops": tflops, "fp64_frac": tflops / 78.6,
+                     "note": "traffic from rocprofv3 FETCH_SIZE/WRITE_SIZE is recorded in profiles/"},
+        "cpu_baseline": cpu,
+        "secondary": sec,
+        "wall_s_timed": wall,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def secondary(rng, dev, a):
+    import numpy as np
+    import torch
+    from phast_amd import autotune
+    from phast_amd import q4 as P
+    res = {}
+    T = lambda z: torch.as_tensor(z, dtype=torch.float64, device=dev)
+    mat = P.Material(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5)
+    for name, (nx, ny, W, H) in {"q4_c0_200x80": (200, 80, 100.0, 40.0),
+                                 "q4_8192x4096": (8192, 4096, 4096.0, 2048.0)}.items():
+        mesh = P.FEMMesh.rectangle(W, H, nx, ny)
+        n, E = mesh.n_nodes, mesh.n_elems
+        u = T(rng.uniform(-1e-4, 1e-4, (n, 2)))
+        d = T(rng.uniform(0, 1, n))
+        best, timings, fem = autotune.tune_q4(lambda t: P.FEMOperators(mesh, mat, tile=t), u, d)
+        steps = max(10, a.steps // (10 if n > 1e6 else 1))
+        t, _ = events_time(lambda: fem.internal_force(u, d), steps, 3)
+        entry = {"apply_GDOF_s": 2 * n / t / 1e9, "apply_GB_s": Q4_APPLY_BYTES_PER_NODE * n / t / 1e9,
+                 "apply_ms": t * 1e3, "tile": list(best)}
+        dyn = P.ExplicitDynamics(fem, dt=0.5 * fem.dt_cfl)
+        v, acc = T(rng.uniform(-1e-2, 1e-2, (n, 2))), T(np.zeros((n, 2)))
+        fext = T(np.zeros((n, 2)))
+        Hq = T(np.zeros((E, 4)))
+        t2, _ = events_time(lambda: dyn._launch(u, v, acc, d, fext, None, None, H_old=Hq), steps, 3)
+        entry.update({"explicit_step_ms": t2 * 1e3, "explicit_step_GB_s": Q4_STEP_BYTES_PER_NODE * n / t2 / 1e9})
+        res[name] = entry
+        del fem, dyn, u, d, v, acc, fext, Hq
+    for nv in (128, 256):
+        op, mats, _ = pore_cube(nv, 12 * nv // 64, dev)
+        x = T(rng.uniform(-1e-6, 1e-6, op.n_mech))
+        d = T(rng.uniform(0, 1, op.n_red))
+        best, _ = autotune.tune_hex8(op, x, d, reps=3)
+        t, _ = events_time(lambda: op.mech_action_d(x, d), 10, 2)
+        res[f"hex8_{nv}^3"] = {"apply_GDOF_s": op.n_mech / t / 1e9,
+                               "apply_GB_s": HEX8_BYTES_PER_NODE * op.n_red / t / 1e9,
+                               "apply_ms": t * 1e3, "tiles": list(best)}
+        del op, x, d
+        torch.cuda.empty_cache()
+    return res
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,227 @@
+/*
+ * phast_mi.h -- C ABI of the MI355X (gfx950) matrix-free phase-field hot path.
+ *
+ * Conventions
+ *   - Every entry point returns int: PHAST_OK (0) or a negative PHAST_ERR_* code;
+ *     phast_last_error() returns a thread-local message for the last failure.
+ *   - Array arguments are DEVICE pointers (float64 unless stated) that the caller owns;
+ *     outputs are written into caller-allocated buffers.  Scalars and descriptor
+ *     structs are host values.  Each call enqueues work on `stream` and returns
+ *     without synchronising unless the comment says otherwise (solvers synchronise to
+ *     read their convergence flags).
+ *   - Plans are opaque handles holding immutable per-problem geometry constants,
+ *     launch tables and solver work buffers.  One plan must not be used from two
+ *     streams at the same time.
+ *
+ * The reference (CEMS-Lab/PhAST) is pure Python/PyTorch; it has no FFI.  Each entry
+ * point names the reference method whose arithmetic it replaces (paths relative to
+ * the reference's src/phast).  The Python binding is phast_amd/_lib.py (ctypes).
+ */
+#ifndef PHAST_MI_H
+#define PHAST_MI_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHAST_OK 0
+#define PHAST_ERR_ARG (-1)               /* shape / value / pointer violation -> ValueError   */
+#define PHAST_ERR_HIP (-2)               /* HIP runtime failure -> RuntimeError               */
+#define PHAST_ERR_NONPOSITIVE (-3)       /* p.Ap <= 0 -> PCGBreakdownError                    */
+#define PHAST_ERR_NONFINITE (-4)         /* non-finite value in a solve -> PCGBreakdownError  */
+#define PHAST_ERR_NOT_CONVERGED (-5)     /* iteration cap reached (info carries the state)    */
+#define PHAST_ERR_UNSUPPORTED (-6)       /* configuration outside the built path              */
+
+/* solver termination codes written into the per-instance status arrays */
+#define PHAST_CG_RUNNING 0
+#define PHAST_CG_CONVERGED 1
+#define PHAST_CG_MAXITER 2
+#define PHAST_CG_NONPOSITIVE 3
+#define PHAST_CG_DIVERGED 4
+#define PHAST_CG_NONFINITE 5
+
+const char *phast_version(void);
+const char *phast_last_error(void);
+
+/* =====================================================================================
+ * 2D native-Q4 structured plates (one or many instances packed in one plan)
+ * ===================================================================================== */
+
+#define PHAST_SPLIT_AMOR 0
+#define PHAST_SPLIT_ISOTROPIC 1
+
+/* Host descriptor of one structured Q4 plate: (nx+1)(ny+1) nodes numbered
+ * j*(nx+1)+i and nx*ny cells numbered j*nx+i with corners (i,j),(i+1,j),(i+1,j+1),(i,j+1)
+ * (core/mesh_generator.py:1735-1752).  Geometry constants (gradN, N, wdetJ of the
+ * congruent cell, core/quad_elements.py:315-349) are supplied by the caller so that the
+ * library uses the same quadrature data as the host mesh precompute. */
+typedef struct phast_q4_desc {
+    int32_t nx, ny;          /* cells along x and y                                       */
+    double gradN[4][4][2];   /* [q][a][x|y] physical shape-function gradients              */
+    double N[4][4];          /* [q][a] shape values at the 2x2 Gauss points                */
+    double wdetJ[4];         /* [q] weight * det J                                         */
+    double lam, mu, kappa;   /* Lame lambda (split-dependent), shear modulus, 3D bulk modulus */
+    double nu;               /* Poisson ratio (plane-stress eps_zz)                        */
+    int32_t plane_stress;    /* 0 plane strain, 1 plane stress                             */
+    int32_t split;           /* PHAST_SPLIT_*                                              */
+    double eta_residual;     /* g(d) = (1-d)^2 (1-eta) + eta  (physics/material.py:529)     */
+    double Gc, l0;           /* AT2 damage constants                                       */
+    double dt;               /* explicit step size                                         */
+} phast_q4_desc;
+
+typedef struct phast_q4_plan phast_q4_plan;
+
+/* tile_x * tile_y must be 256; supported (64,4) (32,8) (16,16); (0,0) = default. */
+int phast_q4_plan_create(const phast_q4_desc *desc, int32_t n_inst, int32_t tile_x, int32_t tile_y,
+                         phast_q4_plan **out);
+int phast_q4_plan_destroy(phast_q4_plan *plan);
+/* total packed nodes / cells of the plan, and the node/cell offset of instance k */
+int phast_q4_plan_sizes(const phast_q4_plan *plan, int64_t *n_nodes, int64_t *n_cells);
+int phast_q4_plan_offsets(const phast_q4_plan *plan, int64_t *node_off, int64_t *cell_off);
+
+/* Scalar lumped mass M_a = sum_q N_a(q) wdetJ(q) gathered per node (core/mesh.py:348-352). */
+int phast_q4_lumped_mass(const phast_q4_plan *plan, double *M, hipStream_t stream);
+
+/* f_int (N,2) = internal force of u (N,2) at damage d (N,)
+ * (core/fem_operators.py:598-612 internal_force, 244-254, 614-653, 301-333, 681-699). */
+int phast_q4_internal_force(const phast_q4_plan *plan, const double *u, const double *d, double *f,
+                            hipStream_t stream);
+
+/* Gauss-point strain, three (E,4) arrays packed as eps[3][E][4] (fem_operators.py:244-254). */
+int phast_q4_strain(const phast_q4_plan *plan, const double *u, double *eps, hipStream_t stream);
+
+/* psi+ (E,4) of u (fem_operators.py:951-1006 compute_psi_plus / 1173 compute_driving_force);
+ * when H_old and H_new are non-null also H_new = max(H_old, psi+)
+ * (solvers/staggered_solver.py:804-845, physics/h_update.py:47-49). */
+int phast_q4_driving_force(const phast_q4_plan *plan, const double *u, double *psi,
+                           const double *H_old, double *H_new, hipStream_t stream);
+
+/* H_new = max(H_old, psi) elementwise over n entries (physics/h_update.py:47-49). */
+int phast_history_max(int64_t n, const double *H_old, const double *psi, double *H_new,
+                      hipStream_t stream);
+
+/* Nodal projection of a Gauss-point field (E,4): sum_e N w f / (M + 1e-30) (core/mesh.py:407-428). */
+int phast_q4_elem_to_node(const phast_q4_plan *plan, const double *field_q, const double *M,
+                          double *out, hipStream_t stream);
+
+/* One central-difference step with lumped mass, fused with psi+ and the history update
+ * (solvers/mechanics_solver.py:153-239 ExplicitDynamics.step; staggered_solver.py:1115-1146):
+ *   u1 = u + dt v + dt^2/2 a, u1[bc] = bc_vals
+ *   a1 = Minv (f_ext - f_int(u1, d)), v1 = v + dt/2 (a + a1), a1[bc] = v1[bc] = 0
+ *   psi = psi+(u1), H_new = max(H_old, psi)
+ * Minv is (N,) = 1/(rho M + 1e-30).  f_ext, bc_mask (uint8 (N,2)), bc_vals, psi may be null.
+ * Outputs must not alias inputs (neighbouring tiles read the old state). */
+int phast_q4_explicit_step(const phast_q4_plan *plan, const double *u, const double *v, const double *a,
+                           const double *d, const double *f_ext, const double *Minv,
+                           const uint8_t *bc_mask, const double *bc_vals,
+                           const double *H_old, double *u1, double *v1, double *a1, double *H_new,
+                           double *psi, hipStream_t stream);
+
+/* Damage operator pieces (solvers/damage_solver.py:2270-2301 _prepare_cg_q4, 1797-1838 _Ax_q4).
+ * Gc_cell (E,) is an optional per-cell Gc field (null: the descriptor's scalar Gc). */
+int phast_q4_damage_rhs_diag(const phast_q4_plan *plan, const double *H, const double *Gc_cell,
+                             double *b, double *diag, hipStream_t stream);
+int phast_q4_damage_apply(const phast_q4_plan *plan, const double *H, const double *Gc_cell,
+                          const double *x, double *y, hipStream_t stream);
+
+/* Box-constrained damage solve on [d_prev, 1] for every instance of the plan.
+ * method 0: projected PCG with step limiting and active set (damage_solver.py:3527-3775);
+ * method 1: unconstrained PCG then clamp (damage_solver.py:3377-3521).
+ * d_init (optional) is the initial iterate (defaults to d_prev); fixed (uint8, optional)
+ * pins nodes to fixed_vals (pf_dirichlet).  Outputs: d_out (N,), active_out (uint8 (N,),
+ * optional, projected method), and host arrays iters[n_inst], status[n_inst]
+ * (PHAST_CG_*).  Synchronises `stream` every `check_every` iterations. */
+int phast_q4_damage_solve(phast_q4_plan *plan, int32_t method, const double *H, const double *Gc_cell,
+                          const double *d_prev, const double *d_init, const uint8_t *fixed,
+                          const double *fixed_vals, double tol, int32_t max_iter,
+                          double *d_out, uint8_t *active_out, int32_t *iters, int32_t *status,
+                          hipStream_t stream);
+
+/* =====================================================================================
+ * 3D periodic hex8 voxel cells
+ * ===================================================================================== */
+
+typedef struct phast_hex8_plan phast_hex8_plan;
+
+/* Triply periodic nx*ny*nz voxel cell of edge lengths (hx,hy,hz)
+ * (solid3d/periodic.py:61-108, 193-221).  bank: host (n_bank,6,6) engineering-Voigt
+ * stiffnesses; bank_index: host (E,) int32 voxel -> bank entry; Gc, lam, mu: host (E,);
+ * driver: host (E,) int8 (0 none, 1 full, 2 spectral; solid3d/operator.py:1415-1444);
+ * kappa: residual stiffness; ell: length scale (solid3d/operator.py:229-251). */
+int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,
+                           const double *bank, int32_t n_bank, const int32_t *bank_index,
+                           const double *Gc, const double *lam, const double *mu, const int8_t *driver,
+                           double kappa, double ell, phast_hex8_plan **out);
+int phast_hex8_plan_destroy(phast_hex8_plan *plan);
+/* Launch shape of the marching apply kernels: tile_x x tile_y node columns per block
+ * ((16,16) (16,8) (8,8) (32,8)) walking kz node layers.  Default (16,8,8). */
+int phast_hex8_set_tiles(phast_hex8_plan *plan, int32_t tile_x, int32_t tile_y, int32_t kz);
+
+/* g (E,8) = (1-kappa)(1-d_q)^2 + kappa, d_q = N d_e (solid3d/operator.py:939-946). */
+int phast_hex8_degradation(const phast_hex8_plan *plan, const double *d, double *g, hipStream_t stream);
+
+/* y (3n+6) = A(g) x for x = [u~ (3n), E (6)] (solid3d/operator.py:1108-1194 _stress_pass /
+ * mech_action).  g (E,8) may be null (undamaged).  The macro-strain rows r_E use a
+ * deterministic two-pass reduction. */
+int phast_hex8_mech_action(const phast_hex8_plan *plan, const double *x, const double *g, double *y,
+                           hipStream_t stream);
+/* Same action with g formed in-kernel from nodal damage d (n,) (fused degradation). */
+int phast_hex8_mech_action_d(const phast_hex8_plan *plan, const double *x, const double *d, double *y,
+                             hipStream_t stream);
+/* Jacobi diagonal (3n+6) of A(g) (solid3d/operator.py:1196-1254). */
+int phast_hex8_mech_diagonal(const phast_hex8_plan *plan, const double *g, double *diag,
+                             hipStream_t stream);
+/* r_E (6) and elastic energy (1) into out7 (device, 7 doubles) (operator.py:1303-1340). */
+int phast_hex8_stress_sums(const phast_hex8_plan *plan, const double *x, const double *g, double *out7,
+                           hipStream_t stream);
+/* Y (E,8) undamaged driving energy by driver code (operator.py:1415-1444; splits.py:68-161,
+ * closed-form eigenvalues of splits.py:93-125). */
+int phast_hex8_driving_force(const phast_hex8_plan *plan, const double *x, double *Y, hipStream_t stream);
+
+/* AT2 damage operator, multiplied by `scale` (operator.py:1450-1509). H may be null. */
+int phast_hex8_damage_action(const phast_hex8_plan *plan, const double *d, const double *H, double scale,
+                             double *y, hipStream_t stream);
+int phast_hex8_damage_rhs(const phast_hex8_plan *plan, const double *H, double scale, double *b,
+                          hipStream_t stream);
+int phast_hex8_damage_diagonal(const phast_hex8_plan *plan, const double *H, double scale, double *diag,
+                               hipStream_t stream);
+
+/* Device-resident solves ---------------------------------------------------------------
+ * solve_constrained(mech_action(., g), diag, fixed_mask, fixed_values, rhs, x0, rtol, atol,
+ * maxiter, check_every) of solid3d/linalg.py:223-287 with the Jacobi PCG of linalg.py:99-220
+ * (same flag semantics: alpha/beta masked on device, flags read every check_every).
+ * g (E,8) or d (n,) (fused degradation) or neither (undamaged); fixed uint8 (n_mech,);
+ * rhs and x0 may be null.  info[5] = iterations, converged, rhs_norm, residual_norm,
+ * relative_residual.  Breakdowns return PHAST_ERR_NONPOSITIVE / PHAST_ERR_NONFINITE. */
+int phast_hex8_mech_solve(const phast_hex8_plan *plan, const double *g, const double *d, const uint8_t *fixed,
+                          const double *fixed_values, const double *rhs, const double *x0, const double *diag,
+                          double rtol, double atol, int32_t maxiter, int32_t check_every, int64_t n_mech,
+                          double *x_out, double *info, hipStream_t stream);
+
+/* solve_h8_scalar_dirichlet_pcg (solvers/box_kkt_h8.py:350-432) of the damage system
+ * scale * A(H) (operator.py:1496-1509); b, diag are the scaled rhs and diagonal; fixed
+ * uint8 (n,), values (n,), initial (n,) or null.  info[5] = iterations, converged,
+ * initial_residual_norm, residual_norm, relative_residual. */
+int phast_hex8_damage_dirichlet_pcg(const phast_hex8_plan *plan, const double *H, double scale, const double *b,
+                                    const double *diag, const uint8_t *fixed, const double *values,
+                                    const double *initial, double tol, int32_t max_iter, int64_t n, double *out,
+                                    double *info, hipStream_t stream);
+
+/* One primal active-set bookkeeping pass of solve_h8_scalar_box_kkt (box_kkt_h8.py:672-733):
+ * grad = scale A(H) cand - b; counts[5] = added lower/upper, released lower/upper, free;
+ * with apply != 0 the masks are updated in place. */
+int phast_hex8_box_update(const phast_hex8_plan *plan, const double *H, double scale, const double *b,
+                          const double *cand, const double *lower, const double *upper, uint8_t *lower_active,
+                          uint8_t *upper_active, const uint8_t *prescribed, double kkt_tol, int32_t apply, int64_t n,
+                          double *grad, int64_t *counts, hipStream_t stream);
+
+/* Frees the process-wide solver scratch of the 3D solves. */
+int phast_release_solver_scratch(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHAST_MI_H */

@@ -1,0 +1,388 @@
+"""Oracle (numpy, float64) for the reference's periodic hex8 voxel path.
+
+TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.
+
+Restates, with the reference's own data layout (local H8 node order, engineering-strain
+``B (8,6,24)`` matrices, element dof gather and scatter-add), the files
+``solid3d/periodic.py``, ``core/hex8.py``, ``solid3d/operator.py``, ``solid3d/splits.py``,
+``solid3d/linalg.py`` and ``solvers/box_kkt_h8.py`` (paths relative to ``src/phast``).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+H8_SIGNS = np.array([(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1),
+                     (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)], dtype=np.float64)  # core/hex8.py:16-36
+
+
+def h8_gauss_points():
+    """Tensor Gauss points, xi fastest, weights 1 (core/hex8.py:157-179)."""
+    a = 1.0 / math.sqrt(3.0)
+    p = np.array([-a, a])
+    zeta, eta, xi = np.meshgrid(p, p, p, indexing="ij")
+    return np.stack((xi, eta, zeta), axis=-1).reshape(-1, 3), np.ones(8)
+
+
+def reference_voxel(h):
+    """N (8q,8a), gradN (8,8,3), w (8), B (8,6,24), M, L (solid3d/periodic.py:193-221; hex8.py:182-293)."""
+    hx, hy, hz = h
+    pts, wts = h8_gauss_points()
+    coords = (H8_SIGNS + 1.0) / 2.0 * np.array([hx, hy, hz])
+    N = np.zeros((8, 8))
+    grad = np.zeros((8, 8, 3))
+    w = np.zeros(8)
+    for q in range(8):
+        xi, eta, zeta = pts[q]
+        s = H8_SIGNS
+        N[q] = 0.125 * (1 + s[:, 0] * xi) * (1 + s[:, 1] * eta) * (1 + s[:, 2] * zeta)
+        dN = np.stack([0.125 * s[:, 0] * (1 + s[:, 1] * eta) * (1 + s[:, 2] * zeta),
+                       0.125 * (1 + s[:, 0] * xi) * s[:, 1] * (1 + s[:, 2] * zeta),
+                       0.125 * (1 + s[:, 0] * xi) * (1 + s[:, 1] * eta) * s[:, 2]], axis=-1)
+        J = np.einsum("ar,aj->rj", dN, coords)
+        grad[q] = np.einsum("ar,jr->aj", dN, np.linalg.inv(J))
+        w[q] = wts[q] * np.linalg.det(J)
+    B = np.zeros((8, 6, 24))
+    for a in range(8):
+        gx, gy, gz = grad[:, a, 0], grad[:, a, 1], grad[:, a, 2]
+        d = 3 * a
+        B[:, 0, d], B[:, 1, d + 1], B[:, 2, d + 2] = gx, gy, gz
+        B[:, 3, d + 1], B[:, 3, d + 2] = gz, gy
+        B[:, 4, d], B[:, 4, d + 2] = gz, gx
+        B[:, 5, d], B[:, 5, d + 1] = gy, gx
+    M = np.einsum("q,qa,qb->ab", w, N, N)
+    L = np.einsum("q,qaj,qbj->ab", w, grad, grad)
+    return N, grad, w, B, 0.5 * (M + M.T), 0.5 * (L + L.T)
+
+
+def periodic_map(shape):
+    """element_nodes (E,8) in local H8 order and element_dofs (E,24) (solid3d/periodic.py:61-108)."""
+    nx, ny, nz = shape
+    ke, je, ie = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    off = ((H8_SIGNS + 1) / 2).astype(np.int64)
+    i = (ie.reshape(-1, 1) + off[:, 0]) % nx
+    j = (je.reshape(-1, 1) + off[:, 1]) % ny
+    k = (ke.reshape(-1, 1) + off[:, 2]) % nz
+    nodes = i + nx * (j + ny * k)
+    dofs = (3 * nodes[..., None] + np.arange(3)).reshape(-1, 24)
+    return nodes, dofs
+
+
+def scatter(index, values, n):
+    return np.bincount(index.ravel(), weights=values.ravel(), minlength=n).astype(np.float64)
+
+
+def isotropic_stiffness_3d(E, nu):
+    """core/hex8.py:296-320."""
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    C = np.zeros((6, 6))
+    C[:3, :3] = lam
+    C[0, 0] += 2 * mu
+    C[1, 1] += 2 * mu
+    C[2, 2] += 2 * mu
+    C[3, 3] = C[4, 4] = C[5, 5] = mu
+    return C
+
+
+def symmetric_eigvalsh_3x3(A):
+    """Closed-form ascending eigenvalues (solid3d/splits.py:93-125)."""
+    a11, a22, a33 = A[..., 0, 0], A[..., 1, 1], A[..., 2, 2]
+    a12, a13, a23 = A[..., 0, 1], A[..., 0, 2], A[..., 1, 2]
+    q = (a11 + a22 + a33) / 3.0
+    b11, b22, b33 = a11 - q, a22 - q, a33 - q
+    p1 = a12 ** 2 + a13 ** 2 + a23 ** 2
+    p = np.sqrt((b11 ** 2 + b22 ** 2 + b33 ** 2 + 2.0 * p1) / 6.0)
+    det_b = b11 * (b22 * b33 - a23 ** 2) - a12 * (a12 * b33 - a23 * a13) + a13 * (a12 * a23 - b22 * a13)
+    iso = p <= np.finfo(np.float64).tiny
+    ps = np.where(iso, 1.0, p)
+    r = np.clip(det_b / (2.0 * ps ** 3), -1.0, 1.0)
+    phi = np.arccos(r) / 3.0
+    largest = q + 2.0 * p * np.cos(phi)
+    smallest = q + 2.0 * p * np.cos(phi + 2.0 * math.pi / 3.0)
+    middle = 3.0 * q - largest - smallest
+    out = np.stack((smallest, middle, largest), axis=-1)
+    return np.where(iso[..., None], q[..., None], out)
+
+
+def engineering_strain_to_tensor(e):
+    """core/voigt.py:12-30 (xx,yy,zz,yz,xz,xy; halved shears)."""
+    T = np.zeros(e.shape[:-1] + (3, 3))
+    T[..., 0, 0], T[..., 1, 1], T[..., 2, 2] = e[..., 0], e[..., 1], e[..., 2]
+    T[..., 1, 2] = T[..., 2, 1] = 0.5 * e[..., 3]
+    T[..., 0, 2] = T[..., 2, 0] = 0.5 * e[..., 4]
+    T[..., 0, 1] = T[..., 1, 0] = 0.5 * e[..., 5]
+    return T
+
+
+def spectral_tensile_energy(eps, lam, mu, method="analytic"):
+    """Miehe tensile energy (solid3d/splits.py:146-161)."""
+    T = engineering_strain_to_tensor(eps)
+    ev = symmetric_eigvalsh_3x3(T) if method == "analytic" else np.linalg.eigvalsh(T)
+    trp = np.maximum(eps[..., :3].sum(axis=-1), 0.0)
+    return 0.5 * lam * trp ** 2 + mu * (np.maximum(ev, 0.0) ** 2).sum(axis=-1)
+
+
+@dataclass
+class VoxelOperator:
+    """Restatement of PeriodicVoxelPhaseFieldOperator, AT2, no closure (solid3d/operator.py)."""
+    shape: tuple
+    h: tuple
+    bank: np.ndarray       # (K,6,6)
+    index: np.ndarray      # (E,)
+    Gc: np.ndarray         # (E,)
+    lam: np.ndarray
+    mu: np.ndarray
+    driver: np.ndarray     # (E,) int
+    kappa: float
+    ell: float
+    pinned_node: int = 0
+    controlled: int = 0
+    fixed_zero: tuple = ()
+
+    def __post_init__(self):
+        self.nodes, self.dofs = periodic_map(self.shape)
+        self.N, self.grad, self.w, self.B, M, L = reference_voxel(self.h)
+        self.B_flat = self.B.transpose(2, 0, 1).reshape(24, 48)   # operator.py:242
+        self.K_damage = M / self.ell + self.ell * L                # operator.py:246
+        self.n_red = int(np.prod(self.shape))
+        self.n_mech = 3 * self.n_red + 6
+        self.E = self.n_red
+        hx, hy, hz = self.h
+        self.volume = float(np.prod(self.shape) * hx * hy * hz)
+        self.damage_scale = self.ell / (float(self.Gc.min()) * hx * hy * hz)
+        self.c_history = 2.0 * (1.0 - self.kappa)
+
+    def degradation(self, d):
+        """operator.py:939-946; splits.py:56-59."""
+        d_q = d[self.nodes] @ self.N.T
+        return (1.0 - self.kappa) * (1.0 - d_q) ** 2 + self.kappa
+
+    def qp_strain(self, x):
+        u, E = x[:3 * self.n_red], x[3 * self.n_red:]
+        return (u[self.dofs] @ self.B_flat).reshape(-1, 8, 6) + E  # operator.py:948-952
+
+    def stress_pass(self, x, g=None, assemble=True):
+        """operator.py:1108-1169 (one batch): r_u, r_E, energy."""
+        eps = self.qp_strain(x)
+        C = self.bank[self.index]
+        sigma = np.einsum("eqm,ekm->eqk", eps, C)
+        if g is not None:
+            sigma = g[..., None] * sigma
+        weighted = sigma * self.w[None, :, None]
+        r_E = weighted.sum(axis=(0, 1))
+        energy = 0.5 * (weighted * eps).sum()
+        r_u = None
+        if assemble:
+            local = weighted.reshape(-1, 48) @ self.B_flat.T
+            r_u = scatter(self.dofs, local, 3 * self.n_red)
+        return r_u, r_E, energy
+
+    def mech_action(self, x, g=None):
+        r_u, r_E, _ = self.stress_pass(x, g)
+        return np.concatenate((r_u, r_E))
+
+    def mech_diagonal(self, g=None):
+        """operator.py:1196-1254."""
+        BB = np.einsum("qia,qja->qaij", self.B, self.B).reshape(8 * 24, 36)
+        C = self.bank[self.index]
+        gw = np.broadcast_to(self.w, (self.E, 8)) if g is None else g * self.w
+        per_qp = (C.reshape(self.E, 36) @ BB.T).reshape(self.E, 8, 24)
+        local = (gw[..., None] * per_qp).sum(axis=1)
+        diag_u = scatter(self.dofs, local, 3 * self.n_red)
+        diag_E = (gw.sum(axis=1)[:, None] * np.diagonal(C, axis1=1, axis2=2)).sum(axis=0)
+        return np.concatenate((diag_u, diag_E))
+
+    def macro_stress(self, x, g=None):
+        return self.stress_pass(x, g, assemble=False)[1] / self.volume
+
+    def elastic_energy(self, x, g=None):
+        return self.stress_pass(x, g, assemble=False)[2]
+
+    def driving_force(self, x, method="analytic"):
+        """operator.py:1415-1444 (drivers none / full / spectral)."""
+        eps = self.qp_strain(x)
+        Y = np.zeros((self.E, 8))
+        full = self.driver == 1
+        if full.any():
+            C = self.bank[self.index[full]][:, None]
+            Y[full] = 0.5 * (eps[full] * np.einsum("eqkm,eqm->eqk", np.broadcast_to(C, (C.shape[0], 8, 6, 6)),
+                                                   eps[full])).sum(-1)
+        spec = self.driver == 2
+        if spec.any():
+            Y[spec] = spectral_tensile_energy(eps[spec], self.lam[spec, None], self.mu[spec, None], method)
+        return Y
+
+    def damage_action(self, d, H=None):
+        """operator.py:1450-1466."""
+        d_e = d[self.nodes]
+        y = self.Gc[:, None] * (d_e @ self.K_damage)
+        if H is not None:
+            d_q = d_e @ self.N.T
+            y = y + self.c_history * ((H * self.w) * d_q) @ self.N
+        return scatter(self.nodes, y, self.n_red)
+
+    def damage_rhs(self, H):
+        return scatter(self.nodes, self.c_history * (H * self.w) @ self.N, self.n_red)
+
+    def damage_diagonal(self, H=None):
+        y = self.Gc[:, None] * np.diagonal(self.K_damage)[None, :]
+        if H is not None:
+            y = y + self.c_history * (H * self.w) @ (self.N ** 2)
+        return scatter(self.nodes, y, self.n_red)
+
+    def mech_fixed_mask(self):
+        m = np.zeros(self.n_mech, dtype=bool)
+        m[3 * self.pinned_node:3 * self.pinned_node + 3] = True
+        m[3 * self.n_red + self.controlled] = True
+        for c in self.fixed_zero:
+            m[3 * self.n_red + c] = True
+        return m
+
+
+class PCGBreakdownError(RuntimeError):
+    pass
+
+
+def pcg(action, rhs, diag, x0=None, rtol=1e-8, atol=0.0, maxiter=5000, check_every=10):
+    """solid3d/linalg.py:99-220 (flag semantics)."""
+    b_norm = float(np.linalg.norm(rhs))
+    if not np.all(np.isfinite(rhs)):
+        raise PCGBreakdownError("PCG right-hand side contains a non-finite value")
+    if not np.all(diag > 0):
+        raise PCGBreakdownError("PCG diagonal has a non-positive entry (non-positive curvature)")
+    if b_norm == 0.0:
+        return np.zeros_like(rhs), (0, True, 0.0, 0.0, 0.0)
+    thr = max(rtol * b_norm, atol)
+    if x0 is None:
+        x, r, r_norm = np.zeros_like(rhs), rhs.copy(), b_norm
+    else:
+        x = x0.copy()
+        r = rhs - action(x)
+        r_norm = float(np.linalg.norm(r))
+    if r_norm <= thr:
+        return x, (0, True, b_norm, r_norm, r_norm / b_norm)
+    inv = 1.0 / diag
+    z = r * inv
+    p = z.copy()
+    rz = float(r @ z)
+    active, breakdown, done, rr = True, False, 0, float(r @ r)
+    for it in range(1, maxiter + 1):
+        Ap = action(p)
+        pAp = float(p @ Ap)
+        positive = pAp > 0.0
+        breakdown = breakdown or (active and not positive)
+        step = active and positive
+        alpha = rz / pAp if step else 0.0
+        x = x + alpha * p
+        r = r - alpha * Ap
+        done += int(step)
+        z = r * inv
+        rz_new = float(r @ z)
+        rr = float(r @ r)
+        active = active and rr > thr * thr
+        beta = rz_new / rz if (active and rz > 0.0) else 0.0
+        p = z + beta * p
+        rz = rz_new
+        if it % check_every == 0 or it == maxiter:
+            if breakdown:
+                raise PCGBreakdownError(f"PCG met non-positive curvature by iteration {it}")
+            r_norm = math.sqrt(rr)
+            if not active:
+                return x, (done, True, b_norm, r_norm, r_norm / b_norm)
+    return x, (maxiter, False, b_norm, r_norm, r_norm / b_norm)
+
+
+def solve_constrained(action, diag, fixed_mask, fixed_values, rhs=None, x0=None, **kw):
+    """solid3d/linalg.py:223-287."""
+    free = ~fixed_mask
+    x_c = np.where(fixed_mask, fixed_values, 0.0)
+    image = action(x_c)
+    b = -image if rhs is None else rhs - image
+    b_f = np.where(free, b, 0.0)
+    diag_f = np.where(free, diag, 1.0)
+    x0_f = None if x0 is None else np.where(free, x0, 0.0)
+    y, info = pcg(lambda v: np.where(free, action(v), 0.0), b_f, diag_f, x0=x0_f, **kw)
+    return np.where(free, y, x_c), info
+
+
+def scalar_pcg(action, rhs, diag, initial, tol, max_iter):
+    """solvers/box_kkt_h8.py:256-347 on the reduced (free) system."""
+    x = np.zeros_like(rhs) if initial is None else initial.copy()
+    res = rhs - action(x)
+    n0 = float(np.linalg.norm(res))
+    ref = max(n0, 1.0)
+    if n0 <= tol * ref:
+        return x, 0, True
+    pre = res / diag
+    dr = pre.copy()
+    rz = float(res @ pre)
+    if not rz > 0:
+        raise RuntimeError("PCG encountered a non-positive initial residual product")
+    for it in range(1, max_iter + 1):
+        im = action(dr)
+        den = float(dr @ im)
+        if not den > 0:
+            raise RuntimeError("PCG operator is not positive definite on the free space")
+        a = rz / den
+        x = x + a * dr
+        res = res - a * im
+        if float(np.linalg.norm(res)) <= tol * ref:
+            return x, it, True
+        pre = res / diag
+        nd = float(res @ pre)
+        if not nd > 0:
+            raise RuntimeError("PCG encountered a non-positive residual product")
+        dr = pre + (nd / rz) * dr
+        rz = nd
+    return x, max_iter, False
+
+
+def dirichlet_pcg(action, rhs, diag, fixed, values, initial=None, tol=1e-10, max_iter=4000):
+    """solvers/box_kkt_h8.py:350-432."""
+    known = np.where(fixed, values, 0.0)
+    free = ~fixed
+    if not free.any():
+        return known, 0, True
+    red_rhs = (rhs - action(known))[free]
+
+    def red_action(v):
+        full = np.zeros_like(rhs)
+        full[free] = v
+        return action(full)[free]
+
+    sol, its, conv = scalar_pcg(red_action, red_rhs, diag[free], None if initial is None else initial[free], tol,
+                                max_iter)
+    out = known.copy()
+    out[free] = sol
+    return out, its, conv
+
+
+def box_kkt(action, rhs, diag, lower, upper, initial=None, lin_tol=1e-10, kkt_tol=1e-9, max_lin=4000, max_as=64):
+    """solvers/box_kkt_h8.py:603-778 (primal active set); returns (d, lower_active, upper_active, its, trace)."""
+    presc = lower == upper
+    la = np.zeros(rhs.shape, dtype=bool)
+    ua = np.zeros(rhs.shape, dtype=bool)
+    cand_init, total, trace = initial, 0, []
+    for it in range(1, max_as + 1):
+        fixed = la | ua | presc
+        values = np.where(ua, upper, lower)
+        cand, its, conv = dirichlet_pcg(action, rhs, diag, fixed, values, cand_init, lin_tol, max_lin)
+        total += its
+        g = action(cand) - rhs
+        free = ~fixed
+        add_l, add_u = free & (cand < lower), free & (cand > upper)
+        rel_l, rel_u = la & (g < -kkt_tol), ua & (g > kkt_tol)
+        trace.append((int(add_l.sum()), int(add_u.sum()), int(rel_l.sum()), int(rel_u.sum()), its))
+        if not conv:
+            raise RuntimeError("H8 scalar box-KKT PCG did not converge")
+        if not (add_l | add_u | rel_l | rel_u).any():
+            return cand, la, ua, it, total, trace
+        if it == max_as:
+            break
+        la = (la | add_l) & ~rel_l
+        ua = (ua | add_u) & ~rel_u
+        cand_init = cand
+    raise RuntimeError("H8 scalar box-KKT active set did not stabilise")

@@ -1,0 +1,51 @@
+"""Launch-shape autotuner: times the candidate tile shapes of an apply on the live device
+and keeps the fastest.  Results are cached per (kind, problem size) for the process."""
+from __future__ import annotations
+
+import torch
+
+HEX8_TILES = [(16, 8, 8), (16, 8, 16), (16, 16, 8), (8, 8, 8), (8, 8, 16), (32, 8, 8), (16, 8, 4), (8, 8, 4)]
+Q4_TILES = [(64, 4), (32, 8), (16, 16)]
+
+_cache: dict = {}
+
+
+def _time(fn, reps: int) -> float:
+    fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def tune_hex8(op, x: torch.Tensor, d: torch.Tensor, reps: int = 10, candidates=HEX8_TILES):
+    """Pick the marching-kernel tile (tx, ty, kz) of ``op`` for the fused apply."""
+    key = ("hex8", op.grid.shape)
+    if key in _cache:
+        op.set_tiles(*_cache[key][0])
+        return _cache[key]
+    timings = {}
+    for t in candidates:
+        if t[2] > op.grid.shape[2]:
+            continue
+        op.set_tiles(*t)
+        timings[t] = _time(lambda: op.mech_action_d(x, d), reps)
+    best = min(timings, key=timings.get)
+    op.set_tiles(*best)
+    _cache[key] = (best, timings)
+    return best, timings
+
+
+def tune_q4(make_fem, u: torch.Tensor, d: torch.Tensor, reps: int = 10, candidates=Q4_TILES):
+    """Pick the Q4 node tile; ``make_fem(tile)`` builds an FEMOperators with that tile."""
+    timings, fems = {}, {}
+    for t in candidates:
+        fem = make_fem(t)
+        fems[t] = fem
+        timings[t] = _time(lambda: fem.internal_force(u, d), reps)
+    best = min(timings, key=timings.get)
+    return best, timings, fems[best]

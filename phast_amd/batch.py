@@ -1,0 +1,109 @@
+"""Many explicit phase-field plates of uneven sizes advanced together.
+
+All plates of a batch live packed in one set of HBM arrays with per-instance offsets;
+every kernel launch of a step (the fused mechanics/psi+/history kernel and each
+kernel of the projected damage CG) covers the whole batch through the plan's
+block -> (instance, tile) table, and each instance keeps its own CG scalars, so the
+batched result of an instance is bitwise the result of running it alone.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr, stream
+from .q4 import F64, FEMMesh, Material, Q4Plan, q4_desc
+
+
+class BatchedExplicit:
+    """Explicit staggered steps (staggered_solver.py:1115-1165 with damage_every = 1) for a
+    batch of plates.  ``items`` is a list of ``(FEMMesh, Material)``; ``Gc_fields`` an
+    optional list of per-cell Gc arrays (E_k,); ``f_ext`` an optional list of (N_k,2)."""
+
+    def __init__(self, items, *, dt_factor: float = 0.5, dts=None, Gc_fields=None, f_ext=None, tol: float = 1e-5,
+                 max_iter: int = 5000, bounds_method: str = "projected_cg", tile=(64, 4)):
+        dev = _lib.require_device()
+        self.device = dev
+        self.meshes = [m for m, _ in items]
+        self.materials = [mat for _, mat in items]
+        cfl = [m.h_min / mat.c_p for m, mat in items]
+        self.dt = list(dts) if dts is not None else [dt_factor * c for c in cfl]
+        self.plan = Q4Plan([q4_desc(m, mat, dt) for (m, mat), dt in zip(items, self.dt)], tile=tile)
+        self.tol, self.max_iter = float(tol), int(max_iter)
+        self.method = 0 if bounds_method == "projected_cg" else 1
+        N, E = self.plan.n_nodes, self.plan.n_cells
+        z2 = lambda: torch.zeros((N, 2), dtype=F64, device=dev)
+        self.u, self.v, self.a = z2(), z2(), z2()
+        self.u1, self.v1, self.a1 = z2(), z2(), z2()
+        self.d = torch.zeros(N, dtype=F64, device=dev)
+        self.d1 = torch.zeros_like(self.d)
+        self.H = torch.zeros((E, 4), dtype=F64, device=dev)
+        self.H1 = torch.zeros_like(self.H)
+        self.psi = torch.zeros_like(self.H)
+        self.f_ext = z2()
+        M = torch.empty(N, dtype=F64, device=dev)
+        check(_lib.lib().phast_q4_lumped_mass(self.plan.handle, ptr(M), stream()), "lumped mass")
+        rho = torch.empty(N, dtype=F64, device=dev)
+        for k, mat in enumerate(self.materials):
+            rho[self._nodes(k)] = mat.rho
+        self.Minv = 1.0 / (rho * M + 1e-30)
+        self.Gc = None
+        if Gc_fields is not None:
+            self.Gc = torch.empty(E, dtype=F64, device=dev)
+            for k, g in enumerate(Gc_fields):
+                self.Gc[self._cells(k)] = torch.as_tensor(np.asarray(g, dtype=np.float64), device=dev)
+        if f_ext is not None:
+            for k, f in enumerate(f_ext):
+                if f is not None:
+                    self.f_ext[self._nodes(k)] = torch.as_tensor(np.asarray(f, dtype=np.float64), device=dev)
+        self.iters = (C.c_int32 * self.plan.n_inst)()
+        self.status = (C.c_int32 * self.plan.n_inst)()
+        self.last_iters = np.zeros(self.plan.n_inst, dtype=np.int64)
+        self.steps = 0
+
+    @property
+    def n_inst(self) -> int:
+        return self.plan.n_inst
+
+    def _nodes(self, k):
+        s = self.plan.node_off[k]
+        return slice(s, s + self.meshes[k].n_nodes)
+
+    def _cells(self, k):
+        s = self.plan.cell_off[k]
+        return slice(s, s + self.meshes[k].n_elems)
+
+    def load_states(self, states):
+        """states[k] = (u, v, a, d, H) device tensors of instance k."""
+        for k, (u, v, a, d, H) in enumerate(states):
+            n, c = self._nodes(k), self._cells(k)
+            self.u[n], self.v[n], self.a[n], self.d[n], self.H[c] = u, v, a, d, H
+
+    def set_damage(self, k: int, d):
+        self.d[self._nodes(k)] = d
+
+    def instance_state(self, k: int):
+        n, c = self._nodes(k), self._cells(k)
+        return self.u[n], self.v[n], self.a[n], self.d[n], self.H[c]
+
+    def step(self, n_steps: int = 1) -> None:
+        L = _lib.lib()
+        s = stream()
+        for _ in range(n_steps):
+            check(L.phast_q4_explicit_step(self.plan.handle, ptr(self.u), ptr(self.v), ptr(self.a), ptr(self.d),
+                                           ptr(self.f_ext), ptr(self.Minv), None, None, ptr(self.H), ptr(self.u1),
+                                           ptr(self.v1), ptr(self.a1), ptr(self.H1), ptr(self.psi), s),
+                  "batched explicit step")
+            self.u, self.u1 = self.u1, self.u
+            self.v, self.v1 = self.v1, self.v
+            self.a, self.a1 = self.a1, self.a
+            self.H, self.H1 = self.H1, self.H
+            check(L.phast_q4_damage_solve(self.plan.handle, self.method, ptr(self.H), ptr(self.Gc), ptr(self.d), None,
+                                          None, None, self.tol, self.max_iter, ptr(self.d1), None, self.iters,
+                                          self.status, s), "batched damage solve")
+            self.d, self.d1 = self.d1, self.d
+            self.last_iters = np.frombuffer(self.iters, dtype=np.int32).astype(np.int64)
+            self.steps += 1

@@ -1,0 +1,91 @@
+// Shared helpers of the gfx950 kernels: error reporting, wave64 reductions, launch checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <cmath>
+
+#include "../../include/phast_mi.h"
+
+namespace phast {
+
+void set_error(const char *fmt, ...);
+
+#define PHAST_HIP(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ::phast::set_error("%s:%d: %s -> %s", __FILE__, __LINE__, #expr,             \
+                               hipGetErrorString(e_));                                   \
+            return PHAST_ERR_HIP;                                                        \
+        }                                                                                \
+    } while (0)
+
+#define PHAST_REQUIRE(cond, ...)                                                         \
+    do {                                                                                 \
+        if (!(cond)) {                                                                   \
+            ::phast::set_error(__VA_ARGS__);                                             \
+            return PHAST_ERR_ARG;                                                        \
+        }                                                                                \
+    } while (0)
+
+// Launch error check after hipLaunchKernelGGL / <<<>>>.
+#define PHAST_LAUNCHED() PHAST_HIP(hipGetLastError())
+
+#define PHAST_TRY0(x)                  \
+    do {                               \
+        int rc0_ = (x);                \
+        if (rc0_ != PHAST_OK) return rc0_; \
+    } while (0)
+
+constexpr int kWave = 64;
+
+// Deterministic wave64 butterfly reductions (fixed pairing order on every run).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Block-wide reductions for NT threads (NT a multiple of 64).  `sh` holds NT/64 doubles.
+// The result is valid in every thread.  Caller must not reuse `sh` before a barrier.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NT / kWave; ++i) s += sh[i];
+    return s;
+}
+template <int NT>
+__device__ __forceinline__ double block_min(double v, double *sh) {
+    v = wave_min(v);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = sh[0];
+#pragma unroll
+    for (int i = 1; i < NT / kWave; ++i) s = fmin(s, sh[i]);
+    return s;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace phast

@@ -1,0 +1,966 @@
+// Triply periodic hex8 voxel kernels for gfx950 (FP64 throughout).
+//
+// HBM layout: reduced node n = i + nx (j + ny k) (solid3d/periodic.py:61-108); the mechanics
+// vector is x = [u~ (n,3) interleaved, E (6)]; per-voxel quadrature fields (g, H, Y) are
+// (E,8) rows with Gauss points ordered xi fastest (core/hex8.py:157-179); material data
+// is per voxel (bank index int32, Gc, lam, mu, driver int8).
+//
+// The element apply is a 2.5D "marching" kernel: a block owns a TX x TY column of nodes
+// and walks KZ node layers in z.  At each voxel layer it stages the two bounding node
+// layers (with a one-node x/y halo) in LDS, evaluates every voxel of the layer once with
+// sum-factorised gradients (the trilinear field's derivative along x is a difference in x
+// interpolated in y and z, and the transpose for the force), writes the 24 corner results
+// to LDS, and gathers each owned node's sum from the 4 voxels below and the 4 above it.
+// There are no atomics: the result is deterministic, and the 6 macro-strain rows are
+// reduced per block then summed by one block in a fixed order.
+#include "common.h"
+
+#include <cstring>
+#include <vector>
+
+namespace phast {
+
+struct Hex8Geo {
+    int32_t nx, ny, nz, pad;
+    double ihx, ihy, ihz;   // 1/h
+    double w;               // Gauss weight * det J = hx hy hz / 8
+    double al, be;          // linear interpolation weights (1 +- 1/sqrt3)/2
+    double kappa, omk, ch;  // residual stiffness, 1-kappa, 2(1-kappa)
+    double K[64];           // AT2 voxel matrix M/ell + ell L, lexicographic corners [a*8+b]
+    double Nq[64];          // N[q*8 + a], lexicographic corners, xi-fastest Gauss points
+};
+
+}  // namespace phast
+
+struct phast_hex8_plan {
+    phast::Hex8Geo geo;
+    int64_t n = 0;  // reduced nodes == voxels
+    double *bank = nullptr;
+    int32_t *bank_index = nullptr;
+    double *Gc = nullptr, *lam = nullptr, *mu = nullptr;
+    int8_t *driver = nullptr;
+    int32_t n_bank = 0;
+    bool iso = false;   // every bank entry isotropic: stress from (lam, mu) per voxel
+    double *part = nullptr;  // block partials (8 per block)
+    unsigned int *counter = nullptr;
+    int32_t max_blocks = 0;
+    int32_t tx = 16, ty = 16, kz = 8;
+};
+
+namespace phast {
+
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// bilinear interpolation of corner values f[a][b] (a,b in {0,1}) to Gauss points o[ga][gb]
+__device__ __forceinline__ void interp2(double al, double be, double f00, double f10, double f01, double f11,
+                                        double o[2][2]) {
+    const double t00 = al * f00 + be * f10, t10 = be * f00 + al * f10;  // [ga] at b=0
+    const double t01 = al * f01 + be * f11, t11 = be * f01 + al * f11;  // [ga] at b=1
+    o[0][0] = al * t00 + be * t01;
+    o[0][1] = be * t00 + al * t01;
+    o[1][0] = al * t10 + be * t11;
+    o[1][1] = be * t10 + al * t11;
+}
+// transpose of interp2: corner sums R[la][lb] = sum_g W(la,ga) W(lb,gb) S[ga][gb]
+__device__ __forceinline__ void interp2T(double al, double be, const double S[2][2], double R[2][2]) {
+    const double t00 = al * S[0][0] + be * S[0][1], t01 = be * S[0][0] + al * S[0][1];  // [ga=0][lb]
+    const double t10 = al * S[1][0] + be * S[1][1], t11 = be * S[1][0] + al * S[1][1];  // [ga=1][lb]
+    R[0][0] = al * t00 + be * t10;
+    R[1][0] = be * t00 + al * t10;
+    R[0][1] = al * t01 + be * t11;
+    R[1][1] = be * t01 + al * t11;
+}
+
+// Gradients of the three displacement components at the Gauss points.
+// gx[c][iy][iz], gy[c][ix][iz], gz[c][ix][iy]; u[c][lx + 2 ly + 4 lz].
+__device__ __forceinline__ void voxel_grad(const Hex8Geo &G, const double u[3][8], double gx[3][2][2],
+                                           double gy[3][2][2], double gz[3][2][2]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        interp2(G.al, G.be, u[c][1] - u[c][0], u[c][3] - u[c][2], u[c][5] - u[c][4], u[c][7] - u[c][6], gx[c]);
+        interp2(G.al, G.be, u[c][2] - u[c][0], u[c][3] - u[c][1], u[c][6] - u[c][4], u[c][7] - u[c][5], gy[c]);
+        interp2(G.al, G.be, u[c][4] - u[c][0], u[c][5] - u[c][1], u[c][6] - u[c][2], u[c][7] - u[c][3], gz[c]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                gx[c][a][b] *= G.ihx;
+                gy[c][a][b] *= G.ihy;
+                gz[c][a][b] *= G.ihz;
+            }
+    }
+}
+
+// engineering strain (xx,yy,zz,yz,xz,xy) at Gauss point (ix,iy,iz) plus the macro strain
+__device__ __forceinline__ void qp_strain(const double gx[3][2][2], const double gy[3][2][2],
+                                          const double gz[3][2][2], const double E[6], int ix, int iy, int iz,
+                                          double e[6]) {
+    e[0] = gx[0][iy][iz] + E[0];
+    e[1] = gy[1][ix][iz] + E[1];
+    e[2] = gz[2][ix][iy] + E[2];
+    e[3] = (gz[1][ix][iy] + gy[2][ix][iz]) + E[3];
+    e[4] = (gz[0][ix][iy] + gx[2][iy][iz]) + E[4];
+    e[5] = (gy[0][ix][iz] + gx[1][iy][iz]) + E[5];
+}
+
+// trilinear interpolation of corner scalars to the 8 Gauss points (xi fastest)
+__device__ __forceinline__ void voxel_interp(const Hex8Geo &G, const double d[8], double dq[8]) {
+    double ax[2][2][2];  // [ly][lz][ix]
+#pragma unroll
+    for (int ly = 0; ly < 2; ++ly)
+#pragma unroll
+        for (int lz = 0; lz < 2; ++lz) {
+            const double c0 = d[2 * ly + 4 * lz], c1 = d[1 + 2 * ly + 4 * lz];
+            ax[ly][lz][0] = G.al * c0 + G.be * c1;
+            ax[ly][lz][1] = G.be * c0 + G.al * c1;
+        }
+#pragma unroll
+    for (int ix = 0; ix < 2; ++ix) {
+        double o[2][2];
+        interp2(G.al, G.be, ax[0][0][ix], ax[1][0][ix], ax[0][1][ix], ax[1][1][ix], o);
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+            for (int iz = 0; iz < 2; ++iz) dq[ix + 2 * iy + 4 * iz] = o[iy][iz];
+    }
+}
+
+// Internal force of one voxel: r[c][corner] = sum_q w B_q^T (g_q C (B_q u + E)).
+// Also returns sum_q w sigma_q (6) and 0.5 sum_q w sigma.eps (operator.py:1108-1169).
+template <bool ISO, bool ENERGY>
+__device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8], const double E[6],
+                                           const double g[8], const double *__restrict__ C, double lam,
+                                           double mu, double r[3][8], double sE[6], double &en) {
+    double gx[3][2][2], gy[3][2][2], gz[3][2][2];
+    voxel_grad(G, u, gx, gy, gz);
+    double Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) Sx[c][a][b] = Sy[c][a][b] = Sz[c][a][b] = 0.0;
+#pragma unroll
+    for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < 2; ++ix) {
+                double e[6];
+                qp_strain(gx, gy, gz, E, ix, iy, iz, e);
+                double s[6];
+                if (ISO) {
+                    const double tr = e[0] + e[1] + e[2];
+                    s[0] = lam * tr + 2.0 * mu * e[0];
+                    s[1] = lam * tr + 2.0 * mu * e[1];
+                    s[2] = lam * tr + 2.0 * mu * e[2];
+                    s[3] = mu * e[3];
+                    s[4] = mu * e[4];
+                    s[5] = mu * e[5];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        double t = 0.0;
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) t += C[k * 6 + m] * e[m];
+                        s[k] = t;
+                    }
+                }
+                const double gw = g[ix + 2 * iy + 4 * iz] * G.w;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    s[k] *= gw;
+                    sE[k] += s[k];
+                    if (ENERGY) en += 0.5 * s[k] * e[k];
+                }
+                Sx[0][iy][iz] += s[0];
+                Sx[1][iy][iz] += s[5];
+                Sx[2][iy][iz] += s[4];
+                Sy[0][ix][iz] += s[5];
+                Sy[1][ix][iz] += s[1];
+                Sy[2][ix][iz] += s[3];
+                Sz[0][ix][iy] += s[4];
+                Sz[1][ix][iy] += s[3];
+                Sz[2][ix][iy] += s[2];
+            }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double R[2][2];
+        interp2T(G.al, G.be, Sx[c], R);  // R[ly][lz]
+#pragma unroll
+        for (int ly = 0; ly < 2; ++ly)
+#pragma unroll
+            for (int lz = 0; lz < 2; ++lz) {
+                const double t = R[ly][lz] * G.ihx;
+                r[c][1 + 2 * ly + 4 * lz] = t;
+                r[c][2 * ly + 4 * lz] = -t;
+            }
+        interp2T(G.al, G.be, Sy[c], R);  // R[lx][lz]
+#pragma unroll
+        for (int lx = 0; lx < 2; ++lx)
+#pragma unroll
+            for (int lz = 0; lz < 2; ++lz) {
+                const double t = R[lx][lz] * G.ihy;
+                r[c][lx + 2 + 4 * lz] += t;
+                r[c][lx + 4 * lz] -= t;
+            }
+        interp2T(G.al, G.be, Sz[c], R);  // R[lx][ly]
+#pragma unroll
+        for (int lx = 0; lx < 2; ++lx)
+#pragma unroll
+            for (int ly = 0; ly < 2; ++ly) {
+                const double t = R[lx][ly] * G.ihz;
+                r[c][lx + 2 * ly + 4] += t;
+                r[c][lx + 2 * ly] -= t;
+            }
+    }
+}
+
+struct MechArgs3 {
+    const double *x;       // (3n+6)
+    const double *g;       // (E,8) or null
+    const double *d;       // (n,) when g is formed in-kernel
+    const double *bank;
+    const int32_t *bank_index;
+    const double *lam, *mu;
+    double *y;             // (3n+6)
+    double *part;          // 8 per block
+    unsigned int *counter; // blocks finished (reset to 0 by the last block)
+    double *y6;            // macro rows destination (y + 3n) or null
+    double *out7;          // stress sums destination or null
+    int32_t kz, tiles_x, tiles_y, nvals;
+};
+
+template <int TX, int TY, int GMODE, bool ISO, bool ENERGY>  // GMODE 0: g=1, 1: g array, 2: g from d
+__global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, MechArgs3 A) {
+    constexpr int NT = TX * TY;
+    constexpr int NXH = TX + 2, NYH = TY + 2, NH = NXH * NYH;
+    constexpr int NXV = TX + 1, NYV = TY + 1, NV = NXV * NYV;
+    constexpr int NF = (GMODE == 2) ? 4 : 3;
+    __shared__ double su[2][NF][NH];
+    __shared__ double sr[24][NV];
+    __shared__ double red[NT / 64];
+
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int bx = blockIdx.x % A.tiles_x, by = (blockIdx.x / A.tiles_x) % A.tiles_y;
+    const int bz = blockIdx.x / (A.tiles_x * A.tiles_y);
+    const int i0 = bx * TX, j0 = by * TY, k0 = bz * A.kz;
+    const int k1 = min(k0 + A.kz, nz);
+    const int64_t n = (int64_t)nx * ny * nz;
+    double E[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) E[k] = A.x[3 * n + k];
+
+    auto load_layer = [&](int buf, int k) {
+        const int kk = wrap(k, nz);
+        for (int s = threadIdx.x; s < NH; s += NT) {
+            const int i = wrap(i0 + s % NXH - 1, nx), j = wrap(j0 + s / NXH - 1, ny);
+            const int64_t node = i + (int64_t)nx * (j + (int64_t)ny * kk);
+            su[buf][0][s] = A.x[3 * node];
+            su[buf][1][s] = A.x[3 * node + 1];
+            su[buf][2][s] = A.x[3 * node + 2];
+            if (GMODE == 2) su[buf][3][s] = A.d[node];
+        }
+    };
+
+    double sE[6] = {0, 0, 0, 0, 0, 0}, en = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};
+    int cur = 0;
+    load_layer(0, k0 - 1);
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    for (int kv = k0 - 1; kv < k1; ++kv) {
+        load_layer(cur ^ 1, kv + 1);
+        __syncthreads();
+        const int kw = wrap(kv, nz);
+        for (int s = threadIdx.x; s < NV; s += NT) {
+            const int li = s % NXV - 1, lj = s / NXV - 1;
+            const int iv = i0 + li, jv = j0 + lj;
+            const int64_t e = wrap(iv, nx) + (int64_t)nx * (wrap(jv, ny) + (int64_t)ny * kw);
+            const int b0 = (lj + 1) * NXH + (li + 1);
+            double u[3][8], g[8];
+#pragma unroll
+            for (int lz = 0; lz < 2; ++lz) {
+                const int bb = lz ? (cur ^ 1) : cur;
+#pragma unroll
+                for (int ly = 0; ly < 2; ++ly)
+#pragma unroll
+                    for (int lx = 0; lx < 2; ++lx) {
+                        const int slot = b0 + ly * NXH + lx, a = lx + 2 * ly + 4 * lz;
+                        u[0][a] = su[bb][0][slot];
+                        u[1][a] = su[bb][1][slot];
+                        u[2][a] = su[bb][2][slot];
+                        if (GMODE == 2) g[a] = su[bb][NF - 1][slot];
+                    }
+            }
+            if (GMODE == 2) {
+                double dq[8];
+                voxel_interp(G, g, dq);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const double omd = 1.0 - dq[q];
+                    g[q] = G.omk * (omd * omd) + G.kappa;  // splits.py:56-59
+                }
+            } else if (GMODE == 1) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) g[q] = A.g[e * 8 + q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) g[q] = 1.0;
+            }
+            const bool own = li >= 0 && lj >= 0 && kv >= k0 && iv < nx && jv < ny;
+            double r[3][8], vE[6] = {0, 0, 0, 0, 0, 0}, ve = 0.0;
+            if (ISO) {
+                voxel_mech<true, ENERGY>(G, u, E, g, nullptr, A.lam[e], A.mu[e], r, vE, ve);
+            } else {
+                voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)A.bank_index[e], 0.0, 0.0, r, vE,
+                                          ve);
+            }
+            if (own) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sE[k] += vE[k];
+                en += ve;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int a = 0; a < 8; ++a) sr[c * 8 + a][s] = r[c][a];
+        }
+        __syncthreads();
+        {
+            // voxels around node (tx,ty): (tx-1,ty-1) corner (1,1), (tx,ty-1) corner (0,1),
+            // (tx-1,ty) corner (1,0), (tx,ty) corner (0,0); bottom faces -> node level kv,
+            // top faces -> node level kv+1.
+            const int s00 = ty * NXV + tx, s10 = s00 + 1, s01 = s00 + NXV, s11 = s01 + 1;
+            const int i = i0 + tx, j = j0 + ty;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double bot = ((sr[c * 8 + 3][s00] + sr[c * 8 + 2][s10]) + sr[c * 8 + 1][s01]) +
+                                   sr[c * 8 + 0][s11];
+                const double top = ((sr[c * 8 + 7][s00] + sr[c * 8 + 6][s10]) + sr[c * 8 + 5][s01]) +
+                                   sr[c * 8 + 4][s11];
+                if (kv >= k0 && i < nx && j < ny) {
+                    const int64_t node = i + (int64_t)nx * (j + (int64_t)ny * kv);
+                    A.y[3 * node + c] = acc[c] + bot;
+                }
+                acc[c] = top;
+            }
+        }
+        cur ^= 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double t = block_sum<NT>(sE[k], red);
+        if (threadIdx.x == 0) A.part[8 * blockIdx.x + k] = t;
+    }
+    if (ENERGY) {
+        const double t = block_sum<NT>(en, red);
+        if (threadIdx.x == 0) A.part[8 * blockIdx.x + 6] = t;
+    }
+    // last-block-done: the block that finishes last sums every block's partials in block
+    // order (a fixed order, so the macro rows are deterministic) and resets the counter.
+    __shared__ unsigned int is_last;
+    __threadfence();
+    if (threadIdx.x == 0) is_last = (atomicAdd(A.counter, 1u) == gridDim.x - 1) ? 1u : 0u;
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        for (int k = 0; k < A.nvals; ++k) {
+            double s = 0.0;
+            for (int b = threadIdx.x; b < (int)gridDim.x; b += NT) s += __builtin_nontemporal_load(&A.part[8 * b + k]);
+            s = block_sum<NT>(s, red);
+            if (threadIdx.x == 0) {
+                if (A.y6 && k < 6) A.y6[k] = s;
+                if (A.out7) A.out7[k] = s;
+            }
+        }
+        if (threadIdx.x == 0) atomicExch(A.counter, 0u);
+    }
+}
+
+// Sum the block partials of the macro rows in a fixed order; write into y[3n + k] and/or out7.
+__global__ __launch_bounds__(256) void hex8_finalize_kernel(const double *part, int nblocks, int nvals,
+                                                            double *y6, double *out7) {
+    __shared__ double red[4];
+    for (int k = 0; k < nvals; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) s += part[8 * b + k];
+        s = block_sum<256>(s, red);
+        if (threadIdx.x == 0) {
+            if (y6 && k < 6) y6[k] = s;
+            if (out7) out7[k] = s;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void hex8_degradation_kernel(const Hex8Geo G, const double *d, double *g) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    if (e >= n) return;
+    const int i = (int)(e % G.nx), j = (int)((e / G.nx) % G.ny), k = (int)(e / ((int64_t)G.nx * G.ny));
+    double dc[8], dq[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int ii = wrap(i + (a & 1), G.nx), jj = wrap(j + ((a >> 1) & 1), G.ny),
+                  kk = wrap(k + (a >> 2), G.nz);
+        dc[a] = d[ii + (int64_t)G.nx * (jj + (int64_t)G.ny * kk)];
+    }
+    voxel_interp(G, dc, dq);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double omd = 1.0 - dq[q];
+        g[e * 8 + q] = G.omk * (omd * omd) + G.kappa;
+    }
+}
+
+// Jacobi diagonal: one thread per node gathers its 8 voxels (operator.py:1196-1254); the
+// thread of node n also forms voxel n's contribution to the macro block.
+__global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const double *g, const double *bank,
+                                                        const int32_t *bank_index, double *diag, double *part) {
+    __shared__ double red[4];
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    const int64_t node = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double dE[6] = {0, 0, 0, 0, 0, 0};
+    if (node < n) {
+        const int i = (int)(node % G.nx), j = (int)((node / G.nx) % G.ny), k = (int)(node / ((int64_t)G.nx * G.ny));
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int v = 0; v < 8; ++v) {  // voxels in increasing (dz,dy,dx) order
+            const int dx = v & 1, dy = (v >> 1) & 1, dz = v >> 2;
+            const int64_t e = wrap(i - 1 + dx, G.nx) + (int64_t)G.nx * (wrap(j - 1 + dy, G.ny) +
+                                                                      (int64_t)G.ny * wrap(k - 1 + dz, G.nz));
+            const int lx = 1 - dx, ly = 1 - dy, lz = 1 - dz;
+            const double sx = lx ? 1.0 : -1.0, sy = ly ? 1.0 : -1.0, sz = lz ? 1.0 : -1.0;
+            const double *C = bank + 36 * (int64_t)bank_index[e];
+            for (int q = 0; q < 8; ++q) {
+                const int ix = q & 1, iy = (q >> 1) & 1, iz = q >> 2;
+                const double Wx = (lx == ix) ? G.al : G.be, Wy = (ly == iy) ? G.al : G.be,
+                             Wz = (lz == iz) ? G.al : G.be;
+                const double bx = sx * G.ihx * Wy * Wz, by = sy * G.ihy * Wx * Wz, bz = sz * G.ihz * Wx * Wy;
+                const double gw = (g ? g[e * 8 + q] : 1.0) * G.w;
+                // B columns of the three dofs: x: rows (0 bx, 4 bz, 5 by); y: (1 by, 3 bz, 5 bx); z: (2 bz, 3 by, 4 bx)
+                const int rx[3] = {0, 4, 5}, ry[3] = {1, 3, 5}, rz[3] = {2, 3, 4};
+                const double vx[3] = {bx, bz, by}, vy[3] = {by, bz, bx}, vz[3] = {bz, by, bx};
+                double tx = 0.0, ty = 0.0, tz = 0.0;
+#pragma unroll
+                for (int p = 0; p < 3; ++p)
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        tx += C[rx[p] * 6 + rx[m]] * vx[p] * vx[m];
+                        ty += C[ry[p] * 6 + ry[m]] * vy[p] * vy[m];
+                        tz += C[rz[p] * 6 + rz[m]] * vz[p] * vz[m];
+                    }
+                acc[0] += gw * tx;
+                acc[1] += gw * ty;
+                acc[2] += gw * tz;
+            }
+        }
+        diag[3 * node] = acc[0];
+        diag[3 * node + 1] = acc[1];
+        diag[3 * node + 2] = acc[2];
+        // macro block of voxel `node`: (sum_q g w) C_kk
+        double gws = 0.0;
+        for (int q = 0; q < 8; ++q) gws += (g ? g[node * 8 + q] : 1.0) * G.w;
+        const double *C = bank + 36 * (int64_t)bank_index[node];
+#pragma unroll
+        for (int k2 = 0; k2 < 6; ++k2) dE[k2] = gws * C[k2 * 7];
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 6; ++k2) {
+        const double t = block_sum<256>(dE[k2], red);
+        if (threadIdx.x == 0) part[8 * blockIdx.x + k2] = t;
+    }
+}
+
+// Closed-form ascending eigenvalues of a symmetric 3x3 (splits.py:93-125).
+__device__ __forceinline__ void eig3(double a11, double a22, double a33, double a12, double a13, double a23,
+                                     double ev[3]) {
+    const double q = (a11 + a22 + a33) / 3.0;
+    const double b11 = a11 - q, b22 = a22 - q, b33 = a33 - q;
+    const double p1 = a12 * a12 + a13 * a13 + a23 * a23;
+    const double p = sqrt((b11 * b11 + b22 * b22 + b33 * b33 + 2.0 * p1) / 6.0);
+    const double det_b = b11 * (b22 * b33 - a23 * a23) - a12 * (a12 * b33 - a23 * a13) + a13 * (a12 * a23 - b22 * a13);
+    const bool iso = p <= 2.2250738585072014e-308;
+    const double ps = iso ? 1.0 : p;
+    double r = det_b / (2.0 * (ps * ps * ps));
+    r = fmin(fmax(r, -1.0), 1.0);
+    const double phi = acos(r) / 3.0;
+    const double largest = q + 2.0 * p * cos(phi);
+    const double smallest = q + 2.0 * p * cos(phi + 2.0 * M_PI / 3.0);
+    const double middle = 3.0 * q - largest - smallest;
+    if (iso) {
+        ev[0] = ev[1] = ev[2] = q;
+    } else {
+        ev[0] = smallest;
+        ev[1] = middle;
+        ev[2] = largest;
+    }
+}
+
+// Undamaged driving energy per Gauss point by driver code (operator.py:1415-1444).
+__global__ __launch_bounds__(256) void hex8_driving_kernel(const Hex8Geo G, const double *x, const double *bank,
+                                                           const int32_t *bank_index, const double *lam,
+                                                           const double *mu, const int8_t *driver, double *Y) {
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int code = driver[e];
+    if (code != 1 && code != 2) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Y[e * 8 + q] = 0.0;
+        return;
+    }
+    const int i = (int)(e % G.nx), j = (int)((e / G.nx) % G.ny), k = (int)(e / ((int64_t)G.nx * G.ny));
+    double u[3][8], E[6];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int64_t node = wrap(i + (a & 1), G.nx) +
+                             (int64_t)G.nx * (wrap(j + ((a >> 1) & 1), G.ny) + (int64_t)G.ny * wrap(k + (a >> 2), G.nz));
+        u[0][a] = x[3 * node];
+        u[1][a] = x[3 * node + 1];
+        u[2][a] = x[3 * node + 2];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) E[c] = x[3 * n + c];
+    double gx[3][2][2], gy[3][2][2], gz[3][2][2];
+    voxel_grad(G, u, gx, gy, gz);
+    const double *C = bank + 36 * (int64_t)bank_index[e];
+    const double la = lam[e], m = mu[e];
+    for (int q = 0; q < 8; ++q) {
+        const int ix = q & 1, iy = (q >> 1) & 1, iz = q >> 2;
+        double s[6];
+        qp_strain(gx, gy, gz, E, ix, iy, iz, s);
+        double val;
+        if (code == 1) {  // 0.5 eps.C.eps (splits.py:68-72)
+            double t = 0.0;
+            for (int kk = 0; kk < 6; ++kk) {
+                double c6 = 0.0;
+                for (int mm = 0; mm < 6; ++mm) c6 += C[kk * 6 + mm] * s[mm];
+                t += s[kk] * c6;
+            }
+            val = 0.5 * t;
+        } else {  // Miehe spectral (splits.py:146-161)
+            double ev[3];
+            eig3(s[0], s[1], s[2], 0.5 * s[5], 0.5 * s[4], 0.5 * s[3], ev);
+            const double trp = fmax(s[0] + s[1] + s[2], 0.0);
+            const double e0 = fmax(ev[0], 0.0), e1 = fmax(ev[1], 0.0), e2 = fmax(ev[2], 0.0);
+            val = 0.5 * la * (trp * trp) + m * ((e0 * e0 + e1 * e1) + e2 * e2);
+        }
+        Y[e * 8 + q] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// AT2 damage operator on the voxel grid (operator.py:1450-1494)
+// ---------------------------------------------------------------------------------------
+
+struct DmgArgs3 {
+    const double *d, *H, *Gc;
+    double *y;
+    double scale;
+    int32_t kz, tiles_x, tiles_y;
+};
+
+template <int TX, int TY, bool HIST>
+__global__ __launch_bounds__(TX *TY) void hex8_damage_kernel(const Hex8Geo G, DmgArgs3 A) {
+    constexpr int NT = TX * TY;
+    constexpr int NXH = TX + 2, NYH = TY + 2, NH = NXH * NYH;
+    constexpr int NXV = TX + 1, NYV = TY + 1, NV = NXV * NYV;
+    __shared__ double su[2][NH];
+    __shared__ double sr[8][NV];
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int bx = blockIdx.x % A.tiles_x, by = (blockIdx.x / A.tiles_x) % A.tiles_y;
+    const int bz = blockIdx.x / (A.tiles_x * A.tiles_y);
+    const int i0 = bx * TX, j0 = by * TY, k0 = bz * A.kz;
+    const int k1 = min(k0 + A.kz, nz);
+    auto load_layer = [&](int buf, int k) {
+        const int kk = wrap(k, nz);
+        for (int s = threadIdx.x; s < NH; s += NT) {
+            const int i = wrap(i0 + s % NXH - 1, nx), j = wrap(j0 + s / NXH - 1, ny);
+            su[buf][s] = A.d[i + (int64_t)nx * (j + (int64_t)ny * kk)];
+        }
+    };
+    double acc = 0.0;
+    int cur = 0;
+    load_layer(0, k0 - 1);
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    for (int kv = k0 - 1; kv < k1; ++kv) {
+        load_layer(cur ^ 1, kv + 1);
+        __syncthreads();
+        const int kw = wrap(kv, nz);
+        for (int s = threadIdx.x; s < NV; s += NT) {
+            const int li = s % NXV - 1, lj = s / NXV - 1;
+            const int64_t e = wrap(i0 + li, nx) + (int64_t)nx * (wrap(j0 + lj, ny) + (int64_t)ny * kw);
+            const int b0 = (lj + 1) * NXH + (li + 1);
+            double dc[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) dc[a] = su[(a >> 2) ? (cur ^ 1) : cur][b0 + ((a >> 1) & 1) * NXH + (a & 1)];
+            const double gc = A.Gc[e];
+            double y[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) t += dc[b] * G.K[b * 8 + a];
+                y[a] = gc * t;
+            }
+            if (HIST) {
+                double dq[8];
+                voxel_interp(G, dc, dq);
+                double hq[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) hq[q] = (A.H[e * 8 + q] * G.w) * dq[q];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) t += hq[q] * G.Nq[q * 8 + a];
+                    y[a] += G.ch * t;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 8; ++a) sr[a][s] = y[a];
+        }
+        __syncthreads();
+        const int s00 = ty * NXV + tx, s10 = s00 + 1, s01 = s00 + NXV, s11 = s01 + 1;
+        const double bot = ((sr[3][s00] + sr[2][s10]) + sr[1][s01]) + sr[0][s11];
+        const double top = ((sr[7][s00] + sr[6][s10]) + sr[5][s01]) + sr[4][s11];
+        const int i = i0 + tx, j = j0 + ty;
+        if (kv >= k0 && i < nx && j < ny) A.y[i + (int64_t)nx * (j + (int64_t)ny * kv)] = A.scale * (acc + bot);
+        acc = top;
+        cur ^= 1;
+    }
+}
+
+// rhs / diagonal of the damage operator: thread per node over its 8 voxels.
+__global__ __launch_bounds__(256) void hex8_damage_rd_kernel(const Hex8Geo G, const double *H, const double *Gc,
+                                                             double scale, double *b, double *diag) {
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    const int64_t node = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (node >= n) return;
+    const int i = (int)(node % G.nx), j = (int)((node / G.nx) % G.ny), k = (int)(node / ((int64_t)G.nx * G.ny));
+    double sb = 0.0, sd = 0.0;
+    for (int v = 0; v < 8; ++v) {
+        const int dx = v & 1, dy = (v >> 1) & 1, dz = v >> 2;
+        const int64_t e = wrap(i - 1 + dx, G.nx) +
+                          (int64_t)G.nx * (wrap(j - 1 + dy, G.ny) + (int64_t)G.ny * wrap(k - 1 + dz, G.nz));
+        const int a = (1 - dx) + 2 * (1 - dy) + 4 * (1 - dz);
+        double hb = 0.0, hd = 0.0;
+        if (H) {
+            for (int q = 0; q < 8; ++q) {
+                const double hw = H[e * 8 + q] * G.w;
+                hb += hw * G.Nq[q * 8 + a];
+                hd += hw * (G.Nq[q * 8 + a] * G.Nq[q * 8 + a]);
+            }
+        }
+        sb += G.ch * hb;
+        sd += Gc[e] * G.K[a * 8 + a] + G.ch * hd;
+    }
+    if (b) b[node] = scale * sb;
+    if (diag) diag[node] = scale * sd;
+}
+
+template <int TX, int TY>
+static int launch_mech_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool energy, hipStream_t st,
+                         int *nblocks) {
+    A.tiles_x = (int)ceil_div(pl->geo.nx, TX);
+    A.tiles_y = (int)ceil_div(pl->geo.ny, TY);
+    const int tz = (int)ceil_div(pl->geo.nz, A.kz);
+    const int nb = A.tiles_x * A.tiles_y * tz;
+    *nblocks = nb;
+    if (nb > pl->max_blocks) {
+        set_error("hex8 apply needs %d blocks, plan holds partials for %d", nb, pl->max_blocks);
+        return PHAST_ERR_ARG;
+    }
+    dim3 grid(nb), block(TX * TY);
+#define PHAST_L(GM, ISO, EN) \
+    hipLaunchKernelGGL((hex8_mech_kernel<TX, TY, GM, ISO, EN>), grid, block, 0, st, pl->geo, A)
+    if (pl->iso) {
+        if (energy) {
+            if (gmode == 0) PHAST_L(0, true, true);
+            else if (gmode == 1) PHAST_L(1, true, true);
+            else PHAST_L(2, true, true);
+        } else {
+            if (gmode == 0) PHAST_L(0, true, false);
+            else if (gmode == 1) PHAST_L(1, true, false);
+            else PHAST_L(2, true, false);
+        }
+    } else {
+        if (energy) {
+            if (gmode == 0) PHAST_L(0, false, true);
+            else if (gmode == 1) PHAST_L(1, false, true);
+            else PHAST_L(2, false, true);
+        } else {
+            if (gmode == 0) PHAST_L(0, false, false);
+            else if (gmode == 1) PHAST_L(1, false, false);
+            else PHAST_L(2, false, false);
+        }
+    }
+#undef PHAST_L
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+static int launch_mech(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
+                       double *out7, bool energy, hipStream_t st) {
+    MechArgs3 A;
+    std::memset(&A, 0, sizeof(A));
+    A.x = x;
+    A.g = g;
+    A.d = d;
+    A.bank = pl->bank;
+    A.bank_index = pl->bank_index;
+    A.lam = pl->lam;
+    A.mu = pl->mu;
+    A.y = y;
+    A.part = pl->part;
+    A.counter = pl->counter;
+    A.y6 = out7 ? nullptr : y + 3 * pl->n;
+    A.out7 = out7;
+    A.nvals = energy ? 7 : 6;
+    A.kz = pl->kz;
+    const int gmode = d ? 2 : (g ? 1 : 0);
+    int nb = 0, rc;
+    if (!y) {  // stress sums only: still needs a write target for the nodal rows
+        set_error("internal: hex8 apply without output");
+        return PHAST_ERR_ARG;
+    }
+    if (pl->tx == 16 && pl->ty == 16) rc = launch_mech_t<16, 16>(pl, A, gmode, energy, st, &nb);
+    else if (pl->tx == 16 && pl->ty == 8) rc = launch_mech_t<16, 8>(pl, A, gmode, energy, st, &nb);
+    else if (pl->tx == 8 && pl->ty == 8) rc = launch_mech_t<8, 8>(pl, A, gmode, energy, st, &nb);
+    else if (pl->tx == 32 && pl->ty == 8) rc = launch_mech_t<32, 8>(pl, A, gmode, energy, st, &nb);
+    else {
+        set_error("unsupported hex8 tile %dx%d", pl->tx, pl->ty);
+        return PHAST_ERR_UNSUPPORTED;
+    }
+    return rc;
+}
+
+template <int TX, int TY>
+static int launch_damage_t(const phast_hex8_plan *pl, DmgArgs3 A, hipStream_t st) {
+    A.tiles_x = (int)ceil_div(pl->geo.nx, TX);
+    A.tiles_y = (int)ceil_div(pl->geo.ny, TY);
+    const int tz = (int)ceil_div(pl->geo.nz, A.kz);
+    dim3 grid(A.tiles_x * A.tiles_y * tz), block(TX * TY);
+    if (A.H) hipLaunchKernelGGL((hex8_damage_kernel<TX, TY, true>), grid, block, 0, st, pl->geo, A);
+    else hipLaunchKernelGGL((hex8_damage_kernel<TX, TY, false>), grid, block, 0, st, pl->geo, A);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+}  // namespace phast
+
+using namespace phast;
+
+// Host construction of the lexicographic-corner voxel matrices (solid3d/periodic.py:193-221;
+// operator.py:242-247): N, gradN at the 2x2x2 Gauss points, M = sum_q w N N^T,
+// L = sum_q w gradN gradN^T, both symmetrised, K = M/ell + ell L.
+static void build_voxel_constants(Hex8Geo &G, double hx, double hy, double hz, double ell) {
+    const double a = 1.0 / std::sqrt(3.0);
+    const double pt[2] = {-a, a};
+    double N[8][8], dN[8][8][3];
+    for (int q = 0; q < 8; ++q) {
+        const double xi = pt[q & 1], eta = pt[(q >> 1) & 1], zeta = pt[q >> 2];
+        for (int c = 0; c < 8; ++c) {
+            const double sx = (c & 1) ? 1.0 : -1.0, sy = ((c >> 1) & 1) ? 1.0 : -1.0, sz = (c >> 2) ? 1.0 : -1.0;
+            N[q][c] = 0.125 * (1.0 + sx * xi) * (1.0 + sy * eta) * (1.0 + sz * zeta);
+            dN[q][c][0] = 0.125 * sx * (1.0 + sy * eta) * (1.0 + sz * zeta) * (2.0 / hx);
+            dN[q][c][1] = 0.125 * (1.0 + sx * xi) * sy * (1.0 + sz * zeta) * (2.0 / hy);
+            dN[q][c][2] = 0.125 * (1.0 + sx * xi) * (1.0 + sy * eta) * sz * (2.0 / hz);
+        }
+    }
+    const double w = hx * hy * hz / 8.0;
+    double M[8][8] = {}, L[8][8] = {};
+    for (int q = 0; q < 8; ++q)
+        for (int i = 0; i < 8; ++i)
+            for (int j = 0; j < 8; ++j) {
+                M[i][j] += w * N[q][i] * N[q][j];
+                L[i][j] += w * (dN[q][i][0] * dN[q][j][0] + dN[q][i][1] * dN[q][j][1] + dN[q][i][2] * dN[q][j][2]);
+            }
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) {
+            const double Ms = 0.5 * (M[i][j] + M[j][i]), Ls = 0.5 * (L[i][j] + L[j][i]);
+            G.K[i * 8 + j] = Ms / ell + ell * Ls;
+        }
+    for (int q = 0; q < 8; ++q)
+        for (int c = 0; c < 8; ++c) G.Nq[q * 8 + c] = N[q][c];
+    G.w = w;
+    G.al = 0.5 * (1.0 + a);
+    G.be = 0.5 * (1.0 - a);
+    G.ihx = 1.0 / hx;
+    G.ihy = 1.0 / hy;
+    G.ihz = 1.0 / hz;
+}
+
+extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,
+                                      const double *bank, int32_t n_bank, const int32_t *bank_index,
+                                      const double *Gc, const double *lam, const double *mu, const int8_t *driver,
+                                      double kappa, double ell, phast_hex8_plan **out) {
+    PHAST_REQUIRE(out && bank && bank_index && Gc && lam && mu && driver, "phast_hex8_plan_create: null argument");
+    PHAST_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "periodic voxel cell needs at least 2 voxels per axis");
+    PHAST_REQUIRE(hx > 0 && hy > 0 && hz > 0 && ell > 0 && n_bank >= 1, "bad geometry / bank size");
+    auto *pl = new phast_hex8_plan();
+    std::memset(&pl->geo, 0, sizeof(pl->geo));
+    pl->geo.nx = nx;
+    pl->geo.ny = ny;
+    pl->geo.nz = nz;
+    pl->geo.kappa = kappa;
+    pl->geo.omk = 1.0 - kappa;
+    pl->geo.ch = 2.0 * (1.0 - kappa);
+    build_voxel_constants(pl->geo, hx, hy, hz, ell);
+    pl->n = (int64_t)nx * ny * nz;
+    pl->n_bank = n_bank;
+    for (int64_t e = 0; e < pl->n; ++e) {
+        if (bank_index[e] < 0 || bank_index[e] >= n_bank) {
+            delete pl;
+            set_error("bank_index[%lld] out of range", (long long)e);
+            return PHAST_ERR_ARG;
+        }
+    }
+    // isotropic bank: every entry has the (lam, mu) structure of core/hex8.py:296-320 and the
+    // per-voxel lam/mu arrays reproduce it exactly
+    bool iso = true;
+    for (int64_t e = 0; e < pl->n && iso; ++e) {
+        const double *C = bank + 36 * (int64_t)bank_index[e];
+        const double l = lam[e], m = mu[e];
+        for (int r = 0; r < 6 && iso; ++r)
+            for (int c = 0; c < 6 && iso; ++c) {
+                double ref = 0.0;
+                if (r < 3 && c < 3) ref = l + (r == c ? 2.0 * m : 0.0);
+                else if (r == c) ref = m;
+                if (C[r * 6 + c] != ref) iso = false;
+            }
+    }
+    pl->iso = iso;
+    pl->max_blocks = (int32_t)(ceil_div(nx, 8) * ceil_div(ny, 8) * nz);
+    const size_t nb = sizeof(double) * (size_t)pl->n;
+    if (hipMalloc(&pl->bank, sizeof(double) * 36 * n_bank) != hipSuccess ||
+        hipMalloc(&pl->bank_index, sizeof(int32_t) * pl->n) != hipSuccess || hipMalloc(&pl->Gc, nb) != hipSuccess ||
+        hipMalloc(&pl->lam, nb) != hipSuccess || hipMalloc(&pl->mu, nb) != hipSuccess ||
+        hipMalloc(&pl->driver, (size_t)pl->n) != hipSuccess ||
+        hipMalloc(&pl->counter, sizeof(unsigned int)) != hipSuccess ||
+        hipMemset(pl->counter, 0, sizeof(unsigned int)) != hipSuccess ||
+        hipMalloc(&pl->part, sizeof(double) * 8 * (size_t)std::max<int64_t>(pl->max_blocks, (pl->n + 255) / 256)) !=
+            hipSuccess ||
+        hipMemcpy(pl->bank, bank, sizeof(double) * 36 * n_bank, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->bank_index, bank_index, sizeof(int32_t) * pl->n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->Gc, Gc, nb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->lam, lam, nb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->mu, mu, nb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->driver, driver, (size_t)pl->n, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("phast_hex8_plan_create: device allocation/copy failed");
+        phast_hex8_plan_destroy(pl);
+        return PHAST_ERR_HIP;
+    }
+    *out = pl;
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
+    if (!pl) return PHAST_OK;
+    (void)hipFree(pl->bank);
+    (void)hipFree(pl->bank_index);
+    (void)hipFree(pl->Gc);
+    (void)hipFree(pl->lam);
+    (void)hipFree(pl->mu);
+    (void)hipFree(pl->driver);
+    (void)hipFree(pl->part);
+    (void)hipFree(pl->counter);
+    delete pl;
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_set_tiles(phast_hex8_plan *pl, int32_t tx, int32_t ty, int32_t kz) {
+    PHAST_REQUIRE(pl && kz >= 1, "phast_hex8_set_tiles: bad argument");
+    PHAST_REQUIRE((tx == 16 && ty == 16) || (tx == 16 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8),
+                  "unsupported hex8 tile %dx%d", tx, ty);
+    pl->tx = tx;
+    pl->ty = ty;
+    pl->kz = kz;
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_degradation(const phast_hex8_plan *pl, const double *d, double *g, hipStream_t st) {
+    PHAST_REQUIRE(pl && d && g, "phast_hex8_degradation: null argument");
+    hipLaunchKernelGGL(hex8_degradation_kernel, dim3((unsigned)ceil_div(pl->n, 256)), dim3(256), 0, st, pl->geo, d, g);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_mech_action(const phast_hex8_plan *pl, const double *x, const double *g, double *y,
+                                      hipStream_t st) {
+    PHAST_REQUIRE(pl && x && y && x != y, "phast_hex8_mech_action: bad argument");
+    return launch_mech(pl, x, g, nullptr, y, nullptr, false, st);
+}
+
+extern "C" int phast_hex8_mech_action_d(const phast_hex8_plan *pl, const double *x, const double *d, double *y,
+                                        hipStream_t st) {
+    PHAST_REQUIRE(pl && x && d && y && x != y, "phast_hex8_mech_action_d: bad argument");
+    return launch_mech(pl, x, nullptr, d, y, nullptr, false, st);
+}
+
+extern "C" int phast_hex8_stress_sums(const phast_hex8_plan *pl, const double *x, const double *g, double *out7,
+                                      hipStream_t st) {
+    PHAST_REQUIRE(pl && x && out7, "phast_hex8_stress_sums: null argument");
+    // the nodal rows are computed and discarded into the solver scratch of the plan
+    double *scratch = nullptr;
+    PHAST_HIP(hipMallocAsync((void **)&scratch, sizeof(double) * (3 * pl->n + 6), st));
+    const int rc = launch_mech(pl, x, g, nullptr, scratch, out7, true, st);
+    PHAST_HIP(hipFreeAsync(scratch, st));
+    return rc;
+}
+
+extern "C" int phast_hex8_mech_diagonal(const phast_hex8_plan *pl, const double *g, double *diag, hipStream_t st) {
+    PHAST_REQUIRE(pl && diag, "phast_hex8_mech_diagonal: null argument");
+    const int nb = (int)ceil_div(pl->n, 256);
+    hipLaunchKernelGGL(hex8_diag_kernel, dim3(nb), dim3(256), 0, st, pl->geo, g, pl->bank, pl->bank_index, diag,
+                       pl->part);
+    PHAST_LAUNCHED();
+    hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, pl->part, nb, 6, diag + 3 * pl->n,
+                       (double *)nullptr);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_driving_force(const phast_hex8_plan *pl, const double *x, double *Y, hipStream_t st) {
+    PHAST_REQUIRE(pl && x && Y, "phast_hex8_driving_force: null argument");
+    hipLaunchKernelGGL(hex8_driving_kernel, dim3((unsigned)ceil_div(pl->n, 256)), dim3(256), 0, st, pl->geo, x,
+                       pl->bank, pl->bank_index, pl->lam, pl->mu, pl->driver, Y);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_damage_action(const phast_hex8_plan *pl, const double *d, const double *H, double scale,
+                                        double *y, hipStream_t st) {
+    PHAST_REQUIRE(pl && d && y && d != y, "phast_hex8_damage_action: bad argument");
+    DmgArgs3 A;
+    std::memset(&A, 0, sizeof(A));
+    A.d = d;
+    A.H = H;
+    A.Gc = pl->Gc;
+    A.y = y;
+    A.scale = scale;
+    A.kz = pl->kz;
+    if (pl->tx == 16 && pl->ty == 16) return launch_damage_t<16, 16>(pl, A, st);
+    if (pl->tx == 16 && pl->ty == 8) return launch_damage_t<16, 8>(pl, A, st);
+    if (pl->tx == 8 && pl->ty == 8) return launch_damage_t<8, 8>(pl, A, st);
+    if (pl->tx == 32 && pl->ty == 8) return launch_damage_t<32, 8>(pl, A, st);
+    set_error("unsupported hex8 tile");
+    return PHAST_ERR_UNSUPPORTED;
+}
+
+extern "C" int phast_hex8_damage_rhs(const phast_hex8_plan *pl, const double *H, double scale, double *b,
+                                     hipStream_t st) {
+    PHAST_REQUIRE(pl && b, "phast_hex8_damage_rhs: null argument");
+    hipLaunchKernelGGL(hex8_damage_rd_kernel, dim3((unsigned)ceil_div(pl->n, 256)), dim3(256), 0, st, pl->geo, H,
+                       pl->Gc, scale, b, (double *)nullptr);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_damage_diagonal(const phast_hex8_plan *pl, const double *H, double scale, double *diag,
+                                          hipStream_t st) {
+    PHAST_REQUIRE(pl && diag, "phast_hex8_damage_diagonal: null argument");
+    hipLaunchKernelGGL(hex8_damage_rd_kernel, dim3((unsigned)ceil_div(pl->n, 256)), dim3(256), 0, st, pl->geo, H,
+                       pl->Gc, scale, (double *)nullptr, diag);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}

@@ -1,0 +1,715 @@
+// Device-resident Jacobi PCG engines and the box-KKT active-set kernels (FP64).
+//
+// Two Krylov loops mirror the reference's two PCG flavours exactly:
+//   * pcg_flags  -- solid3d/linalg.py:99-220: alpha/beta masked by device flags, the host
+//                   reads (nonfinite, breakdown, active, rr, done) every check_every
+//                   iterations; iterates after convergence leave x unchanged.
+//   * pcg_stop   -- solvers/box_kkt_h8.py:256-347: stops at the first iteration with
+//                   ||r|| <= tol max(||r0||, 1); breakdowns raise.
+// All scalars live on the device; every reduction is a fixed-order two-pass sum
+// (per-block wave64 butterflies into partials, then one block), so runs are bitwise
+// reproducible.  Operators are supplied as host callbacks that enqueue an apply kernel.
+#include "common.h"
+
+#include <cstring>
+#include <vector>
+
+struct phast_hex8_plan;  // defined in hex8.hip
+
+namespace phast {
+
+constexpr int kVecBlocks = 1024;  // grid of the vector kernels (grid-stride)
+
+struct PcgState {
+    double rz, rz_new, pAp, alpha, beta, rr, thr2, b_norm;
+    int32_t iter, done, active, breakdown, nonfinite, steps, status, pad;
+};
+
+enum : int { PS_RUNNING = 0, PS_CONVERGED = 1, PS_NOT_PD = 2, PS_NONPOS_RZ = 3, PS_NONFINITE = 4, PS_MAXITER = 5 };
+
+__device__ __forceinline__ void write_parts(double *part, double a, double b, double c, double *red) {
+    const double sa = block_sum<256>(a, red);
+    const double sb = block_sum<256>(b, red);
+    const double sc = block_sum<256>(c, red);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sa;
+        part[kVecBlocks + blockIdx.x] = sb;
+        part[2 * kVecBlocks + blockIdx.x] = sc;
+    }
+}
+
+__device__ __forceinline__ void write_parts5(double *part, const double v[5], double *red) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double t = block_sum<256>(v[k], red);
+        if (threadIdx.x == 0) part[k * kVecBlocks + blockIdx.x] = t;
+    }
+}
+
+__device__ __forceinline__ double sum_parts(const double *part, int k, double *red) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < kVecBlocks; i += 256) s += part[k * kVecBlocks + i];
+    return block_sum<256>(s, red);
+}
+
+#define GRID_STRIDE(i, n) for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < (n); i += (int64_t)kVecBlocks * 256)
+
+// y <- free ? y : 0 ; partials: x.y, count(!finite(y))
+__global__ __launch_bounds__(256) void k_mask_dot(int64_t n, const double *x, double *y, const uint8_t *fixed,
+                                                  const PcgState *S, double *part) {
+    __shared__ double red[4];
+    if (S && S->done) return;
+    double s = 0.0, bad = 0.0;
+    GRID_STRIDE(i, n) {
+        double v = y[i];
+        if (fixed && fixed[i]) {
+            v = 0.0;
+            y[i] = 0.0;
+        }
+        if (!isfinite(v)) bad += 1.0;
+        s += x[i] * v;
+    }
+    write_parts(part, s, bad, 0.0, red);
+}
+
+// generic partials: x.x, x.y, y.y over masked vectors
+__global__ __launch_bounds__(256) void k_dots(int64_t n, const double *x, const double *y, double *part) {
+    __shared__ double red[4];
+    double a = 0.0, b = 0.0, c = 0.0, bad = 0.0;
+    GRID_STRIDE(i, n) {
+        const double xv = x[i];
+        a += xv * xv;
+        if (!isfinite(xv)) bad += 1.0;
+        if (y) {
+            const double yv = y[i];
+            b += xv * yv;
+            c += yv * yv;
+            if (!isfinite(yv)) bad += 1.0;
+        }
+    }
+    (void)c;
+    write_parts(part, a, b, bad, red);
+}
+
+// linalg.py:194-203: x += a p, r -= a Ap, z = r * inv_diag; partials r.z, r.r
+// (mode 1, box-KKT: z = r / diag; partials r.r, r.z).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_update(int64_t n, double *x, double *r, const double *p, const double *Ap,
+                                                const double *dg, double *z, const PcgState *S, double *part) {
+    __shared__ double red[4];
+    if (S->done) return;
+    const double a = S->alpha;
+    double rz = 0.0, rr = 0.0;
+    GRID_STRIDE(i, n) {
+        x[i] = x[i] + a * p[i];
+        const double ri = r[i] - a * Ap[i];
+        r[i] = ri;
+        const double zi = MODE == 0 ? ri * dg[i] : ri / dg[i];
+        z[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+    write_parts(part, rz, rr, 0.0, red);
+}
+
+__global__ __launch_bounds__(256) void k_pupdate(int64_t n, double *p, const double *z, const PcgState *S) {
+    if (S->done) return;
+    const double b = S->beta;
+    GRID_STRIDE(i, n) p[i] = z[i] + b * p[i];
+}
+
+// linalg.pcg scalar steps
+__global__ __launch_bounds__(256) void k_scalar_flags_a(PcgState *S, const double *part) {
+    __shared__ double red[4];
+    const double pAp = sum_parts(part, 0, red);
+    const double bad = sum_parts(part, 1, red);
+    if (threadIdx.x) return;
+    const bool positive = pAp > 0.0;
+    if (S->active && !positive) S->breakdown = 1;
+    if (!isfinite(pAp) || bad > 0.0) S->nonfinite = 1;
+    const bool step = S->active && positive;
+    S->pAp = pAp;
+    S->alpha = step ? S->rz / pAp : 0.0;
+    S->steps += step ? 1 : 0;
+    S->iter += 1;
+}
+__global__ __launch_bounds__(256) void k_scalar_flags_b(PcgState *S, const double *part) {
+    __shared__ double red[4];
+    const double rz_new = sum_parts(part, 0, red);
+    const double rr = sum_parts(part, 1, red);
+    if (threadIdx.x) return;
+    if (!isfinite(rz_new) || !isfinite(rr)) S->nonfinite = 1;
+    S->rr = rr;
+    S->active = S->active && (rr > S->thr2);
+    S->beta = (S->active && S->rz > 0.0) ? rz_new / S->rz : 0.0;
+    S->rz = rz_new;
+}
+
+// box_kkt_h8._pcg scalar steps (stop at first convergence; breakdowns end the loop)
+__global__ __launch_bounds__(256) void k_scalar_stop_a(PcgState *S, const double *part) {
+    __shared__ double red[4];
+    if (S->done) return;
+    const double den = sum_parts(part, 0, red);
+    const double bad = sum_parts(part, 1, red);
+    if (threadIdx.x) return;
+    S->iter += 1;
+    if (bad > 0.0) {
+        S->done = 1;
+        S->status = PS_NONFINITE;
+        return;
+    }
+    if (!isfinite(den) || den <= 0.0) {
+        S->done = 1;
+        S->status = PS_NOT_PD;
+        return;
+    }
+    S->pAp = den;
+    S->alpha = S->rz / den;
+}
+__global__ __launch_bounds__(256) void k_scalar_stop_b(PcgState *S, const double *part, int32_t max_iter) {
+    __shared__ double red[4];
+    if (S->done) return;
+    const double rz_new = sum_parts(part, 0, red);
+    const double rr = sum_parts(part, 1, red);
+    if (threadIdx.x) return;
+    S->rr = rr;
+    if (sqrt(rr) <= S->thr2) {  // thr2 holds tol * reference_norm here
+        S->done = 1;
+        S->status = PS_CONVERGED;
+        return;
+    }
+    if (!isfinite(rz_new) || rz_new <= 0.0) {
+        S->done = 1;
+        S->status = PS_NONPOS_RZ;
+        return;
+    }
+    S->beta = rz_new / S->rz;
+    S->rz = rz_new;
+    if (S->iter >= max_iter) {
+        S->done = 1;
+        S->status = PS_MAXITER;
+    }
+}
+
+// elementwise helpers
+__global__ __launch_bounds__(256) void k_axpby_mask(int64_t n, double alpha, const double *x, double beta,
+                                                    const double *y, const uint8_t *fixed, double fixed_val,
+                                                    double *out) {
+    GRID_STRIDE(i, n) {
+        double v = alpha * x[i] + (y ? beta * y[i] : 0.0);
+        if (fixed && fixed[i]) v = fixed_val;
+        out[i] = v;
+    }
+}
+__global__ __launch_bounds__(256) void k_where(int64_t n, const uint8_t *mask, const double *a, const double *b,
+                                               double bval, double *out) {
+    GRID_STRIDE(i, n) out[i] = mask[i] ? a[i] : (b ? b[i] : bval);
+}
+__global__ __launch_bounds__(256) void k_zero_fixed(int64_t n, const uint8_t *fixed, double *out) {
+    GRID_STRIDE(i, n) if (fixed[i]) out[i] = 0.0;
+}
+__global__ __launch_bounds__(256) void k_invert(int64_t n, const double *d, double *out) {
+    GRID_STRIDE(i, n) out[i] = 1.0 / d[i];
+}
+__global__ __launch_bounds__(256) void k_diag_stats(int64_t n, const double *dg, const double *b, double *part) {
+    __shared__ double red[4];
+    double nonpos = 0.0, bad = 0.0, bb = 0.0;
+    GRID_STRIDE(i, n) {
+        const double v = dg[i];
+        if (!(v > 0.0)) nonpos += 1.0;
+        if (!isfinite(v)) bad += 1.0;
+        if (b) {
+            const double bv = b[i];
+            bb += bv * bv;
+            if (!isfinite(bv)) bad += 1.0;
+        }
+    }
+    write_parts(part, nonpos, bad, bb, red);
+}
+__global__ __launch_bounds__(256) void k_final_parts(const double *part, double *out3, int nv) {
+    __shared__ double red[4];
+    for (int k = 0; k < nv; ++k) {
+        const double s = sum_parts(part, k, red);
+        if (threadIdx.x == 0) out3[k] = s;
+    }
+}
+
+// box-KKT active-set bookkeeping (box_kkt_h8.py:672-685, 731-733):
+//   grad = A cand - b ; add/release masks ; counts of each.
+// states: lower_active, upper_active, prescribed (lb == ub).
+__global__ __launch_bounds__(256) void k_box_update(int64_t n, const double *Acand, const double *b, const double *cand,
+                                                    const double *lo, const double *up, uint8_t *lact, uint8_t *uact,
+                                                    const uint8_t *presc, double kkt_tol, int apply, double *grad,
+                                                    double *part) {
+    __shared__ double red[4];
+    double cnt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    GRID_STRIDE(i, n) {
+        const double g = Acand[i] - b[i];
+        grad[i] = g;
+        const bool la = lact[i], ua = uact[i], fixed = la || ua || presc[i];
+        const bool add_l = !fixed && cand[i] < lo[i];
+        const bool add_u = !fixed && cand[i] > up[i];
+        const bool rel_l = la && g < -kkt_tol;
+        const bool rel_u = ua && g > kkt_tol;
+        cnt[0] += add_l ? 1.0 : 0.0;
+        cnt[1] += add_u ? 1.0 : 0.0;
+        cnt[2] += rel_l ? 1.0 : 0.0;
+        cnt[3] += rel_u ? 1.0 : 0.0;
+        cnt[4] += fixed ? 0.0 : 1.0;
+        if (apply) {
+            lact[i] = (la || add_l) && !rel_l;
+            uact[i] = (ua || add_u) && !rel_u;
+        }
+    }
+    write_parts5(part, cnt, red);
+}
+
+// ---------------------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------------------
+
+typedef int (*ApplyFn)(void *ctx, const double *in, double *out, hipStream_t st);
+
+struct Work {
+    double *part = nullptr;
+    PcgState *S = nullptr;
+    PcgState *hS = nullptr;
+    double *h3 = nullptr, *d3 = nullptr;
+    double *r = nullptr, *p = nullptr, *z = nullptr, *Ap = nullptr, *dg = nullptr;
+    int64_t cap = 0;
+
+    int ensure(int64_t n) {
+        if (!part) {
+            PHAST_HIP(hipMalloc(&part, sizeof(double) * 8 * kVecBlocks));
+            PHAST_HIP(hipMalloc(&S, sizeof(PcgState)));
+            PHAST_HIP(hipHostMalloc(&hS, sizeof(PcgState), hipHostMallocDefault));
+            PHAST_HIP(hipHostMalloc(&h3, sizeof(double) * 8, hipHostMallocDefault));
+            PHAST_HIP(hipMalloc(&d3, sizeof(double) * 8));
+        }
+        if (n > cap) {
+            (void)hipFree(r);
+            (void)hipFree(p);
+            (void)hipFree(z);
+            (void)hipFree(Ap);
+            (void)hipFree(dg);
+            PHAST_HIP(hipMalloc(&r, sizeof(double) * n));
+            PHAST_HIP(hipMalloc(&p, sizeof(double) * n));
+            PHAST_HIP(hipMalloc(&z, sizeof(double) * n));
+            PHAST_HIP(hipMalloc(&Ap, sizeof(double) * n));
+            PHAST_HIP(hipMalloc(&dg, sizeof(double) * n));
+            cap = n;
+        }
+        return PHAST_OK;
+    }
+    void release() {
+        (void)hipFree(part);
+        (void)hipFree(S);
+        if (hS) (void)hipHostFree(hS);
+        if (h3) (void)hipHostFree(h3);
+        (void)hipFree(d3);
+        (void)hipFree(r);
+        (void)hipFree(p);
+        (void)hipFree(z);
+        (void)hipFree(Ap);
+        (void)hipFree(dg);
+        *this = Work();
+    }
+};
+
+static Work g_work;  // solver scratch shared by the 3D solves of one process
+
+static inline int nblk(int64_t n) { return (int)std::min<int64_t>(kVecBlocks, ceil_div(std::max<int64_t>(n, 1), 256)); }
+
+// three fixed-order sums of the current partials -> host
+static int readn(Work &W, hipStream_t st, double *out, int nv) {
+    hipLaunchKernelGGL(k_final_parts, dim3(1), dim3(256), 0, st, W.part, W.d3, nv);
+    PHAST_LAUNCHED();
+    PHAST_HIP(hipMemcpyAsync(W.h3, W.d3, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
+    PHAST_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < nv; ++k) out[k] = W.h3[k];
+    return PHAST_OK;
+}
+static int read3(Work &W, hipStream_t st, double out[3]) { return readn(W, st, out, 3); }
+
+static int zero_parts(Work &W, hipStream_t st) {
+    PHAST_HIP(hipMemsetAsync(W.part, 0, sizeof(double) * 8 * kVecBlocks, st));
+    return PHAST_OK;
+}
+
+// linalg.pcg on the free set: `free_apply` must return the masked action.
+// info: [iterations, converged, rhs_norm, residual_norm, relative_residual]; returns
+// PHAST_ERR_NONFINITE / PHAST_ERR_NONPOSITIVE on breakdown (messages as in the reference).
+static int pcg_flags(ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, const double *b, const double *diag,
+                     const double *x0, double *x, double rtol, double atol, int maxiter, int check_every,
+                     double *info, hipStream_t st) {
+    Work &W = g_work;
+    if (int rc = W.ensure(n)) return rc;
+    const int nb = nblk(n);
+    // stats: ||b||, finite(b), diag > 0, finite(diag)   (linalg.py:139-155)
+    PHAST_TRY0(zero_parts(W, st));
+    hipLaunchKernelGGL(k_diag_stats, dim3(nb), dim3(256), 0, st, n, diag, b, W.part);
+    PHAST_LAUNCHED();
+    double s3[3];
+    if (int rc = read3(W, st, s3)) return rc;
+    const double b_norm = std::sqrt(s3[2]);
+    if (!std::isfinite(b_norm)) {
+        set_error("PCG right-hand side contains a non-finite value");
+        return PHAST_ERR_NONFINITE;
+    }
+    if (s3[1] > 0.0) {
+        set_error("PCG diagonal contains a non-finite value");
+        return PHAST_ERR_NONFINITE;
+    }
+    if (s3[0] > 0.0) {
+        set_error("PCG diagonal has a non-positive entry (non-positive curvature)");
+        return PHAST_ERR_NONPOSITIVE;
+    }
+    if (b_norm == 0.0) {
+        hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, 0.0, b, 0.0, (const double *)nullptr,
+                           (const uint8_t *)nullptr, 0.0, x);
+        PHAST_LAUNCHED();
+        info[0] = 0; info[1] = 1; info[2] = 0; info[3] = 0; info[4] = 0;
+        return PHAST_OK;
+    }
+    const double threshold = std::max(rtol * b_norm, atol);
+    double r_norm = b_norm;
+    if (x0) {
+        PHAST_HIP(hipMemcpyAsync(x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        if (int rc = apply(ctx, x, W.Ap, st)) return rc;
+        hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, 1.0, b, -1.0, W.Ap, fixed, 0.0, W.r);
+        PHAST_LAUNCHED();
+        PHAST_TRY0(zero_parts(W, st));
+        hipLaunchKernelGGL(k_dots, dim3(nb), dim3(256), 0, st, n, W.r, (const double *)nullptr, W.part);
+        PHAST_LAUNCHED();
+        if (int rc = read3(W, st, s3)) return rc;
+        r_norm = std::sqrt(s3[0]);
+        if (!std::isfinite(r_norm)) {
+            set_error("PCG initial residual is not finite");
+            return PHAST_ERR_NONFINITE;
+        }
+    } else {
+        PHAST_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+        PHAST_HIP(hipMemcpyAsync(W.r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    }
+    if (r_norm <= threshold) {
+        info[0] = 0; info[1] = 1; info[2] = b_norm; info[3] = r_norm; info[4] = r_norm / b_norm;
+        return PHAST_OK;
+    }
+    // inv_diag, z = r inv_diag, p = z, rz = r.z, rr = r.r
+    hipLaunchKernelGGL(k_invert, dim3(nb), dim3(256), 0, st, n, diag, W.dg);
+    PHAST_LAUNCHED();
+    {
+        PcgState h;
+        std::memset(&h, 0, sizeof(h));
+        h.active = 1;
+        h.thr2 = threshold * threshold;
+        h.b_norm = b_norm;
+        h.alpha = 0.0;
+        *W.hS = h;
+        PHAST_HIP(hipMemcpyAsync(W.S, W.hS, sizeof(PcgState), hipMemcpyHostToDevice, st));
+        // z = r * inv_diag via an update with alpha = 0
+        hipLaunchKernelGGL((k_update<0>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.r, W.r, W.dg, W.z, W.S, W.part);
+        PHAST_LAUNCHED();
+        PHAST_HIP(hipMemcpyAsync(W.p, W.z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_scalar_flags_b, dim3(1), dim3(256), 0, st, W.S, W.part);  // sets rz, rr (active stays)
+        PHAST_LAUNCHED();
+    }
+    // restore beta-free state: k_scalar_flags_b may clear active if r is already tiny (it is not:
+    // r_norm > threshold was checked above).
+    for (int it = 1; it <= maxiter; ++it) {
+        if (int rc = apply(ctx, W.p, W.Ap, st)) return rc;
+        hipLaunchKernelGGL(k_mask_dot, dim3(nb), dim3(256), 0, st, n, W.p, W.Ap, fixed, (const PcgState *)nullptr,
+                           W.part);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL(k_scalar_flags_a, dim3(1), dim3(256), 0, st, W.S, W.part);
+        PHAST_LAUNCHED();
+        // x, r, z update (alpha = 0 for inactive iterations leaves x and r unchanged)
+        hipLaunchKernelGGL((k_update<0>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S, W.part);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL(k_scalar_flags_b, dim3(1), dim3(256), 0, st, W.S, W.part);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL(k_pupdate, dim3(nb), dim3(256), 0, st, n, W.p, W.z, W.S);
+        PHAST_LAUNCHED();
+        if (it % check_every == 0 || it == maxiter) {
+            PHAST_HIP(hipMemcpyAsync(W.hS, W.S, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+            PHAST_HIP(hipStreamSynchronize(st));
+            const PcgState &h = *W.hS;
+            if (h.nonfinite) {
+                set_error("PCG produced a non-finite value by iteration %d", it);
+                return PHAST_ERR_NONFINITE;
+            }
+            if (h.breakdown) {
+                set_error("PCG met non-positive curvature p.A p <= 0 by iteration %d; the operator is not positive "
+                          "definite on the free set", it);
+                return PHAST_ERR_NONPOSITIVE;
+            }
+            r_norm = std::sqrt(h.rr);
+            if (!h.active) {
+                info[0] = h.steps; info[1] = 1; info[2] = b_norm; info[3] = r_norm; info[4] = r_norm / b_norm;
+                return PHAST_OK;
+            }
+        }
+    }
+    info[0] = maxiter; info[1] = 0; info[2] = b_norm; info[3] = r_norm; info[4] = r_norm / b_norm;
+    return PHAST_OK;
+}
+
+// box_kkt_h8._pcg on the free set (vectors zero on `fixed`).  rhs must already be masked.
+// info: [iterations, converged, initial_norm, residual_norm, relative_residual]
+static int pcg_stop(ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, const double *rhs, const double *diag,
+                    const double *x0, double *x, double tol, int max_iter, int poll, double *info, hipStream_t st) {
+    Work &W = g_work;
+    if (int rc = W.ensure(n)) return rc;
+    const int nb = nblk(n);
+    // diag: the reduced diagonal; on fixed entries use 1 (vectors are zero there anyway)
+    hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, 1.0, diag, 0.0, (const double *)nullptr, fixed,
+                       1.0, W.dg);
+    PHAST_LAUNCHED();
+    if (x0) {
+        hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, 1.0, x0, 0.0, (const double *)nullptr, fixed,
+                           0.0, x);
+        PHAST_LAUNCHED();
+    } else {
+        PHAST_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+    }
+    if (int rc = apply(ctx, x, W.Ap, st)) return rc;
+    hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, 1.0, rhs, -1.0, W.Ap, fixed, 0.0, W.r);
+    PHAST_LAUNCHED();
+    PHAST_TRY0(zero_parts(W, st));
+    hipLaunchKernelGGL(k_dots, dim3(nb), dim3(256), 0, st, n, W.r, (const double *)nullptr, W.part);
+    PHAST_LAUNCHED();
+    double s3[3];
+    if (int rc = read3(W, st, s3)) return rc;
+    if (s3[2] > 0.0) {
+        set_error("PCG action returned a non-finite vector");
+        return PHAST_ERR_NONFINITE;
+    }
+    const double initial = std::sqrt(s3[0]);
+    const double ref = std::max(initial, 1.0);
+    if (initial <= tol * ref) {
+        info[0] = 0; info[1] = 1; info[2] = initial; info[3] = initial; info[4] = initial / ref;
+        return PHAST_OK;
+    }
+    // preconditioned = r / diag ; direction = preconditioned ; rz = r.preconditioned
+    {
+        PcgState h;
+        std::memset(&h, 0, sizeof(h));
+        h.thr2 = tol * ref;
+        *W.hS = h;
+        PHAST_HIP(hipMemcpyAsync(W.S, W.hS, sizeof(PcgState), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL((k_update<1>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.r, W.r, W.dg, W.z, W.S, W.part);
+        PHAST_LAUNCHED();
+        if (int rc = read3(W, st, s3)) return rc;  // [r.z, r.r, -]
+        if (!std::isfinite(s3[0]) || s3[0] <= 0.0) {
+            set_error("PCG encountered a non-positive initial residual product");
+            return PHAST_ERR_NONPOSITIVE;
+        }
+        W.hS->rz = s3[0];
+        PHAST_HIP(hipMemcpyAsync(W.S, W.hS, sizeof(PcgState), hipMemcpyHostToDevice, st));
+        PHAST_HIP(hipMemcpyAsync(W.p, W.z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    }
+    int it = 0;
+    while (it < max_iter) {
+        if (int rc = apply(ctx, W.p, W.Ap, st)) return rc;
+        hipLaunchKernelGGL(k_mask_dot, dim3(nb), dim3(256), 0, st, n, W.p, W.Ap, fixed, W.S, W.part);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL(k_scalar_stop_a, dim3(1), dim3(256), 0, st, W.S, W.part);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL((k_update<1>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S, W.part);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL(k_scalar_stop_b, dim3(1), dim3(256), 0, st, W.S, W.part, max_iter);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL(k_pupdate, dim3(nb), dim3(256), 0, st, n, W.p, W.z, W.S);
+        PHAST_LAUNCHED();
+        ++it;
+        if (it % poll == 0 || it == max_iter) {
+            PHAST_HIP(hipMemcpyAsync(W.hS, W.S, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+            PHAST_HIP(hipStreamSynchronize(st));
+            if (W.hS->done) break;
+        }
+    }
+    const PcgState &h = *W.hS;
+    const double rn = std::sqrt(h.rr);
+    if (h.status == PS_NONFINITE) {
+        set_error("PCG action returned a non-finite vector");
+        return PHAST_ERR_NONFINITE;
+    }
+    if (h.status == PS_NOT_PD) {
+        set_error("PCG operator is not positive definite on the free space");
+        return PHAST_ERR_NONPOSITIVE;
+    }
+    if (h.status == PS_NONPOS_RZ) {
+        set_error("PCG encountered a non-positive residual product");
+        return PHAST_ERR_NONPOSITIVE;
+    }
+    info[0] = h.iter;
+    info[1] = h.status == PS_CONVERGED ? 1 : 0;
+    info[2] = initial;
+    info[3] = rn;
+    info[4] = rn / ref;
+    return PHAST_OK;
+}
+
+}  // namespace phast
+
+using namespace phast;
+
+// ---------------------------------------------------------------------------------------
+// operator adaptors and C entry points
+// ---------------------------------------------------------------------------------------
+
+struct MechCtx {
+    const phast_hex8_plan *plan;
+    const double *g, *d;
+    const uint8_t *fixed;
+    int64_t n;
+};
+
+static int mech_apply_cb(void *c, const double *in, double *out, hipStream_t st) {
+    auto *m = static_cast<MechCtx *>(c);
+    int rc = m->d ? phast_hex8_mech_action_d(m->plan, in, m->d, out, st) : phast_hex8_mech_action(m->plan, in, m->g, out, st);
+    if (rc) return rc;
+    if (m->fixed) {
+        hipLaunchKernelGGL(k_zero_fixed, dim3(nblk(m->n)), dim3(256), 0, st, m->n, m->fixed, out);
+        PHAST_LAUNCHED();
+    }
+    return PHAST_OK;
+}
+
+struct DmgCtx {
+    const phast_hex8_plan *plan;
+    const double *H;
+    double scale;
+};
+
+static int dmg_apply_cb(void *c, const double *in, double *out, hipStream_t st) {
+    auto *m = static_cast<DmgCtx *>(c);
+    return phast_hex8_damage_action(m->plan, in, m->H, m->scale, out, st);
+}
+
+__global__ void k_not_mask(int64_t n, const uint8_t *fixed, uint8_t *freem) {
+    GRID_STRIDE(i, n) freem[i] = fixed[i] ? 0 : 1;
+}
+
+// solid3d/linalg.py:223-287 solve_constrained with the hex8 mechanics action A(g)
+// (d given: g formed in-kernel from nodal damage).  info[5] as in pcg_flags.
+extern "C" int phast_hex8_mech_solve(const phast_hex8_plan *plan, const double *g, const double *d,
+                                     const uint8_t *fixed, const double *fixed_values, const double *rhs,
+                                     const double *x0, const double *diag, double rtol, double atol, int32_t maxiter,
+                                     int32_t check_every, int64_t n_mech, double *x_out, double *info,
+                                     hipStream_t st) {
+    PHAST_REQUIRE(plan && fixed && fixed_values && diag && x_out && info, "phast_hex8_mech_solve: null argument");
+    PHAST_REQUIRE(maxiter >= 1 && check_every >= 1, "maxiter and check_every must be >= 1");
+    PHAST_REQUIRE(!(rtol == 0.0 && atol == 0.0) && rtol >= 0.0 && atol >= 0.0, "rtol and atol must not both be zero");
+    Work &W = g_work;
+    if (int rc = W.ensure(n_mech)) return rc;
+    const int64_t n = n_mech;
+    const int nb = nblk(n);
+    double *xc = nullptr, *bf = nullptr, *df = nullptr, *x0f = nullptr, *y = nullptr;
+    PHAST_HIP(hipMallocAsync((void **)&xc, sizeof(double) * n, st));
+    PHAST_HIP(hipMallocAsync((void **)&bf, sizeof(double) * n, st));
+    PHAST_HIP(hipMallocAsync((void **)&df, sizeof(double) * n, st));
+    PHAST_HIP(hipMallocAsync((void **)&y, sizeof(double) * n, st));
+    if (x0) PHAST_HIP(hipMallocAsync((void **)&x0f, sizeof(double) * n, st));
+    uint8_t *freem = nullptr;
+    PHAST_HIP(hipMallocAsync((void **)&freem, (size_t)n, st));
+    hipLaunchKernelGGL(k_not_mask, dim3(nb), dim3(256), 0, st, n, fixed, freem);
+    // x_c = where(fixed, values, 0); image = A x_c; b_f = where(free, rhs - image, 0)
+    hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, fixed, fixed_values, (const double *)nullptr, 0.0, xc);
+    PHAST_LAUNCHED();
+    MechCtx raw{plan, g, d, nullptr, n};
+    int rc = mech_apply_cb(&raw, xc, W.Ap, st);
+    if (rc == PHAST_OK) {
+        hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, rhs ? 1.0 : 0.0, rhs ? rhs : W.Ap, -1.0, W.Ap,
+                           fixed, 0.0, bf);
+        hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, diag, (const double *)nullptr, 1.0, df);
+        if (x0) hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, x0, (const double *)nullptr, 0.0, x0f);
+        MechCtx ctx{plan, g, d, fixed, n};
+        rc = pcg_flags(mech_apply_cb, &ctx, n, fixed, bf, df, x0f, y, rtol, atol, maxiter, check_every, info, st);
+        // x = where(free, y, x_c)
+        if (rc == PHAST_OK)
+            hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, y, xc, 0.0, x_out);
+    }
+    (void)hipFreeAsync(xc, st);
+    (void)hipFreeAsync(bf, st);
+    (void)hipFreeAsync(df, st);
+    (void)hipFreeAsync(y, st);
+    if (x0f) (void)hipFreeAsync(x0f, st);
+    (void)hipFreeAsync(freem, st);
+    return rc;
+}
+
+// box_kkt_h8.py:350-432 solve_h8_scalar_dirichlet_pcg for the scaled AT2 damage system
+// (operator.py:1496-1509).  b and diag are the scaled rhs / diagonal.  info[5] as pcg_stop.
+extern "C" int phast_hex8_damage_dirichlet_pcg(const phast_hex8_plan *plan, const double *H, double scale,
+                                               const double *b, const double *diag, const uint8_t *fixed,
+                                               const double *values, const double *initial, double tol,
+                                               int32_t max_iter, int64_t n, double *out, double *info,
+                                               hipStream_t st) {
+    PHAST_REQUIRE(plan && b && diag && fixed && values && out && info, "phast_hex8_damage_dirichlet_pcg: null argument");
+    PHAST_REQUIRE(tol > 0.0 && max_iter >= 1, "tolerance must be positive and max_iterations >= 1");
+    Work &W = g_work;
+    if (int rc = W.ensure(n)) return rc;
+    const int nb = nblk(n);
+    double *known = nullptr, *rr = nullptr, *y = nullptr;
+    PHAST_HIP(hipMallocAsync((void **)&known, sizeof(double) * n, st));
+    PHAST_HIP(hipMallocAsync((void **)&rr, sizeof(double) * n, st));
+    PHAST_HIP(hipMallocAsync((void **)&y, sizeof(double) * n, st));
+    hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, fixed, values, (const double *)nullptr, 0.0, known);
+    DmgCtx ctx{plan, H, scale};
+    int rc = dmg_apply_cb(&ctx, known, W.Ap, st);
+    if (rc == PHAST_OK) {
+        // reduced rhs = (b - A known)[free]
+        hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, 1.0, b, -1.0, W.Ap, fixed, 0.0, rr);
+        struct Masked {
+            DmgCtx *c;
+            const uint8_t *fixed;
+            int64_t n;
+        } mctx{&ctx, fixed, n};
+        auto masked = [](void *c, const double *in, double *out, hipStream_t s) -> int {
+            auto *m = static_cast<Masked *>(c);
+            int r = dmg_apply_cb(m->c, in, out, s);
+            if (r) return r;
+            hipLaunchKernelGGL(k_zero_fixed, dim3(nblk(m->n)), dim3(256), 0, s, m->n, m->fixed, out);
+            PHAST_LAUNCHED();
+            return PHAST_OK;
+        };
+        rc = pcg_stop(masked, &mctx, n, fixed, rr, diag, initial, y, tol, max_iter, 4, info, st);
+        // solution = known with free entries from y
+        if (rc == PHAST_OK)
+            hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, fixed, known, y, 0.0, out);
+    }
+    (void)hipFreeAsync(known, st);
+    (void)hipFreeAsync(rr, st);
+    (void)hipFreeAsync(y, st);
+    return rc;
+}
+
+// One active-set bookkeeping pass: grad = A(cand) - b (scaled system), add/release counts
+// and, with apply != 0, the updated masks.  counts: [added_lower, added_upper,
+// released_lower, released_upper, free]; grad (n) is written.
+extern "C" int phast_hex8_box_update(const phast_hex8_plan *plan, const double *H, double scale, const double *b,
+                                     const double *cand, const double *lower, const double *upper,
+                                     uint8_t *lower_active, uint8_t *upper_active, const uint8_t *prescribed,
+                                     double kkt_tol, int32_t apply, int64_t n, double *grad, int64_t *counts,
+                                     hipStream_t st) {
+    PHAST_REQUIRE(plan && b && cand && lower && upper && lower_active && upper_active && prescribed && grad && counts,
+                  "phast_hex8_box_update: null argument");
+    Work &W = g_work;
+    if (int rc = W.ensure(n)) return rc;
+    DmgCtx ctx{plan, H, scale};
+    if (int rc = dmg_apply_cb(&ctx, cand, W.Ap, st)) return rc;
+    PHAST_TRY0(zero_parts(W, st));
+    hipLaunchKernelGGL(k_box_update, dim3(nblk(n)), dim3(256), 0, st, n, W.Ap, b, cand, lower, upper, lower_active,
+                       upper_active, prescribed, kkt_tol, apply, grad, W.part);
+    PHAST_LAUNCHED();
+    double s5[5];
+    if (int rc = readn(W, st, s5, 5)) return rc;
+    for (int k = 0; k < 5; ++k) counts[k] = (int64_t)s5[k];
+    return PHAST_OK;
+}
+
+extern "C" int phast_release_solver_scratch(void) {
+    g_work.release();
+    return PHAST_OK;
+}

@@ -1,0 +1,1330 @@
+// Native-Q4 structured-plate kernels for gfx950 (FP64 throughout).
+//
+// Layout in HBM (packed over all instances of a plan):
+//   nodes  : u, v, a, f as (N,2) interleaved float64 (16 B per node, double2 loads);
+//            d, M, b, r, p, z, Ap, lb as (N,) float64; masks as (N,) uint8.
+//   cells  : H, psi as (E,4) float64 (one 32 B row per cell, Gauss points in
+//            (xi,eta) = (-,-),(+,-),(-,+),(+,+) order).
+//
+// Work decomposition: one 256-thread block per tile of TX x TY nodes of one instance.
+// A block (1) stages the (TX+2)(TY+2) node halo in LDS, (2) evaluates each of the
+// (TX+1)(TY+1) cells touching its nodes once, writing the four corner contributions to
+// LDS, and (3) gathers every owned node's sum from the four cells around it, in cell
+// order.  No atomics: results are deterministic and race-free; each cell is owned (for
+// per-cell outputs such as H) by the tile of its lower-left node.  The block -> (instance,
+// tile) table makes one launch cover a whole batch of plates of uneven sizes.
+#include "common.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace phast {
+
+struct Q4Inst {
+    int32_t nx, ny, tiles_x, tiles_y;
+    int64_t node_off, cell_off;
+    int32_t tile_off, n_tiles;
+    double gx[16], gy[16], Nq[16], w[4];  // [q*4 + a]
+    double dlap[4];                       // sum_q w (gx^2 + gy^2) of corner a
+    double lam, mu, kappa, ezz_coef, eta;
+    double Gc_l0, Gc_over_l0, l0, dt;
+    int32_t split, pad;
+};
+
+// Per-instance state of the device-resident damage CG.
+struct Q4CG {
+    double rz, rz_new, pAp, alpha, rr0, bb, tol_sq, rfn, beta, tol;
+    int32_t iter, max_iter, done, limited, check, restart, status, changed;
+};
+
+constexpr int kParts = 4;  // per-tile partial sums kept between kernels
+
+}  // namespace phast
+
+struct phast_q4_plan {
+    int32_t n_inst = 0, tx = 64, ty = 4, n_tiles = 0;
+    int64_t n_nodes = 0, n_cells = 0;
+    std::vector<phast::Q4Inst> h_inst;
+    phast::Q4Inst *d_inst = nullptr;
+    int32_t *d_block_inst = nullptr;
+    // solver work (allocated lazily on first solve)
+    double *r = nullptr, *p = nullptr, *z = nullptr, *Ap = nullptr, *b = nullptr, *minv = nullptr,
+           *lb = nullptr, *part = nullptr;
+    uint8_t *active = nullptr;
+    phast::Q4CG *cg = nullptr;
+    phast::Q4CG *h_cg = nullptr;  // pinned mirror
+    int32_t last_iters = 0;
+};
+
+namespace phast {
+
+// ---------------------------------------------------------------------------------------
+// per-cell arithmetic
+// ---------------------------------------------------------------------------------------
+
+__device__ __forceinline__ double degrade(double dq, double eta) {
+    const double omd = 1.0 - dq;
+    return omd * omd * (1.0 - eta) + eta;  // physics/material.py:529
+}
+
+// Internal force contributions of one cell at its four corners, and (optionally) psi+ at
+// its four Gauss points.  fem_operators.py:244-254 (strain), 626-630 (g at qp),
+// 301-333 (Amor) / 277-290 (isotropic), 681-699 (assembly), 983-1006 (psi+).
+template <bool PSI>
+__device__ __forceinline__ void cell_mech(const Q4Inst &P, const double ux[4], const double uy[4],
+                                          const double dd[4], double fx[4], double fy[4], double psi[4]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) fx[a] = fy[a] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double exx = 0.0, eyy = 0.0, gxa = 0.0, gxb = 0.0, dq = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            exx += P.gx[q * 4 + a] * ux[a];
+            eyy += P.gy[q * 4 + a] * uy[a];
+            gxa += P.gy[q * 4 + a] * ux[a];
+            gxb += P.gx[q * 4 + a] * uy[a];
+            dq += P.Nq[q * 4 + a] * dd[a];
+        }
+        const double gxy = gxa + gxb;
+        const double g = degrade(dq, P.eta);
+        double sxx, syy, sxy;
+        double tr = 0.0, trp = 0.0, ezz = 0.0;
+        if (P.split == PHAST_SPLIT_AMOR) {
+            const double tr2 = exx + eyy;
+            ezz = P.ezz_coef * tr2;
+            tr = tr2 + ezz;
+            trp = fmax(tr, 0.0);
+            const double trm = tr - trp;
+            const double dxx = exx - tr / 3.0, dyy = eyy - tr / 3.0;
+            sxx = g * (P.kappa * trp + 2.0 * P.mu * dxx) + P.kappa * trm;
+            syy = g * (P.kappa * trp + 2.0 * P.mu * dyy) + P.kappa * trm;
+            sxy = g * P.mu * gxy;
+        } else {
+            const double c11 = P.lam + 2.0 * P.mu;
+            sxx = g * (c11 * exx + P.lam * eyy);
+            syy = g * (P.lam * exx + c11 * eyy);
+            sxy = g * (P.mu * gxy);
+        }
+        const double wq = P.w[q];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            fx[a] += (P.gx[q * 4 + a] * sxx + P.gy[q * 4 + a] * sxy) * wq;
+            fy[a] += (P.gy[q * 4 + a] * syy + P.gx[q * 4 + a] * sxy) * wq;
+        }
+        if (PSI) {
+            const double exy = gxy / 2.0;
+            if (P.split == PHAST_SPLIT_AMOR) {
+                const double dxx = exx - tr / 3.0, dyy = eyy - tr / 3.0, dzz = ezz - tr / 3.0;
+                psi[q] = 0.5 * P.kappa * trp * trp +
+                         P.mu * (dxx * dxx + dyy * dyy + dzz * dzz + 2.0 * exy * exy);
+            } else {
+                const double t = exx + eyy;
+                psi[q] = 0.5 * P.lam * t * t + P.mu * (exx * exx + eyy * eyy + 2.0 * exy * exy);
+            }
+        }
+    }
+}
+
+// AT2 damage operator of one cell: y_a = Gc l0 sum_q w gradN_a.grad x_q + sum_q N_a rho_q x_q w
+// (damage_solver.py:1797-1838); rho_q = 2 H_q + Gc/l0.
+__device__ __forceinline__ void cell_damage(const Q4Inst &P, const double x[4], const double rho[4],
+                                            double gcl0, double y[4]) {
+    double gdx[4], gdy[4], xq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double sx = 0.0, sy = 0.0, sq = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            sx += P.gx[q * 4 + a] * x[a];
+            sy += P.gy[q * 4 + a] * x[a];
+            sq += P.Nq[q * 4 + a] * x[a];
+        }
+        gdx[q] = sx;
+        gdy[q] = sy;
+        xq[q] = sq;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double lap = 0.0, mass = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            lap += P.w[q] * (P.gx[q * 4 + a] * gdx[q] + P.gy[q * 4 + a] * gdy[q]);
+            mass += P.Nq[q * 4 + a] * (rho[q] * xq[q] * P.w[q]);
+        }
+        y[a] = gcl0 * lap + mass;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// tile geometry
+// ---------------------------------------------------------------------------------------
+
+struct Tile {
+    int inst, i0, j0, nxn, nyn;
+};
+
+template <int TX>
+__device__ __forceinline__ Tile tile_of_block(const Q4Inst *inst, const int32_t *block_inst) {
+    constexpr int TY = 256 / TX;
+    Tile t;
+    t.inst = block_inst[blockIdx.x];
+    const Q4Inst &P = inst[t.inst];
+    const int k = blockIdx.x - P.tile_off;
+    t.i0 = (k % P.tiles_x) * TX;
+    t.j0 = (k / P.tiles_x) * TY;
+    t.nxn = P.nx + 1;
+    t.nyn = P.ny + 1;
+    return t;
+}
+
+// ---------------------------------------------------------------------------------------
+// mechanics: internal force / explicit step (+ psi+, history)
+// ---------------------------------------------------------------------------------------
+
+struct MechArgs {
+    const Q4Inst *inst;
+    const int32_t *block_inst;
+    const double2 *u, *v, *a;
+    const double *d;
+    const double2 *fext;
+    const double *minv;
+    const uint8_t *bcm;
+    const double *bcv;
+    const double *Hold;
+    double2 *u1, *v1, *a1, *f;
+    double *Hnew, *psi;
+};
+
+template <int TX, bool STEP, bool HIST>
+__global__ __launch_bounds__(256) void q4_mech_kernel(MechArgs A) {
+    constexpr int TY = 256 / TX;
+    constexpr int NXH = TX + 2, NYH = TY + 2, NXE = TX + 1, NYE = TY + 1;
+    __shared__ double sux[NXH * NYH], suy[NXH * NYH], sd[NXH * NYH];
+    __shared__ double sf[8][NXE * NYE];
+
+    const Tile T = tile_of_block<TX>(A.inst, A.block_inst);
+    const Q4Inst &P = A.inst[T.inst];
+    const double dt = P.dt, c2 = 0.5 * P.dt * P.dt;
+
+    for (int s = threadIdx.x; s < NXH * NYH; s += 256) {
+        const int i = T.i0 + s % NXH - 1, j = T.j0 + s / NXH - 1;
+        double ux = 0.0, uy = 0.0, dd = 0.0;
+        if (i >= 0 && i < T.nxn && j >= 0 && j < T.nyn) {
+            const int64_t n = P.node_off + (int64_t)j * T.nxn + i;
+            const double2 uu = A.u[n];
+            if (STEP) {
+                const double2 vv = A.v[n], aa = A.a[n];
+                ux = (uu.x + dt * vv.x) + c2 * aa.x;  // mechanics_solver.py:160
+                uy = (uu.y + dt * vv.y) + c2 * aa.y;
+                if (A.bcm) {
+                    if (A.bcm[2 * n]) ux = A.bcv[2 * n];
+                    if (A.bcm[2 * n + 1]) uy = A.bcv[2 * n + 1];
+                }
+            } else {
+                ux = uu.x;
+                uy = uu.y;
+            }
+            dd = A.d[n];
+        }
+        sux[s] = ux;
+        suy[s] = uy;
+        sd[s] = dd;
+    }
+    __syncthreads();
+
+    for (int s = threadIdx.x; s < NXE * NYE; s += 256) {
+        const int li = s % NXE - 1, lj = s / NXE - 1;
+        const int ie = T.i0 + li, je = T.j0 + lj;
+        double fx[4] = {0.0, 0.0, 0.0, 0.0}, fy[4] = {0.0, 0.0, 0.0, 0.0};
+        if (ie >= 0 && ie < P.nx && je >= 0 && je < P.ny) {
+            const int b0 = (lj + 1) * NXH + (li + 1);
+            const int id[4] = {b0, b0 + 1, b0 + 1 + NXH, b0 + NXH};
+            double ux[4], uy[4], dd[4], ps[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ux[k] = sux[id[k]];
+                uy[k] = suy[id[k]];
+                dd[k] = sd[id[k]];
+            }
+            const bool own = HIST && li >= 0 && lj >= 0;
+            if (own) {
+                cell_mech<true>(P, ux, uy, dd, fx, fy, ps);
+                const int64_t c = P.cell_off + (int64_t)je * P.nx + ie;
+                const double2 h01 = reinterpret_cast<const double2 *>(A.Hold)[2 * c];
+                const double2 h23 = reinterpret_cast<const double2 *>(A.Hold)[2 * c + 1];
+                double2 n01, n23;  // fmax returns one operand bit-exactly (no NaN inputs)
+                n01.x = h01.x >= ps[0] ? h01.x : ps[0];
+                n01.y = h01.y >= ps[1] ? h01.y : ps[1];
+                n23.x = h23.x >= ps[2] ? h23.x : ps[2];
+                n23.y = h23.y >= ps[3] ? h23.y : ps[3];
+                reinterpret_cast<double2 *>(A.Hnew)[2 * c] = n01;
+                reinterpret_cast<double2 *>(A.Hnew)[2 * c + 1] = n23;
+                if (A.psi) {
+                    reinterpret_cast<double2 *>(A.psi)[2 * c] = make_double2(ps[0], ps[1]);
+                    reinterpret_cast<double2 *>(A.psi)[2 * c + 1] = make_double2(ps[2], ps[3]);
+                }
+            } else {
+                cell_mech<false>(P, ux, uy, dd, fx, fy, ps);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            sf[2 * k][s] = fx[k];
+            sf[2 * k + 1][s] = fy[k];
+        }
+    }
+    __syncthreads();
+
+    {
+        const int s = threadIdx.x;
+        const int li = s % TX, lj = s / TX;
+        const int i = T.i0 + li, j = T.j0 + lj;
+        if (i < T.nxn && j < T.nyn) {
+            // cells (i-1,j-1) corner 2, (i,j-1) corner 3, (i-1,j) corner 1, (i,j) corner 0
+            const int e00 = lj * NXE + li, e10 = e00 + 1, e01 = e00 + NXE, e11 = e01 + 1;
+            const double fx = ((sf[4][e00] + sf[6][e10]) + sf[2][e01]) + sf[0][e11];
+            const double fy = ((sf[5][e00] + sf[7][e10]) + sf[3][e01]) + sf[1][e11];
+            const int64_t n = P.node_off + (int64_t)j * T.nxn + i;
+            if (!STEP) {
+                A.f[n] = make_double2(fx, fy);
+            } else {
+                const double2 vv = A.v[n], aa = A.a[n];
+                const double2 fe = A.fext ? A.fext[n] : make_double2(0.0, 0.0);
+                const double mi = A.minv[n];
+                double ax = mi * (fe.x - fx), ay = mi * (fe.y - fy);  // mechanics_solver.py:194-198
+                double vx = vv.x + 0.5 * dt * (aa.x + ax), vy = vv.y + 0.5 * dt * (aa.y + ay);
+                if (A.bcm) {
+                    if (A.bcm[2 * n]) ax = vx = 0.0;
+                    if (A.bcm[2 * n + 1]) ay = vy = 0.0;
+                }
+                const int sh = (lj + 1) * NXH + li + 1;
+                A.u1[n] = make_double2(sux[sh], suy[sh]);
+                A.v1[n] = make_double2(vx, vy);
+                A.a1[n] = make_double2(ax, ay);
+            }
+        }
+    }
+}
+
+// Element-parallel strain / psi+ (one thread per cell, nodes read from global memory).
+struct CellArgs {
+    const Q4Inst *inst;
+    int n_inst;
+    const double2 *u;
+    double *eps;  // [3][E][4]
+    double *psi;
+    const double *Hold;
+    double *Hnew;
+    int64_t n_cells;
+};
+
+__device__ __forceinline__ int find_inst_by_cell(const Q4Inst *inst, int n_inst, int64_t c) {
+    int lo = 0, hi = n_inst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (inst[mid].cell_off <= c) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void q4_cell_kernel(CellArgs A) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= A.n_cells) return;
+    const Q4Inst &P = A.inst[find_inst_by_cell(A.inst, A.n_inst, c)];
+    const int64_t lc = c - P.cell_off;
+    const int ie = (int)(lc % P.nx), je = (int)(lc / P.nx);
+    const int64_t n0 = P.node_off + (int64_t)je * (P.nx + 1) + ie;
+    const int64_t id[4] = {n0, n0 + 1, n0 + P.nx + 2, n0 + P.nx + 1};
+    double ux[4], uy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double2 uu = A.u[id[k]];
+        ux[k] = uu.x;
+        uy[k] = uu.y;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double exx = 0.0, eyy = 0.0, gxa = 0.0, gxb = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            exx += P.gx[q * 4 + a] * ux[a];
+            eyy += P.gy[q * 4 + a] * uy[a];
+            gxa += P.gy[q * 4 + a] * ux[a];
+            gxb += P.gx[q * 4 + a] * uy[a];
+        }
+        const double gxy = gxa + gxb;
+        if (A.eps) {
+            A.eps[c * 4 + q] = exx;
+            A.eps[(A.n_cells + c) * 4 + q] = eyy;
+            A.eps[(2 * A.n_cells + c) * 4 + q] = gxy;
+        }
+        if (A.psi || A.Hnew) {
+            const double exy = gxy / 2.0;
+            double ps;
+            if (P.split == PHAST_SPLIT_AMOR) {
+                const double tr2 = exx + eyy, ezz = P.ezz_coef * tr2, tr = tr2 + ezz;
+                const double trp = fmax(tr, 0.0);
+                const double dxx = exx - tr / 3.0, dyy = eyy - tr / 3.0, dzz = ezz - tr / 3.0;
+                ps = 0.5 * P.kappa * trp * trp + P.mu * (dxx * dxx + dyy * dyy + dzz * dzz + 2.0 * exy * exy);
+            } else {
+                const double t = exx + eyy;
+                ps = 0.5 * P.lam * t * t + P.mu * (exx * exx + eyy * eyy + 2.0 * exy * exy);
+            }
+            if (A.psi) A.psi[c * 4 + q] = ps;
+            if (A.Hnew) {
+                const double h = A.Hold[c * 4 + q];
+                A.Hnew[c * 4 + q] = h >= ps ? h : ps;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void history_max_kernel(int64_t n, const double *H, const double *psi,
+                                                          double *out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const double h = H[i], p = psi[i];
+        out[i] = h >= p ? h : p;  // torch.maximum for non-NaN inputs (h_update.py:47-49)
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// nodal gathers of per-cell scalar quantities (lumped mass, elem_to_node, damage operator)
+// ---------------------------------------------------------------------------------------
+
+enum DmgOp : int {
+    OP_MASS = 0,       // out = sum_q N_a w                          (M_scalar)
+    OP_E2N = 1,        // out = sum_q N_a f_q w / (M + 1e-30)         (elem_to_node)
+    OP_RHSDIAG = 2,    // b, diag
+    OP_APPLY = 3,      // y = A x
+    OP_CG_INIT = 4,    // r = b - A d, active set, z, p; partials r.r, b.b, rf.z
+    OP_CG_AP = 5,      // Ap = A p; partials p.Ap, min step ratios
+    OP_CG_RESID = 6,   // r = b - A d; new active set; partials rf.rf, changed
+    OP_PC_INIT = 7,    // post-clamp: r = b - A d, z = r Minv, p = z; partials r.r, b.b, r.z
+    OP_PC_AP = 8,      // post-clamp: Ap = A p; partial p.Ap
+    OP_PC_RESID = 9,   // post-clamp true residual r = b - A d
+};
+
+struct DmgArgs {
+    const Q4Inst *inst;
+    const int32_t *block_inst;
+    const double *H;      // (E,4)
+    const double *Gc;     // (E,) or null
+    const double *x;      // gathered field (d, p, field_q for E2N)
+    const double *M;      // lumped mass for E2N
+    double *out, *out2;   // outputs (y / b, diag)
+    // CG vectors
+    double *d, *r, *z, *p, *Ap;
+    const double *b, *minv, *lb;
+    const uint8_t *fixed;
+    uint8_t *active;
+    Q4CG *cg;
+    double *part;
+    int32_t n_tiles;
+};
+
+template <int TX, int OP>
+__global__ __launch_bounds__(256) void q4_dmg_kernel(DmgArgs A) {
+    constexpr int TY = 256 / TX;
+    constexpr int NXH = TX + 2, NYH = TY + 2, NXE = TX + 1, NYE = TY + 1;
+    constexpr bool GATHER_X = (OP == OP_APPLY || OP == OP_CG_INIT || OP == OP_CG_AP || OP == OP_CG_RESID ||
+                               OP == OP_PC_INIT || OP == OP_PC_AP || OP == OP_PC_RESID);
+    constexpr int NOUT = (OP == OP_RHSDIAG) ? 2 : 1;
+    __shared__ double sx[GATHER_X ? NXH * NYH : 1];
+    __shared__ double sy[NOUT * 4][NXE * NYE];
+    __shared__ double red[4];
+
+    const Tile T = tile_of_block<TX>(A.inst, A.block_inst);
+    const Q4Inst &P = A.inst[T.inst];
+    if (OP >= OP_CG_AP && OP != OP_PC_INIT && A.cg[T.inst].done) return;
+    if ((OP == OP_CG_RESID) && !(A.cg[T.inst].limited || A.cg[T.inst].check)) return;
+    if ((OP == OP_PC_RESID) && !A.cg[T.inst].check) return;
+
+    if (GATHER_X) {
+        for (int s = threadIdx.x; s < NXH * NYH; s += 256) {
+            const int i = T.i0 + s % NXH - 1, j = T.j0 + s / NXH - 1;
+            double xv = 0.0;
+            if (i >= 0 && i < T.nxn && j >= 0 && j < T.nyn) xv = A.x[P.node_off + (int64_t)j * T.nxn + i];
+            sx[s] = xv;
+        }
+        __syncthreads();
+    }
+
+    for (int s = threadIdx.x; s < NXE * NYE; s += 256) {
+        const int li = s % NXE - 1, lj = s / NXE - 1;
+        const int ie = T.i0 + li, je = T.j0 + lj;
+        double y[NOUT * 4];
+#pragma unroll
+        for (int k = 0; k < NOUT * 4; ++k) y[k] = 0.0;
+        if (ie >= 0 && ie < P.nx && je >= 0 && je < P.ny) {
+            const int64_t c = P.cell_off + (int64_t)je * P.nx + ie;
+            if (OP == OP_MASS) {
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    double m = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) m += P.Nq[q * 4 + a] * P.w[q];
+                    y[a] = m;
+                }
+            } else if (OP == OP_E2N) {
+                double fq[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) fq[q] = A.x[c * 4 + q];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    double m = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) m += P.Nq[q * 4 + a] * fq[q] * P.w[q];
+                    y[a] = m;
+                }
+            } else {
+                const double gc = A.Gc ? A.Gc[c] : 0.0;
+                const double gcl0 = A.Gc ? gc * P.l0 : P.Gc_l0;
+                const double ratio = A.Gc ? gc / P.l0 : P.Gc_over_l0;
+                double Hq[4], rho[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    Hq[q] = A.H[c * 4 + q];
+                    rho[q] = 2.0 * Hq[q] + ratio;
+                }
+                if (OP == OP_RHSDIAG) {
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        double bb = 0.0, dg = 0.0;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            bb += P.Nq[q * 4 + a] * (2.0 * Hq[q] * P.w[q]);
+                            dg += (P.Nq[q * 4 + a] * P.Nq[q * 4 + a]) * (rho[q] * P.w[q]);
+                        }
+                        y[a] = bb;
+                        y[4 + a] = gcl0 * P.dlap[a] + dg;
+                    }
+                } else {
+                    const int b0 = (lj + 1) * NXH + (li + 1);
+                    const double xe[4] = {sx[b0], sx[b0 + 1], sx[b0 + 1 + NXH], sx[b0 + NXH]};
+                    cell_damage(P, xe, rho, gcl0, y);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NOUT * 4; ++k) sy[k][s] = y[k];
+    }
+    __syncthreads();
+
+    const int s = threadIdx.x;
+    const int li = s % TX, lj = s / TX;
+    const int i = T.i0 + li, j = T.j0 + lj;
+    const bool valid = i < T.nxn && j < T.nyn;
+    const int e00 = lj * NXE + li, e10 = e00 + 1, e01 = e00 + NXE, e11 = e01 + 1;
+    double g0 = 0.0, g1 = 0.0;
+    int64_t n = 0;
+    if (valid) {
+        n = P.node_off + (int64_t)j * T.nxn + i;
+        g0 = ((sy[2][e00] + sy[3][e10]) + sy[1][e01]) + sy[0][e11];
+        if (NOUT == 2) g1 = ((sy[6][e00] + sy[7][e10]) + sy[5][e01]) + sy[4][e11];
+    }
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;  // partial sums of this tile
+    constexpr double INF = __builtin_huge_val();
+    if (OP == OP_CG_AP) {
+        p1 = INF;
+        p2 = INF;
+    }
+    if (valid) {
+        if (OP == OP_MASS || OP == OP_APPLY) {
+            A.out[n] = g0;
+        } else if (OP == OP_E2N) {
+            A.out[n] = g0 / (A.M[n] + 1e-30);
+        } else if (OP == OP_RHSDIAG) {
+            A.out[n] = g0;
+            A.out2[n] = g1;
+        } else if (OP == OP_CG_INIT || OP == OP_CG_RESID) {
+            const double dn = A.d[n], lb = A.lb[n];
+            const bool fx = A.fixed && A.fixed[n];
+            const double bn = A.b[n];
+            double r = bn - g0;
+            if (fx) r = 0.0;
+            const bool act = ((dn <= lb + 1e-14) && (r < 0.0)) || ((dn >= 1.0 - 1e-14) && (r > 0.0)) || fx;
+            const double rf = act ? 0.0 : r;
+            A.r[n] = r;
+            if (OP == OP_CG_RESID) {
+                p1 = (act != (A.active[n] != 0)) ? 1.0 : 0.0;
+                p0 = rf * rf;
+            } else {
+                double zz = rf * A.minv[n];
+                if (act) zz = 0.0;
+                A.z[n] = zz;
+                A.p[n] = zz;
+                p0 = r * r;
+                p1 = bn * bn;
+                p2 = rf * zz;
+            }
+            A.active[n] = act ? 1 : 0;
+        } else if (OP == OP_CG_AP) {
+            const double pn = A.x[n];
+            A.Ap[n] = g0;
+            p0 = pn * g0;
+            const double dn = A.d[n];
+            if (pn > 0.0) p1 = (1.0 - dn) / pn;
+            if (pn < 0.0) p2 = (A.lb[n] - dn) / pn;
+        } else if (OP == OP_PC_INIT) {
+            const bool fx = A.fixed && A.fixed[n];
+            const double bn = A.b[n];
+            double r = bn - g0;
+            if (fx) r = 0.0;
+            const double zz = r * (fx ? 0.0 : A.minv[n]);
+            A.r[n] = r;
+            A.z[n] = zz;
+            A.p[n] = zz;
+            p0 = r * r;
+            p1 = bn * bn;
+            p2 = r * zz;
+        } else if (OP == OP_PC_AP) {
+            A.Ap[n] = g0;
+            p0 = A.x[n] * g0;
+        } else if (OP == OP_PC_RESID) {
+            double r = A.b[n] - g0;
+            if (A.fixed && A.fixed[n]) r = 0.0;
+            A.r[n] = r;
+        }
+    }
+    if (OP == OP_CG_INIT || OP == OP_CG_RESID || OP == OP_CG_AP || OP == OP_PC_INIT || OP == OP_PC_AP) {
+        const int64_t slot = blockIdx.x;
+        const double s0 = block_sum<256>(p0, red);
+        if (threadIdx.x == 0) A.part[slot] = s0;
+        if (OP == OP_CG_AP) {
+            const double s1 = block_min<256>(p1, red);
+            const double s2 = block_min<256>(p2, red);
+            if (threadIdx.x == 0) {
+                A.part[A.n_tiles + slot] = s1;
+                A.part[2 * A.n_tiles + slot] = s2;
+            }
+        } else if (OP != OP_PC_AP) {
+            const double s1 = block_sum<256>(p1, red);
+            if (threadIdx.x == 0) A.part[A.n_tiles + slot] = s1;
+            if (OP == OP_CG_INIT || OP == OP_PC_INIT) {
+                const double s2 = block_sum<256>(p2, red);
+                if (threadIdx.x == 0) A.part[2 * A.n_tiles + slot] = s2;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// node-wise CG vector kernels (same tile decomposition so partial slots line up)
+// ---------------------------------------------------------------------------------------
+
+enum VecOp : int {
+    V_SETUP = 0,     // d = fixed ? vals : init ; lb = fixed ? vals : d_prev
+    V_UPDATE_D = 1,  // projected: limited ? clamp(d + a p) : d += a p, r -= a Ap
+    V_PRECOND = 2,   // projected: z = active ? 0 : r Minv ; partial rf.z
+    V_UPDATE_P = 3,  // projected: p = active ? 0 : (restart ? z : beta p + z)
+    V_FINISH = 4,    // d = min(max(d, lb), 1), fixed -> vals ; copy active
+    V_PC_UPDATE = 5, // post-clamp: d += a p, r -= a Ap
+    V_PC_PRECOND = 6,// post-clamp: z = r Minv ; partials r.r, r.z
+    V_PC_UPDATE_P = 7,
+    V_PC_FINISH = 8, // d = clamp(max(d, lb), 0, 1), fixed -> vals
+};
+
+struct VecArgs {
+    const Q4Inst *inst;
+    const int32_t *block_inst;
+    double *d, *r, *z, *p, *lb;
+    const double *Ap, *minv, *init, *dprev, *vals;
+    const uint8_t *fixed;
+    uint8_t *active, *active_out;
+    double *d_out;
+    Q4CG *cg;
+    double *part;
+    int32_t n_tiles;
+};
+
+template <int TX, int OP>
+__global__ __launch_bounds__(256) void q4_vec_kernel(VecArgs A) {
+    __shared__ double red[4];
+    const Tile T = tile_of_block<TX>(A.inst, A.block_inst);
+    const Q4Inst &P = A.inst[T.inst];
+    const Q4CG &S = A.cg[T.inst];
+    if (OP != V_SETUP && OP != V_FINISH && OP != V_PC_FINISH && S.done) return;
+    const int li = threadIdx.x % TX, lj = threadIdx.x / TX;
+    const int i = T.i0 + li, j = T.j0 + lj;
+    const bool valid = i < T.nxn && j < T.nyn;
+    const int64_t n = P.node_off + (int64_t)j * T.nxn + i;
+    double p0 = 0.0, p1 = 0.0;
+    if (valid) {
+        if (OP == V_SETUP) {
+            const bool fx = A.fixed && A.fixed[n];
+            const double dp = A.dprev[n];
+            A.d[n] = fx ? A.vals[n] : (A.init ? A.init[n] : dp);
+            A.lb[n] = fx ? A.vals[n] : dp;
+        } else if (OP == V_UPDATE_D) {
+            const double pn = A.p[n];
+            if (S.limited) {
+                double dn = A.d[n] + S.alpha * pn;
+                const double lb = A.lb[n];
+                dn = dn >= lb ? dn : lb;      // torch.maximum(d, lb)
+                dn = dn <= 1.0 ? dn : 1.0;    // clamp(max=1)
+                A.d[n] = dn;
+            } else {
+                A.d[n] = A.d[n] + S.alpha * pn;
+                A.r[n] = A.r[n] - S.alpha * A.Ap[n];
+            }
+        } else if (OP == V_PRECOND) {
+            const bool act = A.active[n] != 0;
+            const double rf = act ? 0.0 : A.r[n];
+            double zz = rf * A.minv[n];
+            if (act) zz = 0.0;
+            A.z[n] = zz;
+            p0 = rf * zz;
+        } else if (OP == V_UPDATE_P) {
+            const bool act = A.active[n] != 0;
+            double pn = S.restart ? A.z[n] : A.p[n] * S.beta + A.z[n];
+            if (act) pn = 0.0;
+            A.p[n] = pn;
+        } else if (OP == V_FINISH) {
+            double dn = A.d[n];
+            const double lb = A.lb[n];
+            dn = dn >= lb ? dn : lb;
+            dn = dn <= 1.0 ? dn : 1.0;
+            if (A.fixed && A.fixed[n]) dn = A.vals[n];
+            A.d_out[n] = dn;
+            if (A.active_out) A.active_out[n] = A.active[n];
+        } else if (OP == V_PC_UPDATE) {
+            A.d[n] = A.d[n] + S.alpha * A.p[n];
+            A.r[n] = A.r[n] - S.alpha * A.Ap[n];
+        } else if (OP == V_PC_PRECOND) {
+            const double rr = A.r[n];
+            const double zz = rr * ((A.fixed && A.fixed[n]) ? 0.0 : A.minv[n]);
+            A.z[n] = zz;
+            p0 = rr * rr;
+            p1 = rr * zz;
+        } else if (OP == V_PC_UPDATE_P) {
+            A.p[n] = A.p[n] * S.beta + A.z[n];
+        } else if (OP == V_PC_FINISH) {
+            double dn = A.d[n];
+            const double lb = A.lb[n];
+            dn = dn >= lb ? dn : lb;
+            dn = fmin(fmax(dn, 0.0), 1.0);
+            if (A.fixed && A.fixed[n]) dn = A.vals[n];
+            A.d_out[n] = dn;
+        }
+    }
+    if (OP == V_PRECOND || OP == V_PC_PRECOND) {
+        const double s0 = block_sum<256>(p0, red);
+        if (threadIdx.x == 0) A.part[blockIdx.x] = s0;
+        if (OP == V_PC_PRECOND) {
+            const double s1 = block_sum<256>(p1, red);
+            if (threadIdx.x == 0) A.part[A.n_tiles + blockIdx.x] = s1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// per-instance scalar steps (one block per instance reduces that instance's tile partials)
+// ---------------------------------------------------------------------------------------
+
+enum ScalarOp : int { S_INIT = 0, S_A = 1, S_B = 2, S_C = 3, S_PC_INIT = 4, S_PC_A = 5, S_PC_C = 6 };
+
+__device__ __forceinline__ double reduce_sum_parts(const double *part, int off, int cnt, double *red) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < cnt; k += 256) s += part[off + k];
+    return block_sum<256>(s, red);
+}
+__device__ __forceinline__ double reduce_min_parts(const double *part, int off, int cnt, double *red) {
+    double s = __builtin_huge_val();
+    for (int k = threadIdx.x; k < cnt; k += 256) s = fmin(s, part[off + k]);
+    return block_min<256>(s, red);
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG *cg, const double *part,
+                                                        int32_t n_tiles, int32_t check_every) {
+    __shared__ double red[4];
+    const Q4Inst &P = inst[blockIdx.x];
+    Q4CG &S = cg[blockIdx.x];
+    if (OP != S_INIT && OP != S_PC_INIT && S.done) return;
+    const int off = P.tile_off, cnt = P.n_tiles;
+    if (OP == S_INIT || OP == S_PC_INIT) {
+        const double rr = reduce_sum_parts(part, off, cnt, red);
+        const double bb = reduce_sum_parts(part, n_tiles + off, cnt, red);
+        const double rz = reduce_sum_parts(part, 2 * n_tiles + off, cnt, red);
+        if (threadIdx.x == 0) {
+            S.rr0 = rr;
+            S.bb = bb;
+            S.rz = rz;
+            const double ref = fmax(fmax(sqrt(bb), sqrt(rr)), 1e-30);
+            S.tol_sq = (S.tol * ref) * (S.tol * ref);
+            S.iter = 0;
+            S.done = S.max_iter <= 0 ? 1 : 0;
+            S.status = S.max_iter <= 0 ? PHAST_CG_MAXITER : PHAST_CG_RUNNING;
+            S.limited = S.check = S.restart = 0;
+        }
+    } else if (OP == S_A || OP == S_PC_A) {
+        const double pAp = reduce_sum_parts(part, off, cnt, red);
+        double up = 0.0, dn = 0.0;
+        if (OP == S_A) {
+            up = reduce_min_parts(part, n_tiles + off, cnt, red);
+            dn = reduce_min_parts(part, 2 * n_tiles + off, cnt, red);
+        }
+        if (threadIdx.x == 0) {
+            S.iter += 1;
+            S.pAp = pAp;
+            if (!(pAp > 0.0)) {  // damage_solver.py:3560-3575 (pAp <= 0 -> break)
+                S.done = 1;
+                S.status = PHAST_CG_NONPOSITIVE;
+                S.limited = 0;
+                return;
+            }
+            const double alpha_cg = S.rz / (pAp + 1e-30);
+            if (OP == S_A) {
+                double amax = alpha_cg;  // damage_solver.py:3579-3605
+                if (up < amax) amax = up;
+                if (dn < amax) amax = dn;
+                S.limited = (amax < alpha_cg && amax >= 0.0) ? 1 : 0;
+                S.alpha = S.limited ? amax : alpha_cg;
+                S.check = (!S.limited && (S.iter % check_every) == 0) ? 1 : 0;
+            } else {
+                S.alpha = alpha_cg;
+                S.check = (S.iter % check_every) == 0 ? 1 : 0;
+            }
+        }
+    } else if (OP == S_B) {
+        if (!(S.limited || S.check)) {
+            if (threadIdx.x == 0) S.restart = 0;
+            return;
+        }
+        const double rfn = reduce_sum_parts(part, off, cnt, red);
+        const double chg = reduce_sum_parts(part, n_tiles + off, cnt, red);
+        if (threadIdx.x == 0) {
+            S.rfn = rfn;
+            if (rfn < S.tol_sq) {
+                S.done = 1;
+                S.status = PHAST_CG_CONVERGED;
+            } else if (rfn > 1e12 * fmax(S.rr0, 1e-30)) {
+                S.done = 1;
+                S.status = PHAST_CG_DIVERGED;
+            } else if (!isfinite(rfn)) {
+                S.done = 1;
+                S.status = PHAST_CG_NONFINITE;
+            }
+            S.changed = chg > 0.0 ? 1 : 0;
+            S.restart = (S.limited || (S.check && chg > 0.0)) ? 1 : 0;
+        }
+    } else if (OP == S_C) {
+        const double rz_new = reduce_sum_parts(part, off, cnt, red);
+        if (threadIdx.x == 0) {
+            S.beta = S.restart ? 0.0 : rz_new / (S.rz + 1e-30);
+            S.rz = rz_new;
+            if (S.iter >= S.max_iter) {
+                S.done = 1;
+                S.status = PHAST_CG_MAXITER;
+            }
+        }
+    } else if (OP == S_PC_C) {
+        const double rr = reduce_sum_parts(part, off, cnt, red);
+        const double rz_new = reduce_sum_parts(part, n_tiles + off, cnt, red);
+        if (threadIdx.x == 0) {
+            S.rfn = rr;
+            if (rr < S.tol_sq) {  // damage_solver.py:3496-3506
+                S.done = 1;
+                S.status = PHAST_CG_CONVERGED;
+                return;
+            }
+            if (rr > 1e12 * fmax(S.rr0, 1e-30)) {
+                S.done = 1;
+                S.status = PHAST_CG_DIVERGED;
+                return;
+            }
+            S.beta = rz_new / (S.rz + 1e-30);
+            S.rz = rz_new;
+            if (S.iter >= S.max_iter) {
+                S.done = 1;
+                S.status = PHAST_CG_MAXITER;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------
+
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+template <int TX>
+static int launch_mech(const phast_q4_plan *pl, MechArgs A, bool step, bool hist, hipStream_t st) {
+    dim3 g(pl->n_tiles), b(256);
+    if (!step && !hist) hipLaunchKernelGGL((q4_mech_kernel<TX, false, false>), g, b, 0, st, A);
+    else if (!step && hist) hipLaunchKernelGGL((q4_mech_kernel<TX, false, true>), g, b, 0, st, A);
+    else if (step && !hist) hipLaunchKernelGGL((q4_mech_kernel<TX, true, false>), g, b, 0, st, A);
+    else hipLaunchKernelGGL((q4_mech_kernel<TX, true, true>), g, b, 0, st, A);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+static int mech_dispatch(const phast_q4_plan *pl, MechArgs A, bool step, bool hist, hipStream_t st) {
+    switch (pl->tx) {
+        case 64: return launch_mech<64>(pl, A, step, hist, st);
+        case 32: return launch_mech<32>(pl, A, step, hist, st);
+        case 16: return launch_mech<16>(pl, A, step, hist, st);
+    }
+    set_error("unsupported tile width %d", pl->tx);
+    return PHAST_ERR_UNSUPPORTED;
+}
+
+template <int TX, int OP>
+static void launch_dmg_tx(const phast_q4_plan *pl, const DmgArgs &A, hipStream_t st) {
+    hipLaunchKernelGGL((q4_dmg_kernel<TX, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A);
+}
+template <int OP>
+static int launch_dmg(const phast_q4_plan *pl, const DmgArgs &A, hipStream_t st) {
+    switch (pl->tx) {
+        case 64: launch_dmg_tx<64, OP>(pl, A, st); break;
+        case 32: launch_dmg_tx<32, OP>(pl, A, st); break;
+        case 16: launch_dmg_tx<16, OP>(pl, A, st); break;
+        default: set_error("unsupported tile width %d", pl->tx); return PHAST_ERR_UNSUPPORTED;
+    }
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+template <int OP>
+static int launch_vec(const phast_q4_plan *pl, const VecArgs &A, hipStream_t st) {
+    switch (pl->tx) {
+        case 64: hipLaunchKernelGGL((q4_vec_kernel<64, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A); break;
+        case 32: hipLaunchKernelGGL((q4_vec_kernel<32, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A); break;
+        case 16: hipLaunchKernelGGL((q4_vec_kernel<16, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A); break;
+        default: set_error("unsupported tile width %d", pl->tx); return PHAST_ERR_UNSUPPORTED;
+    }
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+template <int OP>
+static int launch_scalar(const phast_q4_plan *pl, int check_every, hipStream_t st) {
+    hipLaunchKernelGGL((q4_scalar_kernel<OP>), dim3(pl->n_inst), dim3(256), 0, st, pl->d_inst, pl->cg,
+                       pl->part, pl->n_tiles, check_every);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+#define PHAST_TRY(x)                 \
+    do {                             \
+        int rc_ = (x);               \
+        if (rc_ != PHAST_OK) return rc_; \
+    } while (0)
+
+}  // namespace phast
+
+using namespace phast;
+
+extern "C" int phast_q4_plan_create(const phast_q4_desc *desc, int32_t n_inst, int32_t tile_x, int32_t tile_y,
+                                    phast_q4_plan **out) {
+    PHAST_REQUIRE(desc && out && n_inst >= 1, "phast_q4_plan_create: null argument or n_inst < 1");
+    if (tile_x == 0 && tile_y == 0) {
+        tile_x = 64;
+        tile_y = 4;
+    }
+    PHAST_REQUIRE(tile_x * tile_y == 256 && (tile_x == 64 || tile_x == 32 || tile_x == 16),
+                  "unsupported tile %dx%d (need 64x4, 32x8 or 16x16)", tile_x, tile_y);
+    auto *pl = new phast_q4_plan();
+    pl->n_inst = n_inst;
+    pl->tx = tile_x;
+    pl->ty = tile_y;
+    std::vector<int32_t> block_inst;
+    int64_t node_off = 0, cell_off = 0;
+    for (int k = 0; k < n_inst; ++k) {
+        const phast_q4_desc &D = desc[k];
+        if (D.nx < 1 || D.ny < 1 || D.split < 0 || D.split > 1) {
+            delete pl;
+            set_error("instance %d: bad nx/ny/split", k);
+            return PHAST_ERR_ARG;
+        }
+        Q4Inst I;
+        std::memset(&I, 0, sizeof(I));
+        I.nx = D.nx;
+        I.ny = D.ny;
+        I.tiles_x = (int32_t)ceil_div(D.nx + 1, tile_x);
+        I.tiles_y = (int32_t)ceil_div(D.ny + 1, tile_y);
+        I.node_off = node_off;
+        I.cell_off = cell_off;
+        I.tile_off = (int32_t)block_inst.size();
+        I.n_tiles = I.tiles_x * I.tiles_y;
+        for (int q = 0; q < 4; ++q) {
+            double dl = 0.0;
+            for (int a = 0; a < 4; ++a) {
+                I.gx[q * 4 + a] = D.gradN[q][a][0];
+                I.gy[q * 4 + a] = D.gradN[q][a][1];
+                I.Nq[q * 4 + a] = D.N[q][a];
+            }
+            I.w[q] = D.wdetJ[q];
+            (void)dl;
+        }
+        for (int a = 0; a < 4; ++a) {  // damage_solver.py:1397-1402
+            double s = 0.0;
+            for (int q = 0; q < 4; ++q)
+                s += D.wdetJ[q] * (D.gradN[q][a][0] * D.gradN[q][a][0] + D.gradN[q][a][1] * D.gradN[q][a][1]);
+            I.dlap[a] = s;
+        }
+        I.lam = D.lam;
+        I.mu = D.mu;
+        I.kappa = D.kappa;
+        I.ezz_coef = D.plane_stress ? (-D.nu / (1.0 - D.nu)) : 0.0;
+        I.eta = D.eta_residual;
+        I.Gc_l0 = D.Gc * D.l0;
+        I.Gc_over_l0 = D.Gc / D.l0;
+        I.l0 = D.l0;
+        I.dt = D.dt;
+        I.split = D.split;
+        for (int t = 0; t < I.n_tiles; ++t) block_inst.push_back(k);
+        node_off += (int64_t)(D.nx + 1) * (D.ny + 1);
+        cell_off += (int64_t)D.nx * D.ny;
+        pl->h_inst.push_back(I);
+    }
+    pl->n_tiles = (int32_t)block_inst.size();
+    pl->n_nodes = node_off;
+    pl->n_cells = cell_off;
+    if (hipMalloc(&pl->d_inst, sizeof(Q4Inst) * n_inst) != hipSuccess ||
+        hipMalloc(&pl->d_block_inst, sizeof(int32_t) * block_inst.size()) != hipSuccess ||
+        hipMemcpy(pl->d_inst, pl->h_inst.data(), sizeof(Q4Inst) * n_inst, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->d_block_inst, block_inst.data(), sizeof(int32_t) * block_inst.size(),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("phast_q4_plan_create: device allocation failed");
+        phast_q4_plan_destroy(pl);
+        return PHAST_ERR_HIP;
+    }
+    *out = pl;
+    return PHAST_OK;
+}
+
+extern "C" int phast_q4_plan_destroy(phast_q4_plan *pl) {
+    if (!pl) return PHAST_OK;
+    (void)hipFree(pl->d_inst);
+    (void)hipFree(pl->d_block_inst);
+    (void)hipFree(pl->r);
+    (void)hipFree(pl->p);
+    (void)hipFree(pl->z);
+    (void)hipFree(pl->Ap);
+    (void)hipFree(pl->b);
+    (void)hipFree(pl->minv);
+    (void)hipFree(pl->lb);
+    (void)hipFree(pl->part);
+    (void)hipFree(pl->active);
+    (void)hipFree(pl->cg);
+    if (pl->h_cg) (void)hipHostFree(pl->h_cg);
+    delete pl;
+    return PHAST_OK;
+}
+
+extern "C" int phast_q4_plan_sizes(const phast_q4_plan *pl, int64_t *n_nodes, int64_t *n_cells) {
+    PHAST_REQUIRE(pl, "null plan");
+    if (n_nodes) *n_nodes = pl->n_nodes;
+    if (n_cells) *n_cells = pl->n_cells;
+    return PHAST_OK;
+}
+
+extern "C" int phast_q4_plan_offsets(const phast_q4_plan *pl, int64_t *node_off, int64_t *cell_off) {
+    PHAST_REQUIRE(pl, "null plan");
+    for (int k = 0; k < pl->n_inst; ++k) {
+        if (node_off) node_off[k] = pl->h_inst[k].node_off;
+        if (cell_off) cell_off[k] = pl->h_inst[k].cell_off;
+    }
+    return PHAST_OK;
+}
+
+static DmgArgs dmg_base(const phast_q4_plan *pl) {
+    DmgArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.inst = pl->d_inst;
+    A.block_inst = pl->d_block_inst;
+    A.n_tiles = pl->n_tiles;
+    return A;
+}
+
+extern "C" int phast_q4_lumped_mass(const phast_q4_plan *pl, double *M, hipStream_t st) {
+    PHAST_REQUIRE(pl && M, "phast_q4_lumped_mass: null argument");
+    DmgArgs A = dmg_base(pl);
+    A.out = M;
+    return launch_dmg<OP_MASS>(pl, A, st);
+}
+
+extern "C" int phast_q4_elem_to_node(const phast_q4_plan *pl, const double *field_q, const double *M,
+                                     double *out, hipStream_t st) {
+    PHAST_REQUIRE(pl && field_q && M && out, "phast_q4_elem_to_node: null argument");
+    DmgArgs A = dmg_base(pl);
+    A.x = field_q;
+    A.M = M;
+    A.out = out;
+    return launch_dmg<OP_E2N>(pl, A, st);
+}
+
+extern "C" int phast_q4_internal_force(const phast_q4_plan *pl, const double *u, const double *d, double *f,
+                                       hipStream_t st) {
+    PHAST_REQUIRE(pl && u && d && f, "phast_q4_internal_force: null argument");
+    PHAST_REQUIRE(aligned16(u) && aligned16(f), "u and f must be 16-byte aligned (N,2) float64 arrays");
+    MechArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.inst = pl->d_inst;
+    A.block_inst = pl->d_block_inst;
+    A.u = reinterpret_cast<const double2 *>(u);
+    A.d = d;
+    A.f = reinterpret_cast<double2 *>(f);
+    return mech_dispatch(pl, A, false, false, st);
+}
+
+extern "C" int phast_q4_explicit_step(const phast_q4_plan *pl, const double *u, const double *v, const double *a,
+                                      const double *d, const double *f_ext, const double *Minv,
+                                      const uint8_t *bc_mask, const double *bc_vals, const double *H_old,
+                                      double *u1, double *v1, double *a1, double *H_new, double *psi,
+                                      hipStream_t st) {
+    PHAST_REQUIRE(pl && u && v && a && d && Minv && u1 && v1 && a1, "phast_q4_explicit_step: null argument");
+    PHAST_REQUIRE(!bc_mask == !bc_vals, "bc_mask and bc_vals must be given together");
+    PHAST_REQUIRE(!H_old == !H_new, "H_old and H_new must be given together");
+    PHAST_REQUIRE(aligned16(u) && aligned16(v) && aligned16(a) && aligned16(u1) && aligned16(v1) &&
+                      aligned16(a1) && aligned16(f_ext) && aligned16(H_old) && aligned16(H_new) && aligned16(psi),
+                  "state arrays must be 16-byte aligned");
+    PHAST_REQUIRE(u1 != u && v1 != v && a1 != a && u1 != v && u1 != a && (!H_new || H_new != H_old),
+                  "explicit step outputs must not alias its inputs");
+    PHAST_REQUIRE(!psi || H_old, "psi output requires the history arrays");
+    MechArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.inst = pl->d_inst;
+    A.block_inst = pl->d_block_inst;
+    A.u = reinterpret_cast<const double2 *>(u);
+    A.v = reinterpret_cast<const double2 *>(v);
+    A.a = reinterpret_cast<const double2 *>(a);
+    A.d = d;
+    A.fext = reinterpret_cast<const double2 *>(f_ext);
+    A.minv = Minv;
+    A.bcm = bc_mask;
+    A.bcv = bc_vals;
+    A.Hold = H_old;
+    A.Hnew = H_new;
+    A.psi = psi;
+    A.u1 = reinterpret_cast<double2 *>(u1);
+    A.v1 = reinterpret_cast<double2 *>(v1);
+    A.a1 = reinterpret_cast<double2 *>(a1);
+    return mech_dispatch(pl, A, true, H_old != nullptr, st);
+}
+
+extern "C" int phast_q4_strain(const phast_q4_plan *pl, const double *u, double *eps, hipStream_t st) {
+    PHAST_REQUIRE(pl && u && eps && aligned16(u), "phast_q4_strain: bad argument");
+    CellArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.inst = pl->d_inst;
+    A.n_inst = pl->n_inst;
+    A.u = reinterpret_cast<const double2 *>(u);
+    A.eps = eps;
+    A.n_cells = pl->n_cells;
+    hipLaunchKernelGGL(q4_cell_kernel, dim3((unsigned)ceil_div(pl->n_cells, 256)), dim3(256), 0, st, A);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_q4_driving_force(const phast_q4_plan *pl, const double *u, double *psi, const double *H_old,
+                                      double *H_new, hipStream_t st) {
+    PHAST_REQUIRE(pl && u && aligned16(u) && (psi || H_new), "phast_q4_driving_force: bad argument");
+    PHAST_REQUIRE(!H_old == !H_new, "H_old and H_new must be given together");
+    CellArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.inst = pl->d_inst;
+    A.n_inst = pl->n_inst;
+    A.u = reinterpret_cast<const double2 *>(u);
+    A.psi = psi;
+    A.Hold = H_old;
+    A.Hnew = H_new;
+    A.n_cells = pl->n_cells;
+    hipLaunchKernelGGL(q4_cell_kernel, dim3((unsigned)ceil_div(pl->n_cells, 256)), dim3(256), 0, st, A);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_history_max(int64_t n, const double *H_old, const double *psi, double *H_new,
+                                 hipStream_t st) {
+    PHAST_REQUIRE(n >= 0 && (n == 0 || (H_old && psi && H_new)), "phast_history_max: bad argument");
+    if (n == 0) return PHAST_OK;
+    hipLaunchKernelGGL(history_max_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, n, H_old, psi, H_new);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_q4_damage_rhs_diag(const phast_q4_plan *pl, const double *H, const double *Gc_cell,
+                                        double *b, double *diag, hipStream_t st) {
+    PHAST_REQUIRE(pl && H && b && diag, "phast_q4_damage_rhs_diag: null argument");
+    DmgArgs A = dmg_base(pl);
+    A.H = H;
+    A.Gc = Gc_cell;
+    A.out = b;
+    A.out2 = diag;
+    return launch_dmg<OP_RHSDIAG>(pl, A, st);
+}
+
+extern "C" int phast_q4_damage_apply(const phast_q4_plan *pl, const double *H, const double *Gc_cell,
+                                     const double *x, double *y, hipStream_t st) {
+    PHAST_REQUIRE(pl && H && x && y, "phast_q4_damage_apply: null argument");
+    DmgArgs A = dmg_base(pl);
+    A.H = H;
+    A.Gc = Gc_cell;
+    A.x = x;
+    A.out = y;
+    return launch_dmg<OP_APPLY>(pl, A, st);
+}
+
+__global__ void minv_kernel(int64_t n, double *diag) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) diag[i] = 1.0 / (diag[i] + 1e-30);  // damage_solver.py:2299
+}
+
+static int ensure_work(phast_q4_plan *pl) {
+    if (pl->r) return PHAST_OK;
+    const size_t nb = sizeof(double) * (size_t)pl->n_nodes;
+    PHAST_HIP(hipMalloc(&pl->r, nb));
+    PHAST_HIP(hipMalloc(&pl->p, nb));
+    PHAST_HIP(hipMalloc(&pl->z, nb));
+    PHAST_HIP(hipMalloc(&pl->Ap, nb));
+    PHAST_HIP(hipMalloc(&pl->b, nb));
+    PHAST_HIP(hipMalloc(&pl->minv, nb));
+    PHAST_HIP(hipMalloc(&pl->lb, nb));
+    PHAST_HIP(hipMalloc(&pl->part, sizeof(double) * kParts * (size_t)pl->n_tiles));
+    PHAST_HIP(hipMalloc(&pl->active, (size_t)pl->n_nodes));
+    PHAST_HIP(hipMalloc(&pl->cg, sizeof(Q4CG) * pl->n_inst));
+    PHAST_HIP(hipHostMalloc(&pl->h_cg, sizeof(Q4CG) * pl->n_inst, hipHostMallocDefault));
+    return PHAST_OK;
+}
+
+extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const double *H, const double *Gc_cell,
+                                     const double *d_prev, const double *d_init, const uint8_t *fixed,
+                                     const double *fixed_vals, double tol, int32_t max_iter, double *d_out,
+                                     uint8_t *active_out, int32_t *iters, int32_t *status, hipStream_t st) {
+    PHAST_REQUIRE(pl && H && d_prev && d_out, "phast_q4_damage_solve: null argument");
+    PHAST_REQUIRE(method == 0 || method == 1, "method must be 0 (projected) or 1 (post-clamp)");
+    PHAST_REQUIRE(!fixed == !fixed_vals, "fixed and fixed_vals must be given together");
+    PHAST_REQUIRE(tol >= 0.0 && max_iter >= 0, "tol and max_iter must be non-negative");
+    PHAST_TRY(ensure_work(pl));
+    // per-instance initial state: tolerance and iteration cap
+    for (int k = 0; k < pl->n_inst; ++k) {
+        Q4CG &S = pl->h_cg[k];
+        std::memset(&S, 0, sizeof(S));
+        S.tol = tol;
+        S.max_iter = max_iter;
+    }
+    PHAST_HIP(hipMemcpyAsync(pl->cg, pl->h_cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyHostToDevice, st));
+
+    DmgArgs D = dmg_base(pl);
+    D.H = H;
+    D.Gc = Gc_cell;
+    D.d = d_out;
+    D.r = pl->r;
+    D.z = pl->z;
+    D.p = pl->p;
+    D.Ap = pl->Ap;
+    D.b = pl->b;
+    D.minv = pl->minv;
+    D.lb = pl->lb;
+    D.fixed = fixed;
+    D.active = pl->active;
+    D.cg = pl->cg;
+    D.part = pl->part;
+
+    VecArgs V;
+    std::memset(&V, 0, sizeof(V));
+    V.inst = pl->d_inst;
+    V.block_inst = pl->d_block_inst;
+    V.d = d_out;
+    V.r = pl->r;
+    V.z = pl->z;
+    V.p = pl->p;
+    V.lb = pl->lb;
+    V.Ap = pl->Ap;
+    V.minv = pl->minv;
+    V.init = d_init;
+    V.dprev = d_prev;
+    V.vals = fixed_vals;
+    V.fixed = fixed;
+    V.active = pl->active;
+    V.active_out = active_out;
+    V.d_out = d_out;
+    V.cg = pl->cg;
+    V.part = pl->part;
+    V.n_tiles = pl->n_tiles;
+
+    // b, diag -> M_inv; d = init, lb = d_prev (with pf_dirichlet pins)
+    {
+        DmgArgs R = dmg_base(pl);
+        R.H = H;
+        R.Gc = Gc_cell;
+        R.out = pl->b;
+        R.out2 = pl->minv;
+        PHAST_TRY(launch_dmg<OP_RHSDIAG>(pl, R, st));
+        hipLaunchKernelGGL(minv_kernel, dim3((unsigned)ceil_div(pl->n_nodes, 256)), dim3(256), 0, st,
+                           pl->n_nodes, pl->minv);
+        PHAST_LAUNCHED();
+    }
+    PHAST_TRY(launch_vec<V_SETUP>(pl, V, st));
+    const int poll = 4;
+    int it = 0;
+    if (method == 0) {
+        D.x = d_out;
+        PHAST_TRY(launch_dmg<OP_CG_INIT>(pl, D, st));
+        PHAST_TRY(launch_scalar<S_INIT>(pl, 10, st));
+        D.x = pl->p;
+        DmgArgs Dr = D;
+        Dr.x = d_out;
+        while (it < max_iter) {
+            PHAST_TRY(launch_dmg<OP_CG_AP>(pl, D, st));
+            PHAST_TRY(launch_scalar<S_A>(pl, 10, st));
+            PHAST_TRY(launch_vec<V_UPDATE_D>(pl, V, st));
+            PHAST_TRY(launch_dmg<OP_CG_RESID>(pl, Dr, st));
+            PHAST_TRY(launch_scalar<S_B>(pl, 10, st));
+            PHAST_TRY(launch_vec<V_PRECOND>(pl, V, st));
+            PHAST_TRY(launch_scalar<S_C>(pl, 10, st));
+            PHAST_TRY(launch_vec<V_UPDATE_P>(pl, V, st));
+            ++it;
+            if (it % poll == 0 || it == max_iter) {
+                PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
+                PHAST_HIP(hipStreamSynchronize(st));
+                bool all = true;
+                for (int k = 0; k < pl->n_inst; ++k) all = all && pl->h_cg[k].done;
+                if (all) break;
+            }
+        }
+        PHAST_TRY(launch_vec<V_FINISH>(pl, V, st));
+    } else {
+        D.x = d_out;
+        PHAST_TRY(launch_dmg<OP_PC_INIT>(pl, D, st));
+        PHAST_TRY(launch_scalar<S_PC_INIT>(pl, 50, st));
+        D.x = pl->p;
+        DmgArgs Dr = D;
+        Dr.x = d_out;
+        while (it < max_iter) {
+            PHAST_TRY(launch_dmg<OP_PC_AP>(pl, D, st));
+            PHAST_TRY(launch_scalar<S_PC_A>(pl, 50, st));
+            PHAST_TRY(launch_vec<V_PC_UPDATE>(pl, V, st));
+            PHAST_TRY(launch_dmg<OP_PC_RESID>(pl, Dr, st));
+            PHAST_TRY(launch_vec<V_PC_PRECOND>(pl, V, st));
+            PHAST_TRY(launch_scalar<S_PC_C>(pl, 50, st));
+            PHAST_TRY(launch_vec<V_PC_UPDATE_P>(pl, V, st));
+            ++it;
+            if (it % poll == 0 || it == max_iter) {
+                PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
+                PHAST_HIP(hipStreamSynchronize(st));
+                bool all = true;
+                for (int k = 0; k < pl->n_inst; ++k) all = all && pl->h_cg[k].done;
+                if (all) break;
+            }
+        }
+        PHAST_TRY(launch_vec<V_PC_FINISH>(pl, V, st));
+    }
+    PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
+    PHAST_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < pl->n_inst; ++k) {
+        const Q4CG &S = pl->h_cg[k];
+        int32_t its = S.iter, stt = S.status;
+        if (!S.done) {  // loop ended by the cap without the flag (max_iter == 0 handled in init)
+            stt = PHAST_CG_MAXITER;
+        }
+        if (iters) iters[k] = its;
+        if (status) status[k] = stt;
+    }
+    pl->last_iters = it;
+    return PHAST_OK;
+}

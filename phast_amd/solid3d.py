@@ -1,0 +1,769 @@
+"""Periodic hex8 voxel path on MI355X: host mirror of ``phast.solid3d`` operator calls.
+
+``PeriodicVoxelPhaseFieldOperator`` keeps the reference's method names and return
+conventions (solid3d/operator.py:133-1544); ``pcg`` / ``solve_constrained`` mirror
+solid3d/linalg.py:99-287 and ``solve_h8_scalar_box_kkt`` mirrors
+solvers/box_kkt_h8.py:603-778.  Operator-bound actions dispatch to the device-resident
+solvers of ``libphast_mi.so``; every apply is a HIP kernel.  Set-up (phase maps, material
+bank, pinned node) runs once on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Mapping, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr, stream
+
+F64 = torch.float64
+VOIGT = ("xx", "yy", "zz", "yz", "xz", "xy")
+PHASE_NAMES = ("fibre", "bn_band", "cvi_sic", "mi_matrix", "pore")  # solid3d/model.py:40-49
+PHASE_ID = {n: i for i, n in enumerate(PHASE_NAMES)}
+DRIVER_NONE, DRIVER_FULL, DRIVER_SPECTRAL = 0, 1, 2  # solid3d/model.py:52
+DEFAULT_LINEAR_TOLERANCE = 1.0e-10  # box_kkt_h8.py:13
+DEFAULT_KKT_TOLERANCE = 1.0e-9      # box_kkt_h8.py:16
+H8_SCALAR_BOX_KKT_IDENTIFIER = "h8_scalar_box_kkt_active_set_v1"
+
+_lib.lib()  # the module is unusable without the HIP library
+_L = _lib.lib()
+for _name, _args in {
+    "phast_hex8_mech_solve": [C.c_void_p] * 8 + [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int64,
+                                                  C.c_void_p, C.POINTER(C.c_double), C.c_void_p],
+    "phast_hex8_damage_dirichlet_pcg": [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 5 +
+                                       [C.c_double, C.c_int32, C.c_int64, C.c_void_p, C.POINTER(C.c_double),
+                                        C.c_void_p],
+    "phast_hex8_box_update": [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7 +
+                             [C.c_double, C.c_int32, C.c_int64, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
+}.items():
+    _fn = getattr(_L, _name)
+    _fn.argtypes = _args
+    _fn.restype = C.c_int
+
+
+class PCGBreakdownError(RuntimeError):
+    """Non-positive curvature, a non-positive diagonal or non-finite values (linalg.py:41)."""
+
+
+@dataclass(frozen=True)
+class PCGInfo:
+    """Outcome of one PCG solve (solid3d/linalg.py:54-68)."""
+    iterations: int
+    converged: bool
+    rhs_norm: float
+    residual_norm: float
+    relative_residual: float
+
+
+# ----------------------------------------------------------------------------------------
+# model: grid, phases, element material arrays
+# ----------------------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class VoxelGrid:
+    """Triply periodic voxel grid (solid3d/model.py:183-262)."""
+    shape: tuple[int, int, int]
+    h: tuple[float, float, float]
+
+    @property
+    def n_elements(self) -> int:
+        return int(np.prod(self.shape))
+
+    @property
+    def lengths(self):
+        return tuple(n * h for n, h in zip(self.shape, self.h))
+
+    @property
+    def volume(self) -> float:
+        lx, ly, lz = self.lengths
+        return lx * ly * lz
+
+    @property
+    def voxel_volume(self) -> float:
+        return self.h[0] * self.h[1] * self.h[2]
+
+    @property
+    def array_shape(self):
+        nx, ny, nz = self.shape
+        return (nz, ny, nx)
+
+
+@dataclass(frozen=True)
+class PhaseMaterial:
+    """Isotropic phase (solid3d/model.py PhaseMaterial, isotropic kind)."""
+    name: str
+    E: float
+    nu: float
+    Gc: float
+    driver: str = "spectral"
+
+
+def isotropic_stiffness_3d(E: float, nu: float) -> np.ndarray:
+    """Engineering-Voigt isotropic stiffness (core/hex8.py:296-320)."""
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    C6 = np.zeros((6, 6))
+    C6[:3, :3] = lam
+    C6[0, 0] += 2.0 * mu
+    C6[1, 1] += 2.0 * mu
+    C6[2, 2] += 2.0 * mu
+    C6[3, 3] = C6[4, 4] = C6[5, 5] = mu
+    return C6
+
+
+@dataclass
+class ElementMaterialArrays:
+    """Per-voxel material data (solid3d/model.py:1137-1196; built as materials.py:474-548)."""
+    stiffness_bank: np.ndarray   # (K,6,6)
+    stiffness_index: np.ndarray  # (E,) int
+    Gc: np.ndarray
+    lam: np.ndarray
+    mu: np.ndarray
+    driver: np.ndarray           # (E,) int8
+    phase: np.ndarray            # (E,) int
+
+
+def build_element_arrays(phase_id: np.ndarray, phases: Mapping[str, PhaseMaterial],
+                         energy_split_isotropic: str = "spectral",
+                         Gc_field: np.ndarray | None = None) -> ElementMaterialArrays:
+    """Bank of the present isotropic phases in phase-id order (materials.py:474-548).
+
+    ``Gc_field`` (E,) optionally replaces the per-phase Gc (per-voxel fracture energy)."""
+    phase_flat = np.asarray(phase_id).reshape(-1).astype(np.int64)
+    present = [n for n in PHASE_NAMES if PHASE_ID[n] in set(np.unique(phase_flat).tolist())]
+    for n in present:
+        if n not in phases:
+            raise ValueError(f"no material for phase {n!r}")
+    index = np.empty(phase_flat.shape[0], dtype=np.int64)
+    bank = []
+    n_ids = len(PHASE_NAMES)
+    Gc_id, lam_id, mu_id = np.zeros(n_ids), np.zeros(n_ids), np.zeros(n_ids)
+    drv_id = np.zeros(n_ids, dtype=np.int8)
+    for slot, name in enumerate(present):
+        m = phases[name]
+        bank.append(isotropic_stiffness_3d(m.E, m.nu))
+        index[phase_flat == PHASE_ID[name]] = slot
+        pid = PHASE_ID[name]
+        Gc_id[pid] = m.Gc
+        lam_id[pid] = m.E * m.nu / ((1.0 + m.nu) * (1.0 - 2.0 * m.nu))
+        mu_id[pid] = m.E / (2.0 * (1.0 + m.nu))
+        if name == "pore" or m.driver == "none":  # materials.py:450-471
+            drv_id[pid] = DRIVER_NONE
+        else:
+            drv_id[pid] = DRIVER_SPECTRAL if energy_split_isotropic == "spectral" else DRIVER_FULL
+    Gc = Gc_id[phase_flat] if Gc_field is None else np.asarray(Gc_field, dtype=np.float64).reshape(-1)
+    return ElementMaterialArrays(np.stack(bank), index, Gc, lam_id[phase_flat], mu_id[phase_flat],
+                                 drv_id[phase_flat], phase_flat)
+
+
+@dataclass(frozen=True)
+class LoadCase:
+    """Mixed macro-strain control (solid3d/model.py:589-671)."""
+    name: str
+    controlled: int
+    targets: tuple
+    free: tuple
+    fixed_zero: tuple
+
+    @classmethod
+    def from_schedule(cls, name, *, controlled, schedule, free, fixed_zero):
+        targets, start = [], 0.0
+        for upto, inc in schedule:
+            n_steps = int(round(abs(upto - start) / abs(inc)))
+            span = upto - start
+            for step in range(1, n_steps):
+                targets.append(start + span * (step / n_steps))
+            targets.append(float(upto))
+            start = float(upto)
+        return cls(name, int(controlled), tuple(targets), tuple(free), tuple(fixed_zero))
+
+
+def choose_pinned_node(shape, phase_flat: np.ndarray, pore_id: int | None = PHASE_ID["pore"]) -> int:
+    """First reduced node none of whose 8 voxels is a pore (solid3d/periodic.py:224-249)."""
+    if pore_id is None:
+        return 0
+    nx, ny, nz = shape
+    pore = (np.asarray(phase_flat).reshape(nz, ny, nx) == pore_id).astype(np.int64)
+    count = np.zeros_like(pore)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):  # node (i,j,k) touches voxels (i-1+dx, j-1+dy, k-1+dz)
+                count += np.roll(pore, shift=(1 - dz, 1 - dy, 1 - dx), axis=(0, 1, 2))
+    return int(np.argmin(count.reshape(-1)))
+
+
+# ----------------------------------------------------------------------------------------
+# the operator
+# ----------------------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class H8ScalarSPDSystem:
+    """Matrix-free SPD data of one scalar subproblem (box_kkt_h8.py:63-122)."""
+    action: Callable[[torch.Tensor], torch.Tensor]
+    rhs: torch.Tensor
+    diagonal: torch.Tensor
+    _op: object = None
+    _H: torch.Tensor | None = None
+    _scale: float = 1.0
+
+    def __post_init__(self):
+        if not callable(self.action):
+            raise TypeError("action must be callable")
+        if self.rhs.ndim != 1 or self.rhs.numel() == 0:
+            raise ValueError("rhs must be a non-empty one-dimensional tensor")
+        if not bool(torch.isfinite(self.rhs).all()):
+            raise ValueError("rhs contains a non-finite value")
+        if self.diagonal.shape != self.rhs.shape or not bool((self.diagonal > 0.0).all()):
+            raise ValueError("diagonal must be strictly positive")
+
+    @property
+    def n_dofs(self) -> int:
+        return self.rhs.numel()
+
+    def apply(self, field: torch.Tensor) -> torch.Tensor:
+        return self.action(field)
+
+
+class PeriodicVoxelPhaseFieldOperator:
+    """Matrix-free mechanics and AT2 damage operators of one triply periodic voxel cell.
+
+    Mirrors solid3d/operator.py:133-1544 for the AT2 law without crack closure or
+    interface drivers.  x = [u~ (3 n), E (6)] with node-major u~ (``3 node + c``).
+    """
+
+    def __init__(self, grid: VoxelGrid, materials: ElementMaterialArrays, load: LoadCase, *, kappa: float = 1e-6,
+                 ell: float, pinned_node: int | None = None, device=None, tiles=(16, 8, 8)):
+        self.device = _lib.require_device() if device is None else torch.device(device)
+        self.grid, self.mat, self.load = grid, materials, load
+        self.dtype = F64
+        nx, ny, nz = grid.shape
+        self.n_elements = grid.n_elements
+        self.n_red = self.n_elements  # triply periodic: one reduced node per voxel
+        self.n_mech = 3 * self.n_red + 6
+        self.volume = float(grid.volume)
+        self.ell, self.kappa = float(ell), float(kappa)
+        Gc = np.asarray(materials.Gc, dtype=np.float64)
+        if not np.all(np.isfinite(Gc)) or not np.all(Gc > 0):
+            raise ValueError("Gc must be finite and positive at every element")
+        self.damage_scale = self.ell / (float(Gc.min()) * float(grid.voxel_volume))  # operator.py:238
+        self._c_history = 2.0 * (1.0 - self.kappa)
+        if pinned_node is None:
+            pinned_node = choose_pinned_node(grid.shape, materials.phase)
+        if not 0 <= int(pinned_node) < self.n_red:
+            raise ValueError(f"pinned_node must be a reduced node in [0, {self.n_red}), got {pinned_node!r}")
+        self.pinned_node = int(pinned_node)
+        bank = np.ascontiguousarray(materials.stiffness_bank, dtype=np.float64)
+        idx = np.ascontiguousarray(materials.stiffness_index, dtype=np.int32)
+        lam = np.ascontiguousarray(materials.lam, dtype=np.float64)
+        mu = np.ascontiguousarray(materials.mu, dtype=np.float64)
+        drv = np.ascontiguousarray(materials.driver, dtype=np.int8)
+        Gc = np.ascontiguousarray(Gc)
+        h = C.c_void_p()
+        check(_L.phast_hex8_plan_create(nx, ny, nz, grid.h[0], grid.h[1], grid.h[2], bank.ctypes.data,
+                                        bank.shape[0], idx.ctypes.data, Gc.ctypes.data, lam.ctypes.data,
+                                        mu.ctypes.data, drv.ctypes.data, self.kappa, self.ell, C.byref(h)),
+              "phast_hex8_plan_create")
+        self._plan = h
+        self.set_tiles(*tiles)
+        self.Gc = torch.as_tensor(Gc, device=self.device)
+
+    def __del__(self):
+        h = getattr(self, "_plan", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _L.phast_hex8_plan_destroy(h)
+            self._plan = None
+
+    def set_tiles(self, tx: int, ty: int, kz: int) -> None:
+        check(_L.phast_hex8_set_tiles(self._plan, int(tx), int(ty), int(kz)), "set_tiles")
+        self.tiles = (int(tx), int(ty), int(kz))
+
+    # -- checks --------------------------------------------------------------------------
+    def _check(self, name, value, shape):
+        if not isinstance(value, torch.Tensor):
+            raise TypeError(f"{name} must be a torch.Tensor")
+        if tuple(value.shape) != tuple(shape):
+            raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(value.shape)}")
+        if value.dtype != F64 or value.device != self.device:
+            raise ValueError(f"{name} must be {F64} on {self.device}")
+        return value.contiguous()
+
+    def _check_qp(self, name, value):
+        return None if value is None else self._check(name, value, (self.n_elements, 8))
+
+    # -- vectors -------------------------------------------------------------------------
+    def split_mech(self, x):
+        x = self._check("x", x, (self.n_mech,))
+        n = 3 * self.n_red
+        return x[:n].reshape(self.n_red, 3), x[n:]
+
+    def pack_mech(self, u, E):
+        u = torch.as_tensor(u, dtype=F64, device=self.device).reshape(-1)
+        E = torch.as_tensor(E, dtype=F64, device=self.device).reshape(-1)
+        if u.shape != (3 * self.n_red,) or E.shape != (6,):
+            raise ValueError(f"u must hold {3 * self.n_red} entries and E 6, got {u.numel()} and {E.numel()}")
+        return torch.cat((u, E))
+
+    def mech_fixed_mask(self) -> torch.Tensor:
+        """Pinned-node dofs, the controlled and fixed_zero E components (operator.py:914-925)."""
+        mask = torch.zeros(self.n_mech, dtype=torch.bool, device=self.device)
+        b = 3 * self.pinned_node
+        mask[b:b + 3] = True
+        off = 3 * self.n_red
+        mask[off + self.load.controlled] = True
+        for c in self.load.fixed_zero:
+            mask[off + c] = True
+        return mask
+
+    def mech_fixed_values(self, e_c) -> torch.Tensor:
+        values = torch.zeros(self.n_mech, dtype=F64, device=self.device)
+        values[3 * self.n_red + self.load.controlled] = float(e_c)
+        return values
+
+    # -- mechanics -----------------------------------------------------------------------
+    def degradation(self, d: torch.Tensor) -> torch.Tensor:
+        """g (E,8) = (1-kappa)(1-d_q)^2 + kappa (operator.py:939-946)."""
+        d = self._check("d", d, (self.n_red,))
+        g = torch.empty((self.n_elements, 8), dtype=F64, device=self.device)
+        check(_L.phast_hex8_degradation(self._plan, ptr(d), ptr(g), stream()), "degradation")
+        return g
+
+    def mech_action(self, x, g=None, *, opening=None, closure=None) -> torch.Tensor:
+        """A(g) x (operator.py:1171-1194); g=None means undamaged."""
+        if opening is not None or closure is not None:
+            raise NotImplementedError("interface openings / crack closure are not on this path")
+        x = self._check("x", x, (self.n_mech,))
+        g = self._check_qp("g", g)
+        y = torch.empty_like(x)
+        check(_L.phast_hex8_mech_action(self._plan, ptr(x), ptr(g), ptr(y), stream()), "mech_action")
+        return y
+
+    def mech_action_d(self, x, d) -> torch.Tensor:
+        """A(g(d)) x with g formed in-kernel from nodal damage (fused degradation)."""
+        x = self._check("x", x, (self.n_mech,))
+        d = self._check("d", d, (self.n_red,))
+        y = torch.empty_like(x)
+        check(_L.phast_hex8_mech_action_d(self._plan, ptr(x), ptr(d), ptr(y), stream()), "mech_action_d")
+        return y
+
+    def mech_diagonal(self, g=None, *, opening=None, closure=None) -> torch.Tensor:
+        """Jacobi diagonal (operator.py:1196-1254)."""
+        g = self._check_qp("g", g)
+        out = torch.empty(self.n_mech, dtype=F64, device=self.device)
+        check(_L.phast_hex8_mech_diagonal(self._plan, ptr(g), ptr(out), stream()), "mech_diagonal")
+        return out
+
+    def _stress_sums(self, x, g):
+        x = self._check("x", x, (self.n_mech,))
+        g = self._check_qp("g", g)
+        out = torch.empty(7, dtype=F64, device=self.device)
+        check(_L.phast_hex8_stress_sums(self._plan, ptr(x), ptr(g), ptr(out), stream()), "stress_sums")
+        return out
+
+    def macro_stress(self, x, g=None, *, opening=None, closure=None) -> torch.Tensor:
+        """Sigma = r_E / V (operator.py:1303-1316)."""
+        return self._stress_sums(x, g)[:6] / self.volume
+
+    def elastic_energy(self, x, g=None, *, opening=None, closure=None) -> torch.Tensor:
+        """0.5 x.A(g) x (operator.py:1318-1340)."""
+        return self._stress_sums(x, g)[6]
+
+    def driving_force(self, x) -> torch.Tensor:
+        """Undamaged driving energy Y (E,8) (operator.py:1415-1444)."""
+        x = self._check("x", x, (self.n_mech,))
+        Y = torch.empty((self.n_elements, 8), dtype=F64, device=self.device)
+        check(_L.phast_hex8_driving_force(self._plan, ptr(x), ptr(Y), stream()), "driving_force")
+        return Y
+
+    # -- damage --------------------------------------------------------------------------
+    def damage_action(self, d, H=None) -> torch.Tensor:
+        """Unscaled AT2 damage action (operator.py:1450-1466)."""
+        return self._damage_action(d, H, 1.0)
+
+    def _damage_action(self, d, H, scale):
+        d = self._check("d", d, (self.n_red,))
+        H = self._check_qp("H", H)
+        y = torch.empty_like(d)
+        check(_L.phast_hex8_damage_action(self._plan, ptr(d), ptr(H), float(scale), ptr(y), stream()),
+              "damage_action")
+        return y
+
+    def damage_rhs(self, H, scale: float = 1.0) -> torch.Tensor:
+        """2(1-kappa) sum_q w H_q N_q (operator.py:1468-1480)."""
+        H = self._check_qp("H", H)
+        out = torch.empty(self.n_red, dtype=F64, device=self.device)
+        check(_L.phast_hex8_damage_rhs(self._plan, ptr(H), float(scale), ptr(out), stream()), "damage_rhs")
+        return out
+
+    def damage_diagonal(self, H=None, scale: float = 1.0) -> torch.Tensor:
+        """Jacobi diagonal of the damage operator (operator.py:1482-1494)."""
+        H = self._check_qp("H", H)
+        out = torch.empty(self.n_red, dtype=F64, device=self.device)
+        check(_L.phast_hex8_damage_diagonal(self._plan, ptr(H), float(scale), ptr(out), stream()),
+              "damage_diagonal")
+        return out
+
+    def damage_system(self, H, *, scaled: bool = True) -> H8ScalarSPDSystem:
+        """Scaled SPD system (operator.py:1496-1509)."""
+        H = self._check_qp("H", H)
+        scale = self.damage_scale if scaled else 1.0
+        return H8ScalarSPDSystem(action=lambda f: self._damage_action(f, H, scale),
+                                 rhs=self.damage_rhs(H, scale), diagonal=self.damage_diagonal(H, scale),
+                                 _op=self, _H=H, _scale=scale)
+
+    # -- device-resident solves ----------------------------------------------------------
+    def solve_mechanics(self, *, g=None, d=None, fixed_mask, fixed_values, diag, rhs=None, x0=None,
+                        rtol=1e-8, atol=0.0, maxiter=5000, check_every=10):
+        """solve_constrained(mech_action(., g), ...) on the device (linalg.py:223-287)."""
+        if g is not None and d is not None:
+            raise ValueError("pass g or d, not both")
+        g = self._check_qp("g", g)
+        d = None if d is None else self._check("d", d, (self.n_red,))
+        fm = self._check_mask(fixed_mask, self.n_mech)
+        fv = self._check("fixed_values", fixed_values, (self.n_mech,))
+        dg = self._check("diag", diag, (self.n_mech,))
+        rhs = None if rhs is None else self._check("rhs", rhs, (self.n_mech,))
+        x0 = None if x0 is None else self._check("x0", x0, (self.n_mech,))
+        if rtol < 0 or atol < 0 or (rtol == 0 and atol == 0):
+            raise ValueError("rtol and atol must be non-negative and not both zero")
+        x = torch.empty(self.n_mech, dtype=F64, device=self.device)
+        info = (C.c_double * 5)()
+        rc = _L.phast_hex8_mech_solve(self._plan, ptr(g), ptr(d), ptr(fm), ptr(fv), ptr(rhs), ptr(x0), ptr(dg),
+                                      float(rtol), float(atol), int(maxiter), int(check_every), self.n_mech, ptr(x),
+                                      info, stream())
+        _raise_pcg(rc)
+        return x, PCGInfo(int(info[0]), bool(info[1]), info[2], info[3], info[4])
+
+    def _check_mask(self, m, n):
+        if not isinstance(m, torch.Tensor) or m.dtype != torch.bool or m.shape != (n,) or m.device != self.device:
+            raise TypeError(f"mask must be a torch.bool tensor of shape ({n},) on {self.device}")
+        return m.contiguous().view(torch.uint8)
+
+
+def _raise_pcg(rc: int) -> None:
+    if rc == 0:
+        return
+    msg = _L.phast_last_error().decode(errors="replace")
+    if rc in (_lib.PHAST_ERR_NONFINITE, _lib.PHAST_ERR_NONPOSITIVE):
+        raise PCGBreakdownError(msg)
+    check(rc)
+
+
+def _raise_scalar(rc: int) -> None:
+    """box_kkt_h8._pcg breakdowns are RuntimeError / ValueError with fixed messages."""
+    if rc == 0:
+        return
+    msg = _L.phast_last_error().decode(errors="replace")
+    if rc == _lib.PHAST_ERR_NONFINITE:
+        raise ValueError(msg)
+    if rc == _lib.PHAST_ERR_NONPOSITIVE:
+        raise RuntimeError(msg)
+    check(rc)
+
+
+# ----------------------------------------------------------------------------------------
+# linalg.pcg / solve_constrained
+# ----------------------------------------------------------------------------------------
+
+def pcg(action, rhs, *, diag, x0=None, rtol=1e-8, atol=0.0, maxiter=5000, check_every=10):
+    """Jacobi PCG (solid3d/linalg.py:99-220) for an arbitrary device action.
+
+    Operator-bound solves should use :meth:`PeriodicVoxelPhaseFieldOperator.solve_mechanics`
+    (device-resident loop).  This generic form keeps the reference's flag semantics with
+    torch device vectors and reads the flags every ``check_every`` iterations.
+    """
+    rtol, atol = float(rtol), float(atol)
+    if rtol < 0 or atol < 0 or (rtol == 0 and atol == 0):
+        raise ValueError("rtol and atol must be non-negative and not both zero")
+    stats = torch.stack((torch.linalg.vector_norm(rhs), torch.isfinite(rhs).all().to(rhs.dtype),
+                         (diag > 0.0).all().to(rhs.dtype), torch.isfinite(diag).all().to(rhs.dtype))).cpu()
+    b_norm = float(stats[0])
+    if stats[1] < 1.0 or not math.isfinite(b_norm):
+        raise PCGBreakdownError("PCG right-hand side contains a non-finite value")
+    if stats[3] < 1.0:
+        raise PCGBreakdownError("PCG diagonal contains a non-finite value")
+    if stats[2] < 1.0:
+        raise PCGBreakdownError("PCG diagonal has a non-positive entry (non-positive curvature)")
+    if b_norm == 0.0:
+        return torch.zeros_like(rhs), PCGInfo(0, True, 0.0, 0.0, 0.0)
+    threshold = max(rtol * b_norm, atol)
+    if x0 is None:
+        x, r, r_norm = torch.zeros_like(rhs), rhs.clone(), b_norm
+    else:
+        x = x0.clone()
+        r = rhs - action(x)
+        r_norm = float(torch.linalg.vector_norm(r))
+        if not math.isfinite(r_norm):
+            raise PCGBreakdownError("PCG initial residual is not finite")
+    if r_norm <= threshold:
+        return x, PCGInfo(0, True, b_norm, r_norm, r_norm / b_norm)
+    inv = 1.0 / diag
+    z = r * inv
+    p = z.clone()
+    rz = torch.dot(r, z)
+    thr2 = torch.tensor(threshold * threshold, dtype=rhs.dtype, device=rhs.device)
+    zero = torch.zeros((), dtype=rhs.dtype, device=rhs.device)
+    active = torch.ones((), dtype=torch.bool, device=rhs.device)
+    breakdown = torch.zeros_like(active)
+    nonfinite = torch.zeros_like(active)
+    done = torch.zeros((), dtype=torch.long, device=rhs.device)
+    rr = torch.dot(r, r)
+    for it in range(1, maxiter + 1):
+        Ap = action(p)
+        pAp = torch.dot(p, Ap)
+        positive = pAp > 0.0
+        breakdown = breakdown | (active & ~positive)
+        nonfinite = nonfinite | ~torch.isfinite(pAp)
+        step = active & positive
+        alpha = torch.where(step, rz / torch.where(positive, pAp, torch.ones_like(pAp)), zero)
+        x.add_(alpha * p)
+        r.sub_(alpha * Ap)
+        done = done + step.to(torch.long)
+        z = r * inv
+        rz_new = torch.dot(r, z)
+        rr = torch.dot(r, r)
+        nonfinite = nonfinite | ~torch.isfinite(rz_new) | ~torch.isfinite(rr)
+        active = active & (rr > thr2)
+        beta = torch.where(active & (rz > 0.0), rz_new / torch.where(rz > 0.0, rz, torch.ones_like(rz)), zero)
+        p = z + beta * p
+        rz = rz_new
+        if it % check_every == 0 or it == maxiter:
+            flags = torch.stack((nonfinite.to(rhs.dtype), breakdown.to(rhs.dtype), active.to(rhs.dtype), rr,
+                                 done.to(rhs.dtype))).cpu()
+            if flags[0] > 0:
+                raise PCGBreakdownError(f"PCG produced a non-finite value by iteration {it}")
+            if flags[1] > 0:
+                raise PCGBreakdownError(f"PCG met non-positive curvature p.A p <= 0 by iteration {it}; "
+                                        "the operator is not positive definite on the free set")
+            r_norm = math.sqrt(float(flags[3]))
+            if flags[2] < 1.0:
+                return x, PCGInfo(int(flags[4]), True, b_norm, r_norm, r_norm / b_norm)
+    return x, PCGInfo(maxiter, False, b_norm, r_norm, r_norm / b_norm)
+
+
+def solve_constrained(action, *, diag, fixed_mask, fixed_values, rhs=None, x0=None, rtol=1e-8, atol=0.0,
+                      maxiter=5000, check_every=10):
+    """Dirichlet-projected PCG (solid3d/linalg.py:223-287).
+
+    ``action`` bound by :func:`mech_action_of` runs entirely on the device."""
+    bound = getattr(action, "_phast_mech", None)
+    if bound is not None:
+        op, g, d = bound
+        return op.solve_mechanics(g=g, d=d, fixed_mask=fixed_mask, fixed_values=fixed_values, diag=diag, rhs=rhs,
+                                  x0=x0, rtol=rtol, atol=atol, maxiter=maxiter, check_every=check_every)
+    free = ~fixed_mask
+    zero = torch.zeros_like(fixed_values)
+    x_c = torch.where(fixed_mask, fixed_values, zero)
+    image = action(x_c)
+    b = -image if rhs is None else rhs - image
+    b_f = torch.where(free, b, zero)
+    diag_f = torch.where(free, diag, torch.ones_like(diag))
+    x0_f = None if x0 is None else torch.where(free, x0, zero)
+    y, info = pcg(lambda v: torch.where(free, action(v), zero), b_f, diag=diag_f, x0=x0_f, rtol=rtol, atol=atol,
+                  maxiter=maxiter, check_every=check_every)
+    return torch.where(free, y, x_c), info
+
+
+def mech_action_of(op: PeriodicVoxelPhaseFieldOperator, g=None, d=None):
+    """``lambda v: op.mech_action(v, g)`` tagged so that solve_constrained runs on the device."""
+    fn = (lambda v: op.mech_action(v, g)) if d is None else (lambda v: op.mech_action_d(v, d))
+    fn._phast_mech = (op, g, d)
+    return fn
+
+
+# ----------------------------------------------------------------------------------------
+# box-KKT damage solve (solvers/box_kkt_h8.py)
+# ----------------------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class ScalarPCGResult:
+    solution: torch.Tensor
+    iterations: int
+    converged: bool
+    initial_residual_norm: float
+    residual_norm: float
+    relative_residual: float
+
+
+@dataclass(frozen=True)
+class ScalarDirichletResult:
+    solution: torch.Tensor
+    krylov: ScalarPCGResult
+    n_free_dofs: int
+
+    @property
+    def converged(self) -> bool:
+        return self.krylov.converged
+
+
+@dataclass(frozen=True)
+class BoxKKTIteration:
+    iteration: int
+    active_lower_dofs: int
+    active_upper_dofs: int
+    prescribed_equal_bound_dofs: int
+    free_dofs: int
+    pcg_iterations: int
+    pcg_converged: bool
+    added_lower_dofs: int
+    added_upper_dofs: int
+    released_lower_dofs: int
+    released_upper_dofs: int
+
+
+@dataclass(frozen=True)
+class BoxKKTDiagnostics:
+    kkt_tolerance: float
+    free_stationarity_maximum: float
+    projected_gradient_maximum: float
+    complementarity_maximum: float
+    lower_sign_violation_dofs: int
+    upper_sign_violation_dofs: int
+    satisfies_kkt: bool
+
+
+@dataclass(frozen=True)
+class H8ScalarBoxKKTResult:
+    formulation_identifier: str
+    solution: torch.Tensor
+    lower_bound: torch.Tensor
+    upper_bound: torch.Tensor
+    lower_active: torch.Tensor
+    upper_active: torch.Tensor
+    prescribed_equal_bound: torch.Tensor
+    last_linear_solve: ScalarDirichletResult
+    active_set_iterations: int
+    total_pcg_iterations: int
+    active_set_trace: tuple
+    kkt: BoxKKTDiagnostics
+
+
+class H8ScalarBoxKKTConvergenceError(RuntimeError):
+    """Raised instead of returning a non-converged or non-KKT field (box_kkt_h8.py:239-253)."""
+
+    def __init__(self, reason, *, last_solution, last_linear_solve, evidence):
+        super().__init__(reason)
+        self.last_solution = last_solution
+        self.last_linear_solve = last_linear_solve
+        self.evidence = evidence
+
+
+def solve_h8_scalar_dirichlet_pcg(system: H8ScalarSPDSystem, *, fixed, values, initial=None,
+                                  tolerance=DEFAULT_LINEAR_TOLERANCE, max_iterations=4000) -> ScalarDirichletResult:
+    """Exact-Dirichlet PCG of the damage system on the device (box_kkt_h8.py:350-432)."""
+    op = system._op
+    if op is None:
+        raise NotImplementedError("the device solve needs a system built by PeriodicVoxelPhaseFieldOperator")
+    n = system.n_dofs
+    fx = op._check_mask(fixed, n)
+    values = op._check("values", values, (n,))
+    init = None if initial is None else op._check("initial", initial, (n,))
+    n_free = int(n - fixed.sum().item())
+    if n_free == 0:
+        known = torch.where(fixed, values, torch.zeros_like(values))
+        z = torch.empty(0, dtype=F64, device=op.device)
+        return ScalarDirichletResult(known, ScalarPCGResult(z, 0, True, 0.0, 0.0, 0.0), 0)
+    if not (tolerance > 0 and math.isfinite(tolerance)):
+        raise ValueError(f"tolerance must be finite and positive, got {tolerance!r}")
+    out = torch.empty(n, dtype=F64, device=op.device)
+    info = (C.c_double * 5)()
+    rc = _L.phast_hex8_damage_dirichlet_pcg(op._plan, ptr(system._H), float(system._scale), ptr(system.rhs),
+                                            ptr(system.diagonal), ptr(fx), ptr(values), ptr(init), float(tolerance),
+                                            int(max_iterations), n, ptr(out), info, stream())
+    _raise_scalar(rc)
+    krylov = ScalarPCGResult(out, int(info[0]), bool(info[1]), info[2], info[3], info[4])
+    return ScalarDirichletResult(out, krylov, n_free)
+
+
+def _kkt_diagnostics(system, solution, lower, upper, lact, uact, presc, kkt_tol) -> BoxKKTDiagnostics:
+    """Audit of box_kkt_h8.py:459-558 (the quantities its ``satisfies_kkt`` depends on)."""
+    grad = system.apply(solution) - system.rhs
+    fixed = lact | uact | presc
+    free = ~fixed
+    zero = torch.zeros_like(grad)
+    free_g = torch.where(free, grad, zero).abs().max()
+    proj = torch.where(free, grad, zero)
+    proj = torch.where(lact, torch.minimum(grad, zero), proj)
+    proj = torch.where(uact, torch.maximum(grad, zero), proj)
+    lmult = torch.where(lact, grad.clamp_min(0.0), zero)
+    umult = torch.where(uact, (-grad).clamp_min(0.0), zero)
+    comp = torch.maximum((lmult * (solution - lower)).abs().max(), (umult * (upper - solution)).abs().max())
+    lsv = (lact & (grad < -kkt_tol)).sum()
+    usv = (uact & (-grad < -kkt_tol)).sum()
+    stats = torch.stack([free_g, proj.abs().max(), comp, lsv.to(F64), usv.to(F64),
+                         (solution < lower).sum().to(F64), (solution > upper).sum().to(F64),
+                         (lower > upper).sum().to(F64)]).cpu().tolist()
+    ok = (stats[5] == 0 and stats[6] == 0 and stats[7] == 0 and stats[3] == 0 and stats[4] == 0
+          and stats[0] <= kkt_tol and stats[1] <= kkt_tol and stats[2] <= kkt_tol)
+    return BoxKKTDiagnostics(kkt_tol, stats[0], stats[1], stats[2], int(stats[3]), int(stats[4]), bool(ok))
+
+
+def solve_h8_scalar_box_kkt(system: H8ScalarSPDSystem, *, lower_bound=0.0, upper_bound=1.0, initial=None,
+                            linear_tolerance=DEFAULT_LINEAR_TOLERANCE, kkt_tolerance=DEFAULT_KKT_TOLERANCE,
+                            max_linear_iterations=4000, max_active_set_iterations=64) -> H8ScalarBoxKKTResult:
+    """Primal active-set box-KKT solve (box_kkt_h8.py:603-778) on the device."""
+    op = system._op
+    if op is None:
+        raise NotImplementedError("the device solve needs a system built by PeriodicVoxelPhaseFieldOperator")
+    n = system.n_dofs
+    dev = op.device
+
+    def bound(v, name):
+        if isinstance(v, torch.Tensor):
+            return op._check(name, v, (n,))
+        return torch.full((n,), float(v), dtype=F64, device=dev)
+
+    lower, upper = bound(lower_bound, "lower_bound"), bound(upper_bound, "upper_bound")
+    if bool((lower > upper).any()):
+        raise ValueError("lower_bound must not exceed upper_bound at any degree of freedom")
+    if initial is not None:
+        initial = op._check("initial", initial, (n,))
+        if bool((initial < lower).any()) or bool((initial > upper).any()):
+            raise ValueError("initial must already satisfy the requested box bounds; it is never clipped")
+    presc = lower == upper
+    lact = torch.zeros(n, dtype=torch.bool, device=dev)
+    uact = torch.zeros_like(lact)
+    cand_init = initial
+    trace, total = [], 0
+    grad = torch.empty(n, dtype=F64, device=dev)
+    counts = (C.c_int64 * 5)()
+    last = None
+    cand = None
+    for it in range(1, max_active_set_iterations + 1):
+        fixed = lact | uact | presc
+        values = torch.where(uact, upper, lower)
+        last = solve_h8_scalar_dirichlet_pcg(system, fixed=fixed, values=values, initial=cand_init,
+                                             tolerance=linear_tolerance, max_iterations=max_linear_iterations)
+        cand = last.solution
+        total += last.krylov.iterations
+        n_la, n_ua, n_pr = (int(v) for v in torch.stack([lact.sum(), uact.sum(), presc.sum()]).cpu())
+        apply = 1 if it < max_active_set_iterations else 0
+        la8, ua8 = lact.view(torch.uint8).clone(), uact.view(torch.uint8).clone()
+        rc = _L.phast_hex8_box_update(op._plan, ptr(system._H), float(system._scale), ptr(system.rhs), ptr(cand),
+                                      ptr(lower), ptr(upper), ptr(la8), ptr(ua8), ptr(presc.view(torch.uint8)),
+                                      float(kkt_tolerance), apply, n, ptr(grad), counts, stream())
+        check(rc, "box_update")
+        add_l, add_u, rel_l, rel_u, n_free = (int(c) for c in counts)
+        trace.append(BoxKKTIteration(it, n_la, n_ua, n_pr, n_free, last.krylov.iterations, last.krylov.converged,
+                                     add_l, add_u, rel_l, rel_u))
+        if not last.krylov.converged:
+            raise H8ScalarBoxKKTConvergenceError("H8 scalar box-KKT PCG did not converge", last_solution=cand,
+                                                 last_linear_solve=last, evidence=(it, total, tuple(trace)))
+        if add_l + add_u + rel_l + rel_u == 0:
+            kkt = _kkt_diagnostics(system, cand, lower, upper, lact, uact, presc, kkt_tolerance)
+            if not kkt.satisfies_kkt:
+                raise H8ScalarBoxKKTConvergenceError("H8 scalar box-KKT active set reached a non-KKT state",
+                                                     last_solution=cand, last_linear_solve=last,
+                                                     evidence=(it, total, tuple(trace), kkt))
+            return H8ScalarBoxKKTResult(H8_SCALAR_BOX_KKT_IDENTIFIER, cand, lower, upper, lact, uact, presc, last, it,
+                                        total, tuple(trace), kkt)
+        if it == max_active_set_iterations:
+            break
+        lact, uact = la8.view(torch.bool), ua8.view(torch.bool)
+        cand_init = cand
+    raise H8ScalarBoxKKTConvergenceError("H8 scalar box-KKT active set did not stabilise", last_solution=cand,
+                                         last_linear_solve=last, evidence=(max_active_set_iterations, total,
+                                                                           tuple(trace)))

@@ -1,0 +1,157 @@
+"""Parity of the periodic hex8 HIP path with the oracle, and the reference's recorded
+pore-in-cube elastic rows (GPU).  Fields within 1e-10 relative (max-norm); masks exact."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hex8 as O
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-10
+GOLDEN = Path(__file__).with_name("golden") / "pore_in_cube_elastic.json"
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+T = lambda x, dev: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
+
+
+def build(shape, h=(1e-3, 1.1e-3, 0.9e-3), seed=0, pores=True, tiles=(16, 8, 8), energy_split="spectral"):
+    from phast_amd import solid3d as S
+    from phast_amd import geometry as G
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    phase = np.full((nz, ny, nx), G.MATRIX, dtype=np.uint8)
+    phase[rng.uniform(0, 1, phase.shape) < 0.3] = G.INCLUSION
+    if pores:
+        phase[rng.uniform(0, 1, phase.shape) < 0.1] = G.PORE
+    phases = {"mi_matrix": S.PhaseMaterial("mi_matrix", 3000.0, 0.35, 5e-4),
+              "fibre": S.PhaseMaterial("fibre", 70000.0, 0.3, 5e-3),
+              "pore": S.PhaseMaterial("pore", 3e-3, 0.35, 5e-4)}
+    mats = S.build_element_arrays(phase, phases, energy_split_isotropic=energy_split)
+    load = S.LoadCase("uni", 0, (1e-3,), (1, 2, 3, 4, 5), ())
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid(shape, h), mats, load, kappa=1e-6, ell=2e-3, tiles=tiles)
+    oop = O.VoxelOperator(shape, h, mats.stiffness_bank, mats.stiffness_index, mats.Gc, mats.lam, mats.mu,
+                          mats.driver.astype(np.int64), kappa=1e-6, ell=2e-3, pinned_node=op.pinned_node,
+                          controlled=0)
+    return S, op, oop
+
+
+@pytest.mark.parametrize("shape", [(6, 5, 7), (16, 16, 16), (17, 9, 10)])
+@pytest.mark.parametrize("tiles", [(16, 8, 8), (8, 8, 4), (16, 16, 3), (32, 8, 16)])
+def test_mech_action(dev, shape, tiles):
+    S, op, oop = build(shape, tiles=tiles)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1e-6, 1e-6, op.n_mech)
+    x[-6:] = rng.uniform(-1e-3, 1e-3, 6)
+    d = rng.uniform(0, 1, op.n_red)
+    g = oop.degradation(d)
+    assert rel(op.degradation(T(d, dev)), g) < 1e-14
+    ref = oop.mech_action(x, g)
+    assert rel(op.mech_action(T(x, dev), T(g, dev)), ref) < RTOL
+    assert rel(op.mech_action_d(T(x, dev), T(d, dev)), ref) < RTOL
+    assert rel(op.mech_action(T(x, dev)), oop.mech_action(x)) < RTOL
+
+
+def test_diag_stress_energy_driving(dev):
+    S, op, oop = build((9, 7, 8))
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1e-6, 1e-6, op.n_mech)
+    x[-6:] = rng.uniform(-1e-3, 1e-3, 6)
+    g = oop.degradation(rng.uniform(0, 1, op.n_red))
+    assert rel(op.mech_diagonal(T(g, dev)), oop.mech_diagonal(g)) < RTOL
+    assert rel(op.mech_diagonal(), oop.mech_diagonal()) < RTOL
+    assert rel(op.macro_stress(T(x, dev), T(g, dev)), oop.macro_stress(x, g)) < RTOL
+    assert rel(op.elastic_energy(T(x, dev), T(g, dev)), oop.elastic_energy(x, g)) < RTOL
+    assert rel(op.driving_force(T(x, dev)), oop.driving_force(x)) < 1e-9
+
+
+def test_full_driver(dev):
+    S, op, oop = build((6, 6, 6), energy_split="full")
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1e-6, 1e-6, op.n_mech)
+    assert rel(op.driving_force(T(x, dev)), oop.driving_force(x)) < RTOL
+
+
+def test_damage_operator(dev):
+    S, op, oop = build((10, 6, 8))
+    rng = np.random.default_rng(4)
+    H = rng.uniform(0, 1e-3, (op.n_elements, 8))
+    x = rng.uniform(0, 1, op.n_red)
+    assert rel(op.damage_action(T(x, dev), T(H, dev)), oop.damage_action(x, H)) < RTOL
+    assert rel(op.damage_action(T(x, dev)), oop.damage_action(x)) < RTOL
+    assert rel(op.damage_rhs(T(H, dev)), oop.damage_rhs(H)) < RTOL
+    assert rel(op.damage_diagonal(T(H, dev)), oop.damage_diagonal(H)) < RTOL
+    sysm = op.damage_system(T(H, dev))
+    assert rel(sysm.apply(T(x, dev)), oop.damage_scale * oop.damage_action(x, H)) < RTOL
+
+
+def test_mechanics_solve_constrained(dev):
+    S, op, oop = build((8, 8, 8))
+    rng = np.random.default_rng(5)
+    g = oop.degradation(rng.uniform(0, 0.6, op.n_red))
+    fixed = op.mech_fixed_mask()
+    vals = op.mech_fixed_values(1e-3)
+    diag = op.mech_diagonal(T(g, dev))
+    x, info = S.solve_constrained(S.mech_action_of(op, g=T(g, dev)), diag=diag, fixed_mask=fixed, fixed_values=vals,
+                                  rtol=1e-12, maxiter=5000)
+    ox, oinfo = O.solve_constrained(lambda v: oop.mech_action(v, g), oop.mech_diagonal(g), fixed.cpu().numpy(),
+                                    vals.cpu().numpy(), rtol=1e-12, maxiter=5000)
+    assert info.converged and oinfo[1]
+    assert abs(info.iterations - oinfo[0]) <= 2
+    assert rel(x, ox) < 1e-9
+    # the generic (python-callable) form agrees with the device-resident loop
+    xg, ig = S.solve_constrained(lambda v: op.mech_action(v, T(g, dev)), diag=diag, fixed_mask=fixed,
+                                 fixed_values=vals, rtol=1e-12, maxiter=5000)
+    assert rel(xg, ox) < 1e-9
+
+
+def test_box_kkt(dev):
+    S, op, oop = build((8, 6, 7))
+    rng = np.random.default_rng(6)
+    H = rng.uniform(0, 4e-3, (op.n_elements, 8)) * (rng.uniform(0, 1, (op.n_elements, 1)) > 0.5)
+    lower = np.clip(rng.uniform(-0.5, 0.6, op.n_red), 0, 1)
+    sysm = op.damage_system(T(H, dev))
+    res = S.solve_h8_scalar_box_kkt(sysm, lower_bound=T(lower, dev), upper_bound=1.0, initial=T(lower, dev),
+                                    linear_tolerance=1e-13, kkt_tolerance=1e-9)
+    s = oop.damage_scale
+    od, ola, oua, its, total, trace = O.box_kkt(lambda v: s * oop.damage_action(v, H), s * oop.damage_rhs(H),
+                                                s * oop.damage_diagonal(H), lower, np.ones(op.n_red), initial=lower,
+                                                lin_tol=1e-13, kkt_tol=1e-9)
+    assert res.kkt.satisfies_kkt
+    assert rel(res.solution, od) < 1e-10
+    assert np.array_equal(res.lower_active.cpu().numpy(), ola)
+    assert np.array_equal(res.upper_active.cpu().numpy(), oua)
+    assert res.active_set_iterations == its
+
+
+def test_pore_in_cube_recorded_elastic_rows(dev):
+    """The reference's stored history (48^3 pore cell) at E_xx = 2e-5, 4e-5, 6e-5."""
+    from phast_amd import solid3d as S
+    from phast_amd import geometry as G
+    gold = json.loads(GOLDEN.read_text())
+    c = gold["cell"]
+    shape = tuple(c["shape"])
+    h = (c["voxel_mm"],) * 3
+    phase = G.sphere_pore(shape, h, c["pore_diameter_fraction"])
+    phases = {"mi_matrix": S.PhaseMaterial("mi_matrix", c["E"], c["nu"], 25e-3),
+              "pore": S.PhaseMaterial("pore", c["E"] * c["pore_stiffness_ratio"], c["nu"], 25e-3)}
+    mats = S.build_element_arrays(phase, phases)
+    load = S.LoadCase("xx", 0, (2e-5,), (1, 2, 3, 4, 5), ())
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid(shape, h), mats, load, kappa=1e-6, ell=1.5e-3)
+    diag = op.mech_diagonal()
+    fixed = op.mech_fixed_mask()
+    x = None
+    for row in gold["rows"]:
+        x, info = op.solve_mechanics(fixed_mask=fixed, fixed_values=op.mech_fixed_values(row["macro_strain_xx"]),
+                                     diag=diag, x0=x, rtol=c["pcg_rtol_mech"], maxiter=20000)
+        assert info.converged
+        sig = float(op.macro_stress(x)[0])
+        assert abs(sig - row["macro_stress_xx"]) / row["macro_stress_xx"] < 2e-7, (sig, row)
