@@ -218,6 +218,29 @@ int phast_hex8_box_update(const phast_hex8_plan *plan, const double *H, double s
                           uint8_t *upper_active, const uint8_t *prescribed, double kkt_tol, int32_t apply, int64_t n,
                           double *grad, int64_t *counts, hipStream_t stream);
 
+/* Geometric multigrid on the periodic voxel hierarchy (no reference counterpart; the
+ * reference's 3D solves are Jacobi-PCG only, solid3d/model.py:782).  Coarse levels halve
+ * every dimension while it stays even and >= min_coarse (at most max_levels levels);
+ * coarse moduli are voxel averages of g*lam, g*mu; damped-Jacobi smoothing (nu sweeps
+ * pre and post, weight omega), coarse_sweeps sweeps on the coarsest level; trilinear P,
+ * R = P^T; every level pins the node above `pinned` and the macro components flagged in
+ * macro_fixed6 (host, 6 bytes).  The fine plan must be isotropic. */
+typedef struct phast_hex8_mg phast_hex8_mg;
+int phast_hex8_mg_create(const phast_hex8_plan *fine, int64_t pinned, const uint8_t *macro_fixed6,
+                         int32_t max_levels, int32_t min_coarse, int32_t nu, int32_t coarse_sweeps, double omega,
+                         phast_hex8_mg **out);
+int phast_hex8_mg_destroy(phast_hex8_mg *mg);
+/* coarse moduli and Jacobi diagonals for the fine degradation g (E,8) or nodal d (n,) */
+int phast_hex8_mg_update(phast_hex8_mg *mg, const double *g, const double *d, hipStream_t stream);
+/* z = V(r) on the fine level (3n+6 vectors, zero on the fixed set) */
+int phast_hex8_mg_apply(phast_hex8_mg *mg, const double *r, double *z, hipStream_t stream);
+int phast_hex8_mg_levels(const phast_hex8_mg *mg, int32_t *levels, int32_t *dims);
+/* phast_hex8_mech_solve preconditioned by the V-cycle (same flag-masked PCG). */
+int phast_hex8_mech_solve_mg(const phast_hex8_plan *plan, phast_hex8_mg *mg, const double *g, const double *d,
+                             const uint8_t *fixed, const double *fixed_values, const double *rhs, const double *x0,
+                             const double *diag, double rtol, double atol, int32_t maxiter, int32_t check_every,
+                             int64_t n_mech, double *x_out, double *info, hipStream_t stream);
+
 /* Frees the process-wide solver scratch of the 3D solves. */
 int phast_release_solver_scratch(void);
 

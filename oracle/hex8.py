@@ -386,3 +386,55 @@ def box_kkt(action, rhs, diag, lower, upper, initial=None, lin_tol=1e-10, kkt_to
         ua = (ua | add_u) & ~rel_u
         cand_init = cand
     raise RuntimeError("H8 scalar box-KKT active set did not stabilise")
+
+
+def staggered_increment(op: VoxelOperator, state: dict, target: float, *, stagger_tol=1e-4, min_stagger=2,
+                        max_stagger=2000, rtol=1e-8, damage_linear_tol=1e-10, damage_kkt_tol=1e-8,
+                        check_every=10):
+    """One accepted increment of solid3d/staggered.py:_attempt (1259-1517), AT2 box-KKT route,
+    no Anderson mixing, final mechanics re-solve.  ``state`` holds x, d, H, E (macro strain).
+    Returns the new state and (stagger_iters, macro_stress)."""
+    c = op.controlled
+    n = 3 * op.n_red
+    previous = float(state["x"][n + c])
+    x_old = state["x"] * (target / previous) if previous != 0.0 else np.zeros_like(state["x"])
+    x_old[n + c] = target
+    d_old, lower = state["d"].copy(), state["d"]
+    fixed = op.mech_fixed_mask()
+    vals = np.zeros(op.n_mech)
+    vals[n + c] = target
+    step = abs(target - previous)
+    u_floor = min(op.h) * math.sqrt(n)
+    H = state["H"]
+
+    def mech(d, x0):
+        g = op.degradation(d)
+        x, info = solve_constrained(lambda v: op.mech_action(v, g), op.mech_diagonal(g), fixed, vals, x0=x0,
+                                    rtol=rtol, maxiter=20000, check_every=check_every)
+        assert info[1]
+        return x
+
+    for k in range(1, max_stagger + 1):
+        x_new = mech(d_old, x_old)
+        H = np.maximum(state["H"], op.driving_force(x_new))
+        s = op.damage_scale
+        act = lambda v: s * op.damage_action(v, H)
+        rhs, dg = s * op.damage_rhs(H), s * op.damage_diagonal(H)
+        r0 = float(np.linalg.norm(rhs - act(d_old)))
+        tol = max(min(damage_linear_tol, 0.1 * damage_kkt_tol / max(r0, 1.0)), 4.0 * np.finfo(np.float64).eps)
+        d_new = box_kkt(act, rhs, dg, lower, np.ones(op.n_red), initial=d_old, lin_tol=tol, kkt_tol=damage_kkt_tol)[0]
+        du = np.linalg.norm(x_new[:n] - x_old[:n])
+        u = np.linalg.norm(x_new[:n])
+        dE = np.abs(x_new[n:] - x_old[n:]).max()
+        E_inf = np.abs(x_new[n:]).max()
+        dd = np.abs(d_new - d_old).max()
+        scale_E = max(E_inf, step)
+        u_ref = max(u, scale_E * u_floor)
+        change = max(du / u_ref if u_ref > 0 else 0.0, dE / scale_E if scale_E > 0 else 0.0, dd)
+        x_old, d_old = x_new, d_new
+        if change <= stagger_tol and k >= min_stagger:
+            break
+    x = mech(d_old, x_old)
+    g = op.degradation(d_old)
+    sigma = op.macro_stress(x, g)
+    return {"x": x, "d": d_old, "H": H}, (k, sigma)

@@ -18,34 +18,7 @@
 #include <cstring>
 #include <vector>
 
-namespace phast {
-
-struct Hex8Geo {
-    int32_t nx, ny, nz, pad;
-    double ihx, ihy, ihz;   // 1/h
-    double w;               // Gauss weight * det J = hx hy hz / 8
-    double al, be;          // linear interpolation weights (1 +- 1/sqrt3)/2
-    double kappa, omk, ch;  // residual stiffness, 1-kappa, 2(1-kappa)
-    double K[64];           // AT2 voxel matrix M/ell + ell L, lexicographic corners [a*8+b]
-    double Nq[64];          // N[q*8 + a], lexicographic corners, xi-fastest Gauss points
-};
-
-}  // namespace phast
-
-struct phast_hex8_plan {
-    phast::Hex8Geo geo;
-    int64_t n = 0;  // reduced nodes == voxels
-    double *bank = nullptr;
-    int32_t *bank_index = nullptr;
-    double *Gc = nullptr, *lam = nullptr, *mu = nullptr;
-    int8_t *driver = nullptr;
-    int32_t n_bank = 0;
-    bool iso = false;   // every bank entry isotropic: stress from (lam, mu) per voxel
-    double *part = nullptr;  // block partials (8 per block)
-    unsigned int *counter = nullptr;
-    int32_t max_blocks = 0;
-    int32_t tx = 16, ty = 16, kz = 8;
-};
+#include "hex8_internal.h"
 
 namespace phast {
 
@@ -414,8 +387,25 @@ __global__ __launch_bounds__(256) void hex8_degradation_kernel(const Hex8Geo G, 
 
 // Jacobi diagonal: one thread per node gathers its 8 voxels (operator.py:1196-1254); the
 // thread of node n also forms voxel n's contribution to the macro block.
+__device__ __forceinline__ void voxel_C(bool iso, const double *bank, const int32_t *bank_index, const double *lam,
+                                        const double *mu, int64_t e, double C[36]) {
+    if (iso) {
+        const double l = lam[e], m = mu[e];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                C[r * 6 + c] = ((r < 3 && c < 3) ? l : 0.0) + (r == c ? (r < 3 ? 2.0 * m : m) : 0.0);
+    } else {
+        const double *B = bank + 36 * (int64_t)bank_index[e];
+#pragma unroll
+        for (int k = 0; k < 36; ++k) C[k] = B[k];
+    }
+}
+
 __global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const double *g, const double *bank,
-                                                        const int32_t *bank_index, double *diag, double *part) {
+                                                        const int32_t *bank_index, const double *lam,
+                                                        const double *mu, int iso, double *diag, double *part) {
     __shared__ double red[4];
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
     const int64_t node = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -429,7 +419,8 @@ __global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const d
                                                                       (int64_t)G.ny * wrap(k - 1 + dz, G.nz));
             const int lx = 1 - dx, ly = 1 - dy, lz = 1 - dz;
             const double sx = lx ? 1.0 : -1.0, sy = ly ? 1.0 : -1.0, sz = lz ? 1.0 : -1.0;
-            const double *C = bank + 36 * (int64_t)bank_index[e];
+            double C[36];
+            voxel_C(iso, bank, bank_index, lam, mu, e, C);
             for (int q = 0; q < 8; ++q) {
                 const int ix = q & 1, iy = (q >> 1) & 1, iz = q >> 2;
                 const double Wx = (lx == ix) ? G.al : G.be, Wy = (ly == iy) ? G.al : G.be,
@@ -459,7 +450,8 @@ __global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const d
         // macro block of voxel `node`: (sum_q g w) C_kk
         double gws = 0.0;
         for (int q = 0; q < 8; ++q) gws += (g ? g[node * 8 + q] : 1.0) * G.w;
-        const double *C = bank + 36 * (int64_t)bank_index[node];
+        double C[36];
+        voxel_C(iso, bank, bank_index, lam, mu, node, C);
 #pragma unroll
         for (int k2 = 0; k2 < 6; ++k2) dE[k2] = gws * C[k2 * 7];
     }
@@ -699,7 +691,7 @@ static int launch_mech_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool
     return PHAST_OK;
 }
 
-static int launch_mech(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
+int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
                        double *out7, bool energy, hipStream_t st) {
     MechArgs3 A;
     std::memset(&A, 0, sizeof(A));
@@ -753,7 +745,7 @@ using namespace phast;
 // Host construction of the lexicographic-corner voxel matrices (solid3d/periodic.py:193-221;
 // operator.py:242-247): N, gradN at the 2x2x2 Gauss points, M = sum_q w N N^T,
 // L = sum_q w gradN gradN^T, both symmetrised, K = M/ell + ell L.
-static void build_voxel_constants(Hex8Geo &G, double hx, double hy, double hz, double ell) {
+void phast::build_voxel_constants(Hex8Geo &G, double hx, double hy, double hz, double ell) {
     const double a = 1.0 / std::sqrt(3.0);
     const double pt[2] = {-a, a};
     double N[8][8], dN[8][8][3];
@@ -856,12 +848,14 @@ extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double
 
 extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
     if (!pl) return PHAST_OK;
-    (void)hipFree(pl->bank);
-    (void)hipFree(pl->bank_index);
-    (void)hipFree(pl->Gc);
+    if (pl->owns_material) {
+        (void)hipFree(pl->bank);
+        (void)hipFree(pl->bank_index);
+        (void)hipFree(pl->Gc);
+        (void)hipFree(pl->driver);
+    }
     (void)hipFree(pl->lam);
     (void)hipFree(pl->mu);
-    (void)hipFree(pl->driver);
     (void)hipFree(pl->part);
     (void)hipFree(pl->counter);
     delete pl;
@@ -888,13 +882,13 @@ extern "C" int phast_hex8_degradation(const phast_hex8_plan *pl, const double *d
 extern "C" int phast_hex8_mech_action(const phast_hex8_plan *pl, const double *x, const double *g, double *y,
                                       hipStream_t st) {
     PHAST_REQUIRE(pl && x && y && x != y, "phast_hex8_mech_action: bad argument");
-    return launch_mech(pl, x, g, nullptr, y, nullptr, false, st);
+    return hex8_apply_impl(pl, x, g, nullptr, y, nullptr, false, st);
 }
 
 extern "C" int phast_hex8_mech_action_d(const phast_hex8_plan *pl, const double *x, const double *d, double *y,
                                         hipStream_t st) {
     PHAST_REQUIRE(pl && x && d && y && x != y, "phast_hex8_mech_action_d: bad argument");
-    return launch_mech(pl, x, nullptr, d, y, nullptr, false, st);
+    return hex8_apply_impl(pl, x, nullptr, d, y, nullptr, false, st);
 }
 
 extern "C" int phast_hex8_stress_sums(const phast_hex8_plan *pl, const double *x, const double *g, double *out7,
@@ -903,7 +897,7 @@ extern "C" int phast_hex8_stress_sums(const phast_hex8_plan *pl, const double *x
     // the nodal rows are computed and discarded into the solver scratch of the plan
     double *scratch = nullptr;
     PHAST_HIP(hipMallocAsync((void **)&scratch, sizeof(double) * (3 * pl->n + 6), st));
-    const int rc = launch_mech(pl, x, g, nullptr, scratch, out7, true, st);
+    const int rc = hex8_apply_impl(pl, x, g, nullptr, scratch, out7, true, st);
     PHAST_HIP(hipFreeAsync(scratch, st));
     return rc;
 }
@@ -911,8 +905,8 @@ extern "C" int phast_hex8_stress_sums(const phast_hex8_plan *pl, const double *x
 extern "C" int phast_hex8_mech_diagonal(const phast_hex8_plan *pl, const double *g, double *diag, hipStream_t st) {
     PHAST_REQUIRE(pl && diag, "phast_hex8_mech_diagonal: null argument");
     const int nb = (int)ceil_div(pl->n, 256);
-    hipLaunchKernelGGL(hex8_diag_kernel, dim3(nb), dim3(256), 0, st, pl->geo, g, pl->bank, pl->bank_index, diag,
-                       pl->part);
+    hipLaunchKernelGGL(hex8_diag_kernel, dim3(nb), dim3(256), 0, st, pl->geo, g, pl->bank, pl->bank_index, pl->lam,
+                       pl->mu, pl->iso ? 1 : 0, diag, pl->part);
     PHAST_LAUNCHED();
     hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, pl->part, nb, 6, diag + 3 * pl->n,
                        (double *)nullptr);
@@ -963,4 +957,9 @@ extern "C" int phast_hex8_damage_diagonal(const phast_hex8_plan *pl, const doubl
                        pl->Gc, scale, (double *)nullptr, diag);
     PHAST_LAUNCHED();
     return PHAST_OK;
+}
+
+int phast::hex8_apply(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
+                      double *out7, bool energy, hipStream_t st) {
+    return hex8_apply_impl(pl, x, g, d, y, out7, energy, st);
 }

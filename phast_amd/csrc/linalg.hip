@@ -104,9 +104,11 @@ __global__ __launch_bounds__(256) void k_update(int64_t n, double *x, double *r,
         x[i] = x[i] + a * p[i];
         const double ri = r[i] - a * Ap[i];
         r[i] = ri;
-        const double zi = MODE == 0 ? ri * dg[i] : ri / dg[i];
-        z[i] = zi;
-        rz += ri * zi;
+        if (MODE != 2) {
+            const double zi = MODE == 0 ? ri * dg[i] : ri / dg[i];
+            z[i] = zi;
+            rz += ri * zi;
+        }
         rr += ri * ri;
     }
     write_parts(part, rz, rr, 0.0, red);
@@ -191,6 +193,21 @@ __global__ __launch_bounds__(256) void k_scalar_stop_b(PcgState *S, const double
     }
 }
 
+// partials r.z, r.r, count(!finite(z)) after an external preconditioner
+__global__ __launch_bounds__(256) void k_rz(int64_t n, const double *r, const double *z, const PcgState *S,
+                                            double *part) {
+    __shared__ double red[4];
+    if (S && S->done) return;
+    double rz = 0.0, rr = 0.0, bad = 0.0;
+    GRID_STRIDE(i, n) {
+        const double ri = r[i], zi = z[i];
+        rz += ri * zi;
+        rr += ri * ri;
+        if (!isfinite(zi)) bad += 1.0;
+    }
+    write_parts(part, rz, rr, bad, red);
+}
+
 // elementwise helpers
 __global__ __launch_bounds__(256) void k_axpby_mask(int64_t n, double alpha, const double *x, double beta,
                                                     const double *y, const uint8_t *fixed, double fixed_val,
@@ -269,6 +286,7 @@ __global__ __launch_bounds__(256) void k_box_update(int64_t n, const double *Aca
 // ---------------------------------------------------------------------------------------
 
 typedef int (*ApplyFn)(void *ctx, const double *in, double *out, hipStream_t st);
+typedef ApplyFn PrecondFn;  // z = M^-1 r (zero on the fixed set)
 
 struct Work {
     double *part = nullptr;
@@ -341,7 +359,7 @@ static int zero_parts(Work &W, hipStream_t st) {
 // PHAST_ERR_NONFINITE / PHAST_ERR_NONPOSITIVE on breakdown (messages as in the reference).
 static int pcg_flags(ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, const double *b, const double *diag,
                      const double *x0, double *x, double rtol, double atol, int maxiter, int check_every,
-                     double *info, hipStream_t st) {
+                     double *info, hipStream_t st, PrecondFn precond = nullptr, void *pctx = nullptr) {
     Work &W = g_work;
     if (int rc = W.ensure(n)) return rc;
     const int nb = nblk(n);
@@ -407,8 +425,13 @@ static int pcg_flags(ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, 
         h.alpha = 0.0;
         *W.hS = h;
         PHAST_HIP(hipMemcpyAsync(W.S, W.hS, sizeof(PcgState), hipMemcpyHostToDevice, st));
-        // z = r * inv_diag via an update with alpha = 0
-        hipLaunchKernelGGL((k_update<0>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.r, W.r, W.dg, W.z, W.S, W.part);
+        // z = r * inv_diag via an update with alpha = 0 (or z = M^-1 r)
+        if (precond) {
+            if (int rc = precond(pctx, W.r, W.z, st)) return rc;
+            hipLaunchKernelGGL(k_rz, dim3(nb), dim3(256), 0, st, n, W.r, W.z, (const PcgState *)nullptr, W.part);
+        } else {
+            hipLaunchKernelGGL((k_update<0>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.r, W.r, W.dg, W.z, W.S, W.part);
+        }
         PHAST_LAUNCHED();
         PHAST_HIP(hipMemcpyAsync(W.p, W.z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_scalar_flags_b, dim3(1), dim3(256), 0, st, W.S, W.part);  // sets rz, rr (active stays)
@@ -424,7 +447,14 @@ static int pcg_flags(ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, 
         hipLaunchKernelGGL(k_scalar_flags_a, dim3(1), dim3(256), 0, st, W.S, W.part);
         PHAST_LAUNCHED();
         // x, r, z update (alpha = 0 for inactive iterations leaves x and r unchanged)
-        hipLaunchKernelGGL((k_update<0>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S, W.part);
+        if (precond) {
+            hipLaunchKernelGGL((k_update<2>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S, W.part);
+            PHAST_LAUNCHED();
+            if (int rc = precond(pctx, W.r, W.z, st)) return rc;
+            hipLaunchKernelGGL(k_rz, dim3(nb), dim3(256), 0, st, n, W.r, W.z, (const PcgState *)nullptr, W.part);
+        } else {
+            hipLaunchKernelGGL((k_update<0>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S, W.part);
+        }
         PHAST_LAUNCHED();
         hipLaunchKernelGGL(k_scalar_flags_b, dim3(1), dim3(256), 0, st, W.S, W.part);
         PHAST_LAUNCHED();
@@ -593,11 +623,43 @@ __global__ void k_not_mask(int64_t n, const uint8_t *fixed, uint8_t *freem) {
 
 // solid3d/linalg.py:223-287 solve_constrained with the hex8 mechanics action A(g)
 // (d given: g formed in-kernel from nodal damage).  info[5] as in pcg_flags.
+struct phast_hex8_mg;
+extern "C" int phast_hex8_mg_apply(phast_hex8_mg *mg, const double *r, double *z, hipStream_t st);
+
+static int mg_precond_cb(void *c, const double *in, double *out, hipStream_t st) {
+    return phast_hex8_mg_apply(static_cast<phast_hex8_mg *>(c), in, out, st);
+}
+
+static int mech_solve_impl(const phast_hex8_plan *plan, phast_hex8_mg *mg, const double *g, const double *d,
+                           const uint8_t *fixed, const double *fixed_values, const double *rhs, const double *x0,
+                           const double *diag, double rtol, double atol, int32_t maxiter, int32_t check_every,
+                           int64_t n_mech, double *x_out, double *info, hipStream_t st);
+
 extern "C" int phast_hex8_mech_solve(const phast_hex8_plan *plan, const double *g, const double *d,
                                      const uint8_t *fixed, const double *fixed_values, const double *rhs,
                                      const double *x0, const double *diag, double rtol, double atol, int32_t maxiter,
                                      int32_t check_every, int64_t n_mech, double *x_out, double *info,
                                      hipStream_t st) {
+    return mech_solve_impl(plan, nullptr, g, d, fixed, fixed_values, rhs, x0, diag, rtol, atol, maxiter, check_every,
+                           n_mech, x_out, info, st);
+}
+
+// The same solve preconditioned by the voxel multigrid V-cycle (phast_hex8_mg_update must
+// have been called with the same g / d).
+extern "C" int phast_hex8_mech_solve_mg(const phast_hex8_plan *plan, phast_hex8_mg *mg, const double *g,
+                                        const double *d, const uint8_t *fixed, const double *fixed_values,
+                                        const double *rhs, const double *x0, const double *diag, double rtol,
+                                        double atol, int32_t maxiter, int32_t check_every, int64_t n_mech,
+                                        double *x_out, double *info, hipStream_t st) {
+    PHAST_REQUIRE(mg, "phast_hex8_mech_solve_mg: null multigrid");
+    return mech_solve_impl(plan, mg, g, d, fixed, fixed_values, rhs, x0, diag, rtol, atol, maxiter, check_every,
+                           n_mech, x_out, info, st);
+}
+
+static int mech_solve_impl(const phast_hex8_plan *plan, phast_hex8_mg *mg, const double *g, const double *d,
+                           const uint8_t *fixed, const double *fixed_values, const double *rhs, const double *x0,
+                           const double *diag, double rtol, double atol, int32_t maxiter, int32_t check_every,
+                           int64_t n_mech, double *x_out, double *info, hipStream_t st) {
     PHAST_REQUIRE(plan && fixed && fixed_values && diag && x_out && info, "phast_hex8_mech_solve: null argument");
     PHAST_REQUIRE(maxiter >= 1 && check_every >= 1, "maxiter and check_every must be >= 1");
     PHAST_REQUIRE(!(rtol == 0.0 && atol == 0.0) && rtol >= 0.0 && atol >= 0.0, "rtol and atol must not both be zero");
@@ -625,7 +687,8 @@ extern "C" int phast_hex8_mech_solve(const phast_hex8_plan *plan, const double *
         hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, diag, (const double *)nullptr, 1.0, df);
         if (x0) hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, x0, (const double *)nullptr, 0.0, x0f);
         MechCtx ctx{plan, g, d, fixed, n};
-        rc = pcg_flags(mech_apply_cb, &ctx, n, fixed, bf, df, x0f, y, rtol, atol, maxiter, check_every, info, st);
+        rc = pcg_flags(mech_apply_cb, &ctx, n, fixed, bf, df, x0f, y, rtol, atol, maxiter, check_every, info, st,
+                       mg ? mg_precond_cb : nullptr, mg);
         // x = where(free, y, x_c)
         if (rc == PHAST_OK)
             hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, y, xc, 0.0, x_out);

@@ -1,0 +1,336 @@
+"""Staggered alternate minimisation per macro-strain increment on the voxel cell.
+
+Mirrors ``solid3d/staggered.py`` (StaggeredSolver: ``_attempt`` 1259-1517,
+``_damage`` 753-802, ``_change`` 1097-1121, ``_predictor`` 1207-1213, ``advance`` 1215-1257,
+``advance_increment`` 1523-1582) for the AT2 law without crack closure, interface
+stress or Anderson mixing.  The mechanics solve is the device-resident constrained PCG
+with the degradation formed in-kernel from the nodal damage; the damage solve is the
+device box-KKT active set.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from dataclasses import dataclass, field
+from time import perf_counter
+
+import numpy as np
+import torch
+
+from .solid3d import (F64, H8ScalarBoxKKTConvergenceError, LoadCase, PCGBreakdownError,
+                      PeriodicVoxelPhaseFieldOperator, solve_h8_scalar_box_kkt)
+
+_MIN_LINEAR_TOLERANCE = 4.0 * float(np.finfo(np.float64).eps)  # staggered.py:97
+
+
+@dataclass(frozen=True)
+class SolverConfig:
+    """The settings of ``solid3d/model.py:SolverConfig`` this route reads."""
+    stagger_tol: float = 1.0e-4
+    max_stagger: int = 2000
+    min_stagger: int = 2
+    pcg_rtol_mech: float = 1.0e-8
+    pcg_maxit_mech: int = 20000
+    check_every: int = 10
+    damage_linear_tol: float = 1.0e-10
+    damage_kkt_tol: float = 1.0e-8
+    damage_maxit: int = 4000
+    damage_max_active_set: int = 64
+    on_stagger_failure: str = "cutback"
+    max_cutbacks: int = 1
+    final_mechanics_resolve: bool = True
+    damaged_threshold: float = 0.95
+    preconditioner: str = "jacobi"
+
+
+@dataclass(eq=False)
+class SolverState:
+    """Accepted solution (solid3d/staggered.py:167-218)."""
+    step: int
+    x: torch.Tensor
+    d: torch.Tensor
+    H: torch.Tensor
+    macro_strain: torch.Tensor
+    macro_stress: torch.Tensor
+    external_work: float
+    d_prev_step: torch.Tensor
+    increment: int = 0
+
+    def clone(self) -> "SolverState":
+        return SolverState(self.step, self.x.clone(), self.d.clone(), self.H.clone(), self.macro_strain.clone(),
+                           self.macro_stress.clone(), float(self.external_work), self.d_prev_step.clone(),
+                           self.increment)
+
+
+@dataclass
+class IncrementRecord:
+    step: int = 0
+    increment: int = 0
+    macro_strain: tuple = (0.0,) * 6
+    macro_stress: tuple = (0.0,) * 6
+    reaction_force: float = 0.0
+    applied_disp: float = 0.0
+    elastic: float = 0.0
+    fracture: float = 0.0
+    external: float = 0.0
+    balance_residual_rel: float = 0.0
+    max_damage: float = 0.0
+    mean_damage: float = 0.0
+    max_H: float = 0.0
+    stagger_iters: int = 0
+    pcg_iters_mech: int = 0
+    pcg_iters_pf: int = 0
+    active_set_iters: int = 0
+    cutbacks: int = 0
+    stagger_change: float = 0.0
+    hill_mandel_rel: float = 0.0
+    wall_s: float = 0.0
+    mech_s: float = 0.0
+    damage_s: float = 0.0
+    history_s: float = 0.0
+
+
+class StaggerConvergenceError(RuntimeError):
+    def __init__(self, message, *, state=None, target=None, increment=None, cutbacks=0, reason="", records=()):
+        super().__init__(message)
+        self.state, self.target, self.increment = state, target, increment
+        self.cutbacks, self.reason, self.records = cutbacks, reason, records
+
+
+class _AttemptFailure(RuntimeError):
+    pass
+
+
+class StaggeredSolver:
+    """AT2 staggered solver of one voxel cell under mixed macro control."""
+
+    def __init__(self, op: PeriodicVoxelPhaseFieldOperator, load: LoadCase | None = None,
+                 cfg: SolverConfig | None = None, preconditioner=None):
+        self.op = op
+        self.load = op.load if load is None else load
+        self.cfg = SolverConfig() if cfg is None else cfg
+        self.device = op.device
+        self._mask = op.mech_fixed_mask()
+        self._n_u = 3 * op.n_red
+        self._controlled = int(self.load.controlled)
+        self._max_abs_target = max(abs(t) for t in self.load.targets)
+        self._u_floor = min(op.grid.h) * math.sqrt(self._n_u)
+        lengths = op.grid.lengths
+        axis = self._controlled if self._controlled < 3 else 0
+        others = tuple(lengths[a] for a in range(3) if a != axis)
+        self._load_axis, self._load_length, self._face_lengths = axis, lengths[axis], others
+        self.preconditioner = preconditioner  # optional callable building an MG preconditioner
+        self.last_state: SolverState | None = None
+        self.n_attempts = 0
+
+    def initial_state(self) -> SolverState:
+        op = self.op
+        z = lambda *s: torch.zeros(s, dtype=F64, device=self.device)
+        return SolverState(0, z(op.n_mech), z(op.n_red), z(op.n_elements, 8), z(6), z(6), 0.0, z(op.n_red), 0)
+
+    # -- pieces ----------------------------------------------------------------------
+    def _predictor(self, state, previous, target):
+        """staggered.py:1207-1213."""
+        x = state.x * (target / previous) if previous != 0.0 else torch.zeros_like(state.x)
+        x[self._n_u + self._controlled] = target
+        return x
+
+    def _mechanics(self, d, values, x0):
+        """staggered.py:550-587 (linear mechanics at fixed damage)."""
+        op, cfg = self.op, self.cfg
+        g = op.degradation(d)
+        diag = op.mech_diagonal(g)
+        try:
+            if self.preconditioner is not None:
+                x, info = self.preconditioner.solve(d=d, g=g, fixed_mask=self._mask, fixed_values=values, diag=diag,
+                                                    x0=x0, rtol=cfg.pcg_rtol_mech, maxiter=cfg.pcg_maxit_mech,
+                                                    check_every=cfg.check_every)
+            else:
+                x, info = op.solve_mechanics(d=d, fixed_mask=self._mask, fixed_values=values, diag=diag, x0=x0,
+                                             rtol=cfg.pcg_rtol_mech, maxiter=cfg.pcg_maxit_mech,
+                                             check_every=cfg.check_every)
+        except PCGBreakdownError as exc:
+            raise _AttemptFailure(f"mechanics PCG breakdown: {exc}") from exc
+        if not info.converged:
+            raise _AttemptFailure(f"mechanics PCG did not converge in {cfg.pcg_maxit_mech} iterations "
+                                  f"(relative residual {info.relative_residual:.3e} > {cfg.pcg_rtol_mech:.3e})")
+        return x, info, g
+
+    def _damage(self, H, lower, initial):
+        """staggered.py:753-802 (box_kkt irreversibility)."""
+        cfg = self.cfg
+        system = self.op.damage_system(H)
+        r0 = float(torch.linalg.vector_norm(system.rhs - system.apply(initial)))
+        if not math.isfinite(r0):
+            raise _AttemptFailure("damage residual is not finite")
+        tol = min(cfg.damage_linear_tol, 0.1 * cfg.damage_kkt_tol / max(r0, 1.0))
+        tol = max(tol, _MIN_LINEAR_TOLERANCE)
+        try:
+            res = solve_h8_scalar_box_kkt(system, lower_bound=lower, upper_bound=1.0, initial=initial,
+                                          linear_tolerance=tol, kkt_tolerance=cfg.damage_kkt_tol,
+                                          max_linear_iterations=cfg.damage_maxit,
+                                          max_active_set_iterations=cfg.damage_max_active_set)
+        except H8ScalarBoxKKTConvergenceError as exc:
+            raise _AttemptFailure(f"damage box-KKT solve failed: {exc}") from exc
+        except (RuntimeError, ValueError) as exc:
+            raise _AttemptFailure(f"damage PCG breakdown: {exc}") from exc
+        return res.solution, res.total_pcg_iterations, res.active_set_iterations, res.kkt.satisfies_kkt
+
+    def _change(self, x_new, x_old, d_new, d_old, step_size):
+        """staggered.py:1097-1121."""
+        n = self._n_u
+        parts = torch.stack([torch.linalg.vector_norm(x_new[:n] - x_old[:n]), torch.linalg.vector_norm(x_new[:n]),
+                             (x_new[n:] - x_old[n:]).abs().max(), x_new[n:].abs().max(),
+                             (d_new - d_old).abs().max()]).cpu()
+        du, u, dE, E_inf, dd = (float(v) for v in parts)
+        if not all(math.isfinite(v) for v in (du, u, dE, E_inf, dd)):
+            raise _AttemptFailure("staggered iterate is not finite")
+        scale_E = max(E_inf, step_size)
+        u_ref = max(u, scale_E * self._u_floor)
+        du_rel = du / u_ref if u_ref > 0.0 else 0.0
+        dE_rel = dE / scale_E if scale_E > 0.0 else 0.0
+        return max(du_rel, dE_rel, dd)
+
+    def fracture_energy(self, d) -> float:
+        """Gamma(d) = 1/2 sum_e Gc d_e.(M/ell + ell L) d_e (operator.py:1511-1544, total)."""
+        return 0.5 * float(torch.dot(d, self.op.damage_action(d)))
+
+    # -- one attempt -----------------------------------------------------------------
+    def _attempt(self, state, target, previous, increment, cutbacks, start):
+        op, cfg = self.op, self.cfg
+        values = op.mech_fixed_values(target)
+        step_size = abs(target - previous)
+        x_old = self._predictor(state, previous, target)
+        d_old = state.d.clone()
+        lower = state.d
+        mech_s = damage_s = history_s = 0.0
+        pcg_mech = pcg_pf = active_set = 0
+        converged, change, H, info = False, math.inf, state.H, None
+        k = 0
+        for k in range(1, cfg.max_stagger + 1):
+            t0 = perf_counter()
+            x_new, info, g = self._mechanics(d_old, values, x_old)
+            pcg_mech += info.iterations
+            t1 = perf_counter()
+            Y = op.driving_force(x_new)
+            H = torch.maximum(state.H, Y)
+            t2 = perf_counter()
+            d_new, iters, sets, kkt_ok = self._damage(H, lower, d_old)
+            pcg_pf += iters
+            active_set += sets
+            t3 = perf_counter()
+            change = self._change(x_new, x_old, d_new, d_old, step_size)
+            mech_s += t1 - t0
+            history_s += t2 - t1
+            damage_s += t3 - t2
+            if change <= cfg.stagger_tol and k >= cfg.min_stagger:
+                x_old, d_old = x_new, d_new
+                converged = True
+                break
+            x_old, d_old = x_new, d_new
+        if not converged:
+            raise _AttemptFailure(f"staggered iteration did not converge in {cfg.max_stagger} iterations "
+                                  f"(change {change:.3e} > stagger_tol {cfg.stagger_tol:.3e})")
+        d = d_old
+        t0 = perf_counter()
+        if cfg.final_mechanics_resolve:
+            x, info, g = self._mechanics(d, values, x_old)
+            pcg_mech += info.iterations
+        else:
+            x, g = x_old, op.degradation(d)
+        mech_s += perf_counter() - t0
+        sums = op._stress_sums(x, g)
+        sigma = sums[:6] / op.volume
+        energy = float(sums[6])
+        E = x[self._n_u:]
+        mean_work = 2.0 * energy / op.volume
+        macro_work = float(torch.dot(sigma, E))
+        scale = max(abs(mean_work), abs(macro_work))
+        hill_rel = abs(mean_work - macro_work) / scale if scale > 0.0 else 0.0
+        fracture = self.fracture_energy(d)
+        elastic = 0.5 * op.volume * mean_work
+        stats = torch.stack([d.max(), d.mean(), H.max()]).cpu().numpy()
+        macro_strain = tuple(float(v) for v in E.cpu())
+        macro_stress = tuple(float(v) for v in sigma.cpu())
+        if not np.isfinite(macro_stress).all() or not math.isfinite(elastic) or not math.isfinite(fracture):
+            raise _AttemptFailure("accepted state has non-finite stress, strain or energy")
+        c = self._controlled
+        sigma_prev = float(state.macro_stress[c])
+        external = state.external_work + op.volume * 0.5 * (macro_stress[c] + sigma_prev) * (target - previous)
+        residual = external - (elastic + fracture)
+        completes = 1 <= increment <= len(self.load.targets) and target == self.load.targets[increment - 1]
+        new_state = SolverState(state.step + 1, x, d, H, E.clone(), sigma, external, state.d,
+                                increment if completes else state.increment)
+        fa, fb = self._face_lengths
+        rec = IncrementRecord(step=new_state.step, increment=increment, macro_strain=macro_strain,
+                              macro_stress=macro_stress, reaction_force=macro_stress[self._load_axis] * fa * fb,
+                              applied_disp=macro_strain[self._load_axis] * self._load_length, elastic=elastic,
+                              fracture=fracture, external=external,
+                              balance_residual_rel=residual / max(abs(external), 1e-300),
+                              max_damage=float(stats[0]), mean_damage=float(stats[1]), max_H=float(stats[2]),
+                              stagger_iters=k, pcg_iters_mech=pcg_mech, pcg_iters_pf=pcg_pf,
+                              active_set_iters=active_set, cutbacks=cutbacks, stagger_change=change,
+                              hill_mandel_rel=hill_rel, wall_s=perf_counter() - start, mech_s=mech_s,
+                              damage_s=damage_s, history_s=history_s)
+        return new_state, rec
+
+    def advance(self, state, target, *, increment=None, cutbacks=0):
+        """staggered.py:1215-1257."""
+        target = float(target)
+        previous = float(state.macro_strain[self._controlled])
+        if target == previous:
+            raise ValueError(f"target {target!r} equals the current controlled strain")
+        increment = state.increment + 1 if increment is None else int(increment)
+        self.n_attempts += 1
+        start = perf_counter()
+        try:
+            new_state, rec = self._attempt(state, target, previous, increment, int(cutbacks), start)
+        except _AttemptFailure as exc:
+            raise StaggerConvergenceError(f"increment {increment} (cutback level {cutbacks}) failed: {exc}",
+                                          state=state, target=target, increment=increment, cutbacks=cutbacks,
+                                          reason=str(exc)) from exc
+        self.last_state = new_state
+        return new_state, rec
+
+    def advance_increment(self, state, target, *, increment=None, on_increment=None):
+        """Recursive half-step cutback (staggered.py:1523-1582)."""
+        increment = state.increment + 1 if increment is None else int(increment)
+        accepted, latest = [], [state]
+
+        def solve(current, end, depth):
+            try:
+                new_state, rec = self.advance(current, end, increment=increment, cutbacks=depth)
+            except StaggerConvergenceError as exc:
+                if self.cfg.on_stagger_failure == "raise" or depth >= self.cfg.max_cutbacks:
+                    raise StaggerConvergenceError(f"increment {increment} failed at cutback level {depth}: "
+                                                  f"{exc.reason}", state=latest[0], target=end,
+                                                  increment=increment, cutbacks=depth, reason=exc.reason,
+                                                  records=tuple(accepted)) from exc
+                begin = float(current.macro_strain[self._controlled])
+                middle = 0.5 * (begin + end)
+                current = solve(current, middle, depth + 1)
+                return solve(current, end, depth + 1)
+            accepted.append(rec)
+            latest[0] = new_state
+            if on_increment is not None:
+                on_increment(new_state, rec)
+            return new_state
+
+        final = solve(state, float(target), 0)
+        if final.increment != increment:
+            final = dataclasses.replace(final, increment=increment)
+            self.last_state = final
+        return final, accepted
+
+    def run(self, *, start_state=None, max_increments=None, on_increment=None):
+        """Follow the load path (staggered.py:1584-1627)."""
+        state = self.initial_state() if start_state is None else start_state
+        records = []
+        stop = len(self.load.targets) if max_increments is None else min(len(self.load.targets),
+                                                                          state.increment + max_increments)
+        while state.increment < stop:
+            state, recs = self.advance_increment(state, self.load.targets[state.increment],
+                                                 on_increment=on_increment)
+            records.extend(recs)
+        self.last_state = state
+        return records

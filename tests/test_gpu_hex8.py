@@ -155,3 +155,42 @@ def test_pore_in_cube_recorded_elastic_rows(dev):
         assert info.converged
         sig = float(op.macro_stress(x)[0])
         assert abs(sig - row["macro_stress_xx"]) / row["macro_stress_xx"] < 2e-7, (sig, row)
+
+
+def test_staggered_increments_track_oracle(dev):
+    """Three AT2 increments of the voxel staggered solver on a small damaged cell."""
+    from phast_amd import staggered3d as ST
+    S, op, oop = build((8, 8, 8), h=(1e-3,) * 3, seed=11, pores=False)
+    cfg = ST.SolverConfig(pcg_rtol_mech=1e-12, damage_kkt_tol=1e-9)
+    solver = ST.StaggeredSolver(op, cfg=cfg)
+    st = solver.initial_state()
+    ost = {"x": np.zeros(op.n_mech), "d": np.zeros(op.n_red), "H": np.zeros((op.n_elements, 8))}
+    for target in (4e-3, 8e-3, 1.2e-2):
+        st, rec = solver.advance(st, target)
+        ost, (k, sigma) = O.staggered_increment(oop, ost, target, rtol=1e-12, damage_kkt_tol=1e-9)
+        assert rec.stagger_iters == k
+        assert rel(st.d, ost["d"]) < 1e-8 and rel(st.x, ost["x"]) < 1e-8
+        assert rel(st.macro_stress, sigma) < 1e-8
+    assert float(st.d.max()) > 0.05  # damage actually evolved
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (24, 16, 32)])
+def test_multigrid_pcg_matches_jacobi_solution(dev, shape):
+    """MG-preconditioned PCG converges to the Jacobi-PCG / oracle solution in far fewer iterations."""
+    S, op, oop = build(shape, seed=9)
+    rng = np.random.default_rng(10)
+    d = T(rng.uniform(0, 0.8, op.n_red), dev)
+    g = op.degradation(d)
+    fixed, vals, diag = op.mech_fixed_mask(), op.mech_fixed_values(1e-3), op.mech_diagonal(g)
+    xj, ij = op.solve_mechanics(d=d, fixed_mask=fixed, fixed_values=vals, diag=diag, rtol=1e-12, maxiter=20000)
+    mg = S.VoxelMultigrid(op, min_coarse=4)
+    assert len(mg.levels) >= 2
+    xm, im = mg.solve(d=d, fixed_mask=fixed, fixed_values=vals, diag=diag, rtol=1e-12, maxiter=2000)
+    assert ij.converged and im.converged
+    assert im.iterations * 3 < ij.iterations, (im.iterations, ij.iterations)
+    assert rel(xm, xj) < 1e-9
+    # the V-cycle is symmetric: u.M v == v.M u
+    u, v = (T(np.where(fixed.cpu().numpy(), 0.0, rng.normal(size=op.n_mech)), dev) for _ in range(2))
+    a, b = float(torch.dot(u, mg.apply(v))), float(torch.dot(v, mg.apply(u)))
+    assert abs(a - b) <= 1e-10 * max(abs(a), abs(b))
+    assert float(torch.dot(u, mg.apply(u))) > 0
