@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--tiles", type=str, default="", help="fixed hex8 tile tx,ty,kz (skips the autotuner)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 5 device copies of 256 MiB (the reference kernel for HBM counter scaling)")
     return ap.parse_args()
 
 
@@ -103,6 +105,14 @@ def main():
     achieved = HEX8_BYTES_PER_NODE * op.n_red / t_apply / 1e9
     flops_per_voxel = 1238.0
     tflops = flops_per_voxel * op.n_elements / t_apply / 1e12
+
+    if a.copy_calibration:  # a plain copy kernel of known size for FETCH_SIZE / WRITE_SIZE scaling
+        src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).uniform_()
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ------------------------------------------------------------------ CPU baseline (oracle port)
     cpu = None

@@ -70,11 +70,13 @@ typedef struct phast_q4_desc {
     double eta_residual;     /* g(d) = (1-d)^2 (1-eta) + eta  (physics/material.py:529)     */
     double Gc, l0;           /* AT2 damage constants                                       */
     double dt;               /* explicit step size                                         */
+    double hx, hy;           /* cell edge lengths (cells are congruent axis-aligned rectangles) */
 } phast_q4_desc;
 
 typedef struct phast_q4_plan phast_q4_plan;
 
-/* tile_x * tile_y must be 256; supported (64,4) (32,8) (16,16); (0,0) = default. */
+/* Work decomposition: strips of tile_x threads (64, 128 or 256) covering tile_x - 1 node
+ * columns and tile_y node rows each; (0,0) = default (128, 8). */
 int phast_q4_plan_create(const phast_q4_desc *desc, int32_t n_inst, int32_t tile_x, int32_t tile_y,
                          phast_q4_plan **out);
 int phast_q4_plan_destroy(phast_q4_plan *plan);
@@ -235,6 +237,9 @@ int phast_hex8_mg_update(phast_hex8_mg *mg, const double *g, const double *d, hi
 /* z = V(r) on the fine level (3n+6 vectors, zero on the fixed set) */
 int phast_hex8_mg_apply(phast_hex8_mg *mg, const double *r, double *z, hipStream_t stream);
 int phast_hex8_mg_levels(const phast_hex8_mg *mg, int32_t *levels, int32_t *dims);
+/* per-level lambda_max(D^-1 A) estimates (12 power iterations at update) and the Jacobi
+ * weights omega_l = omega / (1.05 lambda_max) in use */
+int phast_hex8_mg_weights(const phast_hex8_mg *mg, double *lam_max, double *omega);
 /* phast_hex8_mech_solve preconditioned by the V-cycle (same flag-masked PCG). */
 int phast_hex8_mech_solve_mg(const phast_hex8_plan *plan, phast_hex8_mg *mg, const double *g, const double *d,
                              const uint8_t *fixed, const double *fixed_values, const double *rhs, const double *x0,

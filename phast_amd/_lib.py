@@ -36,6 +36,7 @@ class Q4Desc(C.Structure):
         ("lam", C.c_double), ("mu", C.c_double), ("kappa", C.c_double), ("nu", C.c_double),
         ("plane_stress", C.c_int32), ("split", C.c_int32),
         ("eta_residual", C.c_double), ("Gc", C.c_double), ("l0", C.c_double), ("dt", C.c_double),
+        ("hx", C.c_double), ("hy", C.c_double),
     ]
 
 

@@ -4,8 +4,9 @@ from __future__ import annotations
 
 import torch
 
-HEX8_TILES = [(16, 8, 8), (16, 8, 16), (16, 16, 8), (8, 8, 8), (8, 8, 16), (32, 8, 8), (16, 8, 4), (8, 8, 4)]
-Q4_TILES = [(64, 4), (32, 8), (16, 16)]
+HEX8_TILES = [(16, 8, 8), (16, 8, 16), (16, 16, 8), (8, 8, 8), (8, 8, 16), (32, 8, 8), (16, 8, 4), (8, 8, 4),
+              (16, 8, 2), (32, 8, 2), (16, 16, 2), (8, 8, 2), (32, 8, 4), (16, 16, 4)]
+Q4_TILES = [(128, 8), (128, 32), (256, 16), (256, 64), (64, 32), (256, 128)]
 
 _cache: dict = {}
 
@@ -41,7 +42,7 @@ def tune_hex8(op, x: torch.Tensor, d: torch.Tensor, reps: int = 10, candidates=H
 
 
 def tune_q4(make_fem, u: torch.Tensor, d: torch.Tensor, reps: int = 10, candidates=Q4_TILES):
-    """Pick the Q4 node tile; ``make_fem(tile)`` builds an FEMOperators with that tile."""
+    """Pick the Q4 strip (threads, rows); ``make_fem(tile)`` builds an FEMOperators with that tile."""
     timings, fems = {}, {}
     for t in candidates:
         fem = make_fem(t)

@@ -24,7 +24,7 @@ class BatchedExplicit:
     optional list of per-cell Gc arrays (E_k,); ``f_ext`` an optional list of (N_k,2)."""
 
     def __init__(self, items, *, dt_factor: float = 0.5, dts=None, Gc_fields=None, f_ext=None, tol: float = 1e-5,
-                 max_iter: int = 5000, bounds_method: str = "projected_cg", tile=(64, 4)):
+                 max_iter: int = 5000, bounds_method: str = "projected_cg", tile=(128, 8)):
         dev = _lib.require_device()
         self.device = dev
         self.meshes = [m for m, _ in items]

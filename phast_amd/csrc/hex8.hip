@@ -44,36 +44,72 @@ __device__ __forceinline__ void interp2T(double al, double be, const double S[2]
     R[1][1] = be * t01 + al * t11;
 }
 
-// Gradients of the three displacement components at the Gauss points.
-// gx[c][iy][iz], gy[c][ix][iz], gz[c][ix][iy]; u[c][lx + 2 ly + 4 lz].
+// interp2 whose second stage carries a scale: weights (as, bs) = (al, be) * h^-1
+__device__ __forceinline__ void interp2s(double al, double be, double as, double bs, double f00, double f10,
+                                         double f01, double f11, double o[2][2]) {
+    const double t00 = al * f00 + be * f10, t10 = be * f00 + al * f10;
+    const double t01 = al * f01 + be * f11, t11 = be * f01 + al * f11;
+    o[0][0] = as * t00 + bs * t01;
+    o[0][1] = bs * t00 + as * t01;
+    o[1][0] = as * t10 + bs * t11;
+    o[1][1] = bs * t10 + as * t11;
+}
+// transpose with scaled second stage
+__device__ __forceinline__ void interp2Ts(double al, double be, double as, double bs, const double S[2][2],
+                                          double R[2][2]) {
+    const double t00 = al * S[0][0] + be * S[0][1], t01 = be * S[0][0] + al * S[0][1];
+    const double t10 = al * S[1][0] + be * S[1][1], t11 = be * S[1][0] + al * S[1][1];
+    R[0][0] = as * t00 + bs * t10;
+    R[1][0] = bs * t00 + as * t10;
+    R[0][1] = as * t01 + bs * t11;
+    R[1][1] = bs * t01 + as * t11;
+}
+
+// Gradients of the three displacement components at the Gauss points (1/h folded into the
+// second interpolation stage): gx[c][iy][iz], gy[c][ix][iz], gz[c][ix][iy]; u[c][lx+2ly+4lz].
+// The first stage interpolates along the second index of the output, the second stage
+// (scaled) along the first... arranged so that every derivative is the difference along its
+// own axis interpolated in the two others.
 __device__ __forceinline__ void voxel_grad(const Hex8Geo &G, const double u[3][8], double gx[3][2][2],
                                            double gy[3][2][2], double gz[3][2][2]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        interp2(G.al, G.be, u[c][1] - u[c][0], u[c][3] - u[c][2], u[c][5] - u[c][4], u[c][7] - u[c][6], gx[c]);
-        interp2(G.al, G.be, u[c][2] - u[c][0], u[c][3] - u[c][1], u[c][6] - u[c][4], u[c][7] - u[c][5], gy[c]);
-        interp2(G.al, G.be, u[c][4] - u[c][0], u[c][5] - u[c][1], u[c][6] - u[c][2], u[c][7] - u[c][3], gz[c]);
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                gx[c][a][b] *= G.ihx;
-                gy[c][a][b] *= G.ihy;
-                gz[c][a][b] *= G.ihz;
-            }
+        // x-difference at (ly,lz) -> gx[iy][iz]: interp over ly (first index) scaled by 1/hx
+        interp2s(G.al, G.be, G.alx, G.bex, u[c][1] - u[c][0], u[c][3] - u[c][2], u[c][5] - u[c][4],
+                 u[c][7] - u[c][6], gx[c]);
+        interp2s(G.al, G.be, G.aly, G.bey, u[c][2] - u[c][0], u[c][3] - u[c][1], u[c][6] - u[c][4],
+                 u[c][7] - u[c][5], gy[c]);
+        interp2s(G.al, G.be, G.alz, G.bez, u[c][4] - u[c][0], u[c][5] - u[c][1], u[c][6] - u[c][2],
+                 u[c][7] - u[c][3], gz[c]);
     }
 }
 
-// engineering strain (xx,yy,zz,yz,xz,xy) at Gauss point (ix,iy,iz) plus the macro strain
+// engineering strain (xx,yy,zz,yz,xz,xy) at Gauss point (ix,iy,iz); the macro strain has
+// been added into the gradient sets beforehand (fold_macro).
 __device__ __forceinline__ void qp_strain(const double gx[3][2][2], const double gy[3][2][2],
-                                          const double gz[3][2][2], const double E[6], int ix, int iy, int iz,
-                                          double e[6]) {
-    e[0] = gx[0][iy][iz] + E[0];
-    e[1] = gy[1][ix][iz] + E[1];
-    e[2] = gz[2][ix][iy] + E[2];
-    e[3] = (gz[1][ix][iy] + gy[2][ix][iz]) + E[3];
-    e[4] = (gz[0][ix][iy] + gx[2][iy][iz]) + E[4];
-    e[5] = (gy[0][ix][iz] + gx[1][iy][iz]) + E[5];
+                                          const double gz[3][2][2], int ix, int iy, int iz, double e[6]) {
+    e[0] = gx[0][iy][iz];
+    e[1] = gy[1][ix][iz];
+    e[2] = gz[2][ix][iy];
+    e[3] = gz[1][ix][iy] + gy[2][ix][iz];
+    e[4] = gz[0][ix][iy] + gx[2][iy][iz];
+    e[5] = gy[0][ix][iz] + gx[1][iy][iz];
+}
+
+// add E into the gradient entries each strain component starts from
+__device__ __forceinline__ void fold_macro(const double E[6], double gx[3][2][2], double gy[3][2][2],
+                                           double gz[3][2][2]) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            gx[0][a][b] += E[0];
+            gy[1][a][b] += E[1];
+            gz[2][a][b] += E[2];
+            gz[1][a][b] += E[3];
+            gz[0][a][b] += E[4];
+            gy[0][a][b] += E[5];
+        }
 }
 
 // trilinear interpolation of corner scalars to the 8 Gauss points (xi fastest)
@@ -98,14 +134,16 @@ __device__ __forceinline__ void voxel_interp(const Hex8Geo &G, const double d[8]
     }
 }
 
-// Internal force of one voxel: r[c][corner] = sum_q w B_q^T (g_q C (B_q u + E)).
-// Also returns sum_q w sigma_q (6) and 0.5 sum_q w sigma.eps (operator.py:1108-1169).
+// Internal force of one voxel: r[c][corner] = sum_q w B_q^T (g_q C (B_q u + E))
+// (operator.py:1108-1169).  With OWN, sE += sum_q w sigma_q and (ENERGY) en += 0.5 sum_q
+// w sigma.eps.  sum-factorised gradients; g w folded into the moduli.
 template <bool ISO, bool ENERGY>
 __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8], const double E[6],
                                            const double g[8], const double *__restrict__ C, double lam,
-                                           double mu, double r[3][8], double sE[6], double &en) {
+                                           double mu, bool own, double r[3][8], double sE[6], double &en) {
     double gx[3][2][2], gy[3][2][2], gz[3][2][2];
     voxel_grad(G, u, gx, gy, gz);
+    fold_macro(E, gx, gy, gz);
     double Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -113,6 +151,7 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) Sx[c][a][b] = Sy[c][a][b] = Sz[c][a][b] = 0.0;
+    double ew = 0.0;
 #pragma unroll
     for (int iz = 0; iz < 2; ++iz)
 #pragma unroll
@@ -120,31 +159,32 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
 #pragma unroll
             for (int ix = 0; ix < 2; ++ix) {
                 double e[6];
-                qp_strain(gx, gy, gz, E, ix, iy, iz, e);
+                qp_strain(gx, gy, gz, ix, iy, iz, e);
+                const double gw = g[ix + 2 * iy + 4 * iz] * G.w;
                 double s[6];
                 if (ISO) {
-                    const double tr = e[0] + e[1] + e[2];
-                    s[0] = lam * tr + 2.0 * mu * e[0];
-                    s[1] = lam * tr + 2.0 * mu * e[1];
-                    s[2] = lam * tr + 2.0 * mu * e[2];
-                    s[3] = mu * e[3];
-                    s[4] = mu * e[4];
-                    s[5] = mu * e[5];
+                    const double lg = lam * gw, mg = mu * gw, mg2 = 2.0 * mg;
+                    const double lt = lg * ((e[0] + e[1]) + e[2]);
+                    s[0] = lt + mg2 * e[0];
+                    s[1] = lt + mg2 * e[1];
+                    s[2] = lt + mg2 * e[2];
+                    s[3] = mg * e[3];
+                    s[4] = mg * e[4];
+                    s[5] = mg * e[5];
                 } else {
 #pragma unroll
                     for (int k = 0; k < 6; ++k) {
                         double t = 0.0;
 #pragma unroll
                         for (int m = 0; m < 6; ++m) t += C[k * 6 + m] * e[m];
-                        s[k] = t;
+                        s[k] = gw * t;
                     }
                 }
-                const double gw = g[ix + 2 * iy + 4 * iz] * G.w;
+                if (ENERGY) {
+                    double t = 0.0;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    s[k] *= gw;
-                    sE[k] += s[k];
-                    if (ENERGY) en += 0.5 * s[k] * e[k];
+                    for (int k = 0; k < 6; ++k) t += s[k] * e[k];
+                    ew += t;
                 }
                 Sx[0][iy][iz] += s[0];
                 Sx[1][iy][iz] += s[5];
@@ -156,35 +196,41 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
                 Sz[1][ix][iy] += s[3];
                 Sz[2][ix][iy] += s[2];
             }
+    if (own) {  // sum_q w sigma_q from the partial sums (each component appears once per channel)
+        sE[0] += (Sx[0][0][0] + Sx[0][1][0]) + (Sx[0][0][1] + Sx[0][1][1]);
+        sE[1] += (Sy[1][0][0] + Sy[1][1][0]) + (Sy[1][0][1] + Sy[1][1][1]);
+        sE[2] += (Sz[2][0][0] + Sz[2][1][0]) + (Sz[2][0][1] + Sz[2][1][1]);
+        sE[3] += (Sy[2][0][0] + Sy[2][1][0]) + (Sy[2][0][1] + Sy[2][1][1]);
+        sE[4] += (Sx[2][0][0] + Sx[2][1][0]) + (Sx[2][0][1] + Sx[2][1][1]);
+        sE[5] += (Sx[1][0][0] + Sx[1][1][0]) + (Sx[1][0][1] + Sx[1][1][1]);
+        if (ENERGY) en += 0.5 * ew;
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         double R[2][2];
-        interp2T(G.al, G.be, Sx[c], R);  // R[ly][lz]
+        interp2Ts(G.al, G.be, G.alx, G.bex, Sx[c], R);  // R[ly][lz]
 #pragma unroll
         for (int ly = 0; ly < 2; ++ly)
 #pragma unroll
             for (int lz = 0; lz < 2; ++lz) {
-                const double t = R[ly][lz] * G.ihx;
-                r[c][1 + 2 * ly + 4 * lz] = t;
-                r[c][2 * ly + 4 * lz] = -t;
+                r[c][1 + 2 * ly + 4 * lz] = R[ly][lz];
+                r[c][2 * ly + 4 * lz] = -R[ly][lz];
             }
-        interp2T(G.al, G.be, Sy[c], R);  // R[lx][lz]
+        interp2Ts(G.al, G.be, G.aly, G.bey, Sy[c], R);  // R[lx][lz]
 #pragma unroll
         for (int lx = 0; lx < 2; ++lx)
 #pragma unroll
             for (int lz = 0; lz < 2; ++lz) {
-                const double t = R[lx][lz] * G.ihy;
-                r[c][lx + 2 + 4 * lz] += t;
-                r[c][lx + 4 * lz] -= t;
+                r[c][lx + 2 + 4 * lz] += R[lx][lz];
+                r[c][lx + 4 * lz] -= R[lx][lz];
             }
-        interp2T(G.al, G.be, Sz[c], R);  // R[lx][ly]
+        interp2Ts(G.al, G.be, G.alz, G.bez, Sz[c], R);  // R[lx][ly]
 #pragma unroll
         for (int lx = 0; lx < 2; ++lx)
 #pragma unroll
             for (int ly = 0; ly < 2; ++ly) {
-                const double t = R[lx][ly] * G.ihz;
-                r[c][lx + 2 * ly + 4] += t;
-                r[c][lx + 2 * ly] -= t;
+                r[c][lx + 2 * ly + 4] += R[lx][ly];
+                r[c][lx + 2 * ly] -= R[lx][ly];
             }
     }
 }
@@ -281,17 +327,12 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
                 for (int q = 0; q < 8; ++q) g[q] = 1.0;
             }
             const bool own = li >= 0 && lj >= 0 && kv >= k0 && iv < nx && jv < ny;
-            double r[3][8], vE[6] = {0, 0, 0, 0, 0, 0}, ve = 0.0;
+            double r[3][8];
             if (ISO) {
-                voxel_mech<true, ENERGY>(G, u, E, g, nullptr, A.lam[e], A.mu[e], r, vE, ve);
+                voxel_mech<true, ENERGY>(G, u, E, g, nullptr, A.lam[e], A.mu[e], own, r, sE, en);
             } else {
-                voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)A.bank_index[e], 0.0, 0.0, r, vE,
-                                          ve);
-            }
-            if (own) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) sE[k] += vE[k];
-                en += ve;
+                voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)A.bank_index[e], 0.0, 0.0, own, r,
+                                          sE, en);
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c)
@@ -514,12 +555,13 @@ __global__ __launch_bounds__(256) void hex8_driving_kernel(const Hex8Geo G, cons
     for (int c = 0; c < 6; ++c) E[c] = x[3 * n + c];
     double gx[3][2][2], gy[3][2][2], gz[3][2][2];
     voxel_grad(G, u, gx, gy, gz);
+    fold_macro(E, gx, gy, gz);
     const double *C = bank + 36 * (int64_t)bank_index[e];
     const double la = lam[e], m = mu[e];
     for (int q = 0; q < 8; ++q) {
         const int ix = q & 1, iy = (q >> 1) & 1, iz = q >> 2;
         double s[6];
-        qp_strain(gx, gy, gz, E, ix, iy, iz, s);
+        qp_strain(gx, gy, gz, ix, iy, iz, s);
         double val;
         if (code == 1) {  // 0.5 eps.C.eps (splits.py:68-72)
             double t = 0.0;
@@ -780,6 +822,12 @@ void phast::build_voxel_constants(Hex8Geo &G, double hx, double hy, double hz, d
     G.ihx = 1.0 / hx;
     G.ihy = 1.0 / hy;
     G.ihz = 1.0 / hz;
+    G.alx = G.al * G.ihx;
+    G.bex = G.be * G.ihx;
+    G.aly = G.al * G.ihy;
+    G.bey = G.be * G.ihy;
+    G.alz = G.al * G.ihz;
+    G.bez = G.be * G.ihz;
 }
 
 extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,

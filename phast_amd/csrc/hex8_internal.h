@@ -10,6 +10,7 @@ struct Hex8Geo {
     double ihx, ihy, ihz;   // 1/h
     double w;               // Gauss weight * det J = hx hy hz / 8
     double al, be;          // linear interpolation weights (1 +- 1/sqrt3)/2
+    double alx, bex, aly, bey, alz, bez;  // al/h, be/h per axis (interpolation + derivative scale)
     double kappa, omk, ch;  // residual stiffness, 1-kappa, 2(1-kappa)
     double K[64];           // AT2 voxel matrix M/ell + ell L, lexicographic corners [a*8+b]
     double Nq[64];          // N[q*8 + a], lexicographic corners, xi-fastest Gauss points

@@ -138,6 +138,39 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, const double *b, co
     r[i] = fixed[i] ? 0.0 : b[i] - Ax[i];
 }
 
+// w = D^-1 A v on free dofs (0 on fixed); partials |w|^2, |v|^2 per block
+__global__ __launch_bounds__(256) void k_power_step(int64_t n, const double *Av, const double *diag, const uint8_t *fixed,
+                                                    const double *v, double *w, double *part) {
+    __shared__ double red[4];
+    double sw = 0.0, sv = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double wi = fixed[i] ? 0.0 : Av[i] / diag[i];
+        w[i] = wi;
+        sw += wi * wi;
+        sv += v[i] * v[i];
+    }
+    sw = block_sum<256>(sw, red);
+    sv = block_sum<256>(sv, red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sw;
+        part[2 * blockIdx.x + 1] = sv;
+    }
+}
+
+__global__ void k_scale_copy(int64_t n, const double *w, double s, double *v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = s * w[i];
+}
+
+// deterministic start vector for the power iteration (zero on the fixed set)
+__global__ void k_power_start(int64_t n, const uint8_t *fixed, double *v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const uint64_t h = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+        v[i] = fixed[i] ? 0.0 : ((double)(h >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+    }
+}
+
 __global__ void k_set_fixed(int64_t n_nodes, int64_t pinned, const uint8_t *macro_fixed6, uint8_t *fixed) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < 3 * n_nodes + 6) {
@@ -163,7 +196,10 @@ struct phast_hex8_mg {
     uint8_t *macro6 = nullptr;
     const double *g = nullptr, *d = nullptr;  // current fine degradation source
     int nu = 2, coarse_sweeps = 30;
-    double omega = 0.6;
+    double omega = 1.2;                  // weight factor: omega_l = omega / lambda_max(D^-1 A)_l
+    double omega_l[kMaxLevels] = {};
+    double lam_max[kMaxLevels] = {};
+    double *hpart = nullptr;             // pinned host partials
 };
 
 static inline unsigned nb256(int64_t n) { return (unsigned)ceil_div(std::max<int64_t>(n, 1), 256); }
@@ -188,6 +224,7 @@ extern "C" int phast_hex8_mg_destroy(phast_hex8_mg *mg) {
         (void)hipFree(mg->fixed[l]);
     }
     (void)hipFree(mg->macro6);
+    if (mg->hpart) (void)hipHostFree(mg->hpart);
     delete mg;
     return PHAST_OK;
 }
@@ -215,7 +252,8 @@ extern "C" int phast_hex8_mg_create(const phast_hex8_plan *fine, int64_t pinned,
         phast_hex8_mg_destroy(mg);
         return PHAST_ERR_HIP;
     };
-    if (hipMalloc(&mg->macro6, 6) != hipSuccess ||
+    if (hipHostMalloc(&mg->hpart, sizeof(double) * 2 * 512, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&mg->macro6, 6) != hipSuccess ||
         hipMemcpy(mg->macro6, macro_fixed6, 6, hipMemcpyHostToDevice) != hipSuccess)
         return fail("alloc");
     int64_t pin = pinned;
@@ -294,6 +332,37 @@ extern "C" int phast_hex8_mg_update(phast_hex8_mg *mg, const double *g, const do
         rc = phast_hex8_mech_diagonal(c, nullptr, mg->diag[l], st);
     }
     if (gtmp) (void)hipFreeAsync(gtmp, st);
+    // lambda_max(D^-1 A) per level by 12 power iterations -> Jacobi weights
+    for (int l = 0; l < mg->levels && rc == PHAST_OK; ++l) {
+        const int64_t nm = 3 * mg->n[l] + 6;
+        const unsigned gb = (unsigned)std::min<int64_t>(512, nb256(nm));
+        const double *g = l == 0 ? mg->g : nullptr;
+        const double *d = l == 0 ? mg->d : nullptr;
+        hipLaunchKernelGGL(k_power_start, dim3(nb256(nm)), dim3(256), 0, st, nm, mg->fixed[l], mg->x[l]);
+        double lam = 0.0;
+        for (int it = 0; it < 12 && rc == PHAST_OK; ++it) {
+            rc = hex8_apply(mg->lv[l], mg->x[l], g, d, mg->Ax[l], nullptr, false, st);
+            if (rc) break;
+            hipLaunchKernelGGL(k_power_step, dim3(gb), dim3(256), 0, st, nm, mg->Ax[l], mg->diag[l], mg->fixed[l],
+                               mg->x[l], mg->r[l], mg->b[l]);
+            PHAST_HIP(hipMemcpyAsync(mg->hpart, mg->b[l], sizeof(double) * 2 * gb, hipMemcpyDeviceToHost, st));
+            PHAST_HIP(hipStreamSynchronize(st));
+            double sw = 0.0, sv = 0.0;
+            for (unsigned k = 0; k < gb; ++k) {
+                sw += mg->hpart[2 * k];
+                sv += mg->hpart[2 * k + 1];
+            }
+            if (!(sw > 0.0) || !(sv > 0.0) || !std::isfinite(sw)) {
+                set_error("multigrid power iteration broke down on level %d", l);
+                return PHAST_ERR_NONFINITE;
+            }
+            lam = std::sqrt(sw / sv);
+            hipLaunchKernelGGL(k_scale_copy, dim3(nb256(nm)), dim3(256), 0, st, nm, mg->r[l], 1.0 / std::sqrt(sw),
+                               mg->x[l]);
+        }
+        mg->lam_max[l] = lam;
+        mg->omega_l[l] = mg->omega / (1.05 * lam);
+    }
     return rc;
 }
 
@@ -307,11 +376,11 @@ static int vcycle(phast_hex8_mg *mg, int l, hipStream_t st) {
     const int pre = coarsest ? mg->coarse_sweeps : mg->nu;
     // x = omega D^-1 b, then pre - 1 more sweeps
     hipLaunchKernelGGL(k_jacobi, dim3(gb), dim3(256), 0, st, nm, mg->x[l], mg->b[l], mg->Ax[l], mg->diag[l],
-                       mg->fixed[l], mg->omega, 1);
+                       mg->fixed[l], mg->omega_l[l], 1);
     for (int s = 1; s < pre; ++s) {
         PHAST_TRY0(hex8_apply(pl, mg->x[l], g, d, mg->Ax[l], nullptr, false, st));
         hipLaunchKernelGGL(k_jacobi, dim3(gb), dim3(256), 0, st, nm, mg->x[l], mg->b[l], mg->Ax[l], mg->diag[l],
-                           mg->fixed[l], mg->omega, 0);
+                           mg->fixed[l], mg->omega_l[l], 0);
     }
     if (coarsest) return PHAST_OK;
     PHAST_TRY0(hex8_apply(pl, mg->x[l], g, d, mg->Ax[l], nullptr, false, st));
@@ -325,7 +394,7 @@ static int vcycle(phast_hex8_mg *mg, int l, hipStream_t st) {
     for (int s = 0; s < mg->nu; ++s) {
         PHAST_TRY0(hex8_apply(pl, mg->x[l], g, d, mg->Ax[l], nullptr, false, st));
         hipLaunchKernelGGL(k_jacobi, dim3(gb), dim3(256), 0, st, nm, mg->x[l], mg->b[l], mg->Ax[l], mg->diag[l],
-                           mg->fixed[l], mg->omega, 0);
+                           mg->fixed[l], mg->omega_l[l], 0);
     }
     PHAST_LAUNCHED();
     return PHAST_OK;
@@ -338,6 +407,15 @@ extern "C" int phast_hex8_mg_apply(phast_hex8_mg *mg, const double *r, double *z
     PHAST_HIP(hipMemcpyAsync(mg->b[0], r, sizeof(double) * nm, hipMemcpyDeviceToDevice, st));
     PHAST_TRY0(vcycle(mg, 0, st));
     PHAST_HIP(hipMemcpyAsync(z, mg->x[0], sizeof(double) * nm, hipMemcpyDeviceToDevice, st));
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_mg_weights(const phast_hex8_mg *mg, double *lam_max, double *omega) {
+    PHAST_REQUIRE(mg, "phast_hex8_mg_weights: null argument");
+    for (int l = 0; l < mg->levels; ++l) {
+        if (lam_max) lam_max[l] = mg->lam_max[l];
+        if (omega) omega[l] = mg->omega_l[l];
+    }
     return PHAST_OK;
 }
 

@@ -6,16 +6,19 @@
 //   cells  : H, psi as (E,4) float64 (one 32 B row per cell, Gauss points in
 //            (xi,eta) = (-,-),(+,-),(-,+),(+,+) order).
 //
-// Work decomposition: one 256-thread block per tile of TX x TY nodes of one instance.
-// A block (1) stages the (TX+2)(TY+2) node halo in LDS, (2) evaluates each of the
-// (TX+1)(TY+1) cells touching its nodes once, writing the four corner contributions to
-// LDS, and (3) gathers every owned node's sum from the four cells around it, in cell
-// order.  No atomics: results are deterministic and race-free; each cell is owned (for
-// per-cell outputs such as H) by the tile of its lower-left node.  The block -> (instance,
-// tile) table makes one launch cover a whole batch of plates of uneven sizes.
+// Work decomposition: one block of NT threads (64, 128 or 256) per strip of NT-1 node
+// columns and KY node rows of one instance.  The block marches up its strip one cell row
+// at a time: each node row is staged once in LDS (plus a one-node x halo on either side),
+// each thread evaluates one cell of the row, the corners going to the right-hand
+// neighbour node pass through LDS and the corners going up stay in registers, and every
+// owned node sums the four cells around it in the reference's element order.  No
+// atomics: results are deterministic and race-free; each cell is owned (for per-cell
+// outputs such as H) by the strip of its lower-left node.  The block -> (instance, strip)
+// table makes one launch cover a whole batch of plates of uneven sizes.
 #include "common.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -27,9 +30,12 @@ struct Q4Inst {
     int32_t tile_off, n_tiles;
     double gx[16], gy[16], Nq[16], w[4];  // [q*4 + a]
     double dlap[4];                       // sum_q w (gx^2 + gy^2) of corner a
+    // sum-factorised rectangle constants: W(l,g) in {al, be} weights a corner row/column
+    // at a Gauss point; xa = al/hx ... fold 1/h, and fxa = w al/hx ... also fold w.
+    double al, be, xa, xb, ya, yb, fxa, fxb, fya, fyb, wal, wbe, wq, ome;
     double lam, mu, kappa, ezz_coef, eta;
     double Gc_l0, Gc_over_l0, l0, dt;
-    int32_t split, pad;
+    int32_t split, ky;
 };
 
 // Per-instance state of the device-resident damage CG.
@@ -43,7 +49,7 @@ constexpr int kParts = 4;  // per-tile partial sums kept between kernels
 }  // namespace phast
 
 struct phast_q4_plan {
-    int32_t n_inst = 0, tx = 64, ty = 4, n_tiles = 0;
+    int32_t n_inst = 0, tx = 128, ty = 8, n_tiles = 0;  // strip width (threads) and rows
     int64_t n_nodes = 0, n_cells = 0;
     std::vector<phast::Q4Inst> h_inst;
     phast::Q4Inst *d_inst = nullptr;
@@ -68,114 +74,132 @@ __device__ __forceinline__ double degrade(double dq, double eta) {
     return omd * omd * (1.0 - eta) + eta;  // physics/material.py:529
 }
 
-// Internal force contributions of one cell at its four corners, and (optionally) psi+ at
-// its four Gauss points.  fem_operators.py:244-254 (strain), 626-630 (g at qp),
-// 301-333 (Amor) / 277-290 (isotropic), 681-699 (assembly), 983-1006 (psi+).
+// Internal force contributions of one rectangular cell at its four corners (c0..c3 =
+// (0,0),(1,0),(1,1),(0,1)), and optionally psi+ at its Gauss points q = ix + 2 iy.
+// Sum-factorised: on a rectangle d/dx of the bilinear field is the row difference
+// interpolated in y (and d/dy the column difference interpolated in x), and the force is
+// the transpose.  Same arithmetic as fem_operators.py:244-254 (strain), 626-630 (g at the
+// Gauss points), 301-333 (Amor) / 277-290 (isotropic), 681-699 (assembly), 983-1006 (psi+).
 template <bool PSI>
 __device__ __forceinline__ void cell_mech(const Q4Inst &P, const double ux[4], const double uy[4],
                                           const double dd[4], double fx[4], double fy[4], double psi[4]) {
-#pragma unroll
-    for (int a = 0; a < 4; ++a) fx[a] = fy[a] = 0.0;
+    const double al = P.al, be = P.be;
+    const double dx0x = ux[1] - ux[0], dx1x = ux[2] - ux[3], dx0y = uy[1] - uy[0], dx1y = uy[2] - uy[3];
+    const double dy0x = ux[3] - ux[0], dy1x = ux[2] - ux[1], dy0y = uy[3] - uy[0], dy1y = uy[2] - uy[1];
+    const double exx_[2] = {P.xa * dx0x + P.xb * dx1x, P.xb * dx0x + P.xa * dx1x};  // [iy]
+    const double vyx_[2] = {P.xa * dx0y + P.xb * dx1y, P.xb * dx0y + P.xa * dx1y};  // d uy/dx [iy]
+    const double eyy_[2] = {P.ya * dy0y + P.yb * dy1y, P.yb * dy0y + P.ya * dy1y};  // [ix]
+    const double uxy_[2] = {P.ya * dy0x + P.yb * dy1x, P.yb * dy0x + P.ya * dy1x};  // d ux/dy [ix]
+    const double r00 = al * dd[0] + be * dd[1], r01 = be * dd[0] + al * dd[1];      // bottom row at ix
+    const double r10 = al * dd[3] + be * dd[2], r11 = be * dd[3] + al * dd[2];      // top row at ix
+    const double dq[4] = {al * r00 + be * r10, al * r01 + be * r11, be * r00 + al * r10, be * r01 + al * r11};
+    double sxx[4], syy[4], sxy[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        double exx = 0.0, eyy = 0.0, gxa = 0.0, gxb = 0.0, dq = 0.0;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            exx += P.gx[q * 4 + a] * ux[a];
-            eyy += P.gy[q * 4 + a] * uy[a];
-            gxa += P.gy[q * 4 + a] * ux[a];
-            gxb += P.gx[q * 4 + a] * uy[a];
-            dq += P.Nq[q * 4 + a] * dd[a];
-        }
-        const double gxy = gxa + gxb;
-        const double g = degrade(dq, P.eta);
-        double sxx, syy, sxy;
-        double tr = 0.0, trp = 0.0, ezz = 0.0;
+        const int ix = q & 1, iy = q >> 1;
+        const double exx = exx_[iy], eyy = eyy_[ix], gxy = uxy_[ix] + vyx_[iy];
+        const double omd = 1.0 - dq[q];
+        const double g = omd * omd * P.ome + P.eta;  // physics/material.py:529
         if (P.split == PHAST_SPLIT_AMOR) {
             const double tr2 = exx + eyy;
-            ezz = P.ezz_coef * tr2;
-            tr = tr2 + ezz;
-            trp = fmax(tr, 0.0);
-            const double trm = tr - trp;
-            const double dxx = exx - tr / 3.0, dyy = eyy - tr / 3.0;
-            sxx = g * (P.kappa * trp + 2.0 * P.mu * dxx) + P.kappa * trm;
-            syy = g * (P.kappa * trp + 2.0 * P.mu * dyy) + P.kappa * trm;
-            sxy = g * P.mu * gxy;
+            const double ezz = P.ezz_coef * tr2;
+            const double tr = tr2 + ezz;
+            const double trp = fmax(tr, 0.0), trm = tr - trp;
+            const double t3 = tr / 3.0;
+            const double dxx = exx - t3, dyy = eyy - t3;
+            const double kp = P.kappa * trp, km = P.kappa * trm;
+            sxx[q] = g * (kp + 2.0 * P.mu * dxx) + km;
+            syy[q] = g * (kp + 2.0 * P.mu * dyy) + km;
+            sxy[q] = g * P.mu * gxy;
+            if (PSI) {
+                const double dzz = ezz - t3, exy = gxy / 2.0;
+                psi[q] = 0.5 * P.kappa * trp * trp + P.mu * (dxx * dxx + dyy * dyy + dzz * dzz + 2.0 * exy * exy);
+            }
         } else {
             const double c11 = P.lam + 2.0 * P.mu;
-            sxx = g * (c11 * exx + P.lam * eyy);
-            syy = g * (P.lam * exx + c11 * eyy);
-            sxy = g * (P.mu * gxy);
-        }
-        const double wq = P.w[q];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            fx[a] += (P.gx[q * 4 + a] * sxx + P.gy[q * 4 + a] * sxy) * wq;
-            fy[a] += (P.gy[q * 4 + a] * syy + P.gx[q * 4 + a] * sxy) * wq;
-        }
-        if (PSI) {
-            const double exy = gxy / 2.0;
-            if (P.split == PHAST_SPLIT_AMOR) {
-                const double dxx = exx - tr / 3.0, dyy = eyy - tr / 3.0, dzz = ezz - tr / 3.0;
-                psi[q] = 0.5 * P.kappa * trp * trp +
-                         P.mu * (dxx * dxx + dyy * dyy + dzz * dzz + 2.0 * exy * exy);
-            } else {
-                const double t = exx + eyy;
+            sxx[q] = g * (c11 * exx + P.lam * eyy);
+            syy[q] = g * (P.lam * exx + c11 * eyy);
+            sxy[q] = g * (P.mu * gxy);
+            if (PSI) {
+                const double t = exx + eyy, exy = gxy / 2.0;
                 psi[q] = 0.5 * P.lam * t * t + P.mu * (exx * exx + eyy * eyy + 2.0 * exy * exy);
             }
         }
     }
+    const double sxx0 = sxx[0] + sxx[1], sxx1 = sxx[2] + sxx[3];      // [iy]
+    const double syy0 = syy[0] + syy[2], syy1 = syy[1] + syy[3];      // [ix]
+    const double sxr0 = sxy[0] + sxy[1], sxr1 = sxy[2] + sxy[3];      // [iy]
+    const double sxc0 = sxy[0] + sxy[2], sxc1 = sxy[1] + sxy[3];      // [ix]
+    const double Rxx0 = P.fxa * sxx0 + P.fxb * sxx1, Rxx1 = P.fxb * sxx0 + P.fxa * sxx1;  // [ly]
+    const double Rxr0 = P.fxa * sxr0 + P.fxb * sxr1, Rxr1 = P.fxb * sxr0 + P.fxa * sxr1;  // [ly]
+    const double Ryy0 = P.fya * syy0 + P.fyb * syy1, Ryy1 = P.fyb * syy0 + P.fya * syy1;  // [lx]
+    const double Rxc0 = P.fya * sxc0 + P.fyb * sxc1, Rxc1 = P.fyb * sxc0 + P.fya * sxc1;  // [lx]
+    fx[0] = -Rxx0 - Rxc0;
+    fx[1] = Rxx0 - Rxc1;
+    fx[2] = Rxx1 + Rxc1;
+    fx[3] = -Rxx1 + Rxc0;
+    fy[0] = -Ryy0 - Rxr0;
+    fy[1] = -Ryy1 + Rxr0;
+    fy[2] = Ryy1 + Rxr1;
+    fy[3] = Ryy0 - Rxr1;
 }
 
-// AT2 damage operator of one cell: y_a = Gc l0 sum_q w gradN_a.grad x_q + sum_q N_a rho_q x_q w
-// (damage_solver.py:1797-1838); rho_q = 2 H_q + Gc/l0.
+// AT2 damage operator of one rectangular cell (damage_solver.py:1797-1838):
+// y_a = Gc l0 sum_q w gradN_a.grad x_q + sum_q N_a rho_q x_q w, rho_q = 2 H_q + Gc/l0,
+// sum-factorised as cell_mech.
 __device__ __forceinline__ void cell_damage(const Q4Inst &P, const double x[4], const double rho[4],
                                             double gcl0, double y[4]) {
-    double gdx[4], gdy[4], xq[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        double sx = 0.0, sy = 0.0, sq = 0.0;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            sx += P.gx[q * 4 + a] * x[a];
-            sy += P.gy[q * 4 + a] * x[a];
-            sq += P.Nq[q * 4 + a] * x[a];
-        }
-        gdx[q] = sx;
-        gdy[q] = sy;
-        xq[q] = sq;
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        double lap = 0.0, mass = 0.0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            lap += P.w[q] * (P.gx[q * 4 + a] * gdx[q] + P.gy[q * 4 + a] * gdy[q]);
-            mass += P.Nq[q * 4 + a] * (rho[q] * xq[q] * P.w[q]);
-        }
-        y[a] = gcl0 * lap + mass;
-    }
+    const double al = P.al, be = P.be;
+    const double dxr0 = x[1] - x[0], dxr1 = x[2] - x[3], dyc0 = x[3] - x[0], dyc1 = x[2] - x[1];
+    const double gdx0 = P.xa * dxr0 + P.xb * dxr1, gdx1 = P.xb * dxr0 + P.xa * dxr1;  // [iy]
+    const double gdy0 = P.ya * dyc0 + P.yb * dyc1, gdy1 = P.yb * dyc0 + P.ya * dyc1;  // [ix]
+    // sum over the two Gauss points along the constant direction: factor 2
+    const double Lx0 = 2.0 * (P.fxa * gdx0 + P.fxb * gdx1), Lx1 = 2.0 * (P.fxb * gdx0 + P.fxa * gdx1);  // [ly]
+    const double Ly0 = 2.0 * (P.fya * gdy0 + P.fyb * gdy1), Ly1 = 2.0 * (P.fyb * gdy0 + P.fya * gdy1);  // [lx]
+    const double lap[4] = {-Lx0 - Ly0, Lx0 - Ly1, Lx1 + Ly1, -Lx1 + Ly0};
+    const double r00 = al * x[0] + be * x[1], r01 = be * x[0] + al * x[1];
+    const double r10 = al * x[3] + be * x[2], r11 = be * x[3] + al * x[2];
+    const double t0 = rho[0] * (al * r00 + be * r10), t1 = rho[1] * (al * r01 + be * r11);
+    const double t2 = rho[2] * (be * r00 + al * r10), t3 = rho[3] * (be * r01 + al * r11);
+    // transpose of the interpolation, w folded: rows (ly) then columns (lx)
+    const double b0 = P.wal * t0 + P.wbe * t2, b1 = P.wal * t1 + P.wbe * t3;  // ly = 0, [ix]
+    const double u0 = P.wbe * t0 + P.wal * t2, u1 = P.wbe * t1 + P.wal * t3;  // ly = 1, [ix]
+    y[0] = gcl0 * lap[0] + (al * b0 + be * b1);
+    y[1] = gcl0 * lap[1] + (be * b0 + al * b1);
+    y[2] = gcl0 * lap[2] + (be * u0 + al * u1);
+    y[3] = gcl0 * lap[3] + (al * u0 + be * u1);
 }
 
 // ---------------------------------------------------------------------------------------
-// tile geometry
+// strip geometry
 // ---------------------------------------------------------------------------------------
+//
+// A block of NT threads owns a strip of NT-2 node columns [i0, i0+NT-2) and KY node rows
+// [j0, j0+KY) of one instance.  Thread t holds node slot t <-> column i0-1+t (slots 0 and
+// NT-1 are the x halo).  The block marches up the strip one cell row at a time: thread t
+// (t <= NT-2) evaluates cell column i0-1+t of the current cell row from two node rows
+// staged in LDS, hands the two corners that belong to its right neighbour node through
+// LDS, and completes node (i0-1+t) of the lower row from its left cell (corners c1, c2),
+// its own cell (c0, c3) and the upper-corner sums of the previous cell row kept in
+// registers -- in the reference's element order, with no atomics.  Every node row is read
+// once per strip (plus the 2-node x halo), so the only redundancy is one extra cell row
+// per strip.
 
 struct Tile {
-    int inst, i0, j0, nxn, nyn;
+    int inst, i0, j0, nxn, nyn, jend;
 };
 
-template <int TX>
-__device__ __forceinline__ Tile tile_of_block(const Q4Inst *inst, const int32_t *block_inst) {
-    constexpr int TY = 256 / TX;
+template <int NT>
+__device__ __forceinline__ Tile tile_of(const Q4Inst *inst, const int32_t *block_inst, int strip) {
     Tile t;
-    t.inst = block_inst[blockIdx.x];
+    t.inst = block_inst[strip];
     const Q4Inst &P = inst[t.inst];
-    const int k = blockIdx.x - P.tile_off;
-    t.i0 = (k % P.tiles_x) * TX;
-    t.j0 = (k / P.tiles_x) * TY;
+    const int k = strip - P.tile_off;
+    t.i0 = (k % P.tiles_x) * (NT - 2);
+    t.j0 = (k / P.tiles_x) * P.ky;
     t.nxn = P.nx + 1;
     t.nyn = P.ny + 1;
+    t.jend = min(t.j0 + P.ky, t.nyn);
     return t;
 }
 
@@ -197,64 +221,110 @@ struct MechArgs {
     double *Hnew, *psi;
 };
 
-template <int TX, bool STEP, bool HIST>
-__global__ __launch_bounds__(256) void q4_mech_kernel(MechArgs A) {
-    constexpr int TY = 256 / TX;
-    constexpr int NXH = TX + 2, NYH = TY + 2, NXE = TX + 1, NYE = TY + 1;
-    __shared__ double sux[NXH * NYH], suy[NXH * NYH], sd[NXH * NYH];
-    __shared__ double sf[8][NXE * NYE];
+// LDS of one strip pass: three node rows (u_x, u_y, d) double-buffered, plus the corners
+// c1 (x,y) and c2 (x,y) of each cell of the current row.
+template <int NT>
+constexpr int kMechLds = 10 * NT;
 
-    const Tile T = tile_of_block<TX>(A.inst, A.block_inst);
+template <int NT, bool STEP, bool HIST>
+__device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double *sm) {
+    double(*sux)[NT] = reinterpret_cast<double(*)[NT]>(sm);
+    double(*suy)[NT] = reinterpret_cast<double(*)[NT]>(sm + 2 * NT);
+    double(*sd)[NT] = reinterpret_cast<double(*)[NT]>(sm + 4 * NT);
+    double(*sf)[NT] = reinterpret_cast<double(*)[NT]>(sm + 6 * NT);
+    __syncthreads();  // the LDS may still be read by the previous strip of this block
+
+    const Tile T = tile_of<NT>(A.inst, A.block_inst, strip);
     const Q4Inst &P = A.inst[T.inst];
     const double dt = P.dt, c2 = 0.5 * P.dt * P.dt;
+    const int t = threadIdx.x;
+    const int ie = T.i0 - 1 + t;  // node column of slot t, cell column of thread t
+    const bool col_ok = ie >= 0 && ie < T.nxn;
+    const bool owner = t >= 1 && t <= NT - 2 && ie < T.nxn;  // slot t holds an owned node column
+    const bool cell_col = t <= NT - 2 && ie >= 0 && ie < P.nx;
+    const bool bc = STEP && A.bcm != nullptr;
 
-    for (int s = threadIdx.x; s < NXH * NYH; s += 256) {
-        const int i = T.i0 + s % NXH - 1, j = T.j0 + s / NXH - 1;
-        double ux = 0.0, uy = 0.0, dd = 0.0;
-        if (i >= 0 && i < T.nxn && j >= 0 && j < T.nyn) {
-            const int64_t n = P.node_off + (int64_t)j * T.nxn + i;
-            const double2 uu = A.u[n];
+    // Slot t of the next node row, fetched one row ahead so that its latency overlaps the
+    // current row's cell arithmetic.
+    double2 pu = make_double2(0.0, 0.0), pv = pu, pa = pu;
+    double pd = 0.0;
+    uint32_t pm = 0;
+    auto fetch = [&](int j) {
+        if (col_ok && j >= 0 && j < T.nyn && j <= T.jend) {
+            const int64_t n = P.node_off + (int64_t)j * T.nxn + ie;
+            pu = A.u[n];
+            pd = A.d[n];
             if (STEP) {
-                const double2 vv = A.v[n], aa = A.a[n];
-                ux = (uu.x + dt * vv.x) + c2 * aa.x;  // mechanics_solver.py:160
-                uy = (uu.y + dt * vv.y) + c2 * aa.y;
-                if (A.bcm) {
-                    if (A.bcm[2 * n]) ux = A.bcv[2 * n];
-                    if (A.bcm[2 * n + 1]) uy = A.bcv[2 * n + 1];
-                }
-            } else {
-                ux = uu.x;
-                uy = uu.y;
+                pv = A.v[n];
+                pa = A.a[n];
+                if (bc) pm = (uint32_t)A.bcm[2 * n] | ((uint32_t)A.bcm[2 * n + 1] << 1);
             }
-            dd = A.d[n];
+        } else {
+            pu = pv = pa = make_double2(0.0, 0.0);
+            pd = 0.0;
+            pm = 0;
         }
-        sux[s] = ux;
-        suy[s] = uy;
-        sd[s] = dd;
-    }
-    __syncthreads();
-
-    for (int s = threadIdx.x; s < NXE * NYE; s += 256) {
-        const int li = s % NXE - 1, lj = s / NXE - 1;
-        const int ie = T.i0 + li, je = T.j0 + lj;
-        double fx[4] = {0.0, 0.0, 0.0, 0.0}, fy[4] = {0.0, 0.0, 0.0, 0.0};
-        if (ie >= 0 && ie < P.nx && je >= 0 && je < P.ny) {
-            const int b0 = (lj + 1) * NXH + (li + 1);
-            const int id[4] = {b0, b0 + 1, b0 + 1 + NXH, b0 + NXH};
-            double ux[4], uy[4], dd[4], ps[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                ux[k] = sux[id[k]];
-                uy[k] = suy[id[k]];
-                dd[k] = sd[id[k]];
+    };
+    auto store = [&](int buf, int j) {
+        double ux = pu.x, uy = pu.y;
+        if (STEP) {
+            ux = (pu.x + dt * pv.x) + c2 * pa.x;  // mechanics_solver.py:160
+            uy = (pu.y + dt * pv.y) + c2 * pa.y;
+            if (pm) {
+                const int64_t n = P.node_off + (int64_t)j * T.nxn + ie;
+                if (pm & 1u) ux = A.bcv[2 * n];
+                if (pm & 2u) uy = A.bcv[2 * n + 1];
             }
-            const bool own = HIST && li >= 0 && lj >= 0;
-            if (own) {
+        }
+        sux[buf][t] = ux;
+        suy[buf][t] = uy;
+        sd[buf][t] = pd;
+    };
+
+    fetch(T.j0 - 1);
+    store(0, T.j0 - 1);
+    fetch(T.j0);
+    double2 gv = make_double2(0.0, 0.0), ga = gv, nv = gv, na = gv;  // v, a of rows jr and jr+1
+    uint32_t gm = 0, nm = 0;
+    double accx = 0.0, accy = 0.0;
+    int cur = 0;
+    for (int jr = T.j0 - 1; jr < T.jend; ++jr) {
+        const int up = cur ^ 1;
+        store(up, jr + 1);
+        if (STEP) {
+            nv = pv;
+            na = pa;
+            nm = pm;
+        }
+        // operands consumed later in this row, issued before the next prefetch so that
+        // waiting for them leaves the prefetch in flight
+        const bool gat = owner && jr >= T.j0;
+        const int64_t ng = P.node_off + (int64_t)jr * T.nxn + ie;
+        double2 fe = make_double2(0.0, 0.0);
+        double mi = 0.0;
+        if (STEP && gat) {
+            if (A.fext) fe = A.fext[ng];
+            mi = A.minv[ng];
+        }
+        const bool cell_ok = cell_col && jr >= 0 && jr < P.ny;
+        const bool own_cell = HIST && cell_ok && t >= 1 && jr >= T.j0;
+        const int64_t c = P.cell_off + (int64_t)jr * P.nx + ie;
+        double2 h01 = make_double2(0.0, 0.0), h23 = h01;
+        if (own_cell) {
+            h01 = reinterpret_cast<const double2 *>(A.Hold)[2 * c];
+            h23 = reinterpret_cast<const double2 *>(A.Hold)[2 * c + 1];
+        }
+        fetch(jr + 2);
+        __syncthreads();
+        double fx[4] = {0.0, 0.0, 0.0, 0.0}, fy[4] = {0.0, 0.0, 0.0, 0.0};
+        if (cell_ok) {
+            const double ux[4] = {sux[cur][t], sux[cur][t + 1], sux[up][t + 1], sux[up][t]};
+            const double uy[4] = {suy[cur][t], suy[cur][t + 1], suy[up][t + 1], suy[up][t]};
+            const double dd[4] = {sd[cur][t], sd[cur][t + 1], sd[up][t + 1], sd[up][t]};
+            double ps[4];
+            if (own_cell) {  // cells owned by this strip: lower-left node inside
                 cell_mech<true>(P, ux, uy, dd, fx, fy, ps);
-                const int64_t c = P.cell_off + (int64_t)je * P.nx + ie;
-                const double2 h01 = reinterpret_cast<const double2 *>(A.Hold)[2 * c];
-                const double2 h23 = reinterpret_cast<const double2 *>(A.Hold)[2 * c + 1];
-                double2 n01, n23;  // fmax returns one operand bit-exactly (no NaN inputs)
+                double2 n01, n23;  // max returns one operand bit-exactly (no NaN inputs)
                 n01.x = h01.x >= ps[0] ? h01.x : ps[0];
                 n01.y = h01.y >= ps[1] ? h01.y : ps[1];
                 n23.x = h23.x >= ps[2] ? h23.x : ps[2];
@@ -269,43 +339,44 @@ __global__ __launch_bounds__(256) void q4_mech_kernel(MechArgs A) {
                 cell_mech<false>(P, ux, uy, dd, fx, fy, ps);
             }
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            sf[2 * k][s] = fx[k];
-            sf[2 * k + 1][s] = fy[k];
-        }
-    }
-    __syncthreads();
-
-    {
-        const int s = threadIdx.x;
-        const int li = s % TX, lj = s / TX;
-        const int i = T.i0 + li, j = T.j0 + lj;
-        if (i < T.nxn && j < T.nyn) {
-            // cells (i-1,j-1) corner 2, (i,j-1) corner 3, (i-1,j) corner 1, (i,j) corner 0
-            const int e00 = lj * NXE + li, e10 = e00 + 1, e01 = e00 + NXE, e11 = e01 + 1;
-            const double fx = ((sf[4][e00] + sf[6][e10]) + sf[2][e01]) + sf[0][e11];
-            const double fy = ((sf[5][e00] + sf[7][e10]) + sf[3][e01]) + sf[1][e11];
-            const int64_t n = P.node_off + (int64_t)j * T.nxn + i;
-            if (!STEP) {
-                A.f[n] = make_double2(fx, fy);
-            } else {
-                const double2 vv = A.v[n], aa = A.a[n];
-                const double2 fe = A.fext ? A.fext[n] : make_double2(0.0, 0.0);
-                const double mi = A.minv[n];
-                double ax = mi * (fe.x - fx), ay = mi * (fe.y - fy);  // mechanics_solver.py:194-198
-                double vx = vv.x + 0.5 * dt * (aa.x + ax), vy = vv.y + 0.5 * dt * (aa.y + ay);
-                if (A.bcm) {
-                    if (A.bcm[2 * n]) ax = vx = 0.0;
-                    if (A.bcm[2 * n + 1]) ay = vy = 0.0;
+        sf[0][t] = fx[1];
+        sf[1][t] = fy[1];
+        sf[2][t] = fx[2];
+        sf[3][t] = fy[2];
+        __syncthreads();
+        if (t >= 1 && t <= NT - 2) {
+            if (gat) {
+                // cells (i-1,j-1) c2, (i,j-1) c3 [acc], (i-1,j) c1, (i,j) c0
+                const double fxn = (accx + sf[0][t - 1]) + fx[0];
+                const double fyn = (accy + sf[1][t - 1]) + fy[0];
+                if (!STEP) {
+                    A.f[ng] = make_double2(fxn, fyn);
+                } else {
+                    double ax = mi * (fe.x - fxn), ay = mi * (fe.y - fyn);  // mechanics_solver.py:194-198
+                    double vx = gv.x + 0.5 * dt * (ga.x + ax), vy = gv.y + 0.5 * dt * (ga.y + ay);
+                    if (gm & 1u) ax = vx = 0.0;
+                    if (gm & 2u) ay = vy = 0.0;
+                    A.u1[ng] = make_double2(sux[cur][t], suy[cur][t]);
+                    A.v1[ng] = make_double2(vx, vy);
+                    A.a1[ng] = make_double2(ax, ay);
                 }
-                const int sh = (lj + 1) * NXH + li + 1;
-                A.u1[n] = make_double2(sux[sh], suy[sh]);
-                A.v1[n] = make_double2(vx, vy);
-                A.a1[n] = make_double2(ax, ay);
             }
+            accx = sf[2][t - 1] + fx[3];
+            accy = sf[3][t - 1] + fy[3];
         }
+        if (STEP) {
+            gv = nv;
+            ga = na;
+            gm = nm;
+        }
+        cur = up;
     }
+}
+
+template <int NT, bool STEP, bool HIST>
+__global__ __launch_bounds__(NT) void q4_mech_kernel(MechArgs A) {
+    __shared__ double sm[kMechLds<NT>];
+    mech_strip<NT, STEP, HIST>(A, blockIdx.x, sm);
 }
 
 // Element-parallel strain / psi+ (one thread per cell, nodes read from global memory).
@@ -425,41 +496,88 @@ struct DmgArgs {
     int32_t n_tiles;
 };
 
-template <int TX, int OP>
-__global__ __launch_bounds__(256) void q4_dmg_kernel(DmgArgs A) {
-    constexpr int TY = 256 / TX;
-    constexpr int NXH = TX + 2, NYH = TY + 2, NXE = TX + 1, NYE = TY + 1;
+template <int NT>
+constexpr int kDmgLds = 6 * NT;
+
+// One strip of a damage-operator pass; per-strip partial sums land in part[slot] with
+// slot = strip (and n_tiles + slot, 2 n_tiles + slot for the second and third sums).
+// For the CG passes the gathered field x is the iterate the node-wise update reads (d for
+// the residual passes, p for the A p passes), so the owned node's value is taken from the
+// staged row.
+template <int NT, int OP>
+__device__ __forceinline__ void dmg_strip(const DmgArgs &A, int strip, double *sm, double *red) {
     constexpr bool GATHER_X = (OP == OP_APPLY || OP == OP_CG_INIT || OP == OP_CG_AP || OP == OP_CG_RESID ||
                                OP == OP_PC_INIT || OP == OP_PC_AP || OP == OP_PC_RESID);
+    constexpr bool CELL_H = (OP != OP_MASS);  // per-cell (E,4) operand: H, or the field for E2N
     constexpr int NOUT = (OP == OP_RHSDIAG) ? 2 : 1;
-    __shared__ double sx[GATHER_X ? NXH * NYH : 1];
-    __shared__ double sy[NOUT * 4][NXE * NYE];
-    __shared__ double red[4];
+    double(*sx)[NT] = reinterpret_cast<double(*)[NT]>(sm);       // gathered field rows
+    double(*sy)[NT] = reinterpret_cast<double(*)[NT]>(sm + 2 * NT);  // corners c1, c2 per output
+    __syncthreads();  // the LDS may still be read by the previous strip of this block
 
-    const Tile T = tile_of_block<TX>(A.inst, A.block_inst);
+    const Tile T = tile_of<NT>(A.inst, A.block_inst, strip);
     const Q4Inst &P = A.inst[T.inst];
-    if (OP >= OP_CG_AP && OP != OP_PC_INIT && A.cg[T.inst].done) return;
-    if ((OP == OP_CG_RESID) && !(A.cg[T.inst].limited || A.cg[T.inst].check)) return;
-    if ((OP == OP_PC_RESID) && !A.cg[T.inst].check) return;
+    const int t = threadIdx.x;
+    const int ie = T.i0 - 1 + t;
+    const bool col_ok = ie >= 0 && ie < T.nxn;
+    const bool owner = t >= 1 && t <= NT - 2 && ie < T.nxn;
+    const bool cell_col = t <= NT - 2 && ie >= 0 && ie < P.nx;
+    const double *Hc = (OP == OP_E2N) ? A.x : A.H;
 
-    if (GATHER_X) {
-        for (int s = threadIdx.x; s < NXH * NYH; s += 256) {
-            const int i = T.i0 + s % NXH - 1, j = T.j0 + s / NXH - 1;
-            double xv = 0.0;
-            if (i >= 0 && i < T.nxn && j >= 0 && j < T.nyn) xv = A.x[P.node_off + (int64_t)j * T.nxn + i];
-            sx[s] = xv;
+    // next node row's x (slot t) and next cell row's (E,4) operand, fetched one row ahead
+    double px = 0.0, pgc = 0.0;
+    double2 ph01 = make_double2(0.0, 0.0), ph23 = ph01;
+    auto fetch_x = [&](int j) {
+        if (GATHER_X) px = (col_ok && j >= 0 && j < T.nyn && j <= T.jend) ? A.x[P.node_off + (int64_t)j * T.nxn + ie]
+                                                                          : 0.0;
+    };
+    auto fetch_cell = [&](int jc) {
+        if (CELL_H && cell_col && jc >= 0 && jc < P.ny && jc < T.jend) {
+            const int64_t c = P.cell_off + (int64_t)jc * P.nx + ie;
+            ph01 = reinterpret_cast<const double2 *>(Hc)[2 * c];
+            ph23 = reinterpret_cast<const double2 *>(Hc)[2 * c + 1];
+            if (OP != OP_E2N && A.Gc) pgc = A.Gc[c];
+        } else {
+            ph01 = ph23 = make_double2(0.0, 0.0);
+            pgc = 0.0;
         }
-        __syncthreads();
-    }
+    };
 
-    for (int s = threadIdx.x; s < NXE * NYE; s += 256) {
-        const int li = s % NXE - 1, lj = s / NXE - 1;
-        const int ie = T.i0 + li, je = T.j0 + lj;
+    constexpr double INF = __builtin_huge_val();
+    double p0 = 0.0, p1 = (OP == OP_CG_AP) ? INF : 0.0, p2 = (OP == OP_CG_AP) ? INF : 0.0;
+    double acc0 = 0.0, acc1 = 0.0;
+    int cur = 0;
+    fetch_x(T.j0 - 1);
+    if (GATHER_X) sx[0][t] = px;
+    fetch_x(T.j0);
+    fetch_cell(T.j0 - 1);
+    for (int jr = T.j0 - 1; jr < T.jend; ++jr) {
+        const int up = cur ^ 1;
+        if (GATHER_X) sx[up][t] = px;
+        const double2 h01 = ph01, h23 = ph23;
+        const double gc = pgc;
+        // node operands of the gather, issued before the prefetch
+        const bool gat = owner && jr >= T.j0;
+        const int64_t n = P.node_off + (int64_t)jr * T.nxn + ie;
+        double o_b = 0.0, o_lb = 0.0, o_mi = 0.0, o_d = 0.0, o_M = 0.0;
+        bool o_fx = false, o_act = false;
+        if (gat) {
+            if (OP == OP_E2N) o_M = A.M[n];
+            if (OP == OP_CG_INIT || OP == OP_CG_RESID || OP == OP_PC_INIT || OP == OP_PC_RESID) {
+                o_b = A.b[n];
+                o_fx = A.fixed && A.fixed[n];
+            }
+            if (OP == OP_CG_INIT || OP == OP_CG_RESID || OP == OP_CG_AP) o_lb = A.lb[n];
+            if (OP == OP_CG_INIT || OP == OP_PC_INIT) o_mi = A.minv[n];
+            if (OP == OP_CG_RESID) o_act = A.active[n] != 0;
+            if (OP == OP_CG_AP) o_d = A.d[n];
+        }
+        fetch_x(jr + 2);
+        fetch_cell(jr + 1);
+        __syncthreads();
         double y[NOUT * 4];
 #pragma unroll
         for (int k = 0; k < NOUT * 4; ++k) y[k] = 0.0;
-        if (ie >= 0 && ie < P.nx && je >= 0 && je < P.ny) {
-            const int64_t c = P.cell_off + (int64_t)je * P.nx + ie;
+        if (cell_col && jr >= 0 && jr < P.ny) {
             if (OP == OP_MASS) {
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
@@ -469,9 +587,7 @@ __global__ __launch_bounds__(256) void q4_dmg_kernel(DmgArgs A) {
                     y[a] = m;
                 }
             } else if (OP == OP_E2N) {
-                double fq[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) fq[q] = A.x[c * 4 + q];
+                const double fq[4] = {h01.x, h01.y, h23.x, h23.y};
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
                     double m = 0.0;
@@ -480,15 +596,12 @@ __global__ __launch_bounds__(256) void q4_dmg_kernel(DmgArgs A) {
                     y[a] = m;
                 }
             } else {
-                const double gc = A.Gc ? A.Gc[c] : 0.0;
                 const double gcl0 = A.Gc ? gc * P.l0 : P.Gc_l0;
                 const double ratio = A.Gc ? gc / P.l0 : P.Gc_over_l0;
-                double Hq[4], rho[4];
+                const double Hq[4] = {h01.x, h01.y, h23.x, h23.y};
+                double rho[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    Hq[q] = A.H[c * 4 + q];
-                    rho[q] = 2.0 * Hq[q] + ratio;
-                }
+                for (int q = 0; q < 4; ++q) rho[q] = 2.0 * Hq[q] + ratio;
                 if (OP == OP_RHSDIAG) {
 #pragma unroll
                     for (int a = 0; a < 4; ++a) {
@@ -502,117 +615,116 @@ __global__ __launch_bounds__(256) void q4_dmg_kernel(DmgArgs A) {
                         y[4 + a] = gcl0 * P.dlap[a] + dg;
                     }
                 } else {
-                    const int b0 = (lj + 1) * NXH + (li + 1);
-                    const double xe[4] = {sx[b0], sx[b0 + 1], sx[b0 + 1 + NXH], sx[b0 + NXH]};
+                    const double xe[4] = {sx[cur][t], sx[cur][t + 1], sx[up][t + 1], sx[up][t]};
                     cell_damage(P, xe, rho, gcl0, y);
                 }
             }
         }
-#pragma unroll
-        for (int k = 0; k < NOUT * 4; ++k) sy[k][s] = y[k];
-    }
-    __syncthreads();
-
-    const int s = threadIdx.x;
-    const int li = s % TX, lj = s / TX;
-    const int i = T.i0 + li, j = T.j0 + lj;
-    const bool valid = i < T.nxn && j < T.nyn;
-    const int e00 = lj * NXE + li, e10 = e00 + 1, e01 = e00 + NXE, e11 = e01 + 1;
-    double g0 = 0.0, g1 = 0.0;
-    int64_t n = 0;
-    if (valid) {
-        n = P.node_off + (int64_t)j * T.nxn + i;
-        g0 = ((sy[2][e00] + sy[3][e10]) + sy[1][e01]) + sy[0][e11];
-        if (NOUT == 2) g1 = ((sy[6][e00] + sy[7][e10]) + sy[5][e01]) + sy[4][e11];
-    }
-    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;  // partial sums of this tile
-    constexpr double INF = __builtin_huge_val();
-    if (OP == OP_CG_AP) {
-        p1 = INF;
-        p2 = INF;
-    }
-    if (valid) {
-        if (OP == OP_MASS || OP == OP_APPLY) {
-            A.out[n] = g0;
-        } else if (OP == OP_E2N) {
-            A.out[n] = g0 / (A.M[n] + 1e-30);
-        } else if (OP == OP_RHSDIAG) {
-            A.out[n] = g0;
-            A.out2[n] = g1;
-        } else if (OP == OP_CG_INIT || OP == OP_CG_RESID) {
-            const double dn = A.d[n], lb = A.lb[n];
-            const bool fx = A.fixed && A.fixed[n];
-            const double bn = A.b[n];
-            double r = bn - g0;
-            if (fx) r = 0.0;
-            const bool act = ((dn <= lb + 1e-14) && (r < 0.0)) || ((dn >= 1.0 - 1e-14) && (r > 0.0)) || fx;
-            const double rf = act ? 0.0 : r;
-            A.r[n] = r;
-            if (OP == OP_CG_RESID) {
-                p1 = (act != (A.active[n] != 0)) ? 1.0 : 0.0;
-                p0 = rf * rf;
-            } else {
-                double zz = rf * A.minv[n];
-                if (act) zz = 0.0;
-                A.z[n] = zz;
-                A.p[n] = zz;
-                p0 = r * r;
-                p1 = bn * bn;
-                p2 = rf * zz;
-            }
-            A.active[n] = act ? 1 : 0;
-        } else if (OP == OP_CG_AP) {
-            const double pn = A.x[n];
-            A.Ap[n] = g0;
-            p0 = pn * g0;
-            const double dn = A.d[n];
-            if (pn > 0.0) p1 = (1.0 - dn) / pn;
-            if (pn < 0.0) p2 = (A.lb[n] - dn) / pn;
-        } else if (OP == OP_PC_INIT) {
-            const bool fx = A.fixed && A.fixed[n];
-            const double bn = A.b[n];
-            double r = bn - g0;
-            if (fx) r = 0.0;
-            const double zz = r * (fx ? 0.0 : A.minv[n]);
-            A.r[n] = r;
-            A.z[n] = zz;
-            A.p[n] = zz;
-            p0 = r * r;
-            p1 = bn * bn;
-            p2 = r * zz;
-        } else if (OP == OP_PC_AP) {
-            A.Ap[n] = g0;
-            p0 = A.x[n] * g0;
-        } else if (OP == OP_PC_RESID) {
-            double r = A.b[n] - g0;
-            if (A.fixed && A.fixed[n]) r = 0.0;
-            A.r[n] = r;
+        sy[0][t] = y[1];
+        sy[1][t] = y[2];
+        if (NOUT == 2) {
+            sy[2][t] = y[5];
+            sy[3][t] = y[6];
         }
+        __syncthreads();
+        if (t >= 1 && t <= NT - 2) {
+            if (gat) {
+                const double g0 = (acc0 + sy[0][t - 1]) + y[0];
+                const double g1 = (NOUT == 2) ? (acc1 + sy[2][t - 1]) + y[4] : 0.0;
+                const double xn = GATHER_X ? sx[cur][t] : 0.0;  // x at the owned node
+                if (OP == OP_MASS || OP == OP_APPLY) {
+                    A.out[n] = g0;
+                } else if (OP == OP_E2N) {
+                    A.out[n] = g0 / (o_M + 1e-30);
+                } else if (OP == OP_RHSDIAG) {
+                    A.out[n] = g0;
+                    A.out2[n] = g1;
+                } else if (OP == OP_CG_INIT || OP == OP_CG_RESID) {
+                    const double dn = xn;
+                    double r = o_b - g0;
+                    if (o_fx) r = 0.0;
+                    const bool act =
+                        ((dn <= o_lb + 1e-14) && (r < 0.0)) || ((dn >= 1.0 - 1e-14) && (r > 0.0)) || o_fx;
+                    const double rf = act ? 0.0 : r;
+                    A.r[n] = r;
+                    if (OP == OP_CG_RESID) {
+                        p1 += (act != o_act) ? 1.0 : 0.0;
+                        p0 += rf * rf;
+                    } else {
+                        double zz = rf * o_mi;
+                        if (act) zz = 0.0;
+                        A.z[n] = zz;
+                        A.p[n] = zz;
+                        p0 += r * r;
+                        p1 += o_b * o_b;
+                        p2 += rf * zz;
+                    }
+                    A.active[n] = act ? 1 : 0;
+                } else if (OP == OP_CG_AP) {
+                    const double pn = xn;
+                    A.Ap[n] = g0;
+                    p0 += pn * g0;
+                    if (pn > 0.0) p1 = fmin(p1, (1.0 - o_d) / pn);
+                    if (pn < 0.0) p2 = fmin(p2, (o_lb - o_d) / pn);
+                } else if (OP == OP_PC_INIT) {
+                    double r = o_b - g0;
+                    if (o_fx) r = 0.0;
+                    const double zz = r * (o_fx ? 0.0 : o_mi);
+                    A.r[n] = r;
+                    A.z[n] = zz;
+                    A.p[n] = zz;
+                    p0 += r * r;
+                    p1 += o_b * o_b;
+                    p2 += r * zz;
+                } else if (OP == OP_PC_AP) {
+                    A.Ap[n] = g0;
+                    p0 += xn * g0;
+                } else if (OP == OP_PC_RESID) {
+                    double r = o_b - g0;
+                    if (o_fx) r = 0.0;
+                    A.r[n] = r;
+                }
+            }
+            acc0 = sy[1][t - 1] + y[3];
+            if (NOUT == 2) acc1 = sy[3][t - 1] + y[7];
+        }
+        cur = up;
     }
     if (OP == OP_CG_INIT || OP == OP_CG_RESID || OP == OP_CG_AP || OP == OP_PC_INIT || OP == OP_PC_AP) {
-        const int64_t slot = blockIdx.x;
-        const double s0 = block_sum<256>(p0, red);
+        const int64_t slot = strip;
+        const double s0 = block_sum<NT>(p0, red);
         if (threadIdx.x == 0) A.part[slot] = s0;
         if (OP == OP_CG_AP) {
-            const double s1 = block_min<256>(p1, red);
-            const double s2 = block_min<256>(p2, red);
+            const double s1 = block_min<NT>(p1, red);
+            const double s2 = block_min<NT>(p2, red);
             if (threadIdx.x == 0) {
                 A.part[A.n_tiles + slot] = s1;
                 A.part[2 * A.n_tiles + slot] = s2;
             }
         } else if (OP != OP_PC_AP) {
-            const double s1 = block_sum<256>(p1, red);
+            const double s1 = block_sum<NT>(p1, red);
             if (threadIdx.x == 0) A.part[A.n_tiles + slot] = s1;
             if (OP == OP_CG_INIT || OP == OP_PC_INIT) {
-                const double s2 = block_sum<256>(p2, red);
+                const double s2 = block_sum<NT>(p2, red);
                 if (threadIdx.x == 0) A.part[2 * A.n_tiles + slot] = s2;
             }
         }
     }
 }
 
+template <int NT, int OP>
+__global__ __launch_bounds__(NT) void q4_dmg_kernel(DmgArgs A) {
+    __shared__ double sm[kDmgLds<NT>];
+    __shared__ double red[NT / 64];
+    const int inst = A.block_inst[blockIdx.x];
+    if (OP >= OP_CG_AP && OP != OP_PC_INIT && A.cg[inst].done) return;
+    if ((OP == OP_CG_RESID) && !(A.cg[inst].limited || A.cg[inst].check)) return;
+    if ((OP == OP_PC_RESID) && !A.cg[inst].check) return;
+    dmg_strip<NT, OP>(A, blockIdx.x, sm, red);
+}
+
 // ---------------------------------------------------------------------------------------
-// node-wise CG vector kernels (same tile decomposition so partial slots line up)
+// node-wise CG vector kernels (same strips, so the partial slots line up)
 // ---------------------------------------------------------------------------------------
 
 enum VecOp : int {
@@ -640,84 +752,90 @@ struct VecArgs {
     int32_t n_tiles;
 };
 
-template <int TX, int OP>
-__global__ __launch_bounds__(256) void q4_vec_kernel(VecArgs A) {
-    __shared__ double red[4];
-    const Tile T = tile_of_block<TX>(A.inst, A.block_inst);
+template <int NT, int OP>
+__device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4CG &S, double *red) {
+    const Tile T = tile_of<NT>(A.inst, A.block_inst, strip);
     const Q4Inst &P = A.inst[T.inst];
-    const Q4CG &S = A.cg[T.inst];
-    if (OP != V_SETUP && OP != V_FINISH && OP != V_PC_FINISH && S.done) return;
-    const int li = threadIdx.x % TX, lj = threadIdx.x / TX;
-    const int i = T.i0 + li, j = T.j0 + lj;
-    const bool valid = i < T.nxn && j < T.nyn;
-    const int64_t n = P.node_off + (int64_t)j * T.nxn + i;
+    const int t = threadIdx.x;
+    const int i = T.i0 - 1 + t;
     double p0 = 0.0, p1 = 0.0;
-    if (valid) {
-        if (OP == V_SETUP) {
-            const bool fx = A.fixed && A.fixed[n];
-            const double dp = A.dprev[n];
-            A.d[n] = fx ? A.vals[n] : (A.init ? A.init[n] : dp);
-            A.lb[n] = fx ? A.vals[n] : dp;
-        } else if (OP == V_UPDATE_D) {
-            const double pn = A.p[n];
-            if (S.limited) {
-                double dn = A.d[n] + S.alpha * pn;
+    if (t >= 1 && t <= NT - 2 && i < T.nxn) {
+        for (int j = T.j0; j < T.jend; ++j) {
+            const int64_t n = P.node_off + (int64_t)j * T.nxn + i;
+            if (OP == V_SETUP) {
+                const bool fx = A.fixed && A.fixed[n];
+                const double dp = A.dprev[n];
+                A.d[n] = fx ? A.vals[n] : (A.init ? A.init[n] : dp);
+                A.lb[n] = fx ? A.vals[n] : dp;
+            } else if (OP == V_UPDATE_D) {
+                const double pn = A.p[n];
+                if (S.limited) {
+                    double dn = A.d[n] + S.alpha * pn;
+                    const double lb = A.lb[n];
+                    dn = dn >= lb ? dn : lb;      // torch.maximum(d, lb)
+                    dn = dn <= 1.0 ? dn : 1.0;    // clamp(max=1)
+                    A.d[n] = dn;
+                } else {
+                    A.d[n] = A.d[n] + S.alpha * pn;
+                    A.r[n] = A.r[n] - S.alpha * A.Ap[n];
+                }
+            } else if (OP == V_PRECOND) {
+                const bool act = A.active[n] != 0;
+                const double rf = act ? 0.0 : A.r[n];
+                double zz = rf * A.minv[n];
+                if (act) zz = 0.0;
+                A.z[n] = zz;
+                p0 += rf * zz;
+            } else if (OP == V_UPDATE_P) {
+                const bool act = A.active[n] != 0;
+                double pn = S.restart ? A.z[n] : A.p[n] * S.beta + A.z[n];
+                if (act) pn = 0.0;
+                A.p[n] = pn;
+            } else if (OP == V_FINISH) {
+                double dn = A.d[n];
                 const double lb = A.lb[n];
-                dn = dn >= lb ? dn : lb;      // torch.maximum(d, lb)
-                dn = dn <= 1.0 ? dn : 1.0;    // clamp(max=1)
-                A.d[n] = dn;
-            } else {
-                A.d[n] = A.d[n] + S.alpha * pn;
+                dn = dn >= lb ? dn : lb;
+                dn = dn <= 1.0 ? dn : 1.0;
+                if (A.fixed && A.fixed[n]) dn = A.vals[n];
+                A.d_out[n] = dn;
+                if (A.active_out) A.active_out[n] = A.active[n];
+            } else if (OP == V_PC_UPDATE) {
+                A.d[n] = A.d[n] + S.alpha * A.p[n];
                 A.r[n] = A.r[n] - S.alpha * A.Ap[n];
+            } else if (OP == V_PC_PRECOND) {
+                const double rr = A.r[n];
+                const double zz = rr * ((A.fixed && A.fixed[n]) ? 0.0 : A.minv[n]);
+                A.z[n] = zz;
+                p0 += rr * rr;
+                p1 += rr * zz;
+            } else if (OP == V_PC_UPDATE_P) {
+                A.p[n] = A.p[n] * S.beta + A.z[n];
+            } else if (OP == V_PC_FINISH) {
+                double dn = A.d[n];
+                const double lb = A.lb[n];
+                dn = dn >= lb ? dn : lb;
+                dn = fmin(fmax(dn, 0.0), 1.0);
+                if (A.fixed && A.fixed[n]) dn = A.vals[n];
+                A.d_out[n] = dn;
             }
-        } else if (OP == V_PRECOND) {
-            const bool act = A.active[n] != 0;
-            const double rf = act ? 0.0 : A.r[n];
-            double zz = rf * A.minv[n];
-            if (act) zz = 0.0;
-            A.z[n] = zz;
-            p0 = rf * zz;
-        } else if (OP == V_UPDATE_P) {
-            const bool act = A.active[n] != 0;
-            double pn = S.restart ? A.z[n] : A.p[n] * S.beta + A.z[n];
-            if (act) pn = 0.0;
-            A.p[n] = pn;
-        } else if (OP == V_FINISH) {
-            double dn = A.d[n];
-            const double lb = A.lb[n];
-            dn = dn >= lb ? dn : lb;
-            dn = dn <= 1.0 ? dn : 1.0;
-            if (A.fixed && A.fixed[n]) dn = A.vals[n];
-            A.d_out[n] = dn;
-            if (A.active_out) A.active_out[n] = A.active[n];
-        } else if (OP == V_PC_UPDATE) {
-            A.d[n] = A.d[n] + S.alpha * A.p[n];
-            A.r[n] = A.r[n] - S.alpha * A.Ap[n];
-        } else if (OP == V_PC_PRECOND) {
-            const double rr = A.r[n];
-            const double zz = rr * ((A.fixed && A.fixed[n]) ? 0.0 : A.minv[n]);
-            A.z[n] = zz;
-            p0 = rr * rr;
-            p1 = rr * zz;
-        } else if (OP == V_PC_UPDATE_P) {
-            A.p[n] = A.p[n] * S.beta + A.z[n];
-        } else if (OP == V_PC_FINISH) {
-            double dn = A.d[n];
-            const double lb = A.lb[n];
-            dn = dn >= lb ? dn : lb;
-            dn = fmin(fmax(dn, 0.0), 1.0);
-            if (A.fixed && A.fixed[n]) dn = A.vals[n];
-            A.d_out[n] = dn;
         }
     }
     if (OP == V_PRECOND || OP == V_PC_PRECOND) {
-        const double s0 = block_sum<256>(p0, red);
-        if (threadIdx.x == 0) A.part[blockIdx.x] = s0;
+        const double s0 = block_sum<NT>(p0, red);
+        if (threadIdx.x == 0) A.part[strip] = s0;
         if (OP == V_PC_PRECOND) {
-            const double s1 = block_sum<256>(p1, red);
-            if (threadIdx.x == 0) A.part[A.n_tiles + blockIdx.x] = s1;
+            const double s1 = block_sum<NT>(p1, red);
+            if (threadIdx.x == 0) A.part[A.n_tiles + strip] = s1;
         }
     }
+}
+
+template <int NT, int OP>
+__global__ __launch_bounds__(NT) void q4_vec_kernel(VecArgs A) {
+    __shared__ double red[NT / 64];
+    const Q4CG &S = A.cg[A.block_inst[blockIdx.x]];
+    if (OP != V_SETUP && OP != V_FINISH && OP != V_PC_FINISH && S.done) return;
+    vec_strip<NT, OP>(A, blockIdx.x, S, red);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -726,29 +844,46 @@ __global__ __launch_bounds__(256) void q4_vec_kernel(VecArgs A) {
 
 enum ScalarOp : int { S_INIT = 0, S_A = 1, S_B = 2, S_C = 3, S_PC_INIT = 4, S_PC_A = 5, S_PC_C = 6 };
 
+// Sum of part[off .. off+cnt) in one fixed order whatever the block width NT: 256 virtual
+// lanes, lane k summing part[k], part[k+256], ...; each virtual wave of 64 lanes reduced by
+// butterfly, then the four wave totals in order.  A block of NT threads runs 256/NT
+// virtual passes, so every block width (and every block that repeats the reduction)
+// produces the same bits.  red holds 4 doubles.
+template <int NT>
 __device__ __forceinline__ double reduce_sum_parts(const double *part, int off, int cnt, double *red) {
-    double s = 0.0;
-    for (int k = threadIdx.x; k < cnt; k += 256) s += part[off + k];
-    return block_sum<256>(s, red);
+    constexpr int V = 256 / NT;
+    double w[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        double s = 0.0;
+        for (int k = threadIdx.x + v * NT; k < cnt; k += 256) s += part[off + k];
+        w[v] = wave_sum(s);
+    }
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) red[v * (NT / kWave) + threadIdx.x / kWave] = w[v];
+    }
+    __syncthreads();
+    return ((red[0] + red[1]) + red[2]) + red[3];
 }
+template <int NT>
 __device__ __forceinline__ double reduce_min_parts(const double *part, int off, int cnt, double *red) {
     double s = __builtin_huge_val();
-    for (int k = threadIdx.x; k < cnt; k += 256) s = fmin(s, part[off + k]);
-    return block_min<256>(s, red);
+    for (int k = threadIdx.x; k < cnt; k += NT) s = fmin(s, part[off + k]);
+    return block_min<NT>(s, red);
 }
 
-template <int OP>
-__global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG *cg, const double *part,
-                                                        int32_t n_tiles, int32_t check_every) {
-    __shared__ double red[4];
-    const Q4Inst &P = inst[blockIdx.x];
-    Q4CG &S = cg[blockIdx.x];
-    if (OP != S_INIT && OP != S_PC_INIT && S.done) return;
+// One per-instance scalar step of the CG (thread 0 writes S).  Every thread of the block
+// takes part in the reductions; all return paths are block-uniform.
+template <int OP, int NT>
+__device__ __forceinline__ void scalar_step(const Q4Inst &P, Q4CG &S, const double *part, int32_t n_tiles,
+                                            int32_t check_every, double *red) {
     const int off = P.tile_off, cnt = P.n_tiles;
     if (OP == S_INIT || OP == S_PC_INIT) {
-        const double rr = reduce_sum_parts(part, off, cnt, red);
-        const double bb = reduce_sum_parts(part, n_tiles + off, cnt, red);
-        const double rz = reduce_sum_parts(part, 2 * n_tiles + off, cnt, red);
+        const double rr = reduce_sum_parts<NT>(part, off, cnt, red);
+        const double bb = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
+        const double rz = reduce_sum_parts<NT>(part, 2 * n_tiles + off, cnt, red);
         if (threadIdx.x == 0) {
             S.rr0 = rr;
             S.bb = bb;
@@ -761,11 +896,11 @@ __global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG
             S.limited = S.check = S.restart = 0;
         }
     } else if (OP == S_A || OP == S_PC_A) {
-        const double pAp = reduce_sum_parts(part, off, cnt, red);
+        const double pAp = reduce_sum_parts<NT>(part, off, cnt, red);
         double up = 0.0, dn = 0.0;
         if (OP == S_A) {
-            up = reduce_min_parts(part, n_tiles + off, cnt, red);
-            dn = reduce_min_parts(part, 2 * n_tiles + off, cnt, red);
+            up = reduce_min_parts<NT>(part, n_tiles + off, cnt, red);
+            dn = reduce_min_parts<NT>(part, 2 * n_tiles + off, cnt, red);
         }
         if (threadIdx.x == 0) {
             S.iter += 1;
@@ -794,8 +929,8 @@ __global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG
             if (threadIdx.x == 0) S.restart = 0;
             return;
         }
-        const double rfn = reduce_sum_parts(part, off, cnt, red);
-        const double chg = reduce_sum_parts(part, n_tiles + off, cnt, red);
+        const double rfn = reduce_sum_parts<NT>(part, off, cnt, red);
+        const double chg = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
         if (threadIdx.x == 0) {
             S.rfn = rfn;
             if (rfn < S.tol_sq) {
@@ -812,7 +947,7 @@ __global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG
             S.restart = (S.limited || (S.check && chg > 0.0)) ? 1 : 0;
         }
     } else if (OP == S_C) {
-        const double rz_new = reduce_sum_parts(part, off, cnt, red);
+        const double rz_new = reduce_sum_parts<NT>(part, off, cnt, red);
         if (threadIdx.x == 0) {
             S.beta = S.restart ? 0.0 : rz_new / (S.rz + 1e-30);
             S.rz = rz_new;
@@ -822,8 +957,8 @@ __global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG
             }
         }
     } else if (OP == S_PC_C) {
-        const double rr = reduce_sum_parts(part, off, cnt, red);
-        const double rz_new = reduce_sum_parts(part, n_tiles + off, cnt, red);
+        const double rr = reduce_sum_parts<NT>(part, off, cnt, red);
+        const double rz_new = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
         if (threadIdx.x == 0) {
             S.rfn = rr;
             if (rr < S.tol_sq) {  // damage_solver.py:3496-3506
@@ -846,44 +981,49 @@ __global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG
     }
 }
 
+template <int OP>
+__global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG *cg, const double *part,
+                                                        int32_t n_tiles, int32_t check_every) {
+    __shared__ double red[4];
+    Q4CG &S = cg[blockIdx.x];
+    if (OP != S_INIT && OP != S_PC_INIT && S.done) return;
+    scalar_step<OP, 256>(inst[blockIdx.x], S, part, n_tiles, check_every, red);
+}
+
 // ---------------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------------
 
 static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
-template <int TX>
+template <int NT>
 static int launch_mech(const phast_q4_plan *pl, MechArgs A, bool step, bool hist, hipStream_t st) {
-    dim3 g(pl->n_tiles), b(256);
-    if (!step && !hist) hipLaunchKernelGGL((q4_mech_kernel<TX, false, false>), g, b, 0, st, A);
-    else if (!step && hist) hipLaunchKernelGGL((q4_mech_kernel<TX, false, true>), g, b, 0, st, A);
-    else if (step && !hist) hipLaunchKernelGGL((q4_mech_kernel<TX, true, false>), g, b, 0, st, A);
-    else hipLaunchKernelGGL((q4_mech_kernel<TX, true, true>), g, b, 0, st, A);
+    dim3 g(pl->n_tiles), b(NT);
+    if (!step && !hist) hipLaunchKernelGGL((q4_mech_kernel<NT, false, false>), g, b, 0, st, A);
+    else if (!step && hist) hipLaunchKernelGGL((q4_mech_kernel<NT, false, true>), g, b, 0, st, A);
+    else if (step && !hist) hipLaunchKernelGGL((q4_mech_kernel<NT, true, false>), g, b, 0, st, A);
+    else hipLaunchKernelGGL((q4_mech_kernel<NT, true, true>), g, b, 0, st, A);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
 
 static int mech_dispatch(const phast_q4_plan *pl, MechArgs A, bool step, bool hist, hipStream_t st) {
     switch (pl->tx) {
+        case 256: return launch_mech<256>(pl, A, step, hist, st);
+        case 128: return launch_mech<128>(pl, A, step, hist, st);
         case 64: return launch_mech<64>(pl, A, step, hist, st);
-        case 32: return launch_mech<32>(pl, A, step, hist, st);
-        case 16: return launch_mech<16>(pl, A, step, hist, st);
     }
-    set_error("unsupported tile width %d", pl->tx);
+    set_error("unsupported strip width %d", pl->tx);
     return PHAST_ERR_UNSUPPORTED;
 }
 
-template <int TX, int OP>
-static void launch_dmg_tx(const phast_q4_plan *pl, const DmgArgs &A, hipStream_t st) {
-    hipLaunchKernelGGL((q4_dmg_kernel<TX, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A);
-}
 template <int OP>
 static int launch_dmg(const phast_q4_plan *pl, const DmgArgs &A, hipStream_t st) {
     switch (pl->tx) {
-        case 64: launch_dmg_tx<64, OP>(pl, A, st); break;
-        case 32: launch_dmg_tx<32, OP>(pl, A, st); break;
-        case 16: launch_dmg_tx<16, OP>(pl, A, st); break;
-        default: set_error("unsupported tile width %d", pl->tx); return PHAST_ERR_UNSUPPORTED;
+        case 256: hipLaunchKernelGGL((q4_dmg_kernel<256, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A); break;
+        case 128: hipLaunchKernelGGL((q4_dmg_kernel<128, OP>), dim3(pl->n_tiles), dim3(128), 0, st, A); break;
+        case 64: hipLaunchKernelGGL((q4_dmg_kernel<64, OP>), dim3(pl->n_tiles), dim3(64), 0, st, A); break;
+        default: set_error("unsupported strip width %d", pl->tx); return PHAST_ERR_UNSUPPORTED;
     }
     PHAST_LAUNCHED();
     return PHAST_OK;
@@ -891,10 +1031,10 @@ static int launch_dmg(const phast_q4_plan *pl, const DmgArgs &A, hipStream_t st)
 template <int OP>
 static int launch_vec(const phast_q4_plan *pl, const VecArgs &A, hipStream_t st) {
     switch (pl->tx) {
-        case 64: hipLaunchKernelGGL((q4_vec_kernel<64, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A); break;
-        case 32: hipLaunchKernelGGL((q4_vec_kernel<32, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A); break;
-        case 16: hipLaunchKernelGGL((q4_vec_kernel<16, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A); break;
-        default: set_error("unsupported tile width %d", pl->tx); return PHAST_ERR_UNSUPPORTED;
+        case 256: hipLaunchKernelGGL((q4_vec_kernel<256, OP>), dim3(pl->n_tiles), dim3(256), 0, st, A); break;
+        case 128: hipLaunchKernelGGL((q4_vec_kernel<128, OP>), dim3(pl->n_tiles), dim3(128), 0, st, A); break;
+        case 64: hipLaunchKernelGGL((q4_vec_kernel<64, OP>), dim3(pl->n_tiles), dim3(64), 0, st, A); break;
+        default: set_error("unsupported strip width %d", pl->tx); return PHAST_ERR_UNSUPPORTED;
     }
     PHAST_LAUNCHED();
     return PHAST_OK;
@@ -921,11 +1061,11 @@ extern "C" int phast_q4_plan_create(const phast_q4_desc *desc, int32_t n_inst, i
                                     phast_q4_plan **out) {
     PHAST_REQUIRE(desc && out && n_inst >= 1, "phast_q4_plan_create: null argument or n_inst < 1");
     if (tile_x == 0 && tile_y == 0) {
-        tile_x = 64;
-        tile_y = 4;
+        tile_x = 128;
+        tile_y = 8;
     }
-    PHAST_REQUIRE(tile_x * tile_y == 256 && (tile_x == 64 || tile_x == 32 || tile_x == 16),
-                  "unsupported tile %dx%d (need 64x4, 32x8 or 16x16)", tile_x, tile_y);
+    PHAST_REQUIRE((tile_x == 64 || tile_x == 128 || tile_x == 256) && tile_y >= 1 && tile_y <= 4096,
+                  "unsupported strip %dx%d (width 64, 128 or 256 threads; 1..4096 rows)", tile_x, tile_y);
     auto *pl = new phast_q4_plan();
     pl->n_inst = n_inst;
     pl->tx = tile_x;
@@ -934,7 +1074,7 @@ extern "C" int phast_q4_plan_create(const phast_q4_desc *desc, int32_t n_inst, i
     int64_t node_off = 0, cell_off = 0;
     for (int k = 0; k < n_inst; ++k) {
         const phast_q4_desc &D = desc[k];
-        if (D.nx < 1 || D.ny < 1 || D.split < 0 || D.split > 1) {
+        if (D.nx < 1 || D.ny < 1 || D.split < 0 || D.split > 1 || !(D.hx > 0) || !(D.hy > 0)) {
             delete pl;
             set_error("instance %d: bad nx/ny/split", k);
             return PHAST_ERR_ARG;
@@ -943,8 +1083,9 @@ extern "C" int phast_q4_plan_create(const phast_q4_desc *desc, int32_t n_inst, i
         std::memset(&I, 0, sizeof(I));
         I.nx = D.nx;
         I.ny = D.ny;
-        I.tiles_x = (int32_t)ceil_div(D.nx + 1, tile_x);
+        I.tiles_x = (int32_t)ceil_div(D.nx + 1, tile_x - 2);
         I.tiles_y = (int32_t)ceil_div(D.ny + 1, tile_y);
+        I.ky = tile_y;
         I.node_off = node_off;
         I.cell_off = cell_off;
         I.tile_off = (int32_t)block_inst.size();
@@ -964,6 +1105,24 @@ extern "C" int phast_q4_plan_create(const phast_q4_desc *desc, int32_t n_inst, i
             for (int q = 0; q < 4; ++q)
                 s += D.wdetJ[q] * (D.gradN[q][a][0] * D.gradN[q][a][0] + D.gradN[q][a][1] * D.gradN[q][a][1]);
             I.dlap[a] = s;
+        }
+        {  // rectangle constants (wdetJ is uniform: hx hy / 4)
+            const double a = 1.0 / std::sqrt(3.0);
+            I.al = 0.5 * (1.0 + a);
+            I.be = 0.5 * (1.0 - a);
+            const double ihx = 1.0 / D.hx, ihy = 1.0 / D.hy, w = D.wdetJ[0];
+            I.xa = I.al * ihx;
+            I.xb = I.be * ihx;
+            I.ya = I.al * ihy;
+            I.yb = I.be * ihy;
+            I.fxa = w * I.xa;
+            I.fxb = w * I.xb;
+            I.fya = w * I.ya;
+            I.fyb = w * I.yb;
+            I.wal = w * I.al;
+            I.wbe = w * I.be;
+            I.wq = w;
+            I.ome = 1.0 - D.eta_residual;
         }
         I.lam = D.lam;
         I.mu = D.mu;

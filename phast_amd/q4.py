@@ -137,6 +137,7 @@ class FEMMesh:
         self.quad_N, self.quad_grad_phi_cell, self.quad_wdetJ_cell = _q4_reference_geometry(nodes[elements[0]])
         edges = [np.linalg.norm(nodes[elements[0], :][k] - nodes[elements[0], :][(k + 1) % 4]) for k in range(4)]
         self.h_min = float(min(edges))  # mesh.py:336-345 (all cells congruent)
+        self.hx, self.hy = float(x[1] - x[0]), float(y[1] - y[0])
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.dtype = F64
         self.node_sets = {
@@ -171,13 +172,14 @@ def q4_desc(mesh: FEMMesh, mat: Material, dt: float = 0.0) -> Q4Desc:
     d.plane_stress = int(bool(mat.plane_stress))
     d.split = 0 if mat.energy_split == "amor" else 1
     d.eta_residual, d.Gc, d.l0, d.dt = mat.eta_residual, mat.Gc, mat.l0, float(dt)
+    d.hx, d.hy = mesh.hx, mesh.hy
     return d
 
 
 class Q4Plan:
     """Owner of a ``phast_q4_plan`` handle over one or more packed plates."""
 
-    def __init__(self, descs: list[Q4Desc], tile=(64, 4)):
+    def __init__(self, descs: list[Q4Desc], tile=(128, 8)):
         _lib.require_device()
         arr = (Q4Desc * len(descs))(*descs)
         h = C.c_void_p()
@@ -221,7 +223,7 @@ def _dev_check(t: torch.Tensor, shape, name: str, device) -> torch.Tensor:
 class FEMOperators:
     """Q4 branch of ``core/fem_operators.py:FEMOperators`` on the HIP library."""
 
-    def __init__(self, mesh: FEMMesh, material: Material, ctx=None, tile=(64, 4)):
+    def __init__(self, mesh: FEMMesh, material: Material, ctx=None, tile=(128, 8)):
         self.mesh, self.material = mesh, material
         self.device, self.dtype = mesh.device, F64
         self._tile = tile
@@ -341,12 +343,13 @@ class ExplicitDynamics:
         if bc_mask is None:
             return None, None
         n = self.fem.mesh.n_nodes
-        m = _dev_check(bc_mask.to(torch.uint8) if bc_mask.dtype == torch.bool else bc_mask, (n, 2), "bc_mask",
-                       self.fem.device)
-        if m.dtype != torch.uint8:
-            raise TypeError("bc_mask must be bool or uint8")
+        if not isinstance(bc_mask, torch.Tensor) or bc_mask.dtype not in (torch.bool, torch.uint8):
+            raise TypeError("bc_mask must be a bool or uint8 tensor")
+        if tuple(bc_mask.shape) != (n, 2) or bc_mask.device != self.fem.device:
+            raise ValueError(f"bc_mask must have shape ({n}, 2) on {self.fem.device}")
+        m = bc_mask.to(torch.uint8).contiguous()
         v = self.fem._u(bc_vals, "bc_vals")
-        return m.contiguous(), v
+        return m, v
 
     def _launch(self, u, v, a, d, f_ext, bc_mask, bc_vals, H_old=None, want_psi=False):
         fem = self.fem

@@ -780,6 +780,7 @@ for _name, _args in {
     "phast_hex8_mg_update": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "phast_hex8_mg_apply": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "phast_hex8_mg_levels": [C.c_void_p, C.POINTER(C.c_int32), C.c_void_p],
+    "phast_hex8_mg_weights": [C.c_void_p, C.c_void_p, C.c_void_p],
     "phast_hex8_mech_solve_mg": [C.c_void_p, C.c_void_p] + [C.c_void_p] * 7 +
                                 [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int64, C.c_void_p,
                                  C.POINTER(C.c_double), C.c_void_p],
@@ -794,13 +795,14 @@ class VoxelMultigrid:
 
     Coarse levels are rediscretisations with voxel-averaged moduli ``mean(g) lam``,
     ``mean(g) mu`` (the Galerkin product for constant coefficients); damped Jacobi
-    smoothing (``nu`` sweeps before and after, weight ``omega``), ``coarse_sweeps`` sweeps
+    smoothing (``nu`` sweeps before and after, weight ``omega / lambda_max(D^-1 A)`` per
+    level from a power iteration), ``coarse_sweeps`` sweeps
     on the coarsest level; trilinear prolongation, restriction its transpose.  The
     preconditioned PCG keeps the flag semantics of solid3d/linalg.py:99-220, so the
     converged solution agrees with the Jacobi-PCG one to the solver tolerance."""
 
     def __init__(self, op: PeriodicVoxelPhaseFieldOperator, *, max_levels: int = 6, min_coarse: int = 4, nu: int = 2,
-                 coarse_sweeps: int = 40, omega: float = 0.6):
+                 coarse_sweeps: int = 40, omega: float = 1.2):
         self.op = op
         macro = np.zeros(6, dtype=np.uint8)
         macro[op.load.controlled] = 1
@@ -829,6 +831,11 @@ class VoxelMultigrid:
         d = None if d is None else op._check("d", d, (op.n_red,))
         check(_L.phast_hex8_mg_update(self._mg, ptr(g), ptr(d), stream()), "mg_update")
         self._src = (g, d)
+        lam = (C.c_double * 8)()
+        om = (C.c_double * 8)()
+        check(_L.phast_hex8_mg_weights(self._mg, lam, om))
+        self.lambda_max = list(lam)[:len(self.levels)]
+        self.omega = list(om)[:len(self.levels)]
 
     def apply(self, r: torch.Tensor) -> torch.Tensor:
         r = self.op._check("r", r, (self.op.n_mech,))

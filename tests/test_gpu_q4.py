@@ -15,11 +15,11 @@ RTOL = 1e-10
 
 def rel(a, b):
     a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
-    b = np.asarray(b)
+    b = b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-def setup(nx=37, ny=23, W=3.7, H=2.3, split="amor", plane_stress=False, tile=(64, 4)):
+def setup(nx=37, ny=23, W=3.7, H=2.3, split="amor", plane_stress=False, tile=(128, 8)):
     from phast_amd import q4 as P
     mat_kw = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5, energy_split=split, plane_stress=plane_stress)
     mesh = P.FEMMesh.rectangle(W, H, nx, ny)
@@ -43,13 +43,41 @@ def state(n, E, seed=0):
 T = lambda x, dev: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
 
 
-@pytest.mark.parametrize("tile", [(64, 4), (32, 8), (16, 16)])
+@pytest.mark.parametrize("tile", [(64, 4), (128, 8), (256, 3), (64, 100)])
 @pytest.mark.parametrize("split,ps", [("amor", False), ("amor", True), ("isotropic", False)])
 def test_internal_force(dev, tile, split, ps):
     P, fem, ofem = setup(split=split, plane_stress=ps, tile=tile)
     u, v, a, d, H = state(fem.mesh.n_nodes, fem.mesh.n_elems)
     f = fem.internal_force(T(u, dev), T(d, dev))
     assert rel(f, ofem.internal_force(u, d)) < RTOL
+
+
+@pytest.mark.parametrize("tile", [(64, 7), (128, 16), (256, 1)])
+def test_multi_strip_paths(dev, tile):
+    """Plates wider and taller than one strip: strip seams in x and y, every path."""
+    P, fem, ofem = setup(nx=200, ny=45, W=20.0, H=4.5, tile=tile)
+    n, E = fem.mesh.n_nodes, fem.mesh.n_elems
+    u, v, a, d, H = state(n, E, seed=11)
+    assert rel(fem.internal_force(T(u, dev), T(d, dev)), ofem.internal_force(u, d)) < RTOL
+    # the oracle's node coordinates (multiples of 0.1) carry rounding the plan's constants do not
+    assert rel(fem.M_scalar, ofem.mesh.M_scalar) < RTOL
+    dt = 0.3 * fem.dt_cfl
+    dyn = P.ExplicitDynamics(fem, dt=dt)
+    z = np.zeros((n, 2))
+    u1, v1, a1, Hn, psi = dyn.step_with_history(T(u, dev), T(v, dev), T(a, dev), T(d, dev), T(z, dev), None, None,
+                                                T(H * 1e-4, dev))
+    ou, ov, oa = O.ExplicitDynamics(ofem, dt=dt).step(u, v, a, d, z, None, None)
+    assert rel(u1, ou) < RTOL and rel(v1, ov) < RTOL and rel(a1, oa) < RTOL
+    assert rel(psi, ofem.compute_psi_plus(ou)) < RTOL
+    assert np.array_equal(Hn.cpu().numpy(), np.maximum(H * 1e-4, psi.cpu().numpy()))
+    dmg = P.PhaseFieldDamageSolver(fem)
+    odmg = O.PhaseFieldDamageSolver(ofem)
+    Hd = H * 5e-2
+    rho, ob, ominv = odmg.prepare(Hd, np.zeros(n))
+    b, diag = dmg.damage_rhs_diag(T(Hd, dev))
+    assert rel(b, ob) < RTOL and rel(diag, 1.0 / ominv - 1e-30) < RTOL
+    assert rel(dmg.damage_apply(T(Hd, dev), T(d, dev)), odmg.Ax(d, rho)) < RTOL
+    assert rel(fem.elem_to_node(T(H, dev)), ofem.mesh.elem_to_node(H)) < RTOL
 
 
 def test_mass_strain_psi_history(dev):
@@ -93,7 +121,8 @@ def test_explicit_step_with_bc_and_history(dev):
     assert np.array_equal(Hn.cpu().numpy(), np.maximum(H * 1e-4, psi.cpu().numpy()))
     u2, v2, a2 = dyn.step(T(u, dev), T(v, dev), T(a, dev), T(d, dev), T(f_ext, dev), torch.as_tensor(bcm, device=dev),
                           T(bcv, dev))
-    assert torch.equal(u2, u1) and torch.equal(v2, v1) and torch.equal(a2, a1)
+    # the kernel without the history epilogue may contract FMAs differently: agree to rounding
+    assert rel(u2, u1) < 1e-14 and rel(v2, v1) < 1e-14 and rel(a2, a1) < 1e-14
 
 
 @pytest.mark.parametrize("gc_field", [False, True])
