@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--tiles", type=str, default="", help="fixed hex8 tile tx,ty,kz (skips the autotuner)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-specimen batched run")
+    ap.add_argument("--batched-steps", type=int, default=2000, help="steps of the 64-specimen batched run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 5 device copies of 256 MiB (the reference kernel for HBM counter scaling)")
     return ap.parse_args()
@@ -144,6 +146,8 @@ def main():
     sec = {}
     if not a.no_secondary:
         sec = secondary(rng, dev, a)
+    if not a.no_batched:
+        sec["c3_batched_64_specimens"] = batched_specimens(dev, a.batched_steps)
 
     out = {
         "metric": "element-apply GDOF/s (periodic hex8 voxel mech_action, 64^3 pore cube)",
@@ -220,6 +224,139 @@ def secondary(rng, dev, a):
                                "apply_ms": t * 1e3, "tiles": list(best)}
         del op, x, d
         torch.cuda.empty_cache()
+    return res
+
+
+def lognormal_field(rng, ny, nx, mean, cov, corr):
+    """Lognormal per-cell field of the given mean and coefficient of variation whose log is a
+    Gaussian random field with a Gaussian correlation of ``corr`` cells (white noise filtered
+    in Fourier space)."""
+    import numpy as np
+    z = rng.standard_normal((ny, nx))
+    ky = np.fft.fftfreq(ny)[:, None]
+    kx = np.fft.fftfreq(nx)[None, :]
+    z = np.real(np.fft.ifft2(np.fft.fft2(z) * np.exp(-2.0 * np.pi ** 2 * corr ** 2 * (kx ** 2 + ky ** 2))))
+    z = (z - z.mean()) / max(z.std(), 1e-300)
+    s2 = np.log1p(cov ** 2)
+    return mean * np.exp(np.sqrt(s2) * z - 0.5 * s2)
+
+
+def c3_batch_inputs(n_spec=64, seed=2024):
+    """64 SENT-like plates: nx ~ U{64..512}, ny = nx/2, 0.5 mm cells, lognormal Gc (mean 3e-3,
+    CoV 0.2, correlation 4 cells), a notch of length U[0.2, 0.5] W held at d = 1 along the
+    mid-height node row, traction +-1 MPa on the top/bottom edges."""
+    import numpy as np
+    from phast_amd import q4 as P
+    rng = np.random.default_rng(seed)
+    mat = P.Material(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5)
+    items, gcs, fext, fixed = [], [], [], []
+    for nx in rng.integers(64, 513, n_spec):
+        nx = int(nx)
+        ny = nx // 2
+        h = 0.5
+        mesh = P.FEMMesh.rectangle(h * nx, h * ny, nx, ny)
+        items.append((mesh, mat))
+        gcs.append(lognormal_field(rng, ny, nx, 3e-3, 0.2, 4.0).ravel())
+        f = np.zeros((mesh.n_nodes, 2))
+        edge = np.full(nx + 1, h)
+        edge[[0, -1]] = 0.5 * h
+        f[ny * (nx + 1):, 1] = 1.0 * edge
+        f[:nx + 1, 1] = -1.0 * edge
+        fext.append(f)
+        a_len = rng.uniform(0.2, 0.5) * h * nx
+        ids = (ny // 2) * (nx + 1) + np.arange(int(np.floor(a_len / h)) + 1)
+        fixed.append((ids, 1.0))
+    return items, gcs, fext, fixed
+
+
+def batched_specimens(dev, steps):
+    """c3: specimen-steps/s of the persistent batched step replayed from a captured graph,
+    against the per-kernel batched path and 64 sequential single-specimen runs."""
+    import numpy as np
+    import torch
+    from phast_amd import batch as B
+    from phast_amd import _lib
+    from phast_amd import autotune
+    items, gcs, fext, fixed = c3_batch_inputs()
+    n_spec = len(items)
+    mk = lambda it, g, f, x, tile=(128, 8): B.BatchedExplicit(it, dt_factor=0.4, Gc_fields=g, f_ext=f, fixed=x,
+                                                              bounds_method="projected_cg", tol=1e-5, max_iter=5000,
+                                                              tile=tile)
+    res = {"specimens": n_spec, "total_nodes": int(sum(m.n_nodes for m, _ in items)),
+           "nx_range": [int(min(m.nx for m, _ in items)), int(max(m.nx for m, _ in items))]}
+    # strip shape of the persistent step: tuned on a state 100 steps into the run
+    warm = mk(items, gcs, fext, fixed)
+    warm.step_persistent(100)
+    warm.check_persistent()
+    state = warm.packed_state()
+    del warm
+    best, timings = autotune.tune_batch(lambda t: mk(items, gcs, fext, fixed, t), state=state, steps=10)
+    res["tile"] = list(best)
+    res["autotune_ms_per_step"] = {f"{k[0]},{k[1]}": round(v, 3) for k, v in timings.items()}
+    del state
+    # persistent kernel + graph over the full run
+    bat = mk(items, gcs, fext, fixed, best)
+    res["grid_blocks"] = list(bat.prepare_persistent())
+    chunk = 20
+    g = bat.capture(chunk)
+    reps = max(steps // chunk, 1)
+    bad = torch.zeros((), dtype=torch.int64, device=dev)
+    its = torch.zeros(n_spec, dtype=torch.int64, device=dev)
+    its_max = torch.zeros((), dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.replay()
+        bad += (g.status != _lib.CG_CONVERGED).sum()
+        its += g.iters.sum(0)
+        its_max = torch.maximum(its_max, g.iters.max().to(torch.int64))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n_steps = reps * chunk
+    bat.check_persistent(g.status[-1])
+    res.update({"steps": n_steps, "graph_chunk": chunk, "persistent_graph_wall_s": wall,
+                "persistent_graph_specimen_steps_s": n_spec * n_steps / wall,
+                "unconverged_solves": int(bad.item()),
+                "damage_cg_iters_per_step_mean": float(its.sum().item()) / (n_spec * n_steps),
+                "damage_cg_iters_max": int(its_max.item()),
+                "ms_per_step": wall / n_steps * 1e3,
+                "d_max_end": float(bat.d.max().item())})
+    del g, bat
+    # per-kernel batched path (one launch per kernel over all specimens, host polls the CG)
+    short = max(min(100, steps // 10), 10)
+    bat = mk(items, gcs, fext, fixed)
+    bat.step(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bat.step(short)
+    torch.cuda.synchronize()
+    res["per_kernel_batched_specimen_steps_s"] = n_spec * short / (time.perf_counter() - t0)
+    del bat
+    # 64 sequential single-specimen runs of the per-kernel path, and parity with the batch
+    seq_steps = 25
+    ref = mk(items, gcs, fext, fixed)
+    ref.step_persistent(seq_steps)
+    ref.check_persistent()
+    singles = [mk([items[k]], [gcs[k]], [fext[k]], [fixed[k]]) for k in range(n_spec)]
+    singles[0].step(1)  # warm the kernels, then rebuild the first specimen
+    singles[0] = mk([items[0]], [gcs[0]], [fext[0]], [fixed[0]])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for one in singles:
+        one.step(seq_steps)
+    torch.cuda.synchronize()
+    wall_seq = time.perf_counter() - t0
+    worst = 0.0
+    for k, one in enumerate(singles):
+        for x, y in zip(ref.instance_state(k), one.instance_state(0)):
+            worst = max(worst, float((x - y).abs().max().item()) / max(float(y.abs().max().item()), 1e-300))
+    res.update({"sequential_steps_per_specimen": seq_steps,
+                "sequential_specimen_steps_s": n_spec * seq_steps / wall_seq,
+                "max_rel_diff_batched_vs_sequential": worst})
+    res["speedup_persistent_graph_vs_sequential"] = (res["persistent_graph_specimen_steps_s"]
+                                                     / res["sequential_specimen_steps_s"])
+    del singles, ref
+    torch.cuda.empty_cache()
     return res
 
 

@@ -142,6 +142,31 @@ int phast_q4_damage_solve(phast_q4_plan *plan, int32_t method, const double *H, 
                           double *d_out, uint8_t *active_out, int32_t *iters, int32_t *status,
                           hipStream_t stream);
 
+/* Persistent batched explicit step (no single-call counterpart in the reference: it fuses
+ * the loop body of solvers/staggered_solver.py:1115-1165 with damage_every = 1, i.e.
+ * ExplicitDynamics.step (mechanics_solver.py:204-239), the history max
+ * (staggered_solver.py:804-845) and PhaseFieldDamageSolver.solve (damage_solver.py:
+ * 3377-3775), for every instance of a plan in two launches: the fused explicit-step
+ * kernel, then ONE persistent kernel running every instance's whole damage CG).
+ * phast_q4_batch_prepare sizes the co-resident grid (occupancy query; blocks shared among
+ * instances in proportion to their strips) and must be called once, outside graph
+ * capture; it returns the grid of each damage method in grid_out[2] (optional).
+ * phast_q4_batch_step launches nothing but those two kernels: it allocates nothing, never
+ * synchronises and may be captured into a hipGraph.  Inputs u, v, a, d (= d_prev), H_old;
+ * outputs u1, v1, a1, H_new, psi (optional), d_out; iters/status are DEVICE int32 arrays
+ * [n_inst].  Outputs must not alias inputs.  Results are bitwise those of
+ * phast_q4_explicit_step followed by phast_q4_damage_solve(H_new, d_prev = d).
+ * phast_q4_batch_error reads (and optionally clears) the plan's barrier-timeout flag
+ * (non-zero: a wait among the blocks of an instance ran out and results are invalid). */
+int phast_q4_batch_prepare(phast_q4_plan *plan, int32_t *grid_out);
+int phast_q4_batch_step(phast_q4_plan *plan, int32_t method, const double *u, const double *v, const double *a,
+                        const double *d, const double *f_ext, const double *M_inv, const uint8_t *bc_mask,
+                        const double *bc_vals, const double *H_old, const double *Gc_cell, const uint8_t *fixed,
+                        const double *fixed_vals, double tol, int32_t max_iter, double *u1, double *v1, double *a1,
+                        double *H_new, double *psi, double *d_out, int32_t *iters, int32_t *status,
+                        hipStream_t stream);
+int phast_q4_batch_error(phast_q4_plan *plan, int32_t *err, int32_t clear);
+
 /* =====================================================================================
  * 3D periodic hex8 voxel cells
  * ===================================================================================== */

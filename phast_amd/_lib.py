@@ -62,6 +62,9 @@ _SIGNATURES = {
     "phast_q4_damage_rhs_diag": ([_P] * 6, C.c_int),
     "phast_q4_damage_apply": ([_P] * 6, C.c_int),
     "phast_q4_damage_solve": ([_P, _I32, _P, _P, _P, _P, _P, _P, _D, _I32, _P, _P, _P, _P, _P], C.c_int),
+    "phast_q4_batch_prepare": ([_P, _P], C.c_int),
+    "phast_q4_batch_step": ([_P, _I32] + [_P] * 12 + [_D, _I32] + [_P] * 9, C.c_int),
+    "phast_q4_batch_error": ([_P, C.POINTER(_I32), _I32], C.c_int),
     "phast_hex8_plan_create": ([_I32, _I32, _I32, _D, _D, _D, _P, _I32, _P, _P, _P, _P, _P, _D, _D,
                                 C.POINTER(_P)], C.c_int),
     "phast_hex8_plan_destroy": ([_P], C.c_int),

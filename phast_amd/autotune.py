@@ -7,6 +7,7 @@ import torch
 HEX8_TILES = [(16, 8, 8), (16, 8, 16), (16, 16, 8), (8, 8, 8), (8, 8, 16), (32, 8, 8), (16, 8, 4), (8, 8, 4),
               (16, 8, 2), (32, 8, 2), (16, 16, 2), (8, 8, 2), (32, 8, 4), (16, 16, 4)]
 Q4_TILES = [(128, 8), (128, 32), (256, 16), (256, 64), (64, 32), (256, 128)]
+BATCH_TILES = [(64, 4), (64, 8), (128, 2), (128, 4), (128, 8), (256, 2), (256, 4), (256, 8), (256, 16)]
 
 _cache: dict = {}
 
@@ -50,3 +51,26 @@ def tune_q4(make_fem, u: torch.Tensor, d: torch.Tensor, reps: int = 10, candidat
         timings[t] = _time(lambda: fem.internal_force(u, d), reps)
     best = min(timings, key=timings.get)
     return best, timings, fems[best]
+
+
+def tune_batch(make_batch, state=None, steps: int = 10, candidates=BATCH_TILES):
+    """Pick the strip shape of the persistent batched step: ``make_batch(tile)`` builds a
+    BatchedExplicit with that strip; each candidate runs ``steps`` persistent steps from the
+    packed ``state`` (or its initial state) and the fastest per step wins."""
+    import time
+    timings = {}
+    for t in candidates:
+        b = make_batch(t)
+        if state is not None:
+            b.set_packed_state(state)
+        b.prepare_persistent()
+        b.step_persistent(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.step_persistent(steps)
+        torch.cuda.synchronize()
+        timings[t] = (time.perf_counter() - t0) / steps * 1e3
+        b.check_persistent()
+        del b
+    best = min(timings, key=timings.get)
+    return best, timings

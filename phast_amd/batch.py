@@ -1,10 +1,18 @@
 """Many explicit phase-field plates of uneven sizes advanced together.
 
-All plates of a batch live packed in one set of HBM arrays with per-instance offsets;
-every kernel launch of a step (the fused mechanics/psi+/history kernel and each
-kernel of the projected damage CG) covers the whole batch through the plan's
-block -> (instance, tile) table, and each instance keeps its own CG scalars, so the
-batched result of an instance is bitwise the result of running it alone.
+All plates of a batch live packed in one set of HBM arrays with per-instance offsets.
+Two ways to advance them:
+
+* ``step``: every kernel launch of a step (the fused mechanics/psi+/history kernel and
+  each kernel of the damage CG) covers the whole batch through the plan's
+  block -> (instance, strip) table; the host polls the CG every few iterations.
+* ``step_persistent`` / ``capture``: ONE persistent kernel per step runs the mechanics
+  and the whole damage solve of every instance, with barriers among the blocks of each
+  instance instead of kernel boundaries and no host round trip, so a sequence of steps
+  can be captured into a graph and replayed.
+
+Each instance keeps its own CG scalars and its own partial sums, so the result of an
+instance is bitwise the result of running it alone, on either path.
 """
 from __future__ import annotations
 
@@ -24,7 +32,7 @@ class BatchedExplicit:
     optional list of per-cell Gc arrays (E_k,); ``f_ext`` an optional list of (N_k,2)."""
 
     def __init__(self, items, *, dt_factor: float = 0.5, dts=None, Gc_fields=None, f_ext=None, tol: float = 1e-5,
-                 max_iter: int = 5000, bounds_method: str = "projected_cg", tile=(128, 8)):
+                 max_iter: int = 5000, bounds_method: str = "projected_cg", tile=(128, 8), fixed=None):
         dev = _lib.require_device()
         self.device = dev
         self.meshes = [m for m, _ in items]
@@ -59,6 +67,24 @@ class BatchedExplicit:
             for k, f in enumerate(f_ext):
                 if f is not None:
                     self.f_ext[self._nodes(k)] = torch.as_tensor(np.asarray(f, dtype=np.float64), device=dev)
+        # pf_dirichlet pins (e.g. a notch held at d = 1): fixed[k] = (node ids, values) or None
+        self.fixed = self.fixed_vals = None
+        if fixed is not None and any(f is not None for f in fixed):
+            fm = np.zeros(N, dtype=np.uint8)
+            fv = np.zeros(N, dtype=np.float64)
+            for k, f in enumerate(fixed):
+                if f is None:
+                    continue
+                ids, vals = f
+                ids = np.asarray(ids, dtype=np.int64) + self.plan.node_off[k]
+                fm[ids] = 1
+                fv[ids] = vals
+            self.fixed = torch.as_tensor(fm, device=dev)
+            self.fixed_vals = torch.as_tensor(fv, device=dev)
+            self.d[self.fixed.bool()] = self.fixed_vals[self.fixed.bool()]
+        self.iters_dev = torch.zeros(self.n_inst, dtype=torch.int32, device=dev)
+        self.status_dev = torch.zeros(self.n_inst, dtype=torch.int32, device=dev)
+        self.grid = None
         self.iters = (C.c_int32 * self.plan.n_inst)()
         self.status = (C.c_int32 * self.plan.n_inst)()
         self.last_iters = np.zeros(self.plan.n_inst, dtype=np.int64)
@@ -102,8 +128,87 @@ class BatchedExplicit:
             self.a, self.a1 = self.a1, self.a
             self.H, self.H1 = self.H1, self.H
             check(L.phast_q4_damage_solve(self.plan.handle, self.method, ptr(self.H), ptr(self.Gc), ptr(self.d), None,
-                                          None, None, self.tol, self.max_iter, ptr(self.d1), None, self.iters,
-                                          self.status, s), "batched damage solve")
+                                          ptr(self.fixed), ptr(self.fixed_vals), self.tol, self.max_iter,
+                                          ptr(self.d1), None, self.iters, self.status, s), "batched damage solve")
             self.d, self.d1 = self.d1, self.d
             self.last_iters = np.frombuffer(self.iters, dtype=np.int32).astype(np.int64)
             self.steps += 1
+
+    def packed_state(self):
+        """Clones of the packed (u, v, a, d, H) of the whole batch."""
+        return tuple(x.clone() for x in (self.u, self.v, self.a, self.d, self.H))
+
+    def set_packed_state(self, state) -> None:
+        """Load a packed state taken from a batch of the same instances (any strip shape)."""
+        for dst, src in zip((self.u, self.v, self.a, self.d, self.H), state):
+            dst.copy_(src)
+
+    # ------------------------------------------------------------------ persistent path
+    def prepare_persistent(self):
+        """Size the co-resident grid of the persistent step kernel (once, outside capture)."""
+        g = (C.c_int32 * 2)()
+        check(_lib.lib().phast_q4_batch_prepare(self.plan.handle, g), "batch prepare")
+        self.grid = tuple(g)
+        return self.grid
+
+    def _launch_persistent(self, iters: torch.Tensor, status: torch.Tensor) -> None:
+        check(_lib.lib().phast_q4_batch_step(
+            self.plan.handle, self.method, ptr(self.u), ptr(self.v), ptr(self.a), ptr(self.d), ptr(self.f_ext),
+            ptr(self.Minv), None, None, ptr(self.H), ptr(self.Gc), ptr(self.fixed), ptr(self.fixed_vals), self.tol,
+            self.max_iter, ptr(self.u1), ptr(self.v1), ptr(self.a1), ptr(self.H1), ptr(self.psi), ptr(self.d1),
+            ptr(iters), ptr(status), stream()), "persistent batched step")
+        self.u, self.u1 = self.u1, self.u
+        self.v, self.v1 = self.v1, self.v
+        self.a, self.a1 = self.a1, self.a
+        self.H, self.H1 = self.H1, self.H
+        self.d, self.d1 = self.d1, self.d
+
+    def step_persistent(self, n_steps: int = 1) -> None:
+        """Advance every instance ``n_steps`` explicit steps, one kernel launch per step."""
+        if self.grid is None:
+            self.prepare_persistent()
+        for _ in range(n_steps):
+            self._launch_persistent(self.iters_dev, self.status_dev)
+            self.steps += 1
+
+    def capture(self, n_steps: int = 2) -> "BatchGraph":
+        """Capture ``n_steps`` (even, so the ping-pong buffers come back) persistent steps into
+        a graph; each replay advances the batch by ``n_steps``."""
+        if n_steps <= 0 or n_steps % 2:
+            raise ValueError("n_steps must be a positive even number")
+        if self.grid is None:
+            self.prepare_persistent()
+        iters = torch.zeros((n_steps, self.n_inst), dtype=torch.int32, device=self.device)
+        status = torch.zeros_like(iters)
+        graph = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph):
+            for k in range(n_steps):
+                self._launch_persistent(iters[k], status[k])
+        return BatchGraph(self, graph, iters, status, n_steps)
+
+    def check_persistent(self, status: torch.Tensor | None = None) -> np.ndarray:
+        """Raise if a barrier wait ran out or an instance's damage solve did not converge;
+        return the device status array as numpy."""
+        err = C.c_int32()
+        check(_lib.lib().phast_q4_batch_error(self.plan.handle, C.byref(err), 1), "batch error flag")
+        if err.value:
+            raise _lib.PhastLibraryError("persistent batched step: a barrier wait timed out (results invalid)")
+        st = (self.status_dev if status is None else status).cpu().numpy()
+        if np.any(st != _lib.CG_CONVERGED):
+            bad = sorted(set(int(v) for v in np.unique(st)) - {_lib.CG_CONVERGED})
+            raise RuntimeError("damage solve did not converge on every instance: status "
+                               + ", ".join(_lib.CG_STATUS_NAMES.get(b, str(b)) for b in bad))
+        return st
+
+
+class BatchGraph:
+    """A captured sequence of persistent batched steps."""
+
+    def __init__(self, batch: BatchedExplicit, graph, iters, status, n_steps: int):
+        self.batch, self.graph, self.iters, self.status, self.n_steps = batch, graph, iters, status, n_steps
+
+    def replay(self, times: int = 1) -> None:
+        for _ in range(times):
+            self.graph.replay()
+            self.batch.steps += self.n_steps

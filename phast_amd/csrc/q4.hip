@@ -61,6 +61,12 @@ struct phast_q4_plan {
     phast::Q4CG *cg = nullptr;
     phast::Q4CG *h_cg = nullptr;  // pinned mirror
     int32_t last_iters = 0;
+    // persistent batched step (set up by phast_q4_batch_prepare)
+    int32_t b_grid[2] = {0, 0};  // grid per damage method
+    int32_t *b_pb_inst[2] = {nullptr, nullptr}, *b_pb_rank[2] = {nullptr, nullptr}, *b_nblk[2] = {nullptr, nullptr};
+    uint32_t *b_bar = nullptr;
+    int32_t *b_err = nullptr;
+    uint64_t b_spin_ticks = 0;
 };
 
 namespace phast {
@@ -752,71 +758,117 @@ struct VecArgs {
     int32_t n_tiles;
 };
 
+// Node-wise pass over the owned nodes of one strip.  Rows are taken U at a time: all the
+// operands of the U rows are loaded before any of them is used, so each thread keeps U
+// rows of loads in flight; partial sums still accumulate row by row.
 template <int NT, int OP>
 __device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4CG &S, double *red) {
+    constexpr int U = 4;
     const Tile T = tile_of<NT>(A.inst, A.block_inst, strip);
     const Q4Inst &P = A.inst[T.inst];
     const int t = threadIdx.x;
     const int i = T.i0 - 1 + t;
     double p0 = 0.0, p1 = 0.0;
     if (t >= 1 && t <= NT - 2 && i < T.nxn) {
-        for (int j = T.j0; j < T.jend; ++j) {
-            const int64_t n = P.node_off + (int64_t)j * T.nxn + i;
-            if (OP == V_SETUP) {
-                const bool fx = A.fixed && A.fixed[n];
-                const double dp = A.dprev[n];
-                A.d[n] = fx ? A.vals[n] : (A.init ? A.init[n] : dp);
-                A.lb[n] = fx ? A.vals[n] : dp;
-            } else if (OP == V_UPDATE_D) {
-                const double pn = A.p[n];
-                if (S.limited) {
-                    double dn = A.d[n] + S.alpha * pn;
-                    const double lb = A.lb[n];
-                    dn = dn >= lb ? dn : lb;      // torch.maximum(d, lb)
-                    dn = dn <= 1.0 ? dn : 1.0;    // clamp(max=1)
-                    A.d[n] = dn;
-                } else {
-                    A.d[n] = A.d[n] + S.alpha * pn;
-                    A.r[n] = A.r[n] - S.alpha * A.Ap[n];
+        for (int j0 = T.j0; j0 < T.jend; j0 += U) {
+            int64_t n[U];
+            bool ok[U];
+            double x0[U], x1[U], x2[U], x3[U];
+            bool m0[U], m1[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                ok[k] = j0 + k < T.jend;
+                n[k] = P.node_off + (int64_t)(j0 + k) * T.nxn + i;
+                x0[k] = x1[k] = x2[k] = x3[k] = 0.0;
+                m0[k] = m1[k] = false;
+                if (!ok[k]) continue;
+                const int64_t q = n[k];
+                if (OP == V_SETUP) {
+                    m0[k] = A.fixed && A.fixed[q];
+                    x0[k] = A.dprev[q];
+                    if (A.fixed) x1[k] = A.vals[q];
+                    if (A.init) x2[k] = A.init[q];
+                } else if (OP == V_UPDATE_D || OP == V_PC_UPDATE) {
+                    x0[k] = A.p[q];
+                    x1[k] = A.d[q];
+                    if (OP == V_UPDATE_D && S.limited) {
+                        x2[k] = A.lb[q];
+                    } else {
+                        x2[k] = A.r[q];
+                        x3[k] = A.Ap[q];
+                    }
+                } else if (OP == V_PRECOND) {
+                    m0[k] = A.active[q] != 0;
+                    x0[k] = A.r[q];
+                    x1[k] = A.minv[q];
+                } else if (OP == V_UPDATE_P || OP == V_PC_UPDATE_P) {
+                    if (OP == V_UPDATE_P) m0[k] = A.active[q] != 0;
+                    x0[k] = A.z[q];
+                    x1[k] = A.p[q];
+                } else if (OP == V_FINISH || OP == V_PC_FINISH) {
+                    x0[k] = A.d[q];
+                    x1[k] = A.lb[q];
+                    m0[k] = A.fixed && A.fixed[q];
+                    if (A.fixed) x2[k] = A.vals[q];
+                    if (OP == V_FINISH && A.active_out) m1[k] = A.active[q] != 0;
+                } else if (OP == V_PC_PRECOND) {
+                    x0[k] = A.r[q];
+                    m0[k] = A.fixed && A.fixed[q];
+                    x1[k] = A.minv[q];
                 }
-            } else if (OP == V_PRECOND) {
-                const bool act = A.active[n] != 0;
-                const double rf = act ? 0.0 : A.r[n];
-                double zz = rf * A.minv[n];
-                if (act) zz = 0.0;
-                A.z[n] = zz;
-                p0 += rf * zz;
-            } else if (OP == V_UPDATE_P) {
-                const bool act = A.active[n] != 0;
-                double pn = S.restart ? A.z[n] : A.p[n] * S.beta + A.z[n];
-                if (act) pn = 0.0;
-                A.p[n] = pn;
-            } else if (OP == V_FINISH) {
-                double dn = A.d[n];
-                const double lb = A.lb[n];
-                dn = dn >= lb ? dn : lb;
-                dn = dn <= 1.0 ? dn : 1.0;
-                if (A.fixed && A.fixed[n]) dn = A.vals[n];
-                A.d_out[n] = dn;
-                if (A.active_out) A.active_out[n] = A.active[n];
-            } else if (OP == V_PC_UPDATE) {
-                A.d[n] = A.d[n] + S.alpha * A.p[n];
-                A.r[n] = A.r[n] - S.alpha * A.Ap[n];
-            } else if (OP == V_PC_PRECOND) {
-                const double rr = A.r[n];
-                const double zz = rr * ((A.fixed && A.fixed[n]) ? 0.0 : A.minv[n]);
-                A.z[n] = zz;
-                p0 += rr * rr;
-                p1 += rr * zz;
-            } else if (OP == V_PC_UPDATE_P) {
-                A.p[n] = A.p[n] * S.beta + A.z[n];
-            } else if (OP == V_PC_FINISH) {
-                double dn = A.d[n];
-                const double lb = A.lb[n];
-                dn = dn >= lb ? dn : lb;
-                dn = fmin(fmax(dn, 0.0), 1.0);
-                if (A.fixed && A.fixed[n]) dn = A.vals[n];
-                A.d_out[n] = dn;
+            }
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                if (!ok[k]) continue;
+                const int64_t q = n[k];
+                if (OP == V_SETUP) {
+                    A.d[q] = m0[k] ? x1[k] : (A.init ? x2[k] : x0[k]);
+                    A.lb[q] = m0[k] ? x1[k] : x0[k];
+                } else if (OP == V_UPDATE_D) {
+                    if (S.limited) {
+                        double dn = x1[k] + S.alpha * x0[k];
+                        dn = dn >= x2[k] ? dn : x2[k];  // torch.maximum(d, lb)
+                        dn = dn <= 1.0 ? dn : 1.0;      // clamp(max=1)
+                        A.d[q] = dn;
+                    } else {
+                        A.d[q] = x1[k] + S.alpha * x0[k];
+                        A.r[q] = x2[k] - S.alpha * x3[k];
+                    }
+                } else if (OP == V_PRECOND) {
+                    const double rf = m0[k] ? 0.0 : x0[k];
+                    double zz = rf * x1[k];
+                    if (m0[k]) zz = 0.0;
+                    A.z[q] = zz;
+                    p0 += rf * zz;
+                } else if (OP == V_UPDATE_P) {
+                    double pn = S.restart ? x0[k] : x1[k] * S.beta + x0[k];
+                    if (m0[k]) pn = 0.0;
+                    A.p[q] = pn;
+                } else if (OP == V_FINISH) {
+                    double dn = x0[k];
+                    dn = dn >= x1[k] ? dn : x1[k];
+                    dn = dn <= 1.0 ? dn : 1.0;
+                    if (m0[k]) dn = x2[k];
+                    A.d_out[q] = dn;
+                    if (A.active_out) A.active_out[q] = m1[k] ? 1 : 0;
+                } else if (OP == V_PC_UPDATE) {
+                    A.d[q] = x1[k] + S.alpha * x0[k];
+                    A.r[q] = x2[k] - S.alpha * x3[k];
+                } else if (OP == V_PC_PRECOND) {
+                    const double rr = x0[k];
+                    const double zz = rr * (m0[k] ? 0.0 : x1[k]);
+                    A.z[q] = zz;
+                    p0 += rr * rr;
+                    p1 += rr * zz;
+                } else if (OP == V_PC_UPDATE_P) {
+                    A.p[q] = x1[k] * S.beta + x0[k];
+                } else if (OP == V_PC_FINISH) {
+                    double dn = x0[k];
+                    dn = dn >= x1[k] ? dn : x1[k];
+                    dn = fmin(fmax(dn, 0.0), 1.0);
+                    if (m0[k]) dn = x2[k];
+                    A.d_out[q] = dn;
+                }
             }
         }
     }
@@ -988,6 +1040,179 @@ __global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG
     Q4CG &S = cg[blockIdx.x];
     if (OP != S_INIT && OP != S_PC_INIT && S.done) return;
     scalar_step<OP, 256>(inst[blockIdx.x], S, part, n_tiles, check_every, red);
+}
+
+// ---------------------------------------------------------------------------------------
+// persistent batched damage solve: every instance's whole CG in one launch
+// ---------------------------------------------------------------------------------------
+//
+// A batched step is two launches: the fused explicit-step kernel over all instances, then
+// this persistent kernel.  Its grid is one wave of co-resident blocks (sized from the
+// occupancy query).  Instance k
+// owns nblk[k] of them; block rank r of instance k runs strips r, r + nblk[k], ... of that
+// instance, so the strip passes, per-strip partial sums and per-instance scalar steps are
+// exactly those of the separate kernels, and the result is bitwise the same.  Kernel
+// boundaries become barriers among the blocks of one instance (arrival counter in device
+// memory, agent-scope fences); each block repeats the instance's scalar CG step itself on
+// an LDS copy of the state, from the same partials in the same order.  Every wait is
+// bounded by a wall-clock limit: a wait that runs out raises the error flag and falls
+// through, so the launch always ends.
+
+struct BatchArgs {
+    MechArgs M;
+    DmgArgs R;   // b, diag
+    DmgArgs Dx;  // x = d (initial and true residuals)
+    DmgArgs Dp;  // x = p (A p)
+    VecArgs V;
+    const int32_t *pb_inst, *pb_rank, *nblk;
+    uint32_t *bar;  // [2 * n_inst]: arrivals, departures
+    int32_t *err;
+    int32_t *iters, *status;
+    double *minv;
+    double tol;
+    int32_t max_iter, method;
+    uint64_t spin_ticks;
+};
+
+__device__ __forceinline__ void inst_barrier(uint32_t *arrive, uint32_t target, int32_t *err, uint64_t spin_ticks) {
+    // Every wave waits for its own stores and writes the L2 back (a fence by one thread
+    // would only wait for that wave's stores), then one thread arrives and spins.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(arrive, 1u);
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            const uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(2);
+                if (wall_clock64() - t0 > spin_ticks) {
+                    atomicOr(reinterpret_cast<uint32_t *>(err), 1u);
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // drop stale L1/L2 lines of this CU/XCD
+    }
+    __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ void minv_strip(const Q4Inst *inst, const int32_t *block_inst, int strip, double *minv) {
+    const Tile T = tile_of<NT>(inst, block_inst, strip);
+    const Q4Inst &P = inst[T.inst];
+    const int t = threadIdx.x, i = T.i0 - 1 + t;
+    if (t >= 1 && t <= NT - 2 && i < T.nxn) {
+        for (int j = T.j0; j < T.jend; ++j) {
+            const int64_t n = P.node_off + (int64_t)j * T.nxn + i;
+            minv[n] = 1.0 / (minv[n] + 1e-30);  // damage_solver.py:2299
+        }
+    }
+}
+
+template <int NT, int METHOD>
+__global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
+    __shared__ double sm[kDmgLds<NT>];
+    __shared__ double red[4];
+    __shared__ Q4CG S;
+    const int k = A.pb_inst[blockIdx.x], rank = A.pb_rank[blockIdx.x], nb = A.nblk[k];
+    const Q4Inst &P = A.M.inst[k];
+    const int n_tiles = A.V.n_tiles;
+    const int check_every = METHOD == 0 ? 10 : 50;
+    uint32_t *arrive = A.bar + 2 * k;
+    uint32_t nbar = 0;
+    auto barrier = [&]() {
+        ++nbar;
+        inst_barrier(arrive, nbar * (uint32_t)nb, A.err, A.spin_ticks);
+    };
+#define PHAST_STRIPS for (int s_ = rank; s_ < P.n_tiles; s_ += nb)
+    const int base = P.tile_off;
+
+    // b, M_inv, initial iterate and bounds (the explicit step ran in the preceding launch)
+    PHAST_STRIPS {
+        dmg_strip<NT, OP_RHSDIAG>(A.R, base + s_, sm, red);
+        minv_strip<NT>(A.V.inst, A.V.block_inst, base + s_, A.minv);
+        vec_strip<NT, V_SETUP>(A.V, base + s_, S, red);
+    }
+    barrier();
+    if (threadIdx.x == 0) {
+        S.tol = A.tol;
+        S.max_iter = A.max_iter;
+    }
+    VecArgs V3 = A.V;  // node-wise partials go to their own slots (no barrier before them)
+    V3.part = A.V.part + (METHOD == 0 ? 3 : 2) * (int64_t)n_tiles;
+    if (METHOD == 0) {
+        PHAST_STRIPS dmg_strip<NT, OP_CG_INIT>(A.Dx, base + s_, sm, red);
+        barrier();
+        scalar_step<S_INIT, NT>(P, S, A.V.part, n_tiles, check_every, red);
+        barrier();
+        while (!S.done) {
+            PHAST_STRIPS dmg_strip<NT, OP_CG_AP>(A.Dp, base + s_, sm, red);
+            barrier();
+            scalar_step<S_A, NT>(P, S, A.V.part, n_tiles, check_every, red);
+            __syncthreads();
+            if (S.done) break;
+            PHAST_STRIPS vec_strip<NT, V_UPDATE_D>(A.V, base + s_, S, red);
+            if (S.limited || S.check) {
+                barrier();
+                PHAST_STRIPS dmg_strip<NT, OP_CG_RESID>(A.Dx, base + s_, sm, red);
+                barrier();
+                scalar_step<S_B, NT>(P, S, A.V.part, n_tiles, check_every, red);
+                __syncthreads();
+                if (S.done) break;
+            } else {
+                __syncthreads();
+                if (threadIdx.x == 0) S.restart = 0;
+                __syncthreads();
+            }
+            PHAST_STRIPS vec_strip<NT, V_PRECOND>(V3, base + s_, S, red);
+            barrier();
+            scalar_step<S_C, NT>(P, S, V3.part, n_tiles, check_every, red);
+            __syncthreads();
+            if (S.done) break;
+            PHAST_STRIPS vec_strip<NT, V_UPDATE_P>(A.V, base + s_, S, red);
+            barrier();
+        }
+        PHAST_STRIPS vec_strip<NT, V_FINISH>(A.V, base + s_, S, red);
+    } else {
+        PHAST_STRIPS dmg_strip<NT, OP_PC_INIT>(A.Dx, base + s_, sm, red);
+        barrier();
+        scalar_step<S_PC_INIT, NT>(P, S, A.V.part, n_tiles, check_every, red);
+        barrier();
+        while (!S.done) {
+            PHAST_STRIPS dmg_strip<NT, OP_PC_AP>(A.Dp, base + s_, sm, red);
+            barrier();
+            scalar_step<S_PC_A, NT>(P, S, A.V.part, n_tiles, check_every, red);
+            __syncthreads();
+            if (S.done) break;
+            PHAST_STRIPS vec_strip<NT, V_PC_UPDATE>(A.V, base + s_, S, red);
+            if (S.check) {
+                barrier();
+                PHAST_STRIPS dmg_strip<NT, OP_PC_RESID>(A.Dx, base + s_, sm, red);
+            }
+            PHAST_STRIPS vec_strip<NT, V_PC_PRECOND>(V3, base + s_, S, red);
+            barrier();
+            scalar_step<S_PC_C, NT>(P, S, V3.part, n_tiles, check_every, red);
+            __syncthreads();
+            if (S.done) break;
+            PHAST_STRIPS vec_strip<NT, V_PC_UPDATE_P>(A.V, base + s_, S, red);
+            barrier();
+        }
+        PHAST_STRIPS vec_strip<NT, V_PC_FINISH>(A.V, base + s_, S, red);
+    }
+#undef PHAST_STRIPS
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (rank == 0) {
+            A.iters[k] = S.iter;
+            A.status[k] = S.status;
+        }
+        __threadfence();
+        if (atomicAdd(arrive + 1, 1u) == (uint32_t)nb - 1) {  // last block of the instance out
+            atomicExch(arrive, 0u);
+            atomicExch(arrive + 1, 0u);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1170,6 +1395,13 @@ extern "C" int phast_q4_plan_destroy(phast_q4_plan *pl) {
     (void)hipFree(pl->active);
     (void)hipFree(pl->cg);
     if (pl->h_cg) (void)hipHostFree(pl->h_cg);
+    for (int m = 0; m < 2; ++m) {
+        (void)hipFree(pl->b_pb_inst[m]);
+        (void)hipFree(pl->b_pb_rank[m]);
+        (void)hipFree(pl->b_nblk[m]);
+    }
+    (void)hipFree(pl->b_bar);
+    (void)hipFree(pl->b_err);
     delete pl;
     return PHAST_OK;
 }
@@ -1485,5 +1717,220 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
         if (status) status[k] = stt;
     }
     pl->last_iters = it;
+    return PHAST_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// persistent batched step (host side)
+// ---------------------------------------------------------------------------------------
+
+namespace phast {
+
+template <int NT, int M>
+static int batch_occupancy(int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, q4_batch_kernel<NT, M>, NT, 0) == hipSuccess
+               ? PHAST_OK
+               : PHAST_ERR_HIP;
+}
+
+static int batch_occupancy_of(int nt, int m, int *bpc) {
+    switch (nt * 2 + m) {
+        case 256 * 2 + 0: return batch_occupancy<256, 0>(bpc);
+        case 256 * 2 + 1: return batch_occupancy<256, 1>(bpc);
+        case 128 * 2 + 0: return batch_occupancy<128, 0>(bpc);
+        case 128 * 2 + 1: return batch_occupancy<128, 1>(bpc);
+        case 64 * 2 + 0: return batch_occupancy<64, 0>(bpc);
+        case 64 * 2 + 1: return batch_occupancy<64, 1>(bpc);
+    }
+    return PHAST_ERR_UNSUPPORTED;
+}
+
+template <int NT>
+static void batch_launch(const phast_q4_plan *pl, int m, const BatchArgs &A, hipStream_t st) {
+    if (m == 0) hipLaunchKernelGGL((q4_batch_kernel<NT, 0>), dim3(pl->b_grid[0]), dim3(NT), 0, st, A);
+    else hipLaunchKernelGGL((q4_batch_kernel<NT, 1>), dim3(pl->b_grid[1]), dim3(NT), 0, st, A);
+}
+
+}  // namespace phast
+
+using namespace phast;
+
+extern "C" int phast_q4_batch_prepare(phast_q4_plan *pl, int32_t *grid_out) {
+    PHAST_REQUIRE(pl, "phast_q4_batch_prepare: null plan");
+    PHAST_TRY(ensure_work(pl));
+    int dev = 0, cus = 0, rate_khz = 0;
+    PHAST_HIP(hipGetDevice(&dev));
+    PHAST_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    PHAST_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+    pl->b_spin_ticks = (uint64_t)(rate_khz > 0 ? rate_khz : 100000) * 1000ull * 2ull;  // 2 s per wait
+    for (int m = 0; m < 2; ++m) {
+        int bpc = 0;
+        const int rc = batch_occupancy_of(pl->tx, m, &bpc);
+        if (rc != PHAST_OK) {
+            set_error("occupancy query failed for the batched step kernel (strip width %d)", pl->tx);
+            return rc;
+        }
+        const int64_t gmax = (int64_t)bpc * cus;
+        PHAST_REQUIRE(gmax >= pl->n_inst, "%d instances exceed the %lld co-resident blocks of the batched step",
+                      pl->n_inst, (long long)gmax);
+        // blocks per instance in proportion to its strips, at least one, at most one per strip
+        std::vector<int32_t> nb(pl->n_inst);
+        int64_t total = 0;
+        for (int k = 0; k < pl->n_inst; ++k) {
+            const int64_t t = pl->h_inst[k].n_tiles;
+            nb[k] = (int32_t)std::max<int64_t>(1, std::min<int64_t>(t, gmax * t / pl->n_tiles));
+            total += nb[k];
+        }
+        while (total > gmax) {  // trim the largest shares until the grid is co-resident
+            const int kmax = (int)(std::max_element(nb.begin(), nb.end()) - nb.begin());
+            nb[kmax] -= 1;
+            total -= 1;
+        }
+        std::vector<int32_t> pb_inst, pb_rank;
+        pb_inst.reserve(total);
+        pb_rank.reserve(total);
+        for (int k = 0; k < pl->n_inst; ++k)
+            for (int r = 0; r < nb[k]; ++r) {
+                pb_inst.push_back(k);
+                pb_rank.push_back(r);
+            }
+        (void)hipFree(pl->b_pb_inst[m]);
+        (void)hipFree(pl->b_pb_rank[m]);
+        (void)hipFree(pl->b_nblk[m]);
+        pl->b_pb_inst[m] = pl->b_pb_rank[m] = pl->b_nblk[m] = nullptr;
+        PHAST_HIP(hipMalloc(&pl->b_pb_inst[m], sizeof(int32_t) * total));
+        PHAST_HIP(hipMalloc(&pl->b_pb_rank[m], sizeof(int32_t) * total));
+        PHAST_HIP(hipMalloc(&pl->b_nblk[m], sizeof(int32_t) * pl->n_inst));
+        PHAST_HIP(hipMemcpy(pl->b_pb_inst[m], pb_inst.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
+        PHAST_HIP(hipMemcpy(pl->b_pb_rank[m], pb_rank.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
+        PHAST_HIP(hipMemcpy(pl->b_nblk[m], nb.data(), sizeof(int32_t) * pl->n_inst, hipMemcpyHostToDevice));
+        pl->b_grid[m] = (int32_t)total;
+    }
+    if (!pl->b_bar) {
+        PHAST_HIP(hipMalloc(&pl->b_bar, sizeof(uint32_t) * 2 * pl->n_inst));
+        PHAST_HIP(hipMemset(pl->b_bar, 0, sizeof(uint32_t) * 2 * pl->n_inst));
+        PHAST_HIP(hipMalloc(&pl->b_err, sizeof(int32_t)));
+        PHAST_HIP(hipMemset(pl->b_err, 0, sizeof(int32_t)));
+        PHAST_HIP(hipDeviceSynchronize());
+    }
+    if (grid_out) {
+        grid_out[0] = pl->b_grid[0];
+        grid_out[1] = pl->b_grid[1];
+    }
+    return PHAST_OK;
+}
+
+extern "C" int phast_q4_batch_step(phast_q4_plan *pl, int32_t method, const double *u, const double *v,
+                                   const double *a, const double *d, const double *f_ext, const double *M_inv,
+                                   const uint8_t *bc_mask, const double *bc_vals, const double *H_old,
+                                   const double *Gc_cell, const uint8_t *fixed, const double *fixed_vals, double tol,
+                                   int32_t max_iter, double *u1, double *v1, double *a1, double *H_new, double *psi,
+                                   double *d_out, int32_t *iters, int32_t *status, hipStream_t st) {
+    PHAST_REQUIRE(pl && u && v && a && d && M_inv && H_old && u1 && v1 && a1 && H_new && d_out && iters && status,
+                  "phast_q4_batch_step: null argument");
+    PHAST_REQUIRE(method == 0 || method == 1, "method must be 0 (projected) or 1 (post-clamp)");
+    PHAST_REQUIRE(pl->b_grid[method] > 0, "phast_q4_batch_step: call phast_q4_batch_prepare first");
+    PHAST_REQUIRE(!bc_mask == !bc_vals, "bc_mask and bc_vals must be given together");
+    PHAST_REQUIRE(!fixed == !fixed_vals, "fixed and fixed_vals must be given together");
+    PHAST_REQUIRE(tol >= 0.0 && max_iter >= 0, "tol and max_iter must be non-negative");
+    PHAST_REQUIRE(u1 != u && v1 != v && a1 != a && H_new != H_old && d_out != d,
+                  "phast_q4_batch_step: outputs must not alias inputs");
+    for (const void *q : {(const void *)u, (const void *)v, (const void *)a, (const void *)f_ext,
+                          (const void *)u1, (const void *)v1, (const void *)a1, (const void *)H_old,
+                          (const void *)H_new, (const void *)psi, (const void *)bc_vals})
+        PHAST_REQUIRE(!q || aligned16(q), "vector arguments must be 16-byte aligned");
+    BatchArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.M.inst = pl->d_inst;
+    A.M.block_inst = pl->d_block_inst;
+    A.M.u = reinterpret_cast<const double2 *>(u);
+    A.M.v = reinterpret_cast<const double2 *>(v);
+    A.M.a = reinterpret_cast<const double2 *>(a);
+    A.M.d = d;
+    A.M.fext = reinterpret_cast<const double2 *>(f_ext);
+    A.M.minv = M_inv;
+    A.M.bcm = bc_mask;
+    A.M.bcv = bc_vals;
+    A.M.Hold = H_old;
+    A.M.u1 = reinterpret_cast<double2 *>(u1);
+    A.M.v1 = reinterpret_cast<double2 *>(v1);
+    A.M.a1 = reinterpret_cast<double2 *>(a1);
+    A.M.Hnew = H_new;
+    A.M.psi = psi;
+
+    A.R = dmg_base(pl);
+    A.R.H = H_new;
+    A.R.Gc = Gc_cell;
+    A.R.out = pl->b;
+    A.R.out2 = pl->minv;
+
+    DmgArgs D = dmg_base(pl);
+    D.H = H_new;
+    D.Gc = Gc_cell;
+    D.d = d_out;
+    D.r = pl->r;
+    D.z = pl->z;
+    D.p = pl->p;
+    D.Ap = pl->Ap;
+    D.b = pl->b;
+    D.minv = pl->minv;
+    D.lb = pl->lb;
+    D.fixed = fixed;
+    D.active = pl->active;
+    D.part = pl->part;
+    A.Dx = D;
+    A.Dx.x = d_out;
+    A.Dp = D;
+    A.Dp.x = pl->p;
+
+    VecArgs &V = A.V;
+    V.inst = pl->d_inst;
+    V.block_inst = pl->d_block_inst;
+    V.d = d_out;
+    V.r = pl->r;
+    V.z = pl->z;
+    V.p = pl->p;
+    V.lb = pl->lb;
+    V.Ap = pl->Ap;
+    V.minv = pl->minv;
+    V.init = nullptr;
+    V.dprev = d;
+    V.vals = fixed_vals;
+    V.fixed = fixed;
+    V.active = pl->active;
+    V.active_out = nullptr;
+    V.d_out = d_out;
+    V.part = pl->part;
+    V.n_tiles = pl->n_tiles;
+
+    A.pb_inst = pl->b_pb_inst[method];
+    A.pb_rank = pl->b_pb_rank[method];
+    A.nblk = pl->b_nblk[method];
+    A.bar = pl->b_bar;
+    A.err = pl->b_err;
+    A.iters = iters;
+    A.status = status;
+    A.minv = pl->minv;
+    A.tol = tol;
+    A.max_iter = max_iter;
+    A.method = method;
+    A.spin_ticks = pl->b_spin_ticks;
+    PHAST_TRY(mech_dispatch(pl, A.M, true, true, st));
+    switch (pl->tx) {
+        case 256: batch_launch<256>(pl, method, A, st); break;
+        case 128: batch_launch<128>(pl, method, A, st); break;
+        case 64: batch_launch<64>(pl, method, A, st); break;
+        default: set_error("unsupported strip width %d", pl->tx); return PHAST_ERR_UNSUPPORTED;
+    }
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_q4_batch_error(phast_q4_plan *pl, int32_t *err, int32_t clear) {
+    PHAST_REQUIRE(pl && err, "phast_q4_batch_error: null argument");
+    *err = 0;
+    if (!pl->b_err) return PHAST_OK;
+    PHAST_HIP(hipMemcpy(err, pl->b_err, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (clear && *err) PHAST_HIP(hipMemset(pl->b_err, 0, sizeof(int32_t)));
     return PHAST_OK;
 }
