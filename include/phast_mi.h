@@ -274,6 +274,33 @@ int phast_hex8_mech_solve_mg(const phast_hex8_plan *plan, phast_hex8_mg *mg, con
 /* Frees the process-wide solver scratch of the 3D solves. */
 int phast_release_solver_scratch(void);
 
+
+/* ---- adjoint pieces of the converged staggered voxel state ----------------------------
+ * No counterpart in the reference (it has no voxel adjoint); each is the transposed
+ * derivative of a reference residual, used by a reverse sweep over load steps.
+ * Gauss-point arrays are (E,8) (xi fastest), stress-like arrays (E,8,6) engineering Voigt.
+ * phast_hex8_set_gc: replace the per-voxel Gc (device (E,)) used by the damage operator.
+ * adj_coupling:  out_qp = w g'(d_q) eps(l)_q . C eps(x)_q   ((dF/dd)^T l before the
+ *                node sum; F = A(g(d)) x, solid3d/operator.py:939-946, 1108-1194)
+ * adj_driving:   S_qp6 = t_q dY/deps_q  (Y of operator.py:1415-1444 / splits.py:68-161)
+ * qp_to_node:    out_n = scale sum_e sum_q N_a(q) f_q
+ * qp_stress_apply: out_mech = [sum_q B_q^T S_q (3n), sum_q S_q (6)]
+ * adj_damage_history: out_qp = scale 2(1-kappa) w (N_q.d - 1)(N_q.l)   ((dG/dH)^T l)
+ * adj_damage_gc: out_e = scale l_e.(M/ell + ell L) d_e                   ((dG/dGc)^T l) */
+int phast_hex8_set_gc(phast_hex8_plan *plan, const double *Gc, hipStream_t stream);
+int phast_hex8_adj_coupling(const phast_hex8_plan *plan, const double *x, const double *l, const double *d,
+                            double *out_qp, hipStream_t stream);
+int phast_hex8_adj_driving(const phast_hex8_plan *plan, const double *x, const double *t_qp, double *S_qp6,
+                           hipStream_t stream);
+int phast_hex8_qp_to_node(const phast_hex8_plan *plan, const double *f_qp, double scale, double *out_n,
+                          hipStream_t stream);
+int phast_hex8_qp_stress_apply(const phast_hex8_plan *plan, const double *S_qp6, double *out_mech,
+                               hipStream_t stream);
+int phast_hex8_adj_damage_history(const phast_hex8_plan *plan, const double *d, const double *l, double scale,
+                                  double *out_qp, hipStream_t stream);
+int phast_hex8_adj_damage_gc(const phast_hex8_plan *plan, const double *d, const double *l, double scale,
+                             double *out_e, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -242,6 +242,77 @@ class VoxelOperator:
         return m
 
 
+# ---------------------------------------------------------------------------------------
+# Adjoint pieces of the converged staggered state.  The reference has no adjoint for the
+# voxel path; these are the transposed derivatives of its residuals (operator.py:939-946,
+# 1108-1194, 1415-1509), written from the mathematics and pinned by finite differences of
+# the functions above.
+# ---------------------------------------------------------------------------------------
+
+def driving_strain_gradient(op: VoxelOperator, eps):
+    """dY/d(engineering strain) per Gauss point (E,8,6): drivers none / full (C eps) /
+    spectral (lam <tr>+ I + 2 mu eps+, shear entries the tensor off-diagonals)."""
+    T = np.zeros(eps.shape)
+    full = op.driver == 1
+    if full.any():
+        C = op.bank[op.index[full]]
+        T[full] = np.einsum("ekm,eqm->eqk", C, eps[full])
+    spec = op.driver == 2
+    if spec.any():
+        e = eps[spec]
+        Et = engineering_strain_to_tensor(e)
+        val, vec = np.linalg.eigh(Et)
+        pos = np.einsum("...ia,...a,...ja->...ij", vec, np.maximum(val, 0.0), vec)
+        trp = np.maximum(e[..., :3].sum(-1), 0.0)
+        lam, mu = op.lam[spec, None], op.mu[spec, None]
+        Tt = 2.0 * mu[..., None, None] * pos
+        Tt[..., 0, 0] += lam * trp
+        Tt[..., 1, 1] += lam * trp
+        Tt[..., 2, 2] += lam * trp
+        T[spec] = np.stack((Tt[..., 0, 0], Tt[..., 1, 1], Tt[..., 2, 2], Tt[..., 1, 2], Tt[..., 0, 2],
+                            Tt[..., 0, 1]), axis=-1)
+    return T
+
+
+def adjoint_coupling(op: VoxelOperator, x, lam_x, d):
+    """(dF/dd)^T lam for F = A(g(d)) x: sum_q N_a w g'(d_q) eps(lam)_q . C eps(x)_q (n,)."""
+    d_q = d[op.nodes] @ op.N.T
+    gp = -2.0 * (1.0 - op.kappa) * (1.0 - d_q)
+    ex, el = op.qp_strain(x), op.qp_strain(lam_x)
+    C = op.bank[op.index]
+    s = np.einsum("eqk,ekm,eqm->eq", el, C, ex) * gp * op.w
+    return scatter(op.nodes, s @ op.N, op.n_red)
+
+
+def qp_stress_apply(op: VoxelOperator, S):
+    """sum_q B_q^T S_q scattered, plus the E rows sum_q S_q (no quadrature weight)."""
+    local = S.reshape(-1, 48) @ op.B_flat.T
+    return np.concatenate((scatter(op.dofs, local, 3 * op.n_red), S.sum(axis=(0, 1))))
+
+
+def adjoint_driving(op: VoxelOperator, x, t):
+    """(dY/dx)^T t for t (E,8): sum_q B_q^T t_q dY/deps_q (mech space incl. E rows)."""
+    T = driving_strain_gradient(op, op.qp_strain(x))
+    return qp_stress_apply(op, t[..., None] * T)
+
+
+def adjoint_damage_history(op: VoxelOperator, d, lam_d):
+    """(dG/dH)^T lam for G = A_H d - b_H (unscaled): c_h w (N_q.d_e - 1)(N_q.lam_e) (E,8)."""
+    d_q = d[op.nodes] @ op.N.T
+    l_q = lam_d[op.nodes] @ op.N.T
+    return op.c_history * op.w * (d_q - 1.0) * l_q
+
+
+def adjoint_damage_gc(op: VoxelOperator, d, lam_d):
+    """(dG/dGc)^T lam (unscaled): lam_e . (M/ell + ell L) d_e per voxel (E,)."""
+    return np.einsum("ea,ab,eb->e", lam_d[op.nodes], op.K_damage, d[op.nodes])
+
+
+def qp_to_node(op: VoxelOperator, f):
+    """sum over the voxels of a node of sum_q N_a(q) f_q (n,)."""
+    return scatter(op.nodes, f @ op.N, op.n_red)
+
+
 class PCGBreakdownError(RuntimeError):
     pass
 

@@ -78,6 +78,13 @@ _SIGNATURES = {
     "phast_hex8_damage_action": ([_P, _P, _P, _D, _P, _P], C.c_int),
     "phast_hex8_damage_rhs": ([_P, _P, _D, _P, _P], C.c_int),
     "phast_hex8_damage_diagonal": ([_P, _P, _D, _P, _P], C.c_int),
+    "phast_hex8_set_gc": ([_P, _P, _P], C.c_int),
+    "phast_hex8_adj_coupling": ([_P] * 6, C.c_int),
+    "phast_hex8_adj_driving": ([_P] * 5, C.c_int),
+    "phast_hex8_qp_to_node": ([_P, _P, _D, _P, _P], C.c_int),
+    "phast_hex8_qp_stress_apply": ([_P] * 4, C.c_int),
+    "phast_hex8_adj_damage_history": ([_P, _P, _P, _D, _P, _P], C.c_int),
+    "phast_hex8_adj_damage_gc": ([_P, _P, _P, _D, _P, _P], C.c_int),
 }
 
 _lib: C.CDLL | None = None

@@ -414,6 +414,66 @@ class PeriodicVoxelPhaseFieldOperator:
                                  rhs=self.damage_rhs(H, scale), diagonal=self.damage_diagonal(H, scale),
                                  _op=self, _H=H, _scale=scale)
 
+    # -- adjoint pieces (transposed derivatives of the residuals) --------------------------
+    def set_gc(self, Gc: torch.Tensor) -> None:
+        """Replace the per-voxel Gc of the damage operator (and the damage scale built on it)."""
+        Gc = self._check("Gc", Gc, (self.n_elements,))
+        gmin = float(Gc.min())
+        if not (gmin > 0.0 and math.isfinite(gmin) and bool(torch.isfinite(Gc).all())):
+            raise ValueError("Gc must be finite and positive at every element")
+        check(_L.phast_hex8_set_gc(self._plan, ptr(Gc), stream()), "set_gc")
+        self.Gc = Gc.clone()
+        self.damage_scale = self.ell / (gmin * float(self.grid.voxel_volume))
+
+    def adj_coupling(self, x, l, d) -> torch.Tensor:
+        """(E,8): w g'(d_q) eps(l)_q . C eps(x)_q (the Gauss-point form of (dF/dd)^T l)."""
+        x = self._check("x", x, (self.n_mech,))
+        l = self._check("l", l, (self.n_mech,))
+        d = self._check("d", d, (self.n_red,))
+        out = torch.empty((self.n_elements, 8), dtype=F64, device=self.device)
+        check(_L.phast_hex8_adj_coupling(self._plan, ptr(x), ptr(l), ptr(d), ptr(out), stream()), "adj_coupling")
+        return out
+
+    def adj_driving(self, x, t) -> torch.Tensor:
+        """(E,8,6): t_q dY/deps_q."""
+        x = self._check("x", x, (self.n_mech,))
+        t = self._check_qp("t", t)
+        out = torch.empty((self.n_elements, 8, 6), dtype=F64, device=self.device)
+        check(_L.phast_hex8_adj_driving(self._plan, ptr(x), ptr(t), ptr(out), stream()), "adj_driving")
+        return out
+
+    def qp_to_node(self, f, scale: float = 1.0) -> torch.Tensor:
+        """(n,): scale sum_e sum_q N_a(q) f_q."""
+        f = self._check_qp("f", f)
+        out = torch.empty(self.n_red, dtype=F64, device=self.device)
+        check(_L.phast_hex8_qp_to_node(self._plan, ptr(f), float(scale), ptr(out), stream()), "qp_to_node")
+        return out
+
+    def qp_stress_apply(self, S) -> torch.Tensor:
+        """(n_mech,): [sum_q B_q^T S_q, sum_q S_q] for S (E,8,6)."""
+        S = self._check("S", S, (self.n_elements, 8, 6))
+        out = torch.empty(self.n_mech, dtype=F64, device=self.device)
+        check(_L.phast_hex8_qp_stress_apply(self._plan, ptr(S), ptr(out), stream()), "qp_stress_apply")
+        return out
+
+    def adj_damage_history(self, d, l, scale: float = 1.0) -> torch.Tensor:
+        """(E,8): scale 2(1-kappa) w (N_q.d - 1)(N_q.l) ((dG/dH)^T l)."""
+        d = self._check("d", d, (self.n_red,))
+        l = self._check("l", l, (self.n_red,))
+        out = torch.empty((self.n_elements, 8), dtype=F64, device=self.device)
+        check(_L.phast_hex8_adj_damage_history(self._plan, ptr(d), ptr(l), float(scale), ptr(out), stream()),
+              "adj_damage_history")
+        return out
+
+    def adj_damage_gc(self, d, l, scale: float = 1.0) -> torch.Tensor:
+        """(E,): scale l_e.(M/ell + ell L) d_e ((dG/dGc)^T l)."""
+        d = self._check("d", d, (self.n_red,))
+        l = self._check("l", l, (self.n_red,))
+        out = torch.empty(self.n_elements, dtype=F64, device=self.device)
+        check(_L.phast_hex8_adj_damage_gc(self._plan, ptr(d), ptr(l), float(scale), ptr(out), stream()),
+              "adj_damage_gc")
+        return out
+
     # -- device-resident solves ----------------------------------------------------------
     def solve_mechanics(self, *, g=None, d=None, fixed_mask, fixed_values, diag, rhs=None, x0=None,
                         rtol=1e-8, atol=0.0, maxiter=5000, check_every=10):

@@ -55,11 +55,13 @@ class SolverState:
     external_work: float
     d_prev_step: torch.Tensor
     increment: int = 0
+    active: tuple | None = None  # (lower, upper) active masks of the accepted damage solve
 
     def clone(self) -> "SolverState":
+        act = None if self.active is None else tuple(a.clone() for a in self.active)
         return SolverState(self.step, self.x.clone(), self.d.clone(), self.H.clone(), self.macro_strain.clone(),
                            self.macro_stress.clone(), float(self.external_work), self.d_prev_step.clone(),
-                           self.increment)
+                           self.increment, act)
 
 
 @dataclass
@@ -174,6 +176,7 @@ class StaggeredSolver:
             raise _AttemptFailure(f"damage box-KKT solve failed: {exc}") from exc
         except (RuntimeError, ValueError) as exc:
             raise _AttemptFailure(f"damage PCG breakdown: {exc}") from exc
+        self._last_active = (res.lower_active, res.upper_active)
         return res.solution, res.total_pcg_iterations, res.active_set_iterations, res.kkt.satisfies_kkt
 
     def _change(self, x_new, x_old, d_new, d_old, step_size):
@@ -260,7 +263,7 @@ class StaggeredSolver:
         residual = external - (elastic + fracture)
         completes = 1 <= increment <= len(self.load.targets) and target == self.load.targets[increment - 1]
         new_state = SolverState(state.step + 1, x, d, H, E.clone(), sigma, external, state.d,
-                                increment if completes else state.increment)
+                                increment if completes else state.increment, self._last_active)
         fa, fb = self._face_lengths
         rec = IncrementRecord(step=new_state.step, increment=increment, macro_strain=macro_strain,
                               macro_stress=macro_stress, reaction_force=macro_stress[self._load_axis] * fa * fb,
