@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--tiles", type=str, default="", help="fixed hex8 tile tx,ty,kz (skips the autotuner)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-specimen batched run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the voxel reaction-curve gradient run")
+    ap.add_argument("--adjoint-size", type=int, default=96, help="voxels per edge of the gradient run")
+    ap.add_argument("--adjoint-steps", type=int, default=30, help="load steps of the gradient run")
+    ap.add_argument("--adjoint-fd", type=int, default=2, help="single-voxel central differences in the gradient run")
     ap.add_argument("--batched-steps", type=int, default=2000, help="steps of the 64-specimen batched run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 5 device copies of 256 MiB (the reference kernel for HBM counter scaling)")
@@ -148,6 +152,8 @@ def main():
         sec = secondary(rng, dev, a)
     if not a.no_batched:
         sec["c3_batched_64_specimens"] = batched_specimens(dev, a.batched_steps)
+    if not a.no_adjoint:
+        sec["c4_adjoint_gradient"] = adjoint_gradient(dev, a.adjoint_size, a.adjoint_steps, a.adjoint_fd)
 
     out = {
         "metric": "element-apply GDOF/s (periodic hex8 voxel mech_action, 64^3 pore cube)",
@@ -358,6 +364,71 @@ def batched_specimens(dev, steps):
     del singles, ref
     torch.cuda.empty_cache()
     return res
+
+
+def adjoint_gradient(dev, n, steps, n_fd):
+    """c4: gradient of L = sum_k (R_k(Gc) - R_k(Gc_target))^2 over a load path of the n^3 cell
+    with 20 seeded periodic pores (radii U[4, 9] voxels), Gc = 5e-4 (1 + 0.1 N(0,1)) per voxel;
+    reverse-sweep wall against forward wall, checked by central differences on the ``n_fd``
+    voxels of largest sensitivity."""
+    import numpy as np
+    import torch
+    from phast_amd import adjoint3d as A
+    from phast_amd import geometry as G
+    from phast_amd import solid3d as S
+    h = (1e-3,) * 3
+    phase = G.random_pores((n, n, n), h, 20, (4.0, 9.0), seed=96)
+    phases = {"mi_matrix": S.PhaseMaterial("mi_matrix", 3000.0, 0.35, 5e-4),
+              "pore": S.PhaseMaterial("pore", 3000.0 * 1e-6, 0.35, 5e-4)}
+    mats = S.build_element_arrays(phase, phases)
+    targets = tuple(1e-4 * k for k in range(1, steps + 1))  # up to 3e-3: before the peak
+    load = S.LoadCase("xx", 0, targets, (1, 2, 3, 4, 5), ())
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid((n, n, n), h), mats, load, kappa=1e-6, ell=2e-3, device=dev)
+    mg = S.VoxelMultigrid(op)
+    t_start = time.perf_counter()
+    log = lambda msg: print(f"[adjoint {time.perf_counter() - t_start:7.1f}s] {msg}", file=sys.stderr, flush=True)
+    curve = A.ReactionCurve(op, preconditioner=mg, progress=log)
+    rng = np.random.default_rng(4)
+    E = op.n_elements
+    Gc = torch.as_tensor(np.maximum(5e-4 * (1 + 0.1 * rng.standard_normal(E)), 1e-5), dtype=torch.float64,
+                         device=dev)
+    Gc_t = Gc * torch.as_tensor(1 + 0.05 * rng.standard_normal(E), dtype=torch.float64, device=dev)
+    R_hat, _ = curve.forward(Gc_t)
+    R_hat = R_hat.detach().clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    R, tape = curve.forward(Gc)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    L = float(((R - R_hat) ** 2).sum())
+    t0 = time.perf_counter()
+    grad = curve.backward(tape, 2.0 * (R - R_hat))
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    d_max = float(tape["records"][steps - 1].d.max())
+    del tape
+
+    def loss(g):
+        Rg, _ = curve.forward(g)
+        return float(((Rg - R_hat) ** 2).sum())
+
+    # central differences with a Gc step of 1e-6 N/mm on the voxels of largest sensitivity
+    errs = []
+    for e in torch.argsort(grad.abs(), descending=True)[:n_fd].tolist():
+        step = 1e-6
+        gp, gm = Gc.clone(), Gc.clone()
+        gp[e] += step
+        gm[e] -= step
+        f1 = (loss(gp) - loss(gm)) / (2 * step)
+        errs.append(abs(float(grad[e]) - f1) / max(abs(f1), 1e-300))
+    return {"cells": n, "voxels": E, "load_steps": steps, "pores": 20, "forward_wall_s": t_fwd,
+            "gradient_wall_s": t_bwd, "gradient_over_forward": t_bwd / t_fwd, "loss": L,
+            "d_max_final": d_max, "voxel_fd_rel_errs": errs, "fd_step": 1e-6,
+            "voxel_fd_rel_err_max": max(errs) if errs else None, "fd_voxels": n_fd,
+            "adjoint_block_iterations": curve.stats.block_iterations[::-1],
+            "adjoint_mech_pcg_iterations": curve.stats.mech_pcg_iterations,
+            "adjoint_damage_pcg_iterations": curve.stats.damage_pcg_iterations,
+            "preconditioner": "voxel multigrid V-cycle"}
 
 
 if __name__ == "__main__":

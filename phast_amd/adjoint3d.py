@@ -68,7 +68,7 @@ class ReactionCurve:
     def __init__(self, op: PeriodicVoxelPhaseFieldOperator, load: LoadCase | None = None,
                  cfg: SolverConfig = TIGHT, *, adjoint_tol: float = 1e-11, max_block_iterations: int = 100,
                  mech_rtol: float = 1e-12, damage_tol: float = 1e-13, checkpoint_every: int = 1,
-                 preconditioner=None):
+                 preconditioner=None, progress=None):
         if checkpoint_every < 1:
             raise ValueError("checkpoint_every must be >= 1")
         self.op = op
@@ -78,6 +78,7 @@ class ReactionCurve:
         self.mech_rtol, self.damage_tol = float(mech_rtol), float(damage_tol)
         self.checkpoint_every = int(checkpoint_every)
         self.preconditioner = preconditioner
+        self.progress = progress  # optional callable(str) told of every finished step
         self.stats: AdjointStats | None = None
 
     # ------------------------------------------------------------------ forward
@@ -100,6 +101,8 @@ class ReactionCurve:
         for k in range(len(self.load.targets)):
             state, rec = self._step(solver, state, k)
             R.append(state.macro_stress[c])
+            if self.progress is not None:
+                self.progress(f"forward step {k + 1}/{len(self.load.targets)}")
             if self.checkpoint_every == 1:
                 tape["records"][k] = rec
             elif (k + 1) % self.checkpoint_every == 0:
@@ -151,15 +154,20 @@ class ReactionCurve:
             diag_x = op.mech_diagonal(g)
             lam_x, lam_d = zeros_x.clone(), zeros_n.clone()
             fd = zeros_n.clone()
+            fresh = True  # the multigrid hierarchy is rebuilt once per state
             it = 0
             for it in range(1, self.max_block + 1):
                 t = torch.where(Q, gH + op.adj_damage_history(rec.d, lam_d, s), 0.0)
                 rhs_x = -(gx + op.qp_stress_apply(op.adj_driving(rec.x, t)))
                 if bool((rhs_x[~mask] != 0).any()):
-                    lam_x, info = op.solve_mechanics(d=rec.d, fixed_mask=mask, fixed_values=zeros_x, diag=diag_x,
-                                                     rhs=rhs_x, x0=lam_x, rtol=self.mech_rtol, atol=1e-300,
-                                                     maxiter=self.cfg.pcg_maxit_mech,
-                                                     check_every=self.cfg.check_every)
+                    kw = dict(d=rec.d, fixed_mask=mask, fixed_values=zeros_x, diag=diag_x, rhs=rhs_x, x0=lam_x,
+                              rtol=self.mech_rtol, atol=1e-300, maxiter=self.cfg.pcg_maxit_mech,
+                              check_every=self.cfg.check_every)
+                    if self.preconditioner is not None:
+                        lam_x, info = self.preconditioner.solve(refresh=fresh, **kw)
+                        fresh = False
+                    else:
+                        lam_x, info = op.solve_mechanics(**kw)
                     if not info.converged:
                         raise RuntimeError(f"adjoint mechanics PCG did not converge at step {k + 1} "
                                            f"(relative residual {info.relative_residual:.3e})")
@@ -186,6 +194,8 @@ class ReactionCurve:
             else:
                 raise RuntimeError(f"adjoint block iteration did not converge at step {k + 1}")
             stats.block_iterations.append(it)
+            if self.progress is not None:
+                self.progress(f"reverse step {k + 1} ({it} block iterations)")
             grad += op.adj_damage_gc(rec.d, lam_d, s)
             hist = gH + op.adj_damage_history(rec.d, lam_d, s)
             gd_next = torch.where(rec.lower, gd + fd + system.apply(lam_d), 0.0)

@@ -904,10 +904,12 @@ class VoxelMultigrid:
         return z
 
     def solve(self, *, g=None, d=None, fixed_mask, fixed_values, diag, rhs=None, x0=None, rtol=1e-8, atol=0.0,
-              maxiter=5000, check_every=10):
-        """MG-preconditioned solve_constrained of mech_action(., g(d))."""
+              maxiter=5000, check_every=10, refresh: bool = True):
+        """MG-preconditioned solve_constrained of mech_action(., g(d)); ``refresh=False`` keeps
+        the hierarchy of the previous call (same degradation)."""
         op = self.op
-        self.update(g=g if d is None else None, d=d)
+        if refresh or self._src is None:
+            self.update(g=g if d is None else None, d=d)
         g = op._check_qp("g", g) if d is None else None
         d = None if d is None else op._check("d", d, (op.n_red,))
         fm = op._check_mask(fixed_mask, op.n_mech)
