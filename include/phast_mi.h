@@ -183,8 +183,10 @@ int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double
                            const double *Gc, const double *lam, const double *mu, const int8_t *driver,
                            double kappa, double ell, phast_hex8_plan **out);
 int phast_hex8_plan_destroy(phast_hex8_plan *plan);
-/* Launch shape of the marching apply kernels: tile_x x tile_y node columns per block
- * ((16,16) (16,8) (8,8) (32,8)) walking kz node layers.  Default (16,8,8). */
+/* Launch shape of the element apply.  (16,16) and (32,8): voxel-parallel kernel, one
+ * thread per voxel of a (tile_x+1) x (tile_y+1) voxel tile, walking kz node layers.
+ * (16,8) and (8,8): node-column marching kernel
+ * walking kz node layers.  Default (16,8,8). */
 int phast_hex8_set_tiles(phast_hex8_plan *plan, int32_t tile_x, int32_t tile_y, int32_t kz);
 
 /* g (E,8) = (1-kappa)(1-d_q)^2 + kappa, d_q = N d_e (solid3d/operator.py:939-946). */

@@ -162,7 +162,7 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
     auto load_layer = [&](int buf, int k) {
         const int kk = wrap(k, nz);
         for (int s = threadIdx.x; s < NH; s += NT) {
-            const int i = wrap(i0 + s % NXH - 1, nx), j = wrap(j0 + s / NXH - 1, ny);
+            const int i = wrapm(i0 + s % NXH - 1, nx), j = wrapm(j0 + s / NXH - 1, ny);
             const int64_t node = i + (int64_t)nx * (j + (int64_t)ny * kk);
             su[buf][0][s] = A.x[3 * node];
             su[buf][1][s] = A.x[3 * node + 1];
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
         for (int s = threadIdx.x; s < NV; s += NT) {
             const int li = s % NXV - 1, lj = s / NXV - 1;
             const int iv = i0 + li, jv = j0 + lj;
-            const int64_t e = wrap(iv, nx) + (int64_t)nx * (wrap(jv, ny) + (int64_t)ny * kw);
+            const int64_t e = wrapm(iv, nx) + (int64_t)nx * (wrapm(jv, ny) + (int64_t)ny * kw);
             const int b0 = (lj + 1) * NXH + (li + 1);
             double u[3][8], g[8];
 #pragma unroll
@@ -261,6 +261,181 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
     }
     // last-block-done: the block that finishes last sums every block's partials in block
     // order (a fixed order, so the macro rows are deterministic) and resets the counter.
+    __shared__ unsigned int is_last;
+    __threadfence();
+    if (threadIdx.x == 0) is_last = (atomicAdd(A.counter, 1u) == gridDim.x - 1) ? 1u : 0u;
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        for (int k = 0; k < A.nvals; ++k) {
+            double s = 0.0;
+            for (int b = threadIdx.x; b < (int)gridDim.x; b += NT) s += __builtin_nontemporal_load(&A.part[8 * b + k]);
+            s = block_sum<NT>(s, red);
+            if (threadIdx.x == 0) {
+                if (A.y6 && k < 6) A.y6[k] = s;
+                if (A.out7) A.out7[k] = s;
+            }
+        }
+        if (threadIdx.x == 0) atomicExch(A.counter, 0u);
+    }
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Voxel-parallel apply: one thread per voxel of a (OX+1) x (OY+1) voxel tile (320 threads,
+// no second pass for the halo voxels), walking a z slab of kz node layers.
+// ---------------------------------------------------------------------------------------
+//
+// Thread t holds voxel (vx, vy) = (t % (OX+1), t / (OX+1)) of the tile, i.e. voxel
+// (i0-1+vx, j0-1+vy); owned node (ox, oy) = (vx-1, vy-1) is corner (0,0) of that voxel, so
+// its owner keeps one corner of each face in registers and reads the other three from the
+// voxels to its left, below and diagonally below-left through LDS -- in the same order as
+// the marching kernel, face by face (bottom face, then top face).  The top-face sum of
+// voxel layer kv is carried in registers to node layer kv+1; the slab starts one voxel
+// layer early (k0-1) to form the carry into its first node layer, so blocks never talk.
+template <int OX, int OY, int GMODE, bool ISO, bool ENERGY>
+__global__ __launch_bounds__(320) void hex8_apply_kernel(const Hex8Geo G, MechArgs3 A) {
+    constexpr int NT = 320;
+    constexpr int NVX = OX + 1, NVY = OY + 1, NV = NVX * NVY;
+    constexpr int NNX = OX + 2, NNY = OY + 2, NN = NNX * NNY;
+    constexpr int NF = (GMODE == 2) ? 4 : 3;
+    constexpr int NS = (NN + NT - 1) / NT;  // node slots loaded per thread
+    static_assert(NV <= NT, "voxel tile larger than the block");
+    __shared__ double su[2][NF][NN];
+    __shared__ double st[18][NV];  // corners (1,0), (0,1), (1,1) of both faces, 3 components each
+    __shared__ double red[NT / 64];
+
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int tiles = A.tiles_x * A.tiles_y;
+    const int bxy = blockIdx.x % tiles, bz = blockIdx.x / tiles;
+    const int i0 = (bxy % A.tiles_x) * OX, j0 = (bxy / A.tiles_x) * OY;
+    const int k0 = bz * A.kz, k1 = min(k0 + A.kz, nz);
+    const int t = threadIdx.x;
+    const int vx = t % NVX, vy = t / NVX;
+    const bool has_voxel = t < NV;
+    const int iv = i0 - 1 + vx, jv = j0 - 1 + vy;
+    const bool owner = has_voxel && vx >= 1 && vy >= 1 && iv < nx && jv < ny;  // node (iv, jv)
+    const int e_ij = wrapm(iv, nx) + nx * wrapm(jv, ny);
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t nxy = (int64_t)nx * ny;
+    double E[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) E[k] = A.x[3 * n + k];
+
+    // in-layer offsets of this thread's node slots
+    int64_t slot_ij[NS];
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+        const int s = t + m * NT;
+        slot_ij[m] = s < NN ? wrapm(i0 - 1 + s % NNX, nx) + (int64_t)nx * wrapm(j0 - 1 + s / NNX, ny) : -1;
+    }
+    // node layer k of this thread's slots, fetched into registers (issued one layer ahead)
+    double pf[NS][NF];
+    auto fetch = [&](int k) {
+        const int64_t base = nxy * wrap(k, nz);
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+            if (slot_ij[m] < 0) continue;
+            const int64_t node = base + slot_ij[m];
+            pf[m][0] = A.x[3 * node];
+            pf[m][1] = A.x[3 * node + 1];
+            pf[m][2] = A.x[3 * node + 2];
+            if (GMODE == 2) pf[m][3] = A.d[node];
+        }
+    };
+    auto stage = [&](int buf) {
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+            if (slot_ij[m] < 0) continue;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) su[buf][f][t + m * NT] = pf[m][f];
+        }
+    };
+
+    double sE[6] = {0, 0, 0, 0, 0, 0}, en = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int s00 = t - NVX - 1, s10 = t - NVX, s01 = t - 1;  // voxels left-below, below, left
+    fetch(k0 - 1);
+    stage(0);
+    fetch(k0);
+    stage(1);
+    int cur = 0;
+    for (int kv = k0 - 1; kv < k1; ++kv) {
+        __syncthreads();
+        if (kv + 2 <= k1) fetch(kv + 2);  // lands in su[cur] once this layer's voxels are done
+        const bool first = kv < k0;  // carry-only layer
+        double r[3][8];
+        if (has_voxel) {
+            const int64_t e = e_ij + nxy * wrap(kv, nz);
+            double u[3][8], g[8];
+#pragma unroll
+            for (int lz = 0; lz < 2; ++lz) {
+                const int buf = lz ? (cur ^ 1) : cur;
+#pragma unroll
+                for (int ly = 0; ly < 2; ++ly)
+#pragma unroll
+                    for (int lx = 0; lx < 2; ++lx) {
+                        const int slot = (vy + ly) * NNX + (vx + lx), a = lx + 2 * ly + 4 * lz;
+                        u[0][a] = su[buf][0][slot];
+                        u[1][a] = su[buf][1][slot];
+                        u[2][a] = su[buf][2][slot];
+                        if (GMODE == 2) g[a] = su[buf][NF - 1][slot];
+                    }
+            }
+            if (GMODE == 2) {
+                double dq[8];
+                voxel_interp(G, g, dq);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const double omd = 1.0 - dq[q];
+                    g[q] = G.omk * (omd * omd) + G.kappa;  // splits.py:56-59
+                }
+            } else if (GMODE == 1) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) g[q] = A.g[e * 8 + q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) g[q] = 1.0;
+            }
+            const bool own = owner && !first;
+            if (ISO) {
+                voxel_mech<true, ENERGY>(G, u, E, g, nullptr, A.lam[e], A.mu[e], own, r, sE, en);
+            } else {
+                voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)A.bank_index[e], 0.0, 0.0, own, r,
+                                          sE, en);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                st[c][t] = r[c][1];
+                st[3 + c][t] = r[c][2];
+                st[6 + c][t] = r[c][3];
+                st[9 + c][t] = r[c][5];
+                st[12 + c][t] = r[c][6];
+                st[15 + c][t] = r[c][7];
+            }
+        }
+        __syncthreads();
+        if (owner) {
+            const int64_t node = iv + (int64_t)nx * (jv + (int64_t)ny * kv);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (!first) A.y[3 * node + c] = acc[c] + (((st[6 + c][s00] + st[3 + c][s10]) + st[c][s01]) + r[c][0]);
+                acc[c] = ((st[15 + c][s00] + st[12 + c][s10]) + st[9 + c][s01]) + r[c][4];
+            }
+        }
+        if (kv + 2 <= k1) stage(cur);  // node layer kv+2 replaces layer kv
+        cur ^= 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double tt = block_sum<NT>(sE[k], red);
+        if (threadIdx.x == 0) A.part[8 * blockIdx.x + k] = tt;
+    }
+    if (ENERGY) {
+        const double tt = block_sum<NT>(en, red);
+        if (threadIdx.x == 0) A.part[8 * blockIdx.x + 6] = tt;
+    }
+    // last-block-done: the last block sums every block's partials in block order
     __shared__ unsigned int is_last;
     __threadfence();
     if (threadIdx.x == 0) is_last = (atomicAdd(A.counter, 1u) == gridDim.x - 1) ? 1u : 0u;
@@ -456,7 +631,7 @@ __global__ __launch_bounds__(TX *TY) void hex8_damage_kernel(const Hex8Geo G, Dm
     auto load_layer = [&](int buf, int k) {
         const int kk = wrap(k, nz);
         for (int s = threadIdx.x; s < NH; s += NT) {
-            const int i = wrap(i0 + s % NXH - 1, nx), j = wrap(j0 + s / NXH - 1, ny);
+            const int i = wrapm(i0 + s % NXH - 1, nx), j = wrapm(j0 + s / NXH - 1, ny);
             su[buf][s] = A.d[i + (int64_t)nx * (j + (int64_t)ny * kk)];
         }
     };
@@ -470,7 +645,7 @@ __global__ __launch_bounds__(TX *TY) void hex8_damage_kernel(const Hex8Geo G, Dm
         const int kw = wrap(kv, nz);
         for (int s = threadIdx.x; s < NV; s += NT) {
             const int li = s % NXV - 1, lj = s / NXV - 1;
-            const int64_t e = wrap(i0 + li, nx) + (int64_t)nx * (wrap(j0 + lj, ny) + (int64_t)ny * kw);
+            const int64_t e = wrapm(i0 + li, nx) + (int64_t)nx * (wrapm(j0 + lj, ny) + (int64_t)ny * kw);
             const int b0 = (lj + 1) * NXH + (li + 1);
             double dc[8];
 #pragma unroll
@@ -581,6 +756,44 @@ static int launch_mech_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool
     return PHAST_OK;
 }
 
+template <int OX, int OY>
+static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool energy, hipStream_t st) {
+    A.tiles_x = (int)ceil_div(pl->geo.nx, OX);
+    A.tiles_y = (int)ceil_div(pl->geo.ny, OY);
+    const int nbz = (int)ceil_div(pl->geo.nz, A.kz);
+    const int nb = A.tiles_x * A.tiles_y * nbz;
+    if (nb > pl->max_blocks) {
+        set_error("hex8 apply needs %d blocks, plan holds partials for %d", nb, pl->max_blocks);
+        return PHAST_ERR_ARG;
+    }
+    dim3 grid(nb), block(320);
+#define PHAST_L(GM, ISO, EN) hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, GM, ISO, EN>), grid, block, 0, st, pl->geo, A)
+    if (pl->iso) {
+        if (energy) {
+            if (gmode == 0) PHAST_L(0, true, true);
+            else if (gmode == 1) PHAST_L(1, true, true);
+            else PHAST_L(2, true, true);
+        } else {
+            if (gmode == 0) PHAST_L(0, true, false);
+            else if (gmode == 1) PHAST_L(1, true, false);
+            else PHAST_L(2, true, false);
+        }
+    } else {
+        if (energy) {
+            if (gmode == 0) PHAST_L(0, false, true);
+            else if (gmode == 1) PHAST_L(1, false, true);
+            else PHAST_L(2, false, true);
+        } else {
+            if (gmode == 0) PHAST_L(0, false, false);
+            else if (gmode == 1) PHAST_L(1, false, false);
+            else PHAST_L(2, false, false);
+        }
+    }
+#undef PHAST_L
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
 int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
                        double *out7, bool energy, hipStream_t st) {
     MechArgs3 A;
@@ -605,10 +818,10 @@ int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g,
         set_error("internal: hex8 apply without output");
         return PHAST_ERR_ARG;
     }
-    if (pl->tx == 16 && pl->ty == 16) rc = launch_mech_t<16, 16>(pl, A, gmode, energy, st, &nb);
+    if (pl->tx == 16 && pl->ty == 16) rc = launch_apply_t<16, 16>(pl, A, gmode, energy, st);
+    else if (pl->tx == 32 && pl->ty == 8) rc = launch_apply_t<32, 8>(pl, A, gmode, energy, st);
     else if (pl->tx == 16 && pl->ty == 8) rc = launch_mech_t<16, 8>(pl, A, gmode, energy, st, &nb);
     else if (pl->tx == 8 && pl->ty == 8) rc = launch_mech_t<8, 8>(pl, A, gmode, energy, st, &nb);
-    else if (pl->tx == 32 && pl->ty == 8) rc = launch_mech_t<32, 8>(pl, A, gmode, energy, st, &nb);
     else {
         set_error("unsupported hex8 tile %dx%d", pl->tx, pl->ty);
         return PHAST_ERR_UNSUPPORTED;

@@ -11,6 +11,8 @@ namespace phast {
 // interpolation weight of corner l at Gauss point i: al if l == i else be.
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+// periodic index for tiles wider than the cell (i in [-n, inf))
+__device__ __forceinline__ int wrapm(int i, int n) { return i < 0 ? i + n : (i >= n ? i % n : i); }
 
 // bilinear interpolation of corner values f[a][b] (a,b in {0,1}) to Gauss points o[ga][gb]
 __device__ __forceinline__ void interp2(double al, double be, double f00, double f10, double f01, double f11,

@@ -31,7 +31,7 @@ struct phast_hex8_plan {
     double *part = nullptr;  // block partials (8 per block)
     unsigned int *counter = nullptr;
     int32_t max_blocks = 0;
-    int32_t tx = 16, ty = 16, kz = 8;
+    int32_t tx = 16, ty = 8, kz = 8;  // launch shape of the apply kernels (phast_hex8_set_tiles)
 };
 
 namespace phast {
