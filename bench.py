@@ -6,10 +6,12 @@ operator on the 64^3 pore cube (a centred spherical pore of radius 12 voxels, vo
 stiffness 1e-6, E = 3 GPa, nu = 0.35, Gc = 0.5 J/m^2, l0 = 2 voxels, voxel 1 um).  One
 step = one apply y = A(g(d)) x with the degradation formed in-kernel from nodal damage,
 inputs resident in HBM.  GDOF = 3 n + 6 output dofs per apply.  ``roofline`` prices the
-apply kernel's algorithmic bytes (57 B per node: u 24 + d 8 + phase 1 + r 24) against
-8 TB/s; ``cpu_baseline`` times the oracle's numpy restatement of the same apply on the
-host cores.  ``secondary`` holds the Q4 apply, the fused explicit step and the larger
-voxel sizes measured in the same run.
+apply kernel's algorithmic FLOPs (940 per voxel, sum-factorised) against the 78.6 TFLOP/s
+FP64 peak -- its intensity (940 FLOP / 57 B) is above the FP64 ridge -- and also reports the
+algorithmic bytes (57 B per node: u 24 + d 8 + phase 1 + r 24) against 8 TB/s;
+``cpu_baseline`` times the oracle's numpy restatement of the same apply on the host cores.
+``secondary`` holds the Q4 apply and fused explicit step, the larger voxel sizes, the
+64-specimen batched run and the reaction-curve gradient, measured in the same run.
 """
 from __future__ import annotations
 
@@ -24,7 +26,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6         # MI355X FP64 (vector and matrix peaks coincide)
 HEX8_BYTES_PER_NODE = 57.0      # u 24 + d 8 + phase 1 + r 24
+HEX8_FLOPS_PER_VOXEL = 940.0    # sum-factorised apply
+# HBM bytes per launch of the 64^3 apply measured with rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE
+# x 1.0 KiB, copy-calibrated), per launch shape; see profiles/hex8_apply_64cube_pore.md
+HEX8_64_TRAFFIC = {(32, 8, 4): (11522.0 * 2.0 + 6176.4) * 1024.0}
 Q4_APPLY_BYTES_PER_NODE = 40.0  # u 16 + d 8 + f 16
 Q4_STEP_BYTES_PER_NODE = 192.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, H r+w 64
 
@@ -36,6 +43,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--secondary", type=str, default="q4,hex8", help="secondary parts to run: q4, hex8")
+    ap.add_argument("--q4-tile", type=str, default="", help="fixed Q4 strip threads,rows (skips the autotuner)")
+    ap.add_argument("--hex8-big-tiles", type=str, default="", help="fixed tx,ty,kz of the 128^3/256^3 runs")
+    ap.add_argument("--batch-tile", type=str, default="", help="fixed strip of the batched run (skips its autotuner)")
+    ap.add_argument("--no-sequential", action="store_true", help="skip the 64 single-specimen runs of the batched case")
     ap.add_argument("--tiles", type=str, default="", help="fixed hex8 tile tx,ty,kz (skips the autotuner)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-specimen batched run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the voxel reaction-curve gradient run")
@@ -108,9 +120,8 @@ def main():
     y = torch.empty_like(x)
     t_apply, wall = events_time(lambda: op.mech_action_d(x, d), a.steps, a.warmup)
     gdofs = op.n_mech / t_apply / 1e9
-    achieved = HEX8_BYTES_PER_NODE * op.n_red / t_apply / 1e9
-    flops_per_voxel = 1238.0
-    tflops = flops_per_voxel * op.n_elements / t_apply / 1e12
+    hbm_gbs = HEX8_BYTES_PER_NODE * op.n_red / t_apply / 1e9
+    tflops = HEX8_FLOPS_PER_VOXEL * op.n_elements / t_apply / 1e12
 
     if a.copy_calibration:  # a plain copy kernel of known size for FETCH_SIZE / WRITE_SIZE scaling
         src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).uniform_()
@@ -151,7 +162,7 @@ def main():
     if not a.no_secondary:
         sec = secondary(rng, dev, a)
     if not a.no_batched:
-        sec["c3_batched_64_specimens"] = batched_specimens(dev, a.batched_steps)
+        sec["c3_batched_64_specimens"] = batched_specimens(dev, a.batched_steps, a.batch_tile, not a.no_sequential)
     if not a.no_adjoint:
         sec["c4_adjoint_gradient"] = adjoint_gradient(dev, a.adjoint_size, a.adjoint_steps, a.adjoint_fd)
 
@@ -168,13 +179,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded random displacement fluctuation, macro strain 5e-4, damage U(0,1))",
-        "config": {"workload": "hex8 periodic voxel element apply, 64^3 cube, centred pore r=12 voxels "
-                               "(configs[1])", "n_nodes": op.n_red, "n_dofs": op.n_mech,
+        "config": {"workload": "hex8 periodic voxel element apply, 64^3 cube, centred pore r=12 voxels",
+                   "n_nodes": op.n_red, "n_dofs": op.n_mech,
                    "tiles": list(op.tiles), "autotune": tune},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "fp64_tflops": tflops, "fp64_frac": tflops / 78.6,
-                     "note": "traffic from rocprofv3 FETCH_SIZE/WRITE_SIZE is recorded in profiles/"},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / FP64_PEAK_TFLOPS, "traffic": HEX8_64_TRAFFIC.get(tuple(op.tiles)),
+                     "flops_per_voxel": HEX8_FLOPS_PER_VOXEL, "hbm_algorithmic_GB_s": hbm_gbs,
+                     "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
+                     "note": "FP64-bound (16.5 FLOP/B > ridge 9.8): peak is the FP64 figure, computed on the "
+                             "vector FP64 units; traffic (bytes/launch) from rocprofv3 counters for the "
+                             "profiled launch shape, null for another"},
         "cpu_baseline": cpu,
         "secondary": sec,
         "wall_s_timed": wall,
@@ -200,13 +214,18 @@ def secondary(rng, dev, a):
     res = {}
     T = lambda z: torch.as_tensor(z, dtype=torch.float64, device=dev)
     mat = P.Material(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5)
-    for name, (nx, ny, W, H) in {"q4_c0_200x80": (200, 80, 100.0, 40.0),
-                                 "q4_8192x4096": (8192, 4096, 4096.0, 2048.0)}.items():
+    parts = set(a.secondary.split(","))
+    q4_cases = {"q4_c0_200x80": (200, 80, 100.0, 40.0), "q4_8192x4096": (8192, 4096, 4096.0, 2048.0)}
+    for name, (nx, ny, W, H) in (q4_cases.items() if "q4" in parts else ()):
         mesh = P.FEMMesh.rectangle(W, H, nx, ny)
         n, E = mesh.n_nodes, mesh.n_elems
         u = T(rng.uniform(-1e-4, 1e-4, (n, 2)))
         d = T(rng.uniform(0, 1, n))
-        best, timings, fem = autotune.tune_q4(lambda t: P.FEMOperators(mesh, mat, tile=t), u, d)
+        if a.q4_tile:
+            best = tuple(int(v) for v in a.q4_tile.split(","))
+            fem = P.FEMOperators(mesh, mat, tile=best)
+        else:
+            best, timings, fem = autotune.tune_q4(lambda t: P.FEMOperators(mesh, mat, tile=t), u, d)
         steps = max(10, a.steps // (10 if n > 1e6 else 1))
         t, _ = events_time(lambda: fem.internal_force(u, d), steps, 3)
         entry = {"apply_GDOF_s": 2 * n / t / 1e9, "apply_GB_s": Q4_APPLY_BYTES_PER_NODE * n / t / 1e9,
@@ -219,11 +238,15 @@ def secondary(rng, dev, a):
         entry.update({"explicit_step_ms": t2 * 1e3, "explicit_step_GB_s": Q4_STEP_BYTES_PER_NODE * n / t2 / 1e9})
         res[name] = entry
         del fem, dyn, u, d, v, acc, fext, Hq
-    for nv in (128, 256):
+    for nv in ((128, 256) if "hex8" in parts else ()):
         op, mats, _ = pore_cube(nv, 12 * nv // 64, dev)
         x = T(rng.uniform(-1e-6, 1e-6, op.n_mech))
         d = T(rng.uniform(0, 1, op.n_red))
-        best, _ = autotune.tune_hex8(op, x, d, reps=3)
+        if a.hex8_big_tiles:
+            best = tuple(int(v) for v in a.hex8_big_tiles.split(","))
+            op.set_tiles(*best)
+        else:
+            best, _ = autotune.tune_hex8(op, x, d, reps=3)
         t, _ = events_time(lambda: op.mech_action_d(x, d), 10, 2)
         res[f"hex8_{nv}^3"] = {"apply_GDOF_s": op.n_mech / t / 1e9,
                                "apply_GB_s": HEX8_BYTES_PER_NODE * op.n_red / t / 1e9,
@@ -275,7 +298,7 @@ def c3_batch_inputs(n_spec=64, seed=2024):
     return items, gcs, fext, fixed
 
 
-def batched_specimens(dev, steps):
+def batched_specimens(dev, steps, fixed_tile="", sequential=True):
     """c3: specimen-steps/s of the persistent batched step replayed from a captured graph,
     against the per-kernel batched path and 64 sequential single-specimen runs."""
     import numpy as np
@@ -285,21 +308,27 @@ def batched_specimens(dev, steps):
     from phast_amd import autotune
     items, gcs, fext, fixed = c3_batch_inputs()
     n_spec = len(items)
+    t_start = time.perf_counter()
+    log = lambda msg: print(f"[batched {time.perf_counter() - t_start:7.1f}s] {msg}", file=sys.stderr, flush=True)
     mk = lambda it, g, f, x, tile=(128, 8): B.BatchedExplicit(it, dt_factor=0.4, Gc_fields=g, f_ext=f, fixed=x,
                                                               bounds_method="projected_cg", tol=1e-5, max_iter=5000,
                                                               tile=tile)
     res = {"specimens": n_spec, "total_nodes": int(sum(m.n_nodes for m, _ in items)),
            "nx_range": [int(min(m.nx for m, _ in items)), int(max(m.nx for m, _ in items))]}
-    # strip shape of the persistent step: tuned on a state 100 steps into the run
-    warm = mk(items, gcs, fext, fixed)
-    warm.step_persistent(100)
-    warm.check_persistent()
-    state = warm.packed_state()
-    del warm
-    best, timings = autotune.tune_batch(lambda t: mk(items, gcs, fext, fixed, t), state=state, steps=10)
-    res["tile"] = list(best)
-    res["autotune_ms_per_step"] = {f"{k[0]},{k[1]}": round(v, 3) for k, v in timings.items()}
-    del state
+    if fixed_tile:
+        best = tuple(int(v) for v in fixed_tile.split(","))
+        res["tile"] = list(best)
+    else:  # strip shape of the persistent step: tuned on a state 100 steps into the run
+        warm = mk(items, gcs, fext, fixed)
+        warm.step_persistent(100)
+        warm.check_persistent()
+        state = warm.packed_state()
+        del warm
+        best, timings = autotune.tune_batch(lambda t: mk(items, gcs, fext, fixed, t), state=state, steps=10)
+        res["tile"] = list(best)
+        res["autotune_ms_per_step"] = {f"{k[0]},{k[1]}": round(v, 3) for k, v in timings.items()}
+        del state
+    log(f"strip {best}")
     # persistent kernel + graph over the full run
     bat = mk(items, gcs, fext, fixed, best)
     res["grid_blocks"] = list(bat.prepare_persistent())
@@ -311,8 +340,10 @@ def batched_specimens(dev, steps):
     its_max = torch.zeros((), dtype=torch.int64, device=dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(reps):
+    for rep_i in range(reps):
         g.replay()
+        if rep_i % 20 == 19:
+            log(f"replayed {(rep_i + 1) * chunk} steps")
         bad += (g.status != _lib.CG_CONVERGED).sum()
         its += g.iters.sum(0)
         its_max = torch.maximum(its_max, g.iters.max().to(torch.int64))
@@ -338,6 +369,8 @@ def batched_specimens(dev, steps):
     torch.cuda.synchronize()
     res["per_kernel_batched_specimen_steps_s"] = n_spec * short / (time.perf_counter() - t0)
     del bat
+    if not sequential:
+        return res
     # 64 sequential single-specimen runs of the per-kernel path, and parity with the batch
     seq_steps = 25
     ref = mk(items, gcs, fext, fixed)
@@ -346,6 +379,7 @@ def batched_specimens(dev, steps):
     singles = [mk([items[k]], [gcs[k]], [fext[k]], [fixed[k]]) for k in range(n_spec)]
     singles[0].step(1)  # warm the kernels, then rebuild the first specimen
     singles[0] = mk([items[0]], [gcs[0]], [fext[0]], [fixed[0]])
+    log("sequential single-specimen runs")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for one in singles:

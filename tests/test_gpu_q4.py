@@ -1,6 +1,6 @@
 """Parity of the native-Q4 HIP path with the oracle (GPU).
 
-Tolerances (``north_star``): fields within 1e-10 relative (max-norm relative to the
+Tolerances: fields within 1e-10 relative (max-norm relative to the
 field's max-norm); history maxima, masks and clamps bit-exact.
 """
 import numpy as np
