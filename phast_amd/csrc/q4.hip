@@ -111,7 +111,7 @@ __device__ __forceinline__ void cell_mech(const Q4Inst &P, const double ux[4], c
             const double ezz = P.ezz_coef * tr2;
             const double tr = tr2 + ezz;
             const double trp = fmax(tr, 0.0), trm = tr - trp;
-            const double t3 = tr / 3.0;
+            const double t3 = tr * (1.0 / 3.0);  // within one ulp of tr / 3 (no IEEE division)
             const double dxx = exx - t3, dyy = eyy - t3;
             const double kp = P.kappa * trp, km = P.kappa * trm;
             sxx[q] = g * (kp + 2.0 * P.mu * dxx) + km;
@@ -232,6 +232,13 @@ struct MechArgs {
 template <int NT>
 constexpr int kMechLds = 10 * NT;
 
+// Node-row operands of one thread's slot, fetched ahead of use.
+struct RowRegs {
+    double2 u, v, a;
+    double d;
+    uint32_t m;  // bc mask bits (x: 1, y: 2)
+};
+
 template <int NT, bool STEP, bool HIST>
 __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double *sm) {
     double(*sux)[NT] = reinterpret_cast<double(*)[NT]>(sm);
@@ -250,57 +257,59 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
     const bool cell_col = t <= NT - 2 && ie >= 0 && ie < P.nx;
     const bool bc = STEP && A.bcm != nullptr;
 
-    // Slot t of the next node row, fetched one row ahead so that its latency overlaps the
-    // current row's cell arithmetic.
-    double2 pu = make_double2(0.0, 0.0), pv = pu, pa = pu;
-    double pd = 0.0;
-    uint32_t pm = 0;
-    auto fetch = [&](int j) {
+    // Slot t of node row j, fetched two rows ahead (two register sets taking turns) so that
+    // its latency overlaps two rows of cell arithmetic.
+    auto fetch = [&](RowRegs &R, int j) {
         if (col_ok && j >= 0 && j < T.nyn && j <= T.jend) {
             const int64_t n = P.node_off + (int64_t)j * T.nxn + ie;
-            pu = A.u[n];
-            pd = A.d[n];
+            R.u = A.u[n];
+            R.d = A.d[n];
             if (STEP) {
-                pv = A.v[n];
-                pa = A.a[n];
-                if (bc) pm = (uint32_t)A.bcm[2 * n] | ((uint32_t)A.bcm[2 * n + 1] << 1);
+                R.v = A.v[n];
+                R.a = A.a[n];
+                R.m = bc ? ((uint32_t)A.bcm[2 * n] | ((uint32_t)A.bcm[2 * n + 1] << 1)) : 0u;
             }
         } else {
-            pu = pv = pa = make_double2(0.0, 0.0);
-            pd = 0.0;
-            pm = 0;
+            R.u = R.v = R.a = make_double2(0.0, 0.0);
+            R.d = 0.0;
+            R.m = 0u;
         }
     };
-    auto store = [&](int buf, int j) {
-        double ux = pu.x, uy = pu.y;
+    auto store = [&](const RowRegs &R, int buf, int j) {
+        double ux = R.u.x, uy = R.u.y;
         if (STEP) {
-            ux = (pu.x + dt * pv.x) + c2 * pa.x;  // mechanics_solver.py:160
-            uy = (pu.y + dt * pv.y) + c2 * pa.y;
-            if (pm) {
+            ux = (R.u.x + dt * R.v.x) + c2 * R.a.x;  // mechanics_solver.py:160
+            uy = (R.u.y + dt * R.v.y) + c2 * R.a.y;
+            if (R.m) {
                 const int64_t n = P.node_off + (int64_t)j * T.nxn + ie;
-                if (pm & 1u) ux = A.bcv[2 * n];
-                if (pm & 2u) uy = A.bcv[2 * n + 1];
+                if (R.m & 1u) ux = A.bcv[2 * n];
+                if (R.m & 2u) uy = A.bcv[2 * n + 1];
             }
         }
         sux[buf][t] = ux;
         suy[buf][t] = uy;
-        sd[buf][t] = pd;
+        sd[buf][t] = R.d;
     };
 
-    fetch(T.j0 - 1);
-    store(0, T.j0 - 1);
-    fetch(T.j0);
+    RowRegs X, Y;
+    X.v = X.a = Y.v = Y.a = make_double2(0.0, 0.0);
+    X.m = Y.m = 0u;
+    fetch(X, T.j0 - 1);
+    store(X, 0, T.j0 - 1);
+    fetch(X, T.j0);
+    fetch(Y, T.j0 + 1);
     double2 gv = make_double2(0.0, 0.0), ga = gv, nv = gv, na = gv;  // v, a of rows jr and jr+1
     uint32_t gm = 0, nm = 0;
     double accx = 0.0, accy = 0.0;
     int cur = 0;
-    for (int jr = T.j0 - 1; jr < T.jend; ++jr) {
+    // one cell row: R holds node row jr+1 on entry and row jr+3 on exit
+    auto row = [&](int jr, RowRegs &R) {
         const int up = cur ^ 1;
-        store(up, jr + 1);
+        store(R, up, jr + 1);
         if (STEP) {
-            nv = pv;
-            na = pa;
-            nm = pm;
+            nv = R.v;
+            na = R.a;
+            nm = R.m;
         }
         // operands consumed later in this row, issued before the next prefetch so that
         // waiting for them leaves the prefetch in flight
@@ -320,7 +329,7 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
             h01 = reinterpret_cast<const double2 *>(A.Hold)[2 * c];
             h23 = reinterpret_cast<const double2 *>(A.Hold)[2 * c + 1];
         }
-        fetch(jr + 2);
+        fetch(R, jr + 3);
         __syncthreads();
         double fx[4] = {0.0, 0.0, 0.0, 0.0}, fy[4] = {0.0, 0.0, 0.0, 0.0};
         if (cell_ok) {
@@ -376,6 +385,10 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
             gm = nm;
         }
         cur = up;
+    };
+    for (int jr = T.j0 - 1; jr < T.jend; jr += 2) {
+        row(jr, X);
+        if (jr + 1 < T.jend) row(jr + 1, Y);
     }
 }
 
@@ -443,7 +456,8 @@ __global__ __launch_bounds__(256) void q4_cell_kernel(CellArgs A) {
             if (P.split == PHAST_SPLIT_AMOR) {
                 const double tr2 = exx + eyy, ezz = P.ezz_coef * tr2, tr = tr2 + ezz;
                 const double trp = fmax(tr, 0.0);
-                const double dxx = exx - tr / 3.0, dyy = eyy - tr / 3.0, dzz = ezz - tr / 3.0;
+                const double t3 = tr * (1.0 / 3.0);
+                const double dxx = exx - t3, dyy = eyy - t3, dzz = ezz - t3;
                 ps = 0.5 * P.kappa * trp * trp + P.mu * (dxx * dxx + dyy * dyy + dzz * dzz + 2.0 * exy * exy);
             } else {
                 const double t = exx + eyy;
