@@ -162,9 +162,16 @@ def main():
     if not a.no_secondary:
         sec = secondary(rng, dev, a)
     if not a.no_batched:
-        sec["c3_batched_64_specimens"] = batched_specimens(dev, a.batched_steps, a.batch_tile, not a.no_sequential)
+        try:
+            sec["c3_batched_64_specimens"] = batched_specimens(dev, a.batched_steps, a.batch_tile,
+                                                               not a.no_sequential)
+        except Exception as exc:  # reported, so that the headline line is still printed
+            sec["c3_batched_64_specimens"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not a.no_adjoint:
-        sec["c4_adjoint_gradient"] = adjoint_gradient(dev, a.adjoint_size, a.adjoint_steps, a.adjoint_fd)
+        try:
+            sec["c4_adjoint_gradient"] = adjoint_gradient(dev, a.adjoint_size, a.adjoint_steps, a.adjoint_fd)
+        except Exception as exc:
+            sec["c4_adjoint_gradient"] = {"error": f"{type(exc).__name__}: {exc}"}
 
     out = {
         "metric": "element-apply GDOF/s (periodic hex8 voxel mech_action, 64^3 pore cube)",

@@ -6,8 +6,8 @@ import torch
 
 # (16,16) and (32,8): voxel-parallel kernel with z slabs of kz layers; (16,8) and (8,8):
 # node-column marching kernel walking kz node layers.
-HEX8_TILES = [(16, 16, 1), (16, 16, 2), (16, 16, 4), (16, 16, 8), (16, 16, 16), (32, 8, 1), (32, 8, 2), (32, 8, 4),
-              (32, 8, 8), (32, 8, 16), (16, 8, 4), (16, 8, 8), (16, 8, 16), (8, 8, 8)]
+HEX8_TILES = [(16, 16, 2), (16, 16, 4), (16, 16, 8), (16, 16, 16), (32, 8, 2), (32, 8, 4), (32, 8, 8), (32, 8, 16),
+              (16, 8, 8), (8, 8, 8)]
 Q4_TILES = [(128, 8), (128, 32), (256, 16), (256, 64), (64, 32), (256, 128)]
 BATCH_TILES = [(64, 4), (64, 8), (128, 2), (128, 4), (128, 8), (256, 2), (256, 4), (256, 8), (256, 16)]
 
@@ -27,7 +27,7 @@ def _time(fn, reps: int) -> float:
 
 
 def tune_hex8(op, x: torch.Tensor, d: torch.Tensor, reps: int = 10, candidates=HEX8_TILES):
-    """Pick the marching-kernel tile (tx, ty, kz) of ``op`` for the fused apply."""
+    """Pick the launch shape (tx, ty, kz) of ``op`` for the fused apply."""
     key = ("hex8", op.grid.shape)
     if key in _cache:
         op.set_tiles(*_cache[key][0])

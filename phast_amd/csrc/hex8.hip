@@ -355,6 +355,20 @@ __global__ __launch_bounds__(320) void hex8_apply_kernel(const Hex8Geo G, MechAr
     double sE[6] = {0, 0, 0, 0, 0, 0}, en = 0.0;
     double acc[3] = {0.0, 0.0, 0.0};
     const int s00 = t - NVX - 1, s10 = t - NVX, s01 = t - 1;  // voxels left-below, below, left
+    // voxel material of layer k (lam, mu or the bank index), fetched one layer ahead
+    double ml0 = 0.0, ml1 = 0.0;
+    int32_t mb = 0;
+    auto fetch_mat = [&](int k, double &l0, double &l1, int32_t &b) {
+        if (!has_voxel) return;
+        const int64_t e = e_ij + nxy * wrap(k, nz);
+        if (ISO) {
+            l0 = A.lam[e];
+            l1 = A.mu[e];
+        } else {
+            b = A.bank_index[e];
+        }
+    };
+    fetch_mat(k0 - 1, ml0, ml1, mb);
     fetch(k0 - 1);
     stage(0);
     fetch(k0);
@@ -363,6 +377,9 @@ __global__ __launch_bounds__(320) void hex8_apply_kernel(const Hex8Geo G, MechAr
     for (int kv = k0 - 1; kv < k1; ++kv) {
         __syncthreads();
         if (kv + 2 <= k1) fetch(kv + 2);  // lands in su[cur] once this layer's voxels are done
+        double nl0 = 0.0, nl1 = 0.0;
+        int32_t nb = 0;
+        if (kv + 1 < k1) fetch_mat(kv + 1, nl0, nl1, nb);
         const bool first = kv < k0;  // carry-only layer
         double r[3][8];
         if (has_voxel) {
@@ -399,10 +416,9 @@ __global__ __launch_bounds__(320) void hex8_apply_kernel(const Hex8Geo G, MechAr
             }
             const bool own = owner && !first;
             if (ISO) {
-                voxel_mech<true, ENERGY>(G, u, E, g, nullptr, A.lam[e], A.mu[e], own, r, sE, en);
+                voxel_mech<true, ENERGY>(G, u, E, g, nullptr, ml0, ml1, own, r, sE, en);
             } else {
-                voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)A.bank_index[e], 0.0, 0.0, own, r,
-                                          sE, en);
+                voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)mb, 0.0, 0.0, own, r, sE, en);
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -424,6 +440,9 @@ __global__ __launch_bounds__(320) void hex8_apply_kernel(const Hex8Geo G, MechAr
             }
         }
         if (kv + 2 <= k1) stage(cur);  // node layer kv+2 replaces layer kv
+        ml0 = nl0;
+        ml1 = nl1;
+        mb = nb;
         cur ^= 1;
     }
 #pragma unroll

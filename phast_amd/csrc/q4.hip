@@ -1784,7 +1784,9 @@ extern "C" int phast_q4_batch_prepare(phast_q4_plan *pl, int32_t *grid_out) {
             set_error("occupancy query failed for the batched step kernel (strip width %d)", pl->tx);
             return rc;
         }
-        const int64_t gmax = (int64_t)bpc * cus;
+        // 7/8 of the occupancy bound: the grid stays co-resident even when a few CUs are
+        // unavailable to this queue
+        const int64_t gmax = std::max<int64_t>((int64_t)bpc * cus * 7 / 8, 1);
         PHAST_REQUIRE(gmax >= pl->n_inst, "%d instances exceed the %lld co-resident blocks of the batched step",
                       pl->n_inst, (long long)gmax);
         // blocks per instance in proportion to its strips, at least one, at most one per strip

@@ -278,6 +278,7 @@ class PeriodicVoxelPhaseFieldOperator:
             self._plan = None
 
     def set_tiles(self, tx: int, ty: int, kz: int) -> None:
+        """Launch shape of the apply: (16,16) / (32,8) voxel-parallel, (16,8) / (8,8) marching."""
         check(_L.phast_hex8_set_tiles(self._plan, int(tx), int(ty), int(kz)), "set_tiles")
         self.tiles = (int(tx), int(ty), int(kz))
 

@@ -61,10 +61,11 @@ def test_mech_action(dev, shape, tiles):
     assert rel(op.mech_action(T(x, dev)), oop.mech_action(x)) < RTOL
 
 
-@pytest.mark.parametrize("tiles", [(16, 16, 2), (32, 8, 3), (16, 16, 7)])
+@pytest.mark.parametrize("tiles", [(16, 16, 2), (32, 8, 3), (16, 16, 7), (32, 8, 1)])
 def test_voxel_parallel_apply_matches_marching_apply(dev, tiles):
-    """Same per-voxel arithmetic and node summation order: the nodal rows agree bitwise;
-    the macro rows and energy (block partials in another order) to rounding."""
+    """Same per-voxel arithmetic and node summation order as the marching kernel: the
+    nodal rows agree bitwise; the macro rows and energy (block partials in another order)
+    to rounding."""
     S, op, oop = build((20, 19, 14), seed=5)
     rng = np.random.default_rng(6)
     x = T(rng.uniform(-1e-6, 1e-6, op.n_mech), dev)
