@@ -293,9 +293,15 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
 // the marching kernel, face by face (bottom face, then top face).  The top-face sum of
 // voxel layer kv is carried in registers to node layer kv+1; the slab starts one voxel
 // layer early (k0-1) to form the carry into its first node layer, so blocks never talk.
+// threads of the voxel-parallel block: the voxel tile rounded up to whole waves
+template <int OX, int OY>
+constexpr int apply_threads() {
+    return ((OX + 1) * (OY + 1) + 63) / 64 * 64;
+}
+
 template <int OX, int OY, int GMODE, bool ISO, bool ENERGY>
-__global__ __launch_bounds__(320) void hex8_apply_kernel(const Hex8Geo G, MechArgs3 A) {
-    constexpr int NT = 320;
+__global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(const Hex8Geo G, MechArgs3 A) {
+    constexpr int NT = apply_threads<OX, OY>();
     constexpr int NVX = OX + 1, NVY = OY + 1, NV = NVX * NVY;
     constexpr int NNX = OX + 2, NNY = OY + 2, NN = NNX * NNY;
     constexpr int NF = (GMODE == 2) ? 4 : 3;
@@ -340,7 +346,7 @@ __global__ __launch_bounds__(320) void hex8_apply_kernel(const Hex8Geo G, MechAr
             pf[m][0] = A.x[3 * node];
             pf[m][1] = A.x[3 * node + 1];
             pf[m][2] = A.x[3 * node + 2];
-            if (GMODE == 2) pf[m][3] = A.d[node];
+            if (GMODE == 2) pf[m][NF - 1] = A.d[node];
         }
     };
     auto stage = [&](int buf) {
@@ -785,7 +791,7 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
         set_error("hex8 apply needs %d blocks, plan holds partials for %d", nb, pl->max_blocks);
         return PHAST_ERR_ARG;
     }
-    dim3 grid(nb), block(320);
+    dim3 grid(nb), block(apply_threads<OX, OY>());
 #define PHAST_L(GM, ISO, EN) hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, GM, ISO, EN>), grid, block, 0, st, pl->geo, A)
     if (pl->iso) {
         if (energy) {
@@ -839,6 +845,8 @@ int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g,
     }
     if (pl->tx == 16 && pl->ty == 16) rc = launch_apply_t<16, 16>(pl, A, gmode, energy, st);
     else if (pl->tx == 32 && pl->ty == 8) rc = launch_apply_t<32, 8>(pl, A, gmode, energy, st);
+    else if (pl->tx == 16 && pl->ty == 14) rc = launch_apply_t<16, 14>(pl, A, gmode, energy, st);
+    else if (pl->tx == 32 && pl->ty == 14) rc = launch_apply_t<32, 14>(pl, A, gmode, energy, st);
     else if (pl->tx == 16 && pl->ty == 8) rc = launch_mech_t<16, 8>(pl, A, gmode, energy, st, &nb);
     else if (pl->tx == 8 && pl->ty == 8) rc = launch_mech_t<8, 8>(pl, A, gmode, energy, st, &nb);
     else {
@@ -992,7 +1000,8 @@ extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
 
 extern "C" int phast_hex8_set_tiles(phast_hex8_plan *pl, int32_t tx, int32_t ty, int32_t kz) {
     PHAST_REQUIRE(pl && kz >= 1, "phast_hex8_set_tiles: bad argument");
-    PHAST_REQUIRE((tx == 16 && ty == 16) || (tx == 16 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8),
+    PHAST_REQUIRE((tx == 16 && ty == 16) || (tx == 16 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8) ||
+                      (tx == 16 && ty == 14) || (tx == 32 && ty == 14),
                   "unsupported hex8 tile %dx%d", tx, ty);
     pl->tx = tx;
     pl->ty = ty;
@@ -1065,6 +1074,7 @@ extern "C" int phast_hex8_damage_action(const phast_hex8_plan *pl, const double 
     if (pl->tx == 16 && pl->ty == 8) return launch_damage_t<16, 8>(pl, A, st);
     if (pl->tx == 8 && pl->ty == 8) return launch_damage_t<8, 8>(pl, A, st);
     if (pl->tx == 32 && pl->ty == 8) return launch_damage_t<32, 8>(pl, A, st);
+    if (pl->ty == 14) return launch_damage_t<16, 16>(pl, A, st);  // damage pass: any column tile
     set_error("unsupported hex8 tile");
     return PHAST_ERR_UNSUPPORTED;
 }

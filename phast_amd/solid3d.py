@@ -236,7 +236,7 @@ class PeriodicVoxelPhaseFieldOperator:
     """
 
     def __init__(self, grid: VoxelGrid, materials: ElementMaterialArrays, load: LoadCase, *, kappa: float = 1e-6,
-                 ell: float, pinned_node: int | None = None, device=None, tiles=(32, 8, 8)):
+                 ell: float, pinned_node: int | None = None, device=None, tiles=(16, 14, 8)):
         self.device = _lib.require_device() if device is None else torch.device(device)
         self.grid, self.mat, self.load = grid, materials, load
         self.dtype = F64
