@@ -31,7 +31,7 @@ HEX8_BYTES_PER_NODE = 57.0      # u 24 + d 8 + phase 1 + r 24
 HEX8_FLOPS_PER_VOXEL = 940.0    # sum-factorised apply
 # HBM bytes per launch of the 64^3 apply measured with rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE
 # x 1.0 KiB, copy-calibrated), per launch shape; see profiles/hex8_apply_64cube_pore.md
-HEX8_64_TRAFFIC = {(32, 8, 4): (11522.0 * 2.0 + 6176.4) * 1024.0}
+HEX8_64_TRAFFIC = {(16, 14, 8): (16164.69 * 2.0 + 6164.15) * 1024.0, (32, 8, 4): (11522.0 * 2.0 + 6176.4) * 1024.0}
 Q4_APPLY_BYTES_PER_NODE = 40.0  # u 16 + d 8 + f 16
 Q4_STEP_BYTES_PER_NODE = 192.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, H r+w 64
 
