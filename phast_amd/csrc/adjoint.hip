@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void qp_to_node_kernel(const Hex8Geo G, const 
 // r = sum_e sum_q B_q^T S_q (3n rows, gathered per node) and partial sums of sum_q S_q
 __global__ __launch_bounds__(256) void qp_stress_apply_kernel(const Hex8Geo G, const double *S, double *out,
                                                               double *part) {
-    __shared__ double red[4];
+    __shared__ double red[24];
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
     const int64_t node = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double sE[6] = {0, 0, 0, 0, 0, 0};
@@ -221,20 +221,19 @@ __global__ __launch_bounds__(256) void qp_stress_apply_kernel(const Hex8Geo G, c
         for (int q = 0; q < 8; ++q)
             for (int c = 0; c < 6; ++c) sE[c] += S[(node * 8 + q) * 6 + c];
     }
-    for (int c = 0; c < 6; ++c) {
-        const double t = block_sum<256>(sE[c], red);
-        if (threadIdx.x == 0) part[8 * blockIdx.x + c] = t;
-    }
+    block_sum_n<256, 6>(sE, red);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 6; ++c) part[8 * blockIdx.x + c] = sE[c];
 }
 
 __global__ __launch_bounds__(256) void sum_parts6_kernel(const double *part, int nblocks, double *out6) {
-    __shared__ double red[4];
-    for (int c = 0; c < 6; ++c) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += 256) s += part[8 * b + c];
-        s = block_sum<256>(s, red);
-        if (threadIdx.x == 0) out6[c] = s;
-    }
+    __shared__ double red[24];
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += 256)
+        for (int c = 0; c < 6; ++c) s[c] += part[8 * b + c];
+    block_sum_n<256, 6>(s, red);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 6; ++c) out6[c] = s[c];
 }
 
 // out[e,q] = scale ch w (N_q.d_e - 1)(N_q.l_e)

@@ -73,6 +73,30 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     for (int i = 0; i < NT / kWave; ++i) s += sh[i];
     return s;
 }
+// K block sums at once, each with exactly block_sum's arithmetic (same bits), sharing one
+// pair of barriers; the K butterflies interleave so their shuffle latencies overlap.
+// `sh` holds K * NT/64 doubles.  v[k] is replaced by its block sum in every thread.
+template <int NT, int K>
+__device__ __forceinline__ void block_sum_n(double v[K], double *sh) {
+    constexpr int NW = NT / kWave;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], o, kWave);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) sh[k * NW + w] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) s += sh[k * NW + i];
+        v[k] = s;
+    }
+}
 template <int NT>
 __device__ __forceinline__ double block_min(double v, double *sh) {
     v = wave_min(v);

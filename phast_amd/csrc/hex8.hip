@@ -33,13 +33,8 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
     double gx[3][2][2], gy[3][2][2], gz[3][2][2];
     voxel_grad(G, u, gx, gy, gz);
     fold_macro(E, gx, gy, gz);
+    // each partial sum takes two Gauss points: the first assigns, the second adds
     double Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) Sx[c][a][b] = Sy[c][a][b] = Sz[c][a][b] = 0.0;
     double ew = 0.0;
 #pragma unroll
     for (int iz = 0; iz < 2; ++iz)
@@ -75,15 +70,33 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
                     for (int k = 0; k < 6; ++k) t += s[k] * e[k];
                     ew += t;
                 }
-                Sx[0][iy][iz] += s[0];
-                Sx[1][iy][iz] += s[5];
-                Sx[2][iy][iz] += s[4];
-                Sy[0][ix][iz] += s[5];
-                Sy[1][ix][iz] += s[1];
-                Sy[2][ix][iz] += s[3];
-                Sz[0][ix][iy] += s[4];
-                Sz[1][ix][iy] += s[3];
-                Sz[2][ix][iy] += s[2];
+                if (ix == 0) {
+                    Sx[0][iy][iz] = s[0];
+                    Sx[1][iy][iz] = s[5];
+                    Sx[2][iy][iz] = s[4];
+                } else {
+                    Sx[0][iy][iz] += s[0];
+                    Sx[1][iy][iz] += s[5];
+                    Sx[2][iy][iz] += s[4];
+                }
+                if (iy == 0) {
+                    Sy[0][ix][iz] = s[5];
+                    Sy[1][ix][iz] = s[1];
+                    Sy[2][ix][iz] = s[3];
+                } else {
+                    Sy[0][ix][iz] += s[5];
+                    Sy[1][ix][iz] += s[1];
+                    Sy[2][ix][iz] += s[3];
+                }
+                if (iz == 0) {
+                    Sz[0][ix][iy] = s[4];
+                    Sz[1][ix][iy] = s[3];
+                    Sz[2][ix][iy] = s[2];
+                } else {
+                    Sz[0][ix][iy] += s[4];
+                    Sz[1][ix][iy] += s[3];
+                    Sz[2][ix][iy] += s[2];
+                }
             }
     if (own) {  // sum_q w sigma_q from the partial sums (each component appears once per channel)
         sE[0] += (Sx[0][0][0] + Sx[0][1][0]) + (Sx[0][0][1] + Sx[0][1][1]);
@@ -139,6 +152,44 @@ struct MechArgs3 {
     int32_t kz, tiles_x, tiles_y, nvals;
 };
 
+// End of a mechanics apply: the block's macro-row (and energy) partials, then the last
+// block to finish sums every block's partials in block order (a fixed order, so the macro
+// rows are deterministic) and resets the counter.  All partials of a block are reduced
+// together and the last block reads them in one pass.  `red` holds 7 * NT/64 doubles.
+template <int NT, bool ENERGY>
+__device__ __forceinline__ void macro_epilogue(const MechArgs3 &A, const double sE[6], double en, double *red) {
+    constexpr int K = ENERGY ? 7 : 6;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = sE[k];
+    if (ENERGY) v[K - 1] = en;
+    block_sum_n<NT, K>(v, red);
+    __shared__ unsigned int is_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) A.part[8 * blockIdx.x + k] = v[k];
+        __threadfence();  // the partials (written by this thread only) before the count
+        is_last = (atomicAdd(A.counter, 1u) == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += NT)
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                if (k < A.nvals) s[k] += __builtin_nontemporal_load(&A.part[8 * b + k]);
+        block_sum_n<NT, 7>(s, red);
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < A.nvals; ++k) {
+                if (A.y6 && k < 6) A.y6[k] = s[k];
+                if (A.out7) A.out7[k] = s[k];
+            }
+            atomicExch(A.counter, 0u);
+        }
+    }
+}
+
 template <int TX, int TY, int GMODE, bool ISO, bool ENERGY>  // GMODE 0: g=1, 1: g array, 2: g from d
 __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, MechArgs3 A) {
     constexpr int NT = TX * TY;
@@ -147,7 +198,7 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
     constexpr int NF = (GMODE == 2) ? 4 : 3;
     __shared__ double su[2][NF][NH];
     __shared__ double sr[24][NV];
-    __shared__ double red[NT / 64];
+    __shared__ double red[7 * (NT / 64)];
 
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const int bx = blockIdx.x % A.tiles_x, by = (blockIdx.x / A.tiles_x) % A.tiles_y;
@@ -250,34 +301,7 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
         }
         cur ^= 1;
     }
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const double t = block_sum<NT>(sE[k], red);
-        if (threadIdx.x == 0) A.part[8 * blockIdx.x + k] = t;
-    }
-    if (ENERGY) {
-        const double t = block_sum<NT>(en, red);
-        if (threadIdx.x == 0) A.part[8 * blockIdx.x + 6] = t;
-    }
-    // last-block-done: the block that finishes last sums every block's partials in block
-    // order (a fixed order, so the macro rows are deterministic) and resets the counter.
-    __shared__ unsigned int is_last;
-    __threadfence();
-    if (threadIdx.x == 0) is_last = (atomicAdd(A.counter, 1u) == gridDim.x - 1) ? 1u : 0u;
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        for (int k = 0; k < A.nvals; ++k) {
-            double s = 0.0;
-            for (int b = threadIdx.x; b < (int)gridDim.x; b += NT) s += __builtin_nontemporal_load(&A.part[8 * b + k]);
-            s = block_sum<NT>(s, red);
-            if (threadIdx.x == 0) {
-                if (A.y6 && k < 6) A.y6[k] = s;
-                if (A.out7) A.out7[k] = s;
-            }
-        }
-        if (threadIdx.x == 0) atomicExch(A.counter, 0u);
-    }
+    macro_epilogue<NT, ENERGY>(A, sE, en, red);
 }
 
 
@@ -309,7 +333,7 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
     static_assert(NV <= NT, "voxel tile larger than the block");
     __shared__ double su[2][NF][NN];
     __shared__ double st[18][NV];  // corners (1,0), (0,1), (1,1) of both faces, 3 components each
-    __shared__ double red[NT / 64];
+    __shared__ double red[7 * (NT / 64)];
 
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const int tiles = A.tiles_x * A.tiles_y;
@@ -337,24 +361,24 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
     }
     // node layer k of this thread's slots, fetched into registers (issued one layer ahead)
     double pf[NS][NF];
-    auto fetch = [&](int k) {
+    auto fetch = [&](int k, double (&dst)[NS][NF]) {
         const int64_t base = nxy * wrap(k, nz);
 #pragma unroll
         for (int m = 0; m < NS; ++m) {
             if (slot_ij[m] < 0) continue;
             const int64_t node = base + slot_ij[m];
-            pf[m][0] = A.x[3 * node];
-            pf[m][1] = A.x[3 * node + 1];
-            pf[m][2] = A.x[3 * node + 2];
-            if (GMODE == 2) pf[m][NF - 1] = A.d[node];
+            dst[m][0] = A.x[3 * node];
+            dst[m][1] = A.x[3 * node + 1];
+            dst[m][2] = A.x[3 * node + 2];
+            if (GMODE == 2) dst[m][NF - 1] = A.d[node];
         }
     };
-    auto stage = [&](int buf) {
+    auto stage = [&](int buf, const double (&src)[NS][NF]) {
 #pragma unroll
         for (int m = 0; m < NS; ++m) {
             if (slot_ij[m] < 0) continue;
 #pragma unroll
-            for (int f = 0; f < NF; ++f) su[buf][f][t + m * NT] = pf[m][f];
+            for (int f = 0; f < NF; ++f) su[buf][f][t + m * NT] = src[m][f];
         }
     };
 
@@ -375,14 +399,17 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
         }
     };
     fetch_mat(k0 - 1, ml0, ml1, mb);
-    fetch(k0 - 1);
-    stage(0);
-    fetch(k0);
-    stage(1);
+    {  // the first two node layers, both in flight before either is staged
+        double pf1[NS][NF];
+        fetch(k0 - 1, pf);
+        fetch(k0, pf1);
+        stage(0, pf);
+        stage(1, pf1);
+    }
     int cur = 0;
     for (int kv = k0 - 1; kv < k1; ++kv) {
         __syncthreads();
-        if (kv + 2 <= k1) fetch(kv + 2);  // lands in su[cur] once this layer's voxels are done
+        if (kv + 2 <= k1) fetch(kv + 2, pf);  // lands in su[cur] once this layer's voxels are done
         double nl0 = 0.0, nl1 = 0.0;
         int32_t nb = 0;
         if (kv + 1 < k1) fetch_mat(kv + 1, nl0, nl1, nb);
@@ -445,39 +472,13 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
                 acc[c] = ((st[15 + c][s00] + st[12 + c][s10]) + st[9 + c][s01]) + r[c][4];
             }
         }
-        if (kv + 2 <= k1) stage(cur);  // node layer kv+2 replaces layer kv
+        if (kv + 2 <= k1) stage(cur, pf);  // node layer kv+2 replaces layer kv
         ml0 = nl0;
         ml1 = nl1;
         mb = nb;
         cur ^= 1;
     }
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const double tt = block_sum<NT>(sE[k], red);
-        if (threadIdx.x == 0) A.part[8 * blockIdx.x + k] = tt;
-    }
-    if (ENERGY) {
-        const double tt = block_sum<NT>(en, red);
-        if (threadIdx.x == 0) A.part[8 * blockIdx.x + 6] = tt;
-    }
-    // last-block-done: the last block sums every block's partials in block order
-    __shared__ unsigned int is_last;
-    __threadfence();
-    if (threadIdx.x == 0) is_last = (atomicAdd(A.counter, 1u) == gridDim.x - 1) ? 1u : 0u;
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        for (int k = 0; k < A.nvals; ++k) {
-            double s = 0.0;
-            for (int b = threadIdx.x; b < (int)gridDim.x; b += NT) s += __builtin_nontemporal_load(&A.part[8 * b + k]);
-            s = block_sum<NT>(s, red);
-            if (threadIdx.x == 0) {
-                if (A.y6 && k < 6) A.y6[k] = s;
-                if (A.out7) A.out7[k] = s;
-            }
-        }
-        if (threadIdx.x == 0) atomicExch(A.counter, 0u);
-    }
+    macro_epilogue<NT, ENERGY>(A, sE, en, red);
 }
 
 // Sum the block partials of the macro rows in a fixed order; write into y[3n + k] and/or out7.

@@ -27,29 +27,37 @@ struct PcgState {
 
 enum : int { PS_RUNNING = 0, PS_CONVERGED = 1, PS_NOT_PD = 2, PS_NONPOS_RZ = 3, PS_NONFINITE = 4, PS_MAXITER = 5 };
 
+// Block partials of up to 5 sums, reduced together (block_sum's arithmetic per value).
+// `red` holds kRed doubles.
+constexpr int kRed = 5 * 4;
 __device__ __forceinline__ void write_parts(double *part, double a, double b, double c, double *red) {
-    const double sa = block_sum<256>(a, red);
-    const double sb = block_sum<256>(b, red);
-    const double sc = block_sum<256>(c, red);
+    double v[3] = {a, b, c};
+    block_sum_n<256, 3>(v, red);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = sa;
-        part[kVecBlocks + blockIdx.x] = sb;
-        part[2 * kVecBlocks + blockIdx.x] = sc;
+        part[blockIdx.x] = v[0];
+        part[kVecBlocks + blockIdx.x] = v[1];
+        part[2 * kVecBlocks + blockIdx.x] = v[2];
     }
 }
 
 __device__ __forceinline__ void write_parts5(double *part, const double v[5], double *red) {
+    double t[5] = {v[0], v[1], v[2], v[3], v[4]};
+    block_sum_n<256, 5>(t, red);
+    if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const double t = block_sum<256>(v[k], red);
-        if (threadIdx.x == 0) part[k * kVecBlocks + blockIdx.x] = t;
-    }
+        for (int k = 0; k < 5; ++k) part[k * kVecBlocks + blockIdx.x] = t[k];
 }
 
-__device__ __forceinline__ double sum_parts(const double *part, int k, double *red) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < kVecBlocks; i += 256) s += part[k * kVecBlocks + i];
-    return block_sum<256>(s, red);
+// Sums of partial rows k0 .. k0+K-1 (each in block order, as one block_sum per row), with
+// every row's loads in flight together.
+template <int K>
+__device__ __forceinline__ void sum_parts_n(const double *part, int k0, double out[K], double *red) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = 0.0;
+    for (int i = threadIdx.x; i < kVecBlocks; i += 256)
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] += part[(k0 + k) * kVecBlocks + i];
+    block_sum_n<256, K>(out, red);
 }
 
 #define GRID_STRIDE(i, n) for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < (n); i += (int64_t)kVecBlocks * 256)
@@ -57,7 +65,7 @@ __device__ __forceinline__ double sum_parts(const double *part, int k, double *r
 // y <- free ? y : 0 ; partials: x.y, count(!finite(y))
 __global__ __launch_bounds__(256) void k_mask_dot(int64_t n, const double *x, double *y, const uint8_t *fixed,
                                                   const PcgState *S, double *part) {
-    __shared__ double red[4];
+    __shared__ double red[kRed];
     if (S && S->done) return;
     double s = 0.0, bad = 0.0;
     GRID_STRIDE(i, n) {
@@ -74,7 +82,7 @@ __global__ __launch_bounds__(256) void k_mask_dot(int64_t n, const double *x, do
 
 // generic partials: x.x, x.y, y.y over masked vectors
 __global__ __launch_bounds__(256) void k_dots(int64_t n, const double *x, const double *y, double *part) {
-    __shared__ double red[4];
+    __shared__ double red[kRed];
     double a = 0.0, b = 0.0, c = 0.0, bad = 0.0;
     GRID_STRIDE(i, n) {
         const double xv = x[i];
@@ -96,7 +104,7 @@ __global__ __launch_bounds__(256) void k_dots(int64_t n, const double *x, const 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_update(int64_t n, double *x, double *r, const double *p, const double *Ap,
                                                 const double *dg, double *z, const PcgState *S, double *part) {
-    __shared__ double red[4];
+    __shared__ double red[kRed];
     if (S->done) return;
     const double a = S->alpha;
     double rz = 0.0, rr = 0.0;
@@ -122,9 +130,10 @@ __global__ __launch_bounds__(256) void k_pupdate(int64_t n, double *p, const dou
 
 // linalg.pcg scalar steps
 __global__ __launch_bounds__(256) void k_scalar_flags_a(PcgState *S, const double *part) {
-    __shared__ double red[4];
-    const double pAp = sum_parts(part, 0, red);
-    const double bad = sum_parts(part, 1, red);
+    __shared__ double red[kRed];
+    double t[2];
+    sum_parts_n<2>(part, 0, t, red);
+    const double pAp = t[0], bad = t[1];
     if (threadIdx.x) return;
     const bool positive = pAp > 0.0;
     if (S->active && !positive) S->breakdown = 1;
@@ -136,9 +145,10 @@ __global__ __launch_bounds__(256) void k_scalar_flags_a(PcgState *S, const doubl
     S->iter += 1;
 }
 __global__ __launch_bounds__(256) void k_scalar_flags_b(PcgState *S, const double *part) {
-    __shared__ double red[4];
-    const double rz_new = sum_parts(part, 0, red);
-    const double rr = sum_parts(part, 1, red);
+    __shared__ double red[kRed];
+    double t[2];
+    sum_parts_n<2>(part, 0, t, red);
+    const double rz_new = t[0], rr = t[1];
     if (threadIdx.x) return;
     if (!isfinite(rz_new) || !isfinite(rr)) S->nonfinite = 1;
     S->rr = rr;
@@ -149,10 +159,11 @@ __global__ __launch_bounds__(256) void k_scalar_flags_b(PcgState *S, const doubl
 
 // box_kkt_h8._pcg scalar steps (stop at first convergence; breakdowns end the loop)
 __global__ __launch_bounds__(256) void k_scalar_stop_a(PcgState *S, const double *part) {
-    __shared__ double red[4];
+    __shared__ double red[kRed];
     if (S->done) return;
-    const double den = sum_parts(part, 0, red);
-    const double bad = sum_parts(part, 1, red);
+    double t[2];
+    sum_parts_n<2>(part, 0, t, red);
+    const double den = t[0], bad = t[1];
     if (threadIdx.x) return;
     S->iter += 1;
     if (bad > 0.0) {
@@ -169,10 +180,11 @@ __global__ __launch_bounds__(256) void k_scalar_stop_a(PcgState *S, const double
     S->alpha = S->rz / den;
 }
 __global__ __launch_bounds__(256) void k_scalar_stop_b(PcgState *S, const double *part, int32_t max_iter) {
-    __shared__ double red[4];
+    __shared__ double red[kRed];
     if (S->done) return;
-    const double rz_new = sum_parts(part, 0, red);
-    const double rr = sum_parts(part, 1, red);
+    double t[2];
+    sum_parts_n<2>(part, 0, t, red);
+    const double rz_new = t[0], rr = t[1];
     if (threadIdx.x) return;
     S->rr = rr;
     if (sqrt(rr) <= S->thr2) {  // thr2 holds tol * reference_norm here
@@ -196,7 +208,7 @@ __global__ __launch_bounds__(256) void k_scalar_stop_b(PcgState *S, const double
 // partials r.z, r.r, count(!finite(z)) after an external preconditioner
 __global__ __launch_bounds__(256) void k_rz(int64_t n, const double *r, const double *z, const PcgState *S,
                                             double *part) {
-    __shared__ double red[4];
+    __shared__ double red[kRed];
     if (S && S->done) return;
     double rz = 0.0, rr = 0.0, bad = 0.0;
     GRID_STRIDE(i, n) {
@@ -229,7 +241,7 @@ __global__ __launch_bounds__(256) void k_invert(int64_t n, const double *d, doub
     GRID_STRIDE(i, n) out[i] = 1.0 / d[i];
 }
 __global__ __launch_bounds__(256) void k_diag_stats(int64_t n, const double *dg, const double *b, double *part) {
-    __shared__ double red[4];
+    __shared__ double red[kRed];
     double nonpos = 0.0, bad = 0.0, bb = 0.0;
     GRID_STRIDE(i, n) {
         const double v = dg[i];
@@ -244,11 +256,11 @@ __global__ __launch_bounds__(256) void k_diag_stats(int64_t n, const double *dg,
     write_parts(part, nonpos, bad, bb, red);
 }
 __global__ __launch_bounds__(256) void k_final_parts(const double *part, double *out3, int nv) {
-    __shared__ double red[4];
-    for (int k = 0; k < nv; ++k) {
-        const double s = sum_parts(part, k, red);
-        if (threadIdx.x == 0) out3[k] = s;
-    }
+    __shared__ double red[kRed];
+    double t[5];  // nv <= 5 (checked by the host)
+    sum_parts_n<5>(part, 0, t, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nv; ++k) out3[k] = t[k];
 }
 
 // box-KKT active-set bookkeeping (box_kkt_h8.py:672-685, 731-733):
@@ -258,7 +270,7 @@ __global__ __launch_bounds__(256) void k_box_update(int64_t n, const double *Aca
                                                     const double *lo, const double *up, uint8_t *lact, uint8_t *uact,
                                                     const uint8_t *presc, double kkt_tol, int apply, double *grad,
                                                     double *part) {
-    __shared__ double red[4];
+    __shared__ double red[kRed];
     double cnt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     GRID_STRIDE(i, n) {
         const double g = Acand[i] - b[i];
@@ -338,8 +350,9 @@ static Work g_work;  // solver scratch shared by the 3D solves of one process
 
 static inline int nblk(int64_t n) { return (int)std::min<int64_t>(kVecBlocks, ceil_div(std::max<int64_t>(n, 1), 256)); }
 
-// three fixed-order sums of the current partials -> host
+// nv (<= 5) fixed-order sums of the current partials -> host
 static int readn(Work &W, hipStream_t st, double *out, int nv) {
+    PHAST_REQUIRE(nv >= 1 && nv <= 5, "internal: %d partial rows requested (at most 5)", nv);
     hipLaunchKernelGGL(k_final_parts, dim3(1), dim3(256), 0, st, W.part, W.d3, nv);
     PHAST_LAUNCHED();
     PHAST_HIP(hipMemcpyAsync(W.h3, W.d3, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
