@@ -168,8 +168,12 @@ __device__ __forceinline__ void macro_epilogue(const MechArgs3 &A, const double 
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) A.part[8 * blockIdx.x + k] = v[k];
-        __threadfence();  // the partials (written by this thread only) before the count
-        is_last = (atomicAdd(A.counter, 1u) == gridDim.x - 1) ? 1u : 0u;
+        // release: the partials (written by this thread only) are visible before the count;
+        // only the last block needs an acquire, so the others leave their caches valid
+        is_last = (__hip_atomic_fetch_add(A.counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) ==
+                   gridDim.x - 1)
+                      ? 1u
+                      : 0u;
     }
     __syncthreads();
     if (is_last) {
@@ -317,6 +321,14 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
 // the marching kernel, face by face (bottom face, then top face).  The top-face sum of
 // voxel layer kv is carried in registers to node layer kv+1; the slab starts one voxel
 // layer early (k0-1) to form the carry into its first node layer, so blocks never talk.
+// Values loaded from memory are made ordinary register values at this point: the empty
+// asm reads and redefines them, so the compiler waits for their loads here rather than at
+// a later use, where its conservative count would also wait for loads issued since.
+__device__ __forceinline__ void settle(double &a, double &b, int32_t &c) {
+    asm volatile("" : "+v"(a), "+v"(b), "+v"(c));
+}
+__device__ __forceinline__ void settle2(double &a, double &b) { asm volatile("" : "+v"(a), "+v"(b)); }
+
 // threads of the voxel-parallel block: the voxel tile rounded up to whole waves
 template <int OX, int OY>
 constexpr int apply_threads() {
@@ -331,7 +343,7 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
     constexpr int NF = (GMODE == 2) ? 4 : 3;
     constexpr int NS = (NN + NT - 1) / NT;  // node slots loaded per thread
     static_assert(NV <= NT, "voxel tile larger than the block");
-    __shared__ double su[2][NF][NN];
+    __shared__ double su[2][NF][NS * NT];  // node layers; slots past NN are padding
     __shared__ double st[18][NV];  // corners (1,0), (0,1), (1,1) of both faces, 3 components each
     __shared__ double red[7 * (NT / 64)];
 
@@ -345,19 +357,21 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
     const bool has_voxel = t < NV;
     const int iv = i0 - 1 + vx, jv = j0 - 1 + vy;
     const bool owner = has_voxel && vx >= 1 && vy >= 1 && iv < nx && jv < ny;  // node (iv, jv)
-    const int e_ij = wrapm(iv, nx) + nx * wrapm(jv, ny);
+    const int e_ij = has_voxel ? wrapm(iv, nx) + nx * wrapm(jv, ny) : 0;
     const int64_t n = (int64_t)nx * ny * nz;
     const int64_t nxy = (int64_t)nx * ny;
     double E[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) E[k] = A.x[3 * n + k];
 
-    // in-layer offsets of this thread's node slots
+    // in-layer offsets of this thread's node slots; a slot past the tile's NN nodes loads
+    // node (0,0) of the layer into LDS padding, so every lane loads and stores without
+    // branching (which keeps the compiler's wait counts exact across the loop)
     int64_t slot_ij[NS];
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
         const int s = t + m * NT;
-        slot_ij[m] = s < NN ? wrapm(i0 - 1 + s % NNX, nx) + (int64_t)nx * wrapm(j0 - 1 + s / NNX, ny) : -1;
+        slot_ij[m] = s < NN ? wrapm(i0 - 1 + s % NNX, nx) + (int64_t)nx * wrapm(j0 - 1 + s / NNX, ny) : 0;
     }
     // node layer k of this thread's slots, fetched into registers (issued one layer ahead)
     double pf[NS][NF];
@@ -365,7 +379,6 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
         const int64_t base = nxy * wrap(k, nz);
 #pragma unroll
         for (int m = 0; m < NS; ++m) {
-            if (slot_ij[m] < 0) continue;
             const int64_t node = base + slot_ij[m];
             dst[m][0] = A.x[3 * node];
             dst[m][1] = A.x[3 * node + 1];
@@ -375,11 +388,9 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
     };
     auto stage = [&](int buf, const double (&src)[NS][NF]) {
 #pragma unroll
-        for (int m = 0; m < NS; ++m) {
-            if (slot_ij[m] < 0) continue;
+        for (int m = 0; m < NS; ++m)
 #pragma unroll
             for (int f = 0; f < NF; ++f) su[buf][f][t + m * NT] = src[m][f];
-        }
     };
 
     double sE[6] = {0, 0, 0, 0, 0, 0}, en = 0.0;
@@ -389,7 +400,6 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
     double ml0 = 0.0, ml1 = 0.0;
     int32_t mb = 0;
     auto fetch_mat = [&](int k, double &l0, double &l1, int32_t &b) {
-        if (!has_voxel) return;
         const int64_t e = e_ij + nxy * wrap(k, nz);
         if (ISO) {
             l0 = A.lam[e];
@@ -406,6 +416,9 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
         stage(0, pf);
         stage(1, pf1);
     }
+    settle(ml0, ml1, mb);
+#pragma unroll
+    for (int k = 0; k < 6; k += 2) settle2(E[k], E[k + 1]);
     int cur = 0;
     for (int kv = k0 - 1; kv < k1; ++kv) {
         __syncthreads();
@@ -464,6 +477,12 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
             }
         }
         __syncthreads();
+        if (kv + 2 <= k1) stage(cur, pf);  // node layer kv+2 replaces layer kv
+        // next layer's material: wait for it here, before this layer's stores are issued
+        ml0 = nl0;
+        ml1 = nl1;
+        mb = nb;
+        settle(ml0, ml1, mb);
         if (owner) {
             const int64_t node = iv + (int64_t)nx * (jv + (int64_t)ny * kv);
 #pragma unroll
@@ -472,10 +491,6 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
                 acc[c] = ((st[15 + c][s00] + st[12 + c][s10]) + st[9 + c][s01]) + r[c][4];
             }
         }
-        if (kv + 2 <= k1) stage(cur, pf);  // node layer kv+2 replaces layer kv
-        ml0 = nl0;
-        ml1 = nl1;
-        mb = nb;
         cur ^= 1;
     }
     macro_epilogue<NT, ENERGY>(A, sE, en, red);
