@@ -46,6 +46,19 @@ struct Q4CG {
 
 constexpr int kParts = 4;  // per-tile partial sums kept between kernels
 
+// Instance descriptors and the block -> instance tables are read-only while any kernel
+// runs.  Read through the constant address space with a wave-uniform index, every field
+// is a scalar load (SGPRs, scalar cache): the descriptor never enters the vector-memory
+// wait counts that the strips' row prefetch relies on.
+typedef const __attribute__((address_space(4))) Q4Inst QI;
+__device__ __forceinline__ QI &inst_at(const Q4Inst *inst, int k) {  // k wave-uniform
+    return *(QI *)(inst + __builtin_amdgcn_readfirstlane(k));
+}
+__device__ __forceinline__ int uniform_at(const int32_t *a, int i) {  // i wave-uniform
+    typedef const __attribute__((address_space(4))) int32_t CI;
+    return __builtin_amdgcn_readfirstlane(*(CI *)(a + __builtin_amdgcn_readfirstlane(i)));
+}
+
 }  // namespace phast
 
 struct phast_q4_plan {
@@ -87,7 +100,7 @@ __device__ __forceinline__ double degrade(double dq, double eta) {
 // the transpose.  Same arithmetic as fem_operators.py:244-254 (strain), 626-630 (g at the
 // Gauss points), 301-333 (Amor) / 277-290 (isotropic), 681-699 (assembly), 983-1006 (psi+).
 template <bool PSI>
-__device__ __forceinline__ void cell_mech(const Q4Inst &P, const double ux[4], const double uy[4],
+__device__ __forceinline__ void cell_mech(QI &P, const double ux[4], const double uy[4],
                                           const double dd[4], double fx[4], double fy[4], double psi[4]) {
     const double al = P.al, be = P.be;
     const double dx0x = ux[1] - ux[0], dx1x = ux[2] - ux[3], dx0y = uy[1] - uy[0], dx1y = uy[2] - uy[3];
@@ -153,7 +166,7 @@ __device__ __forceinline__ void cell_mech(const Q4Inst &P, const double ux[4], c
 // AT2 damage operator of one rectangular cell (damage_solver.py:1797-1838):
 // y_a = Gc l0 sum_q w gradN_a.grad x_q + sum_q N_a rho_q x_q w, rho_q = 2 H_q + Gc/l0,
 // sum-factorised as cell_mech.
-__device__ __forceinline__ void cell_damage(const Q4Inst &P, const double x[4], const double rho[4],
+__device__ __forceinline__ void cell_damage(QI &P, const double x[4], const double rho[4],
                                             double gcl0, double y[4]) {
     const double al = P.al, be = P.be;
     const double dxr0 = x[1] - x[0], dxr1 = x[2] - x[3], dyc0 = x[3] - x[0], dyc1 = x[2] - x[1];
@@ -198,8 +211,8 @@ struct Tile {
 template <int NT>
 __device__ __forceinline__ Tile tile_of(const Q4Inst *inst, const int32_t *block_inst, int strip) {
     Tile t;
-    t.inst = block_inst[strip];
-    const Q4Inst &P = inst[t.inst];
+    t.inst = uniform_at(block_inst, strip);
+    QI &P = inst_at(inst, t.inst);
     const int k = strip - P.tile_off;
     t.i0 = (k % P.tiles_x) * (NT - 2);
     t.j0 = (k / P.tiles_x) * P.ky;
@@ -236,7 +249,8 @@ constexpr int kMechLds = 10 * NT;
 struct RowRegs {
     double2 u, v, a;
     double d;
-    uint32_t m;  // bc mask bits (x: 1, y: 2)
+    uint32_t m;  // bc mask bytes of the node (x in the low byte, y in the high byte)
+    bool ok;     // slot inside the plate (the loads of other slots read a clamped node)
 };
 
 template <int NT, bool STEP, bool HIST>
@@ -248,7 +262,7 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
     __syncthreads();  // the LDS may still be read by the previous strip of this block
 
     const Tile T = tile_of<NT>(A.inst, A.block_inst, strip);
-    const Q4Inst &P = A.inst[T.inst];
+    QI &P = inst_at(A.inst, T.inst);
     const double dt = P.dt, c2 = 0.5 * P.dt * P.dt;
     const int t = threadIdx.x;
     const int ie = T.i0 - 1 + t;  // node column of slot t, cell column of thread t
@@ -258,80 +272,116 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
     const bool bc = STEP && A.bcm != nullptr;
 
     // Slot t of node row j, fetched two rows ahead (two register sets taking turns) so that
-    // its latency overlaps two rows of cell arithmetic.
+    // its latency overlaps two rows of cell arithmetic.  Every lane loads (out-of-range slots
+    // read a clamped node and keep zeros), and these loads are the last memory operations
+    // of a row, after its stores and operand loads: the compiler's wait for the row staged
+    // next then leaves the newest prefetch in flight.
+    const int ic = min(max(ie, 0), T.nxn - 1);
+    const uint8_t *bcm = bc ? A.bcm : reinterpret_cast<const uint8_t *>(A.d);  // any readable bytes
     auto fetch = [&](RowRegs &R, int j) {
-        if (col_ok && j >= 0 && j < T.nyn && j <= T.jend) {
-            const int64_t n = P.node_off + (int64_t)j * T.nxn + ie;
-            R.u = A.u[n];
-            R.d = A.d[n];
-            if (STEP) {
-                R.v = A.v[n];
-                R.a = A.a[n];
-                R.m = bc ? ((uint32_t)A.bcm[2 * n] | ((uint32_t)A.bcm[2 * n + 1] << 1)) : 0u;
-            }
-        } else {
-            R.u = R.v = R.a = make_double2(0.0, 0.0);
-            R.d = 0.0;
-            R.m = 0u;
+        R.ok = col_ok && j >= 0 && j < T.nyn && j <= T.jend;
+        const int64_t n = P.node_off + (int64_t)min(max(j, 0), T.nyn - 1) * T.nxn + ic;
+        R.u = A.u[n];
+        R.d = A.d[n];
+        if (STEP) {
+            R.v = A.v[n];
+            R.a = A.a[n];
+            R.m = *reinterpret_cast<const uint16_t *>(bcm + 2 * n);  // both bytes, decoded when staged
         }
     };
+    double2 gv = make_double2(0.0, 0.0), ga = gv, nv = gv, na = gv;  // v, a of rows jr and jr+1
+    uint32_t gm = 0, nm = 0;
+    // stage a fetched row into LDS (slots outside the plate hold zeros); the values of the
+    // other operands are masked here too, where the row's loads are waited for anyway
     auto store = [&](const RowRegs &R, int buf, int j) {
-        double ux = R.u.x, uy = R.u.y;
+        const double2 z2 = make_double2(0.0, 0.0);
+        const double2 u = R.ok ? R.u : z2;
+        double ux = u.x, uy = u.y;
         if (STEP) {
-            ux = (R.u.x + dt * R.v.x) + c2 * R.a.x;  // mechanics_solver.py:160
-            uy = (R.u.y + dt * R.v.y) + c2 * R.a.y;
-            if (R.m) {
+            const double2 v = R.ok ? R.v : z2, a = R.ok ? R.a : z2;
+            const uint32_t m = (R.ok && bc) ? ((R.m & 0xffu) | ((R.m >> 8) << 1)) : 0u;
+            ux = (u.x + dt * v.x) + c2 * a.x;  // mechanics_solver.py:160
+            uy = (u.y + dt * v.y) + c2 * a.y;
+            if (m) {
                 const int64_t n = P.node_off + (int64_t)j * T.nxn + ie;
-                if (R.m & 1u) ux = A.bcv[2 * n];
-                if (R.m & 2u) uy = A.bcv[2 * n + 1];
+                if (m & 1u) ux = A.bcv[2 * n];
+                if (m & 2u) uy = A.bcv[2 * n + 1];
             }
+            nv = v;
+            na = a;
+            nm = m;
         }
         sux[buf][t] = ux;
         suy[buf][t] = uy;
-        sd[buf][t] = R.d;
+        sd[buf][t] = R.ok ? R.d : 0.0;
+    };
+
+    // results of a row, written at the start of the next row (before its loads)
+    bool w_node = false, w_cell = false;
+    int64_t w_n = 0, w_c = 0;
+    double2 w_f = make_double2(0.0, 0.0), w_v = w_f, w_a = w_f, w_h01 = w_f, w_h23 = w_f, w_p01 = w_f,
+            w_p23 = w_f;
+    auto flush = [&]() {
+        if (w_node) {
+            if (!STEP) {
+                A.f[w_n] = w_f;
+            } else {
+                A.u1[w_n] = w_f;
+                A.v1[w_n] = w_v;
+                A.a1[w_n] = w_a;
+            }
+        }
+        if (HIST && w_cell) {
+            reinterpret_cast<double2 *>(A.Hnew)[2 * w_c] = w_h01;
+            reinterpret_cast<double2 *>(A.Hnew)[2 * w_c + 1] = w_h23;
+            if (A.psi) {
+                reinterpret_cast<double2 *>(A.psi)[2 * w_c] = w_p01;
+                reinterpret_cast<double2 *>(A.psi)[2 * w_c + 1] = w_p23;
+            }
+        }
     };
 
     RowRegs X, Y;
     X.v = X.a = Y.v = Y.a = make_double2(0.0, 0.0);
     X.m = Y.m = 0u;
+    X.ok = Y.ok = false;
     fetch(X, T.j0 - 1);
     store(X, 0, T.j0 - 1);
     fetch(X, T.j0);
+    __builtin_amdgcn_sched_barrier(0);  // row j0's loads issue before row j0+1's
     fetch(Y, T.j0 + 1);
-    double2 gv = make_double2(0.0, 0.0), ga = gv, nv = gv, na = gv;  // v, a of rows jr and jr+1
-    uint32_t gm = 0, nm = 0;
     double accx = 0.0, accy = 0.0;
     int cur = 0;
+    const int cc = min(max(ie, 0), P.nx - 1);
     // one cell row: R holds node row jr+1 on entry and row jr+3 on exit
     auto row = [&](int jr, RowRegs &R) {
         const int up = cur ^ 1;
-        store(R, up, jr + 1);
-        if (STEP) {
-            nv = R.v;
-            na = R.a;
-            nm = R.m;
-        }
-        // operands consumed later in this row, issued before the next prefetch so that
-        // waiting for them leaves the prefetch in flight
+        store(R, up, jr + 1);  // also sets nv, na, nm (v, a, bc bits of row jr+1)
+        flush();
+        // operands consumed later in this row (every lane loads a clamped index)
         const bool gat = owner && jr >= T.j0;
         const int64_t ng = P.node_off + (int64_t)jr * T.nxn + ie;
         double2 fe = make_double2(0.0, 0.0);
         double mi = 0.0;
-        if (STEP && gat) {
-            if (A.fext) fe = A.fext[ng];
-            mi = A.minv[ng];
+        if (STEP) {
+            const int64_t nc = P.node_off + (int64_t)max(jr, 0) * T.nxn + ic;
+            if (A.fext) fe = A.fext[nc];
+            mi = A.minv[nc];
         }
         const bool cell_ok = cell_col && jr >= 0 && jr < P.ny;
         const bool own_cell = HIST && cell_ok && t >= 1 && jr >= T.j0;
         const int64_t c = P.cell_off + (int64_t)jr * P.nx + ie;
         double2 h01 = make_double2(0.0, 0.0), h23 = h01;
-        if (own_cell) {
-            h01 = reinterpret_cast<const double2 *>(A.Hold)[2 * c];
-            h23 = reinterpret_cast<const double2 *>(A.Hold)[2 * c + 1];
+        if (HIST) {
+            const int64_t ccl = P.cell_off + (int64_t)min(max(jr, 0), P.ny - 1) * P.nx + cc;
+            h01 = reinterpret_cast<const double2 *>(A.Hold)[2 * ccl];
+            h23 = reinterpret_cast<const double2 *>(A.Hold)[2 * ccl + 1];
         }
+        __builtin_amdgcn_sched_barrier(0);  // the prefetch issues after the operand loads
         fetch(R, jr + 3);
         __syncthreads();
         double fx[4] = {0.0, 0.0, 0.0, 0.0}, fy[4] = {0.0, 0.0, 0.0, 0.0};
+        w_cell = false;
         if (cell_ok) {
             const double ux[4] = {sux[cur][t], sux[cur][t + 1], sux[up][t + 1], sux[up][t]};
             const double uy[4] = {suy[cur][t], suy[cur][t + 1], suy[up][t + 1], suy[up][t]};
@@ -339,17 +389,15 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
             double ps[4];
             if (own_cell) {  // cells owned by this strip: lower-left node inside
                 cell_mech<true>(P, ux, uy, dd, fx, fy, ps);
-                double2 n01, n23;  // max returns one operand bit-exactly (no NaN inputs)
-                n01.x = h01.x >= ps[0] ? h01.x : ps[0];
-                n01.y = h01.y >= ps[1] ? h01.y : ps[1];
-                n23.x = h23.x >= ps[2] ? h23.x : ps[2];
-                n23.y = h23.y >= ps[3] ? h23.y : ps[3];
-                reinterpret_cast<double2 *>(A.Hnew)[2 * c] = n01;
-                reinterpret_cast<double2 *>(A.Hnew)[2 * c + 1] = n23;
-                if (A.psi) {
-                    reinterpret_cast<double2 *>(A.psi)[2 * c] = make_double2(ps[0], ps[1]);
-                    reinterpret_cast<double2 *>(A.psi)[2 * c + 1] = make_double2(ps[2], ps[3]);
-                }
+                // max returns one operand bit-exactly (no NaN inputs)
+                w_h01.x = h01.x >= ps[0] ? h01.x : ps[0];
+                w_h01.y = h01.y >= ps[1] ? h01.y : ps[1];
+                w_h23.x = h23.x >= ps[2] ? h23.x : ps[2];
+                w_h23.y = h23.y >= ps[3] ? h23.y : ps[3];
+                w_p01 = make_double2(ps[0], ps[1]);
+                w_p23 = make_double2(ps[2], ps[3]);
+                w_c = c;
+                w_cell = true;
             } else {
                 cell_mech<false>(P, ux, uy, dd, fx, fy, ps);
             }
@@ -359,22 +407,25 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
         sf[2][t] = fx[2];
         sf[3][t] = fy[2];
         __syncthreads();
+        w_node = false;
         if (t >= 1 && t <= NT - 2) {
             if (gat) {
                 // cells (i-1,j-1) c2, (i,j-1) c3 [acc], (i-1,j) c1, (i,j) c0
                 const double fxn = (accx + sf[0][t - 1]) + fx[0];
                 const double fyn = (accy + sf[1][t - 1]) + fy[0];
                 if (!STEP) {
-                    A.f[ng] = make_double2(fxn, fyn);
+                    w_f = make_double2(fxn, fyn);
                 } else {
                     double ax = mi * (fe.x - fxn), ay = mi * (fe.y - fyn);  // mechanics_solver.py:194-198
                     double vx = gv.x + 0.5 * dt * (ga.x + ax), vy = gv.y + 0.5 * dt * (ga.y + ay);
                     if (gm & 1u) ax = vx = 0.0;
                     if (gm & 2u) ay = vy = 0.0;
-                    A.u1[ng] = make_double2(sux[cur][t], suy[cur][t]);
-                    A.v1[ng] = make_double2(vx, vy);
-                    A.a1[ng] = make_double2(ax, ay);
+                    w_f = make_double2(sux[cur][t], suy[cur][t]);
+                    w_v = make_double2(vx, vy);
+                    w_a = make_double2(ax, ay);
                 }
+                w_n = ng;
+                w_node = true;
             }
             accx = sf[2][t - 1] + fx[3];
             accy = sf[3][t - 1] + fy[3];
@@ -386,10 +437,15 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
         }
         cur = up;
     };
-    for (int jr = T.j0 - 1; jr < T.jend; jr += 2) {
+    // both halves unconditional inside the loop (an odd last row runs after it), so the
+    // compiler sees row Y's loads issued between two stagings of row X on every path
+    int jr = T.j0 - 1;
+    for (; jr + 1 < T.jend; jr += 2) {
         row(jr, X);
-        if (jr + 1 < T.jend) row(jr + 1, Y);
+        row(jr + 1, Y);
     }
+    if (jr < T.jend) row(jr, X);
+    flush();
 }
 
 template <int NT, bool STEP, bool HIST>
@@ -422,7 +478,7 @@ __device__ __forceinline__ int find_inst_by_cell(const Q4Inst *inst, int n_inst,
 __global__ __launch_bounds__(256) void q4_cell_kernel(CellArgs A) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= A.n_cells) return;
-    const Q4Inst &P = A.inst[find_inst_by_cell(A.inst, A.n_inst, c)];
+    QI &P = *(QI *)(A.inst + find_inst_by_cell(A.inst, A.n_inst, c));
     const int64_t lc = c - P.cell_off;
     const int ie = (int)(lc % P.nx), je = (int)(lc / P.nx);
     const int64_t n0 = P.node_off + (int64_t)je * (P.nx + 1) + ie;
@@ -535,7 +591,7 @@ __device__ __forceinline__ void dmg_strip(const DmgArgs &A, int strip, double *s
     __syncthreads();  // the LDS may still be read by the previous strip of this block
 
     const Tile T = tile_of<NT>(A.inst, A.block_inst, strip);
-    const Q4Inst &P = A.inst[T.inst];
+    QI &P = inst_at(A.inst, T.inst);
     const int t = threadIdx.x;
     const int ie = T.i0 - 1 + t;
     const bool col_ok = ie >= 0 && ie < T.nxn;
@@ -736,7 +792,7 @@ template <int NT, int OP>
 __global__ __launch_bounds__(NT) void q4_dmg_kernel(DmgArgs A) {
     __shared__ double sm[kDmgLds<NT>];
     __shared__ double red[NT / 64];
-    const int inst = A.block_inst[blockIdx.x];
+    const int inst = uniform_at(A.block_inst, blockIdx.x);
     if (OP >= OP_CG_AP && OP != OP_PC_INIT && A.cg[inst].done) return;
     if ((OP == OP_CG_RESID) && !(A.cg[inst].limited || A.cg[inst].check)) return;
     if ((OP == OP_PC_RESID) && !A.cg[inst].check) return;
@@ -779,7 +835,7 @@ template <int NT, int OP>
 __device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4CG &S, double *red) {
     constexpr int U = 4;
     const Tile T = tile_of<NT>(A.inst, A.block_inst, strip);
-    const Q4Inst &P = A.inst[T.inst];
+    QI &P = inst_at(A.inst, T.inst);
     const int t = threadIdx.x;
     const int i = T.i0 - 1 + t;
     double p0 = 0.0, p1 = 0.0;
@@ -899,7 +955,7 @@ __device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4C
 template <int NT, int OP>
 __global__ __launch_bounds__(NT) void q4_vec_kernel(VecArgs A) {
     __shared__ double red[NT / 64];
-    const Q4CG &S = A.cg[A.block_inst[blockIdx.x]];
+    const Q4CG &S = A.cg[uniform_at(A.block_inst, blockIdx.x)];
     if (OP != V_SETUP && OP != V_FINISH && OP != V_PC_FINISH && S.done) return;
     vec_strip<NT, OP>(A, blockIdx.x, S, red);
 }
@@ -943,7 +999,7 @@ __device__ __forceinline__ double reduce_min_parts(const double *part, int off, 
 // One per-instance scalar step of the CG (thread 0 writes S).  Every thread of the block
 // takes part in the reductions; all return paths are block-uniform.
 template <int OP, int NT>
-__device__ __forceinline__ void scalar_step(const Q4Inst &P, Q4CG &S, const double *part, int32_t n_tiles,
+__device__ __forceinline__ void scalar_step(QI &P, Q4CG &S, const double *part, int32_t n_tiles,
                                             int32_t check_every, double *red) {
     const int off = P.tile_off, cnt = P.n_tiles;
     if (OP == S_INIT || OP == S_PC_INIT) {
@@ -1053,7 +1109,7 @@ __global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG
     __shared__ double red[4];
     Q4CG &S = cg[blockIdx.x];
     if (OP != S_INIT && OP != S_PC_INIT && S.done) return;
-    scalar_step<OP, 256>(inst[blockIdx.x], S, part, n_tiles, check_every, red);
+    scalar_step<OP, 256>(inst_at(inst, blockIdx.x), S, part, n_tiles, check_every, red);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1113,7 +1169,7 @@ __device__ __forceinline__ void inst_barrier(uint32_t *arrive, uint32_t target, 
 template <int NT>
 __device__ __forceinline__ void minv_strip(const Q4Inst *inst, const int32_t *block_inst, int strip, double *minv) {
     const Tile T = tile_of<NT>(inst, block_inst, strip);
-    const Q4Inst &P = inst[T.inst];
+    QI &P = inst_at(inst, T.inst);
     const int t = threadIdx.x, i = T.i0 - 1 + t;
     if (t >= 1 && t <= NT - 2 && i < T.nxn) {
         for (int j = T.j0; j < T.jend; ++j) {
@@ -1128,8 +1184,9 @@ __global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
     __shared__ double sm[kDmgLds<NT>];
     __shared__ double red[4];
     __shared__ Q4CG S;
-    const int k = A.pb_inst[blockIdx.x], rank = A.pb_rank[blockIdx.x], nb = A.nblk[k];
-    const Q4Inst &P = A.M.inst[k];
+    const int k = uniform_at(A.pb_inst, blockIdx.x), rank = uniform_at(A.pb_rank, blockIdx.x),
+              nb = uniform_at(A.nblk, k);
+    QI &P = inst_at(A.M.inst, k);
     const int n_tiles = A.V.n_tiles;
     const int check_every = METHOD == 0 ? 10 : 50;
     uint32_t *arrive = A.bar + 2 * k;
