@@ -280,7 +280,8 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
     const uint8_t *bcm = bc ? A.bcm : reinterpret_cast<const uint8_t *>(A.d);  // any readable bytes
     auto fetch = [&](RowRegs &R, int j) {
         R.ok = col_ok && j >= 0 && j < T.nyn && j <= T.jend;
-        const int64_t n = P.node_off + (int64_t)min(max(j, 0), T.nyn - 1) * T.nxn + ic;
+        // rows outside [0, min(jend, nyn-1)] read a row of the strip that is read anyway
+        const int64_t n = P.node_off + (int64_t)min(max(j, 0), min(T.jend, T.nyn - 1)) * T.nxn + ic;
         R.u = A.u[n];
         R.d = A.d[n];
         if (STEP) {
@@ -341,22 +342,29 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
         }
     };
 
-    RowRegs X, Y;
-    X.v = X.a = Y.v = Y.a = make_double2(0.0, 0.0);
-    X.m = Y.m = 0u;
-    X.ok = Y.ok = false;
-    fetch(X, T.j0 - 1);
-    store(X, 0, T.j0 - 1);
-    fetch(X, T.j0);
-    __builtin_amdgcn_sched_barrier(0);  // row j0's loads issue before row j0+1's
-    fetch(Y, T.j0 + 1);
+    // D register sets take turns: node rows are fetched D rows ahead of their staging
+    constexpr int D = 2;
+    RowRegs R[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        R[k].v = R[k].a = make_double2(0.0, 0.0);
+        R[k].m = 0u;
+        R[k].ok = false;
+    }
+    fetch(R[0], T.j0 - 1);
+    store(R[0], 0, T.j0 - 1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        fetch(R[k], T.j0 + k);
+        __builtin_amdgcn_sched_barrier(0);  // rows are issued in order
+    }
     double accx = 0.0, accy = 0.0;
     int cur = 0;
     const int cc = min(max(ie, 0), P.nx - 1);
-    // one cell row: R holds node row jr+1 on entry and row jr+3 on exit
-    auto row = [&](int jr, RowRegs &R) {
+    // one cell row: Q holds node row jr+1 on entry and row jr+1+D on exit
+    auto row = [&](int jr, RowRegs &Q) {
         const int up = cur ^ 1;
-        store(R, up, jr + 1);  // also sets nv, na, nm (v, a, bc bits of row jr+1)
+        store(Q, up, jr + 1);  // also sets nv, na, nm (v, a, bc bits of row jr+1)
         flush();
         // operands consumed later in this row (every lane loads a clamped index)
         const bool gat = owner && jr >= T.j0;
@@ -378,7 +386,7 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
             h23 = reinterpret_cast<const double2 *>(A.Hold)[2 * ccl + 1];
         }
         __builtin_amdgcn_sched_barrier(0);  // the prefetch issues after the operand loads
-        fetch(R, jr + 3);
+        fetch(Q, jr + 1 + D);
         __syncthreads();
         double fx[4] = {0.0, 0.0, 0.0, 0.0}, fy[4] = {0.0, 0.0, 0.0, 0.0};
         w_cell = false;
@@ -437,14 +445,17 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
         }
         cur = up;
     };
-    // both halves unconditional inside the loop (an odd last row runs after it), so the
-    // compiler sees row Y's loads issued between two stagings of row X on every path
+    // all D rows unconditional inside the loop (the last < D rows run after it), so the
+    // compiler sees the other sets' loads issued between two stagings of one set on
+    // every path
     int jr = T.j0 - 1;
-    for (; jr + 1 < T.jend; jr += 2) {
-        row(jr, X);
-        row(jr + 1, Y);
+    for (; jr + D - 1 < T.jend; jr += D) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) row(jr + k, R[k]);
     }
-    if (jr < T.jend) row(jr, X);
+#pragma unroll
+    for (int k = 0; k < D - 1; ++k)
+        if (jr + k < T.jend) row(jr + k, R[k]);
     flush();
 }
 
@@ -792,11 +803,12 @@ template <int NT, int OP>
 __global__ __launch_bounds__(NT) void q4_dmg_kernel(DmgArgs A) {
     __shared__ double sm[kDmgLds<NT>];
     __shared__ double red[NT / 64];
-    const int inst = uniform_at(A.block_inst, blockIdx.x);
+    const int strip = blockIdx.x;
+    const int inst = uniform_at(A.block_inst, strip);
     if (OP >= OP_CG_AP && OP != OP_PC_INIT && A.cg[inst].done) return;
     if ((OP == OP_CG_RESID) && !(A.cg[inst].limited || A.cg[inst].check)) return;
     if ((OP == OP_PC_RESID) && !A.cg[inst].check) return;
-    dmg_strip<NT, OP>(A, blockIdx.x, sm, red);
+    dmg_strip<NT, OP>(A, strip, sm, red);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -955,9 +967,10 @@ __device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4C
 template <int NT, int OP>
 __global__ __launch_bounds__(NT) void q4_vec_kernel(VecArgs A) {
     __shared__ double red[NT / 64];
-    const Q4CG &S = A.cg[uniform_at(A.block_inst, blockIdx.x)];
+    const int strip = blockIdx.x;
+    const Q4CG &S = A.cg[uniform_at(A.block_inst, strip)];
     if (OP != V_SETUP && OP != V_FINISH && OP != V_PC_FINISH && S.done) return;
-    vec_strip<NT, OP>(A, blockIdx.x, S, red);
+    vec_strip<NT, OP>(A, strip, S, red);
 }
 
 // ---------------------------------------------------------------------------------------
