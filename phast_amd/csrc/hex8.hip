@@ -146,16 +146,16 @@ struct MechArgs3 {
     const double *lam, *mu;
     double *y;             // (3n+6)
     double *part;          // 8 per block
-    unsigned int *counter; // blocks finished (reset to 0 by the last block)
     double *y6;            // macro rows destination (y + 3n) or null
     double *out7;          // stress sums destination or null
     int32_t kz, tiles_x, tiles_y, nvals;
 };
 
-// End of a mechanics apply: the block's macro-row (and energy) partials, then the last
-// block to finish sums every block's partials in block order (a fixed order, so the macro
-// rows are deterministic) and resets the counter.  All partials of a block are reduced
-// together and the last block reads them in one pass.  `red` holds 7 * NT/64 doubles.
+// End of a mechanics apply: the block's macro-row (and energy) partials, all reduced
+// together, go to part[8 b + k]; hex8_finalize_kernel, launched right after on the same
+// stream, sums them in block order (a fixed order, so the macro rows are deterministic).
+// The kernel boundary orders the partials before that sum: no fence, counter or atomic.
+// `red` holds 7 * NT/64 doubles.
 template <int NT, bool ENERGY>
 __device__ __forceinline__ void macro_epilogue(const MechArgs3 &A, const double sE[6], double en, double *red) {
     constexpr int K = ENERGY ? 7 : 6;
@@ -164,34 +164,9 @@ __device__ __forceinline__ void macro_epilogue(const MechArgs3 &A, const double 
     for (int k = 0; k < 6; ++k) v[k] = sE[k];
     if (ENERGY) v[K - 1] = en;
     block_sum_n<NT, K>(v, red);
-    __shared__ unsigned int is_last;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < K; ++k) A.part[8 * blockIdx.x + k] = v[k];
-        // release: the partials (written by this thread only) are visible before the count;
-        // only the last block needs an acquire, so the others leave their caches valid
-        is_last = (__hip_atomic_fetch_add(A.counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) ==
-                   gridDim.x - 1)
-                      ? 1u
-                      : 0u;
-    }
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += NT)
-#pragma unroll
-            for (int k = 0; k < 7; ++k)
-                if (k < A.nvals) s[k] += __builtin_nontemporal_load(&A.part[8 * b + k]);
-        block_sum_n<NT, 7>(s, red);
-        if (threadIdx.x == 0) {
-            for (int k = 0; k < A.nvals; ++k) {
-                if (A.y6 && k < 6) A.y6[k] = s[k];
-                if (A.out7) A.out7[k] = s[k];
-            }
-            atomicExch(A.counter, 0u);
-        }
-    }
 }
 
 template <int TX, int TY, int GMODE, bool ISO, bool ENERGY>  // GMODE 0: g=1, 1: g array, 2: g from d
@@ -496,19 +471,22 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
     macro_epilogue<NT, ENERGY>(A, sE, en, red);
 }
 
-// Sum the block partials of the macro rows in a fixed order; write into y[3n + k] and/or out7.
+// Sum the block partials of the macro rows in a fixed order (block order per row, all
+// rows' loads in flight together); write into y[3n + k] and/or out7.
 __global__ __launch_bounds__(256) void hex8_finalize_kernel(const double *part, int nblocks, int nvals,
                                                             double *y6, double *out7) {
-    __shared__ double red[4];
-    for (int k = 0; k < nvals; ++k) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += 256) s += part[8 * b + k];
-        s = block_sum<256>(s, red);
-        if (threadIdx.x == 0) {
-            if (y6 && k < 6) y6[k] = s;
-            if (out7) out7[k] = s;
+    __shared__ double red[7 * 4];
+    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += 256)
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            if (k < nvals) s[k] += part[8 * b + k];
+    block_sum_n<256, 7>(s, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nvals; ++k) {
+            if (y6 && k < 6) y6[k] = s[k];
+            if (out7) out7[k] = s[k];
         }
-    }
 }
 
 __global__ __launch_bounds__(256) void hex8_degradation_kernel(const Hex8Geo G, const double *d, double *g) {
@@ -794,6 +772,8 @@ static int launch_mech_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool
     }
 #undef PHAST_L
     PHAST_LAUNCHED();
+    hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, A.part, nb, A.nvals, A.y6, A.out7);
+    PHAST_LAUNCHED();
     return PHAST_OK;
 }
 
@@ -832,6 +812,8 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
     }
 #undef PHAST_L
     PHAST_LAUNCHED();
+    hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, A.part, nb, A.nvals, A.y6, A.out7);
+    PHAST_LAUNCHED();
     return PHAST_OK;
 }
 
@@ -848,7 +830,6 @@ int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g,
     A.mu = pl->mu;
     A.y = y;
     A.part = pl->part;
-    A.counter = pl->counter;
     A.y6 = out7 ? nullptr : y + 3 * pl->n;
     A.out7 = out7;
     A.nvals = energy ? 7 : 6;
@@ -980,8 +961,6 @@ extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double
         hipMalloc(&pl->bank_index, sizeof(int32_t) * pl->n) != hipSuccess || hipMalloc(&pl->Gc, nb) != hipSuccess ||
         hipMalloc(&pl->lam, nb) != hipSuccess || hipMalloc(&pl->mu, nb) != hipSuccess ||
         hipMalloc(&pl->driver, (size_t)pl->n) != hipSuccess ||
-        hipMalloc(&pl->counter, sizeof(unsigned int)) != hipSuccess ||
-        hipMemset(pl->counter, 0, sizeof(unsigned int)) != hipSuccess ||
         hipMalloc(&pl->part, sizeof(double) * 8 * (size_t)std::max<int64_t>(pl->max_blocks, (pl->n + 255) / 256)) !=
             hipSuccess ||
         hipMemcpy(pl->bank, bank, sizeof(double) * 36 * n_bank, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1009,7 +988,6 @@ extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
     (void)hipFree(pl->lam);
     (void)hipFree(pl->mu);
     (void)hipFree(pl->part);
-    (void)hipFree(pl->counter);
     delete pl;
     return PHAST_OK;
 }

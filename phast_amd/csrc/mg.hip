@@ -211,7 +211,6 @@ extern "C" int phast_hex8_mg_destroy(phast_hex8_mg *mg) {
             (void)hipFree(mg->lv[l]->lam);
             (void)hipFree(mg->lv[l]->mu);
             (void)hipFree(mg->lv[l]->part);
-            (void)hipFree(mg->lv[l]->counter);
             (void)hipFree(mg->lv[l]->bank);
             (void)hipFree(mg->lv[l]->bank_index);
             delete mg->lv[l];
@@ -279,8 +278,6 @@ extern "C" int phast_hex8_mg_create(const phast_hex8_plan *fine, int64_t pinned,
         if (hipMalloc(&pl->lam, nb) != hipSuccess || hipMalloc(&pl->mu, nb) != hipSuccess ||
             hipMalloc(&pl->part, sizeof(double) * 8 * (size_t)std::max<int64_t>(pl->max_blocks, (pl->n + 255) / 256)) !=
                 hipSuccess ||
-            hipMalloc(&pl->counter, sizeof(unsigned int)) != hipSuccess ||
-            hipMemset(pl->counter, 0, sizeof(unsigned int)) != hipSuccess ||
             hipMalloc(&pl->bank, sizeof(double) * 36) != hipSuccess ||
             hipMemset(pl->bank, 0, sizeof(double) * 36) != hipSuccess ||
             hipMalloc(&pl->bank_index, sizeof(int32_t) * pl->n) != hipSuccess ||
