@@ -148,6 +148,7 @@ struct MechArgs3 {
     double *part;          // 8 per block
     double *y6;            // macro rows destination (y + 3n) or null
     double *out7;          // stress sums destination or null
+    double *seam;          // voxel-parallel apply: top-face sums of each slab's last voxel layer
     int32_t kz, tiles_x, tiles_y, nvals;
 };
 
@@ -294,8 +295,11 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
 // its owner keeps one corner of each face in registers and reads the other three from the
 // voxels to its left, below and diagonally below-left through LDS -- in the same order as
 // the marching kernel, face by face (bottom face, then top face).  The top-face sum of
-// voxel layer kv is carried in registers to node layer kv+1; the slab starts one voxel
-// layer early (k0-1) to form the carry into its first node layer, so blocks never talk.
+// voxel layer kv is carried in registers to node layer kv+1.  A slab evaluates only its
+// own voxel layers [k0, k1): its first node layer gets the bottom-face sums here, and the
+// top-face sums of its last voxel layer go to a seam buffer; hex8_seam_finalize_kernel,
+// launched next on the stream, adds each seam into the first node layer of the slab
+// above as (top + bottom) -- the marching kernel's sum, operand for operand.
 // Values loaded from memory are made ordinary register values at this point: the empty
 // asm reads and redefines them, so the compiler waits for their loads here rather than at
 // a later use, where its conservative count would also wait for loads issued since.
@@ -383,11 +387,11 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
             b = A.bank_index[e];
         }
     };
-    fetch_mat(k0 - 1, ml0, ml1, mb);
+    fetch_mat(k0, ml0, ml1, mb);
     {  // the first two node layers, both in flight before either is staged
         double pf1[NS][NF];
-        fetch(k0 - 1, pf);
-        fetch(k0, pf1);
+        fetch(k0, pf);
+        fetch(k0 + 1, pf1);
         stage(0, pf);
         stage(1, pf1);
     }
@@ -395,13 +399,12 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
 #pragma unroll
     for (int k = 0; k < 6; k += 2) settle2(E[k], E[k + 1]);
     int cur = 0;
-    for (int kv = k0 - 1; kv < k1; ++kv) {
+    for (int kv = k0; kv < k1; ++kv) {
         __syncthreads();
         if (kv + 2 <= k1) fetch(kv + 2, pf);  // lands in su[cur] once this layer's voxels are done
         double nl0 = 0.0, nl1 = 0.0;
         int32_t nb = 0;
         if (kv + 1 < k1) fetch_mat(kv + 1, nl0, nl1, nb);
-        const bool first = kv < k0;  // carry-only layer
         double r[3][8];
         if (has_voxel) {
             const int64_t e = e_ij + nxy * wrap(kv, nz);
@@ -435,11 +438,10 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
 #pragma unroll
                 for (int q = 0; q < 8; ++q) g[q] = 1.0;
             }
-            const bool own = owner && !first;
             if (ISO) {
-                voxel_mech<true, ENERGY>(G, u, E, g, nullptr, ml0, ml1, own, r, sE, en);
+                voxel_mech<true, ENERGY>(G, u, E, g, nullptr, ml0, ml1, owner, r, sE, en);
             } else {
-                voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)mb, 0.0, 0.0, own, r, sE, en);
+                voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)mb, 0.0, 0.0, owner, r, sE, en);
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -462,11 +464,18 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
             const int64_t node = iv + (int64_t)nx * (jv + (int64_t)ny * kv);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                if (!first) A.y[3 * node + c] = acc[c] + (((st[6 + c][s00] + st[3 + c][s10]) + st[c][s01]) + r[c][0]);
+                const double bot = ((st[6 + c][s00] + st[3 + c][s10]) + st[c][s01]) + r[c][0];
+                A.y[3 * node + c] = kv == k0 ? bot : acc[c] + bot;  // first layer: seam added later
                 acc[c] = ((st[15 + c][s00] + st[12 + c][s10]) + st[9 + c][s01]) + r[c][4];
             }
         }
         cur ^= 1;
+    }
+    if (owner) {  // top faces of voxel layer k1-1 -> seam of the slab holding node layer k1 (mod nz)
+        const int nslabs = (int)gridDim.x / tiles;
+        const int64_t at = ((int64_t)((bz + 1) % nslabs) * nxy + iv + (int64_t)nx * jv) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) A.seam[at + c] = acc[c];
     }
     macro_epilogue<NT, ENERGY>(A, sE, en, red);
 }
@@ -487,6 +496,36 @@ __global__ __launch_bounds__(256) void hex8_finalize_kernel(const double *part, 
             if (y6 && k < 6) y6[k] = s[k];
             if (out7) out7[k] = s[k];
         }
+}
+
+// After the voxel-parallel apply: block 0 sums the macro-row partials (as
+// hex8_finalize_kernel); the other blocks add each slab's seam (top-face sums of the voxel
+// layer below its first node layer, computed by the slab below) to that node layer:
+// y = seam + bottom, the same sum the marching kernel forms.
+__global__ __launch_bounds__(256) void hex8_seam_finalize_kernel(const Hex8Geo G, const double *seam, int kz,
+                                                                 int nslabs, double *y, const double *part,
+                                                                 int nblocks, int nvals, double *y6, double *out7) {
+    if (blockIdx.x == 0) {
+        __shared__ double red[7 * 4];
+        double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int b = threadIdx.x; b < nblocks; b += 256)
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                if (k < nvals) s[k] += part[8 * b + k];
+        block_sum_n<256, 7>(s, red);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < nvals; ++k) {
+                if (y6 && k < 6) y6[k] = s[k];
+                if (out7) out7[k] = s[k];
+            }
+        return;
+    }
+    const int64_t nxy = (int64_t)G.nx * G.ny;
+    const int64_t i = (int64_t)(blockIdx.x - 1) * 256 + threadIdx.x;  // (slab, node in layer, component)
+    if (i >= 3 * nxy * nslabs) return;
+    const int64_t q = i / 3, sl = q / nxy;
+    const int64_t at = 3 * (q - sl * nxy + nxy * (sl * kz)) + (i - 3 * q);
+    y[at] = seam[i] + y[at];
 }
 
 __global__ __launch_bounds__(256) void hex8_degradation_kernel(const Hex8Geo G, const double *d, double *g) {
@@ -787,6 +826,11 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
         set_error("hex8 apply needs %d blocks, plan holds partials for %d", nb, pl->max_blocks);
         return PHAST_ERR_ARG;
     }
+    if (!pl->seam || pl->seam_cap < 3 * (int64_t)pl->geo.nx * pl->geo.ny * nbz) {
+        set_error("internal: hex8 seam buffer too small for kz = %d", A.kz);
+        return PHAST_ERR_ARG;
+    }
+    A.seam = pl->seam;
     dim3 grid(nb), block(apply_threads<OX, OY>());
 #define PHAST_L(GM, ISO, EN) hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, GM, ISO, EN>), grid, block, 0, st, pl->geo, A)
     if (pl->iso) {
@@ -812,7 +856,9 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
     }
 #undef PHAST_L
     PHAST_LAUNCHED();
-    hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, A.part, nb, A.nvals, A.y6, A.out7);
+    const int64_t nseam = 3 * (int64_t)pl->geo.nx * pl->geo.ny * nbz;
+    hipLaunchKernelGGL(hex8_seam_finalize_kernel, dim3((unsigned)(1 + ceil_div(nseam, 256))), dim3(256), 0, st, pl->geo,
+                       A.seam, A.kz, nbz, A.y, A.part, nb, A.nvals, A.y6, A.out7);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
@@ -915,6 +961,8 @@ void phast::build_voxel_constants(Hex8Geo &G, double hx, double hy, double hz, d
     G.bez = G.be * G.ihz;
 }
 
+static int ensure_seam(phast_hex8_plan *pl, int32_t kz);
+
 extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,
                                       const double *bank, int32_t n_bank, const int32_t *bank_index,
                                       const double *Gc, const double *lam, const double *mu, const int8_t *driver,
@@ -973,6 +1021,10 @@ extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double
         phast_hex8_plan_destroy(pl);
         return PHAST_ERR_HIP;
     }
+    if (int rc = ensure_seam(pl, pl->kz)) {
+        phast_hex8_plan_destroy(pl);
+        return rc;
+    }
     *out = pl;
     return PHAST_OK;
 }
@@ -988,7 +1040,21 @@ extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
     (void)hipFree(pl->lam);
     (void)hipFree(pl->mu);
     (void)hipFree(pl->part);
+    (void)hipFree(pl->seam);
     delete pl;
+    return PHAST_OK;
+}
+
+// Seam buffer of the voxel-parallel apply for slabs of kz node layers: 3 doubles per node
+// of each slab's first layer.  Grown here (set-up time), never inside an apply.
+static int ensure_seam(phast_hex8_plan *pl, int32_t kz) {
+    const int64_t need = 3 * (int64_t)pl->geo.nx * pl->geo.ny * ceil_div(pl->geo.nz, kz);
+    if (pl->seam && pl->seam_cap >= need) return PHAST_OK;
+    (void)hipFree(pl->seam);
+    pl->seam = nullptr;
+    pl->seam_cap = 0;
+    PHAST_HIP(hipMalloc(&pl->seam, sizeof(double) * (size_t)need));
+    pl->seam_cap = need;
     return PHAST_OK;
 }
 
@@ -997,6 +1063,7 @@ extern "C" int phast_hex8_set_tiles(phast_hex8_plan *pl, int32_t tx, int32_t ty,
     PHAST_REQUIRE((tx == 16 && ty == 16) || (tx == 16 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8) ||
                       (tx == 16 && ty == 14) || (tx == 32 && ty == 14),
                   "unsupported hex8 tile %dx%d", tx, ty);
+    if (int rc = ensure_seam(pl, kz)) return rc;
     pl->tx = tx;
     pl->ty = ty;
     pl->kz = kz;

@@ -29,6 +29,8 @@ struct phast_hex8_plan {
     bool iso = false;   // every bank entry isotropic: stress from (lam, mu) per voxel
     bool owns_material = true;
     double *part = nullptr;  // block partials (8 per block)
+    double *seam = nullptr;  // voxel-parallel apply: slab seams (seam_cap doubles)
+    int64_t seam_cap = 0;
     int32_t max_blocks = 0;
     int32_t tx = 16, ty = 8, kz = 8;  // launch shape of the apply kernels (phast_hex8_set_tiles)
 };
