@@ -29,9 +29,11 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6         # MI355X FP64 (vector and matrix peaks coincide)
 HEX8_BYTES_PER_NODE = 57.0      # u 24 + d 8 + phase 1 + r 24
 HEX8_FLOPS_PER_VOXEL = 940.0    # sum-factorised apply
-# HBM bytes per launch of the 64^3 apply measured with rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE
-# x 1.0 KiB, copy-calibrated), per launch shape; see profiles/hex8_apply_64cube_pore.md
-HEX8_64_TRAFFIC = {(16, 14, 8): (16164.69 * 2.0 + 6164.15) * 1024.0, (32, 8, 4): (11522.0 * 2.0 + 6176.4) * 1024.0}
+# HBM bytes per 64^3 apply (the apply kernel plus its seam/finalize kernel) measured with
+# rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, copy-calibrated), per launch shape;
+# see profiles/hex8_apply_64cube_pore.md
+HEX8_64_TRAFFIC = {(32, 14, 3): ((12528.61 + 2147.25) * 2.0 + 8269.75 + 2112.06) * 1024.0,
+                   (16, 14, 6): ((15001.75 + 1091.25) * 2.0 + 7213.75 + 1056.06) * 1024.0}
 Q4_APPLY_BYTES_PER_NODE = 40.0  # u 16 + d 8 + f 16
 Q4_STEP_BYTES_PER_NODE = 192.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, H r+w 64
 
@@ -194,8 +196,9 @@ def main():
                      "flops_per_voxel": HEX8_FLOPS_PER_VOXEL, "hbm_algorithmic_GB_s": hbm_gbs,
                      "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
                      "note": "FP64-bound (16.5 FLOP/B > ridge 9.8): peak is the FP64 figure, computed on the "
-                             "vector FP64 units; traffic (bytes/launch) from rocprofv3 counters for the "
-                             "profiled launch shape, null for another"},
+                             "vector FP64 units; one launch = apply kernel + seam/finalize kernel, timed "
+                             "together; traffic (bytes/launch) from rocprofv3 counters for the profiled "
+                             "launch shape, null for another"},
         "cpu_baseline": cpu,
         "secondary": sec,
         "wall_s_timed": wall,
