@@ -231,6 +231,7 @@ def secondary(rng, dev, a):
         n, E = mesh.n_nodes, mesh.n_elems
         u = T(rng.uniform(-1e-4, 1e-4, (n, 2)))
         d = T(rng.uniform(0, 1, n))
+        timings = {}
         if a.q4_tile:
             best = tuple(int(v) for v in a.q4_tile.split(","))
             fem = P.FEMOperators(mesh, mat, tile=best)
@@ -239,7 +240,8 @@ def secondary(rng, dev, a):
         steps = max(10, a.steps // (10 if n > 1e6 else 1))
         t, _ = events_time(lambda: fem.internal_force(u, d), steps, 3)
         entry = {"apply_GDOF_s": 2 * n / t / 1e9, "apply_GB_s": Q4_APPLY_BYTES_PER_NODE * n / t / 1e9,
-                 "apply_ms": t * 1e3, "tile": list(best)}
+                 "apply_ms": t * 1e3, "tile": list(best),
+                 "autotune_ms": {f"{k[0]},{k[1]}": round(v, 4) for k, v in timings.items()}}
         dyn = P.ExplicitDynamics(fem, dt=0.5 * fem.dt_cfl)
         v, acc = T(rng.uniform(-1e-2, 1e-2, (n, 2))), T(np.zeros((n, 2)))
         fext = T(np.zeros((n, 2)))

@@ -9,7 +9,7 @@ import torch
 HEX8_TILES = [(16, 16, 8), (16, 16, 16), (32, 8, 2), (32, 8, 3), (32, 8, 4), (32, 8, 8), (32, 8, 16), (16, 14, 2),
               (16, 14, 3), (16, 14, 4), (16, 14, 6), (16, 14, 8), (16, 14, 12), (16, 14, 16), (32, 14, 2), (32, 14, 3),
               (32, 14, 4), (32, 14, 6), (32, 14, 8), (32, 14, 16), (16, 8, 8)]
-Q4_TILES = [(128, 8), (128, 32), (256, 16), (256, 64), (64, 32), (256, 128)]
+Q4_TILES = [(128, 8), (128, 32), (256, 8), (256, 16), (256, 32), (256, 64), (64, 32), (256, 128)]
 BATCH_TILES = [(64, 4), (64, 8), (128, 2), (128, 4), (128, 8), (256, 2), (256, 4), (256, 8), (256, 16)]
 
 _cache: dict = {}

@@ -76,7 +76,7 @@ def test_voxel_parallel_apply_matches_marching_apply(dev, tiles):
     ref, ref_d = op.mech_action(x, g), op.mech_action_d(x, d)
     ref_s = op._stress_sums(x, g)
     op.set_tiles(*tiles)
-    for _ in range(2):  # the seam counters must come back to zero between launches
+    for _ in range(2):  # repeated launches reuse the partial and seam buffers
         y, y_d = op.mech_action(x, g), op.mech_action_d(x, d)
         n = 3 * op.n_red
         assert torch.equal(y[:n], ref[:n]) and torch.equal(y_d[:n], ref_d[:n])
@@ -84,6 +84,43 @@ def test_voxel_parallel_apply_matches_marching_apply(dev, tiles):
         assert rel(op._stress_sums(x, g), ref_s) < 1e-13
     xo, go = x.cpu().numpy(), g.cpu().numpy()
     assert rel(op.mech_action(x, g), oop.mech_action(xo, go)) < RTOL
+
+
+@pytest.mark.parametrize("tiles", [(16, 8, 8), (32, 14, 3), (16, 14, 6)])
+def test_general_stiffness_bank(dev, tiles):
+    """Bank entries that are not isotropic take the kernels' general 6x6 stiffness path
+    (per-voxel bank index instead of lam/mu): apply, fused apply, stress sums, diagonal."""
+    import dataclasses
+    from phast_amd import solid3d as S
+    from phast_amd import geometry as G
+    shape, h = (13, 11, 9), (1e-3, 1.1e-3, 0.9e-3)
+    rng = np.random.default_rng(11)
+    phase = np.full(shape[::-1], G.MATRIX, dtype=np.uint8)
+    phase[rng.uniform(0, 1, phase.shape) < 0.4] = G.INCLUSION
+    phases = {"mi_matrix": S.PhaseMaterial("mi_matrix", 3000.0, 0.35, 5e-4),
+              "fibre": S.PhaseMaterial("fibre", 70000.0, 0.3, 5e-3),
+              "pore": S.PhaseMaterial("pore", 3e-3, 0.35, 5e-4)}
+    mats = S.build_element_arrays(phase, phases, energy_split_isotropic="spectral")
+    bank = np.array(mats.stiffness_bank, dtype=np.float64)
+    for k in range(bank.shape[0]):  # symmetric positive perturbation couples every component
+        a = rng.uniform(-1, 1, (6, 6))
+        bank[k] += 0.05 * np.abs(bank[k]).max() * (a @ a.T) / 6.0
+    mats = dataclasses.replace(mats, stiffness_bank=bank)
+    load = S.LoadCase("uni", 0, (1e-3,), (1, 2, 3, 4, 5), ())
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid(shape, h), mats, load, kappa=1e-6, ell=2e-3, tiles=tiles)
+    oop = O.VoxelOperator(shape, h, bank, mats.stiffness_index, mats.Gc, mats.lam, mats.mu,
+                          mats.driver.astype(np.int64), kappa=1e-6, ell=2e-3, pinned_node=op.pinned_node,
+                          controlled=0)
+    x = rng.uniform(-1e-6, 1e-6, op.n_mech)
+    x[-6:] = rng.uniform(-1e-3, 1e-3, 6)
+    d = rng.uniform(0, 1, op.n_red)
+    g = oop.degradation(d)
+    ref = oop.mech_action(x, g)
+    assert rel(op.mech_action(T(x, dev), T(g, dev)), ref) < RTOL
+    assert rel(op.mech_action_d(T(x, dev), T(d, dev)), ref) < RTOL
+    assert rel(op.macro_stress(T(x, dev), T(g, dev)), oop.macro_stress(x, g)) < RTOL
+    assert rel(op.elastic_energy(T(x, dev), T(g, dev)), oop.elastic_energy(x, g)) < RTOL
+    assert rel(op.mech_diagonal(T(g, dev)), oop.mech_diagonal(g)) < RTOL
 
 
 def test_diag_stress_energy_driving(dev):
