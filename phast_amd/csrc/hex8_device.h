@@ -166,4 +166,118 @@ __device__ __forceinline__ void eig3(double a11, double a22, double a33, double 
     }
 }
 
+// Internal force of one voxel: r[c][corner] = sum_q w B_q^T (g_q C (B_q u + E))
+// (operator.py:1108-1169).  With OWN, sE += sum_q w sigma_q and (ENERGY) en += 0.5 sum_q
+// w sigma.eps.  sum-factorised gradients; g w folded into the moduli.
+template <bool ISO, bool ENERGY>
+__device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8], const double E[6],
+                                           const double g[8], const double *__restrict__ C, double lam,
+                                           double mu, bool own, double r[3][8], double sE[6], double &en) {
+    double gx[3][2][2], gy[3][2][2], gz[3][2][2];
+    voxel_grad(G, u, gx, gy, gz);
+    fold_macro(E, gx, gy, gz);
+    // each partial sum takes two Gauss points: the first assigns, the second adds
+    double Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
+    double ew = 0.0;
+#pragma unroll
+    for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < 2; ++ix) {
+                double e[6];
+                qp_strain(gx, gy, gz, ix, iy, iz, e);
+                const double gw = g[ix + 2 * iy + 4 * iz] * G.w;
+                double s[6];
+                if (ISO) {
+                    const double lg = lam * gw, mg = mu * gw, mg2 = 2.0 * mg;
+                    const double lt = lg * ((e[0] + e[1]) + e[2]);
+                    s[0] = lt + mg2 * e[0];
+                    s[1] = lt + mg2 * e[1];
+                    s[2] = lt + mg2 * e[2];
+                    s[3] = mg * e[3];
+                    s[4] = mg * e[4];
+                    s[5] = mg * e[5];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        double t = 0.0;
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) t += C[k * 6 + m] * e[m];
+                        s[k] = gw * t;
+                    }
+                }
+                if (ENERGY) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) t += s[k] * e[k];
+                    ew += t;
+                }
+                if (ix == 0) {
+                    Sx[0][iy][iz] = s[0];
+                    Sx[1][iy][iz] = s[5];
+                    Sx[2][iy][iz] = s[4];
+                } else {
+                    Sx[0][iy][iz] += s[0];
+                    Sx[1][iy][iz] += s[5];
+                    Sx[2][iy][iz] += s[4];
+                }
+                if (iy == 0) {
+                    Sy[0][ix][iz] = s[5];
+                    Sy[1][ix][iz] = s[1];
+                    Sy[2][ix][iz] = s[3];
+                } else {
+                    Sy[0][ix][iz] += s[5];
+                    Sy[1][ix][iz] += s[1];
+                    Sy[2][ix][iz] += s[3];
+                }
+                if (iz == 0) {
+                    Sz[0][ix][iy] = s[4];
+                    Sz[1][ix][iy] = s[3];
+                    Sz[2][ix][iy] = s[2];
+                } else {
+                    Sz[0][ix][iy] += s[4];
+                    Sz[1][ix][iy] += s[3];
+                    Sz[2][ix][iy] += s[2];
+                }
+            }
+    if (own) {  // sum_q w sigma_q from the partial sums (each component appears once per channel)
+        sE[0] += (Sx[0][0][0] + Sx[0][1][0]) + (Sx[0][0][1] + Sx[0][1][1]);
+        sE[1] += (Sy[1][0][0] + Sy[1][1][0]) + (Sy[1][0][1] + Sy[1][1][1]);
+        sE[2] += (Sz[2][0][0] + Sz[2][1][0]) + (Sz[2][0][1] + Sz[2][1][1]);
+        sE[3] += (Sy[2][0][0] + Sy[2][1][0]) + (Sy[2][0][1] + Sy[2][1][1]);
+        sE[4] += (Sx[2][0][0] + Sx[2][1][0]) + (Sx[2][0][1] + Sx[2][1][1]);
+        sE[5] += (Sx[1][0][0] + Sx[1][1][0]) + (Sx[1][0][1] + Sx[1][1][1]);
+        if (ENERGY) en += 0.5 * ew;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double R[2][2];
+        interp2Ts(G.al, G.be, G.alx, G.bex, Sx[c], R);  // R[ly][lz]
+#pragma unroll
+        for (int ly = 0; ly < 2; ++ly)
+#pragma unroll
+            for (int lz = 0; lz < 2; ++lz) {
+                r[c][1 + 2 * ly + 4 * lz] = R[ly][lz];
+                r[c][2 * ly + 4 * lz] = -R[ly][lz];
+            }
+        interp2Ts(G.al, G.be, G.aly, G.bey, Sy[c], R);  // R[lx][lz]
+#pragma unroll
+        for (int lx = 0; lx < 2; ++lx)
+#pragma unroll
+            for (int lz = 0; lz < 2; ++lz) {
+                r[c][lx + 2 + 4 * lz] += R[lx][lz];
+                r[c][lx + 4 * lz] -= R[lx][lz];
+            }
+        interp2Ts(G.al, G.be, G.alz, G.bez, Sz[c], R);  // R[lx][ly]
+#pragma unroll
+        for (int lx = 0; lx < 2; ++lx)
+#pragma unroll
+            for (int ly = 0; ly < 2; ++ly) {
+                r[c][lx + 2 * ly + 4] += R[lx][ly];
+                r[c][lx + 2 * ly] -= R[lx][ly];
+            }
+    }
+}
+
 }  // namespace phast

@@ -10,6 +10,7 @@
 // after, same weight), the coarsest level takes a fixed number of Jacobi sweeps, so the
 // V-cycle is a fixed symmetric positive operator and plain PCG applies.  Every level
 // pins the node above the fine pinned node and the same macro components.
+#include "hex8_device.h"
 #include "hex8_internal.h"
 
 #include <cstring>
@@ -128,6 +129,82 @@ __global__ __launch_bounds__(256) void k_jacobi(int64_t n, double *x, const doub
     }
     const double r = b[i] - (first ? 0.0 : Ax[i]);
     x[i] = (first ? 0.0 : x[i]) + omega * r / diag[i];
+}
+
+// Every damped-Jacobi sweep of a small coarsest level in one block (one thread per voxel,
+// at most kCoarseMax voxels): x, the per-voxel corner forces and the macro rows of A x live
+// in LDS, so a sweep costs two barriers instead of three kernel launches.  Per sweep the
+// same arithmetic as hex8_apply (g = 1, isotropic moduli) followed by k_jacobi; the nodal
+// gather visits a node's 8 voxels in a fixed order, and the macro rows are block sums.
+constexpr int kCoarseMax = 512;
+__global__ __launch_bounds__(kCoarseMax) void k_coarse_sweeps(const Hex8Geo G, const double *lam, const double *mu,
+                                                              const double *b, const double *diag,
+                                                              const uint8_t *fixed, double omega, int sweeps,
+                                                              double *x_out) {
+    constexpr int NT = kCoarseMax;
+    __shared__ double sx[3 * NT + 6];
+    __shared__ double sr[24][NT];
+    __shared__ double red[6 * (NT / 64)];
+    const int nx = G.nx, ny = G.ny, nz = G.nz, n = nx * ny * nz;
+    const int t = threadIdx.x;
+    const bool mine = t < n;
+    const int i = mine ? t % nx : 0, j = mine ? (t / nx) % ny : 0, k = mine ? t / (nx * ny) : 0;
+    // this thread's dofs: node t (3 components) and, for t < 6, macro dof 3n + t
+    double bb[4] = {0.0, 0.0, 0.0, 0.0}, dg[4] = {1.0, 1.0, 1.0, 1.0};
+    bool fx[4] = {true, true, true, true};
+    if (mine)
+        for (int c = 0; c < 3; ++c) {
+            bb[c] = b[3 * t + c];
+            dg[c] = diag[3 * t + c];
+            fx[c] = fixed[3 * t + c] != 0;
+        }
+    if (t < 6) {
+        bb[3] = b[3 * n + t];
+        dg[3] = diag[3 * n + t];
+        fx[3] = fixed[3 * n + t] != 0;
+    }
+    const double l0 = mine ? lam[t] : 0.0, m0 = mine ? mu[t] : 0.0;
+    // first sweep: x = omega D^-1 b on free dofs, 0 on fixed
+    if (mine)
+        for (int c = 0; c < 3; ++c) sx[3 * t + c] = fx[c] ? 0.0 : omega * bb[c] / dg[c];
+    if (t < 6) sx[3 * n + t] = fx[3] ? 0.0 : omega * bb[3] / dg[3];
+    __syncthreads();
+    const double g1[8] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+    for (int s = 1; s < sweeps; ++s) {
+        double sE[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, en = 0.0;
+        if (mine) {
+            double u[3][8], E[6], r[3][8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const int node = wrap(i + (a & 1), nx) + nx * (wrap(j + ((a >> 1) & 1), ny) + ny * wrap(k + (a >> 2), nz));
+#pragma unroll
+                for (int c = 0; c < 3; ++c) u[c][a] = sx[3 * node + c];
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) E[q] = sx[3 * n + q];
+            voxel_mech<true, false>(G, u, E, g1, nullptr, l0, m0, true, r, sE, en);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int a = 0; a < 8; ++a) sr[c * 8 + a][t] = r[c][a];
+        }
+        block_sum_n<NT, 6>(sE, red);  // also the barrier after the corner forces
+        double ax[3] = {0.0, 0.0, 0.0};
+        if (mine)
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {  // node t is corner a of voxel (i-lx, j-ly, k-lz)
+                const int v = wrap(i - (a & 1), nx) + nx * (wrap(j - ((a >> 1) & 1), ny) + ny * wrap(k - (a >> 2), nz));
+#pragma unroll
+                for (int c = 0; c < 3; ++c) ax[c] += sr[c * 8 + a][v];
+            }
+        __syncthreads();  // every thread has read x for this sweep's apply
+        if (mine)
+            for (int c = 0; c < 3; ++c)
+                if (!fx[c]) sx[3 * t + c] += omega * (bb[c] - ax[c]) / dg[c];
+        if (t < 6 && !fx[3]) sx[3 * n + t] += omega * (bb[3] - sE[t]) / dg[3];
+        __syncthreads();
+    }
+    for (int q = t; q < 3 * n + 6; q += NT) x_out[q] = sx[q];
 }
 
 // r = b - Ax on free dofs, 0 on fixed.
@@ -371,6 +448,12 @@ static int vcycle(phast_hex8_mg *mg, int l, hipStream_t st) {
     const double *d = l == 0 ? mg->d : nullptr;
     const bool coarsest = l == mg->levels - 1;
     const int pre = coarsest ? mg->coarse_sweeps : mg->nu;
+    if (coarsest && mg->n[l] <= kCoarseMax && l > 0) {  // all sweeps in one block
+        hipLaunchKernelGGL(k_coarse_sweeps, dim3(1), dim3(kCoarseMax), 0, st, pl->geo, pl->lam, pl->mu, mg->b[l],
+                           mg->diag[l], mg->fixed[l], mg->omega_l[l], pre, mg->x[l]);
+        PHAST_LAUNCHED();
+        return PHAST_OK;
+    }
     // x = omega D^-1 b, then pre - 1 more sweeps
     hipLaunchKernelGGL(k_jacobi, dim3(gb), dim3(256), 0, st, nm, mg->x[l], mg->b[l], mg->Ax[l], mg->diag[l],
                        mg->fixed[l], mg->omega_l[l], 1);
