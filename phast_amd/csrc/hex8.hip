@@ -193,6 +193,7 @@ __device__ __forceinline__ void settle(double &a, double &b, int32_t &c) {
     asm volatile("" : "+v"(a), "+v"(b), "+v"(c));
 }
 __device__ __forceinline__ void settle2(double &a, double &b) { asm volatile("" : "+v"(a), "+v"(b)); }
+__device__ __forceinline__ void settle1(double &a) { asm volatile("" : "+v"(a)); }
 
 // threads of the voxel-parallel block: the voxel tile rounded up to whole waves
 template <int OX, int OY>
@@ -785,6 +786,140 @@ int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g,
     return rc;
 }
 
+// Voxel-parallel damage operator y = scale (Gc K d + ch N^T (w H) N d) (operator.py:1450-1509),
+// organised as hex8_apply_kernel: one thread per voxel of an (OX+1) x (OY+1) tile, node
+// layers of d and the voxel operands (Gc, H) fetched one layer ahead with branch-free loads,
+// 6 corner values per voxel staged in LDS, and the owner of node (ox, oy) gathering its 8
+// voxels in the marching kernel's order -- so y is bitwise the marching kernel's.  The slab
+// starts one voxel layer early (k0-1) to form the carry into its first node layer.
+template <int OX, int OY, bool HIST>
+__global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_damage_vp_kernel(const Hex8Geo G, DmgArgs3 A) {
+    constexpr int NT = apply_threads<OX, OY>();
+    constexpr int NVX = OX + 1, NVY = OY + 1, NV = NVX * NVY;
+    constexpr int NNX = OX + 2, NNY = OY + 2, NN = NNX * NNY;
+    constexpr int NS = (NN + NT - 1) / NT;
+    __shared__ double su[2][NS * NT];  // node layers of d; slots past NN are padding
+    __shared__ double st[6][NV];       // corners 1, 2, 3, 5, 6, 7 of each voxel
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int tiles = A.tiles_x * A.tiles_y;
+    const int bxy = blockIdx.x % tiles, bz = blockIdx.x / tiles;
+    const int i0 = (bxy % A.tiles_x) * OX, j0 = (bxy / A.tiles_x) * OY;
+    const int k0 = bz * A.kz, k1 = min(k0 + A.kz, nz);
+    const int t = threadIdx.x;
+    const int vx = t % NVX, vy = t / NVX;
+    const bool has_voxel = t < NV;
+    const int iv = i0 - 1 + vx, jv = j0 - 1 + vy;
+    const bool owner = has_voxel && vx >= 1 && vy >= 1 && iv < nx && jv < ny;
+    const int e_ij = has_voxel ? wrapm(iv, nx) + nx * wrapm(jv, ny) : 0;
+    const int64_t nxy = (int64_t)nx * ny;
+    int64_t slot_ij[NS];
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+        const int s = t + m * NT;
+        slot_ij[m] = s < NN ? wrapm(i0 - 1 + s % NNX, nx) + (int64_t)nx * wrapm(j0 - 1 + s / NNX, ny) : 0;
+    }
+    double pf[NS];
+    auto fetch = [&](int k, double (&dst)[NS]) {
+        const int64_t base = nxy * wrap(k, nz);
+#pragma unroll
+        for (int m = 0; m < NS; ++m) dst[m] = A.d[base + slot_ij[m]];
+    };
+    auto stage = [&](int buf, const double (&src)[NS]) {
+#pragma unroll
+        for (int m = 0; m < NS; ++m) su[buf][t + m * NT] = src[m];
+    };
+    auto fetch_vox = [&](int k, double &gc, double (&h)[8]) {
+        const int64_t e = e_ij + nxy * wrap(k, nz);
+        gc = A.Gc[e];
+        if (HIST)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) h[q] = A.H[e * 8 + q];
+    };
+    double gc0 = 0.0, h0[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    fetch_vox(k0 - 1, gc0, h0);
+    {
+        double pf1[NS];
+        fetch(k0 - 1, pf);
+        fetch(k0, pf1);
+        stage(0, pf);
+        stage(1, pf1);
+    }
+    settle1(gc0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) settle1(h0[q]);
+    const int s00 = t - NVX - 1, s10 = t - NVX, s01 = t - 1;
+    double acc = 0.0;
+    int cur = 0;
+    for (int kv = k0 - 1; kv < k1; ++kv) {
+        __syncthreads();
+        if (kv + 2 <= k1) fetch(kv + 2, pf);
+        double gc1 = 0.0, h1[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (kv + 1 < k1) fetch_vox(kv + 1, gc1, h1);
+        double y[8];
+        if (has_voxel) {
+            double dc[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+                dc[a] = su[(a >> 2) ? (cur ^ 1) : cur][(vy + ((a >> 1) & 1)) * NNX + (vx + (a & 1))];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                double tt = 0.0;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) tt += dc[b] * G.K[b * 8 + a];
+                y[a] = gc0 * tt;
+            }
+            if (HIST) {
+                double dq[8];
+                voxel_interp(G, dc, dq);
+                double hq[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) hq[q] = (h0[q] * G.w) * dq[q];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    double tt = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) tt += hq[q] * G.Nq[q * 8 + a];
+                    y[a] += G.ch * tt;
+                }
+            }
+            st[0][t] = y[1];
+            st[1][t] = y[2];
+            st[2][t] = y[3];
+            st[3][t] = y[5];
+            st[4][t] = y[6];
+            st[5][t] = y[7];
+        }
+        __syncthreads();
+        if (kv + 2 <= k1) stage(cur, pf);  // node layer kv+2 replaces layer kv
+        gc0 = gc1;
+        settle1(gc0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            h0[q] = h1[q];
+            settle1(h0[q]);
+        }
+        if (owner) {
+            const double bot = ((st[2][s00] + st[1][s10]) + st[0][s01]) + y[0];
+            const double top = ((st[5][s00] + st[4][s10]) + st[3][s01]) + y[4];
+            if (kv >= k0) A.y[iv + (int64_t)nx * (jv + (int64_t)ny * kv)] = A.scale * (acc + bot);
+            acc = top;
+        }
+        cur ^= 1;
+    }
+}
+
+template <int OX, int OY>
+static int launch_damage_vp_t(const phast_hex8_plan *pl, DmgArgs3 A, hipStream_t st) {
+    A.tiles_x = (int)ceil_div(pl->geo.nx, OX);
+    A.tiles_y = (int)ceil_div(pl->geo.ny, OY);
+    const int tz = (int)ceil_div(pl->geo.nz, A.kz);
+    dim3 grid(A.tiles_x * A.tiles_y * tz), block(apply_threads<OX, OY>());
+    if (A.H) hipLaunchKernelGGL((hex8_damage_vp_kernel<OX, OY, true>), grid, block, 0, st, pl->geo, A);
+    else hipLaunchKernelGGL((hex8_damage_vp_kernel<OX, OY, false>), grid, block, 0, st, pl->geo, A);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
 template <int TX, int TY>
 static int launch_damage_t(const phast_hex8_plan *pl, DmgArgs3 A, hipStream_t st) {
     A.tiles_x = (int)ceil_div(pl->geo.nx, TX);
@@ -1017,11 +1152,12 @@ extern "C" int phast_hex8_damage_action(const phast_hex8_plan *pl, const double 
     A.y = y;
     A.scale = scale;
     A.kz = pl->kz;
-    if (pl->tx == 16 && pl->ty == 16) return launch_damage_t<16, 16>(pl, A, st);
+    if (pl->tx == 16 && pl->ty == 16) return launch_damage_vp_t<16, 16>(pl, A, st);
     if (pl->tx == 16 && pl->ty == 8) return launch_damage_t<16, 8>(pl, A, st);
     if (pl->tx == 8 && pl->ty == 8) return launch_damage_t<8, 8>(pl, A, st);
-    if (pl->tx == 32 && pl->ty == 8) return launch_damage_t<32, 8>(pl, A, st);
-    if (pl->ty == 14) return launch_damage_t<16, 16>(pl, A, st);  // damage pass: any column tile
+    if (pl->tx == 32 && pl->ty == 8) return launch_damage_vp_t<32, 8>(pl, A, st);
+    if (pl->tx == 16 && pl->ty == 14) return launch_damage_vp_t<16, 14>(pl, A, st);
+    if (pl->tx == 32 && pl->ty == 14) return launch_damage_vp_t<32, 14>(pl, A, st);
     set_error("unsupported hex8 tile");
     return PHAST_ERR_UNSUPPORTED;
 }
