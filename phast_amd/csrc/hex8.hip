@@ -415,6 +415,53 @@ __global__ __launch_bounds__(256) void hex8_seam_finalize_kernel(const Hex8Geo G
     y[at] = seam[i] + y[at];
 }
 
+// A multigrid smoothing step's finalize: the macro rows (block 0) and, after the
+// voxel-parallel apply, the seams, fused with the damped Jacobi update
+// x += omega (b - A x) / diag on free dofs (fixed dofs stay 0) -- k_jacobi's arithmetic.
+__global__ __launch_bounds__(256) void hex8_finalize_jacobi_kernel(const Hex8Geo G, const double *seam, int kz,
+                                                                   double *Ax, const double *part, int nblocks,
+                                                                   const double *b, const double *diag,
+                                                                   const uint8_t *fixed, double omega, double *x) {
+    const int64_t nxy = (int64_t)G.nx * G.ny, n3 = 3 * nxy * G.nz;
+    if (blockIdx.x == 0) {
+        __shared__ double red[6 * 4];
+        double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int bb = threadIdx.x; bb < nblocks; bb += 256)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s[k] += part[8 * bb + k];
+        block_sum_n<256, 6>(s, red);
+        const int k = threadIdx.x;
+        if (k < 6) {
+            const double v = k == 0 ? s[0] : k == 1 ? s[1] : k == 2 ? s[2] : k == 3 ? s[3] : k == 4 ? s[4] : s[5];
+            const int64_t i = n3 + k;
+            Ax[i] = v;
+            if (fixed[i]) {
+                x[i] = 0.0;
+            } else {
+                const double r = b[i] - v;
+                x[i] = x[i] + omega * r / diag[i];
+            }
+        }
+        return;
+    }
+    const int64_t i = (int64_t)(blockIdx.x - 1) * 256 + threadIdx.x;
+    if (i >= n3) return;
+    double v = Ax[i];
+    if (seam) {
+        const int64_t q = i / 3, layer = q / nxy;
+        if (layer % kz == 0) {
+            v = seam[3 * ((layer / kz) * nxy + (q - layer * nxy)) + (i - 3 * q)] + v;
+            Ax[i] = v;
+        }
+    }
+    if (fixed[i]) {
+        x[i] = 0.0;
+        return;
+    }
+    const double r = b[i] - v;
+    x[i] = x[i] + omega * r / diag[i];
+}
+
 __global__ __launch_bounds__(256) void hex8_degradation_kernel(const Hex8Geo G, const double *d, double *g) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
@@ -660,9 +707,17 @@ __global__ __launch_bounds__(256) void hex8_damage_rd_kernel(const Hex8Geo G, co
     if (diag) diag[node] = scale * sd;
 }
 
+// Damped Jacobi update fused into the apply's finalize (multigrid smoothing).
+struct JacobiSpec {
+    const double *b, *diag;
+    const uint8_t *fixed;
+    double omega;
+    double *x;
+};
+
 template <int TX, int TY>
 static int launch_mech_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool energy, hipStream_t st,
-                         int *nblocks) {
+                         int *nblocks, const JacobiSpec *jac) {
     A.tiles_x = (int)ceil_div(pl->geo.nx, TX);
     A.tiles_y = (int)ceil_div(pl->geo.ny, TY);
     const int tz = (int)ceil_div(pl->geo.nz, A.kz);
@@ -698,13 +753,19 @@ static int launch_mech_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool
     }
 #undef PHAST_L
     PHAST_LAUNCHED();
-    hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, A.part, nb, A.nvals, A.y6, A.out7);
+    if (jac)
+        hipLaunchKernelGGL(hex8_finalize_jacobi_kernel, dim3((unsigned)(1 + ceil_div(3 * pl->n, 256))), dim3(256), 0,
+                           st, pl->geo, (const double *)nullptr, 1, A.y, A.part, nb, jac->b, jac->diag, jac->fixed,
+                           jac->omega, jac->x);
+    else
+        hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, A.part, nb, A.nvals, A.y6, A.out7);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
 
 template <int OX, int OY>
-static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool energy, hipStream_t st) {
+static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool energy, hipStream_t st,
+                          const JacobiSpec *jac) {
     A.tiles_x = (int)ceil_div(pl->geo.nx, OX);
     A.tiles_y = (int)ceil_div(pl->geo.ny, OY);
     const int nbz = (int)ceil_div(pl->geo.nz, A.kz);
@@ -744,14 +805,19 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
 #undef PHAST_L
     PHAST_LAUNCHED();
     const int64_t nseam = 3 * (int64_t)pl->geo.nx * pl->geo.ny * nbz;
-    hipLaunchKernelGGL(hex8_seam_finalize_kernel, dim3((unsigned)(1 + ceil_div(nseam, 256))), dim3(256), 0, st, pl->geo,
-                       A.seam, A.kz, nbz, A.y, A.part, nb, A.nvals, A.y6, A.out7);
+    if (jac)
+        hipLaunchKernelGGL(hex8_finalize_jacobi_kernel, dim3((unsigned)(1 + ceil_div(3 * pl->n, 256))), dim3(256), 0,
+                           st, pl->geo, A.seam, A.kz, A.y, A.part, nb, jac->b, jac->diag, jac->fixed, jac->omega,
+                           jac->x);
+    else
+        hipLaunchKernelGGL(hex8_seam_finalize_kernel, dim3((unsigned)(1 + ceil_div(nseam, 256))), dim3(256), 0, st,
+                           pl->geo, A.seam, A.kz, nbz, A.y, A.part, nb, A.nvals, A.y6, A.out7);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
 
-int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
-                       double *out7, bool energy, hipStream_t st) {
+static int hex8_apply_any(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
+                          double *out7, bool energy, hipStream_t st, const JacobiSpec *jac) {
     MechArgs3 A;
     std::memset(&A, 0, sizeof(A));
     A.x = x;
@@ -773,17 +839,28 @@ int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g,
         set_error("internal: hex8 apply without output");
         return PHAST_ERR_ARG;
     }
-    if (pl->tx == 16 && pl->ty == 16) rc = launch_apply_t<16, 16>(pl, A, gmode, energy, st);
-    else if (pl->tx == 32 && pl->ty == 8) rc = launch_apply_t<32, 8>(pl, A, gmode, energy, st);
-    else if (pl->tx == 16 && pl->ty == 14) rc = launch_apply_t<16, 14>(pl, A, gmode, energy, st);
-    else if (pl->tx == 32 && pl->ty == 14) rc = launch_apply_t<32, 14>(pl, A, gmode, energy, st);
-    else if (pl->tx == 16 && pl->ty == 8) rc = launch_mech_t<16, 8>(pl, A, gmode, energy, st, &nb);
-    else if (pl->tx == 8 && pl->ty == 8) rc = launch_mech_t<8, 8>(pl, A, gmode, energy, st, &nb);
+    if (pl->tx == 16 && pl->ty == 16) rc = launch_apply_t<16, 16>(pl, A, gmode, energy, st, jac);
+    else if (pl->tx == 32 && pl->ty == 8) rc = launch_apply_t<32, 8>(pl, A, gmode, energy, st, jac);
+    else if (pl->tx == 16 && pl->ty == 14) rc = launch_apply_t<16, 14>(pl, A, gmode, energy, st, jac);
+    else if (pl->tx == 32 && pl->ty == 14) rc = launch_apply_t<32, 14>(pl, A, gmode, energy, st, jac);
+    else if (pl->tx == 16 && pl->ty == 8) rc = launch_mech_t<16, 8>(pl, A, gmode, energy, st, &nb, jac);
+    else if (pl->tx == 8 && pl->ty == 8) rc = launch_mech_t<8, 8>(pl, A, gmode, energy, st, &nb, jac);
     else {
         set_error("unsupported hex8 tile %dx%d", pl->tx, pl->ty);
         return PHAST_ERR_UNSUPPORTED;
     }
     return rc;
+}
+
+int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
+                    double *out7, bool energy, hipStream_t st) {
+    return hex8_apply_any(pl, x, g, d, y, out7, energy, st, nullptr);
+}
+
+int hex8_apply_jacobi(const phast_hex8_plan *pl, double *x, const double *g, const double *d, double *Ax,
+                      const double *b, const double *diag, const uint8_t *fixed, double omega, hipStream_t st) {
+    const JacobiSpec jac{b, diag, fixed, omega, x};
+    return hex8_apply_any(pl, x, g, d, Ax, nullptr, false, st, &jac);
 }
 
 // Voxel-parallel damage operator y = scale (Gc K d + ch N^T (w H) N d) (operator.py:1450-1509),
@@ -982,7 +1059,6 @@ void phast::build_voxel_constants(Hex8Geo &G, double hx, double hy, double hz, d
     G.bez = G.be * G.ihz;
 }
 
-static int ensure_seam(phast_hex8_plan *pl, int32_t kz);
 
 extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,
                                       const double *bank, int32_t n_bank, const int32_t *bank_index,
@@ -1042,7 +1118,7 @@ extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double
         phast_hex8_plan_destroy(pl);
         return PHAST_ERR_HIP;
     }
-    if (int rc = ensure_seam(pl, pl->kz)) {
+    if (int rc = hex8_ensure_seam(pl, pl->kz)) {
         phast_hex8_plan_destroy(pl);
         return rc;
     }
@@ -1068,7 +1144,7 @@ extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
 
 // Seam buffer of the voxel-parallel apply for slabs of kz node layers: 3 doubles per node
 // of each slab's first layer.  Grown here (set-up time), never inside an apply.
-static int ensure_seam(phast_hex8_plan *pl, int32_t kz) {
+int phast::hex8_ensure_seam(phast_hex8_plan *pl, int32_t kz) {
     const int64_t need = 3 * (int64_t)pl->geo.nx * pl->geo.ny * ceil_div(pl->geo.nz, kz);
     if (pl->seam && pl->seam_cap >= need) return PHAST_OK;
     (void)hipFree(pl->seam);
@@ -1084,7 +1160,7 @@ extern "C" int phast_hex8_set_tiles(phast_hex8_plan *pl, int32_t tx, int32_t ty,
     PHAST_REQUIRE((tx == 16 && ty == 16) || (tx == 16 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8) ||
                       (tx == 16 && ty == 14) || (tx == 32 && ty == 14),
                   "unsupported hex8 tile %dx%d", tx, ty);
-    if (int rc = ensure_seam(pl, kz)) return rc;
+    if (int rc = hex8_ensure_seam(pl, kz)) return rc;
     pl->tx = tx;
     pl->ty = ty;
     pl->kz = kz;

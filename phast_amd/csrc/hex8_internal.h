@@ -41,4 +41,10 @@ namespace phast {
 int hex8_apply(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
                double *out7, bool energy, hipStream_t st);
 void build_voxel_constants(Hex8Geo &G, double hx, double hy, double hz, double ell);
+// One multigrid smoothing step: Ax = A(g) x, then x += omega (b - Ax) / diag on free dofs
+// (fixed dofs set to 0), the update fused into the apply's finalize kernel.
+int hex8_apply_jacobi(const phast_hex8_plan *pl, double *x, const double *g, const double *d, double *Ax,
+                      const double *b, const double *diag, const uint8_t *fixed, double omega, hipStream_t st);
+// Grow the seam buffer of the voxel-parallel apply for slabs of kz node layers (set-up time).
+int hex8_ensure_seam(phast_hex8_plan *pl, int32_t kz);
 }  // namespace phast

@@ -288,6 +288,7 @@ extern "C" int phast_hex8_mg_destroy(phast_hex8_mg *mg) {
             (void)hipFree(mg->lv[l]->lam);
             (void)hipFree(mg->lv[l]->mu);
             (void)hipFree(mg->lv[l]->part);
+            (void)hipFree(mg->lv[l]->seam);
             (void)hipFree(mg->lv[l]->bank);
             (void)hipFree(mg->lv[l]->bank_index);
             delete mg->lv[l];
@@ -347,7 +348,9 @@ extern "C" int phast_hex8_mg_create(const phast_hex8_plan *fine, int64_t pinned,
         pl->iso = true;
         pl->owns_material = false;
         pl->n_bank = 1;
-        pl->tx = 8; pl->ty = 8; pl->kz = 4;
+        // voxel-parallel apply with slab seams; thin slabs so that small levels still
+        // spread over many blocks
+        pl->tx = 16; pl->ty = 14; pl->kz = nz >= 32 ? 3 : (nz >= 16 ? 2 : 1);
         pl->max_blocks = (int32_t)(ceil_div(nx, 8) * ceil_div(ny, 8) * nz);
         mg->lv[l] = pl;
         mg->levels = l + 1;
@@ -358,7 +361,7 @@ extern "C" int phast_hex8_mg_create(const phast_hex8_plan *fine, int64_t pinned,
             hipMalloc(&pl->bank, sizeof(double) * 36) != hipSuccess ||
             hipMemset(pl->bank, 0, sizeof(double) * 36) != hipSuccess ||
             hipMalloc(&pl->bank_index, sizeof(int32_t) * pl->n) != hipSuccess ||
-            hipMemset(pl->bank_index, 0, sizeof(int32_t) * pl->n) != hipSuccess)
+            hipMemset(pl->bank_index, 0, sizeof(int32_t) * pl->n) != hipSuccess || hex8_ensure_seam(pl, pl->kz) != PHAST_OK)
             return fail("alloc");
         const int pi = (int)(pin % pnx), pj = (int)((pin / pnx) % pny), pk = (int)(pin / ((int64_t)pnx * pny));
         pin = (pi / 2) + (int64_t)nx * ((pj / 2) + (int64_t)ny * (pk / 2));
@@ -457,11 +460,9 @@ static int vcycle(phast_hex8_mg *mg, int l, hipStream_t st) {
     // x = omega D^-1 b, then pre - 1 more sweeps
     hipLaunchKernelGGL(k_jacobi, dim3(gb), dim3(256), 0, st, nm, mg->x[l], mg->b[l], mg->Ax[l], mg->diag[l],
                        mg->fixed[l], mg->omega_l[l], 1);
-    for (int s = 1; s < pre; ++s) {
-        PHAST_TRY0(hex8_apply(pl, mg->x[l], g, d, mg->Ax[l], nullptr, false, st));
-        hipLaunchKernelGGL(k_jacobi, dim3(gb), dim3(256), 0, st, nm, mg->x[l], mg->b[l], mg->Ax[l], mg->diag[l],
-                           mg->fixed[l], mg->omega_l[l], 0);
-    }
+    for (int s = 1; s < pre; ++s)
+        PHAST_TRY0(hex8_apply_jacobi(pl, mg->x[l], g, d, mg->Ax[l], mg->b[l], mg->diag[l], mg->fixed[l],
+                                     mg->omega_l[l], st));
     if (coarsest) return PHAST_OK;
     PHAST_TRY0(hex8_apply(pl, mg->x[l], g, d, mg->Ax[l], nullptr, false, st));
     hipLaunchKernelGGL(k_residual, dim3(gb), dim3(256), 0, st, nm, mg->b[l], mg->Ax[l], mg->fixed[l], mg->r[l]);
@@ -471,11 +472,9 @@ static int vcycle(phast_hex8_mg *mg, int l, hipStream_t st) {
     PHAST_TRY0(vcycle(mg, l + 1, st));
     hipLaunchKernelGGL(k_prolong_add, dim3(nb256(mg->n[l])), dim3(256), 0, st, c->geo.nx, c->geo.ny, c->geo.nz,
                        mg->x[l + 1], mg->x[l], mg->fixed[l]);
-    for (int s = 0; s < mg->nu; ++s) {
-        PHAST_TRY0(hex8_apply(pl, mg->x[l], g, d, mg->Ax[l], nullptr, false, st));
-        hipLaunchKernelGGL(k_jacobi, dim3(gb), dim3(256), 0, st, nm, mg->x[l], mg->b[l], mg->Ax[l], mg->diag[l],
-                           mg->fixed[l], mg->omega_l[l], 0);
-    }
+    for (int s = 0; s < mg->nu; ++s)
+        PHAST_TRY0(hex8_apply_jacobi(pl, mg->x[l], g, d, mg->Ax[l], mg->b[l], mg->diag[l], mg->fixed[l],
+                                     mg->omega_l[l], st));
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
