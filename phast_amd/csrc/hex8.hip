@@ -5,14 +5,19 @@
 // (E,8) rows with Gauss points ordered xi fastest (core/hex8.py:157-179); material data
 // is per voxel (bank index int32, Gc, lam, mu, driver int8).
 //
-// The element apply is a 2.5D "marching" kernel: a block owns a TX x TY column of nodes
-// and walks KZ node layers in z.  At each voxel layer it stages the two bounding node
-// layers (with a one-node x/y halo) in LDS, evaluates every voxel of the layer once with
-// sum-factorised gradients (the trilinear field's derivative along x is a difference in x
-// interpolated in y and z, and the transpose for the force), writes the 24 corner results
-// to LDS, and gathers each owned node's sum from the 4 voxels below and the 4 above it.
+// The element applies walk z slabs: a block owns a column of nodes and a slab of kz node
+// layers.  At each voxel layer it stages the two bounding node layers (with a one-node x/y
+// halo) in LDS, evaluates every voxel of the layer once with sum-factorised gradients (the
+// trilinear field's derivative along x is a difference in x interpolated in y and z, and
+// the transpose for the force), stages the corner results in LDS, and gathers each owned
+// node's sum from the 4 voxels below and the 4 above it.  Two organisations:
+//   * hex8_mech_kernel ("marching"): TX x TY threads = owned node columns, a strided loop
+//     over the (TX+1) x (TY+1) voxels, one extra voxel layer below the slab (the carry);
+//   * hex8_apply_kernel (voxel-parallel, the default): one thread per voxel, whole-wave
+//     blocks, node layers and material prefetched a layer ahead, and no carry layer -- the
+//     top faces of a slab's last layer go to a seam buffer that the follow-on kernel adds.
 // There are no atomics: the result is deterministic, and the 6 macro-strain rows are
-// reduced per block then summed by one block in a fixed order.
+// reduced per block then summed by one block of a follow-on kernel in a fixed order.
 #include "common.h"
 
 #include <cstring>
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
 
 
 // ---------------------------------------------------------------------------------------
-// Voxel-parallel apply: one thread per voxel of a (OX+1) x (OY+1) voxel tile (320 threads,
+// Voxel-parallel apply: one thread per voxel of a (OX+1) x (OY+1) voxel tile (whole waves,
 // no second pass for the halo voxels), walking a z slab of kz node layers.
 // ---------------------------------------------------------------------------------------
 //
