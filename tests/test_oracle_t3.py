@@ -1,0 +1,156 @@
+"""Pins of the T3 oracle (CPU): closed forms, patch tests, energy gradients, dense assembly."""
+import numpy as np
+import pytest
+
+from oracle import t3 as O
+from oracle.q4 import Material
+
+SPLITS = ["isotropic", "amor", "spectral"]
+
+
+def mesh(nx=5, ny=4, jitter=0.2, seed=0):
+    nodes, els = O.structured_t3(nx, ny, 5.0, 4.0, jitter=jitter, seed=seed)
+    return O.T3Mesh(nodes, els)
+
+
+def mat(split="amor", plane_stress=False):
+    return Material(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5, plane_stress=plane_stress,
+                    energy_split=split)
+
+
+def test_geometry_closed_forms():
+    m = O.T3Mesh(np.array([[0.0, 0.0], [2.0, 0.0], [0.0, 3.0]]), np.array([[0, 1, 2]]))
+    assert m.areas[0] == pytest.approx(3.0, rel=1e-15)
+    # phi0 = 1 - x/2 - y/3, phi1 = x/2, phi2 = y/3
+    np.testing.assert_allclose(m.grad_phi[0], [[-0.5, -1 / 3], [0.5, 0.0], [0.0, 1 / 3]], rtol=1e-15, atol=1e-16)
+    np.testing.assert_allclose(m.M_scalar, [1.0, 1.0, 1.0], rtol=1e-15)
+    mm = mesh()
+    np.testing.assert_allclose(mm.grad_phi.sum(1), 0.0, atol=1e-13)
+    assert mm.M_scalar.sum() == pytest.approx(20.0, rel=1e-13)
+    assert mm.areas.sum() == pytest.approx(20.0, rel=1e-13)
+
+
+@pytest.mark.parametrize("split", SPLITS)
+def test_linear_field_strain_and_patch(split):
+    mm = mesh()
+    op = O.T3Operators(mm, mat(split))
+    G = np.array([[1.1e-4, -0.4e-4], [0.7e-4, -0.5e-4]])
+    u = mm.nodes @ G.T + np.array([1e-3, -2e-3])
+    exx, eyy, gxy = op.compute_strain(u)
+    np.testing.assert_allclose(exx, G[0, 0], rtol=1e-9)
+    np.testing.assert_allclose(eyy, G[1, 1], rtol=1e-9)
+    np.testing.assert_allclose(gxy, G[0, 1] + G[1, 0], rtol=1e-9)
+    f = op.internal_force(u, np.zeros(mm.n_nodes))  # constant stress: interior nodes balanced
+    xs, ys = mm.nodes[:, 0], mm.nodes[:, 1]
+    inner = (xs > 1e-9) & (xs < 5 - 1e-9) & (ys > 1e-9) & (ys < 4 - 1e-9)
+    assert np.abs(f[inner]).max() < 1e-10 * np.abs(f).max()
+
+
+@pytest.mark.parametrize("split", ["isotropic", "amor"])
+def test_rigid_motion_has_no_force(split):
+    mm = mesh()
+    op = O.T3Operators(mm, mat(split))
+    th = 1e-9
+    u = np.stack([-th * mm.nodes[:, 1], th * mm.nodes[:, 0]], 1) + np.array([3e-4, -1e-4])
+    d = np.random.default_rng(1).uniform(0, 1, mm.n_nodes)
+    assert np.abs(op.internal_force(u, d)).max() < 1e-12
+
+
+def energy(op, u, d):
+    """sum_e A_e (g_e psi+_e + psi-_e) with psi- = psi_total - psi+ (plane strain)."""
+    s = op.compute_strain(u)
+    m = op.material
+    exx, eyy, gxy = s
+    psi_tot = 0.5 * m.lam * (exx + eyy) ** 2 + m.mu * (exx ** 2 + eyy ** 2 + 0.5 * gxy ** 2)
+    if m.energy_split == "isotropic":
+        return float((op.mesh.areas * op.degradation_e(d) * psi_tot).sum())
+    pp = op.compute_psi_plus(u, s)
+    if m.energy_split == "amor":  # psi- = 0.5 kappa <tr>-^2 (plane strain)
+        tr = exx + eyy
+        pm = 0.5 * m.kappa * np.minimum(tr, 0.0) ** 2
+    else:
+        pm = psi_tot - pp
+    return float((op.mesh.areas * (op.degradation_e(d) * pp + pm)).sum())
+
+
+@pytest.mark.parametrize("split", SPLITS)
+def test_force_is_energy_gradient(split):
+    mm = mesh(4, 3)
+    op = O.T3Operators(mm, mat(split))
+    rng = np.random.default_rng(2)
+    u = rng.uniform(-1e-4, 1e-4, (mm.n_nodes, 2))
+    d = rng.uniform(0, 0.9, mm.n_nodes)
+    f = op.internal_force(u, d)
+    h = 1e-9
+    fd = np.zeros_like(f)
+    for n in range(mm.n_nodes):
+        for c in range(2):
+            up, um = u.copy(), u.copy()
+            up[n, c] += h
+            um[n, c] -= h
+            fd[n, c] = (energy(op, up, d) - energy(op, um, d)) / (2 * h)
+    assert np.abs(f - fd).max() < 1e-5 * np.abs(f).max()
+
+
+def test_amor_hand_values_and_psi_split():
+    mm = O.T3Mesh(np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]), np.array([[0, 1, 2]]))
+    op = O.T3Operators(mm, mat("amor"))
+    m = op.material
+    u = np.array([[0.0, 0.0], [-1e-4, 0.0], [0.0, -2e-4]])  # compression: exx=-1e-4, eyy=-2e-4
+    sxx, syy, sxy = op.compute_stress(u, np.array([0.5, 0.5, 0.5]))
+    g = m.degradation(np.array([0.5]))[0]
+    tr = -3e-4
+    assert sxx[0] == pytest.approx(g * 2 * m.mu * (-1e-4 - tr / 3) + m.kappa * tr, rel=1e-13)
+    assert syy[0] == pytest.approx(g * 2 * m.mu * (-2e-4 - tr / 3) + m.kappa * tr, rel=1e-13)
+    assert sxy[0] == 0.0
+    iso = O.T3Operators(mm, mat("isotropic"))
+    u2 = np.array([[0.0, 0.0], [1e-4, 3e-5], [2e-5, -5e-5]])
+    s = iso.compute_strain(u2)
+    sx, sy, sxy2 = iso.compute_stress(u2, np.zeros(3))
+    psi = iso.compute_psi_plus(u2, s)
+    assert psi[0] == pytest.approx(0.5 * (sx[0] * s[0][0] + sy[0] * s[1][0] + sxy2[0] * s[2][0]), rel=1e-12)
+
+
+def test_spectral_principal_values():
+    mm = O.T3Mesh(np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]), np.array([[0, 1, 2]]))
+    op = O.T3Operators(mm, mat("spectral"))
+    m = op.material
+    u = np.array([[0.0, 0.0], [2e-4, 0.0], [0.0, -1e-4]])  # exx=2e-4, eyy=-1e-4, gxy=0
+    psi = op.compute_psi_plus(u)
+    e1 = 0.5e-4 + np.sqrt(1.5e-4 ** 2 + O.SPECTRAL_EPS)
+    assert psi[0] == pytest.approx(0.5 * m.lam * 1e-4 ** 2 + m.mu * e1 ** 2, rel=1e-12)
+
+
+def dense_damage(mm, matl, H):
+    """Textbook assembly: Gc l0 int grad phi_a . grad phi_b + (2H + Gc/l0) int phi_a phi_b."""
+    n = mm.n_nodes
+    K = np.zeros((n, n))
+    Mc = np.array([[2.0, 1.0, 1.0], [1.0, 2.0, 1.0], [1.0, 1.0, 2.0]]) / 12.0
+    for e, nodes in enumerate(mm.elements):
+        gp = mm.grad_phi[e]
+        ke = matl.Gc * matl.l0 * mm.areas[e] * gp @ gp.T + (2.0 * H[e] + matl.Gc / matl.l0) * mm.areas[e] * Mc
+        K[np.ix_(nodes, nodes)] += ke
+    return K
+
+
+def test_damage_operator_dense_assembly():
+    mm = mesh(4, 3)
+    matl = mat()
+    dm = O.T3Damage(mm, matl)
+    H = np.random.default_rng(3).uniform(0, 1e-3, mm.n_elems)
+    K = dense_damage(mm, matl, H)
+    D = dm.dense(H)
+    assert np.abs(D - K).max() < 1e-13 * np.abs(K).max()
+    np.testing.assert_allclose(dm.diagonal(H), np.diag(K), rtol=1e-13)
+    assert np.abs(D - D.T).max() < 1e-14 * np.abs(D).max()
+    assert dm.rhs(H).sum() == pytest.approx(float((2.0 * H * mm.areas).sum()), rel=1e-13)
+
+
+def test_damage_uniform_history_closed_form():
+    mm = mesh()
+    matl = mat()
+    dm = O.T3Damage(mm, matl)
+    Hc = 2e-4
+    H = np.full(mm.n_elems, Hc)
+    d = np.full(mm.n_nodes, 2 * Hc / (matl.Gc / matl.l0 + 2 * Hc))
+    np.testing.assert_allclose(dm.apply(d, H), dm.rhs(H), rtol=1e-12)
