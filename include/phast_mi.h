@@ -52,6 +52,7 @@ const char *phast_last_error(void);
 
 #define PHAST_SPLIT_AMOR 0
 #define PHAST_SPLIT_ISOTROPIC 1
+#define PHAST_SPLIT_SPECTRAL 2 /* T3 only (core/fem_operators.py:335-390) */
 
 /* Host descriptor of one structured Q4 plate: (nx+1)(ny+1) nodes numbered
  * j*(nx+1)+i and nx*ny cells numbered j*nx+i with corners (i,j),(i+1,j),(i+1,j+1),(i,j+1)
@@ -166,6 +167,54 @@ int phast_q4_batch_step(phast_q4_plan *plan, int32_t method, const double *u, co
                         double *H_new, double *psi, double *d_out, int32_t *iters, int32_t *status,
                         hipStream_t stream);
 int phast_q4_batch_error(phast_q4_plan *plan, int32_t *err, int32_t clear);
+
+/* =====================================================================================
+ * 2D linear triangles (T3) on an unstructured mesh
+ * ===================================================================================== */
+
+typedef struct phast_t3_plan phast_t3_plan;
+
+/* Mesh upload and host precompute (core/mesh.py:226-300 _precompute_t3): nodes host (N,2),
+ * elements host (E,3) int32.  Builds the per-node incidence list in e*3+a order, the order in
+ * which scatter_add over elements.flatten() visits a node (fem_operators.py:661-679).
+ * Material constants as physics/material.py:379-388 (lam, mu, kappa) and :529 (g(d));
+ * split = PHAST_SPLIT_*. */
+int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const double *nodes, const int32_t *elements,
+                         double E, double nu, double Gc, double l0, double eta_residual, int32_t plane_stress,
+                         int32_t split, phast_t3_plan **out);
+int phast_t3_plan_destroy(phast_t3_plan *plan);
+/* Host copies of M_scalar (N,) = scatter(area/3) (core/mesh.py:287-289), h_min
+ * (mesh.py:276-283) and the total area.  Any pointer may be null. */
+int phast_t3_mesh_info(const phast_t3_plan *plan, double *M_scalar, double *h_min, double *total_area);
+
+/* f (N,2) = internal force of u (N,2) at nodal damage d (N,): centroid strain
+ * (fem_operators.py:223-242), element-mean degradation (:626-630), the split
+ * (:277-290 isotropic, :301-333 amor, :335-390 spectral) and the scatter (:661-679). */
+int phast_t3_internal_force(const phast_t3_plan *plan, const double *u, const double *d, double *f,
+                            hipStream_t stream);
+/* internal force plus psi+ (E,) of u (fem_operators.py:951-1030) and H_new = max(H_old, psi+)
+ * (staggered_solver.py:804-845).  psi may be null. */
+int phast_t3_force_history(const phast_t3_plan *plan, const double *u, const double *d, const double *H_old,
+                           double *f, double *H_new, double *psi, hipStream_t stream);
+/* One central-difference step with lumped mass (solvers/mechanics_solver.py:153-239), with
+ * psi+(u1) and the history max, in two launches (predictor; force + corrector):
+ *   u1 = u + dt v + dt^2/2 a, u1[bc] = bc_vals;  a1 = Minv (f_ext - f_int(u1, d)),
+ *   v1 = v + dt/2 (a + a1), a1[bc] = v1[bc] = 0;  H_new = max(H_old, psi+(u1)).
+ * Minv (N,) = 1/(rho M + 1e-30); f_ext, bc_mask (uint8 (N,2)) + bc_vals, psi may be null.
+ * Outputs must not alias inputs. */
+int phast_t3_explicit_step(const phast_t3_plan *plan, double dt, const double *u, const double *v, const double *a,
+                           const double *d, const double *f_ext, const double *Minv, const uint8_t *bc_mask,
+                           const double *bc_vals, const double *H_old, double *u1, double *v1, double *a1,
+                           double *H_new, double *psi, hipStream_t stream);
+/* AT2 damage operator of element history H (E,) (solvers/damage_solver.py:1715-1755 _Ax):
+ *   y = Gc l0 sum_e A (grad phi_a . grad d_e) + sum_e rc_e (d_a + sum_b d_b),
+ *   rc = (2H + Gc/l0) A / 12  (damage_solver.py:2174-2176);  x and y must differ. */
+int phast_t3_damage_apply(const phast_t3_plan *plan, const double *H, const double *x, double *y,
+                          hipStream_t stream);
+/* b = sum_e 2H A/3 (damage_solver.py:2252-2257) and the Jacobi diagonal
+ * sum_e (Gc l0 A |grad phi_a|^2 + 2 rc) (damage_solver.py:2112-2119); either may be null. */
+int phast_t3_damage_rhs_diag(const phast_t3_plan *plan, const double *H, double *b, double *diag,
+                             hipStream_t stream);
 
 /* =====================================================================================
  * 3D periodic hex8 voxel cells

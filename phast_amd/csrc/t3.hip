@@ -1,0 +1,398 @@
+// Linear-triangle (T3) kernels for gfx950 (FP64 throughout).
+//
+// Layout: nodes (N,2) float64 interleaved, elements (E,3) int32, per-element fields (E,)
+// (psi, H), nodal u/v/a/f (N,2), d/Minv (N,).  The plan holds a CSR of element corners per
+// node: inc[ptr[n] .. ptr[n+1]) = e*3 + a for every element e with node n at corner a, in
+// increasing e*3 + a -- the order in which the reference's scatter_add over elem_flat visits
+// the node (core/fem_operators.py:661-679 with _elem_flat = elements.flatten()).
+//
+// Every pass is node-centric: one thread per node walks its incident corners, gathers the 3
+// nodes of each incident element (neighbours are mostly cache hits for a row-ordered mesh),
+// recomputes that element's geometry, strain and stress, and sums the corner's
+// contribution.  No element buffer round trip and no atomics, so results are deterministic.
+// Element-wise outputs (psi+, history) are written by the thread of the element's corner 0.
+#include "common.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/phast_mi.h"
+
+namespace phast {
+
+enum : int { T3_AMOR = PHAST_SPLIT_AMOR, T3_ISO = PHAST_SPLIT_ISOTROPIC, T3_SPECTRAL = PHAST_SPLIT_SPECTRAL };
+constexpr double kSpectralEps = 1e-12;  // core/fem_operators.py:126 (float64)
+
+struct T3Mat {
+    double lam, mu, kappa, ezz_coef, eta, Gc_l0, Gc_over_l0;
+    int32_t split, plane_stress;
+};
+
+struct T3Mesh {
+    const double2 *xy;
+    const int32_t *el;   // (E,3)
+    const int32_t *ptr;  // (N+1)
+    const int32_t *inc;  // incidences e*3 + a
+    int64_t n_nodes, n_elems;
+};
+
+// Area and shape-function gradients of one triangle (core/mesh.py:236-268).
+__device__ __forceinline__ void t3_geometry(double2 p0, double2 p1, double2 p2, double &area, double gp[3][2]) {
+    const double v01x = p1.x - p0.x, v01y = p1.y - p0.y, v02x = p2.x - p0.x, v02y = p2.y - p0.y;
+    const double s2a = v01x * v02y - v01y * v02x;
+    area = 0.5 * fabs(s2a);
+    const double sgn = s2a > 0.0 ? 1.0 : (s2a < 0.0 ? -1.0 : 1.0);
+    const double inv = 1.0 / (fabs(s2a) > 1e-10 ? s2a : sgn * 1e-10);
+    gp[0][0] = (p1.y - p2.y) * inv;
+    gp[0][1] = (p2.x - p1.x) * inv;
+    gp[1][0] = (p2.y - p0.y) * inv;
+    gp[1][1] = (p0.x - p2.x) * inv;
+    gp[2][0] = (p0.y - p1.y) * inv;
+    gp[2][1] = (p1.x - p0.x) * inv;
+}
+
+// Degraded stress of one element for the split (fem_operators.py:277-290 isotropic,
+// 301-333 amor, 335-390 spectral), g = g(mean of the corner damages).
+__device__ __forceinline__ void t3_stress(const T3Mat &M, double exx, double eyy, double gxy, double g, double &sxx,
+                                          double &syy, double &sxy) {
+    const double lam = M.lam, mu = M.mu;
+    if (M.split == T3_ISO) {
+        const double c11 = lam + 2.0 * mu;
+        sxx = g * (c11 * exx + lam * eyy);
+        syy = g * (lam * exx + c11 * eyy);
+        sxy = g * (mu * gxy);
+    } else if (M.split == T3_AMOR) {
+        const double tr2 = exx + eyy;
+        const double ezz = M.plane_stress ? M.ezz_coef * tr2 : 0.0;
+        const double tr = tr2 + ezz;
+        const double trp = tr > 0.0 ? tr : 0.0;
+        const double trm = tr - trp;
+        const double dxx = exx - tr / 3.0, dyy = eyy - tr / 3.0;
+        sxx = g * (M.kappa * trp + 2 * mu * dxx) + M.kappa * trm;
+        syy = g * (M.kappa * trp + 2 * mu * dyy) + M.kappa * trm;
+        sxy = g * mu * gxy;
+    } else {
+        const double exy = gxy / 2.0, trace = exx + eyy, hd = (exx - eyy) / 2.0;
+        const double delta = sqrt(hd * hd + exy * exy + kSpectralEps);
+        const double e1 = trace / 2.0 + delta, e2 = trace / 2.0 - delta;
+        const double e1p = e1 > 0.0 ? e1 : 0.0, e2p = e2 > 0.0 ? e2 : 0.0;
+        const double trp = trace > 0.0 ? trace : 0.0, trm = trace - trp;
+        const double inv = 1.0 / (2.0 * delta + 1e-30);
+        const double pxx = (hd + delta) * inv, pyy = (-hd + delta) * inv, pxy = exy * inv;
+        const double exx_p = e1p * pxx + e2p * (1.0 - pxx), eyy_p = e1p * pyy + e2p * (1.0 - pyy);
+        const double exy_p = (e1p - e2p) * pxy;
+        const double exx_m = exx - exx_p, eyy_m = eyy - eyy_p, exy_m = exy - exy_p;
+        sxx = g * (lam * trp + 2 * mu * exx_p) + (lam * trm + 2 * mu * exx_m);
+        syy = g * (lam * trp + 2 * mu * eyy_p) + (lam * trm + 2 * mu * eyy_m);
+        sxy = g * (2 * mu * exy_p) + (2 * mu * exy_m);
+    }
+}
+
+// Tensile energy density (fem_operators.py:983-1030).
+__device__ __forceinline__ double t3_psi_plus(const T3Mat &M, double exx, double eyy, double gxy) {
+    const double lam = M.lam, mu = M.mu, exy = gxy / 2.0;
+    if (M.split == T3_ISO) {
+        const double tr = exx + eyy;
+        return 0.5 * lam * (tr * tr) + mu * ((exx * exx + eyy * eyy) + 2 * (exy * exy));
+    }
+    if (M.split == T3_AMOR) {
+        const double tr2 = exx + eyy;
+        const double ezz = M.plane_stress ? M.ezz_coef * tr2 : 0.0;
+        const double tr = tr2 + ezz, trp = tr > 0.0 ? tr : 0.0;
+        const double dxx = exx - tr / 3.0, dyy = eyy - tr / 3.0, dzz = ezz - tr / 3.0;
+        return 0.5 * M.kappa * (trp * trp) + mu * (((dxx * dxx + dyy * dyy) + dzz * dzz) + 2.0 * (exy * exy));
+    }
+    const double trace = exx + eyy, hd = (exx - eyy) / 2.0;
+    const double delta = sqrt(hd * hd + exy * exy + kSpectralEps);
+    const double e1 = trace / 2.0 + delta, e2 = trace / 2.0 - delta;
+    const double e1p = e1 > 0.0 ? e1 : 0.0, e2p = e2 > 0.0 ? e2 : 0.0, trp = trace > 0.0 ? trace : 0.0;
+    return 0.5 * lam * (trp * trp) + mu * (e1p * e1p + e2p * e2p);
+}
+
+// One element seen from one of its corners: geometry, strain, degradation.
+struct T3Elem {
+    int32_t n[3];
+    double area, gp[3][2], exx, eyy, gxy, g;
+};
+
+__device__ __forceinline__ void t3_load(const T3Mesh &m, const T3Mat &M, int64_t e, const double2 *u, const double *d,
+                                        T3Elem &E) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) E.n[k] = m.el[3 * e + k];
+    t3_geometry(m.xy[E.n[0]], m.xy[E.n[1]], m.xy[E.n[2]], E.area, E.gp);
+    const double2 u0 = u[E.n[0]], u1 = u[E.n[1]], u2 = u[E.n[2]];
+    E.exx = (E.gp[0][0] * u0.x + E.gp[1][0] * u1.x) + E.gp[2][0] * u2.x;
+    E.eyy = (E.gp[0][1] * u0.y + E.gp[1][1] * u1.y) + E.gp[2][1] * u2.y;
+    E.gxy = ((E.gp[0][1] * u0.x + E.gp[1][1] * u1.x) + E.gp[2][1] * u2.x) +
+            ((E.gp[0][0] * u0.y + E.gp[1][0] * u1.y) + E.gp[2][0] * u2.y);
+    if (d) {
+        const double dm = ((d[E.n[0]] + d[E.n[1]]) + d[E.n[2]]) / 3.0;
+        const double omd = 1.0 - dm;
+        E.g = (omd * omd) * (1.0 - M.eta) + M.eta;  // physics/material.py:529
+    } else {
+        E.g = 1.0;
+    }
+}
+
+// MODE 0: f = internal force; 1: + psi+ and H = max(H_old, psi+) per element;
+// 2: explicit-step corrector on top of 1 (u is the predicted u1).
+template <int MODE>
+__global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3Mat M, const double2 *u, const double *d,
+                                                       double2 *f, const double *H_old, double *H_new, double *psi,
+                                                       double dt, const double2 *v, const double2 *a,
+                                                       const double2 *fext, const double *minv, const uint8_t *bcm,
+                                                       double2 *v1, double2 *a1) {
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n >= m.n_nodes) return;
+    double fx = 0.0, fy = 0.0;
+    for (int32_t k = m.ptr[n]; k < m.ptr[n + 1]; ++k) {
+        const int32_t ic = m.inc[k];
+        const int64_t e = ic / 3;
+        const int a_ = ic - 3 * (int)e;
+        T3Elem E;
+        t3_load(m, M, e, u, d, E);
+        double sxx, syy, sxy;
+        t3_stress(M, E.exx, E.eyy, E.gxy, E.g, sxx, syy, sxy);
+        // fem_operators.py:669-674: A (s . grad phi_a); the corner's row picked by selects, not
+        // a dynamic index (which would put gp in scratch memory)
+        const double gx = a_ == 0 ? E.gp[0][0] : (a_ == 1 ? E.gp[1][0] : E.gp[2][0]);
+        const double gy = a_ == 0 ? E.gp[0][1] : (a_ == 1 ? E.gp[1][1] : E.gp[2][1]);
+        fx += E.area * (sxx * gx + sxy * gy);
+        fy += E.area * (sxy * gx + syy * gy);
+        if (MODE >= 1 && a_ == 0) {
+            const double p = t3_psi_plus(M, E.exx, E.eyy, E.gxy);
+            if (psi) psi[e] = p;
+            const double h = H_old[e];
+            H_new[e] = h >= p ? h : p;  // max returns one operand bit-exactly
+        }
+    }
+    if (MODE < 2) {
+        f[n] = make_double2(fx, fy);
+        return;
+    }
+    // mechanics_solver.py:194-198: a1 = Minv (f_ext - f_int), v1 = v + dt/2 (a + a1); BC zero
+    const double mi = minv[n];
+    const double2 fe = fext ? fext[n] : make_double2(0.0, 0.0);
+    const double2 v0 = v[n], a0 = a[n];
+    double ax = mi * (fe.x - fx), ay = mi * (fe.y - fy);
+    double vx = v0.x + 0.5 * dt * (a0.x + ax), vy = v0.y + 0.5 * dt * (a0.y + ay);
+    if (bcm) {
+        if (bcm[2 * n]) ax = vx = 0.0;
+        if (bcm[2 * n + 1]) ay = vy = 0.0;
+    }
+    v1[n] = make_double2(vx, vy);
+    a1[n] = make_double2(ax, ay);
+}
+
+// u1 = u + dt v + dt^2/2 a with prescribed components overridden (mechanics_solver.py:160-170).
+__global__ __launch_bounds__(256) void t3_predictor_kernel(int64_t nn, double dt, const double2 *u, const double2 *v,
+                                                           const double2 *a, const uint8_t *bcm, const double *bcv,
+                                                           double2 *u1) {
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n >= nn) return;
+    const double c2 = 0.5 * dt * dt;
+    const double2 u0 = u[n], v0 = v[n], a0 = a[n];
+    double ux = (u0.x + dt * v0.x) + c2 * a0.x, uy = (u0.y + dt * v0.y) + c2 * a0.y;
+    if (bcm) {
+        if (bcm[2 * n]) ux = bcv[2 * n];
+        if (bcm[2 * n + 1]) uy = bcv[2 * n + 1];
+    }
+    u1[n] = make_double2(ux, uy);
+}
+
+// Damage operator of the element-H AT2 path (solvers/damage_solver.py):
+// OP 0: y = Gc l0 sum_e A (grad phi_a . grad d_e) + sum_e rc_e (d_a + sum_e d), rc = (2H + Gc/l0) A/12
+//       (the two scatters of :1726-1755 summed separately, then combined);
+// OP 1: b = sum_e 2H A/3 (:2252-2257) and diag = sum_e (Gc l0 A |grad phi_a|^2 + 2 rc) (:2112-2119).
+template <int OP>
+__global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3Mat M, const double *H, const double *x,
+                                                        double *y, double *y2) {
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n >= m.n_nodes) return;
+    double s0 = 0.0, s1 = 0.0;
+    for (int32_t k = m.ptr[n]; k < m.ptr[n + 1]; ++k) {
+        const int32_t ic = m.inc[k];
+        const int64_t e = ic / 3;
+        const int a_ = ic - 3 * (int)e;
+        int32_t nd[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) nd[q] = m.el[3 * e + q];
+        double area, gp[3][2];
+        t3_geometry(m.xy[nd[0]], m.xy[nd[1]], m.xy[nd[2]], area, gp);
+        const double h = H[e];
+        const double rc = (2.0 * h + M.Gc_over_l0) * area / 12.0;
+        const double gx = a_ == 0 ? gp[0][0] : (a_ == 1 ? gp[1][0] : gp[2][0]);
+        const double gy = a_ == 0 ? gp[0][1] : (a_ == 1 ? gp[1][1] : gp[2][1]);
+        if (OP == 0) {
+            const double d0 = x[nd[0]], d1 = x[nd[1]], d2 = x[nd[2]];
+            const double gdx = (gp[0][0] * d0 + gp[1][0] * d1) + gp[2][0] * d2;
+            const double gdy = (gp[0][1] * d0 + gp[1][1] * d1) + gp[2][1] * d2;
+            s0 += area * (gx * gdx + gy * gdy);
+            const double da = a_ == 0 ? d0 : (a_ == 1 ? d1 : d2);
+            s1 += rc * (da + ((d0 + d1) + d2));
+        } else {
+            s0 += 2.0 * h * (area / 3.0);
+            s1 += M.Gc_l0 * (area * (gx * gx + gy * gy)) + 2.0 * rc;
+        }
+    }
+    if (OP == 0) {
+        y[n] = M.Gc_l0 * s0 + s1;
+    } else {
+        if (y) y[n] = s0;
+        if (y2) y2[n] = s1;
+    }
+}
+
+}  // namespace phast
+
+using namespace phast;
+
+struct phast_t3_plan {
+    T3Mesh mesh;
+    T3Mat mat;
+    double2 *xy = nullptr;
+    int32_t *el = nullptr, *ptr = nullptr, *inc = nullptr;
+    std::vector<double> M_scalar;  // lumped mass per node (area/3 scatter), host copy
+    double h_min = 0.0, total_area = 0.0;
+};
+
+extern "C" int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const double *nodes, const int32_t *elements,
+                                    double E, double nu, double Gc, double l0, double eta_residual,
+                                    int32_t plane_stress, int32_t split, phast_t3_plan **out) {
+    PHAST_REQUIRE(out && nodes && elements && n_nodes >= 3 && n_elems >= 1, "phast_t3_plan_create: bad argument");
+    PHAST_REQUIRE(3 * n_elems < ((int64_t)1 << 31) && n_nodes < ((int64_t)1 << 31),
+                  "phast_t3_plan_create: mesh too large for 32-bit incidence indices");
+    PHAST_REQUIRE(split == T3_ISO || split == T3_AMOR || split == T3_SPECTRAL, "unknown T3 energy split %d", split);
+    for (int64_t k = 0; k < 3 * n_elems; ++k)
+        PHAST_REQUIRE(elements[k] >= 0 && elements[k] < n_nodes, "element %lld refers to node %d outside the mesh",
+                      (long long)(k / 3), elements[k]);
+    auto *pl = new phast_t3_plan();
+    T3Mat &M = pl->mat;
+    M.lam = plane_stress ? E * nu / (1.0 - nu * nu) : E * nu / ((1 + nu) * (1 - 2 * nu));  // material.py:379-388
+    M.mu = E / (2 * (1 + nu));
+    M.kappa = E / (3.0 * (1.0 - 2.0 * nu));
+    M.ezz_coef = -nu / (1.0 - nu);
+    M.eta = eta_residual;
+    M.Gc_l0 = Gc * l0;
+    M.Gc_over_l0 = Gc / l0;
+    M.split = split;
+    M.plane_stress = plane_stress;
+    // host geometry: lumped mass in elem_flat order and h_min (core/mesh.py:270-286)
+    pl->M_scalar.assign((size_t)n_nodes, 0.0);
+    double hmin = INFINITY, tot = 0.0;
+    std::vector<int32_t> cnt((size_t)n_nodes + 1, 0);
+    for (int64_t e = 0; e < n_elems; ++e) {
+        const int32_t *q = elements + 3 * e;
+        const double *p0 = nodes + 2 * q[0], *p1 = nodes + 2 * q[1], *p2 = nodes + 2 * q[2];
+        const double v01x = p1[0] - p0[0], v01y = p1[1] - p0[1], v02x = p2[0] - p0[0], v02y = p2[1] - p0[1];
+        const double v12x = p2[0] - p1[0], v12y = p2[1] - p1[1];
+        const double area = 0.5 * std::fabs(v01x * v02y - v01y * v02x);
+        const double per = std::sqrt(v01x * v01x + v01y * v01y) + std::sqrt(v12x * v12x + v12y * v12y) +
+                           std::sqrt(v02x * v02x + v02y * v02y);
+        hmin = std::min(hmin, 4.0 * area / (per + 1e-30));
+        tot += area;
+        for (int k = 0; k < 3; ++k) {
+            pl->M_scalar[(size_t)q[k]] += area / 3.0;
+            ++cnt[(size_t)q[k] + 1];
+        }
+    }
+    pl->h_min = hmin;
+    pl->total_area = tot;
+    for (int64_t i = 0; i < n_nodes; ++i) cnt[(size_t)i + 1] += cnt[(size_t)i];
+    std::vector<int32_t> inc((size_t)(3 * n_elems)), fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t k = 0; k < 3 * n_elems; ++k) inc[(size_t)fill[(size_t)elements[k]]++] = (int32_t)k;
+    if (hipMalloc(&pl->xy, sizeof(double2) * (size_t)n_nodes) != hipSuccess ||
+        hipMalloc(&pl->el, sizeof(int32_t) * 3 * (size_t)n_elems) != hipSuccess ||
+        hipMalloc(&pl->ptr, sizeof(int32_t) * (size_t)(n_nodes + 1)) != hipSuccess ||
+        hipMalloc(&pl->inc, sizeof(int32_t) * 3 * (size_t)n_elems) != hipSuccess ||
+        hipMemcpy(pl->xy, nodes, sizeof(double2) * (size_t)n_nodes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->el, elements, sizeof(int32_t) * 3 * (size_t)n_elems, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->ptr, cnt.data(), sizeof(int32_t) * (size_t)(n_nodes + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->inc, inc.data(), sizeof(int32_t) * 3 * (size_t)n_elems, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("phast_t3_plan_create: device allocation/copy failed");
+        phast_t3_plan_destroy(pl);
+        return PHAST_ERR_HIP;
+    }
+    pl->mesh = T3Mesh{pl->xy, pl->el, pl->ptr, pl->inc, n_nodes, n_elems};
+    *out = pl;
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_plan_destroy(phast_t3_plan *pl) {
+    if (!pl) return PHAST_OK;
+    (void)hipFree(pl->xy);
+    (void)hipFree(pl->el);
+    (void)hipFree(pl->ptr);
+    (void)hipFree(pl->inc);
+    delete pl;
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_mesh_info(const phast_t3_plan *pl, double *M_scalar, double *h_min, double *total_area) {
+    PHAST_REQUIRE(pl, "phast_t3_mesh_info: null plan");
+    if (M_scalar) std::memcpy(M_scalar, pl->M_scalar.data(), sizeof(double) * pl->M_scalar.size());
+    if (h_min) *h_min = pl->h_min;
+    if (total_area) *total_area = pl->total_area;
+    return PHAST_OK;
+}
+
+static inline dim3 node_grid(const phast_t3_plan *pl) { return dim3((unsigned)ceil_div(pl->mesh.n_nodes, 256)); }
+
+extern "C" int phast_t3_internal_force(const phast_t3_plan *pl, const double *u, const double *d, double *f,
+                                       hipStream_t st) {
+    PHAST_REQUIRE(pl && u && d && f, "phast_t3_internal_force: null argument");
+    hipLaunchKernelGGL(t3_force_kernel<0>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, (const double2 *)u, d,
+                       (double2 *)f, (const double *)nullptr, (double *)nullptr, (double *)nullptr, 0.0,
+                       (const double2 *)nullptr, (const double2 *)nullptr, (const double2 *)nullptr,
+                       (const double *)nullptr, (const uint8_t *)nullptr, (double2 *)nullptr, (double2 *)nullptr);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_force_history(const phast_t3_plan *pl, const double *u, const double *d, const double *H_old,
+                                      double *f, double *H_new, double *psi, hipStream_t st) {
+    PHAST_REQUIRE(pl && u && d && H_old && f && H_new, "phast_t3_force_history: null argument");
+    hipLaunchKernelGGL(t3_force_kernel<1>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, (const double2 *)u, d,
+                       (double2 *)f, H_old, H_new, psi, 0.0, (const double2 *)nullptr, (const double2 *)nullptr,
+                       (const double2 *)nullptr, (const double *)nullptr, (const uint8_t *)nullptr,
+                       (double2 *)nullptr, (double2 *)nullptr);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_explicit_step(const phast_t3_plan *pl, double dt, const double *u, const double *v,
+                                      const double *a, const double *d, const double *f_ext, const double *Minv,
+                                      const uint8_t *bc_mask, const double *bc_vals, const double *H_old, double *u1,
+                                      double *v1, double *a1, double *H_new, double *psi, hipStream_t st) {
+    PHAST_REQUIRE(pl && u && v && a && d && Minv && H_old && u1 && v1 && a1 && H_new,
+                  "phast_t3_explicit_step: null argument");
+    PHAST_REQUIRE(!bc_mask == !bc_vals, "phast_t3_explicit_step: bc_mask and bc_vals go together");
+    PHAST_REQUIRE(u1 != u && u1 != v && u1 != a && v1 != v && a1 != a, "phast_t3_explicit_step: outputs alias inputs");
+    hipLaunchKernelGGL(t3_predictor_kernel, node_grid(pl), dim3(256), 0, st, pl->mesh.n_nodes, dt, (const double2 *)u,
+                       (const double2 *)v, (const double2 *)a, bc_mask, bc_vals, (double2 *)u1);
+    PHAST_LAUNCHED();
+    hipLaunchKernelGGL(t3_force_kernel<2>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, (const double2 *)u1, d,
+                       (double2 *)nullptr, H_old, H_new, psi, dt, (const double2 *)v, (const double2 *)a,
+                       (const double2 *)f_ext, Minv, bc_mask, (double2 *)v1, (double2 *)a1);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_damage_apply(const phast_t3_plan *pl, const double *H, const double *x, double *y,
+                                     hipStream_t st) {
+    PHAST_REQUIRE(pl && H && x && y && x != y, "phast_t3_damage_apply: bad argument");
+    hipLaunchKernelGGL(t3_damage_kernel<0>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H, x, y,
+                       (double *)nullptr);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_damage_rhs_diag(const phast_t3_plan *pl, const double *H, double *b, double *diag,
+                                        hipStream_t st) {
+    PHAST_REQUIRE(pl && H && (b || diag), "phast_t3_damage_rhs_diag: null argument");
+    hipLaunchKernelGGL(t3_damage_kernel<1>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H,
+                       (const double *)nullptr, b, diag);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}

@@ -45,9 +45,10 @@ class Material:
     degradation_type: str = "standard"
 
     def __post_init__(self):
-        if self.energy_split not in ("amor", "isotropic"):
+        if self.energy_split not in ("amor", "isotropic", "spectral"):
             raise NotImplementedError(
-                f"energy_split={self.energy_split!r}: the Q4 HIP path implements 'amor' and 'isotropic'")
+                f"energy_split={self.energy_split!r}: the HIP path implements 'amor', 'isotropic' and "
+                "'spectral' (T3 only)")
         if self.pf_model != "AT2" or self.degradation_type != "standard":
             raise NotImplementedError("the Q4 HIP path implements AT2 with the standard degradation")
 
@@ -170,6 +171,9 @@ def q4_desc(mesh: FEMMesh, mat: Material, dt: float = 0.0) -> Q4Desc:
         d.wdetJ[i] = float(v)
     d.lam, d.mu, d.kappa, d.nu = mat.lam, mat.mu, mat.kappa, mat.nu
     d.plane_stress = int(bool(mat.plane_stress))
+    if mat.energy_split not in ("amor", "isotropic"):
+        raise NotImplementedError(f"energy_split={mat.energy_split!r}: the Q4 HIP path implements 'amor' and "
+                                  "'isotropic'")
     d.split = 0 if mat.energy_split == "amor" else 1
     d.eta_residual, d.Gc, d.l0, d.dt = mat.eta_residual, mat.Gc, mat.l0, float(dt)
     d.hx, d.hy = mesh.hx, mesh.hy

@@ -1,0 +1,236 @@
+"""Linear-triangle (T3) phase-field path on MI355X: host mirror of the reference interface.
+
+The classes keep the names, arguments and return conventions of the reference's T3 branch
+of ``FEMOperators`` and of ``ExplicitDynamics`` and the T3 ``_Ax`` pieces of
+``PhaseFieldDamageSolver``.  All arithmetic runs in ``libphast_mi.so``; this module uploads
+the mesh once (the plan), allocates outputs with ``torch.empty`` on the fields' device and
+passes raw pointers.  There is no CPU fallback.
+
+Reference files (paths relative to the reference's ``src/phast``):
+``core/mesh.py`` (``_precompute_t3``), ``core/fem_operators.py``, ``physics/material.py``,
+``solvers/mechanics_solver.py``, ``solvers/damage_solver.py``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr, stream
+from .q4 import F64, Material, _dev_check, history_update  # noqa: F401  (history_update re-exported)
+
+_SPLIT_ID = {"amor": 0, "isotropic": 1, "spectral": 2}
+
+
+class T3Plan:
+    """Owner of a ``phast_t3_plan``: the mesh on the device plus its per-node incidence lists."""
+
+    def __init__(self, nodes: np.ndarray, elements: np.ndarray, material: Material):
+        _lib.require_device()
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        els = np.ascontiguousarray(elements, dtype=np.int32)
+        h = C.c_void_p()
+        check(_lib.lib().phast_t3_plan_create(
+            len(nodes), len(els), nodes.ctypes.data, els.ctypes.data, material.E, material.nu, material.Gc,
+            material.l0, material.eta_residual, int(bool(material.plane_stress)),
+            _SPLIT_ID[material.energy_split], C.byref(h)), "phast_t3_plan_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib.lib().phast_t3_plan_destroy(h)
+            self.handle = None
+
+
+class FEMMesh:
+    """Unstructured T3 mesh (``core/mesh.py:FEMMesh`` with ``element_type='T3'``).
+
+    ``nodes`` (N,2) and ``elements`` (E,3) stay on the CPU for host-side set-up; the
+    geometry the kernels use (areas, shape-function gradients) is recomputed on the device
+    from the node coordinates with the formulas of ``_precompute_t3`` (mesh.py:226-300).
+    """
+
+    element_type = "T3"
+
+    def __init__(self, nodes, elements, device=None):
+        nodes = np.asarray(nodes, dtype=np.float64)
+        elements = np.asarray(elements, dtype=np.int64)
+        if nodes.ndim != 2 or nodes.shape[1] != 2 or elements.ndim != 2 or elements.shape[1] != 3:
+            raise ValueError("T3 mesh needs nodes (N,2) and elements (E,3)")
+        if elements.size and (elements.min() < 0 or elements.max() >= len(nodes)):
+            raise ValueError("element connectivity refers to nodes outside the mesh")
+        if 3 * len(elements) >= 2 ** 31:
+            raise ValueError("T3 meshes are limited to 2^31 / 3 elements (32-bit incidence indices)")
+        self.nodes, self.elements = nodes, elements
+        self.n_nodes, self.n_elems = len(nodes), len(elements)
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.dtype = F64
+        p = nodes[elements]
+        v01, v02, v12 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], p[:, 2] - p[:, 1]
+        self.areas_np = 0.5 * np.abs(v01[:, 0] * v02[:, 1] - v01[:, 1] * v02[:, 0])
+        if np.any(self.areas_np <= 0.0):
+            raise ValueError("degenerate (zero-area) triangle in the mesh")
+        per = np.linalg.norm(v01, axis=1) + np.linalg.norm(v12, axis=1) + np.linalg.norm(v02, axis=1)
+        self.h_min = float((4.0 * self.areas_np / (per + 1e-30)).min())  # mesh.py:276-283
+
+
+class FEMOperators:
+    """T3 branch of ``core/fem_operators.py:FEMOperators`` on the HIP library."""
+
+    def __init__(self, mesh: FEMMesh, material: Material, ctx=None):
+        if material.energy_split not in _SPLIT_ID:
+            raise NotImplementedError(f"energy_split={material.energy_split!r}")
+        self.mesh, self.material = mesh, material
+        self.device, self.dtype = mesh.device, F64
+        with torch.cuda.device(self.device):
+            self.plan = T3Plan(mesh.nodes, mesh.elements, material)
+        M = np.empty(mesh.n_nodes)
+        hmin, area = C.c_double(), C.c_double()
+        check(_lib.lib().phast_t3_mesh_info(self.plan.handle, M.ctypes.data, C.byref(hmin), C.byref(area)),
+              "mesh_info")
+        self.M_scalar = torch.from_numpy(M).to(self.device)  # mesh.py:287-289
+        # fem_operators.py:83-86: interleaved vector mass and its guarded inverse
+        self.M_vec = material.rho * self.M_scalar.unsqueeze(1).expand(-1, 2).flatten()
+        self.M_vec_inv = 1.0 / (self.M_vec + 1e-30)
+        self.M_inv_node = 1.0 / (material.rho * self.M_scalar + 1e-30)
+        self.dt_cfl = hmin.value / material.c_p  # fem_operators.py:88-89
+
+    def _u(self, u, name="u"):
+        return _dev_check(u, (self.mesh.n_nodes, 2), name, self.device)
+
+    def _d(self, d, name="d"):
+        return _dev_check(d, (self.mesh.n_nodes,), name, self.device)
+
+    def _e(self, h, name="H"):
+        return _dev_check(h, (self.mesh.n_elems,), name, self.device)
+
+    def internal_force(self, u: torch.Tensor, d: torch.Tensor) -> torch.Tensor:
+        """``f_int (N,2)`` (fem_operators.py:598-612, T3: 223-242, 614-653, 661-679)."""
+        u, d = self._u(u), self._d(d)
+        f = torch.empty_like(u)
+        check(_lib.lib().phast_t3_internal_force(self.plan.handle, ptr(u), ptr(d), ptr(f), stream()),
+              "internal_force")
+        return f
+
+    def internal_force_and_history(self, u: torch.Tensor, d: torch.Tensor, H_old: torch.Tensor):
+        """``(f_int, psi+, max(H_old, psi+))`` of one pass over the mesh."""
+        u, d, H_old = self._u(u), self._d(d), self._e(H_old, "H_old")
+        f, psi, H_new = torch.empty_like(u), torch.empty_like(H_old), torch.empty_like(H_old)
+        check(_lib.lib().phast_t3_force_history(self.plan.handle, ptr(u), ptr(d), ptr(H_old), ptr(f), ptr(H_new),
+                                                ptr(psi), stream()), "force_history")
+        return f, psi, H_new
+
+    def compute_psi_plus(self, u: torch.Tensor, strain=None, d=None) -> torch.Tensor:
+        """psi+ (E,) at the centroids (fem_operators.py:951-1030); ``strain`` is recomputed."""
+        n = self.mesh.n_elems
+        zero = torch.zeros(n, dtype=F64, device=self.device)
+        _, psi, _ = self.internal_force_and_history(u, torch.zeros(self.mesh.n_nodes, dtype=F64,
+                                                                   device=self.device), zero)
+        return psi
+
+    def compute_driving_force(self, u: torch.Tensor, strain=None, d=None) -> torch.Tensor:
+        """'strain_energy' driving force == psi+ (fem_operators.py:1173-1204)."""
+        return self.compute_psi_plus(u, strain=strain, d=d)
+
+    def compute_reaction_force(self, u, d, node_indices, component: int = 1, f_ext=None) -> float:
+        """fem_operators.py:1502-1530."""
+        f = self.internal_force(u, d)
+        idx = torch.as_tensor(node_indices, dtype=torch.long, device=self.device)
+        r = f[idx, component]
+        if f_ext is not None:
+            r = r - f_ext[idx, component]
+        return float(r.sum().item())
+
+
+class ExplicitDynamics:
+    """Central difference with lumped mass (solvers/mechanics_solver.py:86-239) on T3 meshes.
+
+    ``step`` returns fresh ``(u, v, a)``; ``step_with_history`` also returns
+    ``H = max(H_old, psi+(u1))`` and ``psi+`` of the staggered explicit branch.
+    """
+
+    def __init__(self, fem: FEMOperators, dt: float | None = None, dt_safety: float = 1.0,
+                 differentiable: bool = False, damping_ratio_max: float = 0.0):
+        if differentiable or damping_ratio_max != 0.0:
+            raise NotImplementedError("differentiable / damped explicit dynamics are not on the HIP path")
+        self.fem = fem
+        self.dt_cfl = fem.dt_cfl * dt_safety
+        self.dt = self.dt_cfl if dt is None else float(dt)
+        self.last_iter = 1
+
+    def _bc(self, bc_mask, bc_vals):
+        if bc_mask is None:
+            return None, None
+        n = self.fem.mesh.n_nodes
+        if not isinstance(bc_mask, torch.Tensor) or bc_mask.dtype not in (torch.bool, torch.uint8):
+            raise TypeError("bc_mask must be a bool or uint8 tensor")
+        if tuple(bc_mask.shape) != (n, 2) or bc_mask.device != self.fem.device:
+            raise ValueError(f"bc_mask must have shape ({n}, 2) on {self.fem.device}")
+        return bc_mask.to(torch.uint8).contiguous(), self.fem._u(bc_vals, "bc_vals")
+
+    def _launch(self, u, v, a, d, f_ext, bc_mask, bc_vals, H_old=None, want_psi=False):
+        fem = self.fem
+        u, v, a, d = fem._u(u), fem._u(v, "v"), fem._u(a, "a"), fem._d(d)
+        fe = None if f_ext is None else fem._u(f_ext, "f_ext")
+        m, bv = self._bc(bc_mask, bc_vals)
+        u1, v1, a1 = torch.empty_like(u), torch.empty_like(v), torch.empty_like(a)
+        keep = H_old is not None
+        if keep:
+            H_old = fem._e(H_old, "H_old")
+        else:
+            H_old = torch.zeros(fem.mesh.n_elems, dtype=F64, device=fem.device)
+        H_new = torch.empty_like(H_old)
+        psi = torch.empty_like(H_old) if want_psi else None
+        check(_lib.lib().phast_t3_explicit_step(
+            fem.plan.handle, self.dt, ptr(u), ptr(v), ptr(a), ptr(d), ptr(fe), ptr(fem.M_inv_node), ptr(m), ptr(bv),
+            ptr(H_old), ptr(u1), ptr(v1), ptr(a1), ptr(H_new), ptr(psi), stream()), "explicit_step")
+        return u1, v1, a1, (H_new if keep else None), psi
+
+    def step(self, u, v, a, d, f_ext, bc_mask=None, bc_vals=None, rigid_connectors=None):
+        """mechanics_solver.py:204-239."""
+        if rigid_connectors:
+            raise NotImplementedError("rigid connectors are not on the HIP path")
+        self.last_iter = 1
+        u1, v1, a1, _, _ = self._launch(u, v, a, d, f_ext, bc_mask, bc_vals)
+        return u1, v1, a1
+
+    def step_with_history(self, u, v, a, d, f_ext, bc_mask, bc_vals, H_old, want_psi=True):
+        """One step plus ``psi = psi+(u1)`` and ``H = max(H_old, psi)``: returns (u, v, a, H, psi)."""
+        self.last_iter = 1
+        return self._launch(u, v, a, d, f_ext, bc_mask, bc_vals, H_old=H_old, want_psi=want_psi)
+
+
+class DamageOperator:
+    """The T3 AT2 damage system of ``PhaseFieldDamageSolver`` (element history, no viscosity):
+    ``apply`` is ``_Ax`` (damage_solver.py:1715-1755), ``rhs`` the AT2 right-hand side
+    (:2252-2257) and ``diagonal`` the Jacobi diagonal (:2112-2119)."""
+
+    def __init__(self, fem: FEMOperators):
+        self.fem = fem
+        self.Gc_l0 = fem.material.Gc * fem.material.l0
+        self.Gc_over_l0 = fem.material.Gc / fem.material.l0
+
+    def apply(self, d: torch.Tensor, H: torch.Tensor) -> torch.Tensor:
+        fem = self.fem
+        d, H = fem._d(d), fem._e(H)
+        y = torch.empty_like(d)
+        check(_lib.lib().phast_t3_damage_apply(fem.plan.handle, ptr(H), ptr(d), ptr(y), stream()), "damage_apply")
+        return y
+
+    def rhs_and_diagonal(self, H: torch.Tensor):
+        fem = self.fem
+        H = fem._e(H)
+        b = torch.empty(fem.mesh.n_nodes, dtype=F64, device=fem.device)
+        dg = torch.empty_like(b)
+        check(_lib.lib().phast_t3_damage_rhs_diag(fem.plan.handle, ptr(H), ptr(b), ptr(dg), stream()),
+              "damage_rhs_diag")
+        return b, dg
+
+    def rhs(self, H: torch.Tensor) -> torch.Tensor:
+        return self.rhs_and_diagonal(H)[0]
+
+    def diagonal(self, H: torch.Tensor) -> torch.Tensor:
+        return self.rhs_and_diagonal(H)[1]

@@ -48,3 +48,19 @@ def test_product_does_not_import_oracle():
     for p in (ROOT / "phast_amd").rglob("*.py"):
         text = p.read_text()
         assert "import oracle" not in text and "from oracle" not in text, p
+
+
+def test_t3_plan_validates_connectivity_before_any_device_call():
+    """Host-side checks of phast_t3_plan_create reject bad meshes with PHAST_ERR_ARG."""
+    import numpy as np
+    from phast_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    L = _lib.lib()
+    nodes = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    h = ctypes.c_void_p()
+    for els, split in ((np.array([[0, 1, 3]], dtype=np.int32), 0), (np.array([[0, 1, -1]], dtype=np.int32), 0),
+                       (np.array([[0, 1, 2]], dtype=np.int32), 7)):
+        rc = L.phast_t3_plan_create(3, 1, nodes.ctypes.data, els.ctypes.data, 1.0, 0.2, 1.0, 1.0, 1e-7, 0, split,
+                                    ctypes.byref(h))
+        assert rc == _lib.PHAST_ERR_ARG and not h.value

@@ -1,0 +1,184 @@
+"""Parity of the T3 (linear triangle) HIP path with the oracle (GPU).
+
+Tolerances: fields within 1e-10 relative (max-norm relative to the field's max-norm);
+the history maximum bit-exact (each entry one of its two operands).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import t3 as O
+from oracle.q4 import Material as OMaterial
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-10
+SPLITS = [("amor", False), ("amor", True), ("isotropic", False), ("isotropic", True), ("spectral", False)]
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+T = lambda x, dev: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
+
+
+def scrambled_mesh(nx, ny, seed=0, W=5.0, H=4.0):
+    """Jittered structured triangles with node numbering, element order and corner order
+    randomly permuted (so half the triangles are clockwise and incidence lists are ragged)."""
+    rng = np.random.default_rng(seed)
+    nodes, els = O.structured_t3(nx, ny, W, H, jitter=0.2, seed=seed)
+    perm = rng.permutation(len(nodes))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(nodes))
+    nodes = nodes[perm]
+    els = inv[els][rng.permutation(len(els))]
+    rot = rng.integers(0, 3, len(els))
+    els = np.stack([np.roll(e, r) for e, r in zip(els, rot)])
+    flip = rng.random(len(els)) < 0.5
+    els[flip] = els[flip][:, ::-1]
+    return nodes, els
+
+
+def setup(nodes, els, split="amor", plane_stress=False):
+    from phast_amd import t3 as P
+    kw = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5, energy_split=split, plane_stress=plane_stress)
+    fem = P.FEMOperators(P.FEMMesh(nodes, els), P.Material(**kw))
+    ofem = O.T3Operators(O.T3Mesh(nodes, els), OMaterial(**kw))
+    return P, fem, ofem
+
+
+def state(n, E, seed=0):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-1e-4, 1e-4, (n, 2))
+    v = rng.uniform(-1e-1, 1e-1, (n, 2))
+    a = rng.uniform(-1e3, 1e3, (n, 2))
+    d = rng.uniform(0, 1, n)
+    H = rng.uniform(0, 1e-4, E)
+    return u, v, a, d, H
+
+
+@pytest.mark.parametrize("split,ps", SPLITS)
+def test_internal_force_psi_history(dev, split, ps):
+    nodes, els = scrambled_mesh(23, 17, seed=1)
+    P, fem, ofem = setup(nodes, els, split, ps)
+    u, v, a, d, H = state(len(nodes), len(els))
+    f = fem.internal_force(T(u, dev), T(d, dev))
+    assert rel(f, ofem.internal_force(u, d)) < RTOL
+    f2, psi, Hn = fem.internal_force_and_history(T(u, dev), T(d, dev), T(H, dev))
+    assert torch.equal(f2, f)
+    assert rel(psi, ofem.compute_psi_plus(u)) < RTOL
+    assert np.array_equal(Hn.cpu().numpy(), np.maximum(H, psi.cpu().numpy()))
+    assert (Hn.cpu().numpy() > H).any() and (Hn.cpu().numpy() == H).any()
+
+
+def test_mesh_precompute(dev):
+    nodes, els = scrambled_mesh(31, 12, seed=2)
+    P, fem, ofem = setup(nodes, els)
+    assert rel(fem.M_scalar, ofem.mesh.M_scalar) < 1e-14
+    assert fem.dt_cfl == pytest.approx(ofem.dt_cfl, rel=1e-14)
+
+
+@pytest.mark.parametrize("split,ps", [("amor", False), ("spectral", False), ("isotropic", True)])
+def test_explicit_step(dev, split, ps):
+    nodes, els = scrambled_mesh(19, 26, seed=3)
+    P, fem, ofem = setup(nodes, els, split, ps)
+    n, E = len(nodes), len(els)
+    u, v, a, d, H = state(n, E, seed=4)
+    rng = np.random.default_rng(5)
+    f_ext = rng.uniform(-1e-2, 1e-2, (n, 2))
+    bc = rng.random((n, 2)) < 0.1
+    bcv = rng.uniform(-1e-4, 1e-4, (n, 2))
+    dt = 0.4 * fem.dt_cfl
+    dyn = P.ExplicitDynamics(fem, dt=dt)
+    u1, v1, a1, Hn, psi = dyn.step_with_history(T(u, dev), T(v, dev), T(a, dev), T(d, dev), T(f_ext, dev),
+                                                torch.as_tensor(bc, device=dev), T(bcv, dev), T(H, dev))
+    # oracle step with the T3 operators (mechanics_solver.py:153-239)
+    ou = (u + dt * v) + (0.5 * dt * dt) * a
+    ou = np.where(bc, bcv, ou)
+    oa = ofem.M_vec_inv.reshape(-1, 2) * (f_ext - ofem.internal_force(ou, d))
+    ov = v + 0.5 * dt * (a + oa)
+    oa, ov = np.where(bc, 0.0, oa), np.where(bc, 0.0, ov)
+    assert rel(u1, ou) < RTOL and rel(v1, ov) < RTOL and rel(a1, oa) < RTOL
+    assert np.array_equal(u1.cpu().numpy()[bc], bcv[bc])
+    assert (a1.cpu().numpy()[bc] == 0).all() and (v1.cpu().numpy()[bc] == 0).all()
+    assert rel(psi, ofem.compute_psi_plus(ou)) < RTOL
+    assert np.array_equal(Hn.cpu().numpy(), np.maximum(H, psi.cpu().numpy()))
+    # the plain step (no f_ext, no BC) agrees with step_with_history's kinematics
+    s1, s2, s3 = dyn.step(T(u, dev), T(v, dev), T(a, dev), T(d, dev), None)
+    ou = (u + dt * v) + (0.5 * dt * dt) * a
+    oa = ofem.M_vec_inv.reshape(-1, 2) * (-ofem.internal_force(ou, d))
+    assert rel(s1, ou) < RTOL and rel(s3, oa) < RTOL and rel(s2, v + 0.5 * dt * (a + oa)) < RTOL
+
+
+def test_damage_operator(dev):
+    nodes, els = scrambled_mesh(27, 21, seed=6)
+    P, fem, ofem = setup(nodes, els)
+    dmg = P.DamageOperator(fem)
+    odmg = O.T3Damage(ofem.mesh, ofem.material)
+    _, _, _, d, H = state(len(nodes), len(els), seed=7)
+    H = H * 30.0
+    assert rel(dmg.apply(T(d, dev), T(H, dev)), odmg.apply(d, H)) < RTOL
+    b, dg = dmg.rhs_and_diagonal(T(H, dev))
+    assert rel(b, odmg.rhs(H)) < RTOL and rel(dg, odmg.diagonal(H)) < RTOL
+
+
+def test_damage_operator_dense_small(dev):
+    """Columns of the device operator on a tiny mesh form the symmetric textbook matrix."""
+    nodes, els = scrambled_mesh(3, 2, seed=8)
+    P, fem, ofem = setup(nodes, els)
+    dmg = P.DamageOperator(fem)
+    H = np.random.default_rng(9).uniform(0, 1e-3, len(els))
+    n = len(nodes)
+    cols = [dmg.apply(T(np.eye(n)[k], dev), T(H, dev)).cpu().numpy() for k in range(n)]
+    D = np.stack(cols, axis=1)
+    K = O.T3Damage(ofem.mesh, ofem.material).dense(H)
+    assert np.abs(D - K).max() < 1e-13 * np.abs(K).max()
+    assert np.abs(D - D.T).max() < 1e-14 * np.abs(D).max()
+    assert rel(dmg.diagonal(T(H, dev)), np.diag(K)) < 1e-13
+
+
+def test_single_triangle_and_unused_node(dev):
+    """Ragged edge cases: one element, and a node no element touches (zero force/mass)."""
+    nodes = np.array([[0.0, 0.0], [2.0, 0.0], [0.0, 3.0], [5.0, 5.0]])
+    els = np.array([[0, 2, 1]])  # clockwise
+    P, fem, ofem = setup(nodes, els, "spectral")
+    u = np.array([[0.0, 0.0], [2e-4, 1e-5], [-3e-5, -1e-4], [1.0, 1.0]])
+    d = np.array([0.1, 0.2, 0.3, 0.9])
+    f = fem.internal_force(T(u, dev), T(d, dev)).cpu().numpy()
+    assert rel(f, ofem.internal_force(u, d)) < RTOL
+    assert (f[3] == 0.0).all()
+    assert fem.M_scalar.cpu().numpy()[3] == 0.0
+
+
+def test_large_mesh(dev):
+    """A 1024 x 512-cell mesh (1M triangles) against the oracle, plus the patch and
+    uniform-history properties that hold at any size."""
+    nodes, els = O.structured_t3(1024, 512, 20.0, 10.0, jitter=0.25, seed=10)
+    P, fem, ofem = setup(nodes, els, "amor")
+    n, E = len(nodes), len(els)
+    u, v, a, d, H = state(n, E, seed=11)
+    assert rel(fem.internal_force(T(u, dev), T(d, dev)), ofem.internal_force(u, d)) < RTOL
+    G = np.array([[1.1e-4, -0.4e-4], [0.7e-4, -0.5e-4]])
+    ul = nodes @ G.T
+    f = fem.internal_force(T(ul, dev), T(d * 0, dev)).cpu().numpy()
+    xs, ys = nodes[:, 0], nodes[:, 1]
+    inner = (xs > 1e-9) & (xs < 20 - 1e-9) & (ys > 1e-9) & (ys < 10 - 1e-9)
+    assert np.abs(f[inner]).max() < 1e-9 * np.abs(f).max()
+    dmg = P.DamageOperator(fem)
+    Hc = 2e-4
+    Hu = T(np.full(E, Hc), dev)
+    du = T(np.full(n, 2 * Hc / (3e-3 / 0.5 + 2 * Hc)), dev)
+    assert rel(dmg.apply(du, Hu), dmg.rhs(Hu)) < 1e-12
+
+
+def test_plan_rejects_bad_connectivity(dev):
+    from phast_amd import t3 as P
+    from phast_amd._lib import PhastLibraryError
+    nodes = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    with pytest.raises(ValueError):
+        P.FEMMesh(nodes, np.array([[0, 1, 3]]))
+    mat = P.Material(E=1.0, nu=0.2, rho=1.0, Gc=1.0, l0=1.0)
+    with pytest.raises((ValueError, PhastLibraryError)):
+        P.T3Plan(nodes, np.array([[0, 1, 7]]), mat)
