@@ -10,8 +10,9 @@ apply kernel's algorithmic FLOPs (940 per voxel, sum-factorised) against the 78.
 FP64 peak -- its intensity (940 FLOP / 57 B) is above the FP64 ridge -- and also reports the
 algorithmic bytes (57 B per node: u 24 + d 8 + phase 1 + r 24) against 8 TB/s;
 ``cpu_baseline`` times the oracle's numpy restatement of the same apply on the host cores.
-``secondary`` holds the Q4 apply and fused explicit step, the larger voxel sizes, the
-64-specimen batched run and the reaction-curve gradient, measured in the same run.
+``secondary`` holds the Q4 apply and fused explicit step, the T3 force / explicit step /
+damage apply on an 8.4 M-node split-quad plate, the larger voxel sizes, the 64-specimen
+batched run and the reaction-curve gradient, measured in the same run.
 """
 from __future__ import annotations
 
@@ -36,6 +37,10 @@ HEX8_64_TRAFFIC = {(32, 14, 3): ((12528.61 + 2147.25) * 2.0 + 8269.75 + 2112.06)
                    (16, 14, 6): ((15001.75 + 1091.25) * 2.0 + 7213.75 + 1056.06) * 1024.0}
 Q4_APPLY_BYTES_PER_NODE = 40.0  # u 16 + d 8 + f 16
 Q4_STEP_BYTES_PER_NODE = 192.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, H r+w 64
+# T3 split-quad plates (2 triangles per node; geometry recomputed from the coordinates)
+T3_APPLY_BYTES_PER_NODE = 80.0  # u 16 + d 8 + f 16 + xy 16 + 2 triangles x 12 B connectivity
+T3_STEP_BYTES_PER_NODE = 216.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, xy 16, conn 24, H r+w + psi 48
+T3_DAMAGE_BYTES_PER_NODE = 72.0  # x 8 + y 8 + H 16 + xy 16 + conn 24
 
 
 def parse():
@@ -45,7 +50,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary measurements")
-    ap.add_argument("--secondary", type=str, default="q4,hex8", help="secondary parts to run: q4, hex8")
+    ap.add_argument("--secondary", type=str, default="q4,hex8,t3", help="secondary parts to run: q4, hex8, t3")
     ap.add_argument("--q4-tile", type=str, default="", help="fixed Q4 strip threads,rows (skips the autotuner)")
     ap.add_argument("--hex8-big-tiles", type=str, default="", help="fixed tx,ty,kz of the 128^3/256^3 runs")
     ap.add_argument("--batch-tile", type=str, default="", help="fixed strip of the batched run (skips its autotuner)")
@@ -250,6 +255,8 @@ def secondary(rng, dev, a):
         entry.update({"explicit_step_ms": t2 * 1e3, "explicit_step_GB_s": Q4_STEP_BYTES_PER_NODE * n / t2 / 1e9})
         res[name] = entry
         del fem, dyn, u, d, v, acc, fext, Hq
+    if "t3" in parts:
+        res["t3_split_quad_4096x2048"] = t3_secondary(rng, dev, a)
     for nv in ((128, 256) if "hex8" in parts else ()):
         op, mats, _ = pore_cube(nv, 12 * nv // 64, dev)
         x = T(rng.uniform(-1e-6, 1e-6, op.n_mech))
@@ -266,6 +273,38 @@ def secondary(rng, dev, a):
         del op, x, d
         torch.cuda.empty_cache()
     return res
+
+
+def t3_secondary(rng, dev, a):
+    """T3 route on a 4096 x 2048-cell split-quad plate (8.4 M nodes, 16.8 M triangles; the
+    mesh of core/mesh_generator.py:1552-1633 at h = 0.125): internal force, explicit step
+    with psi+ and history, damage apply."""
+    import numpy as np
+    import torch
+    from phast_amd import t3 as P3
+    T = lambda z: torch.as_tensor(z, dtype=torch.float64, device=dev)
+    mesh = P3.FEMMesh.sent_split_quad(512.0, 256.0, 4096, 2048, device=dev)
+    mat = P3.Material(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.25)
+    fem = P3.FEMOperators(mesh, mat)
+    n, E = mesh.n_nodes, mesh.n_elems
+    u = T(rng.uniform(-1e-4, 1e-4, (n, 2)))
+    d = T(rng.uniform(0, 1, n))
+    steps = max(10, a.steps // 10)
+    t, _ = events_time(lambda: fem.internal_force(u, d), steps, 3)
+    entry = {"nodes": n, "triangles": E, "apply_GDOF_s": 2 * n / t / 1e9,
+             "apply_GB_s": T3_APPLY_BYTES_PER_NODE * n / t / 1e9, "apply_ms": t * 1e3}
+    dyn = P3.ExplicitDynamics(fem, dt=0.5 * fem.dt_cfl)
+    v, acc, fext = T(rng.uniform(-1e-2, 1e-2, (n, 2))), T(np.zeros((n, 2))), T(np.zeros((n, 2)))
+    H = T(np.zeros(E))
+    t2, _ = events_time(lambda: dyn._launch(u, v, acc, d, fext, None, None, H_old=H, want_psi=True), steps, 3)
+    dmg = P3.PhaseFieldDamageSolver(fem)
+    Hd = T(rng.uniform(0, 1e-3, E))
+    t3, _ = events_time(lambda: dmg.damage_apply(Hd, d), steps, 3)
+    entry.update({"explicit_step_ms": t2 * 1e3, "explicit_step_GB_s": T3_STEP_BYTES_PER_NODE * n / t2 / 1e9,
+                  "damage_apply_ms": t3 * 1e3, "damage_apply_GB_s": T3_DAMAGE_BYTES_PER_NODE * n / t3 / 1e9})
+    del fem, dyn, dmg, u, d, v, acc, fext, H, Hd
+    torch.cuda.empty_cache()
+    return entry
 
 
 def lognormal_field(rng, ny, nx, mean, cov, corr):

@@ -208,13 +208,26 @@ int phast_t3_explicit_step(const phast_t3_plan *plan, double dt, const double *u
                            double *H_new, double *psi, hipStream_t stream);
 /* AT2 damage operator of element history H (E,) (solvers/damage_solver.py:1715-1755 _Ax):
  *   y = Gc l0 sum_e A (grad phi_a . grad d_e) + sum_e rc_e (d_a + sum_b d_b),
- *   rc = (2H + Gc/l0) A / 12  (damage_solver.py:2174-2176);  x and y must differ. */
-int phast_t3_damage_apply(const phast_t3_plan *plan, const double *H, const double *x, double *y,
+ *   rc = (2H + Gc/l0) A / 12  (damage_solver.py:2168-2177);  x and y must differ.
+ * Gc_e (E,) optional per-element Gc (the _Gc_l0_e / _Gc_over_l0_e branch,
+ * damage_solver.py:1412-1433): then Gc_e l0 weights each element's Laplacian term. */
+int phast_t3_damage_apply(const phast_t3_plan *plan, const double *H, const double *Gc_e, const double *x, double *y,
                           hipStream_t stream);
-/* b = sum_e 2H A/3 (damage_solver.py:2252-2257) and the Jacobi diagonal
+/* b = sum_e 2H A/3 (damage_solver.py:2249-2258) and the Jacobi diagonal
  * sum_e (Gc l0 A |grad phi_a|^2 + 2 rc) (damage_solver.py:2112-2119); either may be null. */
-int phast_t3_damage_rhs_diag(const phast_t3_plan *plan, const double *H, double *b, double *diag,
-                             hipStream_t stream);
+int phast_t3_damage_rhs_diag(const phast_t3_plan *plan, const double *H, const double *Gc_e, double *b,
+                             double *diag, hipStream_t stream);
+/* AT2 damage solve with the Jacobi-preconditioned CG of PhaseFieldDamageSolver.solve
+ * (damage_solver.py:2493-2684): method 0 = projected CG with the bounds d_prev <= d <= 1
+ * (:3527-3775), 1 = CG then clamp (:3377-3521); same arguments, state machine and
+ * scalar steps as phast_q4_damage_solve.  fixed (uint8 (N,)) + fixed_vals pin nodes
+ * (pf_dirichlet); d_init optional initial guess; active_out (uint8 (N,)) optional final
+ * active set of method 0; iters/status are host int32.  Allocates its workspace on the
+ * plan's first solve. */
+int phast_t3_damage_solve(phast_t3_plan *plan, int32_t method, const double *H, const double *Gc_e,
+                          const double *d_prev, const double *d_init, const uint8_t *fixed, const double *fixed_vals,
+                          double tol, int32_t max_iter, double *d_out, uint8_t *active_out, int32_t *iters,
+                          int32_t *status, hipStream_t stream);
 
 /* =====================================================================================
  * 3D periodic hex8 voxel cells

@@ -13,7 +13,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .q4 import Material
+from .q4 import Material, PhaseFieldDamageSolver
 
 SPECTRAL_EPS = 1e-12  # core/fem_operators.py:126 (float64)
 
@@ -177,43 +177,74 @@ class T3Operators:
 
 
 class T3Damage:
-    """AT2 damage operator of the element-H path (solvers/damage_solver.py)."""
+    """AT2 damage operator of the element-H path (solvers/damage_solver.py).  ``Gc_e`` is the
+    optional per-element Gc field (the ``_Gc_l0_e`` / ``_Gc_over_l0_e`` branch,
+    damage_solver.py:1412-1433, 603-611)."""
 
-    def __init__(self, mesh: T3Mesh, material: Material):
+    def __init__(self, mesh: T3Mesh, material: Material, Gc_e: np.ndarray | None = None):
         self.mesh, self.material = mesh, material
         gp = mesh.grad_phi
         self.diag_lap = mesh.areas[:, None] * (gp[:, :, 0] ** 2 + gp[:, :, 1] ** 2)  # :1404-1406
         self.Gc_l0 = material.Gc * material.l0
         self.Gc_over_l0 = material.Gc / material.l0
+        self.Gc_l0_e = None if Gc_e is None else Gc_e * material.l0
+        self.Gc_over_l0_e = None if Gc_e is None else Gc_e / material.l0
 
     def reaction_coeff(self, H):
-        """(2 H + Gc/l0) area / 12 (damage_solver.py:2174-2176, no viscosity)."""
-        return (2.0 * H + self.Gc_over_l0) * self.mesh.areas / 12.0
+        """(2 H + Gc/l0) area / 12 (damage_solver.py:2168-2177, no viscosity)."""
+        ratio = self.Gc_over_l0 if self.Gc_over_l0_e is None else self.Gc_over_l0_e
+        return (2.0 * H + ratio) * self.mesh.areas / 12.0
 
     def rhs(self, H):
-        """b = scatter(2 H area/3) (damage_solver.py:2252-2257, AT2)."""
+        """b = scatter(2 H area/3) (damage_solver.py:2249-2258, AT2)."""
         m = self.mesh
         return scatter(m.elem_flat, np.repeat(2.0 * H * (m.areas / 3.0), 3), m.n_nodes)
 
     def diagonal(self, H):
         """Jacobi diagonal Gc l0 diag_lap + 2 rc (damage_solver.py:2112-2119)."""
         m = self.mesh
-        contrib = self.Gc_l0 * self.diag_lap + 2.0 * self.reaction_coeff(H)[:, None]
+        lap = self.Gc_l0 * self.diag_lap if self.Gc_l0_e is None else self.Gc_l0_e[:, None] * self.diag_lap
+        contrib = lap + 2.0 * self.reaction_coeff(H)[:, None]
         return scatter(m.elem_flat, contrib.ravel(), m.n_nodes)
 
     def apply(self, d, H):
-        """(Gc l0 K_lap + M_react) d (damage_solver.py:1726-1755)."""
+        """(Gc l0 K_lap + M_react) d (damage_solver.py:1719-1755)."""
         m = self.mesh
         d_e = d[m.elements]
         gp = m.grad_phi
         gdx, gdy = (gp[:, :, 0] * d_e).sum(1), (gp[:, :, 1] * d_e).sum(1)
         lap = m.areas[:, None] * (gp[:, :, 0] * gdx[:, None] + gp[:, :, 1] * gdy[:, None])
+        if self.Gc_l0_e is not None:
+            lap = self.Gc_l0_e[:, None] * lap
         out = scatter(m.elem_flat, lap.ravel(), m.n_nodes)
         rc = self.reaction_coeff(H)
         react = scatter(m.elem_flat, (rc[:, None] * (d_e + d_e.sum(1)[:, None])).ravel(), m.n_nodes)
+        if self.Gc_l0_e is not None:
+            return out + react
         return self.Gc_l0 * out + react
 
     def dense(self, H):
         """Dense matrix of ``apply`` (columns of the identity) for checks on small meshes."""
         n = self.mesh.n_nodes
         return np.stack([self.apply(np.eye(n)[k], H) for k in range(n)], axis=1)
+
+
+class T3DamageSolver(PhaseFieldDamageSolver):
+    """Projected / post-clamp Jacobi CG of ``PhaseFieldDamageSolver`` on the T3 operator:
+    the solve loops (damage_solver.py:3377-3775) are the element-type independent ones of
+    the Q4 oracle; ``Ax`` is ``_Ax`` T3 and ``prepare`` is ``_prepare_cg`` for element H
+    (damage_solver.py:2160-2268: reaction coefficient, b, M_inv = 1/(diag + 1e-30))."""
+
+    def __init__(self, mesh: T3Mesh, material: Material, tol=1e-5, max_iter=5000,
+                 bounds_method="projected_cg", Gc_e: np.ndarray | None = None):
+        self.mesh = mesh
+        self.op = T3Damage(mesh, material, Gc_e)
+        self.tol, self.max_iter, self.bounds_method = tol, max_iter, bounds_method
+        self.last_iter = 0
+        self.last_status = ""
+
+    def Ax(self, d, H):
+        return self.op.apply(d, H)
+
+    def prepare(self, H, d_prev):
+        return H, self.op.rhs(H), 1.0 / (self.op.diagonal(H) + 1e-30)

@@ -16,6 +16,7 @@
 // outputs such as H) by the strip of its lower-left node.  The block -> (instance, strip)
 // table makes one launch cover a whole batch of plates of uneven sizes.
 #include "common.h"
+#include "damage_cg.h"
 
 #include <algorithm>
 #include <cmath>
@@ -38,11 +39,8 @@ struct Q4Inst {
     int32_t split, ky;
 };
 
-// Per-instance state of the device-resident damage CG.
-struct Q4CG {
-    double rz, rz_new, pAp, alpha, rr0, bb, tol_sq, rfn, beta, tol;
-    int32_t iter, max_iter, done, limited, check, restart, status, changed;
-};
+
+using Q4CG = DamageCG;
 
 constexpr int kParts = 4;  // per-tile partial sums kept between kernels
 
@@ -973,156 +971,16 @@ __global__ __launch_bounds__(NT) void q4_vec_kernel(VecArgs A) {
     vec_strip<NT, OP>(A, strip, S, red);
 }
 
-// ---------------------------------------------------------------------------------------
-// per-instance scalar steps (one block per instance reduces that instance's tile partials)
-// ---------------------------------------------------------------------------------------
-
-enum ScalarOp : int { S_INIT = 0, S_A = 1, S_B = 2, S_C = 3, S_PC_INIT = 4, S_PC_A = 5, S_PC_C = 6 };
-
-// Sum of part[off .. off+cnt) in one fixed order whatever the block width NT: 256 virtual
-// lanes, lane k summing part[k], part[k+256], ...; each virtual wave of 64 lanes reduced by
-// butterfly, then the four wave totals in order.  A block of NT threads runs 256/NT
-// virtual passes, so every block width (and every block that repeats the reduction)
-// produces the same bits.  red holds 4 doubles.
-template <int NT>
-__device__ __forceinline__ double reduce_sum_parts(const double *part, int off, int cnt, double *red) {
-    constexpr int V = 256 / NT;
-    double w[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-        double s = 0.0;
-        for (int k = threadIdx.x + v * NT; k < cnt; k += 256) s += part[off + k];
-        w[v] = wave_sum(s);
-    }
-    __syncthreads();
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-#pragma unroll
-        for (int v = 0; v < V; ++v) red[v * (NT / kWave) + threadIdx.x / kWave] = w[v];
-    }
-    __syncthreads();
-    return ((red[0] + red[1]) + red[2]) + red[3];
-}
-template <int NT>
-__device__ __forceinline__ double reduce_min_parts(const double *part, int off, int cnt, double *red) {
-    double s = __builtin_huge_val();
-    for (int k = threadIdx.x; k < cnt; k += NT) s = fmin(s, part[off + k]);
-    return block_min<NT>(s, red);
-}
-
-// One per-instance scalar step of the CG (thread 0 writes S).  Every thread of the block
-// takes part in the reductions; all return paths are block-uniform.
-template <int OP, int NT>
-__device__ __forceinline__ void scalar_step(QI &P, Q4CG &S, const double *part, int32_t n_tiles,
-                                            int32_t check_every, double *red) {
-    const int off = P.tile_off, cnt = P.n_tiles;
-    if (OP == S_INIT || OP == S_PC_INIT) {
-        const double rr = reduce_sum_parts<NT>(part, off, cnt, red);
-        const double bb = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
-        const double rz = reduce_sum_parts<NT>(part, 2 * n_tiles + off, cnt, red);
-        if (threadIdx.x == 0) {
-            S.rr0 = rr;
-            S.bb = bb;
-            S.rz = rz;
-            const double ref = fmax(fmax(sqrt(bb), sqrt(rr)), 1e-30);
-            S.tol_sq = (S.tol * ref) * (S.tol * ref);
-            S.iter = 0;
-            S.done = S.max_iter <= 0 ? 1 : 0;
-            S.status = S.max_iter <= 0 ? PHAST_CG_MAXITER : PHAST_CG_RUNNING;
-            S.limited = S.check = S.restart = 0;
-        }
-    } else if (OP == S_A || OP == S_PC_A) {
-        const double pAp = reduce_sum_parts<NT>(part, off, cnt, red);
-        double up = 0.0, dn = 0.0;
-        if (OP == S_A) {
-            up = reduce_min_parts<NT>(part, n_tiles + off, cnt, red);
-            dn = reduce_min_parts<NT>(part, 2 * n_tiles + off, cnt, red);
-        }
-        if (threadIdx.x == 0) {
-            S.iter += 1;
-            S.pAp = pAp;
-            if (!(pAp > 0.0)) {  // damage_solver.py:3560-3575 (pAp <= 0 -> break)
-                S.done = 1;
-                S.status = PHAST_CG_NONPOSITIVE;
-                S.limited = 0;
-                return;
-            }
-            const double alpha_cg = S.rz / (pAp + 1e-30);
-            if (OP == S_A) {
-                double amax = alpha_cg;  // damage_solver.py:3579-3605
-                if (up < amax) amax = up;
-                if (dn < amax) amax = dn;
-                S.limited = (amax < alpha_cg && amax >= 0.0) ? 1 : 0;
-                S.alpha = S.limited ? amax : alpha_cg;
-                S.check = (!S.limited && (S.iter % check_every) == 0) ? 1 : 0;
-            } else {
-                S.alpha = alpha_cg;
-                S.check = (S.iter % check_every) == 0 ? 1 : 0;
-            }
-        }
-    } else if (OP == S_B) {
-        if (!(S.limited || S.check)) {
-            if (threadIdx.x == 0) S.restart = 0;
-            return;
-        }
-        const double rfn = reduce_sum_parts<NT>(part, off, cnt, red);
-        const double chg = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
-        if (threadIdx.x == 0) {
-            S.rfn = rfn;
-            if (rfn < S.tol_sq) {
-                S.done = 1;
-                S.status = PHAST_CG_CONVERGED;
-            } else if (rfn > 1e12 * fmax(S.rr0, 1e-30)) {
-                S.done = 1;
-                S.status = PHAST_CG_DIVERGED;
-            } else if (!isfinite(rfn)) {
-                S.done = 1;
-                S.status = PHAST_CG_NONFINITE;
-            }
-            S.changed = chg > 0.0 ? 1 : 0;
-            S.restart = (S.limited || (S.check && chg > 0.0)) ? 1 : 0;
-        }
-    } else if (OP == S_C) {
-        const double rz_new = reduce_sum_parts<NT>(part, off, cnt, red);
-        if (threadIdx.x == 0) {
-            S.beta = S.restart ? 0.0 : rz_new / (S.rz + 1e-30);
-            S.rz = rz_new;
-            if (S.iter >= S.max_iter) {
-                S.done = 1;
-                S.status = PHAST_CG_MAXITER;
-            }
-        }
-    } else if (OP == S_PC_C) {
-        const double rr = reduce_sum_parts<NT>(part, off, cnt, red);
-        const double rz_new = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
-        if (threadIdx.x == 0) {
-            S.rfn = rr;
-            if (rr < S.tol_sq) {  // damage_solver.py:3496-3506
-                S.done = 1;
-                S.status = PHAST_CG_CONVERGED;
-                return;
-            }
-            if (rr > 1e12 * fmax(S.rr0, 1e-30)) {
-                S.done = 1;
-                S.status = PHAST_CG_DIVERGED;
-                return;
-            }
-            S.beta = rz_new / (S.rz + 1e-30);
-            S.rz = rz_new;
-            if (S.iter >= S.max_iter) {
-                S.done = 1;
-                S.status = PHAST_CG_MAXITER;
-            }
-        }
-    }
-}
-
 template <int OP>
 __global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG *cg, const double *part,
                                                         int32_t n_tiles, int32_t check_every) {
     __shared__ double red[4];
     Q4CG &S = cg[blockIdx.x];
     if (OP != S_INIT && OP != S_PC_INIT && S.done) return;
-    scalar_step<OP, 256>(inst_at(inst, blockIdx.x), S, part, n_tiles, check_every, red);
+    {
+        QI &P = inst_at(inst, blockIdx.x);
+        scalar_step<OP, 256>(P.tile_off, P.n_tiles, S, part, n_tiles, check_every, red);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1227,12 +1085,12 @@ __global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
     if (METHOD == 0) {
         PHAST_STRIPS dmg_strip<NT, OP_CG_INIT>(A.Dx, base + s_, sm, red);
         barrier();
-        scalar_step<S_INIT, NT>(P, S, A.V.part, n_tiles, check_every, red);
+        scalar_step<S_INIT, NT>(P.tile_off, P.n_tiles, S, A.V.part, n_tiles, check_every, red);
         barrier();
         while (!S.done) {
             PHAST_STRIPS dmg_strip<NT, OP_CG_AP>(A.Dp, base + s_, sm, red);
             barrier();
-            scalar_step<S_A, NT>(P, S, A.V.part, n_tiles, check_every, red);
+            scalar_step<S_A, NT>(P.tile_off, P.n_tiles, S, A.V.part, n_tiles, check_every, red);
             __syncthreads();
             if (S.done) break;
             PHAST_STRIPS vec_strip<NT, V_UPDATE_D>(A.V, base + s_, S, red);
@@ -1240,7 +1098,7 @@ __global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
                 barrier();
                 PHAST_STRIPS dmg_strip<NT, OP_CG_RESID>(A.Dx, base + s_, sm, red);
                 barrier();
-                scalar_step<S_B, NT>(P, S, A.V.part, n_tiles, check_every, red);
+                scalar_step<S_B, NT>(P.tile_off, P.n_tiles, S, A.V.part, n_tiles, check_every, red);
                 __syncthreads();
                 if (S.done) break;
             } else {
@@ -1250,7 +1108,7 @@ __global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
             }
             PHAST_STRIPS vec_strip<NT, V_PRECOND>(V3, base + s_, S, red);
             barrier();
-            scalar_step<S_C, NT>(P, S, V3.part, n_tiles, check_every, red);
+            scalar_step<S_C, NT>(P.tile_off, P.n_tiles, S, V3.part, n_tiles, check_every, red);
             __syncthreads();
             if (S.done) break;
             PHAST_STRIPS vec_strip<NT, V_UPDATE_P>(A.V, base + s_, S, red);
@@ -1260,12 +1118,12 @@ __global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
     } else {
         PHAST_STRIPS dmg_strip<NT, OP_PC_INIT>(A.Dx, base + s_, sm, red);
         barrier();
-        scalar_step<S_PC_INIT, NT>(P, S, A.V.part, n_tiles, check_every, red);
+        scalar_step<S_PC_INIT, NT>(P.tile_off, P.n_tiles, S, A.V.part, n_tiles, check_every, red);
         barrier();
         while (!S.done) {
             PHAST_STRIPS dmg_strip<NT, OP_PC_AP>(A.Dp, base + s_, sm, red);
             barrier();
-            scalar_step<S_PC_A, NT>(P, S, A.V.part, n_tiles, check_every, red);
+            scalar_step<S_PC_A, NT>(P.tile_off, P.n_tiles, S, A.V.part, n_tiles, check_every, red);
             __syncthreads();
             if (S.done) break;
             PHAST_STRIPS vec_strip<NT, V_PC_UPDATE>(A.V, base + s_, S, red);
@@ -1275,7 +1133,7 @@ __global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
             }
             PHAST_STRIPS vec_strip<NT, V_PC_PRECOND>(V3, base + s_, S, red);
             barrier();
-            scalar_step<S_PC_C, NT>(P, S, V3.part, n_tiles, check_every, red);
+            scalar_step<S_PC_C, NT>(P.tile_off, P.n_tiles, S, V3.part, n_tiles, check_every, red);
             __syncthreads();
             if (S.done) break;
             PHAST_STRIPS vec_strip<NT, V_PC_UPDATE_P>(A.V, base + s_, S, red);

@@ -12,6 +12,7 @@
 // contribution.  No element buffer round trip and no atomics, so results are deterministic.
 // Element-wise outputs (psi+, history) are written by the thread of the element's corner 0.
 #include "common.h"
+#include "damage_cg.h"
 
 #include <cmath>
 #include <cstring>
@@ -25,7 +26,7 @@ enum : int { T3_AMOR = PHAST_SPLIT_AMOR, T3_ISO = PHAST_SPLIT_ISOTROPIC, T3_SPEC
 constexpr double kSpectralEps = 1e-12;  // core/fem_operators.py:126 (float64)
 
 struct T3Mat {
-    double lam, mu, kappa, ezz_coef, eta, Gc_l0, Gc_over_l0;
+    double lam, mu, kappa, ezz_coef, eta, Gc_l0, Gc_over_l0, l0;
     int32_t split, plane_stress;
 };
 
@@ -201,15 +202,15 @@ __global__ __launch_bounds__(256) void t3_predictor_kernel(int64_t nn, double dt
     u1[n] = make_double2(ux, uy);
 }
 
-// Damage operator of the element-H AT2 path (solvers/damage_solver.py):
-// OP 0: y = Gc l0 sum_e A (grad phi_a . grad d_e) + sum_e rc_e (d_a + sum_e d), rc = (2H + Gc/l0) A/12
-//       (the two scatters of :1726-1755 summed separately, then combined);
-// OP 1: b = sum_e 2H A/3 (:2252-2257) and diag = sum_e (Gc l0 A |grad phi_a|^2 + 2 rc) (:2112-2119).
+// Damage operator of the element-H AT2 path (solvers/damage_solver.py), row n:
+// OP 0: returns (A x)_n = Gc l0 sum_e A (grad phi_a . grad x_e) + sum_e rc_e (x_a + sum_b x_b),
+//       rc = (2H + Gc/l0) A/12 (the two scatters of :1719-1755 summed separately, then
+//       combined; with a per-element Gc field the Gc l0 weight goes inside the first sum);
+// OP 1: b_n = sum_e 2H A/3 (:2249-2258) into s0 and diag_n = sum_e (Gc l0 A |grad phi_a|^2
+//       + 2 rc) (:2112-2119) into s1.
 template <int OP>
-__global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3Mat M, const double *H, const double *x,
-                                                        double *y, double *y2) {
-    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (n >= m.n_nodes) return;
+__device__ __forceinline__ double t3_damage_row(const T3Mesh &m, const T3Mat &M, const double *H, const double *Gc,
+                                                const double *x, int64_t n, double *s1_out = nullptr) {
     double s0 = 0.0, s1 = 0.0;
     for (int32_t k = m.ptr[n]; k < m.ptr[n + 1]; ++k) {
         const int32_t ic = m.inc[k];
@@ -221,27 +222,247 @@ __global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3
         double area, gp[3][2];
         t3_geometry(m.xy[nd[0]], m.xy[nd[1]], m.xy[nd[2]], area, gp);
         const double h = H[e];
-        const double rc = (2.0 * h + M.Gc_over_l0) * area / 12.0;
+        const double ge = Gc ? Gc[e] : 0.0;
+        const double ratio = Gc ? ge / M.l0 : M.Gc_over_l0;  // damage_solver.py:2168-2177
+        const double rc = (2.0 * h + ratio) * area / 12.0;
         const double gx = a_ == 0 ? gp[0][0] : (a_ == 1 ? gp[1][0] : gp[2][0]);
         const double gy = a_ == 0 ? gp[0][1] : (a_ == 1 ? gp[1][1] : gp[2][1]);
         if (OP == 0) {
             const double d0 = x[nd[0]], d1 = x[nd[1]], d2 = x[nd[2]];
             const double gdx = (gp[0][0] * d0 + gp[1][0] * d1) + gp[2][0] * d2;
             const double gdy = (gp[0][1] * d0 + gp[1][1] * d1) + gp[2][1] * d2;
-            s0 += area * (gx * gdx + gy * gdy);
+            const double lap = area * (gx * gdx + gy * gdy);
+            s0 += Gc ? (ge * M.l0) * lap : lap;
             const double da = a_ == 0 ? d0 : (a_ == 1 ? d1 : d2);
             s1 += rc * (da + ((d0 + d1) + d2));
         } else {
             s0 += 2.0 * h * (area / 3.0);
-            s1 += M.Gc_l0 * (area * (gx * gx + gy * gy)) + 2.0 * rc;
+            s1 += (Gc ? ge * M.l0 : M.Gc_l0) * (area * (gx * gx + gy * gy)) + 2.0 * rc;
         }
     }
+    if (OP == 0) return Gc ? s0 + s1 : M.Gc_l0 * s0 + s1;
+    *s1_out = s1;
+    return s0;
+}
+
+// OP 0: y = A x; OP 1: y = b, y2 = diag; OP 2: y = b, y2 = M_inv = 1/(diag + 1e-30)
+// (damage_solver.py:2265)
+template <int OP>
+__global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3Mat M, const double *H, const double *Gc,
+                                                        const double *x, double *y, double *y2) {
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n >= m.n_nodes) return;
     if (OP == 0) {
-        y[n] = M.Gc_l0 * s0 + s1;
+        y[n] = t3_damage_row<0>(m, M, H, Gc, x, n);
     } else {
-        if (y) y[n] = s0;
-        if (y2) y2[n] = s1;
+        double dg;
+        const double bb = t3_damage_row<1>(m, M, H, Gc, x, n, &dg);
+        if (y) y[n] = bb;
+        if (y2) y2[n] = OP == 2 ? 1.0 / (dg + 1e-30) : dg;
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// device-resident projected / post-clamp Jacobi CG on the T3 operator (the Q4 solve's state
+// machine and scalar steps; one thread per node, per-block partial sums in part[k*nblk+b])
+// ---------------------------------------------------------------------------------------
+
+enum T3CgOp : int {
+    T_CG_INIT = 0,   // r = b - A d, active set, z, p; partials r.r, b.b, rf.z
+    T_CG_AP = 1,     // Ap = A p; partials p.Ap, min step ratios to the bounds
+    T_CG_RESID = 2,  // r = b - A d; new active set; partials rf.rf, changed
+    T_PC_INIT = 3,   // post-clamp: r = b - A d, z = r Minv, p = z; partials r.r, b.b, r.z
+    T_PC_AP = 4,     // post-clamp: Ap = A p; partial p.Ap
+    T_PC_RESID = 5,  // post-clamp true residual r = b - A d
+};
+
+enum T3VecOp : int {
+    TV_SETUP = 0,       // d = fixed ? vals : init ; lb = fixed ? vals : d_prev
+    TV_UPDATE_D = 1,    // projected: limited ? clamp(d + a p) : d += a p, r -= a Ap
+    TV_PRECOND = 2,     // projected: z = active ? 0 : r Minv ; partial rf.z
+    TV_UPDATE_P = 3,    // projected: p = active ? 0 : (restart ? z : beta p + z)
+    TV_FINISH = 4,      // d = min(max(d, lb), 1), fixed -> vals ; copy active
+    TV_PC_UPDATE = 5,   // post-clamp: d += a p, r -= a Ap
+    TV_PC_PRECOND = 6,  // post-clamp: z = r Minv ; partials r.r, r.z
+    TV_PC_UPDATE_P = 7,
+    TV_PC_FINISH = 8,   // d = clamp(max(d, lb), 0, 1), fixed -> vals
+};
+
+struct T3Cg {
+    const double *H, *Gc;
+    double *d, *r, *z, *p, *Ap, *lb;
+    const double *b, *minv, *init, *dprev, *vals;
+    const uint8_t *fixed;
+    uint8_t *active, *active_out;
+    double *d_out;
+    DamageCG *cg;
+    double *part;
+    int32_t nblk;
+};
+
+template <int OP>
+__global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Mat M, T3Cg A) {
+    __shared__ double red[4];
+    const DamageCG &S = A.cg[0];
+    if (OP != T_CG_INIT && OP != T_PC_INIT && S.done) return;
+    if (OP == T_CG_RESID && !(S.limited || S.check)) return;
+    if (OP == T_PC_RESID && !S.check) return;
+    constexpr double INF = __builtin_huge_val();
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double p0 = 0.0, p1 = (OP == T_CG_AP) ? INF : 0.0, p2 = (OP == T_CG_AP) ? INF : 0.0;
+    if (n < m.n_nodes) {
+        const double *x = (OP == T_CG_AP || OP == T_PC_AP) ? A.p : A.d;
+        const double g0 = t3_damage_row<0>(m, M, A.H, A.Gc, x, n);
+        const double xn = x[n];
+        if (OP == T_CG_INIT || OP == T_CG_RESID) {
+            const double ob = A.b[n], olb = A.lb[n];
+            const bool ofx = A.fixed && A.fixed[n];
+            double r = ob - g0;
+            if (ofx) r = 0.0;
+            // damage_solver.py:109-124 (projected_damage_active_mask)
+            const bool act = ((xn <= olb + 1e-14) && (r < 0.0)) || ((xn >= 1.0 - 1e-14) && (r > 0.0)) || ofx;
+            const double rf = act ? 0.0 : r;
+            A.r[n] = r;
+            if (OP == T_CG_RESID) {
+                p1 = (act != (A.active[n] != 0)) ? 1.0 : 0.0;
+                p0 = rf * rf;
+            } else {
+                double zz = rf * A.minv[n];
+                if (act) zz = 0.0;
+                A.z[n] = zz;
+                A.p[n] = zz;
+                p0 = r * r;
+                p1 = ob * ob;
+                p2 = rf * zz;
+            }
+            A.active[n] = act ? 1 : 0;
+        } else if (OP == T_CG_AP) {
+            const double od = A.d[n], olb = A.lb[n];
+            A.Ap[n] = g0;
+            p0 = xn * g0;
+            if (xn > 0.0) p1 = (1.0 - od) / xn;
+            if (xn < 0.0) p2 = (olb - od) / xn;
+        } else if (OP == T_PC_INIT) {
+            const double ob = A.b[n];
+            const bool ofx = A.fixed && A.fixed[n];
+            double r = ob - g0;
+            if (ofx) r = 0.0;
+            const double zz = r * (ofx ? 0.0 : A.minv[n]);
+            A.r[n] = r;
+            A.z[n] = zz;
+            A.p[n] = zz;
+            p0 = r * r;
+            p1 = ob * ob;
+            p2 = r * zz;
+        } else if (OP == T_PC_AP) {
+            A.Ap[n] = g0;
+            p0 = xn * g0;
+        } else {
+            const bool ofx = A.fixed && A.fixed[n];
+            double r = A.b[n] - g0;
+            if (ofx) r = 0.0;
+            A.r[n] = r;
+        }
+    }
+    if (OP == T_PC_RESID) return;
+    const int b = blockIdx.x;
+    const double s0 = block_sum<256>(p0, red);
+    if (threadIdx.x == 0) A.part[b] = s0;
+    if (OP == T_CG_AP) {
+        const double s1 = block_min<256>(p1, red);
+        const double s2 = block_min<256>(p2, red);
+        if (threadIdx.x == 0) {
+            A.part[A.nblk + b] = s1;
+            A.part[2 * A.nblk + b] = s2;
+        }
+    } else if (OP != T_PC_AP) {
+        const double s1 = block_sum<256>(p1, red);
+        if (threadIdx.x == 0) A.part[A.nblk + b] = s1;
+        if (OP == T_CG_INIT || OP == T_PC_INIT) {
+            const double s2 = block_sum<256>(p2, red);
+            if (threadIdx.x == 0) A.part[2 * A.nblk + b] = s2;
+        }
+    }
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
+    __shared__ double red[4];
+    const DamageCG &S = A.cg[0];
+    if (OP != TV_SETUP && OP != TV_FINISH && OP != TV_PC_FINISH && S.done) return;
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double p0 = 0.0, p1 = 0.0;
+    if (q < nn) {
+        if (OP == TV_SETUP) {
+            const bool fx = A.fixed && A.fixed[q];
+            const double dp = A.dprev[q];
+            const double v = A.fixed ? A.vals[q] : 0.0;
+            A.d[q] = fx ? v : (A.init ? A.init[q] : dp);
+            A.lb[q] = fx ? v : dp;
+        } else if (OP == TV_UPDATE_D) {
+            if (S.limited) {
+                double dn = A.d[q] + S.alpha * A.p[q];
+                const double l = A.lb[q];
+                dn = dn >= l ? dn : l;      // torch.maximum(d, lb)
+                dn = dn <= 1.0 ? dn : 1.0;  // clamp(max=1)
+                A.d[q] = dn;
+            } else {
+                A.d[q] = A.d[q] + S.alpha * A.p[q];
+                A.r[q] = A.r[q] - S.alpha * A.Ap[q];
+            }
+        } else if (OP == TV_PRECOND) {
+            const bool act = A.active[q] != 0;
+            const double rf = act ? 0.0 : A.r[q];
+            double zz = rf * A.minv[q];
+            if (act) zz = 0.0;
+            A.z[q] = zz;
+            p0 = rf * zz;
+        } else if (OP == TV_UPDATE_P) {
+            double pn = S.restart ? A.z[q] : A.p[q] * S.beta + A.z[q];
+            if (A.active[q]) pn = 0.0;
+            A.p[q] = pn;
+        } else if (OP == TV_FINISH || OP == TV_PC_FINISH) {
+            double dn = A.d[q];
+            const double l = A.lb[q];
+            dn = dn >= l ? dn : l;
+            if (OP == TV_FINISH) {
+                dn = dn <= 1.0 ? dn : 1.0;
+            } else {
+                dn = fmin(fmax(dn, 0.0), 1.0);
+            }
+            if (A.fixed && A.fixed[q]) dn = A.vals[q];
+            A.d_out[q] = dn;
+            if (OP == TV_FINISH && A.active_out) A.active_out[q] = A.active[q] ? 1 : 0;
+        } else if (OP == TV_PC_UPDATE) {
+            A.d[q] = A.d[q] + S.alpha * A.p[q];
+            A.r[q] = A.r[q] - S.alpha * A.Ap[q];
+        } else if (OP == TV_PC_PRECOND) {
+            const double rr = A.r[q];
+            const double zz = rr * ((A.fixed && A.fixed[q]) ? 0.0 : A.minv[q]);
+            A.z[q] = zz;
+            p0 = rr * rr;
+            p1 = rr * zz;
+        } else if (OP == TV_PC_UPDATE_P) {
+            A.p[q] = A.p[q] * S.beta + A.z[q];
+        }
+    }
+    if (OP == TV_PRECOND || OP == TV_PC_PRECOND) {
+        const double s0 = block_sum<256>(p0, red);
+        if (threadIdx.x == 0) A.part[blockIdx.x] = s0;
+        if (OP == TV_PC_PRECOND) {
+            const double s1 = block_sum<256>(p1, red);
+            if (threadIdx.x == 0) A.part[A.nblk + blockIdx.x] = s1;
+        }
+    }
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void t3_scalar_kernel(DamageCG *cg, const double *part, int32_t nblk,
+                                                        int32_t check_every) {
+    __shared__ double red[4];
+    DamageCG &S = cg[0];
+    if (OP != S_INIT && OP != S_PC_INIT && S.done) return;
+    scalar_step<OP, 256>(0, nblk, S, part, nblk, check_every, red);
 }
 
 }  // namespace phast
@@ -254,6 +475,12 @@ struct phast_t3_plan {
     double2 *xy = nullptr;
     int32_t *el = nullptr, *ptr = nullptr, *inc = nullptr;
     std::vector<double> M_scalar;  // lumped mass per node (area/3 scatter), host copy
+    // damage-solve workspace, allocated on the first solve
+    double *r = nullptr, *p = nullptr, *z = nullptr, *Ap = nullptr, *b = nullptr, *minv = nullptr, *lb = nullptr;
+    double *part = nullptr;
+    uint8_t *active = nullptr;
+    DamageCG *cg = nullptr, *h_cg = nullptr;
+    int32_t nblk = 0;
     double h_min = 0.0, total_area = 0.0;
 };
 
@@ -276,6 +503,7 @@ extern "C" int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const doub
     M.eta = eta_residual;
     M.Gc_l0 = Gc * l0;
     M.Gc_over_l0 = Gc / l0;
+    M.l0 = l0;
     M.split = split;
     M.plane_stress = plane_stress;
     // host geometry: lumped mass in elem_flat order and h_min (core/mesh.py:270-286)
@@ -325,6 +553,10 @@ extern "C" int phast_t3_plan_destroy(phast_t3_plan *pl) {
     (void)hipFree(pl->el);
     (void)hipFree(pl->ptr);
     (void)hipFree(pl->inc);
+    for (double *q : {pl->r, pl->p, pl->z, pl->Ap, pl->b, pl->minv, pl->lb, pl->part}) (void)hipFree(q);
+    (void)hipFree(pl->active);
+    (void)hipFree(pl->cg);
+    if (pl->h_cg) (void)hipHostFree(pl->h_cg);
     delete pl;
     return PHAST_OK;
 }
@@ -379,20 +611,153 @@ extern "C" int phast_t3_explicit_step(const phast_t3_plan *pl, double dt, const 
     return PHAST_OK;
 }
 
-extern "C" int phast_t3_damage_apply(const phast_t3_plan *pl, const double *H, const double *x, double *y,
-                                     hipStream_t st) {
+extern "C" int phast_t3_damage_apply(const phast_t3_plan *pl, const double *H, const double *Gc_e, const double *x,
+                                     double *y, hipStream_t st) {
     PHAST_REQUIRE(pl && H && x && y && x != y, "phast_t3_damage_apply: bad argument");
-    hipLaunchKernelGGL(t3_damage_kernel<0>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H, x, y,
+    hipLaunchKernelGGL(t3_damage_kernel<0>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H, Gc_e, x, y,
                        (double *)nullptr);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
 
-extern "C" int phast_t3_damage_rhs_diag(const phast_t3_plan *pl, const double *H, double *b, double *diag,
-                                        hipStream_t st) {
+extern "C" int phast_t3_damage_rhs_diag(const phast_t3_plan *pl, const double *H, const double *Gc_e, double *b,
+                                        double *diag, hipStream_t st) {
     PHAST_REQUIRE(pl && H && (b || diag), "phast_t3_damage_rhs_diag: null argument");
-    hipLaunchKernelGGL(t3_damage_kernel<1>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H,
+    hipLaunchKernelGGL(t3_damage_kernel<1>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H, Gc_e,
                        (const double *)nullptr, b, diag);
     PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+#define T3_TRY(x)                         \
+    do {                                  \
+        const int rc_ = (x);              \
+        if (rc_ != PHAST_OK) return rc_;  \
+    } while (0)
+
+static int t3_ensure_work(phast_t3_plan *pl) {
+    if (pl->r) return PHAST_OK;
+    const size_t nb = sizeof(double) * (size_t)pl->mesh.n_nodes;
+    pl->nblk = (int32_t)ceil_div(pl->mesh.n_nodes, 256);
+    for (double **q : {&pl->r, &pl->p, &pl->z, &pl->Ap, &pl->b, &pl->minv, &pl->lb}) PHAST_HIP(hipMalloc(q, nb));
+    PHAST_HIP(hipMalloc(&pl->part, sizeof(double) * 3 * (size_t)pl->nblk));
+    PHAST_HIP(hipMalloc(&pl->active, (size_t)pl->mesh.n_nodes));
+    PHAST_HIP(hipMalloc(&pl->cg, sizeof(DamageCG)));
+    PHAST_HIP(hipHostMalloc(&pl->h_cg, sizeof(DamageCG), hipHostMallocDefault));
+    return PHAST_OK;
+}
+
+template <int OP>
+static int t3_cg(const phast_t3_plan *pl, const T3Cg &A, hipStream_t st) {
+    hipLaunchKernelGGL(t3_cg_kernel<OP>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, A);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+template <int OP>
+static int t3_vec(const phast_t3_plan *pl, const T3Cg &A, hipStream_t st) {
+    hipLaunchKernelGGL(t3_vec_kernel<OP>, node_grid(pl), dim3(256), 0, st, pl->mesh.n_nodes, A);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+template <int OP>
+static int t3_scalar(const phast_t3_plan *pl, int check_every, hipStream_t st) {
+    hipLaunchKernelGGL(t3_scalar_kernel<OP>, dim3(1), dim3(256), 0, st, pl->cg, (const double *)pl->part, pl->nblk,
+                       check_every);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_damage_solve(phast_t3_plan *pl, int32_t method, const double *H, const double *Gc_e,
+                                     const double *d_prev, const double *d_init, const uint8_t *fixed,
+                                     const double *fixed_vals, double tol, int32_t max_iter, double *d_out,
+                                     uint8_t *active_out, int32_t *iters, int32_t *status, hipStream_t st) {
+    PHAST_REQUIRE(pl && H && d_prev && d_out, "phast_t3_damage_solve: null argument");
+    PHAST_REQUIRE(method == 0 || method == 1, "method must be 0 (projected) or 1 (post-clamp)");
+    PHAST_REQUIRE(!fixed == !fixed_vals, "fixed and fixed_vals must be given together");
+    PHAST_REQUIRE(tol >= 0.0 && max_iter >= 0, "tol and max_iter must be non-negative");
+    PHAST_REQUIRE(d_out != d_prev && d_out != d_init, "d_out must not alias d_prev or d_init");
+    T3_TRY(t3_ensure_work(pl));
+    DamageCG &h = *pl->h_cg;
+    std::memset(&h, 0, sizeof(h));
+    h.tol = tol;
+    h.max_iter = max_iter;
+    PHAST_HIP(hipMemcpyAsync(pl->cg, pl->h_cg, sizeof(DamageCG), hipMemcpyHostToDevice, st));
+
+    T3Cg A;
+    std::memset(&A, 0, sizeof(A));
+    A.H = H;
+    A.Gc = Gc_e;
+    A.d = d_out;
+    A.r = pl->r;
+    A.z = pl->z;
+    A.p = pl->p;
+    A.Ap = pl->Ap;
+    A.lb = pl->lb;
+    A.b = pl->b;
+    A.minv = pl->minv;
+    A.init = d_init;
+    A.dprev = d_prev;
+    A.vals = fixed_vals;
+    A.fixed = fixed;
+    A.active = pl->active;
+    A.active_out = active_out;
+    A.d_out = d_out;
+    A.cg = pl->cg;
+    A.part = pl->part;
+    A.nblk = pl->nblk;
+
+    // b and M_inv (damage_solver.py:2249-2265); d = init or d_prev, lb = d_prev (pins applied)
+    hipLaunchKernelGGL(t3_damage_kernel<2>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H, Gc_e,
+                       (const double *)nullptr, pl->b, pl->minv);
+    PHAST_LAUNCHED();
+    T3_TRY(t3_vec<TV_SETUP>(pl, A, st));
+    const int poll = 4;
+    int it = 0;
+    auto all_done = [&]() -> int {
+        PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(DamageCG), hipMemcpyDeviceToHost, st));
+        PHAST_HIP(hipStreamSynchronize(st));
+        return PHAST_OK;
+    };
+    if (method == 0) {  // damage_solver.py:3527-3775
+        T3_TRY(t3_cg<T_CG_INIT>(pl, A, st));
+        T3_TRY(t3_scalar<S_INIT>(pl, 10, st));
+        while (it < max_iter) {
+            T3_TRY(t3_cg<T_CG_AP>(pl, A, st));
+            T3_TRY(t3_scalar<S_A>(pl, 10, st));
+            T3_TRY(t3_vec<TV_UPDATE_D>(pl, A, st));
+            T3_TRY(t3_cg<T_CG_RESID>(pl, A, st));
+            T3_TRY(t3_scalar<S_B>(pl, 10, st));
+            T3_TRY(t3_vec<TV_PRECOND>(pl, A, st));
+            T3_TRY(t3_scalar<S_C>(pl, 10, st));
+            T3_TRY(t3_vec<TV_UPDATE_P>(pl, A, st));
+            ++it;
+            if (it % poll == 0 || it == max_iter) {
+                T3_TRY(all_done());
+                if (h.done) break;
+            }
+        }
+        T3_TRY(t3_vec<TV_FINISH>(pl, A, st));
+    } else {  // damage_solver.py:3377-3521
+        T3_TRY(t3_cg<T_PC_INIT>(pl, A, st));
+        T3_TRY(t3_scalar<S_PC_INIT>(pl, 50, st));
+        while (it < max_iter) {
+            T3_TRY(t3_cg<T_PC_AP>(pl, A, st));
+            T3_TRY(t3_scalar<S_PC_A>(pl, 50, st));
+            T3_TRY(t3_vec<TV_PC_UPDATE>(pl, A, st));
+            T3_TRY(t3_cg<T_PC_RESID>(pl, A, st));
+            T3_TRY(t3_vec<TV_PC_PRECOND>(pl, A, st));
+            T3_TRY(t3_scalar<S_PC_C>(pl, 50, st));
+            T3_TRY(t3_vec<TV_PC_UPDATE_P>(pl, A, st));
+            ++it;
+            if (it % poll == 0 || it == max_iter) {
+                T3_TRY(all_done());
+                if (h.done) break;
+            }
+        }
+        T3_TRY(t3_vec<TV_PC_FINISH>(pl, A, st));
+    }
+    T3_TRY(all_done());
+    if (iters) *iters = h.iter;
+    if (status) *status = h.done ? h.status : PHAST_CG_MAXITER;
     return PHAST_OK;
 }

@@ -203,7 +203,7 @@ class Q4Plan:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib.lib().phast_q4_plan_destroy(h)
             self.handle = None
 

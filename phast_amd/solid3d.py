@@ -273,7 +273,7 @@ class PeriodicVoxelPhaseFieldOperator:
 
     def __del__(self):
         h = getattr(self, "_plan", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _L.phast_hex8_plan_destroy(h)
             self._plan = None
 
@@ -881,7 +881,7 @@ class VoxelMultigrid:
 
     def __del__(self):
         h = getattr(self, "_mg", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _L.phast_hex8_mg_destroy(h)
             self._mg = None
 

@@ -1,8 +1,8 @@
 """Linear-triangle (T3) phase-field path on MI355X: host mirror of the reference interface.
 
 The classes keep the names, arguments and return conventions of the reference's T3 branch
-of ``FEMOperators`` and of ``ExplicitDynamics`` and the T3 ``_Ax`` pieces of
-``PhaseFieldDamageSolver``.  All arithmetic runs in ``libphast_mi.so``; this module uploads
+of ``FEMOperators``, of ``ExplicitDynamics`` and of ``PhaseFieldDamageSolver`` (element history,
+Jacobi-preconditioned projected / post-clamp CG).  All arithmetic runs in ``libphast_mi.so``; this module uploads
 the mesh once (the plan), allocates outputs with ``torch.empty`` on the fields' device and
 passes raw pointers.  There is no CPU fallback.
 
@@ -40,7 +40,7 @@ class T3Plan:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib.lib().phast_t3_plan_destroy(h)
             self.handle = None
 
@@ -75,6 +75,38 @@ class FEMMesh:
             raise ValueError("degenerate (zero-area) triangle in the mesh")
         per = np.linalg.norm(v01, axis=1) + np.linalg.norm(v12, axis=1) + np.linalg.norm(v02, axis=1)
         self.h_min = float((4.0 * self.areas_np / (per + 1e-30)).min())  # mesh.py:276-283
+        self.node_sets: dict[str, np.ndarray] = {}
+
+    @classmethod
+    def sent_split_quad(cls, W: float, H: float, nx: int, ny: int, a: float | None = None,
+                        device=None) -> "FEMMesh":
+        """Structured split-quad plate ``[0,W] x [0,H]`` of ``nx x ny`` cells, each split into
+        two triangles with the checkerboard diagonal mirrored about mid-height
+        (core/mesh_generator.py:1552-1633 ``rectangular_sent_liu_structured``: nodes
+        ``j*(nx+1)+i``, triangles cell by cell in row order).  Node sets ``bottom``, ``top``,
+        ``left``, ``right`` and, when ``a`` is given, the notch line ``crack`` (nodes with
+        ``x <= a`` on ``y = H/2``)."""
+        if nx < 2 or ny < 2:
+            raise ValueError("structured SENT mesh needs at least 2 cells in x/y")
+        x, y = np.linspace(0.0, W, nx + 1), np.linspace(0.0, H, ny + 1)
+        xx, yy = np.meshgrid(x, y, indexing="xy")
+        nodes = np.column_stack([xx.ravel(), yy.ravel()])
+        j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        n00 = j * (nx + 1) + i
+        n10, n01, n11 = n00 + 1, n00 + nx + 1, n00 + nx + 2
+        even = ((i + np.minimum(j, ny - 1 - j)) % 2 == 0)[..., None]
+        t0 = np.where(even, np.stack([n00, n10, n11], -1), np.stack([n00, n10, n01], -1))
+        t1 = np.where(even, np.stack([n00, n11, n01], -1), np.stack([n10, n11, n01], -1))
+        els = np.stack([t0, t1], axis=2).reshape(-1, 3)
+        mesh = cls(nodes, els, device=device)
+        ids = np.arange(len(nodes)).reshape(ny + 1, nx + 1)
+        mesh.node_sets = {"bottom": ids[0], "top": ids[-1], "left": ids[:, 0], "right": ids[:, -1]}
+        if a is not None:
+            jm, ic = int(round((H / 2.0) / (H / ny))), int(round(a / (W / nx)))
+            if abs(y[jm] - H / 2.0) > 1e-10 or abs(x[ic] - a) > 1e-10:  # mesh_generator.py:1647-1652
+                raise ValueError("H/2 and a must lie on the structured grid")
+            mesh.node_sets["crack"] = ids[jm, : ic + 1]
+        return mesh
 
 
 class FEMOperators:
@@ -203,34 +235,88 @@ class ExplicitDynamics:
         return self._launch(u, v, a, d, f_ext, bc_mask, bc_vals, H_old=H_old, want_psi=want_psi)
 
 
-class DamageOperator:
-    """The T3 AT2 damage system of ``PhaseFieldDamageSolver`` (element history, no viscosity):
-    ``apply`` is ``_Ax`` (damage_solver.py:1715-1755), ``rhs`` the AT2 right-hand side
-    (:2252-2257) and ``diagonal`` the Jacobi diagonal (:2112-2119)."""
+class PhaseFieldDamageSolver:
+    """AT2 damage solve of ``solvers/damage_solver.py:PhaseFieldDamageSolver`` on T3 meshes
+    with element history H (E,) and no viscosity (damage_solver.py:1157-1487, 2160-2268,
+    2493-2684, 3377-3775).
 
-    def __init__(self, fem: FEMOperators):
-        self.fem = fem
-        self.Gc_l0 = fem.material.Gc * fem.material.l0
-        self.Gc_over_l0 = fem.material.Gc / fem.material.l0
+    ``bounds_method`` 'projected_cg' runs the projected Jacobi-PCG on ``[d_prev, 1]``;
+    'post_clamp' (the reference default) runs unconstrained PCG and clamps.  The iteration
+    lives on the device; the host reads a convergence flag every few iterations.
+    ``Gc_field`` (E,) selects the per-element Gc branch.
+    """
 
-    def apply(self, d: torch.Tensor, H: torch.Tensor) -> torch.Tensor:
-        fem = self.fem
-        d, H = fem._d(d), fem._e(H)
-        y = torch.empty_like(d)
-        check(_lib.lib().phast_t3_damage_apply(fem.plan.handle, ptr(H), ptr(d), ptr(y), stream()), "damage_apply")
-        return y
+    def __init__(self, fem: FEMOperators, tol: float = 1e-5, max_iter: int = 5000, ctx=None,
+                 use_multigrid: bool = True, bounds_method: str = "post_clamp", preconditioner: str | None = None,
+                 nodal_H: bool = False, Gc_field: torch.Tensor | None = None):
+        if bounds_method not in ("post_clamp", "projected_cg", "direct"):
+            raise ValueError(f"bounds_method must be 'post_clamp', 'projected_cg', or 'direct', got '{bounds_method}'")
+        if bounds_method == "direct":
+            raise NotImplementedError("the T3 HIP damage path runs the matrix-free CG bounds methods "
+                                      "('post_clamp' and 'projected_cg')")
+        if preconditioner not in (None, "jacobi"):
+            raise NotImplementedError("the T3 HIP damage path uses the Jacobi preconditioner")
+        if nodal_H:
+            raise NotImplementedError("the T3 HIP damage path takes element history H (E,)")
+        self.fem, self.mesh, self.material = fem, fem.mesh, fem.material
+        self.tol, self.max_iter = float(tol), int(max_iter)
+        self._bounds_method = bounds_method
+        self.Gc_field = None if Gc_field is None else fem._e(Gc_field, "Gc_field")
+        self.last_iter = 0
+        self.last_converged = False
+        self.last_status = ""
+        self.last_active = None
+        self.n_solves = 0
 
-    def rhs_and_diagonal(self, H: torch.Tensor):
-        fem = self.fem
-        H = fem._e(H)
-        b = torch.empty(fem.mesh.n_nodes, dtype=F64, device=fem.device)
+    def damage_rhs_diag(self, H_input):
+        """``(b, diag)`` of the AT2 system at element history H (damage_solver.py:2112-2119, 2249-2258)."""
+        H = self.fem._e(H_input, "H_input")
+        b = torch.empty(self.mesh.n_nodes, dtype=F64, device=self.fem.device)
         dg = torch.empty_like(b)
-        check(_lib.lib().phast_t3_damage_rhs_diag(fem.plan.handle, ptr(H), ptr(b), ptr(dg), stream()),
-              "damage_rhs_diag")
+        check(_lib.lib().phast_t3_damage_rhs_diag(self.fem.plan.handle, ptr(H), ptr(self.Gc_field), ptr(b), ptr(dg),
+                                                  stream()), "damage_rhs_diag")
         return b, dg
 
-    def rhs(self, H: torch.Tensor) -> torch.Tensor:
-        return self.rhs_and_diagonal(H)[0]
+    def damage_apply(self, H_input, x):
+        """``A x`` of the AT2 operator at history H (``_Ax`` T3, damage_solver.py:1715-1755)."""
+        H = self.fem._e(H_input, "H_input")
+        x = self.fem._d(x, "x")
+        y = torch.empty_like(x)
+        check(_lib.lib().phast_t3_damage_apply(self.fem.plan.handle, ptr(H), ptr(self.Gc_field), ptr(x), ptr(y),
+                                               stream()), "damage_apply")
+        return y
 
-    def diagonal(self, H: torch.Tensor) -> torch.Tensor:
-        return self.rhs_and_diagonal(H)[1]
+    def solve(self, H_input, d_prev, Gc=None, l0=None, Gc_field=None, pf_dirichlet_mask=None,
+              pf_dirichlet_values=None, initial_guess=None):
+        """damage_solver.py:2493-2640 (forward, non-differentiable)."""
+        self.n_solves += 1
+        if Gc is not None or l0 is not None:
+            raise NotImplementedError("scalar Gc/l0 overrides: rebuild the operators with the new material")
+        if any(isinstance(t, torch.Tensor) and t.requires_grad for t in (H_input, d_prev, Gc_field)):
+            raise NotImplementedError("differentiable T3 damage solves are not on the HIP path")
+        fem = self.fem
+        d_prev = fem._d(d_prev, "d_prev")
+        if not bool(torch.isfinite(d_prev).all()):  # damage_solver.py:2647-2650
+            self.last_iter, self.last_converged = 0, False
+            return d_prev.clone()
+        H = fem._e(H_input, "H_input")
+        gcf = self.Gc_field if Gc_field is None else fem._e(Gc_field, "Gc_field")
+        fixed = vals = None
+        if pf_dirichlet_mask is not None and bool(pf_dirichlet_mask.any()):
+            if pf_dirichlet_values is None:
+                raise ValueError("pf_dirichlet_values must be supplied with mask")
+            fixed = fem._d(pf_dirichlet_mask.to(F64), "pf_dirichlet_mask").to(torch.uint8)
+            vals = fem._d(pf_dirichlet_values.to(F64), "pf_dirichlet_values")
+        init = None if initial_guess is None else fem._d(initial_guess.to(F64), "initial_guess")
+        d_out = torch.empty_like(d_prev)
+        method = 0 if self._bounds_method == "projected_cg" else 1
+        active = torch.empty(self.mesh.n_nodes, dtype=torch.uint8, device=fem.device) if method == 0 else None
+        iters, status = C.c_int32(), C.c_int32()
+        check(_lib.lib().phast_t3_damage_solve(
+            fem.plan.handle, method, ptr(H), ptr(gcf), ptr(d_prev), ptr(init), ptr(fixed), ptr(vals), self.tol,
+            self.max_iter, ptr(d_out), ptr(active), C.byref(iters), C.byref(status), stream()), "damage_solve")
+        self.last_iter = int(iters.value)
+        self.last_status = _lib.CG_STATUS_NAMES.get(int(status.value), str(status.value))
+        self.last_converged = int(status.value) == _lib.CG_CONVERGED
+        self.last_active = None if active is None else active.bool()
+        return d_out

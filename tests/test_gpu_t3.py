@@ -115,28 +115,30 @@ def test_explicit_step(dev, split, ps):
 def test_damage_operator(dev):
     nodes, els = scrambled_mesh(27, 21, seed=6)
     P, fem, ofem = setup(nodes, els)
-    dmg = P.DamageOperator(fem)
-    odmg = O.T3Damage(ofem.mesh, ofem.material)
     _, _, _, d, H = state(len(nodes), len(els), seed=7)
     H = H * 30.0
-    assert rel(dmg.apply(T(d, dev), T(H, dev)), odmg.apply(d, H)) < RTOL
-    b, dg = dmg.rhs_and_diagonal(T(H, dev))
-    assert rel(b, odmg.rhs(H)) < RTOL and rel(dg, odmg.diagonal(H)) < RTOL
+    Gc_e = np.random.default_rng(8).uniform(1e-3, 5e-3, len(els))
+    for gc in (None, Gc_e):
+        dmg = P.PhaseFieldDamageSolver(fem, Gc_field=None if gc is None else T(gc, dev))
+        odmg = O.T3Damage(ofem.mesh, ofem.material, gc)
+        assert rel(dmg.damage_apply(T(H, dev), T(d, dev)), odmg.apply(d, H)) < RTOL
+        b, dg = dmg.damage_rhs_diag(T(H, dev))
+        assert rel(b, odmg.rhs(H)) < RTOL and rel(dg, odmg.diagonal(H)) < RTOL
 
 
 def test_damage_operator_dense_small(dev):
     """Columns of the device operator on a tiny mesh form the symmetric textbook matrix."""
     nodes, els = scrambled_mesh(3, 2, seed=8)
     P, fem, ofem = setup(nodes, els)
-    dmg = P.DamageOperator(fem)
+    dmg = P.PhaseFieldDamageSolver(fem)
     H = np.random.default_rng(9).uniform(0, 1e-3, len(els))
     n = len(nodes)
-    cols = [dmg.apply(T(np.eye(n)[k], dev), T(H, dev)).cpu().numpy() for k in range(n)]
+    cols = [dmg.damage_apply(T(H, dev), T(np.eye(n)[k], dev)).cpu().numpy() for k in range(n)]
     D = np.stack(cols, axis=1)
     K = O.T3Damage(ofem.mesh, ofem.material).dense(H)
     assert np.abs(D - K).max() < 1e-13 * np.abs(K).max()
     assert np.abs(D - D.T).max() < 1e-14 * np.abs(D).max()
-    assert rel(dmg.diagonal(T(H, dev)), np.diag(K)) < 1e-13
+    assert rel(dmg.damage_rhs_diag(T(H, dev))[1], np.diag(K)) < 1e-13
 
 
 def test_single_triangle_and_unused_node(dev):
@@ -166,11 +168,66 @@ def test_large_mesh(dev):
     xs, ys = nodes[:, 0], nodes[:, 1]
     inner = (xs > 1e-9) & (xs < 20 - 1e-9) & (ys > 1e-9) & (ys < 10 - 1e-9)
     assert np.abs(f[inner]).max() < 1e-9 * np.abs(f).max()
-    dmg = P.DamageOperator(fem)
+    dmg = P.PhaseFieldDamageSolver(fem)
     Hc = 2e-4
     Hu = T(np.full(E, Hc), dev)
-    du = T(np.full(n, 2 * Hc / (3e-3 / 0.5 + 2 * Hc)), dev)
-    assert rel(dmg.apply(du, Hu), dmg.rhs(Hu)) < 1e-12
+    dc = 2 * Hc / (3e-3 / 0.5 + 2 * Hc)
+    assert rel(dmg.damage_apply(Hu, T(np.full(n, dc), dev)), dmg.damage_rhs_diag(Hu)[0]) < 1e-12
+
+
+def test_uniform_history_solve_closed_form(dev):
+    """Uniform H: the solve returns d = 2H / (Gc/l0 + 2H) at every node."""
+    nodes, els = scrambled_mesh(40, 30, seed=16)
+    P, fem, ofem = setup(nodes, els)
+    n, E = len(nodes), len(els)
+    Hc = 1.3 * 3e-3 / 0.5
+    dmg = P.PhaseFieldDamageSolver(fem, tol=1e-13, max_iter=1000, bounds_method="projected_cg")
+    d = dmg.solve(torch.full((E,), Hc, dtype=torch.float64, device=dev),
+                  torch.zeros(n, dtype=torch.float64, device=dev))
+    assert dmg.last_status == "converged"
+    assert rel(d, np.full(n, 2 * Hc / (3e-3 / 0.5 + 2 * Hc))) < 1e-10
+
+
+@pytest.mark.parametrize("method", ["projected_cg", "post_clamp"])
+def test_damage_solve(dev, method):
+    nodes, els = scrambled_mesh(33, 21, seed=12)
+    P, fem, ofem = setup(nodes, els)
+    n, E = len(nodes), len(els)
+    rng = np.random.default_rng(13)
+    H = rng.uniform(0, 0.05, E) * (rng.uniform(0, 1, E) > 0.7)
+    d_prev = np.clip(rng.uniform(-0.5, 0.8, n), 0, 1)
+    dmg = P.PhaseFieldDamageSolver(fem, tol=1e-8, max_iter=3000, bounds_method=method)
+    odmg = O.T3DamageSolver(ofem.mesh, ofem.material, tol=1e-8, max_iter=3000, bounds_method=method)
+    d = dmg.solve(T(H, dev), T(d_prev, dev))
+    od = odmg.solve(H, d_prev)
+    assert dmg.last_iter == odmg.last_iter, (dmg.last_iter, odmg.last_iter, dmg.last_status, odmg.last_status)
+    assert dmg.last_status == odmg.last_status
+    assert rel(d, od) < RTOL
+    dc = d.cpu().numpy()
+    assert np.all(dc >= d_prev) and np.all(dc <= 1.0)  # bound clamps exact
+    if method == "projected_cg":
+        assert np.array_equal(dmg.last_active.cpu().numpy(), odmg.last_active)
+
+
+def test_damage_solve_gc_field_pins_and_initial_guess(dev):
+    nodes, els = scrambled_mesh(21, 14, seed=14)
+    P, fem, ofem = setup(nodes, els)
+    n, E = len(nodes), len(els)
+    rng = np.random.default_rng(15)
+    H = rng.uniform(0, 0.02, E)
+    d_prev = rng.uniform(0, 0.3, n)
+    Gc_e = rng.uniform(1e-3, 6e-3, E)
+    fixed = nodes[:, 0] < 1e-9
+    vals = np.where(fixed, 1.0, 0.0)
+    init = np.clip(d_prev + 0.1, 0, 1)
+    dmg = P.PhaseFieldDamageSolver(fem, tol=1e-12, max_iter=2000, bounds_method="projected_cg",
+                                   Gc_field=T(Gc_e, dev))
+    d = dmg.solve(T(H, dev), T(d_prev, dev), pf_dirichlet_mask=torch.as_tensor(fixed, device=dev),
+                  pf_dirichlet_values=T(vals, dev), initial_guess=T(init, dev))
+    od = O.T3DamageSolver(ofem.mesh, ofem.material, tol=1e-12, max_iter=2000, Gc_e=Gc_e).solve(
+        H, d_prev, initial_guess=init, fixed=fixed, vals=vals)
+    assert rel(d, od) < RTOL
+    assert np.all(d.cpu().numpy()[fixed] == 1.0)
 
 
 def test_plan_rejects_bad_connectivity(dev):

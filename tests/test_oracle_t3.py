@@ -154,3 +154,66 @@ def test_damage_uniform_history_closed_form():
     H = np.full(mm.n_elems, Hc)
     d = np.full(mm.n_nodes, 2 * Hc / (matl.Gc / matl.l0 + 2 * Hc))
     np.testing.assert_allclose(dm.apply(d, H), dm.rhs(H), rtol=1e-12)
+
+
+def dense_damage_gc(mm, matl, H, Gc_e):
+    n = mm.n_nodes
+    K = np.zeros((n, n))
+    Mc = np.array([[2.0, 1.0, 1.0], [1.0, 2.0, 1.0], [1.0, 1.0, 2.0]]) / 12.0
+    for e, nodes in enumerate(mm.elements):
+        gp = mm.grad_phi[e]
+        ke = Gc_e[e] * matl.l0 * mm.areas[e] * gp @ gp.T + (2.0 * H[e] + Gc_e[e] / matl.l0) * mm.areas[e] * Mc
+        K[np.ix_(nodes, nodes)] += ke
+    return K
+
+
+def test_damage_per_element_gc_dense_and_uniform():
+    mm = mesh(4, 3)
+    matl = mat()
+    rng = np.random.default_rng(4)
+    H = rng.uniform(0, 1e-3, mm.n_elems)
+    Gc_e = rng.uniform(1e-3, 5e-3, mm.n_elems)
+    dm = O.T3Damage(mm, matl, Gc_e)
+    K = dense_damage_gc(mm, matl, H, Gc_e)
+    assert np.abs(dm.dense(H) - K).max() < 1e-13 * np.abs(K).max()
+    np.testing.assert_allclose(dm.diagonal(H), np.diag(K), rtol=1e-13)
+    flat = O.T3Damage(mm, matl, np.full(mm.n_elems, matl.Gc))
+    d = rng.uniform(0, 1, mm.n_nodes)
+    np.testing.assert_allclose(flat.apply(d, H), O.T3Damage(mm, matl).apply(d, H), rtol=1e-14)
+
+
+def test_damage_solve_unconstrained_matches_dense_solve():
+    """With d_prev = 0 and moderate H the bounds stay inactive: both CG variants reach the
+    solution of the assembled linear system."""
+    mm = mesh(6, 5)
+    matl = mat()
+    H = np.random.default_rng(5).uniform(0, 2e-3, mm.n_elems)
+    x = np.linalg.solve(dense_damage(mm, matl, H), O.T3Damage(mm, matl).rhs(H))
+    assert 0 < x.min() and x.max() < 1
+    for method in ("projected_cg", "post_clamp"):
+        s = O.T3DamageSolver(O.T3Mesh(mm.nodes, mm.elements), matl, tol=1e-12, bounds_method=method)
+        d = s.solve(H, np.zeros(mm.n_nodes))
+        assert s.last_status == "converged"
+        np.testing.assert_allclose(d, x, rtol=1e-9, atol=1e-12)
+
+
+def test_projected_damage_solve_is_the_bound_constrained_minimiser():
+    """Brute force on a small mesh: L-BFGS-B on 1/2 d.K d - b.d over lb <= d <= 1."""
+    from scipy.optimize import minimize
+    mm = mesh(5, 4)
+    matl = mat()
+    rng = np.random.default_rng(6)
+    H = rng.uniform(0, 2e-3, mm.n_elems)
+    lb = rng.uniform(0, 0.6, mm.n_nodes)
+    lb[rng.permutation(mm.n_nodes)[:3]] = 1.0  # fully damaged nodes sit on both bounds
+    K = dense_damage(mm, matl, H)
+    b = O.T3Damage(mm, matl).rhs(H)
+    res = minimize(lambda z: 0.5 * z @ K @ z - b @ z, lb.copy(), jac=lambda z: K @ z - b,
+                   bounds=list(zip(lb, np.ones_like(lb))), method="L-BFGS-B",
+                   options=dict(ftol=1e-15, gtol=1e-13, maxiter=20000))
+    s = O.T3DamageSolver(O.T3Mesh(mm.nodes, mm.elements), matl, tol=1e-12)
+    d = s.solve(H, lb)
+    assert s.last_status == "converged"
+    assert (d >= lb).all() and (d <= 1.0).all()
+    assert np.isclose(d, lb, rtol=0, atol=1e-12)[lb < 1].any() and (d > lb + 1e-3).any()
+    np.testing.assert_allclose(d, res.x, atol=1e-6)
