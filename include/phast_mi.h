@@ -206,6 +206,10 @@ int phast_t3_explicit_step(const phast_t3_plan *plan, double dt, const double *u
                            const double *d, const double *f_ext, const double *Minv, const uint8_t *bc_mask,
                            const double *bc_vals, const double *H_old, double *u1, double *v1, double *a1,
                            double *H_new, double *psi, hipStream_t stream);
+/* Area-weighted projection of an element field f (E,) to the nodes (core/mesh.py:407-433):
+ * out_n = sum_e f_e A_e / (3 M_n + 1e-30), M = M_scalar (device (N,)). */
+int phast_t3_elem_to_node(const phast_t3_plan *plan, const double *field_e, const double *M, double *out,
+                          hipStream_t stream);
 /* AT2 damage operator of element history H (E,) (solvers/damage_solver.py:1715-1755 _Ax):
  *   y = Gc l0 sum_e A (grad phi_a . grad d_e) + sum_e rc_e (d_a + sum_b d_b),
  *   rc = (2H + Gc/l0) A / 12  (damage_solver.py:2168-2177);  x and y must differ.

@@ -50,6 +50,11 @@ class T3Mesh:
         self.elem_flat = self.elements.reshape(-1)
         self.M_scalar = scatter(self.elem_flat, np.repeat(self.areas / 3.0, 3), self.n_nodes)
 
+    def elem_to_node(self, field_e: np.ndarray) -> np.ndarray:
+        """Area-weighted projection of an element field (core/mesh.py:430-433)."""
+        ns = scatter(self.elem_flat, np.repeat(field_e * self.areas, 3), self.n_nodes)
+        return ns / (3.0 * self.M_scalar + 1e-30)
+
 
 def structured_t3(nx: int, ny: int, W: float, H: float, jitter: float = 0.0, seed: int = 0):
     """A rectangle of nx x ny cells, each split into two triangles along alternating
@@ -174,6 +179,10 @@ class T3Operators:
             trp = np.maximum(trace, 0.0)
             return 0.5 * lam * trp ** 2 + mu * (e1p ** 2 + e2p ** 2)
         raise ValueError(m.energy_split)
+
+    def compute_driving_force(self, u, strain=None, d=None):
+        """'strain_energy' driving force == psi+ (fem_operators.py:1173-1204)."""
+        return self.compute_psi_plus(u, strain)
 
 
 class T3Damage:

@@ -262,6 +262,21 @@ __global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3
     }
 }
 
+// Area-weighted projection of an element field (core/mesh.py:430-433):
+// out_n = sum_e f_e A_e / (3 M_n + 1e-30).
+__global__ __launch_bounds__(256) void t3_e2n_kernel(const T3Mesh m, const double *fe, const double *Mn, double *out) {
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n >= m.n_nodes) return;
+    double s = 0.0;
+    for (int32_t k = m.ptr[n]; k < m.ptr[n + 1]; ++k) {
+        const int64_t e = m.inc[k] / 3;
+        const double2 p0 = m.xy[m.el[3 * e]], p1 = m.xy[m.el[3 * e + 1]], p2 = m.xy[m.el[3 * e + 2]];
+        const double s2a = (p1.x - p0.x) * (p2.y - p0.y) - (p1.y - p0.y) * (p2.x - p0.x);
+        s += fe[e] * (0.5 * fabs(s2a));
+    }
+    out[n] = s / (3.0 * Mn[n] + 1e-30);
+}
+
 // ---------------------------------------------------------------------------------------
 // device-resident projected / post-clamp Jacobi CG on the T3 operator (the Q4 solve's state
 // machine and scalar steps; one thread per node, per-block partial sums in part[k*nblk+b])
@@ -607,6 +622,14 @@ extern "C" int phast_t3_explicit_step(const phast_t3_plan *pl, double dt, const 
     hipLaunchKernelGGL(t3_force_kernel<2>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, (const double2 *)u1, d,
                        (double2 *)nullptr, H_old, H_new, psi, dt, (const double2 *)v, (const double2 *)a,
                        (const double2 *)f_ext, Minv, bc_mask, (double2 *)v1, (double2 *)a1);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_elem_to_node(const phast_t3_plan *pl, const double *field_e, const double *M, double *out,
+                                     hipStream_t st) {
+    PHAST_REQUIRE(pl && field_e && M && out, "phast_t3_elem_to_node: null argument");
+    hipLaunchKernelGGL(t3_e2n_kernel, node_grid(pl), dim3(256), 0, st, pl->mesh, field_e, M, out);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }

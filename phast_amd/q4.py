@@ -488,9 +488,11 @@ class PhaseFieldDamageSolver:
 class StaggeredSolver:
     """Explicit branch of ``solvers/staggered_solver.py:StaggeredSolver``.
 
-    State: ``u, v, a (N,2)``, ``d (N,)``, ``H_elem (E,4)``, ``H_nodal (N,)`` (lazy).
+    State: ``u, v, a (N,2)``, ``d (N,)``, ``H_elem`` ((E,4) Gauss-point history on Q4
+    plates, (E,) element history on T3 meshes), ``H_nodal (N,)`` (lazy).
     ``step_full`` = mechanics step fused with psi+ and the history update, then the
-    damage solve gated by ``damage_every`` (staggered_solver.py:1115-1165).
+    damage solve gated by ``damage_every`` (staggered_solver.py:1115-1165).  The same class
+    drives the Q4 and the T3 operators (``phast_amd.t3``).
     """
 
     def __init__(self, fem: FEMOperators, dyn: ExplicitDynamics, damage: PhaseFieldDamageSolver,
@@ -501,7 +503,8 @@ class StaggeredSolver:
         self.v = torch.zeros_like(self.u)
         self.a = torch.zeros_like(self.u)
         self.d = torch.zeros(n, dtype=F64, device=dev)
-        self.H_elem = torch.zeros((E, 4), dtype=F64, device=dev)
+        hist = (E, 4) if getattr(fem.mesh, "element_type", "Q4") == "Q4" else (E,)
+        self.H_elem = torch.zeros(hist, dtype=F64, device=dev)
         self.f_ext = torch.zeros_like(self.u) if f_ext is None else f_ext
         self.bc_mask, self.bc_vals = bc_mask, bc_vals
         self.damage_every = int(damage_every)

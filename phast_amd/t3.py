@@ -19,7 +19,7 @@ import torch
 
 from . import _lib
 from ._lib import check, ptr, stream
-from .q4 import F64, Material, _dev_check, history_update  # noqa: F401  (history_update re-exported)
+from .q4 import F64, Material, StaggeredSolver, _dev_check, history_update  # noqa: F401  (re-exported)
 
 _SPLIT_ID = {"amor": 0, "isotropic": 1, "spectral": 2}
 
@@ -166,6 +166,14 @@ class FEMOperators:
     def compute_driving_force(self, u: torch.Tensor, strain=None, d=None) -> torch.Tensor:
         """'strain_energy' driving force == psi+ (fem_operators.py:1173-1204)."""
         return self.compute_psi_plus(u, strain=strain, d=d)
+
+    def elem_to_node(self, field_e: torch.Tensor) -> torch.Tensor:
+        """Area-weighted element -> node projection (core/mesh.py:407-433)."""
+        field_e = self._e(field_e.contiguous(), "field")
+        out = torch.empty(self.mesh.n_nodes, dtype=F64, device=self.device)
+        check(_lib.lib().phast_t3_elem_to_node(self.plan.handle, ptr(field_e), ptr(self.M_scalar), ptr(out),
+                                               stream()), "elem_to_node")
+        return out
 
     def compute_reaction_force(self, u, d, node_indices, component: int = 1, f_ext=None) -> float:
         """fem_operators.py:1502-1530."""

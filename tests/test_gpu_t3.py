@@ -230,6 +230,47 @@ def test_damage_solve_gc_field_pins_and_initial_guess(dev):
     assert np.all(d.cpu().numpy()[fixed] == 1.0)
 
 
+def test_elem_to_node(dev):
+    nodes, els = scrambled_mesh(17, 13, seed=17)
+    P, fem, ofem = setup(nodes, els)
+    f = np.random.default_rng(18).uniform(0, 1, len(els))
+    assert rel(fem.elem_to_node(T(f, dev)), ofem.mesh.elem_to_node(f)) < RTOL
+    np.testing.assert_allclose(fem.elem_to_node(T(np.full(len(els), 2.5), dev)).cpu().numpy(), 2.5, rtol=1e-13)
+
+
+def test_explicit_staggered_steps_track_oracle(dev):
+    """30 explicit steps of the staggered solver (mechanics + history + projected damage solve)
+    on a notched split-quad plate pulled at top and bottom."""
+    from oracle import q4 as OQ
+    P3 = __import__("phast_amd.t3", fromlist=["t3"])
+    mesh = P3.FEMMesh.sent_split_quad(10.0, 4.0, 40, 16, a=5.0)
+    kw = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5)
+    fem = P3.FEMOperators(mesh, P3.Material(**kw))
+    ofem = O.T3Operators(O.T3Mesh(mesh.nodes, mesh.elements), OMaterial(**kw))
+    n, E = mesh.n_nodes, mesh.n_elems
+    f_ext = np.zeros((n, 2))
+    f_ext[mesh.node_sets["top"], 1] = 40.0
+    f_ext[mesh.node_sets["bottom"], 1] = -40.0
+    dt = 0.5 * fem.dt_cfl
+    dyn = P3.ExplicitDynamics(fem, dt=dt)
+    dmg = P3.PhaseFieldDamageSolver(fem, tol=1e-12, max_iter=2000, bounds_method="projected_cg")
+    st = P3.StaggeredSolver(fem, dyn, dmg, f_ext=T(f_ext, dev))
+    d0 = np.zeros(n)
+    d0[mesh.node_sets["crack"]] = 1.0  # preseeded notch
+    st.d = T(d0, dev)
+    ost = OQ.ExplicitState(u=np.zeros((n, 2)), v=np.zeros((n, 2)), a=np.zeros((n, 2)), d=d0.copy(), H=np.zeros(E))
+    odyn = OQ.ExplicitDynamics(ofem, dt=dt)
+    odmg = O.T3DamageSolver(ofem.mesh, ofem.material, tol=1e-12, max_iter=2000)
+    for k in range(30):
+        psi = st.step_full()
+        opsi = OQ.step_full_explicit(odyn, odmg, ost, f_ext)
+    assert rel(psi, opsi) < 1e-9
+    assert rel(st.u, ost.u) < 1e-9 and rel(st.v, ost.v) < 1e-9
+    assert rel(st.d, ost.d) < 1e-9 and rel(st.H_elem, ost.H) < 1e-9
+    assert float(st.d[torch.as_tensor(d0 == 0, device=dev)].max()) > 0.0
+    assert rel(st.H_nodal, ofem.mesh.elem_to_node(ost.H)) < 1e-9
+
+
 def test_plan_rejects_bad_connectivity(dev):
     from phast_amd import t3 as P
     from phast_amd._lib import PhastLibraryError
