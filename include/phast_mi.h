@@ -187,6 +187,11 @@ int phast_t3_plan_destroy(phast_t3_plan *plan);
  * (mesh.py:276-283) and the total area.  Any pointer may be null. */
 int phast_t3_mesh_info(const phast_t3_plan *plan, double *M_scalar, double *h_min, double *total_area);
 
+/* Organisation of the T3 passes: path 0 = element patches (each triangle evaluated once per
+ * patch in LDS; the default when the mesh's patch lists fit the LDS budget), 1 = every
+ * incidence re-evaluates its triangle.  Both give the same bits.  active_path / max_list
+ * (optional) return the organisation in use and the longest patch triangle list. */
+int phast_t3_set_path(phast_t3_plan *plan, int32_t path, int32_t *active_path, int32_t *max_list);
 /* f (N,2) = internal force of u (N,2) at nodal damage d (N,): centroid strain
  * (fem_operators.py:223-242), element-mean degradation (:626-630), the split
  * (:277-290 isotropic, :301-333 amor, :335-390 spectral) and the scatter (:661-679). */

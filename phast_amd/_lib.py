@@ -68,6 +68,7 @@ _SIGNATURES = {
     "phast_t3_plan_create": ([_I64, _I64, _P, _P, _D, _D, _D, _D, _D, _I32, _I32, C.POINTER(_P)], C.c_int),
     "phast_t3_plan_destroy": ([_P], C.c_int),
     "phast_t3_mesh_info": ([_P, _P, _P, _P], C.c_int),
+    "phast_t3_set_path": ([_P, _I32, C.POINTER(_I32), C.POINTER(_I32)], C.c_int),
     "phast_t3_internal_force": ([_P, _P, _P, _P, _P], C.c_int),
     "phast_t3_force_history": ([_P] * 8, C.c_int),
     "phast_t3_explicit_step": ([_P, _D] + [_P] * 15, C.c_int),

@@ -6,19 +6,34 @@
 // increasing e*3 + a -- the order in which the reference's scatter_add over elem_flat visits
 // the node (core/fem_operators.py:661-679 with _elem_flat = elements.flatten()).
 //
-// Every pass is node-centric: one thread per node walks its incident corners, gathers the 3
-// nodes of each incident element (neighbours are mostly cache hits for a row-ordered mesh),
-// recomputes that element's geometry, strain and stress, and sums the corner's
-// contribution.  No element buffer round trip and no atomics, so results are deterministic.
-// Element-wise outputs (psi+, history) are written by the thread of the element's corner 0.
+// Every pass is a gather: one thread per node sums its corners' contributions in that order,
+// so there are no atomics and results are deterministic.  Two organisations produce the same
+// bits:
+//  * patches (default): the nodes are grouped into patches of 256 in the Morton order of
+//    their coordinates; a block first evaluates each triangle touching its patch once
+//    (geometry from the corner coordinates, strain, degradation, the split's stress) into
+//    LDS, then each thread sums its node's corners from LDS;
+//  * recompute (meshes whose patch lists exceed the LDS budget, or on request): each
+//    incidence re-evaluates its triangle.
+// Element-wise outputs (psi+, history) are written once per triangle: by the patch that owns
+// its corner 0, or by the thread of corner 0.
 #include "common.h"
 #include "damage_cg.h"
 
+#include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "../../include/phast_mi.h"
+
+// No FMA contraction in this file: every product and sum rounds on its own, as in the
+// reference's tensor expressions, and the two pass organisations (which the compiler would
+// otherwise contract differently) give the same bits.
+#pragma clang fp contract(off)
 
 namespace phast {
 
@@ -111,7 +126,7 @@ __device__ __forceinline__ double t3_psi_plus(const T3Mat &M, double exx, double
     return 0.5 * lam * (trp * trp) + mu * (e1p * e1p + e2p * e2p);
 }
 
-// One element seen from one of its corners: geometry, strain, degradation.
+// One element: geometry, strain, degradation.
 struct T3Elem {
     int32_t n[3];
     double area, gp[3][2], exx, eyy, gxy, g;
@@ -136,38 +151,122 @@ __device__ __forceinline__ void t3_load(const T3Mesh &m, const T3Mat &M, int64_t
     }
 }
 
-// MODE 0: f = internal force; 1: + psi+ and H = max(H_old, psi+) per element;
-// 2: explicit-step corrector on top of 1 (u is the predicted u1).
-template <int MODE>
-__global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3Mat M, const double2 *u, const double *d,
-                                                       double2 *f, const double *H_old, double *H_new, double *psi,
-                                                       double dt, const double2 *v, const double2 *a,
-                                                       const double2 *fext, const double *minv, const uint8_t *bcm,
-                                                       double2 *v1, double2 *a1) {
-    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (n >= m.n_nodes) return;
-    double fx = 0.0, fy = 0.0;
-    for (int32_t k = m.ptr[n]; k < m.ptr[n + 1]; ++k) {
-        const int32_t ic = m.inc[k];
-        const int64_t e = ic / 3;
-        const int a_ = ic - 3 * (int)e;
-        T3Elem E;
-        t3_load(m, M, e, u, d, E);
-        double sxx, syy, sxy;
-        t3_stress(M, E.exx, E.eyy, E.gxy, E.g, sxx, syy, sxy);
-        // fem_operators.py:669-674: A (s . grad phi_a); the corner's row picked by selects, not
-        // a dynamic index (which would put gp in scratch memory)
-        const double gx = a_ == 0 ? E.gp[0][0] : (a_ == 1 ? E.gp[1][0] : E.gp[2][0]);
-        const double gy = a_ == 0 ? E.gp[0][1] : (a_ == 1 ? E.gp[1][1] : E.gp[2][1]);
-        fx += E.area * (sxx * gx + sxy * gy);
-        fy += E.area * (sxy * gx + syy * gy);
-        if (MODE >= 1 && a_ == 0) {
-            const double p = t3_psi_plus(M, E.exx, E.eyy, E.gxy);
-            if (psi) psi[e] = p;
-            const double h = H_old[e];
-            H_new[e] = h >= p ? h : p;  // max returns one operand bit-exactly
+// Per-node incidence walk over the element patches: the plan groups the nodes into patches
+// of 256 (a Morton order of the coordinates, so a patch is a compact piece of the mesh), and
+// lists for each patch the triangles touching it.  A block evaluates each listed triangle
+// once into LDS (2 values per corner), then each thread sums its node's corners from LDS in
+// the node's e*3+a order.  Without patches every incidence re-evaluates its triangle.
+struct T3Patch {
+    const int32_t *pnode;    // (N) node of each patch slot, patch b = slots [256 b, 256 b + 256)
+    const int32_t *pel_ptr;  // (npatch + 1)
+    const int32_t *pel;      // triangles of each patch, ascending; sign bit = the patch owns corner 0
+    const uint16_t *loc;     // (3E) per CSR entry: LDS slot (list position * 3 + corner)
+};
+
+// Two values per corner of one triangle, as named scalars: selecting a corner from an array
+// with a run-time index would put the array in scratch memory.
+struct Corners {
+    double c00, c01, c10, c11, c20, c21;
+    __device__ __forceinline__ void set(int a, double v0, double v1) {
+        if (a == 0) {
+            c00 = v0;
+            c01 = v1;
+        } else if (a == 1) {
+            c10 = v0;
+            c11 = v1;
+        } else {
+            c20 = v0;
+            c21 = v1;
         }
     }
+};
+
+// Fill the block's LDS with the corner values of its patch's triangles; corners(e, c, owner)
+// fills the Corners of triangle e.  Every thread of the block must call it.
+template <class F>
+__device__ __forceinline__ void t3_patch_fill(const T3Patch &P, double *lds, F corners) {
+    const int32_t j0 = P.pel_ptr[blockIdx.x], nel = P.pel_ptr[blockIdx.x + 1] - j0;
+    for (int32_t j = threadIdx.x; j < nel; j += 256) {
+        const int32_t w = P.pel[j0 + j];
+        Corners c;
+        corners((int64_t)(w & 0x7fffffff), c, w < 0);
+        lds[6 * j] = c.c00;
+        lds[6 * j + 1] = c.c01;
+        lds[6 * j + 2] = c.c10;
+        lds[6 * j + 3] = c.c11;
+        lds[6 * j + 4] = c.c20;
+        lds[6 * j + 5] = c.c21;
+    }
+    __syncthreads();
+}
+
+// Sums over node n's incidences, in e*3+a order, of the two corner values.  __dadd_rn keeps
+// each addition a separate rounding in both organisations, so they agree bitwise.
+template <bool PATCH, class F>
+__device__ __forceinline__ void t3_node_sum(const T3Mesh &m, const T3Patch &P, const double *lds, int64_t n, F corners,
+                                            double &s0, double &s1) {
+    s0 = 0.0;
+    s1 = 0.0;
+    for (int32_t k = m.ptr[n]; k < m.ptr[n + 1]; ++k) {
+        double v0, v1;
+        if (PATCH) {
+            const int sl = P.loc[k];
+            v0 = lds[2 * sl];
+            v1 = lds[2 * sl + 1];
+        } else {
+            const int32_t ic = m.inc[k];
+            const int64_t e = ic / 3;
+            const int a_ = ic - 3 * (int)e;
+            Corners c;
+            corners(e, c, a_ == 0);
+            v0 = a_ == 0 ? c.c00 : (a_ == 1 ? c.c10 : c.c20);
+            v1 = a_ == 0 ? c.c01 : (a_ == 1 ? c.c11 : c.c21);
+        }
+        s0 = __dadd_rn(s0, v0);
+        s1 = __dadd_rn(s1, v1);
+    }
+}
+
+// Corner forces A (s . grad phi_a) of one triangle (fem_operators.py:661-679); with HIST the
+// owner also writes psi+ and H = max(H_old, psi+) (fem_operators.py:951-1030,
+// staggered_solver.py:804-845).
+template <bool HIST>
+__device__ __forceinline__ void t3_force_corners(const T3Mesh &m, const T3Mat &M, const double2 *u, const double *d,
+                                                 const double *H_old, double *H_new, double *psi, int64_t e,
+                                                 Corners &c, bool owner) {
+    T3Elem E;
+    t3_load(m, M, e, u, d, E);
+    double sxx, syy, sxy;
+    t3_stress(M, E.exx, E.eyy, E.gxy, E.g, sxx, syy, sxy);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        c.set(a, E.area * (sxx * E.gp[a][0] + sxy * E.gp[a][1]), E.area * (sxy * E.gp[a][0] + syy * E.gp[a][1]));
+    if (HIST && owner) {
+        const double p = t3_psi_plus(M, E.exx, E.eyy, E.gxy);
+        if (psi) psi[e] = p;
+        const double h = H_old[e];
+        H_new[e] = h >= p ? h : p;  // max returns one operand bit-exactly
+    }
+}
+
+// MODE 0: f = internal force; 1: + psi+ and H = max(H_old, psi+) per element;
+// 2: explicit-step corrector on top of 1 (u is the predicted u1).
+template <int MODE, bool PATCH>
+__global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3Patch P, const T3Mat M, const double2 *u,
+                                                       const double *d, double2 *f, const double *H_old, double *H_new,
+                                                       double *psi, double dt, const double2 *v, const double2 *a,
+                                                       const double2 *fext, const double *minv, const uint8_t *bcm,
+                                                       double2 *v1, double2 *a1) {
+    extern __shared__ double lds[];
+    auto corners = [&](int64_t e, Corners &c, bool owner) {
+        t3_force_corners<(MODE >= 1)>(m, M, u, d, H_old, H_new, psi, e, c, owner);
+    };
+    if (PATCH) t3_patch_fill(P, lds, corners);
+    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (slot >= m.n_nodes) return;
+    const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
+    double fx, fy;
+    t3_node_sum<PATCH>(m, P, lds, n, corners, fx, fy);
     if (MODE < 2) {
         f[n] = make_double2(fx, fy);
         return;
@@ -202,63 +301,60 @@ __global__ __launch_bounds__(256) void t3_predictor_kernel(int64_t nn, double dt
     u1[n] = make_double2(ux, uy);
 }
 
-// Damage operator of the element-H AT2 path (solvers/damage_solver.py), row n:
-// OP 0: returns (A x)_n = Gc l0 sum_e A (grad phi_a . grad x_e) + sum_e rc_e (x_a + sum_b x_b),
-//       rc = (2H + Gc/l0) A/12 (the two scatters of :1719-1755 summed separately, then
-//       combined; with a per-element Gc field the Gc l0 weight goes inside the first sum);
-// OP 1: b_n = sum_e 2H A/3 (:2249-2258) into s0 and diag_n = sum_e (Gc l0 A |grad phi_a|^2
-//       + 2 rc) (:2112-2119) into s1.
+// Damage operator of the element-H AT2 path (solvers/damage_solver.py), corner values of
+// one triangle:
+// OP 0: c[a] = (A grad phi_a . grad x_e [x Gc_e l0], rc (x_a + sum_b x_b)),
+//       rc = (2H + Gc/l0) A/12: the two scatters of :1719-1755, summed separately and
+//       combined as Gc l0 s0 + s1 (s0 + s1 with a per-element Gc field);
+// OP 1: c[a] = (2H A/3 (:2249-2258), Gc l0 A |grad phi_a|^2 + 2 rc (:2112-2119)).
 template <int OP>
-__device__ __forceinline__ double t3_damage_row(const T3Mesh &m, const T3Mat &M, const double *H, const double *Gc,
-                                                const double *x, int64_t n, double *s1_out = nullptr) {
-    double s0 = 0.0, s1 = 0.0;
-    for (int32_t k = m.ptr[n]; k < m.ptr[n + 1]; ++k) {
-        const int32_t ic = m.inc[k];
-        const int64_t e = ic / 3;
-        const int a_ = ic - 3 * (int)e;
-        int32_t nd[3];
+__device__ __forceinline__ void t3_damage_corners(const T3Mesh &m, const T3Mat &M, const double *H, const double *Gc,
+                                                  const double *x, int64_t e, Corners &c) {
+    int32_t nd[3];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) nd[q] = m.el[3 * e + q];
-        double area, gp[3][2];
-        t3_geometry(m.xy[nd[0]], m.xy[nd[1]], m.xy[nd[2]], area, gp);
-        const double h = H[e];
-        const double ge = Gc ? Gc[e] : 0.0;
-        const double ratio = Gc ? ge / M.l0 : M.Gc_over_l0;  // damage_solver.py:2168-2177
-        const double rc = (2.0 * h + ratio) * area / 12.0;
-        const double gx = a_ == 0 ? gp[0][0] : (a_ == 1 ? gp[1][0] : gp[2][0]);
-        const double gy = a_ == 0 ? gp[0][1] : (a_ == 1 ? gp[1][1] : gp[2][1]);
-        if (OP == 0) {
-            const double d0 = x[nd[0]], d1 = x[nd[1]], d2 = x[nd[2]];
-            const double gdx = (gp[0][0] * d0 + gp[1][0] * d1) + gp[2][0] * d2;
-            const double gdy = (gp[0][1] * d0 + gp[1][1] * d1) + gp[2][1] * d2;
-            const double lap = area * (gx * gdx + gy * gdy);
-            s0 += Gc ? (ge * M.l0) * lap : lap;
-            const double da = a_ == 0 ? d0 : (a_ == 1 ? d1 : d2);
-            s1 += rc * (da + ((d0 + d1) + d2));
-        } else {
-            s0 += 2.0 * h * (area / 3.0);
-            s1 += (Gc ? ge * M.l0 : M.Gc_l0) * (area * (gx * gx + gy * gy)) + 2.0 * rc;
+    for (int q = 0; q < 3; ++q) nd[q] = m.el[3 * e + q];
+    double area, gp[3][2];
+    t3_geometry(m.xy[nd[0]], m.xy[nd[1]], m.xy[nd[2]], area, gp);
+    const double h = H[e];
+    const double ge = Gc ? Gc[e] : 0.0;
+    const double ratio = Gc ? ge / M.l0 : M.Gc_over_l0;  // damage_solver.py:2168-2177
+    const double rc = (2.0 * h + ratio) * area / 12.0;
+    if (OP == 0) {
+        const double d0 = x[nd[0]], d1 = x[nd[1]], d2 = x[nd[2]];
+        const double gdx = (gp[0][0] * d0 + gp[1][0] * d1) + gp[2][0] * d2;
+        const double gdy = (gp[0][1] * d0 + gp[1][1] * d1) + gp[2][1] * d2;
+        const double ds = (d0 + d1) + d2;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double lap = area * (gp[a][0] * gdx + gp[a][1] * gdy);
+            c.set(a, Gc ? (ge * M.l0) * lap : lap, rc * ((a == 0 ? d0 : (a == 1 ? d1 : d2)) + ds));
         }
+    } else {
+        const double gl = Gc ? ge * M.l0 : M.Gc_l0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            c.set(a, 2.0 * h * (area / 3.0), gl * (area * (gp[a][0] * gp[a][0] + gp[a][1] * gp[a][1])) + 2.0 * rc);
     }
-    if (OP == 0) return Gc ? s0 + s1 : M.Gc_l0 * s0 + s1;
-    *s1_out = s1;
-    return s0;
 }
 
 // OP 0: y = A x; OP 1: y = b, y2 = diag; OP 2: y = b, y2 = M_inv = 1/(diag + 1e-30)
 // (damage_solver.py:2265)
-template <int OP>
-__global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3Mat M, const double *H, const double *Gc,
-                                                        const double *x, double *y, double *y2) {
-    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (n >= m.n_nodes) return;
+template <int OP, bool PATCH>
+__global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3Patch P, const T3Mat M, const double *H,
+                                                        const double *Gc, const double *x, double *y, double *y2) {
+    extern __shared__ double lds[];
+    auto corners = [&](int64_t e, Corners &c, bool) { t3_damage_corners<(OP == 0 ? 0 : 1)>(m, M, H, Gc, x, e, c); };
+    if (PATCH) t3_patch_fill(P, lds, corners);
+    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (slot >= m.n_nodes) return;
+    const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
+    double s0, s1;
+    t3_node_sum<PATCH>(m, P, lds, n, corners, s0, s1);
     if (OP == 0) {
-        y[n] = t3_damage_row<0>(m, M, H, Gc, x, n);
+        y[n] = Gc ? s0 + s1 : M.Gc_l0 * s0 + s1;
     } else {
-        double dg;
-        const double bb = t3_damage_row<1>(m, M, H, Gc, x, n, &dg);
-        if (y) y[n] = bb;
-        if (y2) y2[n] = OP == 2 ? 1.0 / (dg + 1e-30) : dg;
+        if (y) y[n] = s0;
+        if (y2) y2[n] = OP == 2 ? 1.0 / (s1 + 1e-30) : s1;
     }
 }
 
@@ -315,19 +411,25 @@ struct T3Cg {
     int32_t nblk;
 };
 
-template <int OP>
-__global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Mat M, T3Cg A) {
+template <int OP, bool PATCH>
+__global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Patch P, const T3Mat M, T3Cg A) {
+    extern __shared__ double lds[];
     __shared__ double red[4];
     const DamageCG &S = A.cg[0];
     if (OP != T_CG_INIT && OP != T_PC_INIT && S.done) return;
     if (OP == T_CG_RESID && !(S.limited || S.check)) return;
     if (OP == T_PC_RESID && !S.check) return;
     constexpr double INF = __builtin_huge_val();
-    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const double *x = (OP == T_CG_AP || OP == T_PC_AP) ? A.p : A.d;
+    auto corners = [&](int64_t e, Corners &c, bool) { t3_damage_corners<0>(m, M, A.H, A.Gc, x, e, c); };
+    if (PATCH) t3_patch_fill(P, lds, corners);
+    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double p0 = 0.0, p1 = (OP == T_CG_AP) ? INF : 0.0, p2 = (OP == T_CG_AP) ? INF : 0.0;
-    if (n < m.n_nodes) {
-        const double *x = (OP == T_CG_AP || OP == T_PC_AP) ? A.p : A.d;
-        const double g0 = t3_damage_row<0>(m, M, A.H, A.Gc, x, n);
+    if (slot < m.n_nodes) {
+        const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
+        double s0, s1;
+        t3_node_sum<PATCH>(m, P, lds, n, corners, s0, s1);
+        const double g0 = A.Gc ? s0 + s1 : M.Gc_l0 * s0 + s1;
         const double xn = x[n];
         if (OP == T_CG_INIT || OP == T_CG_RESID) {
             const double ob = A.b[n], olb = A.lb[n];
@@ -497,7 +599,99 @@ struct phast_t3_plan {
     DamageCG *cg = nullptr, *h_cg = nullptr;
     int32_t nblk = 0;
     double h_min = 0.0, total_area = 0.0;
+    // element patches (see T3Patch); use_patch selects them, lds_bytes sizes a block's LDS
+    int32_t *pnode = nullptr, *pel_ptr = nullptr, *pel = nullptr;
+    uint16_t *loc = nullptr;
+    T3Patch patch{};
+    bool has_patch = false, use_patch = false;
+    int32_t max_list = 0;
+    size_t lds_bytes = 0;
 };
+
+namespace {
+
+constexpr int32_t kPatch = 256;                 // nodes per patch (= threads per block)
+constexpr size_t kPatchLdsMax = 60 * 1024;      // 6 doubles per listed triangle (+ static LDS < 64 KiB)
+constexpr int32_t kPatchListMax = (int32_t)(kPatchLdsMax / 48);
+
+inline uint64_t spread_bits(uint64_t v) {  // bit i -> bit 2i (Morton interleave)
+    v &= 0xffffffffull;
+    v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+    v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+    v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
+}
+
+// Patches of kPatch nodes in the Morton order of the coordinates, each with the ascending
+// list of triangles touching it and, per CSR entry, the LDS slot of its corner.  Leaves the
+// plan without patches when a list would not fit the LDS budget.
+int t3_build_patches(phast_t3_plan *pl, int64_t nn, int64_t ne, const double *nodes, const int32_t *els,
+                     const std::vector<int32_t> &ptr, const std::vector<int32_t> &inc) {
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int64_t i = 0; i < nn; ++i) {
+        x0 = std::min(x0, nodes[2 * i]);
+        x1 = std::max(x1, nodes[2 * i]);
+        y0 = std::min(y0, nodes[2 * i + 1]);
+        y1 = std::max(y1, nodes[2 * i + 1]);
+    }
+    const double sx = x1 > x0 ? 2147483647.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 2147483647.0 / (y1 - y0) : 0.0;
+    std::vector<std::pair<uint64_t, int32_t>> key((size_t)nn);
+    for (int64_t i = 0; i < nn; ++i) {
+        const uint64_t qx = (uint64_t)((nodes[2 * i] - x0) * sx), qy = (uint64_t)((nodes[2 * i + 1] - y0) * sy);
+        key[(size_t)i] = {spread_bits(qx) | (spread_bits(qy) << 1), (int32_t)i};
+    }
+    std::sort(key.begin(), key.end());
+    const int64_t npatch = ceil_div(nn, kPatch);
+    std::vector<int32_t> pnode((size_t)nn), owner_patch((size_t)nn);
+    for (int64_t s = 0; s < nn; ++s) {
+        pnode[(size_t)s] = key[(size_t)s].second;
+        owner_patch[(size_t)key[(size_t)s].second] = (int32_t)(s / kPatch);
+    }
+    std::vector<int32_t> pel_ptr((size_t)npatch + 1, 0), pel;
+    pel.reserve((size_t)(ne * 3 / 2));
+    std::vector<uint16_t> loc((size_t)(3 * ne));
+    std::vector<int32_t> list;
+    int32_t max_list = 0;
+    for (int64_t b = 0; b < npatch; ++b) {
+        const int64_t s0 = b * kPatch, s1 = std::min<int64_t>(nn, s0 + kPatch);
+        list.clear();
+        for (int64_t s = s0; s < s1; ++s) {
+            const int32_t n = pnode[(size_t)s];
+            for (int32_t k = ptr[(size_t)n]; k < ptr[(size_t)n + 1]; ++k) list.push_back(inc[(size_t)k] / 3);
+        }
+        std::sort(list.begin(), list.end());
+        list.erase(std::unique(list.begin(), list.end()), list.end());
+        max_list = std::max<int32_t>(max_list, (int32_t)list.size());
+        if ((int32_t)list.size() > kPatchListMax) return PHAST_OK;  // no patches for this mesh
+        for (int64_t s = s0; s < s1; ++s) {
+            const int32_t n = pnode[(size_t)s];
+            for (int32_t k = ptr[(size_t)n]; k < ptr[(size_t)n + 1]; ++k) {
+                const int32_t e = inc[(size_t)k] / 3, a = inc[(size_t)k] - 3 * e;
+                const int32_t j = (int32_t)(std::lower_bound(list.begin(), list.end(), e) - list.begin());
+                loc[(size_t)k] = (uint16_t)(3 * j + a);
+            }
+        }
+        for (int32_t e : list) pel.push_back(owner_patch[(size_t)els[3 * (int64_t)e]] == b ? (e | INT32_MIN) : e);
+        pel_ptr[(size_t)b + 1] = (int32_t)pel.size();
+    }
+    PHAST_HIP(hipMalloc(&pl->pnode, sizeof(int32_t) * pnode.size()));
+    PHAST_HIP(hipMalloc(&pl->pel_ptr, sizeof(int32_t) * pel_ptr.size()));
+    PHAST_HIP(hipMalloc(&pl->pel, sizeof(int32_t) * pel.size()));
+    PHAST_HIP(hipMalloc(&pl->loc, sizeof(uint16_t) * loc.size()));
+    PHAST_HIP(hipMemcpy(pl->pnode, pnode.data(), sizeof(int32_t) * pnode.size(), hipMemcpyHostToDevice));
+    PHAST_HIP(hipMemcpy(pl->pel_ptr, pel_ptr.data(), sizeof(int32_t) * pel_ptr.size(), hipMemcpyHostToDevice));
+    PHAST_HIP(hipMemcpy(pl->pel, pel.data(), sizeof(int32_t) * pel.size(), hipMemcpyHostToDevice));
+    PHAST_HIP(hipMemcpy(pl->loc, loc.data(), sizeof(uint16_t) * loc.size(), hipMemcpyHostToDevice));
+    pl->patch = T3Patch{pl->pnode, pl->pel_ptr, pl->pel, pl->loc};
+    pl->max_list = max_list;
+    pl->lds_bytes = 48 * (size_t)max_list;
+    pl->has_patch = pl->use_patch = true;
+    return PHAST_OK;
+}
+
+}  // namespace
 
 extern "C" int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const double *nodes, const int32_t *elements,
                                     double E, double nu, double Gc, double l0, double eta_residual,
@@ -558,6 +752,10 @@ extern "C" int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const doub
         return PHAST_ERR_HIP;
     }
     pl->mesh = T3Mesh{pl->xy, pl->el, pl->ptr, pl->inc, n_nodes, n_elems};
+    if (t3_build_patches(pl, n_nodes, n_elems, nodes, elements, cnt, inc) != PHAST_OK) {
+        phast_t3_plan_destroy(pl);
+        return PHAST_ERR_HIP;
+    }
     *out = pl;
     return PHAST_OK;
 }
@@ -568,6 +766,10 @@ extern "C" int phast_t3_plan_destroy(phast_t3_plan *pl) {
     (void)hipFree(pl->el);
     (void)hipFree(pl->ptr);
     (void)hipFree(pl->inc);
+    (void)hipFree(pl->pnode);
+    (void)hipFree(pl->pel_ptr);
+    (void)hipFree(pl->pel);
+    (void)hipFree(pl->loc);
     for (double *q : {pl->r, pl->p, pl->z, pl->Ap, pl->b, pl->minv, pl->lb, pl->part}) (void)hipFree(q);
     (void)hipFree(pl->active);
     (void)hipFree(pl->cg);
@@ -586,26 +788,56 @@ extern "C" int phast_t3_mesh_info(const phast_t3_plan *pl, double *M_scalar, dou
 
 static inline dim3 node_grid(const phast_t3_plan *pl) { return dim3((unsigned)ceil_div(pl->mesh.n_nodes, 256)); }
 
+extern "C" int phast_t3_set_path(phast_t3_plan *pl, int32_t path, int32_t *active_path, int32_t *max_list) {
+    PHAST_REQUIRE(pl && (path == 0 || path == 1), "phast_t3_set_path: path must be 0 (patches) or 1 (recompute)");
+    pl->use_patch = path == 0 && pl->has_patch;
+    if (active_path) *active_path = pl->use_patch ? 0 : 1;
+    if (max_list) *max_list = pl->max_list;
+    return PHAST_OK;
+}
+
+template <int MODE>
+static int launch_force(const phast_t3_plan *pl, const double *u, const double *d, double *f, const double *H_old,
+                        double *H_new, double *psi, double dt, const double *v, const double *a, const double *fext,
+                        const double *minv, const uint8_t *bcm, double *v1, double *a1, hipStream_t st) {
+    if (pl->use_patch)
+        hipLaunchKernelGGL((t3_force_kernel<MODE, true>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh,
+                           pl->patch, pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, dt,
+                           (const double2 *)v, (const double2 *)a, (const double2 *)fext, minv, bcm, (double2 *)v1,
+                           (double2 *)a1);
+    else
+        hipLaunchKernelGGL((t3_force_kernel<MODE, false>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
+                           pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, dt, (const double2 *)v,
+                           (const double2 *)a, (const double2 *)fext, minv, bcm, (double2 *)v1, (double2 *)a1);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+template <int OP>
+static int launch_damage(const phast_t3_plan *pl, const double *H, const double *Gc, const double *x, double *y,
+                         double *y2, hipStream_t st) {
+    if (pl->use_patch)
+        hipLaunchKernelGGL((t3_damage_kernel<OP, true>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh,
+                           pl->patch, pl->mat, H, Gc, x, y, y2);
+    else
+        hipLaunchKernelGGL((t3_damage_kernel<OP, false>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
+                           pl->mat, H, Gc, x, y, y2);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
 extern "C" int phast_t3_internal_force(const phast_t3_plan *pl, const double *u, const double *d, double *f,
                                        hipStream_t st) {
     PHAST_REQUIRE(pl && u && d && f, "phast_t3_internal_force: null argument");
-    hipLaunchKernelGGL(t3_force_kernel<0>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, (const double2 *)u, d,
-                       (double2 *)f, (const double *)nullptr, (double *)nullptr, (double *)nullptr, 0.0,
-                       (const double2 *)nullptr, (const double2 *)nullptr, (const double2 *)nullptr,
-                       (const double *)nullptr, (const uint8_t *)nullptr, (double2 *)nullptr, (double2 *)nullptr);
-    PHAST_LAUNCHED();
-    return PHAST_OK;
+    return launch_force<0>(pl, u, d, f, nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, st);
 }
 
 extern "C" int phast_t3_force_history(const phast_t3_plan *pl, const double *u, const double *d, const double *H_old,
                                       double *f, double *H_new, double *psi, hipStream_t st) {
     PHAST_REQUIRE(pl && u && d && H_old && f && H_new, "phast_t3_force_history: null argument");
-    hipLaunchKernelGGL(t3_force_kernel<1>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, (const double2 *)u, d,
-                       (double2 *)f, H_old, H_new, psi, 0.0, (const double2 *)nullptr, (const double2 *)nullptr,
-                       (const double2 *)nullptr, (const double *)nullptr, (const uint8_t *)nullptr,
-                       (double2 *)nullptr, (double2 *)nullptr);
-    PHAST_LAUNCHED();
-    return PHAST_OK;
+    return launch_force<1>(pl, u, d, f, H_old, H_new, psi, 0.0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                           nullptr, st);
 }
 
 extern "C" int phast_t3_explicit_step(const phast_t3_plan *pl, double dt, const double *u, const double *v,
@@ -619,11 +851,7 @@ extern "C" int phast_t3_explicit_step(const phast_t3_plan *pl, double dt, const 
     hipLaunchKernelGGL(t3_predictor_kernel, node_grid(pl), dim3(256), 0, st, pl->mesh.n_nodes, dt, (const double2 *)u,
                        (const double2 *)v, (const double2 *)a, bc_mask, bc_vals, (double2 *)u1);
     PHAST_LAUNCHED();
-    hipLaunchKernelGGL(t3_force_kernel<2>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, (const double2 *)u1, d,
-                       (double2 *)nullptr, H_old, H_new, psi, dt, (const double2 *)v, (const double2 *)a,
-                       (const double2 *)f_ext, Minv, bc_mask, (double2 *)v1, (double2 *)a1);
-    PHAST_LAUNCHED();
-    return PHAST_OK;
+    return launch_force<2>(pl, u1, d, nullptr, H_old, H_new, psi, dt, v, a, f_ext, Minv, bc_mask, v1, a1, st);
 }
 
 extern "C" int phast_t3_elem_to_node(const phast_t3_plan *pl, const double *field_e, const double *M, double *out,
@@ -637,19 +865,13 @@ extern "C" int phast_t3_elem_to_node(const phast_t3_plan *pl, const double *fiel
 extern "C" int phast_t3_damage_apply(const phast_t3_plan *pl, const double *H, const double *Gc_e, const double *x,
                                      double *y, hipStream_t st) {
     PHAST_REQUIRE(pl && H && x && y && x != y, "phast_t3_damage_apply: bad argument");
-    hipLaunchKernelGGL(t3_damage_kernel<0>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H, Gc_e, x, y,
-                       (double *)nullptr);
-    PHAST_LAUNCHED();
-    return PHAST_OK;
+    return launch_damage<0>(pl, H, Gc_e, x, y, nullptr, st);
 }
 
 extern "C" int phast_t3_damage_rhs_diag(const phast_t3_plan *pl, const double *H, const double *Gc_e, double *b,
                                         double *diag, hipStream_t st) {
     PHAST_REQUIRE(pl && H && (b || diag), "phast_t3_damage_rhs_diag: null argument");
-    hipLaunchKernelGGL(t3_damage_kernel<1>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H, Gc_e,
-                       (const double *)nullptr, b, diag);
-    PHAST_LAUNCHED();
-    return PHAST_OK;
+    return launch_damage<1>(pl, H, Gc_e, nullptr, b, diag, st);
 }
 
 #define T3_TRY(x)                         \
@@ -672,7 +894,12 @@ static int t3_ensure_work(phast_t3_plan *pl) {
 
 template <int OP>
 static int t3_cg(const phast_t3_plan *pl, const T3Cg &A, hipStream_t st) {
-    hipLaunchKernelGGL(t3_cg_kernel<OP>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, A);
+    if (pl->use_patch)
+        hipLaunchKernelGGL((t3_cg_kernel<OP, true>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh, pl->patch,
+                           pl->mat, A);
+    else
+        hipLaunchKernelGGL((t3_cg_kernel<OP, false>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch, pl->mat,
+                           A);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
@@ -730,9 +957,7 @@ extern "C" int phast_t3_damage_solve(phast_t3_plan *pl, int32_t method, const do
     A.nblk = pl->nblk;
 
     // b and M_inv (damage_solver.py:2249-2265); d = init or d_prev, lb = d_prev (pins applied)
-    hipLaunchKernelGGL(t3_damage_kernel<2>, node_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, H, Gc_e,
-                       (const double *)nullptr, pl->b, pl->minv);
-    PHAST_LAUNCHED();
+    T3_TRY(launch_damage<2>(pl, H, Gc_e, nullptr, pl->b, pl->minv, st));
     T3_TRY(t3_vec<TV_SETUP>(pl, A, st));
     const int poll = 4;
     int it = 0;

@@ -129,6 +129,15 @@ class FEMOperators:
         self.M_vec_inv = 1.0 / (self.M_vec + 1e-30)
         self.M_inv_node = 1.0 / (material.rho * self.M_scalar + 1e-30)
         self.dt_cfl = hmin.value / material.c_p  # fem_operators.py:88-89
+        self.path = self.set_path(0)
+
+    def set_path(self, path: int) -> int:
+        """0: element patches in LDS (default when the mesh allows), 1: every incidence
+        re-evaluates its triangle.  Both give the same bits; returns the organisation in use."""
+        act, mx = C.c_int32(), C.c_int32()
+        check(_lib.lib().phast_t3_set_path(self.plan.handle, int(path), C.byref(act), C.byref(mx)), "set_path")
+        self.path, self.max_patch_list = int(act.value), int(mx.value)
+        return self.path
 
     def _u(self, u, name="u"):
         return _dev_check(u, (self.mesh.n_nodes, 2), name, self.device)
