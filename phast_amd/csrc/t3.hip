@@ -132,10 +132,11 @@ struct T3Elem {
     double area, gp[3][2], exx, eyy, gxy, g;
 };
 
-__device__ __forceinline__ void t3_load(const T3Mesh &m, const T3Mat &M, int64_t e, const double2 *u, const double *d,
+__device__ __forceinline__ void t3_load(const T3Mesh &m, const T3Mat &M, int4 t, const double2 *u, const double *d,
                                         T3Elem &E) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) E.n[k] = m.el[3 * e + k];
+    E.n[0] = t.y;
+    E.n[1] = t.z;
+    E.n[2] = t.w;
     t3_geometry(m.xy[E.n[0]], m.xy[E.n[1]], m.xy[E.n[2]], E.area, E.gp);
     const double2 u0 = u[E.n[0]], u1 = u[E.n[1]], u2 = u[E.n[2]];
     E.exx = (E.gp[0][0] * u0.x + E.gp[1][0] * u1.x) + E.gp[2][0] * u2.x;
@@ -156,12 +157,21 @@ __device__ __forceinline__ void t3_load(const T3Mesh &m, const T3Mat &M, int64_t
 // lists for each patch the triangles touching it.  A block evaluates each listed triangle
 // once into LDS (2 values per corner), then each thread sums its node's corners from LDS in
 // the node's e*3+a order.  Without patches every incidence re-evaluates its triangle.
+// A triangle is passed around as int4 {e, n0, n1, n2}.  Everything a block reads for its
+// patch is contiguous: the triangle table, the slot -> node map and, per slot in patch order,
+// the LDS slots of the node's corners.
 struct T3Patch {
     const int32_t *pnode;    // (N) node of each patch slot, patch b = slots [256 b, 256 b + 256)
     const int32_t *pel_ptr;  // (npatch + 1)
-    const int32_t *pel;      // triangles of each patch, ascending; sign bit = the patch owns corner 0
-    const uint16_t *loc;     // (3E) per CSR entry: LDS slot (list position * 3 + corner)
+    const int4 *pel;         // {e, n0, n1, n2} of each patch's triangles, e ascending; the sign bit
+                             // of e marks the patch that owns corner 0
+    const int32_t *sptr;     // (N + 1) incidences of slot s: sloc[sptr[s] .. sptr[s+1])
+    const uint16_t *sloc;    // (3E) LDS slot (list position * 3 + corner), in the node's e*3+a order
 };
+
+__device__ __forceinline__ int4 tri_of(const T3Mesh &m, int64_t e) {
+    return make_int4((int32_t)e, m.el[3 * e], m.el[3 * e + 1], m.el[3 * e + 2]);
+}
 
 // Two values per corner of one triangle, as named scalars: selecting a corner from an array
 // with a run-time index would put the array in scratch memory.
@@ -181,15 +191,17 @@ struct Corners {
     }
 };
 
-// Fill the block's LDS with the corner values of its patch's triangles; corners(e, c, owner)
-// fills the Corners of triangle e.  Every thread of the block must call it.
+// Fill the block's LDS with the corner values of its patch's triangles; corners(t, c, owner)
+// fills the Corners of triangle t.  Every thread of the block must call it.
 template <class F>
 __device__ __forceinline__ void t3_patch_fill(const T3Patch &P, double *lds, F corners) {
     const int32_t j0 = P.pel_ptr[blockIdx.x], nel = P.pel_ptr[blockIdx.x + 1] - j0;
     for (int32_t j = threadIdx.x; j < nel; j += 256) {
-        const int32_t w = P.pel[j0 + j];
+        int4 t = P.pel[j0 + j];
+        const bool owner = t.x < 0;
+        t.x &= 0x7fffffff;
         Corners c;
-        corners((int64_t)(w & 0x7fffffff), c, w < 0);
+        corners(t, c, owner);
         lds[6 * j] = c.c00;
         lds[6 * j + 1] = c.c01;
         lds[6 * j + 2] = c.c10;
@@ -203,14 +215,15 @@ __device__ __forceinline__ void t3_patch_fill(const T3Patch &P, double *lds, F c
 // Sums over node n's incidences, in e*3+a order, of the two corner values.  __dadd_rn keeps
 // each addition a separate rounding in both organisations, so they agree bitwise.
 template <bool PATCH, class F>
-__device__ __forceinline__ void t3_node_sum(const T3Mesh &m, const T3Patch &P, const double *lds, int64_t n, F corners,
-                                            double &s0, double &s1) {
+__device__ __forceinline__ void t3_node_sum(const T3Mesh &m, const T3Patch &P, const double *lds, int64_t slot,
+                                            int64_t n, F corners, double &s0, double &s1) {
     s0 = 0.0;
     s1 = 0.0;
-    for (int32_t k = m.ptr[n]; k < m.ptr[n + 1]; ++k) {
+    const int32_t kb = PATCH ? P.sptr[slot] : m.ptr[n], ke = PATCH ? P.sptr[slot + 1] : m.ptr[n + 1];
+    for (int32_t k = kb; k < ke; ++k) {
         double v0, v1;
         if (PATCH) {
-            const int sl = P.loc[k];
+            const int sl = P.sloc[k];
             v0 = lds[2 * sl];
             v1 = lds[2 * sl + 1];
         } else {
@@ -218,7 +231,7 @@ __device__ __forceinline__ void t3_node_sum(const T3Mesh &m, const T3Patch &P, c
             const int64_t e = ic / 3;
             const int a_ = ic - 3 * (int)e;
             Corners c;
-            corners(e, c, a_ == 0);
+            corners(tri_of(m, e), c, a_ == 0);
             v0 = a_ == 0 ? c.c00 : (a_ == 1 ? c.c10 : c.c20);
             v1 = a_ == 0 ? c.c01 : (a_ == 1 ? c.c11 : c.c21);
         }
@@ -232,10 +245,11 @@ __device__ __forceinline__ void t3_node_sum(const T3Mesh &m, const T3Patch &P, c
 // staggered_solver.py:804-845).
 template <bool HIST>
 __device__ __forceinline__ void t3_force_corners(const T3Mesh &m, const T3Mat &M, const double2 *u, const double *d,
-                                                 const double *H_old, double *H_new, double *psi, int64_t e,
+                                                 const double *H_old, double *H_new, double *psi, int4 t,
                                                  Corners &c, bool owner) {
+    const int64_t e = t.x;
     T3Elem E;
-    t3_load(m, M, e, u, d, E);
+    t3_load(m, M, t, u, d, E);
     double sxx, syy, sxy;
     t3_stress(M, E.exx, E.eyy, E.gxy, E.g, sxx, syy, sxy);
 #pragma unroll
@@ -258,15 +272,15 @@ __global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3P
                                                        const double2 *fext, const double *minv, const uint8_t *bcm,
                                                        double2 *v1, double2 *a1) {
     extern __shared__ double lds[];
-    auto corners = [&](int64_t e, Corners &c, bool owner) {
-        t3_force_corners<(MODE >= 1)>(m, M, u, d, H_old, H_new, psi, e, c, owner);
+    auto corners = [&](int4 t, Corners &c, bool owner) {
+        t3_force_corners<(MODE >= 1)>(m, M, u, d, H_old, H_new, psi, t, c, owner);
     };
     if (PATCH) t3_patch_fill(P, lds, corners);
     const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (slot >= m.n_nodes) return;
     const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
     double fx, fy;
-    t3_node_sum<PATCH>(m, P, lds, n, corners, fx, fy);
+    t3_node_sum<PATCH>(m, P, lds, slot, n, corners, fx, fy);
     if (MODE < 2) {
         f[n] = make_double2(fx, fy);
         return;
@@ -309,10 +323,9 @@ __global__ __launch_bounds__(256) void t3_predictor_kernel(int64_t nn, double dt
 // OP 1: c[a] = (2H A/3 (:2249-2258), Gc l0 A |grad phi_a|^2 + 2 rc (:2112-2119)).
 template <int OP>
 __device__ __forceinline__ void t3_damage_corners(const T3Mesh &m, const T3Mat &M, const double *H, const double *Gc,
-                                                  const double *x, int64_t e, Corners &c) {
-    int32_t nd[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) nd[q] = m.el[3 * e + q];
+                                                  const double *x, int4 t, Corners &c) {
+    const int64_t e = t.x;
+    const int32_t nd[3] = {t.y, t.z, t.w};
     double area, gp[3][2];
     t3_geometry(m.xy[nd[0]], m.xy[nd[1]], m.xy[nd[2]], area, gp);
     const double h = H[e];
@@ -343,13 +356,13 @@ template <int OP, bool PATCH>
 __global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3Patch P, const T3Mat M, const double *H,
                                                         const double *Gc, const double *x, double *y, double *y2) {
     extern __shared__ double lds[];
-    auto corners = [&](int64_t e, Corners &c, bool) { t3_damage_corners<(OP == 0 ? 0 : 1)>(m, M, H, Gc, x, e, c); };
+    auto corners = [&](int4 t, Corners &c, bool) { t3_damage_corners<(OP == 0 ? 0 : 1)>(m, M, H, Gc, x, t, c); };
     if (PATCH) t3_patch_fill(P, lds, corners);
     const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (slot >= m.n_nodes) return;
     const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
     double s0, s1;
-    t3_node_sum<PATCH>(m, P, lds, n, corners, s0, s1);
+    t3_node_sum<PATCH>(m, P, lds, slot, n, corners, s0, s1);
     if (OP == 0) {
         y[n] = Gc ? s0 + s1 : M.Gc_l0 * s0 + s1;
     } else {
@@ -421,14 +434,14 @@ __global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Patc
     if (OP == T_PC_RESID && !S.check) return;
     constexpr double INF = __builtin_huge_val();
     const double *x = (OP == T_CG_AP || OP == T_PC_AP) ? A.p : A.d;
-    auto corners = [&](int64_t e, Corners &c, bool) { t3_damage_corners<0>(m, M, A.H, A.Gc, x, e, c); };
+    auto corners = [&](int4 t, Corners &c, bool) { t3_damage_corners<0>(m, M, A.H, A.Gc, x, t, c); };
     if (PATCH) t3_patch_fill(P, lds, corners);
     const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double p0 = 0.0, p1 = (OP == T_CG_AP) ? INF : 0.0, p2 = (OP == T_CG_AP) ? INF : 0.0;
     if (slot < m.n_nodes) {
         const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
         double s0, s1;
-        t3_node_sum<PATCH>(m, P, lds, n, corners, s0, s1);
+        t3_node_sum<PATCH>(m, P, lds, slot, n, corners, s0, s1);
         const double g0 = A.Gc ? s0 + s1 : M.Gc_l0 * s0 + s1;
         const double xn = x[n];
         if (OP == T_CG_INIT || OP == T_CG_RESID) {
@@ -600,8 +613,9 @@ struct phast_t3_plan {
     int32_t nblk = 0;
     double h_min = 0.0, total_area = 0.0;
     // element patches (see T3Patch); use_patch selects them, lds_bytes sizes a block's LDS
-    int32_t *pnode = nullptr, *pel_ptr = nullptr, *pel = nullptr;
-    uint16_t *loc = nullptr;
+    int32_t *pnode = nullptr, *pel_ptr = nullptr, *sptr = nullptr;
+    int4 *pel = nullptr;
+    uint16_t *sloc = nullptr;
     T3Patch patch{};
     bool has_patch = false, use_patch = false;
     int32_t max_list = 0;
@@ -649,11 +663,12 @@ int t3_build_patches(phast_t3_plan *pl, int64_t nn, int64_t ne, const double *no
         pnode[(size_t)s] = key[(size_t)s].second;
         owner_patch[(size_t)key[(size_t)s].second] = (int32_t)(s / kPatch);
     }
-    std::vector<int32_t> pel_ptr((size_t)npatch + 1, 0), pel;
+    std::vector<int32_t> pel_ptr((size_t)npatch + 1, 0), sptr((size_t)nn + 1, 0);
+    std::vector<int4> pel;
     pel.reserve((size_t)(ne * 3 / 2));
-    std::vector<uint16_t> loc((size_t)(3 * ne));
+    std::vector<uint16_t> sloc((size_t)(3 * ne));
     std::vector<int32_t> list;
-    int32_t max_list = 0;
+    int32_t max_list = 0, nk = 0;
     for (int64_t b = 0; b < npatch; ++b) {
         const int64_t s0 = b * kPatch, s1 = std::min<int64_t>(nn, s0 + kPatch);
         list.clear();
@@ -670,21 +685,27 @@ int t3_build_patches(phast_t3_plan *pl, int64_t nn, int64_t ne, const double *no
             for (int32_t k = ptr[(size_t)n]; k < ptr[(size_t)n + 1]; ++k) {
                 const int32_t e = inc[(size_t)k] / 3, a = inc[(size_t)k] - 3 * e;
                 const int32_t j = (int32_t)(std::lower_bound(list.begin(), list.end(), e) - list.begin());
-                loc[(size_t)k] = (uint16_t)(3 * j + a);
+                sloc[(size_t)nk++] = (uint16_t)(3 * j + a);
             }
+            sptr[(size_t)s + 1] = nk;
         }
-        for (int32_t e : list) pel.push_back(owner_patch[(size_t)els[3 * (int64_t)e]] == b ? (e | INT32_MIN) : e);
+        for (int32_t e : list) {
+            const int32_t *q = els + 3 * (int64_t)e;
+            pel.push_back(make_int4(owner_patch[(size_t)q[0]] == b ? (e | INT32_MIN) : e, q[0], q[1], q[2]));
+        }
         pel_ptr[(size_t)b + 1] = (int32_t)pel.size();
     }
     PHAST_HIP(hipMalloc(&pl->pnode, sizeof(int32_t) * pnode.size()));
     PHAST_HIP(hipMalloc(&pl->pel_ptr, sizeof(int32_t) * pel_ptr.size()));
-    PHAST_HIP(hipMalloc(&pl->pel, sizeof(int32_t) * pel.size()));
-    PHAST_HIP(hipMalloc(&pl->loc, sizeof(uint16_t) * loc.size()));
+    PHAST_HIP(hipMalloc(&pl->pel, sizeof(int4) * pel.size()));
+    PHAST_HIP(hipMalloc(&pl->sptr, sizeof(int32_t) * sptr.size()));
+    PHAST_HIP(hipMalloc(&pl->sloc, sizeof(uint16_t) * sloc.size()));
     PHAST_HIP(hipMemcpy(pl->pnode, pnode.data(), sizeof(int32_t) * pnode.size(), hipMemcpyHostToDevice));
     PHAST_HIP(hipMemcpy(pl->pel_ptr, pel_ptr.data(), sizeof(int32_t) * pel_ptr.size(), hipMemcpyHostToDevice));
-    PHAST_HIP(hipMemcpy(pl->pel, pel.data(), sizeof(int32_t) * pel.size(), hipMemcpyHostToDevice));
-    PHAST_HIP(hipMemcpy(pl->loc, loc.data(), sizeof(uint16_t) * loc.size(), hipMemcpyHostToDevice));
-    pl->patch = T3Patch{pl->pnode, pl->pel_ptr, pl->pel, pl->loc};
+    PHAST_HIP(hipMemcpy(pl->pel, pel.data(), sizeof(int4) * pel.size(), hipMemcpyHostToDevice));
+    PHAST_HIP(hipMemcpy(pl->sptr, sptr.data(), sizeof(int32_t) * sptr.size(), hipMemcpyHostToDevice));
+    PHAST_HIP(hipMemcpy(pl->sloc, sloc.data(), sizeof(uint16_t) * sloc.size(), hipMemcpyHostToDevice));
+    pl->patch = T3Patch{pl->pnode, pl->pel_ptr, pl->pel, pl->sptr, pl->sloc};
     pl->max_list = max_list;
     pl->lds_bytes = 48 * (size_t)max_list;
     pl->has_patch = pl->use_patch = true;
@@ -769,7 +790,8 @@ extern "C" int phast_t3_plan_destroy(phast_t3_plan *pl) {
     (void)hipFree(pl->pnode);
     (void)hipFree(pl->pel_ptr);
     (void)hipFree(pl->pel);
-    (void)hipFree(pl->loc);
+    (void)hipFree(pl->sptr);
+    (void)hipFree(pl->sloc);
     for (double *q : {pl->r, pl->p, pl->z, pl->Ap, pl->b, pl->minv, pl->lb, pl->part}) (void)hipFree(q);
     (void)hipFree(pl->active);
     (void)hipFree(pl->cg);
