@@ -211,6 +211,11 @@ int phast_t3_explicit_step(const phast_t3_plan *plan, double dt, const double *u
                            const double *d, const double *f_ext, const double *Minv, const uint8_t *bc_mask,
                            const double *bc_vals, const double *H_old, double *u1, double *v1, double *a1,
                            double *H_new, double *psi, hipStream_t stream);
+/* Element fields of u (N,2) at damage d (N,), one thread per triangle, each output optional:
+ * eps (3,E) = centroid strain exx, eyy, gxy (fem_operators.py:223-242); sig (3,E) = the split's
+ * degraded stress sxx, syy, sxy (:614-653; needs d); psi (E,) = psi+ (:951-1030). */
+int phast_t3_element_fields(const phast_t3_plan *plan, const double *u, const double *d, double *eps, double *sig,
+                            double *psi, hipStream_t stream);
 /* Area-weighted projection of an element field f (E,) to the nodes (core/mesh.py:407-433):
  * out_n = sum_e f_e A_e / (3 M_n + 1e-30), M = M_scalar (device (N,)). */
 int phast_t3_elem_to_node(const phast_t3_plan *plan, const double *field_e, const double *M, double *out,

@@ -72,6 +72,7 @@ _SIGNATURES = {
     "phast_t3_internal_force": ([_P, _P, _P, _P, _P], C.c_int),
     "phast_t3_force_history": ([_P] * 8, C.c_int),
     "phast_t3_explicit_step": ([_P, _D] + [_P] * 15, C.c_int),
+    "phast_t3_element_fields": ([_P] * 7, C.c_int),
     "phast_t3_elem_to_node": ([_P] * 5, C.c_int),
     "phast_t3_damage_apply": ([_P] * 6, C.c_int),
     "phast_t3_damage_rhs_diag": ([_P] * 6, C.c_int),

@@ -132,13 +132,14 @@ struct T3Elem {
     double area, gp[3][2], exx, eyy, gxy, g;
 };
 
-__device__ __forceinline__ void t3_load(const T3Mesh &m, const T3Mat &M, int4 t, const double2 *u, const double *d,
-                                        T3Elem &E) {
+// ux(n) returns the displacement of node n (read, or predicted by the explicit step).
+template <class UX>
+__device__ __forceinline__ void t3_load(const T3Mesh &m, const T3Mat &M, int4 t, UX ux, const double *d, T3Elem &E) {
     E.n[0] = t.y;
     E.n[1] = t.z;
     E.n[2] = t.w;
     t3_geometry(m.xy[E.n[0]], m.xy[E.n[1]], m.xy[E.n[2]], E.area, E.gp);
-    const double2 u0 = u[E.n[0]], u1 = u[E.n[1]], u2 = u[E.n[2]];
+    const double2 u0 = ux(E.n[0]), u1 = ux(E.n[1]), u2 = ux(E.n[2]);
     E.exx = (E.gp[0][0] * u0.x + E.gp[1][0] * u1.x) + E.gp[2][0] * u2.x;
     E.eyy = (E.gp[0][1] * u0.y + E.gp[1][1] * u1.y) + E.gp[2][1] * u2.y;
     E.gxy = ((E.gp[0][1] * u0.x + E.gp[1][1] * u1.x) + E.gp[2][1] * u2.x) +
@@ -243,13 +244,13 @@ __device__ __forceinline__ void t3_node_sum(const T3Mesh &m, const T3Patch &P, c
 // Corner forces A (s . grad phi_a) of one triangle (fem_operators.py:661-679); with HIST the
 // owner also writes psi+ and H = max(H_old, psi+) (fem_operators.py:951-1030,
 // staggered_solver.py:804-845).
-template <bool HIST>
-__device__ __forceinline__ void t3_force_corners(const T3Mesh &m, const T3Mat &M, const double2 *u, const double *d,
+template <bool HIST, class UX>
+__device__ __forceinline__ void t3_force_corners(const T3Mesh &m, const T3Mat &M, UX ux, const double *d,
                                                  const double *H_old, double *H_new, double *psi, int4 t,
                                                  Corners &c, bool owner) {
     const int64_t e = t.x;
     T3Elem E;
-    t3_load(m, M, t, u, d, E);
+    t3_load(m, M, t, ux, d, E);
     double sxx, syy, sxy;
     t3_stress(M, E.exx, E.eyy, E.gxy, E.g, sxx, syy, sxy);
 #pragma unroll
@@ -263,17 +264,46 @@ __device__ __forceinline__ void t3_force_corners(const T3Mesh &m, const T3Mat &M
     }
 }
 
+// Operands of the explicit step (solvers/mechanics_solver.py:153-239).
+struct T3Step {
+    double dt, c2;  // c2 = dt^2 / 2
+    const double2 *v, *a, *fext;
+    const double *minv;
+    const uint8_t *bcm;  // (N,2) prescribed components, or null
+    const double *bcv;
+    double2 *u1, *v1, *a1;
+};
+
+// Predictor u1 = u + dt v + dt^2/2 a with prescribed components overridden
+// (mechanics_solver.py:160-170).
+__device__ __forceinline__ double2 t3_predict(const T3Step &S, const double2 *u, int64_t n) {
+    const double2 u0 = u[n], v0 = S.v[n], a0 = S.a[n];
+    double ux = (u0.x + S.dt * v0.x) + S.c2 * a0.x, uy = (u0.y + S.dt * v0.y) + S.c2 * a0.y;
+    if (S.bcm) {
+        if (S.bcm[2 * n]) ux = S.bcv[2 * n];
+        if (S.bcm[2 * n + 1]) uy = S.bcv[2 * n + 1];
+    }
+    return make_double2(ux, uy);
+}
+
+// The predictor as its own pass.  Evaluating it inside the force pass, for the corners of
+// every listed triangle, was measured slower (8.4 M nodes: 0.675 ms against 0.625 ms): the
+// triangles then gather u, v and a instead of u1 alone.
+__global__ __launch_bounds__(256) void t3_predictor_kernel(int64_t nn, const double2 *u, const T3Step S) {
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n < nn) S.u1[n] = t3_predict(S, u, n);
+}
+
 // MODE 0: f = internal force; 1: + psi+ and H = max(H_old, psi+) per element;
-// 2: explicit-step corrector on top of 1 (u is the predicted u1).
+// 2: explicit-step corrector on top of 1 (u is the predicted u1; writes v1, a1).
 template <int MODE, bool PATCH>
 __global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3Patch P, const T3Mat M, const double2 *u,
                                                        const double *d, double2 *f, const double *H_old, double *H_new,
-                                                       double *psi, double dt, const double2 *v, const double2 *a,
-                                                       const double2 *fext, const double *minv, const uint8_t *bcm,
-                                                       double2 *v1, double2 *a1) {
+                                                       double *psi, const T3Step S) {
     extern __shared__ double lds[];
+    auto ux = [&](int64_t n) { return u[n]; };
     auto corners = [&](int4 t, Corners &c, bool owner) {
-        t3_force_corners<(MODE >= 1)>(m, M, u, d, H_old, H_new, psi, t, c, owner);
+        t3_force_corners<(MODE >= 1)>(m, M, ux, d, H_old, H_new, psi, t, c, owner);
     };
     if (PATCH) t3_patch_fill(P, lds, corners);
     const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -286,33 +316,18 @@ __global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3P
         return;
     }
     // mechanics_solver.py:194-198: a1 = Minv (f_ext - f_int), v1 = v + dt/2 (a + a1); BC zero
-    const double mi = minv[n];
-    const double2 fe = fext ? fext[n] : make_double2(0.0, 0.0);
-    const double2 v0 = v[n], a0 = a[n];
+    const double dt = S.dt;
+    const double mi = S.minv[n];
+    const double2 fe = S.fext ? S.fext[n] : make_double2(0.0, 0.0);
+    const double2 v0 = S.v[n], a0 = S.a[n];
     double ax = mi * (fe.x - fx), ay = mi * (fe.y - fy);
     double vx = v0.x + 0.5 * dt * (a0.x + ax), vy = v0.y + 0.5 * dt * (a0.y + ay);
-    if (bcm) {
-        if (bcm[2 * n]) ax = vx = 0.0;
-        if (bcm[2 * n + 1]) ay = vy = 0.0;
+    if (S.bcm) {
+        if (S.bcm[2 * n]) ax = vx = 0.0;
+        if (S.bcm[2 * n + 1]) ay = vy = 0.0;
     }
-    v1[n] = make_double2(vx, vy);
-    a1[n] = make_double2(ax, ay);
-}
-
-// u1 = u + dt v + dt^2/2 a with prescribed components overridden (mechanics_solver.py:160-170).
-__global__ __launch_bounds__(256) void t3_predictor_kernel(int64_t nn, double dt, const double2 *u, const double2 *v,
-                                                           const double2 *a, const uint8_t *bcm, const double *bcv,
-                                                           double2 *u1) {
-    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (n >= nn) return;
-    const double c2 = 0.5 * dt * dt;
-    const double2 u0 = u[n], v0 = v[n], a0 = a[n];
-    double ux = (u0.x + dt * v0.x) + c2 * a0.x, uy = (u0.y + dt * v0.y) + c2 * a0.y;
-    if (bcm) {
-        if (bcm[2 * n]) ux = bcv[2 * n];
-        if (bcm[2 * n + 1]) uy = bcv[2 * n + 1];
-    }
-    u1[n] = make_double2(ux, uy);
+    S.v1[n] = make_double2(vx, vy);
+    S.a1[n] = make_double2(ax, ay);
 }
 
 // Damage operator of the element-H AT2 path (solvers/damage_solver.py), corner values of
@@ -369,6 +384,30 @@ __global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3
         if (y) y[n] = s0;
         if (y2) y2[n] = OP == 2 ? 1.0 / (s1 + 1e-30) : s1;
     }
+}
+
+// Element fields, one thread per triangle: centroid strain (fem_operators.py:223-242), the
+// split's degraded stress (:614-653) and psi+ (:951-1030); each output optional.
+__global__ __launch_bounds__(256) void t3_elem_fields_kernel(const T3Mesh m, const T3Mat M, const double2 *u,
+                                                             const double *d, double *eps, double *sig, double *psi) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m.n_elems) return;
+    T3Elem E;
+    t3_load(m, M, tri_of(m, e), [&](int64_t n) { return u[n]; }, d, E);
+    const int64_t ne = m.n_elems;
+    if (eps) {
+        eps[e] = E.exx;
+        eps[ne + e] = E.eyy;
+        eps[2 * ne + e] = E.gxy;
+    }
+    if (sig) {
+        double sxx, syy, sxy;
+        t3_stress(M, E.exx, E.eyy, E.gxy, E.g, sxx, syy, sxy);
+        sig[e] = sxx;
+        sig[ne + e] = syy;
+        sig[2 * ne + e] = sxy;
+    }
+    if (psi) psi[e] = t3_psi_plus(M, E.exx, E.eyy, E.gxy);
 }
 
 // Area-weighted projection of an element field (core/mesh.py:430-433):
@@ -820,17 +859,13 @@ extern "C" int phast_t3_set_path(phast_t3_plan *pl, int32_t path, int32_t *activ
 
 template <int MODE>
 static int launch_force(const phast_t3_plan *pl, const double *u, const double *d, double *f, const double *H_old,
-                        double *H_new, double *psi, double dt, const double *v, const double *a, const double *fext,
-                        const double *minv, const uint8_t *bcm, double *v1, double *a1, hipStream_t st) {
+                        double *H_new, double *psi, const T3Step &S, hipStream_t st) {
     if (pl->use_patch)
         hipLaunchKernelGGL((t3_force_kernel<MODE, true>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh,
-                           pl->patch, pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, dt,
-                           (const double2 *)v, (const double2 *)a, (const double2 *)fext, minv, bcm, (double2 *)v1,
-                           (double2 *)a1);
+                           pl->patch, pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, S);
     else
         hipLaunchKernelGGL((t3_force_kernel<MODE, false>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
-                           pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, dt, (const double2 *)v,
-                           (const double2 *)a, (const double2 *)fext, minv, bcm, (double2 *)v1, (double2 *)a1);
+                           pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, S);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
@@ -851,15 +886,13 @@ static int launch_damage(const phast_t3_plan *pl, const double *H, const double 
 extern "C" int phast_t3_internal_force(const phast_t3_plan *pl, const double *u, const double *d, double *f,
                                        hipStream_t st) {
     PHAST_REQUIRE(pl && u && d && f, "phast_t3_internal_force: null argument");
-    return launch_force<0>(pl, u, d, f, nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, nullptr, nullptr, nullptr,
-                           nullptr, nullptr, st);
+    return launch_force<0>(pl, u, d, f, nullptr, nullptr, nullptr, T3Step{}, st);
 }
 
 extern "C" int phast_t3_force_history(const phast_t3_plan *pl, const double *u, const double *d, const double *H_old,
                                       double *f, double *H_new, double *psi, hipStream_t st) {
     PHAST_REQUIRE(pl && u && d && H_old && f && H_new, "phast_t3_force_history: null argument");
-    return launch_force<1>(pl, u, d, f, H_old, H_new, psi, 0.0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                           nullptr, st);
+    return launch_force<1>(pl, u, d, f, H_old, H_new, psi, T3Step{}, st);
 }
 
 extern "C" int phast_t3_explicit_step(const phast_t3_plan *pl, double dt, const double *u, const double *v,
@@ -869,11 +902,33 @@ extern "C" int phast_t3_explicit_step(const phast_t3_plan *pl, double dt, const 
     PHAST_REQUIRE(pl && u && v && a && d && Minv && H_old && u1 && v1 && a1 && H_new,
                   "phast_t3_explicit_step: null argument");
     PHAST_REQUIRE(!bc_mask == !bc_vals, "phast_t3_explicit_step: bc_mask and bc_vals go together");
-    PHAST_REQUIRE(u1 != u && u1 != v && u1 != a && v1 != v && a1 != a, "phast_t3_explicit_step: outputs alias inputs");
-    hipLaunchKernelGGL(t3_predictor_kernel, node_grid(pl), dim3(256), 0, st, pl->mesh.n_nodes, dt, (const double2 *)u,
-                       (const double2 *)v, (const double2 *)a, bc_mask, bc_vals, (double2 *)u1);
+    PHAST_REQUIRE(u1 != u && u1 != v && u1 != a && v1 != u && v1 != v && v1 != a && a1 != u && a1 != v && a1 != a,
+                  "phast_t3_explicit_step: outputs alias inputs");
+    T3Step S;
+    S.dt = dt;
+    S.c2 = 0.5 * dt * dt;
+    S.v = (const double2 *)v;
+    S.a = (const double2 *)a;
+    S.fext = (const double2 *)f_ext;
+    S.minv = Minv;
+    S.bcm = bc_mask;
+    S.bcv = bc_vals;
+    S.u1 = (double2 *)u1;
+    S.v1 = (double2 *)v1;
+    S.a1 = (double2 *)a1;
+    hipLaunchKernelGGL(t3_predictor_kernel, node_grid(pl), dim3(256), 0, st, pl->mesh.n_nodes, (const double2 *)u, S);
     PHAST_LAUNCHED();
-    return launch_force<2>(pl, u1, d, nullptr, H_old, H_new, psi, dt, v, a, f_ext, Minv, bc_mask, v1, a1, st);
+    return launch_force<2>(pl, u1, d, nullptr, H_old, H_new, psi, S, st);
+}
+
+extern "C" int phast_t3_element_fields(const phast_t3_plan *pl, const double *u, const double *d, double *eps,
+                                       double *sig, double *psi, hipStream_t st) {
+    PHAST_REQUIRE(pl && u && (eps || sig || psi), "phast_t3_element_fields: null argument");
+    PHAST_REQUIRE(d || !sig, "phast_t3_element_fields: the stress needs the damage d");
+    hipLaunchKernelGGL(t3_elem_fields_kernel, dim3((unsigned)ceil_div(pl->mesh.n_elems, 256)), dim3(256), 0, st,
+                       pl->mesh, pl->mat, (const double2 *)u, d, eps, sig, psi);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
 }
 
 extern "C" int phast_t3_elem_to_node(const phast_t3_plan *pl, const double *field_e, const double *M, double *out,

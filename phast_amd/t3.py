@@ -164,13 +164,31 @@ class FEMOperators:
                                                 ptr(psi), stream()), "force_history")
         return f, psi, H_new
 
+    def _fields(self, u, d=None, strain=False, stress=False, psi=False):
+        u = self._u(u)
+        d = None if d is None else self._d(d)
+        E = self.mesh.n_elems
+        eps = torch.empty((3, E), dtype=F64, device=self.device) if strain else None
+        sig = torch.empty((3, E), dtype=F64, device=self.device) if stress else None
+        ps = torch.empty(E, dtype=F64, device=self.device) if psi else None
+        check(_lib.lib().phast_t3_element_fields(self.plan.handle, ptr(u), ptr(d), ptr(eps), ptr(sig), ptr(ps),
+                                                 stream()), "element_fields")
+        return eps, sig, ps
+
+    def compute_strain(self, u: torch.Tensor):
+        """Centroid strains ``(eps_xx, eps_yy, gam_xy)``, each (E,) (fem_operators.py:223-242)."""
+        eps, _, _ = self._fields(u, strain=True)
+        return eps[0], eps[1], eps[2]
+
+    def compute_stress(self, u: torch.Tensor, d: torch.Tensor, strain=None):
+        """Degraded element stresses ``(sxx, syy, sxy)``, each (E,), for the material's split
+        (fem_operators.py:614-653); ``strain`` is recomputed from ``u``."""
+        _, sig, _ = self._fields(u, d, stress=True)
+        return sig[0], sig[1], sig[2]
+
     def compute_psi_plus(self, u: torch.Tensor, strain=None, d=None) -> torch.Tensor:
         """psi+ (E,) at the centroids (fem_operators.py:951-1030); ``strain`` is recomputed."""
-        n = self.mesh.n_elems
-        zero = torch.zeros(n, dtype=F64, device=self.device)
-        _, psi, _ = self.internal_force_and_history(u, torch.zeros(self.mesh.n_nodes, dtype=F64,
-                                                                   device=self.device), zero)
-        return psi
+        return self._fields(u, psi=True)[2]
 
     def compute_driving_force(self, u: torch.Tensor, strain=None, d=None) -> torch.Tensor:
         """'strain_energy' driving force == psi+ (fem_operators.py:1173-1204)."""

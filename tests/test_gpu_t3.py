@@ -82,6 +82,19 @@ def test_internal_force_psi_history(dev, split, ps):
     assert (Hn.cpu().numpy() > H).any() and (Hn.cpu().numpy() == H).any()
 
 
+@pytest.mark.parametrize("split,ps", [("amor", True), ("spectral", False), ("isotropic", False)])
+def test_element_fields(dev, split, ps):
+    nodes, els = scrambled_mesh(21, 19, seed=22)
+    P, fem, ofem = setup(nodes, els, split, ps)
+    u, v, a, d, H = state(len(nodes), len(els), seed=23)
+    s = fem.compute_strain(T(u, dev))
+    for x, y in zip(s, ofem.compute_strain(u)):
+        assert exact(x, y)
+    for x, y in zip(fem.compute_stress(T(u, dev), T(d, dev)), ofem.compute_stress(u, d)):
+        assert exact(x, y)
+    assert exact(fem.compute_psi_plus(T(u, dev)), ofem.compute_psi_plus(u))
+
+
 def test_mesh_precompute(dev):
     nodes, els = scrambled_mesh(31, 12, seed=2)
     P, fem, ofem = setup(nodes, els)
