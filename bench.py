@@ -37,6 +37,11 @@ HEX8_64_TRAFFIC = {(32, 14, 3): ((12528.61 + 2147.25) * 2.0 + 8269.75 + 2112.06)
                    (16, 14, 6): ((15001.75 + 1091.25) * 2.0 + 7213.75 + 1056.06) * 1024.0}
 Q4_APPLY_BYTES_PER_NODE = 40.0  # u 16 + d 8 + f 16
 Q4_STEP_BYTES_PER_NODE = 192.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, H r+w 64
+# one projected damage-CG iteration without a bound hit or residual check (per node): A p with
+# the step-limit ratios (p 8, H 32, d 8, lb 8, Ap 8), d/r update (32 + 16), preconditioner
+# (r 8, Minv 8, active 1, z 8), direction update (z 8, p 8, active 1, p 8); the residual check
+# of every 10th iteration (about 66 B) adds 6.6 on average
+Q4_CG_BYTES_PER_NODE = 64.0 + 48.0 + 25.0 + 25.0 + 6.6
 # T3 split-quad plates (2 triangles per node; geometry recomputed from the coordinates)
 T3_APPLY_BYTES_PER_NODE = 80.0  # u 16 + d 8 + f 16 + xy 16 + 2 triangles x 12 B connectivity
 T3_STEP_BYTES_PER_NODE = 216.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, xy 16, conn 24, H r+w + psi 48
@@ -260,6 +265,8 @@ def secondary(rng, dev, a):
         Hq = T(np.zeros((E, 4)))
         t2, _ = events_time(lambda: dyn._launch(u, v, acc, d, fext, None, None, H_old=Hq), steps, 3)
         entry.update({"explicit_step_ms": t2 * 1e3, "explicit_step_GB_s": Q4_STEP_BYTES_PER_NODE * n / t2 / 1e9})
+        if n > 1e6:
+            entry.update(q4_cg_iteration(P, fem, rng, T))
         res[name] = entry
         del fem, dyn, u, d, v, acc, fext, Hq
     if "t3" in parts:
@@ -280,6 +287,32 @@ def secondary(rng, dev, a):
         del op, x, d
         torch.cuda.empty_cache()
     return res
+
+
+def q4_cg_iteration(P, fem, rng, T):
+    """Per-iteration cost of the device-resident projected damage CG: the difference of two
+    solves run to a fixed iteration count (tol 0), so set-up and finish cancel."""
+    import time
+    import torch
+    E, n = fem.mesh.n_elems, fem.mesh.n_nodes
+    H = T(rng.uniform(0.0, 2e-3, (E, 4)))
+    d0 = torch.zeros(n, dtype=torch.float64, device=H.device)
+    out = {}
+    walls = {}
+    for k in (20, 60):
+        dmg = P.PhaseFieldDamageSolver(fem, tol=0.0, max_iter=k, bounds_method="projected_cg")
+        dmg.solve(H, d0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dmg.solve(H, d0)
+        torch.cuda.synchronize()
+        walls[k] = (time.perf_counter() - t0, dmg.last_iter, dmg.last_status)
+    (w1, i1, _), (w2, i2, st2) = walls[20], walls[60]
+    if i2 > i1:
+        t_it = (w2 - w1) / (i2 - i1)
+        out = {"cg_iteration_ms": t_it * 1e3, "cg_iteration_GB_s": Q4_CG_BYTES_PER_NODE * n / t_it / 1e9,
+               "cg_iterations_timed": [i1, i2], "cg_status": st2}
+    return out
 
 
 def t3_secondary(rng, dev, a):
