@@ -718,7 +718,10 @@ int t3_build_patches(phast_t3_plan *pl, int64_t nn, int64_t ne, const double *no
         std::sort(list.begin(), list.end());
         list.erase(std::unique(list.begin(), list.end()), list.end());
         max_list = std::max<int32_t>(max_list, (int32_t)list.size());
-        if ((int32_t)list.size() > kPatchListMax) return PHAST_OK;  // no patches for this mesh
+        if ((int32_t)list.size() > kPatchListMax) {  // no patches for this mesh
+            pl->max_list = max_list;
+            return PHAST_OK;
+        }
         for (int64_t s = s0; s < s1; ++s) {
             const int32_t n = pnode[(size_t)s];
             for (int32_t k = ptr[(size_t)n]; k < ptr[(size_t)n + 1]; ++k) {

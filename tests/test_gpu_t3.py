@@ -336,6 +336,23 @@ def test_patch_and_recompute_organisations_agree_bitwise(dev, which):
         assert rel(s0[0], s1[0]) < 1e-8
 
 
+def test_fan_mesh_beyond_the_patch_budget_uses_the_recompute_passes(dev):
+    """A fan of 3000 triangles around one node: that node's patch lists more triangles than
+    the LDS budget holds, so the plan has no patches and every pass recomputes per incidence;
+    results still equal the oracle bit for bit."""
+    k = 3000
+    ang = np.linspace(0.0, 2 * np.pi, k, endpoint=False)
+    nodes = np.vstack([[0.0, 0.0], np.stack([np.cos(ang), np.sin(ang)], 1)])
+    els = np.stack([np.zeros(k, int), 1 + np.arange(k), 1 + (np.arange(k) + 1) % k], 1)
+    P, fem, ofem = setup(nodes, els, "amor")
+    assert fem.path == 1 and fem.set_path(0) == 1 and fem.max_patch_list > 1280
+    u, v, a, d, H = state(len(nodes), len(els), seed=24)
+    u *= 1e-3
+    assert exact(fem.internal_force(T(u, dev), T(d, dev)), ofem.internal_force(u, d))
+    dmg = P.PhaseFieldDamageSolver(fem)
+    assert exact(dmg.damage_apply(T(H, dev), T(d, dev)), O.T3Damage(ofem.mesh, ofem.material).apply(d, H))
+
+
 def test_plan_rejects_bad_connectivity(dev):
     from phast_amd import t3 as P
     from phast_amd._lib import PhastLibraryError
