@@ -66,8 +66,8 @@ struct phast_q4_plan {
     phast::Q4Inst *d_inst = nullptr;
     int32_t *d_block_inst = nullptr;
     // solver work (allocated lazily on first solve)
-    double *r = nullptr, *p = nullptr, *z = nullptr, *Ap = nullptr, *b = nullptr, *minv = nullptr,
-           *lb = nullptr, *part = nullptr;
+    double *r = nullptr, *p = nullptr, *p2 = nullptr, *z = nullptr, *Ap = nullptr, *b = nullptr,
+           *minv = nullptr, *lb = nullptr, *part = nullptr;  // p2: second direction buffer
     uint8_t *active = nullptr;
     phast::Q4CG *cg = nullptr;
     phast::Q4CG *h_cg = nullptr;  // pinned mirror
@@ -570,6 +570,12 @@ enum DmgOp : int {
     OP_PC_INIT = 7,    // post-clamp: r = b - A d, z = r Minv, p = z; partials r.r, b.b, r.z
     OP_PC_AP = 8,      // post-clamp: Ap = A p; partial p.Ap
     OP_PC_RESID = 9,   // post-clamp true residual r = b - A d
+    // A p with the previous iteration's direction update fused in: the staged p of every node
+    // is (active ? 0 :) (restart ? z : beta p_old + z), the arithmetic of V_UPDATE_P /
+    // V_PC_UPDATE_P; the owner writes it to pout.  x holds p_old and pout is another buffer,
+    // since neighbouring strips still stage p_old for their halo rows.
+    OP_CG_APF = 10,
+    OP_PC_APF = 11,
 };
 
 struct DmgArgs {
@@ -582,6 +588,7 @@ struct DmgArgs {
     double *out, *out2;   // outputs (y / b, diag)
     // CG vectors
     double *d, *r, *z, *p, *Ap;
+    double *pout;  // OP_CG_APF / OP_PC_APF: the updated direction
     const double *b, *minv, *lb;
     const uint8_t *fixed;
     uint8_t *active;
@@ -600,8 +607,10 @@ constexpr int kDmgLds = 6 * NT;
 // staged row.
 template <int NT, int OP>
 __device__ __forceinline__ void dmg_strip(const DmgArgs &A, int strip, double *sm, double *red) {
-    constexpr bool GATHER_X = (OP == OP_APPLY || OP == OP_CG_INIT || OP == OP_CG_AP || OP == OP_CG_RESID ||
-                               OP == OP_PC_INIT || OP == OP_PC_AP || OP == OP_PC_RESID);
+    constexpr bool FUSEP = (OP == OP_CG_APF || OP == OP_PC_APF);
+    constexpr bool AP_CG = (OP == OP_CG_AP || OP == OP_CG_APF), AP_PC = (OP == OP_PC_AP || OP == OP_PC_APF);
+    constexpr bool GATHER_X = (OP == OP_APPLY || OP == OP_CG_INIT || AP_CG || OP == OP_CG_RESID ||
+                               OP == OP_PC_INIT || AP_PC || OP == OP_PC_RESID);
     constexpr bool CELL_H = (OP != OP_MASS);  // per-cell (E,4) operand: H, or the field for E2N
     constexpr int NOUT = (OP == OP_RHSDIAG) ? 2 : 1;
     double(*sx)[NT] = reinterpret_cast<double(*)[NT]>(sm);       // gathered field rows
@@ -620,9 +629,28 @@ __device__ __forceinline__ void dmg_strip(const DmgArgs &A, int strip, double *s
     // next node row's x (slot t) and next cell row's (E,4) operand, fetched one row ahead
     double px = 0.0, pgc = 0.0;
     double2 ph01 = make_double2(0.0, 0.0), ph23 = ph01;
+    double beta = 0.0;
+    int32_t restart = 0;
+    if (FUSEP) {
+        const Q4CG &S = A.cg[T.inst];
+        beta = S.beta;
+        restart = S.restart;
+    }
     auto fetch_x = [&](int j) {
-        if (GATHER_X) px = (col_ok && j >= 0 && j < T.nyn && j <= T.jend) ? A.x[P.node_off + (int64_t)j * T.nxn + ie]
-                                                                          : 0.0;
+        if (!GATHER_X) return;
+        const bool in = col_ok && j >= 0 && j < T.nyn && j <= T.jend;
+        if (!FUSEP) {
+            px = in ? A.x[P.node_off + (int64_t)j * T.nxn + ie] : 0.0;
+        } else {
+            // every lane loads (a clamped node for slots outside the strip), then masks
+            const int jc = min(max(j, 0), min(T.jend, T.nyn - 1)), icl = min(max(ie, 0), T.nxn - 1);
+            const int64_t q = P.node_off + (int64_t)jc * T.nxn + icl;
+            const double zq = A.z[q], pq = A.x[q];
+            const bool act = OP == OP_CG_APF && A.active[q];
+            double pn = (OP == OP_CG_APF && restart) ? zq : pq * beta + zq;  // q4_vec_kernel V_UPDATE_P
+            if (act) pn = 0.0;
+            px = in ? pn : 0.0;
+        }
     };
     auto fetch_cell = [&](int jc) {
         if (CELL_H && cell_col && jc >= 0 && jc < P.ny && jc < T.jend) {
@@ -637,7 +665,7 @@ __device__ __forceinline__ void dmg_strip(const DmgArgs &A, int strip, double *s
     };
 
     constexpr double INF = __builtin_huge_val();
-    double p0 = 0.0, p1 = (OP == OP_CG_AP) ? INF : 0.0, p2 = (OP == OP_CG_AP) ? INF : 0.0;
+    double p0 = 0.0, p1 = AP_CG ? INF : 0.0, p2 = AP_CG ? INF : 0.0;
     double acc0 = 0.0, acc1 = 0.0;
     int cur = 0;
     fetch_x(T.j0 - 1);
@@ -660,10 +688,10 @@ __device__ __forceinline__ void dmg_strip(const DmgArgs &A, int strip, double *s
                 o_b = A.b[n];
                 o_fx = A.fixed && A.fixed[n];
             }
-            if (OP == OP_CG_INIT || OP == OP_CG_RESID || OP == OP_CG_AP) o_lb = A.lb[n];
+            if (OP == OP_CG_INIT || OP == OP_CG_RESID || AP_CG) o_lb = A.lb[n];
             if (OP == OP_CG_INIT || OP == OP_PC_INIT) o_mi = A.minv[n];
             if (OP == OP_CG_RESID) o_act = A.active[n] != 0;
-            if (OP == OP_CG_AP) o_d = A.d[n];
+            if (AP_CG) o_d = A.d[n];
         }
         fetch_x(jr + 2);
         fetch_cell(jr + 1);
@@ -754,8 +782,9 @@ __device__ __forceinline__ void dmg_strip(const DmgArgs &A, int strip, double *s
                         p2 += rf * zz;
                     }
                     A.active[n] = act ? 1 : 0;
-                } else if (OP == OP_CG_AP) {
+                } else if (AP_CG) {
                     const double pn = xn;
+                    if (FUSEP) A.pout[n] = pn;
                     A.Ap[n] = g0;
                     p0 += pn * g0;
                     if (pn > 0.0) p1 = fmin(p1, (1.0 - o_d) / pn);
@@ -770,7 +799,8 @@ __device__ __forceinline__ void dmg_strip(const DmgArgs &A, int strip, double *s
                     p0 += r * r;
                     p1 += o_b * o_b;
                     p2 += r * zz;
-                } else if (OP == OP_PC_AP) {
+                } else if (AP_PC) {
+                    if (FUSEP) A.pout[n] = xn;
                     A.Ap[n] = g0;
                     p0 += xn * g0;
                 } else if (OP == OP_PC_RESID) {
@@ -784,18 +814,18 @@ __device__ __forceinline__ void dmg_strip(const DmgArgs &A, int strip, double *s
         }
         cur = up;
     }
-    if (OP == OP_CG_INIT || OP == OP_CG_RESID || OP == OP_CG_AP || OP == OP_PC_INIT || OP == OP_PC_AP) {
+    if (OP == OP_CG_INIT || OP == OP_CG_RESID || AP_CG || OP == OP_PC_INIT || AP_PC) {
         const int64_t slot = strip;
         const double s0 = block_sum<NT>(p0, red);
         if (threadIdx.x == 0) A.part[slot] = s0;
-        if (OP == OP_CG_AP) {
+        if (AP_CG) {
             const double s1 = block_min<NT>(p1, red);
             const double s2 = block_min<NT>(p2, red);
             if (threadIdx.x == 0) {
                 A.part[A.n_tiles + slot] = s1;
                 A.part[2 * A.n_tiles + slot] = s2;
             }
-        } else if (OP != OP_PC_AP) {
+        } else if (!AP_PC) {
             const double s1 = block_sum<NT>(p1, red);
             if (threadIdx.x == 0) A.part[A.n_tiles + slot] = s1;
             if (OP == OP_CG_INIT || OP == OP_PC_INIT) {
@@ -1337,6 +1367,7 @@ extern "C" int phast_q4_plan_destroy(phast_q4_plan *pl) {
     (void)hipFree(pl->d_block_inst);
     (void)hipFree(pl->r);
     (void)hipFree(pl->p);
+    (void)hipFree(pl->p2);
     (void)hipFree(pl->z);
     (void)hipFree(pl->Ap);
     (void)hipFree(pl->b);
@@ -1521,6 +1552,7 @@ static int ensure_work(phast_q4_plan *pl) {
     const size_t nb = sizeof(double) * (size_t)pl->n_nodes;
     PHAST_HIP(hipMalloc(&pl->r, nb));
     PHAST_HIP(hipMalloc(&pl->p, nb));
+    PHAST_HIP(hipMalloc(&pl->p2, nb));
     PHAST_HIP(hipMalloc(&pl->z, nb));
     PHAST_HIP(hipMalloc(&pl->Ap, nb));
     PHAST_HIP(hipMalloc(&pl->b, nb));
@@ -1608,18 +1640,27 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
         D.x = d_out;
         PHAST_TRY(launch_dmg<OP_CG_INIT>(pl, D, st));
         PHAST_TRY(launch_scalar<S_INIT>(pl, 10, st));
-        D.x = pl->p;
         DmgArgs Dr = D;
         Dr.x = d_out;
+        // the direction update of iteration k runs inside the A p pass of iteration k+1
+        // (OP_CG_APF), alternating between the two direction buffers
+        double *pc = pl->p, *pn = pl->p2;
         while (it < max_iter) {
-            PHAST_TRY(launch_dmg<OP_CG_AP>(pl, D, st));
+            D.x = pc;
+            if (it == 0) {
+                PHAST_TRY(launch_dmg<OP_CG_AP>(pl, D, st));
+            } else {
+                D.pout = pn;
+                PHAST_TRY(launch_dmg<OP_CG_APF>(pl, D, st));
+                std::swap(pc, pn);
+            }
             PHAST_TRY(launch_scalar<S_A>(pl, 10, st));
+            V.p = pc;
             PHAST_TRY(launch_vec<V_UPDATE_D>(pl, V, st));
             PHAST_TRY(launch_dmg<OP_CG_RESID>(pl, Dr, st));
             PHAST_TRY(launch_scalar<S_B>(pl, 10, st));
             PHAST_TRY(launch_vec<V_PRECOND>(pl, V, st));
             PHAST_TRY(launch_scalar<S_C>(pl, 10, st));
-            PHAST_TRY(launch_vec<V_UPDATE_P>(pl, V, st));
             ++it;
             if (it % poll == 0 || it == max_iter) {
                 PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
@@ -1634,17 +1675,24 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
         D.x = d_out;
         PHAST_TRY(launch_dmg<OP_PC_INIT>(pl, D, st));
         PHAST_TRY(launch_scalar<S_PC_INIT>(pl, 50, st));
-        D.x = pl->p;
         DmgArgs Dr = D;
         Dr.x = d_out;
+        double *pc = pl->p, *pn = pl->p2;
         while (it < max_iter) {
-            PHAST_TRY(launch_dmg<OP_PC_AP>(pl, D, st));
+            D.x = pc;
+            if (it == 0) {
+                PHAST_TRY(launch_dmg<OP_PC_AP>(pl, D, st));
+            } else {
+                D.pout = pn;
+                PHAST_TRY(launch_dmg<OP_PC_APF>(pl, D, st));
+                std::swap(pc, pn);
+            }
             PHAST_TRY(launch_scalar<S_PC_A>(pl, 50, st));
+            V.p = pc;
             PHAST_TRY(launch_vec<V_PC_UPDATE>(pl, V, st));
             PHAST_TRY(launch_dmg<OP_PC_RESID>(pl, Dr, st));
             PHAST_TRY(launch_vec<V_PC_PRECOND>(pl, V, st));
             PHAST_TRY(launch_scalar<S_PC_C>(pl, 50, st));
-            PHAST_TRY(launch_vec<V_PC_UPDATE_P>(pl, V, st));
             ++it;
             if (it % poll == 0 || it == max_iter) {
                 PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
