@@ -39,9 +39,10 @@ Q4_APPLY_BYTES_PER_NODE = 40.0  # u 16 + d 8 + f 16
 Q4_STEP_BYTES_PER_NODE = 192.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, H r+w 64
 # one projected damage-CG iteration without a bound hit or residual check (per node): A p with
 # the previous direction update fused in (z 8, p_old 8, active 1, H 32, d 8, lb 8; Ap 8 and
-# p 8 written), d/r update (32 + 16), preconditioner (r 8, Minv 8, active 1, z 8); the
-# residual check of every 10th iteration (about 66 B) adds 6.6 on average
-Q4_CG_BYTES_PER_NODE = 81.0 + 48.0 + 25.0 + 6.6
+# p 8 written), then the d/r update fused with the preconditioner (p, d, r, Ap, Minv, active
+# read: 41; d, r, z written: 24); the residual check of every 10th iteration (about 66 B,
+# then with the two halves as separate passes) adds 6.6 on average
+Q4_CG_BYTES_PER_NODE = 81.0 + 65.0 + 6.6
 # T3 split-quad plates (2 triangles per node; geometry recomputed from the coordinates)
 T3_APPLY_BYTES_PER_NODE = 80.0  # u 16 + d 8 + f 16 + xy 16 + 2 triangles x 12 B connectivity
 T3_STEP_BYTES_PER_NODE = 216.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, xy 16, conn 24, H r+w + psi 48

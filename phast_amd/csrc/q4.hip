@@ -862,6 +862,14 @@ enum VecOp : int {
     V_PC_PRECOND = 6,// post-clamp: z = r Minv ; partials r.r, r.z
     V_PC_UPDATE_P = 7,
     V_PC_FINISH = 8, // d = clamp(max(d, lb), 0, 1), fixed -> vals
+    // iterations without a residual recomputation (neither a limited step nor a check):
+    // V_UPDATE_D and V_PRECOND in one pass, the latter then skipped (V_PRECOND_CHK); the same
+    // for the post-clamp pair.  The arithmetic and the partial sums are those of the separate
+    // passes.
+    V_UPDATE_D_PRE = 9,
+    V_PRECOND_CHK = 10,
+    V_PC_UPDATE_PRE = 11,
+    V_PC_PRECOND_CHK = 12,
 };
 
 struct VecArgs {
@@ -880,8 +888,16 @@ struct VecArgs {
 // Node-wise pass over the owned nodes of one strip.  Rows are taken U at a time: all the
 // operands of the U rows are loaded before any of them is used, so each thread keeps U
 // rows of loads in flight; partial sums still accumulate row by row.
-template <int NT, int OP>
+template <int NT, int OP0>
 __device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4CG &S, double *red) {
+    // the fused passes run as the update alone when the residual is recomputed in between
+    constexpr bool FUSED = (OP0 == V_UPDATE_D_PRE || OP0 == V_PC_UPDATE_PRE);
+    const bool fuse = FUSED && !(OP0 == V_UPDATE_D_PRE ? (S.limited || S.check) : S.check);
+    constexpr int OP = OP0 == V_PRECOND_CHK ? V_PRECOND
+                       : OP0 == V_PC_PRECOND_CHK ? V_PC_PRECOND
+                       : OP0 == V_UPDATE_D_PRE ? V_UPDATE_D
+                       : OP0 == V_PC_UPDATE_PRE ? V_PC_UPDATE
+                                                : OP0;
     constexpr int U = 4;
     const Tile T = tile_of<NT>(A.inst, A.block_inst, strip);
     QI &P = inst_at(A.inst, T.inst);
@@ -892,16 +908,20 @@ __device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4C
         for (int j0 = T.j0; j0 < T.jend; j0 += U) {
             int64_t n[U];
             bool ok[U];
-            double x0[U], x1[U], x2[U], x3[U];
-            bool m0[U], m1[U];
+            double x0[U], x1[U], x2[U], x3[U], xm[U];
+            bool m0[U], m1[U], ma[U];
 #pragma unroll
             for (int k = 0; k < U; ++k) {
                 ok[k] = j0 + k < T.jend;
                 n[k] = P.node_off + (int64_t)(j0 + k) * T.nxn + i;
-                x0[k] = x1[k] = x2[k] = x3[k] = 0.0;
-                m0[k] = m1[k] = false;
+                x0[k] = x1[k] = x2[k] = x3[k] = xm[k] = 0.0;
+                m0[k] = m1[k] = ma[k] = false;
                 if (!ok[k]) continue;
                 const int64_t q = n[k];
+                if (FUSED && fuse) {  // operands of the preconditioner half
+                    xm[k] = A.minv[q];
+                    ma[k] = (OP0 == V_UPDATE_D_PRE) ? A.active[q] != 0 : (A.fixed && A.fixed[q]);
+                }
                 if (OP == V_SETUP) {
                     m0[k] = A.fixed && A.fixed[q];
                     x0[k] = A.dprev[q];
@@ -951,7 +971,15 @@ __device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4C
                         A.d[q] = dn;
                     } else {
                         A.d[q] = x1[k] + S.alpha * x0[k];
-                        A.r[q] = x2[k] - S.alpha * x3[k];
+                        const double rn = x2[k] - S.alpha * x3[k];
+                        A.r[q] = rn;
+                        if (FUSED && fuse) {  // as V_PRECOND
+                            const double rf = ma[k] ? 0.0 : rn;
+                            double zz = rf * xm[k];
+                            if (ma[k]) zz = 0.0;
+                            A.z[q] = zz;
+                            p0 += rf * zz;
+                        }
                     }
                 } else if (OP == V_PRECOND) {
                     const double rf = m0[k] ? 0.0 : x0[k];
@@ -972,7 +1000,14 @@ __device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4C
                     if (A.active_out) A.active_out[q] = m1[k] ? 1 : 0;
                 } else if (OP == V_PC_UPDATE) {
                     A.d[q] = x1[k] + S.alpha * x0[k];
-                    A.r[q] = x2[k] - S.alpha * x3[k];
+                    const double rn = x2[k] - S.alpha * x3[k];
+                    A.r[q] = rn;
+                    if (FUSED && fuse) {  // as V_PC_PRECOND
+                        const double zz = rn * (ma[k] ? 0.0 : xm[k]);
+                        A.z[q] = zz;
+                        p0 += rn * rn;
+                        p1 += rn * zz;
+                    }
                 } else if (OP == V_PC_PRECOND) {
                     const double rr = x0[k];
                     const double zz = rr * (m0[k] ? 0.0 : x1[k]);
@@ -991,10 +1026,10 @@ __device__ __forceinline__ void vec_strip(const VecArgs &A, int strip, const Q4C
             }
         }
     }
-    if (OP == V_PRECOND || OP == V_PC_PRECOND) {
+    if (OP == V_PRECOND || OP == V_PC_PRECOND || (FUSED && fuse)) {
         const double s0 = block_sum<NT>(p0, red);
         if (threadIdx.x == 0) A.part[strip] = s0;
-        if (OP == V_PC_PRECOND) {
+        if (OP == V_PC_PRECOND || (OP0 == V_PC_UPDATE_PRE && fuse)) {
             const double s1 = block_sum<NT>(p1, red);
             if (threadIdx.x == 0) A.part[A.n_tiles + strip] = s1;
         }
@@ -1007,6 +1042,8 @@ __global__ __launch_bounds__(NT) void q4_vec_kernel(VecArgs A) {
     const int strip = blockIdx.x;
     const Q4CG &S = A.cg[uniform_at(A.block_inst, strip)];
     if (OP != V_SETUP && OP != V_FINISH && OP != V_PC_FINISH && S.done) return;
+    if (OP == V_PRECOND_CHK && !(S.limited || S.check)) return;  // done by V_UPDATE_D_PRE
+    if (OP == V_PC_PRECOND_CHK && !S.check) return;             // done by V_PC_UPDATE_PRE
     vec_strip<NT, OP>(A, strip, S, red);
 }
 
@@ -1656,10 +1693,10 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
             }
             PHAST_TRY(launch_scalar<S_A>(pl, 10, st));
             V.p = pc;
-            PHAST_TRY(launch_vec<V_UPDATE_D>(pl, V, st));
+            PHAST_TRY(launch_vec<V_UPDATE_D_PRE>(pl, V, st));
             PHAST_TRY(launch_dmg<OP_CG_RESID>(pl, Dr, st));
             PHAST_TRY(launch_scalar<S_B>(pl, 10, st));
-            PHAST_TRY(launch_vec<V_PRECOND>(pl, V, st));
+            PHAST_TRY(launch_vec<V_PRECOND_CHK>(pl, V, st));
             PHAST_TRY(launch_scalar<S_C>(pl, 10, st));
             ++it;
             if (it % poll == 0 || it == max_iter) {
@@ -1689,9 +1726,9 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
             }
             PHAST_TRY(launch_scalar<S_PC_A>(pl, 50, st));
             V.p = pc;
-            PHAST_TRY(launch_vec<V_PC_UPDATE>(pl, V, st));
+            PHAST_TRY(launch_vec<V_PC_UPDATE_PRE>(pl, V, st));
             PHAST_TRY(launch_dmg<OP_PC_RESID>(pl, Dr, st));
-            PHAST_TRY(launch_vec<V_PC_PRECOND>(pl, V, st));
+            PHAST_TRY(launch_vec<V_PC_PRECOND_CHK>(pl, V, st));
             PHAST_TRY(launch_scalar<S_PC_C>(pl, 50, st));
             ++it;
             if (it % poll == 0 || it == max_iter) {
