@@ -449,6 +449,13 @@ enum T3VecOp : int {
     TV_PC_PRECOND = 6,  // post-clamp: z = r Minv ; partials r.r, r.z
     TV_PC_UPDATE_P = 7,
     TV_PC_FINISH = 8,   // d = clamp(max(d, lb), 0, 1), fixed -> vals
+    // without a residual recomputation in between (no limited step, no check), the d/r update
+    // and the preconditioner share one pass and the separate preconditioner pass returns at
+    // once; arithmetic and partial sums are those of the separate passes
+    TV_UPDATE_D_PRE = 9,
+    TV_PRECOND_CHK = 10,
+    TV_PC_UPDATE_PRE = 11,
+    TV_PC_PRECOND_CHK = 12,
 };
 
 struct T3Cg {
@@ -554,11 +561,19 @@ __global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Patc
     }
 }
 
-template <int OP>
+template <int OP0>
 __global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
     __shared__ double red[4];
     const DamageCG &S = A.cg[0];
-    if (OP != TV_SETUP && OP != TV_FINISH && OP != TV_PC_FINISH && S.done) return;
+    if (OP0 != TV_SETUP && OP0 != TV_FINISH && OP0 != TV_PC_FINISH && S.done) return;
+    if (OP0 == TV_PRECOND_CHK && !(S.limited || S.check)) return;  // done by TV_UPDATE_D_PRE
+    if (OP0 == TV_PC_PRECOND_CHK && !S.check) return;             // done by TV_PC_UPDATE_PRE
+    constexpr int OP = OP0 == TV_PRECOND_CHK ? TV_PRECOND
+                       : OP0 == TV_PC_PRECOND_CHK ? TV_PC_PRECOND
+                       : OP0 == TV_UPDATE_D_PRE ? TV_UPDATE_D
+                       : OP0 == TV_PC_UPDATE_PRE ? TV_PC_UPDATE
+                                                 : OP0;
+    const bool fuse = (OP0 == TV_UPDATE_D_PRE && !(S.limited || S.check)) || (OP0 == TV_PC_UPDATE_PRE && !S.check);
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double p0 = 0.0, p1 = 0.0;
     if (q < nn) {
@@ -577,7 +592,16 @@ __global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
                 A.d[q] = dn;
             } else {
                 A.d[q] = A.d[q] + S.alpha * A.p[q];
-                A.r[q] = A.r[q] - S.alpha * A.Ap[q];
+                const double rn = A.r[q] - S.alpha * A.Ap[q];
+                A.r[q] = rn;
+                if (fuse) {  // as TV_PRECOND
+                    const bool act = A.active[q] != 0;
+                    const double rf = act ? 0.0 : rn;
+                    double zz = rf * A.minv[q];
+                    if (act) zz = 0.0;
+                    A.z[q] = zz;
+                    p0 = rf * zz;
+                }
             }
         } else if (OP == TV_PRECOND) {
             const bool act = A.active[q] != 0;
@@ -604,7 +628,14 @@ __global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
             if (OP == TV_FINISH && A.active_out) A.active_out[q] = A.active[q] ? 1 : 0;
         } else if (OP == TV_PC_UPDATE) {
             A.d[q] = A.d[q] + S.alpha * A.p[q];
-            A.r[q] = A.r[q] - S.alpha * A.Ap[q];
+            const double rn = A.r[q] - S.alpha * A.Ap[q];
+            A.r[q] = rn;
+            if (fuse) {  // as TV_PC_PRECOND
+                const double zz = rn * ((A.fixed && A.fixed[q]) ? 0.0 : A.minv[q]);
+                A.z[q] = zz;
+                p0 = rn * rn;
+                p1 = rn * zz;
+            }
         } else if (OP == TV_PC_PRECOND) {
             const double rr = A.r[q];
             const double zz = rr * ((A.fixed && A.fixed[q]) ? 0.0 : A.minv[q]);
@@ -615,10 +646,10 @@ __global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
             A.p[q] = A.p[q] * S.beta + A.z[q];
         }
     }
-    if (OP == TV_PRECOND || OP == TV_PC_PRECOND) {
+    if (OP == TV_PRECOND || OP == TV_PC_PRECOND || fuse) {
         const double s0 = block_sum<256>(p0, red);
         if (threadIdx.x == 0) A.part[blockIdx.x] = s0;
-        if (OP == TV_PC_PRECOND) {
+        if (OP == TV_PC_PRECOND || (OP0 == TV_PC_UPDATE_PRE && fuse)) {
             const double s1 = block_sum<256>(p1, red);
             if (threadIdx.x == 0) A.part[A.nblk + blockIdx.x] = s1;
         }
@@ -1052,10 +1083,10 @@ extern "C" int phast_t3_damage_solve(phast_t3_plan *pl, int32_t method, const do
         while (it < max_iter) {
             T3_TRY(t3_cg<T_CG_AP>(pl, A, st));
             T3_TRY(t3_scalar<S_A>(pl, 10, st));
-            T3_TRY(t3_vec<TV_UPDATE_D>(pl, A, st));
+            T3_TRY(t3_vec<TV_UPDATE_D_PRE>(pl, A, st));
             T3_TRY(t3_cg<T_CG_RESID>(pl, A, st));
             T3_TRY(t3_scalar<S_B>(pl, 10, st));
-            T3_TRY(t3_vec<TV_PRECOND>(pl, A, st));
+            T3_TRY(t3_vec<TV_PRECOND_CHK>(pl, A, st));
             T3_TRY(t3_scalar<S_C>(pl, 10, st));
             T3_TRY(t3_vec<TV_UPDATE_P>(pl, A, st));
             ++it;
@@ -1071,9 +1102,9 @@ extern "C" int phast_t3_damage_solve(phast_t3_plan *pl, int32_t method, const do
         while (it < max_iter) {
             T3_TRY(t3_cg<T_PC_AP>(pl, A, st));
             T3_TRY(t3_scalar<S_PC_A>(pl, 50, st));
-            T3_TRY(t3_vec<TV_PC_UPDATE>(pl, A, st));
+            T3_TRY(t3_vec<TV_PC_UPDATE_PRE>(pl, A, st));
             T3_TRY(t3_cg<T_PC_RESID>(pl, A, st));
-            T3_TRY(t3_vec<TV_PC_PRECOND>(pl, A, st));
+            T3_TRY(t3_vec<TV_PC_PRECOND_CHK>(pl, A, st));
             T3_TRY(t3_scalar<S_PC_C>(pl, 50, st));
             T3_TRY(t3_vec<TV_PC_UPDATE_P>(pl, A, st));
             ++it;
