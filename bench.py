@@ -47,6 +47,10 @@ Q4_CG_BYTES_PER_NODE = 81.0 + 65.0 + 6.6
 T3_APPLY_BYTES_PER_NODE = 80.0  # u 16 + d 8 + f 16 + xy 16 + 2 triangles x 12 B connectivity
 T3_STEP_BYTES_PER_NODE = 216.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, xy 16, conn 24, H r+w + psi 48
 T3_DAMAGE_BYTES_PER_NODE = 72.0  # x 8 + y 8 + H 16 + xy 16 + conn 24
+# one projected damage-CG iteration: A p with step ratios (p 8, H 16, xy 16, conn 24, d 8, lb 8,
+# Ap 8), d/r update with the preconditioner (41 read + 24 written), direction update (z, p,
+# active 17 + p 8), the residual check of every 10th iteration (~90 B) 9 on average
+T3_CG_BYTES_PER_NODE = 88.0 + 65.0 + 25.0 + 9.0
 
 
 def parse():
@@ -343,6 +347,21 @@ def t3_secondary(rng, dev, a):
     t3, _ = events_time(lambda: dmg.damage_apply(Hd, d), steps, 3)
     entry.update({"explicit_step_ms": t2 * 1e3, "explicit_step_GB_s": T3_STEP_BYTES_PER_NODE * n / t2 / 1e9,
                   "damage_apply_ms": t3 * 1e3, "damage_apply_GB_s": T3_DAMAGE_BYTES_PER_NODE * n / t3 / 1e9})
+    # per-iteration cost of the projected damage CG: difference of 60- and 20-iteration solves
+    import time
+    walls = {}
+    for k in (20, 60):
+        solver = P3.PhaseFieldDamageSolver(fem, tol=0.0, max_iter=k, bounds_method="projected_cg")
+        d0 = torch.zeros(n, dtype=torch.float64, device=dev)
+        solver.solve(Hd, d0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.solve(Hd, d0)
+        torch.cuda.synchronize()
+        walls[k] = (time.perf_counter() - t0, solver.last_iter)
+    if walls[60][1] > walls[20][1]:
+        t_it = (walls[60][0] - walls[20][0]) / (walls[60][1] - walls[20][1])
+        entry.update({"cg_iteration_ms": t_it * 1e3, "cg_iteration_GB_s": T3_CG_BYTES_PER_NODE * n / t_it / 1e9})
     del fem, dyn, dmg, u, d, v, acc, fext, H, Hd
     torch.cuda.empty_cache()
     return entry
