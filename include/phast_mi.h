@@ -72,6 +72,8 @@ typedef struct phast_q4_desc {
     double Gc, l0;           /* AT2 damage constants                                       */
     double dt;               /* explicit step size                                         */
     double hx, hy;           /* cell edge lengths (cells are congruent axis-aligned rectangles) */
+    int32_t stress_split;    /* PHAST_SPLIT_* of the stress: ISOTROPIC under stress_degradation
+                                'full' (physics/material.py:367-376), else the same as split  */
 } phast_q4_desc;
 
 typedef struct phast_q4_plan phast_q4_plan;
@@ -178,10 +180,11 @@ typedef struct phast_t3_plan phast_t3_plan;
  * elements host (E,3) int32.  Builds the per-node incidence list in e*3+a order, the order in
  * which scatter_add over elements.flatten() visits a node (fem_operators.py:661-679).
  * Material constants as physics/material.py:379-388 (lam, mu, kappa) and :529 (g(d));
- * split = PHAST_SPLIT_*. */
+ * split = PHAST_SPLIT_* of psi+ and the history; stress_split that of the stress
+ * (PHAST_SPLIT_ISOTROPIC under stress_degradation 'full', material.py:367-376). */
 int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const double *nodes, const int32_t *elements,
                          double E, double nu, double Gc, double l0, double eta_residual, int32_t plane_stress,
-                         int32_t split, phast_t3_plan **out);
+                         int32_t split, int32_t stress_split, phast_t3_plan **out);
 int phast_t3_plan_destroy(phast_t3_plan *plan);
 /* Host copies of M_scalar (N,) = scatter(area/3) (core/mesh.py:287-289), h_min
  * (mesh.py:276-283) and the total area.  Any pointer may be null. */

@@ -139,6 +139,13 @@ class Material:
     eta_residual: float = 1e-7
     plane_stress: bool = False
     energy_split: str = "amor"
+    stress_degradation: str = "split"
+
+    @property
+    def stress_split(self) -> str:
+        """Split of the stress: 'isotropic' under stress_degradation='full', which degrades the
+        whole stress by one factor; the history keeps energy_split (material.py:367-376)."""
+        return "isotropic" if self.stress_degradation == "full" else self.energy_split
 
     @property
     def lam(self) -> float:
@@ -194,12 +201,12 @@ class FEMOperators:
             strain = self.compute_strain(u)
         exx, eyy, gxy = strain
         m = self.material
-        if m.energy_split == "isotropic":
+        if m.stress_split == "isotropic":
             lam, mu = m.lam, m.mu
             c11 = lam + 2.0 * mu
             return g * (c11 * exx + lam * eyy), g * (lam * exx + c11 * eyy), g * (mu * gxy)
-        if m.energy_split != "amor":
-            raise ValueError(m.energy_split)
+        if m.stress_split != "amor":
+            raise ValueError(m.stress_split)
         mu, kappa = m.mu, m.kappa
         tr2 = exx + eyy
         ezz = (-m.nu / (1.0 - m.nu) * tr2) if m.plane_stress else 0.0

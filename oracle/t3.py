@@ -109,10 +109,11 @@ class T3Operators:
         exx, eyy, gxy = self.compute_strain(u) if strain is None else strain
         m = self.material
         lam, mu, kappa = m.lam, m.mu, m.kappa
-        if m.energy_split == "isotropic":  # fem_operators.py:277-290
+        split = m.stress_split  # fem_operators.py:24-33, 641-653
+        if split == "isotropic":  # fem_operators.py:277-290
             c11 = lam + 2.0 * mu
             return g * (c11 * exx + lam * eyy), g * (lam * exx + c11 * eyy), g * (mu * gxy)
-        if m.energy_split == "amor":  # fem_operators.py:301-333
+        if split == "amor":  # fem_operators.py:301-333
             tr2 = exx + eyy
             ezz = -m.nu / (1.0 - m.nu) * tr2 if m.plane_stress else 0.0
             tr = tr2 + ezz
@@ -122,7 +123,7 @@ class T3Operators:
             sxx = g * (kappa * trp + 2 * mu * dxx) + kappa * trm
             syy = g * (kappa * trp + 2 * mu * dyy) + kappa * trm
             return sxx, syy, g * mu * gxy
-        if m.energy_split == "spectral":  # fem_operators.py:335-390
+        if split == "spectral":  # fem_operators.py:335-390
             exy = gxy / 2.0
             trace = exx + eyy
             hd = (exx - eyy) / 2.0
@@ -141,7 +142,7 @@ class T3Operators:
             syy = g * (lam * trp + 2 * mu * eyy_p) + (lam * trm + 2 * mu * eyy_m)
             sxy = g * (2 * mu * exy_p) + (2 * mu * exy_m)
             return sxx, syy, sxy
-        raise ValueError(m.energy_split)
+        raise ValueError(split)
 
     def assemble_force(self, sxx, syy, sxy):
         """Nodal force (fem_operators.py:661-679)."""

@@ -37,6 +37,7 @@ class Q4Desc(C.Structure):
         ("plane_stress", C.c_int32), ("split", C.c_int32),
         ("eta_residual", C.c_double), ("Gc", C.c_double), ("l0", C.c_double), ("dt", C.c_double),
         ("hx", C.c_double), ("hy", C.c_double),
+        ("stress_split", C.c_int32),
     ]
 
 
@@ -65,7 +66,7 @@ _SIGNATURES = {
     "phast_q4_batch_prepare": ([_P, _P], C.c_int),
     "phast_q4_batch_step": ([_P, _I32] + [_P] * 12 + [_D, _I32] + [_P] * 9, C.c_int),
     "phast_q4_batch_error": ([_P, C.POINTER(_I32), _I32], C.c_int),
-    "phast_t3_plan_create": ([_I64, _I64, _P, _P, _D, _D, _D, _D, _D, _I32, _I32, C.POINTER(_P)], C.c_int),
+    "phast_t3_plan_create": ([_I64, _I64, _P, _P, _D, _D, _D, _D, _D, _I32, _I32, _I32, C.POINTER(_P)], C.c_int),
     "phast_t3_plan_destroy": ([_P], C.c_int),
     "phast_t3_mesh_info": ([_P, _P, _P, _P], C.c_int),
     "phast_t3_set_path": ([_P, _I32, C.POINTER(_I32), C.POINTER(_I32)], C.c_int),

@@ -37,6 +37,7 @@ struct Q4Inst {
     double lam, mu, kappa, ezz_coef, eta;
     double Gc_l0, Gc_over_l0, l0, dt;
     int32_t split, ky;
+    int32_t ssplit;  // split of the stress (isotropic under stress_degradation 'full')
 };
 
 
@@ -117,28 +118,32 @@ __device__ __forceinline__ void cell_mech(QI &P, const double ux[4], const doubl
         const double exx = exx_[iy], eyy = eyy_[ix], gxy = uxy_[ix] + vyx_[iy];
         const double omd = 1.0 - dq[q];
         const double g = omd * omd * P.ome + P.eta;  // physics/material.py:529
-        if (P.split == PHAST_SPLIT_AMOR) {
-            const double tr2 = exx + eyy;
-            const double ezz = P.ezz_coef * tr2;
-            const double tr = tr2 + ezz;
-            const double trp = fmax(tr, 0.0), trm = tr - trp;
-            const double t3 = tr * (1.0 / 3.0);  // within one ulp of tr / 3 (no IEEE division)
-            const double dxx = exx - t3, dyy = eyy - t3;
+        // stress by the stress split, psi+ by the energy split (the same unless
+        // stress_degradation is 'full')
+        const double tr2 = exx + eyy;
+        const double ezz = P.ezz_coef * tr2;
+        const double tr = tr2 + ezz;
+        const double trp = fmax(tr, 0.0), trm = tr - trp;
+        const double t3 = tr * (1.0 / 3.0);  // within one ulp of tr / 3 (no IEEE division)
+        const double dxx = exx - t3, dyy = eyy - t3;
+        if (P.ssplit == PHAST_SPLIT_AMOR) {
             const double kp = P.kappa * trp, km = P.kappa * trm;
             sxx[q] = g * (kp + 2.0 * P.mu * dxx) + km;
             syy[q] = g * (kp + 2.0 * P.mu * dyy) + km;
             sxy[q] = g * P.mu * gxy;
-            if (PSI) {
-                const double dzz = ezz - t3, exy = gxy / 2.0;
-                psi[q] = 0.5 * P.kappa * trp * trp + P.mu * (dxx * dxx + dyy * dyy + dzz * dzz + 2.0 * exy * exy);
-            }
         } else {
             const double c11 = P.lam + 2.0 * P.mu;
             sxx[q] = g * (c11 * exx + P.lam * eyy);
             syy[q] = g * (P.lam * exx + c11 * eyy);
             sxy[q] = g * (P.mu * gxy);
-            if (PSI) {
-                const double t = exx + eyy, exy = gxy / 2.0;
+        }
+        if (PSI) {
+            const double exy = gxy / 2.0;
+            if (P.split == PHAST_SPLIT_AMOR) {
+                const double dzz = ezz - t3;
+                psi[q] = 0.5 * P.kappa * trp * trp + P.mu * (dxx * dxx + dyy * dyy + dzz * dzz + 2.0 * exy * exy);
+            } else {
+                const double t = exx + eyy;
                 psi[q] = 0.5 * P.lam * t * t + P.mu * (exx * exx + eyy * eyy + 2.0 * exy * exy);
             }
         }
@@ -1317,7 +1322,8 @@ extern "C" int phast_q4_plan_create(const phast_q4_desc *desc, int32_t n_inst, i
     int64_t node_off = 0, cell_off = 0;
     for (int k = 0; k < n_inst; ++k) {
         const phast_q4_desc &D = desc[k];
-        if (D.nx < 1 || D.ny < 1 || D.split < 0 || D.split > 1 || !(D.hx > 0) || !(D.hy > 0)) {
+        if (D.nx < 1 || D.ny < 1 || D.split < 0 || D.split > 1 || D.stress_split < 0 || D.stress_split > 1 ||
+            !(D.hx > 0) || !(D.hy > 0)) {
             delete pl;
             set_error("instance %d: bad nx/ny/split", k);
             return PHAST_ERR_ARG;
@@ -1377,6 +1383,7 @@ extern "C" int phast_q4_plan_create(const phast_q4_desc *desc, int32_t n_inst, i
         I.l0 = D.l0;
         I.dt = D.dt;
         I.split = D.split;
+        I.ssplit = D.stress_split;
         for (int t = 0; t < I.n_tiles; ++t) block_inst.push_back(k);
         node_off += (int64_t)(D.nx + 1) * (D.ny + 1);
         cell_off += (int64_t)D.nx * D.ny;

@@ -43,6 +43,7 @@ constexpr double kSpectralEps = 1e-12;  // core/fem_operators.py:126 (float64)
 struct T3Mat {
     double lam, mu, kappa, ezz_coef, eta, Gc_l0, Gc_over_l0, l0;
     int32_t split, plane_stress;
+    int32_t ssplit;  // split of the stress (isotropic under stress_degradation 'full')
 };
 
 struct T3Mesh {
@@ -68,17 +69,17 @@ __device__ __forceinline__ void t3_geometry(double2 p0, double2 p1, double2 p2, 
     gp[2][1] = (p1.x - p0.x) * inv;
 }
 
-// Degraded stress of one element for the split (fem_operators.py:277-290 isotropic,
-// 301-333 amor, 335-390 spectral), g = g(mean of the corner damages).
+// Degraded stress of one element for the stress split (fem_operators.py:277-290 isotropic,
+// 301-333 amor, 335-390 spectral; chosen as :24-33, 641-653), g = g(mean of the corner damages).
 __device__ __forceinline__ void t3_stress(const T3Mat &M, double exx, double eyy, double gxy, double g, double &sxx,
                                           double &syy, double &sxy) {
     const double lam = M.lam, mu = M.mu;
-    if (M.split == T3_ISO) {
+    if (M.ssplit == T3_ISO) {
         const double c11 = lam + 2.0 * mu;
         sxx = g * (c11 * exx + lam * eyy);
         syy = g * (lam * exx + c11 * eyy);
         sxy = g * (mu * gxy);
-    } else if (M.split == T3_AMOR) {
+    } else if (M.ssplit == T3_AMOR) {
         const double tr2 = exx + eyy;
         const double ezz = M.plane_stress ? M.ezz_coef * tr2 : 0.0;
         const double tr = tr2 + ezz;
@@ -789,11 +790,14 @@ int t3_build_patches(phast_t3_plan *pl, int64_t nn, int64_t ne, const double *no
 
 extern "C" int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const double *nodes, const int32_t *elements,
                                     double E, double nu, double Gc, double l0, double eta_residual,
-                                    int32_t plane_stress, int32_t split, phast_t3_plan **out) {
+                                    int32_t plane_stress, int32_t split, int32_t stress_split,
+                                    phast_t3_plan **out) {
     PHAST_REQUIRE(out && nodes && elements && n_nodes >= 3 && n_elems >= 1, "phast_t3_plan_create: bad argument");
     PHAST_REQUIRE(3 * n_elems < ((int64_t)1 << 31) && n_nodes < ((int64_t)1 << 31),
                   "phast_t3_plan_create: mesh too large for 32-bit incidence indices");
     PHAST_REQUIRE(split == T3_ISO || split == T3_AMOR || split == T3_SPECTRAL, "unknown T3 energy split %d", split);
+    PHAST_REQUIRE(stress_split == T3_ISO || stress_split == T3_AMOR || stress_split == T3_SPECTRAL,
+                  "unknown T3 stress split %d", stress_split);
     for (int64_t k = 0; k < 3 * n_elems; ++k)
         PHAST_REQUIRE(elements[k] >= 0 && elements[k] < n_nodes, "element %lld refers to node %d outside the mesh",
                       (long long)(k / 3), elements[k]);
@@ -808,6 +812,7 @@ extern "C" int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const doub
     M.Gc_over_l0 = Gc / l0;
     M.l0 = l0;
     M.split = split;
+    M.ssplit = stress_split;
     M.plane_stress = plane_stress;
     // host geometry: lumped mass in elem_flat order and h_min (core/mesh.py:270-286)
     pl->M_scalar.assign((size_t)n_nodes, 0.0);

@@ -43,14 +43,23 @@ class Material:
     energy_split: str = "amor"
     pf_model: str = "AT2"
     degradation_type: str = "standard"
+    stress_degradation: str = "split"
 
     def __post_init__(self):
+        if self.stress_degradation not in ("split", "full"):  # material.py:185-188
+            raise ValueError(f"stress_degradation must be 'split' or 'full', got {self.stress_degradation!r}")
         if self.energy_split not in ("amor", "isotropic", "spectral"):
             raise NotImplementedError(
                 f"energy_split={self.energy_split!r}: the HIP path implements 'amor', 'isotropic' and "
                 "'spectral' (T3 only)")
         if self.pf_model != "AT2" or self.degradation_type != "standard":
             raise NotImplementedError("the Q4 HIP path implements AT2 with the standard degradation")
+
+    @property
+    def stress_split(self) -> str:
+        """Split of the stress (material.py:367-376): 'isotropic' under
+        ``stress_degradation='full'``; psi+ and the history keep ``energy_split``."""
+        return "isotropic" if self.stress_degradation == "full" else self.energy_split
 
     @property
     def lam(self) -> float:  # material.py:379-388
@@ -175,6 +184,7 @@ def q4_desc(mesh: FEMMesh, mat: Material, dt: float = 0.0) -> Q4Desc:
         raise NotImplementedError(f"energy_split={mat.energy_split!r}: the Q4 HIP path implements 'amor' and "
                                   "'isotropic'")
     d.split = 0 if mat.energy_split == "amor" else 1
+    d.stress_split = 0 if mat.stress_split == "amor" else 1
     d.eta_residual, d.Gc, d.l0, d.dt = mat.eta_residual, mat.Gc, mat.l0, float(dt)
     d.hx, d.hy = mesh.hx, mesh.hy
     return d

@@ -239,3 +239,24 @@ def test_batched_plan_is_bitwise_equal_to_single_plans(dev):
             st.step_full()
         bu, bv, ba, bd, bH = bat.instance_state(k)
         assert torch.equal(bu, st.u) and torch.equal(bd, st.d) and torch.equal(bH, st.H_elem)
+
+
+@pytest.mark.parametrize("ps", [False, True])
+def test_full_stress_degradation(dev, ps):
+    """stress_degradation='full': isotropic stress, Amor psi+ and history (material.py:367-376)."""
+    from phast_amd import q4 as P
+    kw = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5, energy_split="amor", plane_stress=ps,
+              stress_degradation="full")
+    mesh = P.FEMMesh.rectangle(3.7, 2.3, 37, 23)
+    fem = P.FEMOperators(mesh, P.Material(**kw))
+    ofem = O.FEMOperators(O.Q4Mesh(*O.structured_q4_mesh(3.7, 2.3, 37, 23)), O.Material(**kw))
+    u, v, a, d, H = state(mesh.n_nodes, mesh.n_elems, seed=3)
+    assert rel(fem.internal_force(T(u, dev), T(d, dev)), ofem.internal_force(u, d)) < RTOL
+    assert rel(fem.compute_psi_plus(T(u, dev)), ofem.compute_psi_plus(u)) < RTOL
+    dt = 0.3 * fem.dt_cfl
+    z = np.zeros((mesh.n_nodes, 2))
+    u1, v1, a1, Hn, psi = P.ExplicitDynamics(fem, dt=dt).step_with_history(
+        T(u, dev), T(v, dev), T(a, dev), T(d, dev), T(z, dev), None, None, T(H * 1e-4, dev))
+    ou, ov, oa = O.ExplicitDynamics(ofem, dt=dt).step(u, v, a, d, z)
+    assert rel(a1, oa) < RTOL and rel(psi, ofem.compute_psi_plus(ou)) < RTOL
+

@@ -95,6 +95,21 @@ def test_element_fields(dev, split, ps):
     assert exact(fem.compute_psi_plus(T(u, dev)), ofem.compute_psi_plus(u))
 
 
+@pytest.mark.parametrize("split", ["amor", "spectral"])
+def test_full_stress_degradation(dev, split):
+    """stress_degradation='full': isotropic stress, the split's psi+ (material.py:367-376)."""
+    from phast_amd import t3 as P
+    nodes, els = scrambled_mesh(17, 15, seed=25)
+    kw = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5, energy_split=split, stress_degradation="full")
+    fem = P.FEMOperators(P.FEMMesh(nodes, els), P.Material(**kw))
+    ofem = O.T3Operators(O.T3Mesh(nodes, els), OMaterial(**kw))
+    u, v, a, d, H = state(len(nodes), len(els), seed=26)
+    assert exact(fem.internal_force(T(u, dev), T(d, dev)), ofem.internal_force(u, d))
+    for x, y in zip(fem.compute_stress(T(u, dev), T(d, dev)), ofem.compute_stress(u, d)):
+        assert exact(x, y)
+    assert exact(fem.compute_psi_plus(T(u, dev)), ofem.compute_psi_plus(u))
+
+
 def test_mesh_precompute(dev):
     nodes, els = scrambled_mesh(31, 12, seed=2)
     P, fem, ofem = setup(nodes, els)

@@ -217,3 +217,20 @@ def test_projected_damage_solve_is_the_bound_constrained_minimiser():
     assert (d >= lb).all() and (d <= 1.0).all()
     assert np.isclose(d, lb, rtol=0, atol=1e-12)[lb < 1].any() and (d > lb + 1e-3).any()
     np.testing.assert_allclose(d, res.x, atol=1e-6)
+
+
+@pytest.mark.parametrize("split", ["amor", "spectral"])
+def test_full_stress_degradation_uses_the_isotropic_stress(split):
+    """stress_degradation='full': the stress is the isotropic one, psi+ keeps the split."""
+    mm = mesh()
+    rng = np.random.default_rng(7)
+    u = rng.uniform(-1e-4, 1e-4, (mm.n_nodes, 2))
+    d = rng.uniform(0, 1, mm.n_nodes)
+    kw = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5)
+    full = O.T3Operators(mm, Material(energy_split=split, stress_degradation="full", **kw))
+    iso = O.T3Operators(mm, Material(energy_split="isotropic", **kw))
+    plain = O.T3Operators(mm, Material(energy_split=split, **kw))
+    for a, b in zip(full.compute_stress(u, d), iso.compute_stress(u, d)):
+        assert np.array_equal(a, b)
+    assert np.array_equal(full.compute_psi_plus(u), plain.compute_psi_plus(u))
+    assert not np.allclose(full.internal_force(u, d), plain.internal_force(u, d))
