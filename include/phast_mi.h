@@ -95,6 +95,11 @@ int phast_q4_lumped_mass(const phast_q4_plan *plan, double *M, hipStream_t strea
 int phast_q4_internal_force(const phast_q4_plan *plan, const double *u, const double *d, double *f,
                             hipStream_t stream);
 
+/* f (N,2) = beta f_int(v, d): the Kelvin-Voigt damping force of ExplicitDynamics with
+ * damping_ratio_max > 0, beta = zeta dt_cfl (mechanics_solver.py:110-148, 192-195). */
+int phast_q4_damping_force(const phast_q4_plan *plan, const double *v, const double *d, double beta, double *f,
+                           hipStream_t stream);
+
 /* Gauss-point strain, three (E,4) arrays packed as eps[3][E][4] (fem_operators.py:244-254). */
 int phast_q4_strain(const phast_q4_plan *plan, const double *u, double *eps, hipStream_t stream);
 
@@ -118,9 +123,12 @@ int phast_q4_elem_to_node(const phast_q4_plan *plan, const double *field_q, cons
  *   a1 = Minv (f_ext - f_int(u1, d)), v1 = v + dt/2 (a + a1), a1[bc] = v1[bc] = 0
  *   psi = psi+(u1), H_new = max(H_old, psi)
  * Minv is (N,) = 1/(rho M + 1e-30).  f_ext, bc_mask (uint8 (N,2)), bc_vals, psi may be null.
+ * f_damp (optional, (N,2)) = beta f_int(v, d), the Kelvin-Voigt damping force of
+ * phast_q4_damping_force, subtracted as f_net = (f_ext - f_int) - f_damp
+ * (mechanics_solver.py:165-201, damping_ratio_max > 0).
  * Outputs must not alias inputs (neighbouring tiles read the old state). */
 int phast_q4_explicit_step(const phast_q4_plan *plan, const double *u, const double *v, const double *a,
-                           const double *d, const double *f_ext, const double *Minv,
+                           const double *d, const double *f_ext, const double *f_damp, const double *Minv,
                            const uint8_t *bc_mask, const double *bc_vals,
                            const double *H_old, double *u1, double *v1, double *a1, double *H_new,
                            double *psi, hipStream_t stream);
@@ -208,10 +216,14 @@ int phast_t3_force_history(const phast_t3_plan *plan, const double *u, const dou
  * psi+(u1) and the history max, in two launches (predictor; force + corrector):
  *   u1 = u + dt v + dt^2/2 a, u1[bc] = bc_vals;  a1 = Minv (f_ext - f_int(u1, d)),
  *   v1 = v + dt/2 (a + a1), a1[bc] = v1[bc] = 0;  H_new = max(H_old, psi+(u1)).
- * Minv (N,) = 1/(rho M + 1e-30); f_ext, bc_mask (uint8 (N,2)) + bc_vals, psi may be null.
- * Outputs must not alias inputs. */
+ * Minv (N,) = 1/(rho M + 1e-30); f_ext, bc_mask (uint8 (N,2)) + bc_vals, psi may be null;
+ * f_damp (optional) = phast_t3_damping_force(v, d, beta), subtracted as
+ * f_net = (f_ext - f_int) - f_damp (mechanics_solver.py:165-201).  Outputs must not alias inputs. */
+int phast_t3_damping_force(const phast_t3_plan *plan, const double *v, const double *d, double beta, double *f,
+                           hipStream_t stream);
 int phast_t3_explicit_step(const phast_t3_plan *plan, double dt, const double *u, const double *v, const double *a,
-                           const double *d, const double *f_ext, const double *Minv, const uint8_t *bc_mask,
+                           const double *d, const double *f_ext, const double *f_damp, const double *Minv,
+                           const uint8_t *bc_mask,
                            const double *bc_vals, const double *H_old, double *u1, double *v1, double *a1,
                            double *H_new, double *psi, hipStream_t stream);
 /* Element fields of u (N,2) at damage d (N,), one thread per triangle, each output optional:

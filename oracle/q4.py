@@ -273,11 +273,13 @@ def history_update(H_old: np.ndarray, psi: np.ndarray) -> np.ndarray:
 # ----------------------------------------------------------------------------------------
 
 class ExplicitDynamics:
-    """Predictor / corrector with lumped mass (solvers/mechanics_solver.py:153-239), no damping."""
+    """Predictor / corrector with lumped mass (solvers/mechanics_solver.py:153-239), with the
+    optional Kelvin-Voigt damping force beta f_int(v_n, d), beta = zeta dt_cfl."""
 
-    def __init__(self, fem: FEMOperators, dt: float | None = None):
+    def __init__(self, fem: FEMOperators, dt: float | None = None, damping_ratio_max: float = 0.0):
         self.fem = fem
         self.dt = fem.dt_cfl if dt is None else dt
+        self.beta = damping_ratio_max * fem.dt_cfl  # mechanics_solver.py:139-142
 
     def predictor(self, u, v, a, bc_mask, bc_vals):
         dt = self.dt
@@ -290,6 +292,8 @@ class ExplicitDynamics:
         dt = self.dt
         f_int = self.fem.internal_force(u_new, d)
         f_net = f_ext - f_int
+        if self.beta > 0.0:  # Kelvin-Voigt damping (mechanics_solver.py:192-195)
+            f_net = f_net - self.beta * self.fem.internal_force(v, d)
         a_new = self.fem.M_vec_inv.reshape(-1, 2) * f_net
         v_new = v + 0.5 * dt * (a + a_new)
         if bc_mask is not None:

@@ -59,7 +59,8 @@ _SIGNATURES = {
     "phast_q4_driving_force": ([_P, _P, _P, _P, _P, _P], C.c_int),
     "phast_history_max": ([_I64, _P, _P, _P, _P], C.c_int),
     "phast_q4_elem_to_node": ([_P, _P, _P, _P, _P], C.c_int),
-    "phast_q4_explicit_step": ([_P] * 16, C.c_int),
+    "phast_q4_explicit_step": ([_P] * 17, C.c_int),
+    "phast_q4_damping_force": ([_P, _P, _P, _D, _P, _P], C.c_int),
     "phast_q4_damage_rhs_diag": ([_P] * 6, C.c_int),
     "phast_q4_damage_apply": ([_P] * 6, C.c_int),
     "phast_q4_damage_solve": ([_P, _I32, _P, _P, _P, _P, _P, _P, _D, _I32, _P, _P, _P, _P, _P], C.c_int),
@@ -72,7 +73,8 @@ _SIGNATURES = {
     "phast_t3_set_path": ([_P, _I32, C.POINTER(_I32), C.POINTER(_I32)], C.c_int),
     "phast_t3_internal_force": ([_P, _P, _P, _P, _P], C.c_int),
     "phast_t3_force_history": ([_P] * 8, C.c_int),
-    "phast_t3_explicit_step": ([_P, _D] + [_P] * 15, C.c_int),
+    "phast_t3_explicit_step": ([_P, _D] + [_P] * 16, C.c_int),
+    "phast_t3_damping_force": ([_P, _P, _P, _D, _P, _P], C.c_int),
     "phast_t3_element_fields": ([_P] * 7, C.c_int),
     "phast_t3_elem_to_node": ([_P] * 5, C.c_int),
     "phast_t3_damage_apply": ([_P] * 6, C.c_int),

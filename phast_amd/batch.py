@@ -120,7 +120,7 @@ class BatchedExplicit:
         s = stream()
         for _ in range(n_steps):
             check(L.phast_q4_explicit_step(self.plan.handle, ptr(self.u), ptr(self.v), ptr(self.a), ptr(self.d),
-                                           ptr(self.f_ext), ptr(self.Minv), None, None, ptr(self.H), ptr(self.u1),
+                                           ptr(self.f_ext), None, ptr(self.Minv), None, None, ptr(self.H), ptr(self.u1),
                                            ptr(self.v1), ptr(self.a1), ptr(self.H1), ptr(self.psi), s),
                   "batched explicit step")
             self.u, self.u1 = self.u1, self.u

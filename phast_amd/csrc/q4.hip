@@ -241,6 +241,8 @@ struct MechArgs {
     const double *Hold;
     double2 *u1, *v1, *a1, *f;
     double *Hnew, *psi;
+    const double2 *fdamp;  // explicit step: Kelvin-Voigt damping force beta f_int(v, d), or null
+    double fscale;         // internal force: f = fscale f_int (1 for the plain force)
 };
 
 // LDS of one strip pass: three node rows (u_x, u_y, d) double-buffered, plus the corners
@@ -381,11 +383,12 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
         // operands consumed later in this row (every lane loads a clamped index)
         const bool gat = owner && jr >= T.j0;
         const int64_t ng = P.node_off + (int64_t)jr * T.nxn + ie;
-        double2 fe = make_double2(0.0, 0.0);
+        double2 fe = make_double2(0.0, 0.0), fdm = fe;
         double mi = 0.0;
         if (STEP) {
             const int64_t nc = P.node_off + (int64_t)max(jr, 0) * T.nxn + ic;
             if (A.fext) fe = A.fext[nc];
+            if (A.fdamp) fdm = A.fdamp[nc];
             mi = A.minv[nc];
         }
         const bool cell_ok = cell_col && jr >= 0 && jr < P.ny;
@@ -434,9 +437,15 @@ __device__ __forceinline__ void mech_strip(const MechArgs &A, int strip, double 
                 const double fxn = (accx + sf[0][t - 1]) + fx[0];
                 const double fyn = (accy + sf[1][t - 1]) + fy[0];
                 if (!STEP) {
-                    w_f = make_double2(fxn, fyn);
+                    w_f = make_double2(A.fscale * fxn, A.fscale * fyn);  // x 1 is exact
                 } else {
-                    double ax = mi * (fe.x - fxn), ay = mi * (fe.y - fyn);  // mechanics_solver.py:194-198
+                    // mechanics_solver.py:192-198: f_net = (f_ext - f_int) [- beta f_int(v, d)]
+                    double nx = fe.x - fxn, ny = fe.y - fyn;
+                    if (A.fdamp) {
+                        nx = nx - fdm.x;
+                        ny = ny - fdm.y;
+                    }
+                    double ax = mi * nx, ay = mi * ny;
                     double vx = gv.x + 0.5 * dt * (ga.x + ax), vy = gv.y + 0.5 * dt * (ga.y + ay);
                     if (gm & 1u) ax = vx = 0.0;
                     if (gm & 2u) ay = vy = 0.0;
@@ -1485,11 +1494,27 @@ extern "C" int phast_q4_internal_force(const phast_q4_plan *pl, const double *u,
     A.u = reinterpret_cast<const double2 *>(u);
     A.d = d;
     A.f = reinterpret_cast<double2 *>(f);
+    A.fscale = 1.0;
+    return mech_dispatch(pl, A, false, false, st);
+}
+
+extern "C" int phast_q4_damping_force(const phast_q4_plan *pl, const double *v, const double *d, double beta,
+                                      double *f, hipStream_t st) {
+    PHAST_REQUIRE(pl && v && d && f, "phast_q4_damping_force: null argument");
+    PHAST_REQUIRE(aligned16(v) && aligned16(f), "v and f must be 16-byte aligned (N,2) float64 arrays");
+    MechArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.inst = pl->d_inst;
+    A.block_inst = pl->d_block_inst;
+    A.u = reinterpret_cast<const double2 *>(v);
+    A.d = d;
+    A.f = reinterpret_cast<double2 *>(f);
+    A.fscale = beta;
     return mech_dispatch(pl, A, false, false, st);
 }
 
 extern "C" int phast_q4_explicit_step(const phast_q4_plan *pl, const double *u, const double *v, const double *a,
-                                      const double *d, const double *f_ext, const double *Minv,
+                                      const double *d, const double *f_ext, const double *f_damp, const double *Minv,
                                       const uint8_t *bc_mask, const double *bc_vals, const double *H_old,
                                       double *u1, double *v1, double *a1, double *H_new, double *psi,
                                       hipStream_t st) {
@@ -1497,7 +1522,8 @@ extern "C" int phast_q4_explicit_step(const phast_q4_plan *pl, const double *u, 
     PHAST_REQUIRE(!bc_mask == !bc_vals, "bc_mask and bc_vals must be given together");
     PHAST_REQUIRE(!H_old == !H_new, "H_old and H_new must be given together");
     PHAST_REQUIRE(aligned16(u) && aligned16(v) && aligned16(a) && aligned16(u1) && aligned16(v1) &&
-                      aligned16(a1) && aligned16(f_ext) && aligned16(H_old) && aligned16(H_new) && aligned16(psi),
+                      aligned16(a1) && aligned16(f_ext) && aligned16(f_damp) && aligned16(H_old) &&
+                      aligned16(H_new) && aligned16(psi),
                   "state arrays must be 16-byte aligned");
     PHAST_REQUIRE(u1 != u && v1 != v && a1 != a && u1 != v && u1 != a && (!H_new || H_new != H_old),
                   "explicit step outputs must not alias its inputs");
@@ -1511,6 +1537,7 @@ extern "C" int phast_q4_explicit_step(const phast_q4_plan *pl, const double *u, 
     A.a = reinterpret_cast<const double2 *>(a);
     A.d = d;
     A.fext = reinterpret_cast<const double2 *>(f_ext);
+    A.fdamp = reinterpret_cast<const double2 *>(f_damp);
     A.minv = Minv;
     A.bcm = bc_mask;
     A.bcv = bc_vals;

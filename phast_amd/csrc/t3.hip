@@ -269,6 +269,8 @@ __device__ __forceinline__ void t3_force_corners(const T3Mesh &m, const T3Mat &M
 struct T3Step {
     double dt, c2;  // c2 = dt^2 / 2
     const double2 *v, *a, *fext;
+    const double2 *fdamp;  // Kelvin-Voigt damping force beta f_int(v, d), or null
+    double fscale;         // MODE 0: f = fscale f_int (1 for the plain force)
     const double *minv;
     const uint8_t *bcm;  // (N,2) prescribed components, or null
     const double *bcv;
@@ -312,7 +314,11 @@ __global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3P
     const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
     double fx, fy;
     t3_node_sum<PATCH>(m, P, lds, slot, n, corners, fx, fy);
-    if (MODE < 2) {
+    if (MODE == 0) {
+        f[n] = make_double2(S.fscale * fx, S.fscale * fy);  // x 1 is exact
+        return;
+    }
+    if (MODE == 1) {
         f[n] = make_double2(fx, fy);
         return;
     }
@@ -321,7 +327,14 @@ __global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3P
     const double mi = S.minv[n];
     const double2 fe = S.fext ? S.fext[n] : make_double2(0.0, 0.0);
     const double2 v0 = S.v[n], a0 = S.a[n];
-    double ax = mi * (fe.x - fx), ay = mi * (fe.y - fy);
+    // mechanics_solver.py:192-198: f_net = (f_ext - f_int) [- beta f_int(v, d)]
+    double nx = fe.x - fx, ny = fe.y - fy;
+    if (S.fdamp) {
+        const double2 fd = S.fdamp[n];
+        nx = nx - fd.x;
+        ny = ny - fd.y;
+    }
+    double ax = mi * nx, ay = mi * ny;
     double vx = v0.x + 0.5 * dt * (a0.x + ax), vy = v0.y + 0.5 * dt * (a0.y + ay);
     if (S.bcm) {
         if (S.bcm[2 * n]) ax = vx = 0.0;
@@ -925,7 +938,17 @@ static int launch_damage(const phast_t3_plan *pl, const double *H, const double 
 extern "C" int phast_t3_internal_force(const phast_t3_plan *pl, const double *u, const double *d, double *f,
                                        hipStream_t st) {
     PHAST_REQUIRE(pl && u && d && f, "phast_t3_internal_force: null argument");
-    return launch_force<0>(pl, u, d, f, nullptr, nullptr, nullptr, T3Step{}, st);
+    T3Step S{};
+    S.fscale = 1.0;
+    return launch_force<0>(pl, u, d, f, nullptr, nullptr, nullptr, S, st);
+}
+
+extern "C" int phast_t3_damping_force(const phast_t3_plan *pl, const double *v, const double *d, double beta,
+                                      double *f, hipStream_t st) {
+    PHAST_REQUIRE(pl && v && d && f, "phast_t3_damping_force: null argument");
+    T3Step S{};
+    S.fscale = beta;
+    return launch_force<0>(pl, v, d, f, nullptr, nullptr, nullptr, S, st);
 }
 
 extern "C" int phast_t3_force_history(const phast_t3_plan *pl, const double *u, const double *d, const double *H_old,
@@ -935,7 +958,8 @@ extern "C" int phast_t3_force_history(const phast_t3_plan *pl, const double *u, 
 }
 
 extern "C" int phast_t3_explicit_step(const phast_t3_plan *pl, double dt, const double *u, const double *v,
-                                      const double *a, const double *d, const double *f_ext, const double *Minv,
+                                      const double *a, const double *d, const double *f_ext, const double *f_damp,
+                                      const double *Minv,
                                       const uint8_t *bc_mask, const double *bc_vals, const double *H_old, double *u1,
                                       double *v1, double *a1, double *H_new, double *psi, hipStream_t st) {
     PHAST_REQUIRE(pl && u && v && a && d && Minv && H_old && u1 && v1 && a1 && H_new,
@@ -943,12 +967,13 @@ extern "C" int phast_t3_explicit_step(const phast_t3_plan *pl, double dt, const 
     PHAST_REQUIRE(!bc_mask == !bc_vals, "phast_t3_explicit_step: bc_mask and bc_vals go together");
     PHAST_REQUIRE(u1 != u && u1 != v && u1 != a && v1 != u && v1 != v && v1 != a && a1 != u && a1 != v && a1 != a,
                   "phast_t3_explicit_step: outputs alias inputs");
-    T3Step S;
+    T3Step S{};
     S.dt = dt;
     S.c2 = 0.5 * dt * dt;
     S.v = (const double2 *)v;
     S.a = (const double2 *)a;
     S.fext = (const double2 *)f_ext;
+    S.fdamp = (const double2 *)f_damp;
     S.minv = Minv;
     S.bcm = bc_mask;
     S.bcv = bc_vals;

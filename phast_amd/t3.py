@@ -13,6 +13,7 @@ Reference files (paths relative to the reference's ``src/phast``):
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import numpy as np
 import torch
@@ -221,10 +222,18 @@ class ExplicitDynamics:
 
     def __init__(self, fem: FEMOperators, dt: float | None = None, dt_safety: float = 1.0,
                  differentiable: bool = False, damping_ratio_max: float = 0.0):
-        if differentiable or damping_ratio_max != 0.0:
-            raise NotImplementedError("differentiable / damped explicit dynamics are not on the HIP path")
+        if differentiable:
+            raise NotImplementedError("differentiable explicit dynamics are not on the HIP path")
+        if damping_ratio_max < 0.0:
+            raise ValueError("damping_ratio_max must be non-negative")
         self.fem = fem
-        self.dt_cfl = fem.dt_cfl * dt_safety
+        # Kelvin-Voigt damping f_damp = beta f_int(v_n, d), beta = zeta dt_cfl; the CFL step
+        # shrinks by sqrt(1 + zeta^2) - zeta (mechanics_solver.py:110-148)
+        self.damping_ratio_max = float(damping_ratio_max)
+        self._damping_beta = self.damping_ratio_max * fem.dt_cfl
+        z = self.damping_ratio_max
+        cfl_factor = math.sqrt(1.0 + z * z) - z if z > 0.0 else 1.0
+        self.dt_cfl = fem.dt_cfl * dt_safety * cfl_factor
         self.dt = self.dt_cfl if dt is None else float(dt)
         self.last_iter = 1
 
@@ -251,9 +260,14 @@ class ExplicitDynamics:
             H_old = torch.zeros(fem.mesh.n_elems, dtype=F64, device=fem.device)
         H_new = torch.empty_like(H_old)
         psi = torch.empty_like(H_old) if want_psi else None
+        fd = None
+        if self._damping_beta > 0.0:  # mechanics_solver.py:192-195
+            fd = torch.empty_like(v)
+            check(_lib.lib().phast_t3_damping_force(fem.plan.handle, ptr(v), ptr(d), self._damping_beta, ptr(fd),
+                                                    stream()), "damping_force")
         check(_lib.lib().phast_t3_explicit_step(
-            fem.plan.handle, self.dt, ptr(u), ptr(v), ptr(a), ptr(d), ptr(fe), ptr(fem.M_inv_node), ptr(m), ptr(bv),
-            ptr(H_old), ptr(u1), ptr(v1), ptr(a1), ptr(H_new), ptr(psi), stream()), "explicit_step")
+            fem.plan.handle, self.dt, ptr(u), ptr(v), ptr(a), ptr(d), ptr(fe), ptr(fd), ptr(fem.M_inv_node), ptr(m),
+            ptr(bv), ptr(H_old), ptr(u1), ptr(v1), ptr(a1), ptr(H_new), ptr(psi), stream()), "explicit_step")
         return u1, v1, a1, (H_new if keep else None), psi
 
     def step(self, u, v, a, d, f_ext, bc_mask=None, bc_vals=None, rigid_connectors=None):

@@ -260,3 +260,19 @@ def test_full_stress_degradation(dev, ps):
     ou, ov, oa = O.ExplicitDynamics(ofem, dt=dt).step(u, v, a, d, z)
     assert rel(a1, oa) < RTOL and rel(psi, ofem.compute_psi_plus(ou)) < RTOL
 
+
+def test_kelvin_voigt_damped_step(dev):
+    """damping_ratio_max > 0: f_net = f_ext - f_int(u1) - beta f_int(v) (mechanics_solver.py:192-195)."""
+    P, fem, ofem = setup(nx=31, ny=17, W=3.1, H=1.7)
+    n, E = fem.mesh.n_nodes, fem.mesh.n_elems
+    u, v, a, d, H = state(n, E, seed=9)
+    f_ext = np.random.default_rng(10).uniform(-1e-2, 1e-2, (n, 2))
+    dyn = P.ExplicitDynamics(fem, damping_ratio_max=0.15)
+    odyn = O.ExplicitDynamics(ofem, dt=dyn.dt, damping_ratio_max=0.15)
+    assert dyn.dt == pytest.approx(fem.dt_cfl * (np.sqrt(1 + 0.15 ** 2) - 0.15), rel=1e-14)
+    u1, v1, a1 = dyn.step(T(u, dev), T(v, dev), T(a, dev), T(d, dev), T(f_ext, dev))
+    ou, ov, oa = odyn.step(u, v, a, d, f_ext)
+    assert rel(u1, ou) < RTOL and rel(v1, ov) < RTOL and rel(a1, oa) < RTOL
+    und = O.ExplicitDynamics(ofem, dt=dyn.dt).step(u, v, a, d, f_ext)[2]
+    assert rel(und, oa) > 1e-6  # the damping term is not negligible here
+

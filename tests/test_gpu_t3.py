@@ -149,6 +149,20 @@ def test_explicit_step(dev, split, ps):
     assert exact(s1, ou) and exact(s3, oa) and exact(s2, v + 0.5 * dt * (a + oa))
 
 
+def test_kelvin_voigt_damped_step(dev):
+    """damping_ratio_max > 0 (mechanics_solver.py:192-195), bit-exact against the oracle."""
+    from oracle import q4 as OQ
+    nodes, els = scrambled_mesh(21, 16, seed=27)
+    P, fem, ofem = setup(nodes, els, "amor")
+    n, E = len(nodes), len(els)
+    u, v, a, d, H = state(n, E, seed=28)
+    f_ext = np.random.default_rng(29).uniform(-1e-2, 1e-2, (n, 2))
+    dyn = P.ExplicitDynamics(fem, damping_ratio_max=0.1)
+    u1, v1, a1 = dyn.step(T(u, dev), T(v, dev), T(a, dev), T(d, dev), T(f_ext, dev))
+    ou, ov, oa = OQ.ExplicitDynamics(ofem, dt=dyn.dt, damping_ratio_max=0.1).step(u, v, a, d, f_ext)
+    assert exact(u1, ou) and exact(v1, ov) and exact(a1, oa)
+
+
 def test_damage_operator(dev):
     nodes, els = scrambled_mesh(27, 21, seed=6)
     P, fem, ofem = setup(nodes, els)
