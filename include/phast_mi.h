@@ -53,6 +53,8 @@ const char *phast_last_error(void);
 #define PHAST_SPLIT_AMOR 0
 #define PHAST_SPLIT_ISOTROPIC 1
 #define PHAST_SPLIT_SPECTRAL 2 /* T3 only (core/fem_operators.py:335-390) */
+#define PHAST_PF_AT2 0
+#define PHAST_PF_AT1 1         /* T3 only (solvers/damage_solver.py:1321-1329) */
 
 /* Host descriptor of one structured Q4 plate: (nx+1)(ny+1) nodes numbered
  * j*(nx+1)+i and nx*ny cells numbered j*nx+i with corners (i,j),(i+1,j),(i+1,j+1),(i,j+1)
@@ -189,10 +191,13 @@ typedef struct phast_t3_plan phast_t3_plan;
  * which scatter_add over elements.flatten() visits a node (fem_operators.py:661-679).
  * Material constants as physics/material.py:379-388 (lam, mu, kappa) and :529 (g(d));
  * split = PHAST_SPLIT_* of psi+ and the history; stress_split that of the stress
- * (PHAST_SPLIT_ISOTROPIC under stress_degradation 'full', material.py:367-376). */
+ * (PHAST_SPLIT_ISOTROPIC under stress_degradation 'full', material.py:367-376).  pf_model
+ * PHAST_PF_AT2 or PHAST_PF_AT1 (AT1: 3/4 Gc l0 on the Laplacian, no Gc/l0 in the reaction,
+ * the source at1_source in the right-hand side 2H - S_H; damage_solver.py:1321-1336). */
 int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const double *nodes, const int32_t *elements,
                          double E, double nu, double Gc, double l0, double eta_residual, int32_t plane_stress,
-                         int32_t split, int32_t stress_split, phast_t3_plan **out);
+                         int32_t split, int32_t stress_split, int32_t pf_model, double at1_source,
+                         phast_t3_plan **out);
 int phast_t3_plan_destroy(phast_t3_plan *plan);
 /* Host copies of M_scalar (N,) = scatter(area/3) (core/mesh.py:287-289), h_min
  * (mesh.py:276-283) and the total area.  Any pointer may be null. */

@@ -140,6 +140,18 @@ class Material:
     plane_stress: bool = False
     energy_split: str = "amor"
     stress_degradation: str = "split"
+    pf_model: str = "AT2"
+    at1_threshold: object = "auto"
+
+    @property
+    def at1_source(self) -> float:
+        """AT1 source term S_H of the damage right-hand side 2H - S_H: the override or
+        3 Gc / (8 l0); 0 for AT2 (material.py:260-271)."""
+        if self.pf_model != "AT1":
+            return 0.0
+        if isinstance(self.at1_threshold, str) and self.at1_threshold == "auto":
+            return 3.0 * self.Gc / (8.0 * self.l0)
+        return float(self.at1_threshold)
 
     @property
     def stress_split(self) -> str:

@@ -187,18 +187,34 @@ class T3Operators:
 
 
 class T3Damage:
-    """AT2 damage operator of the element-H path (solvers/damage_solver.py).  ``Gc_e`` is the
-    optional per-element Gc field (the ``_Gc_l0_e`` / ``_Gc_over_l0_e`` branch,
-    damage_solver.py:1412-1433, 603-611)."""
+    """AT2 / AT1 damage operator of the element-H path (solvers/damage_solver.py).  ``Gc_e`` is
+    the optional per-element Gc field (the ``_Gc_l0_e`` / ``_Gc_over_l0_e`` branch,
+    damage_solver.py:1412-1433, 591-611).  AT1: 3/4 Gc l0 on the Laplacian, no Gc/l0 in the
+    reaction, and the source S_H in the right-hand side (damage_solver.py:1321-1336)."""
 
     def __init__(self, mesh: T3Mesh, material: Material, Gc_e: np.ndarray | None = None):
         self.mesh, self.material = mesh, material
         gp = mesh.grad_phi
         self.diag_lap = mesh.areas[:, None] * (gp[:, :, 0] ** 2 + gp[:, :, 1] ** 2)  # :1404-1406
-        self.Gc_l0 = material.Gc * material.l0
-        self.Gc_over_l0 = material.Gc / material.l0
-        self.Gc_l0_e = None if Gc_e is None else Gc_e * material.l0
-        self.Gc_over_l0_e = None if Gc_e is None else Gc_e / material.l0
+        at1 = getattr(material, "pf_model", "AT2") == "AT1"
+        if at1:
+            self.Gc_l0 = 0.75 * material.Gc * material.l0
+            self.Gc_over_l0 = 0.0
+            self.at1_source = material.at1_source
+        else:
+            self.Gc_l0 = material.Gc * material.l0
+            self.Gc_over_l0 = material.Gc / material.l0
+            self.at1_source = 0.0
+        self.at1_source_e = None
+        if Gc_e is None:
+            self.Gc_l0_e = self.Gc_over_l0_e = None
+        elif at1:
+            self.Gc_l0_e = 0.75 * Gc_e * material.l0
+            self.Gc_over_l0_e = np.zeros_like(Gc_e)
+            self.at1_source_e = 3.0 * Gc_e / (8.0 * material.l0)
+        else:
+            self.Gc_l0_e = Gc_e * material.l0
+            self.Gc_over_l0_e = Gc_e / material.l0
 
     def reaction_coeff(self, H):
         """(2 H + Gc/l0) area / 12 (damage_solver.py:2168-2177, no viscosity)."""
@@ -206,9 +222,10 @@ class T3Damage:
         return (2.0 * H + ratio) * self.mesh.areas / 12.0
 
     def rhs(self, H):
-        """b = scatter(2 H area/3) (damage_solver.py:2249-2258, AT2)."""
+        """b = scatter((2 H - S_H) area/3) (damage_solver.py:2249-2258; S_H = 0 for AT2)."""
         m = self.mesh
-        return scatter(m.elem_flat, np.repeat(2.0 * H * (m.areas / 3.0), 3), m.n_nodes)
+        src = self.at1_source if self.at1_source_e is None else self.at1_source_e
+        return scatter(m.elem_flat, np.repeat((2.0 * H - src) * (m.areas / 3.0), 3), m.n_nodes)
 
     def diagonal(self, H):
         """Jacobi diagonal Gc l0 diag_lap + 2 rc (damage_solver.py:2112-2119)."""

@@ -44,6 +44,8 @@ struct T3Mat {
     double lam, mu, kappa, ezz_coef, eta, Gc_l0, Gc_over_l0, l0;
     int32_t split, plane_stress;
     int32_t ssplit;  // split of the stress (isotropic under stress_degradation 'full')
+    int32_t at1;     // AT1 damage law (else AT2)
+    double src;      // AT1 source S_H of the right-hand side (0 for AT2)
 };
 
 struct T3Mesh {
@@ -359,7 +361,9 @@ __device__ __forceinline__ void t3_damage_corners(const T3Mesh &m, const T3Mat &
     t3_geometry(m.xy[nd[0]], m.xy[nd[1]], m.xy[nd[2]], area, gp);
     const double h = H[e];
     const double ge = Gc ? Gc[e] : 0.0;
-    const double ratio = Gc ? ge / M.l0 : M.Gc_over_l0;  // damage_solver.py:2168-2177
+    // per-element Gc: damage_solver.py:591-611 (AT1: 0.75 Gc l0, no Gc/l0, S_H = 3 Gc/(8 l0))
+    const double ratio = Gc ? (M.at1 ? 0.0 : ge / M.l0) : M.Gc_over_l0;  // damage_solver.py:2168-2177
+    const double gl = Gc ? (M.at1 ? 0.75 * ge : ge) * M.l0 : M.Gc_l0;
     const double rc = (2.0 * h + ratio) * area / 12.0;
     if (OP == 0) {
         const double d0 = x[nd[0]], d1 = x[nd[1]], d2 = x[nd[2]];
@@ -369,13 +373,14 @@ __device__ __forceinline__ void t3_damage_corners(const T3Mesh &m, const T3Mat &
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const double lap = area * (gp[a][0] * gdx + gp[a][1] * gdy);
-            c.set(a, Gc ? (ge * M.l0) * lap : lap, rc * ((a == 0 ? d0 : (a == 1 ? d1 : d2)) + ds));
+            c.set(a, Gc ? gl * lap : lap, rc * ((a == 0 ? d0 : (a == 1 ? d1 : d2)) + ds));
         }
     } else {
-        const double gl = Gc ? ge * M.l0 : M.Gc_l0;
+        const double src = Gc ? (M.at1 ? 3.0 * ge / (8.0 * M.l0) : 0.0) : M.src;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
-            c.set(a, 2.0 * h * (area / 3.0), gl * (area * (gp[a][0] * gp[a][0] + gp[a][1] * gp[a][1])) + 2.0 * rc);
+            c.set(a, (2.0 * h - src) * (area / 3.0),
+                  gl * (area * (gp[a][0] * gp[a][0] + gp[a][1] * gp[a][1])) + 2.0 * rc);
     }
 }
 
@@ -803,14 +808,15 @@ int t3_build_patches(phast_t3_plan *pl, int64_t nn, int64_t ne, const double *no
 
 extern "C" int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const double *nodes, const int32_t *elements,
                                     double E, double nu, double Gc, double l0, double eta_residual,
-                                    int32_t plane_stress, int32_t split, int32_t stress_split,
-                                    phast_t3_plan **out) {
+                                    int32_t plane_stress, int32_t split, int32_t stress_split, int32_t pf_model,
+                                    double at1_source, phast_t3_plan **out) {
     PHAST_REQUIRE(out && nodes && elements && n_nodes >= 3 && n_elems >= 1, "phast_t3_plan_create: bad argument");
     PHAST_REQUIRE(3 * n_elems < ((int64_t)1 << 31) && n_nodes < ((int64_t)1 << 31),
                   "phast_t3_plan_create: mesh too large for 32-bit incidence indices");
     PHAST_REQUIRE(split == T3_ISO || split == T3_AMOR || split == T3_SPECTRAL, "unknown T3 energy split %d", split);
     PHAST_REQUIRE(stress_split == T3_ISO || stress_split == T3_AMOR || stress_split == T3_SPECTRAL,
                   "unknown T3 stress split %d", stress_split);
+    PHAST_REQUIRE(pf_model == PHAST_PF_AT2 || pf_model == PHAST_PF_AT1, "unknown phase-field model %d", pf_model);
     for (int64_t k = 0; k < 3 * n_elems; ++k)
         PHAST_REQUIRE(elements[k] >= 0 && elements[k] < n_nodes, "element %lld refers to node %d outside the mesh",
                       (long long)(k / 3), elements[k]);
@@ -821,8 +827,11 @@ extern "C" int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const doub
     M.kappa = E / (3.0 * (1.0 - 2.0 * nu));
     M.ezz_coef = -nu / (1.0 - nu);
     M.eta = eta_residual;
-    M.Gc_l0 = Gc * l0;
-    M.Gc_over_l0 = Gc / l0;
+    // damage_solver.py:1321-1336: AT1 puts 3/4 Gc l0 on the Laplacian and no Gc/l0 in the reaction
+    M.at1 = pf_model == PHAST_PF_AT1;
+    M.Gc_l0 = M.at1 ? 0.75 * Gc * l0 : Gc * l0;
+    M.Gc_over_l0 = M.at1 ? 0.0 : Gc / l0;
+    M.src = M.at1 ? at1_source : 0.0;
     M.l0 = l0;
     M.split = split;
     M.ssplit = stress_split;

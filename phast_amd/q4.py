@@ -44,6 +44,7 @@ class Material:
     pf_model: str = "AT2"
     degradation_type: str = "standard"
     stress_degradation: str = "split"
+    at1_threshold: object = "auto"
 
     def __post_init__(self):
         if self.stress_degradation not in ("split", "full"):  # material.py:185-188
@@ -52,8 +53,19 @@ class Material:
             raise NotImplementedError(
                 f"energy_split={self.energy_split!r}: the HIP path implements 'amor', 'isotropic' and "
                 "'spectral' (T3 only)")
-        if self.pf_model != "AT2" or self.degradation_type != "standard":
-            raise NotImplementedError("the Q4 HIP path implements AT2 with the standard degradation")
+        if self.pf_model not in ("AT2", "AT1") or self.degradation_type != "standard":
+            raise NotImplementedError("the HIP path implements AT2 (and AT1 on T3 meshes) with the standard "
+                                      "degradation")
+
+    @property
+    def at1_source(self) -> float:
+        """AT1 source S_H of the damage right-hand side 2H - S_H: the override or 3 Gc/(8 l0);
+        0 for AT2 (material.py:260-271)."""
+        if self.pf_model != "AT1":
+            return 0.0
+        if isinstance(self.at1_threshold, str) and self.at1_threshold == "auto":
+            return 3.0 * self.Gc / (8.0 * self.l0)
+        return float(self.at1_threshold)
 
     @property
     def stress_split(self) -> str:
@@ -183,6 +195,8 @@ def q4_desc(mesh: FEMMesh, mat: Material, dt: float = 0.0) -> Q4Desc:
     if mat.energy_split not in ("amor", "isotropic"):
         raise NotImplementedError(f"energy_split={mat.energy_split!r}: the Q4 HIP path implements 'amor' and "
                                   "'isotropic'")
+    if mat.pf_model != "AT2":  # damage_solver.py:1256-1263: the native Q4 damage is AT2 only
+        raise NotImplementedError("native Q4 damage supports only pf_model='AT2'")
     d.split = 0 if mat.energy_split == "amor" else 1
     d.stress_split = 0 if mat.stress_split == "amor" else 1
     d.eta_residual, d.Gc, d.l0, d.dt = mat.eta_residual, mat.Gc, mat.l0, float(dt)

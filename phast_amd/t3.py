@@ -36,7 +36,8 @@ class T3Plan:
         check(_lib.lib().phast_t3_plan_create(
             len(nodes), len(els), nodes.ctypes.data, els.ctypes.data, material.E, material.nu, material.Gc,
             material.l0, material.eta_residual, int(bool(material.plane_stress)),
-            _SPLIT_ID[material.energy_split], _SPLIT_ID[material.stress_split], C.byref(h)), "phast_t3_plan_create")
+            _SPLIT_ID[material.energy_split], _SPLIT_ID[material.stress_split], 1 if material.pf_model == "AT1" else 0,
+            material.at1_source, C.byref(h)), "phast_t3_plan_create")
         self.handle = h
 
     def __del__(self):

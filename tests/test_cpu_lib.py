@@ -62,5 +62,5 @@ def test_t3_plan_validates_connectivity_before_any_device_call():
     for els, split in ((np.array([[0, 1, 3]], dtype=np.int32), 0), (np.array([[0, 1, -1]], dtype=np.int32), 0),
                        (np.array([[0, 1, 2]], dtype=np.int32), 7)):
         rc = L.phast_t3_plan_create(3, 1, nodes.ctypes.data, els.ctypes.data, 1.0, 0.2, 1.0, 1.0, 1e-7, 0, split,
-                                    split, ctypes.byref(h))
+                                    split, 0, 0.0, ctypes.byref(h))
         assert rc == _lib.PHAST_ERR_ARG and not h.value

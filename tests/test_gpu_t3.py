@@ -177,6 +177,33 @@ def test_damage_operator(dev):
         assert exact(b, odmg.rhs(H)) and exact(dg, odmg.diagonal(H))
 
 
+def test_at1_damage(dev):
+    """AT1 (damage_solver.py:1321-1336): operator pieces bit-exact, solve to the oracle's iterations."""
+    from phast_amd import t3 as P
+    nodes, els = scrambled_mesh(23, 19, seed=30)
+    kw = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5, pf_model="AT1")
+    fem = P.FEMOperators(P.FEMMesh(nodes, els), P.Material(**kw))
+    omesh, omat = O.T3Mesh(nodes, els), OMaterial(**kw)
+    rng = np.random.default_rng(31)
+    n, E = len(nodes), len(els)
+    H = rng.uniform(0, 0.02, E)
+    d = rng.uniform(0, 1, n)
+    Gc_e = rng.uniform(1e-3, 5e-3, E)
+    for gc in (None, Gc_e):
+        dmg = P.PhaseFieldDamageSolver(fem, Gc_field=None if gc is None else T(gc, dev))
+        od = O.T3Damage(omesh, omat, gc)
+        assert exact(dmg.damage_apply(T(H, dev), T(d, dev)), od.apply(d, H))
+        b, dg = dmg.damage_rhs_diag(T(H, dev))
+        assert exact(b, od.rhs(H)) and exact(dg, od.diagonal(H))
+    d_prev = np.clip(rng.uniform(-0.3, 0.5, n), 0, 1)
+    dmg = P.PhaseFieldDamageSolver(fem, tol=1e-9, max_iter=3000, bounds_method="projected_cg")
+    dd = dmg.solve(T(H, dev), T(d_prev, dev))
+    osol = O.T3DamageSolver(omesh, omat, tol=1e-9, max_iter=3000)
+    ods = osol.solve(H, d_prev)
+    assert dmg.last_iter == osol.last_iter and rel(dd, ods) < RTOL
+    assert np.all(dd.cpu().numpy() >= d_prev)
+
+
 def test_damage_operator_dense_small(dev):
     """Columns of the device operator on a tiny mesh form the symmetric textbook matrix."""
     nodes, els = scrambled_mesh(3, 2, seed=8)

@@ -266,3 +266,34 @@ def test_kelvin_voigt_damping_dissipates():
     e0, e1 = run(0.0), run(0.2)
     assert e0[-50:].mean() > 0.5 * e0[:50].mean()
     assert e1[-50:].mean() < 0.1 * e1[:50].mean()
+
+
+def test_at1_operator_dense_and_uniform_history():
+    """AT1: 3/4 Gc l0 Laplacian, reaction 2H only, source S_H = 3 Gc/(8 l0) (damage_solver.py:1321-1336)."""
+    mm = mesh(5, 4)
+    kw = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5)
+    m1 = Material(pf_model="AT1", **kw)
+    dm = O.T3Damage(mm, m1)
+    rng = np.random.default_rng(11)
+    H = rng.uniform(0, 1e-2, mm.n_elems)
+    n = mm.n_nodes
+    K = np.zeros((n, n))
+    Mc = np.array([[2.0, 1.0, 1.0], [1.0, 2.0, 1.0], [1.0, 1.0, 2.0]]) / 12.0
+    for e, nodes in enumerate(mm.elements):
+        gp = mm.grad_phi[e]
+        K[np.ix_(nodes, nodes)] += 0.75 * 3e-3 * 0.5 * mm.areas[e] * gp @ gp.T + 2.0 * H[e] * mm.areas[e] * Mc
+    assert np.abs(dm.dense(H) - K).max() < 1e-13 * np.abs(K).max()
+    src = 3.0 * 3e-3 / (8.0 * 0.5)
+    assert dm.at1_source == pytest.approx(src, rel=1e-15)
+    assert dm.rhs(H).sum() == pytest.approx(float(((2.0 * H - src) * mm.areas).sum()), rel=1e-12)
+    # uniform H above the threshold: d = 1 - S_H/(2H); below it the bound d >= 0 holds d = 0
+    for Hc, expect in ((4.0 * src, 1.0 - 1.0 / 8.0), (0.2 * src, 0.0)):
+        s = O.T3DamageSolver(O.T3Mesh(mm.nodes, mm.elements), m1, tol=1e-13)
+        d = s.solve(np.full(mm.n_elems, Hc), np.zeros(n))
+        np.testing.assert_allclose(d, expect, atol=1e-10)
+    # a uniform per-element Gc reproduces the scalar-Gc AT1 system
+    dme = O.T3Damage(mm, m1, np.full(mm.n_elems, 3e-3))
+    x = np.linspace(0.0, 1.0, n)
+    np.testing.assert_allclose(dme.apply(x, H), dm.apply(x, H), rtol=1e-13)
+    np.testing.assert_allclose(dme.rhs(H), dm.rhs(H), rtol=1e-13)
+    np.testing.assert_allclose(dme.diagonal(H), dm.diagonal(H), rtol=1e-13)
