@@ -32,8 +32,18 @@ __device__ __forceinline__ double reduce_sum_parts(const double *part, int off, 
     double w[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
+        // lane sums part[k], part[k+256], ... in that order; four loads in flight at a time
         double s = 0.0;
-        for (int k = threadIdx.x + v * NT; k < cnt; k += 256) s += part[off + k];
+        int k = threadIdx.x + v * NT;
+        for (; k + 3 * 256 < cnt; k += 4 * 256) {
+            const double a = part[off + k], b = part[off + k + 256], c = part[off + k + 512],
+                         e = part[off + k + 768];
+            s += a;
+            s += b;
+            s += c;
+            s += e;
+        }
+        for (; k < cnt; k += 256) s += part[off + k];
         w[v] = wave_sum(s);
     }
     __syncthreads();
@@ -46,9 +56,16 @@ __device__ __forceinline__ double reduce_sum_parts(const double *part, int off, 
 }
 template <int NT>
 __device__ __forceinline__ double reduce_min_parts(const double *part, int off, int cnt, double *red) {
-    double s = __builtin_huge_val();
-    for (int k = threadIdx.x; k < cnt; k += NT) s = fmin(s, part[off + k]);
-    return block_min<NT>(s, red);
+    double s = __builtin_huge_val(), s2 = s, s3 = s, s4 = s;  // the minimum is exact in any order
+    int k = threadIdx.x;
+    for (; k + 3 * NT < cnt; k += 4 * NT) {
+        s = fmin(s, part[off + k]);
+        s2 = fmin(s2, part[off + k + NT]);
+        s3 = fmin(s3, part[off + k + 2 * NT]);
+        s4 = fmin(s4, part[off + k + 3 * NT]);
+    }
+    for (; k < cnt; k += NT) s = fmin(s, part[off + k]);
+    return block_min<NT>(fmin(fmin(s, s2), fmin(s3, s4)), red);
 }
 
 // One per-instance scalar step of the CG (thread 0 writes S).  Every thread of the block
