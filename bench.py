@@ -51,6 +51,9 @@ T3_DAMAGE_BYTES_PER_NODE = 72.0  # x 8 + y 8 + H 16 + xy 16 + conn 24
 # Ap 8), d/r update with the preconditioner (41 read + 24 written), direction update (z, p,
 # active 17 + p 8), the residual check of every 10th iteration (~90 B) 9 on average
 T3_CG_BYTES_PER_NODE = 88.0 + 65.0 + 25.0 + 9.0
+# one iteration of the adjoint PCG of the backward: A p (p 8, H 16, xy 16, conn 24; Ap 8
+# written), x/r/z update (x, p, r, Ap, Minv read 40; x, r, z written 24), direction (z, p 16; p 8)
+T3_ADJ_BYTES_PER_NODE = 72.0 + 64.0 + 24.0
 
 
 def parse():
@@ -323,7 +326,7 @@ def q4_cg_iteration(P, fem, rng, T):
 def t3_secondary(rng, dev, a):
     """T3 route on a 4096 x 2048-cell split-quad plate (8.4 M nodes, 16.8 M triangles; the
     mesh of core/mesh_generator.py:1552-1633 at h = 0.125): internal force, explicit step
-    with psi+ and history, damage apply."""
+    with psi+ and history, damage apply, one damage-CG iteration and one adjoint-PCG iteration."""
     import numpy as np
     import torch
     from phast_amd import t3 as P3
@@ -362,7 +365,23 @@ def t3_secondary(rng, dev, a):
     if walls[60][1] > walls[20][1]:
         t_it = (walls[60][0] - walls[20][0]) / (walls[60][1] - walls[20][1])
         entry.update({"cg_iteration_ms": t_it * 1e3, "cg_iteration_GB_s": T3_CG_BYTES_PER_NODE * n / t_it / 1e9})
-    del fem, dyn, dmg, u, d, v, acc, fext, H, Hd
+    # per-iteration cost of the adjoint PCG that the backward runs twice (rel_tol 0: fixed count)
+    g = T(rng.uniform(-1.0, 1.0, n))
+    adj = P3.PhaseFieldDamageSolver(fem)
+    walls = {}
+    for k in (20, 60):
+        adj.max_iter = k
+        adj.adjoint_pcg(Hd, g, rel_tol=0.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        adj.adjoint_pcg(Hd, g, rel_tol=0.0)
+        torch.cuda.synchronize()
+        walls[k] = (time.perf_counter() - t0, adj.last_adjoint_iter)
+    if walls[60][1] > walls[20][1]:
+        t_it = (walls[60][0] - walls[20][0]) / (walls[60][1] - walls[20][1])
+        entry.update({"adjoint_pcg_iteration_ms": t_it * 1e3,
+                      "adjoint_pcg_iteration_GB_s": T3_ADJ_BYTES_PER_NODE * n / t_it / 1e9})
+    del fem, dyn, dmg, adj, u, d, v, acc, fext, H, Hd, g
     torch.cuda.empty_cache()
     return entry
 

@@ -262,6 +262,25 @@ int phast_t3_damage_solve(phast_t3_plan *plan, int32_t method, const double *H, 
                           const double *d_prev, const double *d_init, const uint8_t *fixed, const double *fixed_vals,
                           double tol, int32_t max_iter, double *d_out, uint8_t *active_out, int32_t *iters,
                           int32_t *status, hipStream_t stream);
+/* Replace the plan's fracture constants (Gc, l0, and the AT1 source S_H) for the launches
+ * enqueued after the call: the temporary material override that _AdjointDamageSolveScalar
+ * installs around its forward and adjoint solves (damage_solver.py:574-585, 686-696). */
+int phast_t3_set_fracture(phast_t3_plan *plan, double Gc, double l0, double at1_source);
+/* Jacobi-PCG of the adjoint recomputation, _pcg_solve_adjoint_system
+ * (damage_solver.py:423-478): solves A x = rhs with the forward operator at (H, Gc_e) and
+ * M_inv = 1/(diag + 1e-30); rhs null = the forward right-hand side b (the unconstrained
+ * damage d_unc).  fixed (uint8 (N,)) pins are eliminated and held at fixed_vals (0 when
+ * fixed_vals is null); x_init optional (else 0).  Stops when r.r < rel_tol max(rhs.rhs,
+ * 1e-30), on p.Ap <= 0, or after max_iter iterations.  iters/status are host int32. */
+int phast_t3_adjoint_solve(phast_t3_plan *plan, const double *H, const double *Gc_e, const double *rhs,
+                           const uint8_t *fixed, const double *fixed_vals, const double *x_init, double rel_tol,
+                           int32_t max_iter, double *x_out, int32_t *iters, int32_t *status, hipStream_t stream);
+/* Per-triangle adjoint terms (E,), each output optional: the history gradient
+ * gH = sum_a lam_a 2A (1/3 - (dp_a + sum dp)/12) with dp = d_pull (damage_solver.py:800-810,
+ * 1140-1149), and the quadratic forms of _adjoint_quadratic_forms_perelem (:284-331)
+ * lKd = A grad lam . grad d_unc, lMd = A/12 (sum lam sum d_unc + lam . d_unc), lM1 = A/3 sum lam. */
+int phast_t3_adjoint_terms(const phast_t3_plan *plan, const double *lam, const double *d_pull, const double *d_unc,
+                           double *gH, double *lKd, double *lMd, double *lM1, hipStream_t stream);
 
 /* =====================================================================================
  * 3D periodic hex8 voxel cells

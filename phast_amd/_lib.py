@@ -81,6 +81,9 @@ _SIGNATURES = {
     "phast_t3_damage_apply": ([_P] * 6, C.c_int),
     "phast_t3_damage_rhs_diag": ([_P] * 6, C.c_int),
     "phast_t3_damage_solve": ([_P, _I32] + [_P] * 6 + [_D, _I32] + [_P] * 5, C.c_int),
+    "phast_t3_set_fracture": ([_P, _D, _D, _D], C.c_int),
+    "phast_t3_adjoint_solve": ([_P] * 7 + [_D, _I32] + [_P] * 4, C.c_int),
+    "phast_t3_adjoint_terms": ([_P] * 9, C.c_int),
     "phast_hex8_plan_create": ([_I32, _I32, _I32, _D, _D, _D, _P, _I32, _P, _P, _P, _P, _P, _D, _D,
                                 C.POINTER(_P)], C.c_int),
     "phast_hex8_plan_destroy": ([_P], C.c_int),

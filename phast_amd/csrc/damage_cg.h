@@ -19,7 +19,9 @@ struct DamageCG {
 // partial row; row k of the partials starts at k * n_tiles)
 // ---------------------------------------------------------------------------------------
 
-enum ScalarOp : int { S_INIT = 0, S_A = 1, S_B = 2, S_C = 3, S_PC_INIT = 4, S_PC_A = 5, S_PC_C = 6 };
+enum ScalarOp : int { S_INIT = 0, S_A = 1, S_B = 2, S_C = 3, S_PC_INIT = 4, S_PC_A = 5, S_PC_C = 6,
+                      // plain Jacobi-PCG of the adjoint recomputation (damage_solver.py:423-478)
+                      S_AJ_INIT = 7, S_AJ_A = 8, S_AJ_C = 9 };
 
 // Sum of part[off .. off+cnt) in one fixed order whatever the block width NT: 256 virtual
 // lanes, lane k summing part[k], part[k+256], ...; each virtual wave of 64 lanes reduced by
@@ -143,6 +145,48 @@ __device__ __forceinline__ void scalar_step(int32_t off, int32_t cnt, DamageCG &
         const double rz_new = reduce_sum_parts<NT>(part, off, cnt, red);
         if (threadIdx.x == 0) {
             S.beta = S.restart ? 0.0 : rz_new / (S.rz + 1e-30);
+            S.rz = rz_new;
+            if (S.iter >= S.max_iter) {
+                S.done = 1;
+                S.status = PHAST_CG_MAXITER;
+            }
+        }
+    } else if (OP == S_AJ_INIT) {
+        // rows: r.r (unused), rhs.rhs, r.z; tol_sq = rel_tol * max(rhs.rhs, 1e-30) with S.tol = rel_tol
+        const double bb = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
+        const double rz = reduce_sum_parts<NT>(part, 2 * n_tiles + off, cnt, red);
+        if (threadIdx.x == 0) {
+            S.bb = bb;
+            S.rz = rz;
+            S.tol_sq = S.tol * fmax(bb, 1e-30);
+            S.iter = 0;
+            S.done = S.max_iter <= 0 ? 1 : 0;
+            S.status = S.max_iter <= 0 ? PHAST_CG_MAXITER : PHAST_CG_RUNNING;
+            S.limited = S.check = S.restart = 0;
+        }
+    } else if (OP == S_AJ_A) {
+        const double pAp = reduce_sum_parts<NT>(part, off, cnt, red);
+        if (threadIdx.x == 0) {
+            S.iter += 1;
+            S.pAp = pAp;
+            if (pAp <= 0.0) {  // `if pAp.item() <= 0: break`, before the update
+                S.done = 1;
+                S.status = PHAST_CG_NONPOSITIVE;
+                return;
+            }
+            S.alpha = S.rz / (pAp + 1e-30);
+        }
+    } else if (OP == S_AJ_C) {
+        const double rr = reduce_sum_parts<NT>(part, off, cnt, red);
+        const double rz_new = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
+        if (threadIdx.x == 0) {
+            S.rfn = rr;
+            if (rr < S.tol_sq) {  // converged after the update, before the direction
+                S.done = 1;
+                S.status = PHAST_CG_CONVERGED;
+                return;
+            }
+            S.beta = rz_new / (S.rz + 1e-30);
             S.rz = rz_new;
             if (S.iter >= S.max_iter) {
                 S.done = 1;

@@ -444,6 +444,41 @@ __global__ __launch_bounds__(256) void t3_e2n_kernel(const T3Mesh m, const doubl
     out[n] = s / (3.0 * Mn[n] + 1e-30);
 }
 
+// Adjoint terms of the damage solve, one thread per triangle (damage_solver.py:284-331 and the
+// history pullback of :800-810, :1140-1149).  With lam, dp (the damage of the history
+// pullback) and du (the unconstrained damage of the quadratic forms) at the corners:
+//   gH  = sum_a lam_a 2A (1/3 - (dp_a + sum dp) / 12)
+//   lKd = A (grad lam . grad du),  lMd = A/12 (sum lam sum du + lam . du),  lM1 = A/3 sum lam
+__global__ __launch_bounds__(256) void t3_adjoint_terms_kernel(const T3Mesh m, const double *lam, const double *dp,
+                                                               const double *du, double *gH, double *lKd,
+                                                               double *lMd, double *lM1) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m.n_elems) return;
+    const int32_t n0 = m.el[3 * e], n1 = m.el[3 * e + 1], n2 = m.el[3 * e + 2];
+    double area, gp[3][2];
+    t3_geometry(m.xy[n0], m.xy[n1], m.xy[n2], area, gp);
+    const double l0 = lam[n0], l1 = lam[n1], l2 = lam[n2];
+    const double ls = (l0 + l1) + l2;
+    if (gH) {
+        const double a0 = dp[n0], a1 = dp[n1], a2 = dp[n2];
+        const double ds = (a0 + a1) + a2, w = 2.0 * area;
+        gH[e] = (l0 * (w * (1.0 / 3.0 - (a0 + ds) / 12.0)) + l1 * (w * (1.0 / 3.0 - (a1 + ds) / 12.0))) +
+                l2 * (w * (1.0 / 3.0 - (a2 + ds) / 12.0));
+    }
+    if (lKd || lMd) {
+        const double d0 = du[n0], d1 = du[n1], d2 = du[n2];
+        if (lKd) {
+            const double glx = (gp[0][0] * l0 + gp[1][0] * l1) + gp[2][0] * l2;
+            const double gly = (gp[0][1] * l0 + gp[1][1] * l1) + gp[2][1] * l2;
+            const double gdx = (gp[0][0] * d0 + gp[1][0] * d1) + gp[2][0] * d2;
+            const double gdy = (gp[0][1] * d0 + gp[1][1] * d1) + gp[2][1] * d2;
+            lKd[e] = area * (glx * gdx + gly * gdy);
+        }
+        if (lMd) lMd[e] = (area / 12.0) * (ls * ((d0 + d1) + d2) + ((l0 * d0 + l1 * d1) + l2 * d2));
+    }
+    if (lM1) lM1[e] = (area / 3.0) * ls;
+}
+
 // ---------------------------------------------------------------------------------------
 // device-resident projected / post-clamp Jacobi CG on the T3 operator (the Q4 solve's state
 // machine and scalar steps; one thread per node, per-block partial sums in part[k*nblk+b])
@@ -475,6 +510,12 @@ enum T3VecOp : int {
     TV_PRECOND_CHK = 10,
     TV_PC_UPDATE_PRE = 11,
     TV_PC_PRECOND_CHK = 12,
+    // plain Jacobi-PCG of the adjoint recomputation (damage_solver.py:423-478); pinned DOFs
+    // are eliminated: held at vals (0 when no values are given), r = z = p = 0 there
+    TA_SETUP = 13,     // x = fixed ? vals : (init or 0)
+    TA_UPDATE = 14,    // x += a p, r -= a Ap, r = 0 at pins, z = M_eff r ; partials r.r, r.z
+    TA_UPDATE_P = 15,  // p = z + beta p, 0 at pins
+    TA_FINISH = 16,    // x = fixed ? vals : x
 };
 
 struct T3Cg {
@@ -584,7 +625,8 @@ template <int OP0>
 __global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
     __shared__ double red[4];
     const DamageCG &S = A.cg[0];
-    if (OP0 != TV_SETUP && OP0 != TV_FINISH && OP0 != TV_PC_FINISH && S.done) return;
+    if (OP0 != TV_SETUP && OP0 != TV_FINISH && OP0 != TV_PC_FINISH && OP0 != TA_SETUP && OP0 != TA_FINISH && S.done)
+        return;
     if (OP0 == TV_PRECOND_CHK && !(S.limited || S.check)) return;  // done by TV_UPDATE_D_PRE
     if (OP0 == TV_PC_PRECOND_CHK && !S.check) return;             // done by TV_PC_UPDATE_PRE
     constexpr int OP = OP0 == TV_PRECOND_CHK ? TV_PRECOND
@@ -663,12 +705,31 @@ __global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
             p1 = rr * zz;
         } else if (OP == TV_PC_UPDATE_P) {
             A.p[q] = A.p[q] * S.beta + A.z[q];
+        } else if (OP == TA_SETUP || OP == TA_FINISH) {
+            const bool fx = A.fixed && A.fixed[q];
+            const double v = A.vals ? A.vals[q] : 0.0;
+            const double x = OP == TA_FINISH ? A.d[q] : (A.init ? A.init[q] : 0.0);
+            A.d[q] = fx ? v : x;
+        } else if (OP == TA_UPDATE) {
+            const bool fx = A.fixed && A.fixed[q];
+            A.d[q] = A.d[q] + S.alpha * A.p[q];
+            double rn = A.r[q] - S.alpha * A.Ap[q];
+            if (fx) rn = 0.0;
+            A.r[q] = rn;
+            const double zz = rn * (fx ? 0.0 : A.minv[q]);
+            A.z[q] = zz;
+            p0 = rn * rn;
+            p1 = rn * zz;
+        } else if (OP == TA_UPDATE_P) {
+            double pn = A.z[q] + S.beta * A.p[q];
+            if (A.fixed && A.fixed[q]) pn = 0.0;
+            A.p[q] = pn;
         }
     }
-    if (OP == TV_PRECOND || OP == TV_PC_PRECOND || fuse) {
+    if (OP == TV_PRECOND || OP == TV_PC_PRECOND || OP == TA_UPDATE || fuse) {
         const double s0 = block_sum<256>(p0, red);
         if (threadIdx.x == 0) A.part[blockIdx.x] = s0;
-        if (OP == TV_PC_PRECOND || (OP0 == TV_PC_UPDATE_PRE && fuse)) {
+        if (OP == TV_PC_PRECOND || OP == TA_UPDATE || (OP0 == TV_PC_UPDATE_PRE && fuse)) {
             const double s1 = block_sum<256>(p1, red);
             if (threadIdx.x == 0) A.part[A.nblk + blockIdx.x] = s1;
         }
@@ -680,7 +741,7 @@ __global__ __launch_bounds__(256) void t3_scalar_kernel(DamageCG *cg, const doub
                                                         int32_t check_every) {
     __shared__ double red[4];
     DamageCG &S = cg[0];
-    if (OP != S_INIT && OP != S_PC_INIT && S.done) return;
+    if (OP != S_INIT && OP != S_PC_INIT && OP != S_AJ_INIT && S.done) return;
     scalar_step<OP, 256>(0, nblk, S, part, nblk, check_every, red);
 }
 
@@ -1157,5 +1218,94 @@ extern "C" int phast_t3_damage_solve(phast_t3_plan *pl, int32_t method, const do
     T3_TRY(all_done());
     if (iters) *iters = h.iter;
     if (status) *status = h.done ? h.status : PHAST_CG_MAXITER;
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_set_fracture(phast_t3_plan *pl, double Gc, double l0, double at1_source) {
+    PHAST_REQUIRE(pl, "phast_t3_set_fracture: null plan");
+    PHAST_REQUIRE(Gc > 0.0 && l0 > 0.0 && std::isfinite(Gc) && std::isfinite(l0), "Gc and l0 must be positive");
+    T3Mat &M = pl->mat;  // as in phast_t3_plan_create (damage_solver.py:574-585)
+    M.Gc_l0 = M.at1 ? 0.75 * Gc * l0 : Gc * l0;
+    M.Gc_over_l0 = M.at1 ? 0.0 : Gc / l0;
+    M.src = M.at1 ? at1_source : 0.0;
+    M.l0 = l0;
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_adjoint_solve(phast_t3_plan *pl, const double *H, const double *Gc_e, const double *rhs,
+                                      const uint8_t *fixed, const double *fixed_vals, const double *x_init,
+                                      double rel_tol, int32_t max_iter, double *x_out, int32_t *iters,
+                                      int32_t *status, hipStream_t st) {
+    PHAST_REQUIRE(pl && H && x_out, "phast_t3_adjoint_solve: null argument");
+    PHAST_REQUIRE(rel_tol >= 0.0 && max_iter >= 0, "rel_tol and max_iter must be non-negative");
+    PHAST_REQUIRE(fixed || !fixed_vals, "fixed_vals needs fixed");
+    PHAST_REQUIRE(x_out != rhs && x_out != H && x_out != Gc_e, "x_out must not alias rhs, H or Gc_e");
+    T3_TRY(t3_ensure_work(pl));
+    DamageCG &h = *pl->h_cg;
+    std::memset(&h, 0, sizeof(h));
+    h.tol = rel_tol;
+    h.max_iter = max_iter;
+    PHAST_HIP(hipMemcpyAsync(pl->cg, pl->h_cg, sizeof(DamageCG), hipMemcpyHostToDevice, st));
+
+    T3Cg A;
+    std::memset(&A, 0, sizeof(A));
+    A.H = H;
+    A.Gc = Gc_e;
+    A.d = x_out;
+    A.d_out = x_out;
+    A.r = pl->r;
+    A.z = pl->z;
+    A.p = pl->p;
+    A.Ap = pl->Ap;
+    A.b = rhs ? rhs : pl->b;
+    A.minv = pl->minv;
+    A.init = x_init;
+    A.vals = fixed_vals;
+    A.fixed = fixed;
+    A.cg = pl->cg;
+    A.part = pl->part;
+    A.nblk = pl->nblk;
+
+    // M_inv = 1/(diag + 1e-30) of the forward system, and its b when no right-hand side is given
+    T3_TRY(launch_damage<2>(pl, H, Gc_e, nullptr, pl->b, pl->minv, st));
+    T3_TRY(t3_vec<TA_SETUP>(pl, A, st));
+    T3_TRY(t3_cg<T_PC_INIT>(pl, A, st));  // r = rhs - A x (0 at pins), z = M_eff r, p = z
+    T3_TRY(t3_scalar<S_AJ_INIT>(pl, 0, st));
+    const int poll = 4;
+    int it = 0;
+    auto sync_state = [&]() -> int {
+        PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(DamageCG), hipMemcpyDeviceToHost, st));
+        PHAST_HIP(hipStreamSynchronize(st));
+        return PHAST_OK;
+    };
+    while (it < max_iter) {
+        T3_TRY(t3_cg<T_PC_AP>(pl, A, st));
+        T3_TRY(t3_scalar<S_AJ_A>(pl, 0, st));
+        T3_TRY(t3_vec<TA_UPDATE>(pl, A, st));
+        T3_TRY(t3_scalar<S_AJ_C>(pl, 0, st));
+        T3_TRY(t3_vec<TA_UPDATE_P>(pl, A, st));
+        ++it;
+        if (it % poll == 0 || it == max_iter) {
+            T3_TRY(sync_state());
+            if (h.done) break;
+        }
+    }
+    T3_TRY(t3_vec<TA_FINISH>(pl, A, st));
+    T3_TRY(sync_state());
+    if (iters) *iters = h.iter;
+    if (status) *status = h.done ? h.status : PHAST_CG_MAXITER;
+    return PHAST_OK;
+}
+
+extern "C" int phast_t3_adjoint_terms(const phast_t3_plan *pl, const double *lam, const double *d_pull,
+                                      const double *d_unc, double *gH, double *lKd, double *lMd, double *lM1,
+                                      hipStream_t st) {
+    PHAST_REQUIRE(pl && lam, "phast_t3_adjoint_terms: null argument");
+    PHAST_REQUIRE(!gH || d_pull, "gH needs d_pull");
+    PHAST_REQUIRE(!(lKd || lMd) || d_unc, "lKd and lMd need d_unc");
+    const int64_t ne = pl->mesh.n_elems;
+    hipLaunchKernelGGL(t3_adjoint_terms_kernel, dim3((unsigned)ceil_div(ne, 256)), dim3(256), 0, st, pl->mesh, lam,
+                       d_pull, d_unc, gH, lKd, lMd, lM1);
+    PHAST_LAUNCHED();
     return PHAST_OK;
 }

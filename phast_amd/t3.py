@@ -336,28 +336,96 @@ class PhaseFieldDamageSolver:
                                                stream()), "damage_apply")
         return y
 
+    # ------------------------------------------------------------------------------------
+    # solve: dispatch to a forward solve or one of the three adjoint rules
+    # ------------------------------------------------------------------------------------
+
     def solve(self, H_input, d_prev, Gc=None, l0=None, Gc_field=None, pf_dirichlet_mask=None,
               pf_dirichlet_values=None, initial_guess=None):
-        """damage_solver.py:2493-2640 (forward, non-differentiable)."""
+        """damage_solver.py:2493-2640.  Without ``differentiable`` or a gradient-tracked Gc, l0 or
+        Gc_field the solve is forward only; otherwise it runs under the matching adjoint rule
+        (``_AdjointDamageSolve``, ``_AdjointDamageSolveScalar``, ``_AdjointDamageSolveField``),
+        whose backward costs two more device-resident CG solves."""
         self.n_solves += 1
-        if Gc is not None or l0 is not None:
-            raise NotImplementedError("scalar Gc/l0 overrides: rebuild the operators with the new material")
-        if any(isinstance(t, torch.Tensor) and t.requires_grad for t in (H_input, d_prev, Gc_field)):
-            raise NotImplementedError("differentiable T3 damage solves are not on the HIP path")
+        self._initial_guess = initial_guess
+        try:
+            tg = _needs_grad
+            Gc_g, l0_g, field_g = tg(Gc), tg(l0), tg(Gc_field)
+            diff = bool(getattr(self, "differentiable", False))
+            if pf_dirichlet_mask is not None and not (diff or Gc_g or l0_g or field_g):
+                return self._forward(H_input, d_prev, Gc, l0, Gc_field, pf_dirichlet_mask, pf_dirichlet_values)
+            if Gc_field is not None:
+                if Gc is not None:
+                    raise ValueError("Pass either Gc (scalar) or Gc_field (per-element), not both.")
+                if l0 is None:
+                    l0 = torch.tensor(self.material.l0, dtype=F64)
+                if not (diff or field_g or l0_g or tg(H_input) or tg(d_prev)):
+                    return self._forward(H_input, d_prev, None, l0, Gc_field, pf_dirichlet_mask,
+                                         pf_dirichlet_values)
+                return _AdjointDamageSolveField.apply(self, H_input, d_prev, Gc_field, l0, pf_dirichlet_mask,
+                                                      pf_dirichlet_values)
+            if Gc is not None or l0 is not None:
+                if Gc is None:
+                    Gc = torch.tensor(self.material.Gc, dtype=F64)
+                if l0 is None:
+                    l0 = torch.tensor(self.material.l0, dtype=F64)
+                if not (diff or Gc_g or l0_g or tg(H_input) or tg(d_prev)):
+                    return self._forward(H_input, d_prev, Gc, l0, None, pf_dirichlet_mask, pf_dirichlet_values)
+                return _AdjointDamageSolveScalar.apply(self, H_input, d_prev, Gc, l0, pf_dirichlet_mask,
+                                                       pf_dirichlet_values)
+            if not diff:
+                return self._forward(H_input, d_prev, None, None, None, pf_dirichlet_mask, pf_dirichlet_values)
+            return _AdjointDamageSolve.apply(self, H_input, d_prev, pf_dirichlet_mask, pf_dirichlet_values)
+        finally:
+            self._initial_guess = None
+
+    def _pins(self, mask, values, need_values=True):
+        """(uint8 mask, values) on the device, or (None, None) (damage_solver.py:2350-2360).  The
+        adjoint solve takes a mask without values (pins held at 0)."""
+        if mask is None or not bool(mask.any()):
+            return None, None
+        if values is None and need_values:
+            raise ValueError("pf_dirichlet_values must be supplied with mask")
         fem = self.fem
-        d_prev = fem._d(d_prev, "d_prev")
+        fixed = fem._d(mask.detach().to(F64), "pf_dirichlet_mask").to(torch.uint8)
+        return fixed, None if values is None else fem._d(values.detach().to(F64), "pf_dirichlet_values")
+
+    def _install(self, Gc=None, l0=None):
+        """Temporary fracture constants (damage_solver.py:574-585): AT1 then uses
+        S_H = 3 Gc/(8 l0).  Returns True when constants were replaced."""
+        if Gc is None and l0 is None:
+            return False
+        m = self.material
+        Gcv = m.Gc if Gc is None else float(Gc)
+        l0v = m.l0 if l0 is None else float(l0)
+        check(_lib.lib().phast_t3_set_fracture(self.fem.plan.handle, Gcv, l0v, 3.0 * Gcv / (8.0 * l0v)),
+              "set_fracture")
+        return True
+
+    def _restore(self):
+        m = self.material
+        check(_lib.lib().phast_t3_set_fracture(self.fem.plan.handle, m.Gc, m.l0, m.at1_source), "set_fracture")
+
+    def _forward(self, H_input, d_prev, Gc, l0, Gc_field, mask, values):
+        """One device solve with optional temporary Gc / l0 (damage_solver.py:2376-2420, 2642-2684)."""
+        installed = self._install(None if Gc is None else _scalar(Gc), None if l0 is None else _scalar(l0))
+        try:
+            return self._solve_device(H_input, d_prev, Gc_field, mask, values)
+        finally:
+            if installed:
+                self._restore()
+
+    def _solve_device(self, H_input, d_prev, Gc_field, mask, values):
+        fem = self.fem
+        d_prev = fem._d(d_prev.detach(), "d_prev")
         if not bool(torch.isfinite(d_prev).all()):  # damage_solver.py:2647-2650
             self.last_iter, self.last_converged = 0, False
             return d_prev.clone()
-        H = fem._e(H_input, "H_input")
-        gcf = self.Gc_field if Gc_field is None else fem._e(Gc_field, "Gc_field")
-        fixed = vals = None
-        if pf_dirichlet_mask is not None and bool(pf_dirichlet_mask.any()):
-            if pf_dirichlet_values is None:
-                raise ValueError("pf_dirichlet_values must be supplied with mask")
-            fixed = fem._d(pf_dirichlet_mask.to(F64), "pf_dirichlet_mask").to(torch.uint8)
-            vals = fem._d(pf_dirichlet_values.to(F64), "pf_dirichlet_values")
-        init = None if initial_guess is None else fem._d(initial_guess.to(F64), "initial_guess")
+        H = fem._e(H_input.detach(), "H_input")
+        gcf = self.Gc_field if Gc_field is None else fem._e(Gc_field.detach(), "Gc_field")
+        fixed, vals = self._pins(mask, values)
+        ig = getattr(self, "_initial_guess", None)
+        init = None if ig is None else fem._d(ig.detach().to(F64), "initial_guess")
         d_out = torch.empty_like(d_prev)
         method = 0 if self._bounds_method == "projected_cg" else 1
         active = torch.empty(self.mesh.n_nodes, dtype=torch.uint8, device=fem.device) if method == 0 else None
@@ -370,3 +438,152 @@ class PhaseFieldDamageSolver:
         self.last_converged = int(status.value) == _lib.CG_CONVERGED
         self.last_active = None if active is None else active.bool()
         return d_out
+
+    def adjoint_pcg(self, H_input, rhs=None, pf_dirichlet_mask=None, pf_dirichlet_values=None, rel_tol=1e-22,
+                    Gc_field=None):
+        """``_pcg_solve_adjoint_system`` (damage_solver.py:423-478) on the device: A x = rhs with the
+        forward operator, pins eliminated (held at their values, 0 without values); ``rhs=None``
+        solves for the unconstrained damage (rhs = b, pins at their values as its start).
+        Returns x; ``last_adjoint_iter`` holds the iteration count."""
+        fem = self.fem
+        H = fem._e(H_input.detach(), "H_input")
+        gcf = self.Gc_field if Gc_field is None else fem._e(Gc_field.detach(), "Gc_field")
+        fixed, vals = self._pins(pf_dirichlet_mask, pf_dirichlet_values, need_values=False)
+        if rhs is not None:
+            rhs = fem._d(rhs.detach().to(F64), "rhs")
+        x = torch.empty(self.mesh.n_nodes, dtype=F64, device=fem.device)
+        iters, status = C.c_int32(), C.c_int32()
+        check(_lib.lib().phast_t3_adjoint_solve(
+            fem.plan.handle, ptr(H), ptr(gcf), ptr(rhs), ptr(fixed), ptr(vals), None, float(rel_tol),
+            min(self.max_iter, 1000), ptr(x), C.byref(iters), C.byref(status), stream()), "adjoint_solve")
+        self.last_adjoint_iter = int(iters.value)
+        return x
+
+    def _backward(self, H_input, d_prev, d_sol, grad_output, mode, mask, values, Gc_field=None):
+        """The shared backward of the three rules (damage_solver.py:700-810, 931-1028, 1074-1152):
+        the active-set split of the upstream gradient, the unconstrained damage and the adjoint
+        by device PCG, then the per-triangle history gradient and quadratic forms."""
+        fem = self.fem
+        n, ne = self.mesh.n_nodes, self.mesh.n_elems
+        d_sol, d_prev = d_sol.detach(), d_prev.detach()
+        g0 = fem._d(grad_output.detach().to(F64).contiguous(), "grad_output")
+        fixed, _ = self._pins(mask, values, need_values=False)
+        # classify_damage_active_set (damage_solver.py:70-105) and the gradient split (:400-421)
+        upper = d_sol >= 1.0 - 1e-12
+        fx = torch.zeros_like(upper) if fixed is None else fixed.bool()
+        lower = (d_sol <= d_prev + 1e-12) & ~upper & ~fx
+        interior = ~(lower | (upper & ~fx) | fx)
+        g = g0 * interior.to(F64)
+        grad_prev = g0 * lower.to(F64)
+        d_unc = self.adjoint_pcg(H_input, None, mask, values, 1e-24, Gc_field)
+        lam = self.adjoint_pcg(H_input, g, mask, None, 1e-10 if mode == "H" else 1e-22, Gc_field)
+        d_pull = d_unc if self._bounds_method in ("post_clamp", "direct") else d_sol  # :269-281
+        gH = torch.empty(ne, dtype=F64, device=fem.device)
+        forms = [torch.empty(ne, dtype=F64, device=fem.device) for _ in range(3)] if mode != "H" else [None] * 3
+        check(_lib.lib().phast_t3_adjoint_terms(fem.plan.handle, ptr(lam), ptr(d_pull), ptr(d_unc), ptr(gH),
+                                                *[ptr(f) for f in forms], stream()), "adjoint_terms")
+        self.last_adjoint = {"lam": lam, "d_unc": d_unc}
+        return gH, grad_prev, forms
+
+
+def _needs_grad(t) -> bool:
+    return bool(torch.is_grad_enabled() and isinstance(t, torch.Tensor) and t.requires_grad)
+
+
+def _scalar(t) -> float:
+    return float(t.detach().item()) if isinstance(t, torch.Tensor) else float(t)
+
+
+class _AdjointDamageSolve(torch.autograd.Function):
+    """Implicit differentiation of the damage solve w.r.t. H and d_prev
+    (damage_solver.py:1038-1152): forward = the device solve, backward = two device PCG solves
+    (unconstrained damage, adjoint at relative tolerance 1e-10) and the per-triangle pullback."""
+
+    @staticmethod
+    def forward(ctx, solver, H_input, d_prev, pf_dirichlet_mask=None, pf_dirichlet_values=None):
+        d_new = solver._solve_device(H_input, d_prev, None, pf_dirichlet_mask, pf_dirichlet_values)
+        ctx.solver = solver
+        ctx.pins = (pf_dirichlet_mask, pf_dirichlet_values)
+        ctx.save_for_backward(H_input, d_prev, d_new)
+        return d_new
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        H_input, d_prev, d_sol = ctx.saved_tensors
+        gH, gprev, _ = ctx.solver._backward(H_input, d_prev, d_sol, grad_output, "H", *ctx.pins)
+        return None, gH, gprev, None, None
+
+
+class _AdjointDamageSolveScalar(torch.autograd.Function):
+    """Implicit differentiation w.r.t. H, d_prev and scalar Gc, l0 (damage_solver.py:482-856,
+    without gamma correction).  Gc and l0 are installed on the plan for the forward solve and
+    again for the backward's solves.  AT2: dL/dGc = -(l0 lKd + lMd/l0),
+    dL/dl0 = -(Gc lKd - Gc/l0^2 lMd); AT1: dL/dGc = -3/(8 l0) lM1 - 0.75 l0 lKd,
+    dL/dl0 = 3 Gc/(8 l0^2) lM1 - 0.75 Gc lKd."""
+
+    @staticmethod
+    def forward(ctx, solver, H_input, d_prev, Gc_t, l0_t, pf_dirichlet_mask=None, pf_dirichlet_values=None):
+        Gcv, l0v = _scalar(Gc_t), _scalar(l0_t)
+        d_new = solver._forward(H_input, d_prev, Gcv, l0v, None, pf_dirichlet_mask, pf_dirichlet_values)
+        ctx.solver, ctx.Gc, ctx.l0 = solver, Gcv, l0v
+        ctx.pins = (pf_dirichlet_mask, pf_dirichlet_values)
+        ctx.like = (Gc_t.dtype, Gc_t.device, l0_t.dtype, l0_t.device)
+        ctx.save_for_backward(H_input, d_prev, d_new)
+        return d_new
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        H_input, d_prev, d_sol = ctx.saved_tensors
+        s, Gcv, l0v = ctx.solver, ctx.Gc, ctx.l0
+        s._install(Gcv, l0v)
+        try:
+            gH, gprev, (lKd, lMd, lM1) = s._backward(H_input, d_prev, d_sol, grad_output, "scalar", *ctx.pins)
+        finally:
+            s._restore()
+        sKd, sMd, sM1 = lKd.sum(), lMd.sum(), lM1.sum()
+        if s.material.pf_model == "AT1":
+            gGc = -3.0 / (8.0 * l0v) * sM1 - 0.75 * l0v * sKd
+            gl0 = 3.0 * Gcv / (8.0 * l0v ** 2) * sM1 - 0.75 * Gcv * sKd
+        else:
+            gGc = -(l0v * sKd + (1.0 / l0v) * sMd)
+            gl0 = -(Gcv * sKd - (Gcv / l0v ** 2) * sMd)
+        gd, gdev, ld, ldev = ctx.like
+        return None, gH, gprev, gGc.to(gd).to(gdev), gl0.to(ld).to(ldev), None, None
+
+
+class _AdjointDamageSolveField(torch.autograd.Function):
+    """Implicit differentiation w.r.t. H, d_prev, a per-element Gc field and l0
+    (damage_solver.py:859-1035).  AT2: dL/dGc_e = -(l0 lKd_e + lMd_e/l0),
+    dL/dl0 = -(sum Gc_e lKd_e - sum Gc_e/l0^2 lMd_e); AT1: dL/dGc_e = -3/(8 l0) lM1_e - 0.75 l0 lKd_e,
+    dL/dl0 = 3/(8 l0^2) sum Gc_e lM1_e - 0.75 sum Gc_e lKd_e."""
+
+    @staticmethod
+    def forward(ctx, solver, H_input, d_prev, Gc_e_t, l0_t, pf_dirichlet_mask=None, pf_dirichlet_values=None):
+        l0v = _scalar(l0_t)
+        d_new = solver._forward(H_input, d_prev, None, l0v, Gc_e_t, pf_dirichlet_mask, pf_dirichlet_values)
+        ctx.solver, ctx.l0 = solver, l0v
+        ctx.pins = (pf_dirichlet_mask, pf_dirichlet_values)
+        ctx.like = (l0_t.dtype, l0_t.device)
+        ctx.save_for_backward(H_input, d_prev, d_new, Gc_e_t)
+        return d_new
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        H_input, d_prev, d_sol, Gc_e = ctx.saved_tensors
+        s, l0v = ctx.solver, ctx.l0
+        s._install(None, l0v)
+        try:
+            gH, gprev, (lKd, lMd, lM1) = s._backward(H_input, d_prev, d_sol, grad_output, "field", *ctx.pins,
+                                                     Gc_field=Gc_e)
+        finally:
+            s._restore()
+        Ge = Gc_e.detach()
+        if s.material.pf_model == "AT1":
+            gGc = -3.0 / (8.0 * l0v) * lM1 - 0.75 * l0v * lKd
+            gl0 = 3.0 / (8.0 * l0v ** 2) * (Ge * lM1).sum() - 0.75 * (Ge * lKd).sum()
+        else:
+            gGc = -(l0v * lKd + (1.0 / l0v) * lMd)
+            gl0 = -((Ge * lKd).sum() - (Ge / l0v ** 2 * lMd).sum())
+        ld, ldev = ctx.like
+        return None, gH, gprev, gGc, gl0.to(ld).to(ldev), None, None
+
