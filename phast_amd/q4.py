@@ -45,6 +45,7 @@ class Material:
     degradation_type: str = "standard"
     stress_degradation: str = "split"
     at1_threshold: object = "auto"
+    gamma_correction: bool = False
 
     def __post_init__(self):
         if self.stress_degradation not in ("split", "full"):  # material.py:185-188
@@ -167,6 +168,15 @@ class FEMMesh:
             "left": np.nonzero(nodes[:, 0] == x[0])[0], "right": np.nonzero(nodes[:, 0] == x[-1])[0],
         }
 
+    @property
+    def elem_h_np(self) -> np.ndarray:
+        """Per-cell minimum edge length (mesh.py:331-338), computed on first use."""
+        if getattr(self, "_elem_h", None) is None:
+            c = self.nodes[self.elements]
+            self._elem_h = np.min(np.stack([np.linalg.norm(c[:, (k + 1) % 4] - c[:, k], axis=1)
+                                            for k in range(4)], axis=1), axis=1)
+        return self._elem_h
+
     @classmethod
     def rectangle(cls, W: float, H: float, nx: int, ny: int, device=None) -> "FEMMesh":
         """Plate ``[0,W] x [0,H]`` with ``nx x ny`` cells (``rectangular_sent_q4_structured``)."""
@@ -230,6 +240,27 @@ class Q4Plan:
         if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib.lib().phast_q4_plan_destroy(h)
             self.handle = None
+
+
+def pf_c_alpha(pf_model: str) -> float:
+    """c_w of the crack-surface density (damage_solver.py:1491-1498): 2 for AT2, 8/3 for AT1."""
+    return 8.0 / 3.0 if pf_model == "AT1" else 2.0
+
+
+def gamma_factor(elem_h: np.ndarray, l0: float, pf_model: str) -> np.ndarray:
+    """Per-element gamma-convergence factor 1 / (1 + h_e / (c_w l0)) (Bourdin et al. 2000;
+    damage_solver.py:1414-1417)."""
+    return 1.0 / (1.0 + elem_h / (pf_c_alpha(pf_model) * l0))
+
+
+def _gamma_Gc_field(solver, Gc_field) -> torch.Tensor:
+    """Gc_eff_e = Gc * gamma_e of a solver built with ``material.gamma_correction``
+    (damage_solver.py:1409-1433); the per-element Gc branch of the kernels then applies it."""
+    if Gc_field is not None:
+        raise ValueError("gamma_correction builds its own per-element Gc; pass Gc_field to solve() instead")
+    m = solver.material
+    g = m.Gc * gamma_factor(solver.mesh.elem_h_np, m.l0, m.pf_model)
+    return torch.as_tensor(g, dtype=F64, device=solver.fem.device)
 
 
 def _dev_check(t: torch.Tensor, shape, name: str, device) -> torch.Tensor:
@@ -450,6 +481,9 @@ class PhaseFieldDamageSolver:
         self.tol, self.max_iter = float(tol), int(max_iter)
         self._bounds_method = bounds_method
         self.Gc_field = None if Gc_field is None else _dev_check(Gc_field, (fem.mesh.n_elems,), "Gc_field", fem.device)
+        self._gamma_correction = bool(getattr(self.material, "gamma_correction", False))
+        if self._gamma_correction:
+            self.Gc_field = _gamma_Gc_field(self, Gc_field)
         self.last_iter = 0
         self.last_converged = False
         self.last_status = ""

@@ -14,13 +14,14 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import warnings
 
 import numpy as np
 import torch
 
 from . import _lib
 from ._lib import check, ptr, stream
-from .q4 import F64, Material, StaggeredSolver, _dev_check, history_update  # noqa: F401  (re-exported)
+from .q4 import F64, Material, StaggeredSolver, _dev_check, _gamma_Gc_field, gamma_factor, history_update  # noqa: F401
 
 _SPLIT_ID = {"amor": 0, "isotropic": 1, "spectral": 2}
 
@@ -76,7 +77,8 @@ class FEMMesh:
         if np.any(self.areas_np <= 0.0):
             raise ValueError("degenerate (zero-area) triangle in the mesh")
         per = np.linalg.norm(v01, axis=1) + np.linalg.norm(v12, axis=1) + np.linalg.norm(v02, axis=1)
-        self.h_min = float((4.0 * self.areas_np / (per + 1e-30)).min())  # mesh.py:276-283
+        self.elem_h_np = 4.0 * self.areas_np / (per + 1e-30)  # mesh.py:275-276
+        self.h_min = float(self.elem_h_np.min())
         self.node_sets: dict[str, np.ndarray] = {}
 
     @classmethod
@@ -312,6 +314,9 @@ class PhaseFieldDamageSolver:
         self.tol, self.max_iter = float(tol), int(max_iter)
         self._bounds_method = bounds_method
         self.Gc_field = None if Gc_field is None else fem._e(Gc_field, "Gc_field")
+        self._gamma_correction = bool(getattr(self.material, "gamma_correction", False))
+        if self._gamma_correction:
+            self.Gc_field = _gamma_Gc_field(self, Gc_field)
         self.last_iter = 0
         self.last_converged = False
         self.last_status = ""
@@ -407,7 +412,11 @@ class PhaseFieldDamageSolver:
         check(_lib.lib().phast_t3_set_fracture(self.fem.plan.handle, m.Gc, m.l0, m.at1_source), "set_fracture")
 
     def _forward(self, H_input, d_prev, Gc, l0, Gc_field, mask, values):
-        """One device solve with optional temporary Gc / l0 (damage_solver.py:2376-2420, 2642-2684)."""
+        """One device solve with optional temporary Gc / l0 (damage_solver.py:2376-2492, 2642-2684).
+        Under gamma correction the per-element Gc built at construction governs a solve without a
+        Gc field, as the reference's per-element arrays do; scalar overrides then change nothing."""
+        if self._gamma_correction and Gc_field is None:
+            Gc = l0 = None
         installed = self._install(None if Gc is None else _scalar(Gc), None if l0 is None else _scalar(l0))
         try:
             return self._solve_device(H_input, d_prev, Gc_field, mask, values)
@@ -524,7 +533,18 @@ class _AdjointDamageSolveScalar(torch.autograd.Function):
     @staticmethod
     def forward(ctx, solver, H_input, d_prev, Gc_t, l0_t, pf_dirichlet_mask=None, pf_dirichlet_values=None):
         Gcv, l0v = _scalar(Gc_t), _scalar(l0_t)
-        d_new = solver._forward(H_input, d_prev, Gcv, l0v, None, pf_dirichlet_mask, pf_dirichlet_values)
+        ctx.gamma = None
+        if solver._gamma_correction:  # damage_solver.py:545-585: per-element Gc_e = Gc gamma_e(l0)
+            if l0_t.requires_grad:
+                warnings.warn("_AdjointDamageSolveScalar with gamma_correction=True returns the l0 gradient "
+                              "through the direct model dependence only; gamma_e(l0) is held fixed",
+                              RuntimeWarning, stacklevel=3)
+            gam = torch.as_tensor(gamma_factor(solver.mesh.elem_h_np, l0v, solver.material.pf_model), dtype=F64,
+                                  device=solver.fem.device)
+            ctx.gamma = gam
+            d_new = solver._forward(H_input, d_prev, None, l0v, Gcv * gam, pf_dirichlet_mask, pf_dirichlet_values)
+        else:
+            d_new = solver._forward(H_input, d_prev, Gcv, l0v, None, pf_dirichlet_mask, pf_dirichlet_values)
         ctx.solver, ctx.Gc, ctx.l0 = solver, Gcv, l0v
         ctx.pins = (pf_dirichlet_mask, pf_dirichlet_values)
         ctx.like = (Gc_t.dtype, Gc_t.device, l0_t.dtype, l0_t.device)
@@ -534,7 +554,24 @@ class _AdjointDamageSolveScalar(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_output):
         H_input, d_prev, d_sol = ctx.saved_tensors
-        s, Gcv, l0v = ctx.solver, ctx.Gc, ctx.l0
+        s, Gcv, l0v, gam = ctx.solver, ctx.Gc, ctx.l0, ctx.gamma
+        gd, gdev, ld, ldev = ctx.like
+        if gam is not None:  # damage_solver.py:766-781: per-element forms reduced through gamma_e
+            Ge = Gcv * gam
+            s._install(None, l0v)
+            try:
+                gH, gprev, (lKd, lMd, lM1) = s._backward(H_input, d_prev, d_sol, grad_output, "scalar", *ctx.pins,
+                                                         Gc_field=Ge)
+            finally:
+                s._restore()
+            if s.material.pf_model == "AT1":
+                gGc_e = -3.0 / (8.0 * l0v) * lM1 - 0.75 * l0v * lKd
+                gl0 = 3.0 / (8.0 * l0v ** 2) * (Ge * lM1).sum() - 0.75 * (Ge * lKd).sum()
+            else:
+                gGc_e = -(l0v * lKd + (1.0 / l0v) * lMd)
+                gl0 = -((Ge * lKd).sum() - (Ge / l0v ** 2 * lMd).sum())
+            gGc = (gam * gGc_e).sum()
+            return None, gH, gprev, gGc.to(gd).to(gdev), gl0.to(ld).to(ldev), None, None
         s._install(Gcv, l0v)
         try:
             gH, gprev, (lKd, lMd, lM1) = s._backward(H_input, d_prev, d_sol, grad_output, "scalar", *ctx.pins)
@@ -547,7 +584,6 @@ class _AdjointDamageSolveScalar(torch.autograd.Function):
         else:
             gGc = -(l0v * sKd + (1.0 / l0v) * sMd)
             gl0 = -(Gcv * sKd - (Gcv / l0v ** 2) * sMd)
-        gd, gdev, ld, ldev = ctx.like
         return None, gH, gprev, gGc.to(gd).to(gdev), gl0.to(ld).to(ldev), None, None
 
 
@@ -586,4 +622,14 @@ class _AdjointDamageSolveField(torch.autograd.Function):
             gl0 = -((Ge * lKd).sum() - (Ge / l0v ** 2 * lMd).sum())
         ld, ldev = ctx.like
         return None, gH, gprev, gGc, gl0.to(ld).to(ldev), None, None
+
+
+def make_gamma_corrected_Gc_field(solver: PhaseFieldDamageSolver, Gc_scalar: torch.Tensor) -> torch.Tensor:
+    """``Gc_scalar * gamma_e`` for ``solve(..., Gc_field=...)`` (damage_solver.py:219-266): a
+    differentiable way to a scalar-Gc gradient under gamma correction."""
+    if not solver._gamma_correction:
+        raise ValueError("Solver was not built with gamma_correction=True. Either enable it via "
+                         "material.gamma_correction=True, or use the scalar Gc path directly.")
+    gam = gamma_factor(solver.mesh.elem_h_np, solver.material.l0, solver.material.pf_model)
+    return Gc_scalar * torch.as_tensor(gam, dtype=Gc_scalar.dtype, device=Gc_scalar.device)
 

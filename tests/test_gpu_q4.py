@@ -276,3 +276,23 @@ def test_kelvin_voigt_damped_step(dev):
     und = O.ExplicitDynamics(ofem, dt=dyn.dt).step(u, v, a, d, f_ext)[2]
     assert rel(und, oa) > 1e-6  # the damping term is not negligible here
 
+
+
+def test_damage_solve_gamma_correction(dev):
+    """material.gamma_correction: Gc_eff = Gc / (1 + h / (2 l0)) per cell with h the shortest
+    edge (damage_solver.py:1409-1433), run through the per-cell Gc branch."""
+    from phast_amd import q4 as P
+    P_, fem0, ofem = setup(nx=29, ny=13)
+    mat_kw = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5)
+    fem = P.FEMOperators(P.FEMMesh.rectangle(3.7, 2.3, 29, 13), P.Material(gamma_correction=True, **mat_kw),
+                         tile=(128, 8))
+    n, E = fem.mesh.n_nodes, fem.mesh.n_elems
+    rng = np.random.default_rng(41)
+    H = rng.uniform(0, 0.2, (E, 4))
+    d_prev = np.clip(rng.uniform(-0.5, 0.5, n), 0, 1)
+    h = min(3.7 / 29, 2.3 / 13)
+    Gc_e = np.full(E, 3e-3 / (1.0 + h / (2.0 * 0.5)))
+    d = P.PhaseFieldDamageSolver(fem, tol=1e-10, max_iter=3000).solve(T(H, dev), T(d_prev, dev))
+    od = O.PhaseFieldDamageSolver(ofem, tol=1e-10, max_iter=3000, bounds_method="post_clamp", Gc_e=Gc_e).solve(
+        H, d_prev)
+    assert rel(d, od) < RTOL

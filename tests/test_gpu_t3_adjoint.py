@@ -198,3 +198,47 @@ def test_scalar_and_field_gradients_match_device_differences(dev, pf_model):
     d = solver.solve(H, d_prev, Gc_field=Ge)
     (w @ d).backward()
     assert abs(float(Ge.grad.sum()) - float(Gc.grad)) <= 1e-9 * abs(float(Gc.grad))
+
+
+@pytest.mark.parametrize("pf_model", ["AT2", "AT1"])
+def test_gamma_correction_solve_and_scalar_adjoint(dev, pf_model):
+    """Gamma correction (damage_solver.py:1409-1433, 545-781): the solve runs with
+    Gc_e = Gc gamma_e, and the scalar rule reduces the per-element Gc gradient through gamma_e
+    with gamma_e(l0) held fixed in the l0 gradient."""
+    from phast_amd import t3 as P
+    nodes, els = scrambled_mesh(17, 12, seed=40)
+    mk = dict(E=32000.0, nu=0.2, rho=2.45e-9, Gc=2.7, l0=0.5, pf_model=pf_model)
+    fem = P.FEMOperators(P.FEMMesh(nodes, els), P.Material(gamma_correction=True, **mk))
+    om, omat = O.T3Mesh(nodes, els), OMaterial(**mk)
+    cw = 2.0 if pf_model == "AT2" else 8.0 / 3.0
+    rng = np.random.default_rng(42)
+    H = rng.uniform(2.0, 6.0, len(els))
+    d_prev = np.clip(rng.uniform(-0.4, 0.5, len(nodes)), 0.0, 1.0)
+    w = rng.uniform(-1.0, 1.0, len(nodes))
+    dmg = P.PhaseFieldDamageSolver(fem, tol=1e-10, max_iter=3000)
+    d = dmg.solve(T(H, dev), T(d_prev, dev))
+    gam = 1.0 / (1.0 + om.elem_h / (cw * 0.5))
+    od = O.T3DamageSolver(om, omat, tol=1e-10, max_iter=3000, bounds_method="post_clamp", Gc_e=2.7 * gam).solve(
+        H, d_prev)
+    assert rel(d, od) < 1e-10
+    Ht = T(H, dev).requires_grad_(True)
+    Gc = torch.tensor(2.9, dtype=torch.float64, requires_grad=True)
+    l0 = torch.tensor(0.45, dtype=torch.float64, requires_grad=True)
+    with pytest.warns(RuntimeWarning):
+        d = dmg.solve(Ht, T(d_prev, dev), Gc=Gc, l0=l0)
+    (d * T(w, dev)).sum().backward()
+    gam2 = 1.0 / (1.0 + om.elem_h / (cw * 0.45))
+    adj = OA.T3DamageAdjoint(om, omat, tol=1e-10, max_iter=3000)
+    assert rel(d, adj.forward(H, d_prev, "field", l0=0.45, Gc_e=2.9 * gam2)) < 1e-10
+    ref = adj.backward(H, d_prev, d.detach().cpu().numpy(), w, "field", l0=0.45, Gc_e=2.9 * gam2)
+    gGc = float((gam2 * ref["Gc"]).sum())
+    assert abs(float(Gc.grad) - gGc) <= TOL * abs(gGc)
+    assert abs(float(l0.grad) - ref["l0"]) <= TOL * abs(ref["l0"])
+    assert rel(Ht.grad, ref["H"]) < TOL
+    # the same gradient through make_gamma_corrected_Gc_field and the field rule
+    Gs = torch.tensor(2.7, dtype=torch.float64, device=dev, requires_grad=True)
+    d = dmg.solve(T(H, dev), T(d_prev, dev), Gc_field=P.make_gamma_corrected_Gc_field(dmg, Gs))
+    (d * T(w, dev)).sum().backward()
+    ref = adj.backward(H, d_prev, d.detach().cpu().numpy(), w, "field", l0=0.5, Gc_e=2.7 * gam)
+    g = float((gam * ref["Gc"]).sum())
+    assert abs(float(Gs.grad) - g) <= TOL * abs(g)

@@ -39,3 +39,16 @@ def test_split_quad_diagonals_mirror_about_mid_height():
     assert np.all(crack[:, 1] == 3.0) and crack[:, 0].max() == 4.0
     with pytest.raises(ValueError):
         FEMMesh.sent_split_quad(8.0, 5.0, 8, 5, a=4.0, device="cpu")  # H/2 off the grid
+
+
+def test_gamma_factor_and_element_sizes():
+    """Per-element h (T3: 4A / perimeter, the in-circle diameter; Q4: the shortest edge) and the
+    gamma-convergence factor 1 / (1 + h / (c_w l0)) with c_w = 2 (AT2), 8/3 (AT1)."""
+    from phast_amd.q4 import FEMMesh as Q4Mesh, gamma_factor, pf_c_alpha
+    m = FEMMesh(np.array([[0.0, 0.0], [3.0, 0.0], [0.0, 4.0]]), np.array([[0, 1, 2]]), device="cpu")
+    assert m.elem_h_np[0] == pytest.approx(4.0 * 6.0 / 12.0, rel=1e-15)  # in-circle diameter 2r = 2
+    q = Q4Mesh.rectangle(4.0, 1.0, 2, 1, device="cpu")  # cells 2 x 1
+    np.testing.assert_array_equal(q.elem_h_np, [1.0, 1.0])
+    assert pf_c_alpha("AT2") == 2.0 and pf_c_alpha("AT1") == pytest.approx(8.0 / 3.0)
+    np.testing.assert_allclose(gamma_factor(np.array([0.0, 1.0, 2.0]), 0.5, "AT2"), [1.0, 0.5, 1.0 / 3.0], rtol=1e-15)
+    np.testing.assert_allclose(gamma_factor(np.array([4.0 / 3.0]), 0.5, "AT1"), [0.5], rtol=1e-15)
