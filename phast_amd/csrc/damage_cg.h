@@ -34,9 +34,16 @@ __device__ __forceinline__ double reduce_sum_parts(const double *part, int off, 
     double w[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-        // lane sums part[k], part[k+256], ... in that order; four loads in flight at a time
+        // lane sums part[k], part[k+256], ... in that order; up to sixteen loads in flight
         double s = 0.0;
         int k = threadIdx.x + v * NT;
+        for (; k + 15 * 256 < cnt; k += 16 * 256) {
+            double a[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = part[off + k + j * 256];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) s += a[j];
+        }
         for (; k + 3 * 256 < cnt; k += 4 * 256) {
             const double a = part[off + k], b = part[off + k + 256], c = part[off + k + 512],
                          e = part[off + k + 768];
@@ -60,6 +67,18 @@ template <int NT>
 __device__ __forceinline__ double reduce_min_parts(const double *part, int off, int cnt, double *red) {
     double s = __builtin_huge_val(), s2 = s, s3 = s, s4 = s;  // the minimum is exact in any order
     int k = threadIdx.x;
+    for (; k + 15 * NT < cnt; k += 16 * NT) {
+        double a[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a[j] = part[off + k + j * NT];
+#pragma unroll
+        for (int j = 0; j < 16; j += 4) {
+            s = fmin(s, a[j]);
+            s2 = fmin(s2, a[j + 1]);
+            s3 = fmin(s3, a[j + 2]);
+            s4 = fmin(s4, a[j + 3]);
+        }
+    }
     for (; k + 3 * NT < cnt; k += 4 * NT) {
         s = fmin(s, part[off + k]);
         s2 = fmin(s2, part[off + k + NT]);
