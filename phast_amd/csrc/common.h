@@ -97,6 +97,37 @@ __device__ __forceinline__ void block_sum_n(double v[K], double *sh) {
         v[k] = s;
     }
 }
+// One block sum (block_sum's arithmetic, same bits) and two block minima sharing one pair of
+// barriers.  `sh` holds 3 * NT/64 doubles; results are valid in every thread.
+template <int NT>
+__device__ __forceinline__ void block_sum_min2(double &s, double &m1, double &m2, double *sh) {
+    constexpr int NW = NT / kWave;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, kWave);
+        m1 = fmin(m1, __shfl_xor(m1, o, kWave));
+        m2 = fmin(m2, __shfl_xor(m2, o, kWave));
+    }
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) {
+        sh[w] = s;
+        sh[NW + w] = m1;
+        sh[2 * NW + w] = m2;
+    }
+    __syncthreads();
+    double a = 0.0, b = sh[NW], c = sh[2 * NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) a += sh[i];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) {
+        b = fmin(b, sh[NW + i]);
+        c = fmin(c, sh[2 * NW + i]);
+    }
+    s = a;
+    m1 = b;
+    m2 = c;
+}
 template <int NT>
 __device__ __forceinline__ double block_min(double v, double *sh) {
     v = wave_min(v);

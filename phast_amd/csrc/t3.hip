@@ -533,7 +533,7 @@ struct T3Cg {
 template <int OP, bool PATCH>
 __global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Patch P, const T3Mat M, T3Cg A) {
     extern __shared__ double lds[];
-    __shared__ double red[4];
+    __shared__ double red[12];
     const DamageCG &S = A.cg[0];
     if (OP != T_CG_INIT && OP != T_PC_INIT && S.done) return;
     if (OP == T_CG_RESID && !(S.limited || S.check)) return;
@@ -601,29 +601,40 @@ __global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Patc
         }
     }
     if (OP == T_PC_RESID) return;
+    // per-block partials; the reductions of one pass share their barriers (same bits as one
+    // block_sum / block_min each)
     const int b = blockIdx.x;
-    const double s0 = block_sum<256>(p0, red);
-    if (threadIdx.x == 0) A.part[b] = s0;
     if (OP == T_CG_AP) {
-        const double s1 = block_min<256>(p1, red);
-        const double s2 = block_min<256>(p2, red);
+        block_sum_min2<256>(p0, p1, p2, red);
         if (threadIdx.x == 0) {
-            A.part[A.nblk + b] = s1;
-            A.part[2 * A.nblk + b] = s2;
+            A.part[b] = p0;
+            A.part[A.nblk + b] = p1;
+            A.part[2 * A.nblk + b] = p2;
         }
-    } else if (OP != T_PC_AP) {
-        const double s1 = block_sum<256>(p1, red);
-        if (threadIdx.x == 0) A.part[A.nblk + b] = s1;
-        if (OP == T_CG_INIT || OP == T_PC_INIT) {
-            const double s2 = block_sum<256>(p2, red);
-            if (threadIdx.x == 0) A.part[2 * A.nblk + b] = s2;
+    } else if (OP == T_PC_AP) {
+        const double s0 = block_sum<256>(p0, red);
+        if (threadIdx.x == 0) A.part[b] = s0;
+    } else if (OP == T_CG_INIT || OP == T_PC_INIT) {
+        double v[3] = {p0, p1, p2};
+        block_sum_n<256, 3>(v, red);
+        if (threadIdx.x == 0) {
+            A.part[b] = v[0];
+            A.part[A.nblk + b] = v[1];
+            A.part[2 * A.nblk + b] = v[2];
+        }
+    } else {
+        double v[2] = {p0, p1};
+        block_sum_n<256, 2>(v, red);
+        if (threadIdx.x == 0) {
+            A.part[b] = v[0];
+            A.part[A.nblk + b] = v[1];
         }
     }
 }
 
 template <int OP0>
 __global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
-    __shared__ double red[4];
+    __shared__ double red[12];
     const DamageCG &S = A.cg[0];
     if (OP0 != TV_SETUP && OP0 != TV_FINISH && OP0 != TV_PC_FINISH && OP0 != TA_SETUP && OP0 != TA_FINISH && S.done)
         return;
@@ -726,13 +737,16 @@ __global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
             A.p[q] = pn;
         }
     }
-    if (OP == TV_PRECOND || OP == TV_PC_PRECOND || OP == TA_UPDATE || fuse) {
+    if (OP == TV_PC_PRECOND || OP == TA_UPDATE || (OP0 == TV_PC_UPDATE_PRE && fuse)) {
+        double v[2] = {p0, p1};
+        block_sum_n<256, 2>(v, red);
+        if (threadIdx.x == 0) {
+            A.part[blockIdx.x] = v[0];
+            A.part[A.nblk + blockIdx.x] = v[1];
+        }
+    } else if (OP == TV_PRECOND || fuse) {
         const double s0 = block_sum<256>(p0, red);
         if (threadIdx.x == 0) A.part[blockIdx.x] = s0;
-        if (OP == TV_PC_PRECOND || OP == TA_UPDATE || (OP0 == TV_PC_UPDATE_PRE && fuse)) {
-            const double s1 = block_sum<256>(p1, red);
-            if (threadIdx.x == 0) A.part[A.nblk + blockIdx.x] = s1;
-        }
     }
 }
 
