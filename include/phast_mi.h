@@ -414,6 +414,12 @@ int phast_hex8_qp_to_node(const phast_hex8_plan *plan, const double *f_qp, doubl
                           hipStream_t stream);
 int phast_hex8_qp_stress_apply(const phast_hex8_plan *plan, const double *S_qp6, double *out_mech,
                                hipStream_t stream);
+/* Element fields of PeriodicVoxelPhaseFieldOperator.field_snapshot (solid3d/operator.py:1561-1608,
+ * without interface drivers or crack closure): per voxel the Gauss-point means of the total
+ * engineering strain (E,6), of the degraded stress g C eps (E,6) and of 0.5 g eps . C eps (E,).
+ * g_qp (E,8) may be null (g = 1). */
+int phast_hex8_field_snapshot(const phast_hex8_plan *plan, const double *x, const double *g_qp, double *strain,
+                              double *stress, double *psi, hipStream_t stream);
 int phast_hex8_adj_damage_history(const phast_hex8_plan *plan, const double *d, const double *l, double scale,
                                   double *out_qp, hipStream_t stream);
 int phast_hex8_adj_damage_gc(const phast_hex8_plan *plan, const double *d, const double *l, double scale,

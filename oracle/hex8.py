@@ -249,6 +249,15 @@ class VoxelOperator:
 # the functions above.
 # ---------------------------------------------------------------------------------------
 
+def field_snapshot_elements(op: VoxelOperator, x, g=None):
+    """Element Gauss-point means of a field snapshot (solid3d/operator.py:1589-1600): strain,
+    degraded stress g C eps and 0.5 g eps . C eps; g None = 1."""
+    eps = op.qp_strain(x)
+    sigma0 = np.einsum("eqm,ekm->eqk", eps, op.bank[op.index])
+    gg = np.ones(eps.shape[:2]) if g is None else g
+    return eps.mean(axis=1), (gg[..., None] * sigma0).mean(axis=1), (0.5 * gg * (eps * sigma0).sum(-1)).mean(axis=1)
+
+
 def driving_strain_gradient(op: VoxelOperator, eps):
     """dY/d(engineering strain) per Gauss point (E,8,6): drivers none / full (C eps) /
     spectral (lam <tr>+ I + 2 mu eps+, shear entries the tensor off-diagonals)."""

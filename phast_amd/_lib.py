@@ -102,6 +102,7 @@ _SIGNATURES = {
     "phast_hex8_adj_driving": ([_P] * 5, C.c_int),
     "phast_hex8_qp_to_node": ([_P, _P, _D, _P, _P], C.c_int),
     "phast_hex8_qp_stress_apply": ([_P] * 4, C.c_int),
+    "phast_hex8_field_snapshot": ([_P] * 7, C.c_int),
     "phast_hex8_adj_damage_history": ([_P, _P, _P, _D, _P, _P], C.c_int),
     "phast_hex8_adj_damage_gc": ([_P, _P, _P, _D, _P, _P], C.c_int),
 }

@@ -163,6 +163,40 @@ __global__ __launch_bounds__(256) void adj_driving_kernel(const Hex8Geo G, const
     }
 }
 
+// Element fields of a field snapshot (solid3d/operator.py:1561-1608, no interface drivers or
+// closure): Gauss-point means of the total strain, of the degraded stress g C eps and of the
+// degraded energy 0.5 g eps . C eps.  g null = 1.
+__global__ __launch_bounds__(256) void snapshot_kernel(const Hex8Geo G, const double *x, const double *g,
+                                                       const double *bank, const int32_t *bank_index,
+                                                       double *strain, double *stress, double *psi) {
+    const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    int i, j, k;
+    voxel_ijk(G, e, i, j, k);
+    double ex[8][6];
+    voxel_strains(G, x, n, i, j, k, ex);
+    const double *C = bank + 36 * (int64_t)bank_index[e];
+    double se[6] = {0, 0, 0, 0, 0, 0}, ss[6] = {0, 0, 0, 0, 0, 0}, sp = 0.0;
+    for (int q = 0; q < 8; ++q) {
+        const double gq = g ? g[e * 8 + q] : 1.0;
+        double en = 0.0;
+        for (int r = 0; r < 6; ++r) {
+            double c6 = 0.0;
+            for (int c = 0; c < 6; ++c) c6 += C[r * 6 + c] * ex[q][c];
+            se[r] += ex[q][r];
+            ss[r] += gq * c6;
+            en += ex[q][r] * c6;
+        }
+        sp += 0.5 * gq * en;
+    }
+    for (int r = 0; r < 6; ++r) {
+        strain[e * 6 + r] = se[r] / 8.0;
+        stress[e * 6 + r] = ss[r] / 8.0;
+    }
+    psi[e] = sp / 8.0;
+}
+
 // out[node] = scale * sum over the 8 voxels of the node of sum_q N_a(q) f[e,q]
 __global__ __launch_bounds__(256) void qp_to_node_kernel(const Hex8Geo G, const double *f, double scale,
                                                          double *out) {
@@ -344,6 +378,15 @@ extern "C" int phast_hex8_adj_damage_gc(const phast_hex8_plan *pl, const double 
                                         double *out_e, hipStream_t st) {
     PHAST_REQUIRE(pl && d && l && out_e, "phast_hex8_adj_damage_gc: null argument");
     hipLaunchKernelGGL(adj_gc_kernel, dim3(voxel_blocks(pl)), dim3(256), 0, st, pl->geo, d, l, scale, out_e);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_field_snapshot(const phast_hex8_plan *pl, const double *x, const double *g_qp,
+                                         double *strain, double *stress, double *psi, hipStream_t st) {
+    PHAST_REQUIRE(pl && x && strain && stress && psi, "phast_hex8_field_snapshot: null argument");
+    hipLaunchKernelGGL(snapshot_kernel, dim3(voxel_blocks(pl)), dim3(256), 0, st, pl->geo, x, g_qp, pl->bank,
+                       pl->bank_index, strain, stress, psi);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }

@@ -45,6 +45,45 @@ for _name, _args in {
     _fn.restype = C.c_int
 
 
+@dataclass
+class FieldSnapshot:
+    """CPU numpy fields at one step (solid3d/model.py:1340-1375): nodal ``displacement``,
+    ``fluctuation``, ``damage`` over the full nodes; element Gauss-point means ``strain``,
+    ``stress``, ``history_mean``, ``psi_elastic`` (mean of g psi0), ``psi_driving`` (mean of Y)
+    and ``history_max``.  The crack-closure fields are not produced (no closure here)."""
+    step: int
+    time_value: float
+    macro_strain: np.ndarray
+    macro_stress: np.ndarray
+    displacement: np.ndarray
+    fluctuation: np.ndarray
+    damage: np.ndarray
+    strain: np.ndarray
+    stress: np.ndarray
+    history_mean: np.ndarray
+    history_max: np.ndarray
+    psi_elastic: np.ndarray
+    psi_driving: np.ndarray
+    psi_closure: np.ndarray | None = None
+    closure_active: np.ndarray | None = None
+    det_F_min: np.ndarray | None = None
+
+
+def _full_to_reduced(shape) -> np.ndarray:
+    """Reduced node of every full node i + (nx+1)(j + (ny+1) k) of a triply periodic grid
+    (solid3d/periodic.py:61-90)."""
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    return ((i % nx) + nx * ((j % ny) + ny * (k % nz))).reshape(-1)
+
+
+def _full_node_offsets(grid) -> np.ndarray:
+    """Full node coordinates relative to the origin, (i hx, j hy, k hz) in full-node order."""
+    nx, ny, nz = grid.shape
+    k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    return np.stack([i.reshape(-1) * grid.h[0], j.reshape(-1) * grid.h[1], k.reshape(-1) * grid.h[2]], axis=1)
+
+
 class PCGBreakdownError(RuntimeError):
     """Non-positive curvature, a non-positive diagonal or non-finite values (linalg.py:41)."""
 
@@ -442,6 +481,38 @@ class PeriodicVoxelPhaseFieldOperator:
         out = torch.empty((self.n_elements, 8, 6), dtype=F64, device=self.device)
         check(_L.phast_hex8_adj_driving(self._plan, ptr(x), ptr(t), ptr(out), stream()), "adj_driving")
         return out
+
+    def field_snapshot(self, step: int, time_value: float, x, d, H=None, g=None) -> "FieldSnapshot":
+        """CPU fields of one state (solid3d/operator.py:1561-1652; no interface drivers or
+        closure).  Element Gauss-point means of the strain, the degraded stress g C eps, H, the
+        degraded energy and the driving energy Y (from ``phast_hex8_field_snapshot`` and
+        ``driving_force``), the maximum of H; nodal fluctuation, damage and total displacement
+        ``E_tensor (X - origin) + P u~`` over the (nx+1)(ny+1)(nz+1) full nodes."""
+        x = self._check("x", x, (self.n_mech,))
+        d = self._check("d", d, (self.n_red,))
+        H = self._check_qp("H", H)
+        g = self.degradation(d) if g is None else self._check_qp("g", g)
+        ne = self.n_elements
+        strain = torch.empty((ne, 6), dtype=F64, device=self.device)
+        stress = torch.empty_like(strain)
+        psi = torch.empty(ne, dtype=F64, device=self.device)
+        check(_L.phast_hex8_field_snapshot(self._plan, ptr(x), ptr(g), ptr(strain), ptr(stress), ptr(psi),
+                                           stream()), "field_snapshot")
+        Y = self.driving_force(x)
+        if H is None:
+            H = torch.zeros((ne, 8), dtype=F64, device=self.device)
+        u, E = x[:3 * self.n_red].reshape(-1, 3), x[3 * self.n_red:]
+        red = _full_to_reduced(self.grid.shape)
+        fluct = u.cpu().numpy()[red]
+        e = E.cpu().numpy()
+        Et = np.array([[e[0], e[5] / 2, e[4] / 2], [e[5] / 2, e[1], e[3] / 2], [e[4] / 2, e[3] / 2, e[2]]])
+        disp = _full_node_offsets(self.grid) @ Et.T + fluct
+        return FieldSnapshot(
+            step=step, time_value=time_value, macro_strain=e.copy(),
+            macro_stress=self.macro_stress(x, g).cpu().numpy(), displacement=disp, fluctuation=fluct,
+            damage=d.cpu().numpy()[red], strain=strain.cpu().numpy(), stress=stress.cpu().numpy(),
+            history_mean=H.mean(dim=1).cpu().numpy(), history_max=H.amax(dim=1).cpu().numpy(),
+            psi_elastic=psi.cpu().numpy(), psi_driving=Y.mean(dim=1).cpu().numpy())
 
     def qp_to_node(self, f, scale: float = 1.0) -> torch.Tensor:
         """(n,): scale sum_e sum_q N_a(q) f_q."""

@@ -259,3 +259,36 @@ def test_multigrid_pcg_matches_jacobi_solution(dev, shape):
     a, b = float(torch.dot(u, mg.apply(v))), float(torch.dot(v, mg.apply(u)))
     assert abs(a - b) <= 1e-10 * max(abs(a), abs(b))
     assert float(torch.dot(u, mg.apply(u))) > 0
+
+
+@pytest.mark.parametrize("shape", [(6, 5, 7), (16, 9, 10)])
+def test_field_snapshot(dev, shape):
+    """field_snapshot (solid3d/operator.py:1561-1652): element means against the oracle, the
+    nodal fields through the periodic map, and the total displacement E (X - origin) + P u~."""
+    S, op, oop = build(shape)
+    rng = np.random.default_rng(21)
+    x = rng.uniform(-1e-6, 1e-6, op.n_mech)
+    x[-6:] = rng.uniform(-1e-3, 1e-3, 6)
+    d = rng.uniform(0, 1, op.n_red)
+    H = rng.uniform(0, 1, (op.n_elements, 8))
+    snap = op.field_snapshot(3, 0.5, T(x, dev), T(d, dev), T(H, dev))
+    g = oop.degradation(d)
+    strain, stress, psi = O.field_snapshot_elements(oop, x, g)
+    assert rel(snap.strain, strain) < 1e-12 and rel(snap.stress, stress) < 1e-12 and rel(snap.psi_elastic, psi) < 1e-12
+    assert rel(snap.psi_driving, oop.driving_force(x).mean(axis=1)) < 1e-12
+    assert rel(snap.macro_stress, oop.macro_stress(x, g)) < 1e-12
+    np.testing.assert_array_equal(snap.history_max, H.max(axis=1))
+    np.testing.assert_allclose(snap.history_mean, H.mean(axis=1), rtol=1e-15)
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    red = ((i % nx) + nx * ((j % ny) + ny * (k % nz))).ravel()
+    np.testing.assert_array_equal(snap.damage, d[red])
+    u = x[:-6].reshape(-1, 3)
+    np.testing.assert_array_equal(snap.fluctuation, u[red])
+    e = x[-6:]
+    X = np.stack([i.ravel() * op.grid.h[0], j.ravel() * op.grid.h[1], k.ravel() * op.grid.h[2]], axis=1)
+    ux = e[0] * X[:, 0] + e[5] / 2 * X[:, 1] + e[4] / 2 * X[:, 2] + u[red, 0]
+    uz = e[4] / 2 * X[:, 0] + e[3] / 2 * X[:, 1] + e[2] * X[:, 2] + u[red, 2]
+    np.testing.assert_allclose(snap.displacement[:, 0], ux, rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(snap.displacement[:, 2], uz, rtol=1e-12, atol=1e-18)
+    assert snap.step == 3 and snap.time_value == 0.5 and snap.psi_closure is None

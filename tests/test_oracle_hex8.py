@@ -195,3 +195,25 @@ def test_sphere_pore_geometry():
     c = (np.arange(16) + 0.5) - 8
     dd = np.sqrt(c[None, None, :] ** 2 + c[None, :, None] ** 2 + c[:, None, None] ** 2)
     assert (phase == G.PORE).sum() == (dd < r).sum()
+
+
+def test_field_snapshot_closed_form_and_invariants():
+    """Snapshot element means: a homogeneous cell gives strain E, stress g C E and energy
+    0.5 g E.C E in every voxel; for any state the voxel-mean stress averages to the macro
+    stress and the energies sum to the elastic energy 0.5 x.A(g) x."""
+    op = _op(two_phase=False)
+    E6 = np.array([1e-3, -2e-4, 3e-4, 1e-4, -5e-4, 2e-4])
+    x = np.concatenate([np.zeros(3 * op.n_red), E6])
+    C = op.bank[0]
+    strain, stress, psi = O.field_snapshot_elements(op, x)
+    np.testing.assert_allclose(strain, np.broadcast_to(E6, strain.shape), rtol=1e-13)
+    np.testing.assert_allclose(stress, np.broadcast_to(C @ E6, stress.shape), rtol=1e-13)
+    np.testing.assert_allclose(psi, 0.5 * E6 @ C @ E6, rtol=1e-13)
+    op = _op(shape=(4, 3, 5))
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-1e-3, 1e-3, 3 * op.n_red), rng.uniform(-1e-3, 1e-3, 6)])
+    g = op.degradation(rng.uniform(0, 1, op.n_red))
+    strain, stress, psi = O.field_snapshot_elements(op, x, g)
+    np.testing.assert_allclose(stress.mean(axis=0), op.macro_stress(x, g), rtol=1e-12)
+    np.testing.assert_allclose(psi.sum() * op.volume / op.E, op.elastic_energy(x, g), rtol=1e-12)
+    np.testing.assert_allclose(strain.mean(axis=0), x[-6:], rtol=1e-12, atol=1e-18)  # periodic fluctuation averages out
