@@ -194,6 +194,12 @@ class StaggeredSolver:
         dE_rel = dE / scale_E if scale_E > 0.0 else 0.0
         return max(du_rel, dE_rel, dd)
 
+    def snapshot(self, state: SolverState):
+        """CPU fields of ``state`` (solid3d/staggered.py:517-522); the time value is the
+        controlled macro strain."""
+        return self.op.field_snapshot(state.step, float(state.macro_strain[self._controlled]), state.x, state.d,
+                                      state.H)
+
     def fracture_energy(self, d) -> float:
         """Gamma(d) = 1/2 sum_e Gc d_e.(M/ell + ell L) d_e (operator.py:1511-1544, total)."""
         return 0.5 * float(torch.dot(d, self.op.damage_action(d)))

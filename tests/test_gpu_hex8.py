@@ -235,6 +235,10 @@ def test_staggered_increments_track_oracle(dev):
         assert rel(st.d, ost["d"]) < 1e-8 and rel(st.x, ost["x"]) < 1e-8
         assert rel(st.macro_stress, sigma) < 1e-8
     assert float(st.d.max()) > 0.05  # damage actually evolved
+    snap = solver.snapshot(st)
+    assert snap.time_value == 1.2e-2 and snap.step == st.step
+    np.testing.assert_allclose(snap.macro_stress, st.macro_stress.cpu().numpy(), rtol=1e-12)
+    assert rel(snap.history_max, st.H.amax(dim=1)) == 0.0
 
 
 @pytest.mark.parametrize("shape", [(16, 16, 16), (24, 16, 32)])
