@@ -1238,6 +1238,7 @@ extern "C" int phast_t3_damage_solve(phast_t3_plan *pl, int32_t method, const do
 extern "C" int phast_t3_set_fracture(phast_t3_plan *pl, double Gc, double l0, double at1_source) {
     PHAST_REQUIRE(pl, "phast_t3_set_fracture: null plan");
     PHAST_REQUIRE(Gc > 0.0 && l0 > 0.0 && std::isfinite(Gc) && std::isfinite(l0), "Gc and l0 must be positive");
+    PHAST_REQUIRE(std::isfinite(at1_source), "at1_source must be finite");
     T3Mat &M = pl->mat;  // as in phast_t3_plan_create (damage_solver.py:574-585)
     M.Gc_l0 = M.at1 ? 0.75 * Gc * l0 : Gc * l0;
     M.Gc_over_l0 = M.at1 ? 0.0 : Gc / l0;

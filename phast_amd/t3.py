@@ -295,7 +295,9 @@ class PhaseFieldDamageSolver:
     ``bounds_method`` 'projected_cg' runs the projected Jacobi-PCG on ``[d_prev, 1]``;
     'post_clamp' (the reference default) runs unconstrained PCG and clamps.  The iteration
     lives on the device; the host reads a convergence flag every few iterations.
-    ``Gc_field`` (E,) selects the per-element Gc branch.
+    ``Gc_field`` (E,) selects the per-element Gc branch.  The preconditioner is Jacobi whatever
+    ``use_multigrid`` says (the reference's ``preconditioner='jacobi'`` setting; its GPU default
+    'auto' would build a two-level aggregation multigrid, which is not on the HIP path).
     """
 
     def __init__(self, fem: FEMOperators, tol: float = 1e-5, max_iter: int = 5000, ctx=None,
@@ -317,6 +319,9 @@ class PhaseFieldDamageSolver:
         self._gamma_correction = bool(getattr(self.material, "gamma_correction", False))
         if self._gamma_correction:
             self.Gc_field = _gamma_Gc_field(self, Gc_field)
+        # the preconditioner actually used (damage_solver.py:1446-1480 'jacobi'); no multigrid
+        self._preconditioner = "jacobi"
+        self._use_multigrid = False
         self.last_iter = 0
         self.last_converged = False
         self.last_status = ""
