@@ -81,10 +81,11 @@ def _state(nodes, els, P, fem, om, omat, seed, fixed=None, vals=None, Gc_e=None)
     return H, d_prev, w
 
 
+@pytest.mark.parametrize("bounds", ["post_clamp", "projected_cg"])
 @pytest.mark.parametrize("pf_model", ["AT2", "AT1"])
 @pytest.mark.parametrize("mode", ["H", "scalar", "field"])
 @pytest.mark.parametrize("with_pins", [False, True])
-def test_backward_matches_oracle(dev, pf_model, mode, with_pins):
+def test_backward_matches_oracle(dev, pf_model, mode, with_pins, bounds):
     nodes, els = scrambled_mesh(21, 13, seed=35)
     P, fem, om, omat = setup(nodes, els, pf_model)
     H, d_prev, w = _state(nodes, els, P, fem, om, omat, 36)
@@ -92,7 +93,7 @@ def test_backward_matches_oracle(dev, pf_model, mode, with_pins):
     mask = None if fixed is None else torch.as_tensor(fixed, device=dev)
     vt = None if vals is None else T(vals, dev)
     Gc_e = omat.Gc * (1.0 + 0.3 * np.random.default_rng(37).uniform(-1.0, 1.0, len(els)))
-    dmg = P.PhaseFieldDamageSolver(fem, tol=1e-10, max_iter=3000)
+    dmg = P.PhaseFieldDamageSolver(fem, tol=1e-10, max_iter=3000, bounds_method=bounds)
     Ht = T(H, dev).requires_grad_(True)
     dp = T(d_prev, dev).requires_grad_(True)
     kw = {}
@@ -107,7 +108,7 @@ def test_backward_matches_oracle(dev, pf_model, mode, with_pins):
     d = dmg.solve(Ht, dp, pf_dirichlet_mask=mask, pf_dirichlet_values=vt, **kw)
     assert d.requires_grad
     (d * T(w, dev)).sum().backward()
-    adj = OA.T3DamageAdjoint(om, omat, tol=1e-10, max_iter=3000)
+    adj = OA.T3DamageAdjoint(om, omat, tol=1e-10, max_iter=3000, bounds_method=bounds)
     ok = dict(Gc=2.9, l0=0.45) if mode == "scalar" else dict(l0=0.45, Gc_e=Gc_e) if mode == "field" else {}
     od = adj.forward(H, d_prev, mode, fixed=fixed, vals=vals, **ok)
     assert rel(d, od) < 1e-10
@@ -125,7 +126,7 @@ def test_backward_matches_oracle(dev, pf_model, mode, with_pins):
         assert abs(float(kw["l0"].grad) - ref["l0"]) <= TOL * abs(ref["l0"])
     # the plan's constants are restored: a plain solve equals the oracle's at the base material
     d2 = dmg.solve(T(H, dev), T(d_prev, dev), pf_dirichlet_mask=mask, pf_dirichlet_values=vt)
-    od2 = O.T3DamageSolver(om, omat, tol=1e-10, max_iter=3000, bounds_method="post_clamp").solve(
+    od2 = O.T3DamageSolver(om, omat, tol=1e-10, max_iter=3000, bounds_method=bounds).solve(
         H, d_prev, fixed=fixed, vals=vals)
     assert rel(d2, od2) < 1e-10
 
