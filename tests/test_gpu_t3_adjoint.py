@@ -243,3 +243,30 @@ def test_gamma_correction_solve_and_scalar_adjoint(dev, pf_model):
     ref = adj.backward(H, d_prev, d.detach().cpu().numpy(), w, "field", l0=0.5, Gc_e=2.7 * gam)
     g = float((gam * ref["Gc"]).sum())
     assert abs(float(Gs.grad) - g) <= TOL * abs(g)
+
+
+def test_backward_edge_states(dev):
+    """Edge states of the active-set split: every node held by irreversibility (the whole
+    upstream gradient passes to d_prev, lam = 0, dL/dH = 0) and saturated nodes (d_prev = d = 1:
+    upper-active, no gradient to either input)."""
+    nodes, els = scrambled_mesh(9, 7, seed=43)
+    P, fem, om, omat = setup(nodes, els)
+    n, ne = len(nodes), len(els)
+    rng = np.random.default_rng(44)
+    w = rng.uniform(-1.0, 1.0, n)
+    dmg = P.PhaseFieldDamageSolver(fem, tol=1e-10, max_iter=3000)
+    dmg.differentiable = True
+    H = T(np.full(ne, 1e-6), dev).requires_grad_(True)
+    dp = T(np.full(n, 0.9), dev).requires_grad_(True)  # far above the tiny-history solution
+    d = dmg.solve(H, dp)
+    assert np.all(d.detach().cpu().numpy() == 0.9)
+    (d * T(w, dev)).sum().backward()
+    assert exact(dp.grad, w) and float(H.grad.abs().max()) == 0.0
+    assert float(dmg.last_adjoint["lam"].abs().max()) == 0.0
+    H2 = T(np.full(ne, 1e-6), dev).requires_grad_(True)
+    sat = rng.uniform(0, 1, n) < 0.3
+    dp2 = T(np.where(sat, 1.0, 0.9), dev).requires_grad_(True)
+    d2 = dmg.solve(H2, dp2)
+    (d2 * T(w, dev)).sum().backward()
+    assert exact(dp2.grad, np.where(sat, 0.0, w))
+    assert float(H2.grad.abs().max()) == 0.0
