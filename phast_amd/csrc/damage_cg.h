@@ -89,6 +89,52 @@ __device__ __forceinline__ double reduce_min_parts(const double *part, int off, 
     return block_min<NT>(fmin(fmin(s, s2), fmin(s3, s4)), red);
 }
 
+// reduce_sum_parts of row 0 and reduce_min_parts of rows 1 and 2 (each n_tiles apart) in one
+// pass: the three rows' loads are in flight together; the sum keeps reduce_sum_parts' order
+// (same bits) and the minima are exact in any order.
+template <int NT>
+__device__ __forceinline__ void reduce_sum_min2_parts(const double *part, int off, int cnt, int n_tiles,
+                                                      double *red, double &sum, double &mn1, double &mn2) {
+    constexpr int V = 256 / NT;
+    const double *p0 = part + off, *p1 = part + n_tiles + off, *p2 = part + 2 * n_tiles + off;
+    double w[V], a1 = __builtin_huge_val(), a2 = a1;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        double s = 0.0;
+        int k = threadIdx.x + v * NT;
+        for (; k + 7 * 256 < cnt; k += 8 * 256) {
+            double a[8], b[8], c[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                a[j] = p0[k + j * 256];
+                b[j] = p1[k + j * 256];
+                c[j] = p2[k + j * 256];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                s += a[j];
+                a1 = fmin(a1, b[j]);
+                a2 = fmin(a2, c[j]);
+            }
+        }
+        for (; k < cnt; k += 256) {
+            s += p0[k];
+            a1 = fmin(a1, p1[k]);
+            a2 = fmin(a2, p2[k]);
+        }
+        w[v] = wave_sum(s);
+    }
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) red[v * (NT / kWave) + threadIdx.x / kWave] = w[v];
+    }
+    __syncthreads();
+    sum = ((red[0] + red[1]) + red[2]) + red[3];
+    mn1 = block_min<NT>(a1, red);
+    mn2 = block_min<NT>(a2, red);
+}
+
 // One per-instance scalar step of the CG (thread 0 writes S).  Every thread of the block
 // takes part in the reductions; all return paths are block-uniform.
 template <int OP, int NT>
@@ -110,11 +156,11 @@ __device__ __forceinline__ void scalar_step(int32_t off, int32_t cnt, DamageCG &
             S.limited = S.check = S.restart = 0;
         }
     } else if (OP == S_A || OP == S_PC_A) {
-        const double pAp = reduce_sum_parts<NT>(part, off, cnt, red);
-        double up = 0.0, dn = 0.0;
+        double pAp, up = 0.0, dn = 0.0;
         if (OP == S_A) {
-            up = reduce_min_parts<NT>(part, n_tiles + off, cnt, red);
-            dn = reduce_min_parts<NT>(part, 2 * n_tiles + off, cnt, red);
+            reduce_sum_min2_parts<NT>(part, off, cnt, n_tiles, red, pAp, up, dn);
+        } else {
+            pAp = reduce_sum_parts<NT>(part, off, cnt, red);
         }
         if (threadIdx.x == 0) {
             S.iter += 1;
