@@ -89,6 +89,52 @@ __device__ __forceinline__ double reduce_min_parts(const double *part, int off, 
     return block_min<NT>(fmin(fmin(s, s2), fmin(s3, s4)), red);
 }
 
+// reduce_sum_parts of two rows (part + off and part + off2) in one pass, each in
+// reduce_sum_parts' order (same bits), the loads of both rows in flight together.
+template <int NT>
+__device__ __forceinline__ void reduce_sum2_parts(const double *part, int off, int off2, int cnt, double *red,
+                                                  double &s1, double &s2) {
+    constexpr int V = 256 / NT;
+    const double *p0 = part + off, *p1 = part + off2;
+    double w0[V], w1[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        double s = 0.0, t = 0.0;
+        int k = threadIdx.x + v * NT;
+        for (; k + 7 * 256 < cnt; k += 8 * 256) {
+            double a[8], b[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                a[j] = p0[k + j * 256];
+                b[j] = p1[k + j * 256];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                s += a[j];
+                t += b[j];
+            }
+        }
+        for (; k < cnt; k += 256) {
+            s += p0[k];
+            t += p1[k];
+        }
+        w0[v] = wave_sum(s);
+        w1[v] = wave_sum(t);
+    }
+    constexpr int NW = NT / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            red[v * NW + threadIdx.x / kWave] = w0[v];
+            red[4 + v * NW + threadIdx.x / kWave] = w1[v];
+        }
+    }
+    __syncthreads();
+    s1 = ((red[0] + red[1]) + red[2]) + red[3];
+    s2 = ((red[4] + red[5]) + red[6]) + red[7];
+}
+
 // reduce_sum_parts of row 0 and reduce_min_parts of rows 1 and 2 (each n_tiles apart) in one
 // pass: the three rows' loads are in flight together; the sum keeps reduce_sum_parts' order
 // (same bits) and the minima are exact in any order.
@@ -135,8 +181,8 @@ __device__ __forceinline__ void reduce_sum_min2_parts(const double *part, int of
     mn2 = block_min<NT>(a2, red);
 }
 
-// One per-instance scalar step of the CG (thread 0 writes S).  Every thread of the block
-// takes part in the reductions; all return paths are block-uniform.
+// One per-instance scalar step of the CG (thread 0 writes S; red holds 8 doubles).  Every
+// thread of the block takes part in the reductions; all return paths are block-uniform.
 template <int OP, int NT>
 __device__ __forceinline__ void scalar_step(int32_t off, int32_t cnt, DamageCG &S, const double *part,
                                             int32_t n_tiles, int32_t check_every, double *red) {
@@ -189,8 +235,8 @@ __device__ __forceinline__ void scalar_step(int32_t off, int32_t cnt, DamageCG &
             if (threadIdx.x == 0) S.restart = 0;
             return;
         }
-        const double rfn = reduce_sum_parts<NT>(part, off, cnt, red);
-        const double chg = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
+        double rfn, chg;
+        reduce_sum2_parts<NT>(part, off, n_tiles + off, cnt, red, rfn, chg);
         if (threadIdx.x == 0) {
             S.rfn = rfn;
             if (rfn < S.tol_sq) {
@@ -218,8 +264,8 @@ __device__ __forceinline__ void scalar_step(int32_t off, int32_t cnt, DamageCG &
         }
     } else if (OP == S_AJ_INIT) {
         // rows: r.r (unused), rhs.rhs, r.z; tol_sq = rel_tol * max(rhs.rhs, 1e-30) with S.tol = rel_tol
-        const double bb = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
-        const double rz = reduce_sum_parts<NT>(part, 2 * n_tiles + off, cnt, red);
+        double bb, rz;
+        reduce_sum2_parts<NT>(part, n_tiles + off, 2 * n_tiles + off, cnt, red, bb, rz);
         if (threadIdx.x == 0) {
             S.bb = bb;
             S.rz = rz;
@@ -242,8 +288,8 @@ __device__ __forceinline__ void scalar_step(int32_t off, int32_t cnt, DamageCG &
             S.alpha = S.rz / (pAp + 1e-30);
         }
     } else if (OP == S_AJ_C) {
-        const double rr = reduce_sum_parts<NT>(part, off, cnt, red);
-        const double rz_new = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
+        double rr, rz_new;
+        reduce_sum2_parts<NT>(part, off, n_tiles + off, cnt, red, rr, rz_new);
         if (threadIdx.x == 0) {
             S.rfn = rr;
             if (rr < S.tol_sq) {  // converged after the update, before the direction
@@ -259,8 +305,8 @@ __device__ __forceinline__ void scalar_step(int32_t off, int32_t cnt, DamageCG &
             }
         }
     } else if (OP == S_PC_C) {
-        const double rr = reduce_sum_parts<NT>(part, off, cnt, red);
-        const double rz_new = reduce_sum_parts<NT>(part, n_tiles + off, cnt, red);
+        double rr, rz_new;
+        reduce_sum2_parts<NT>(part, off, n_tiles + off, cnt, red, rr, rz_new);
         if (threadIdx.x == 0) {
             S.rfn = rr;
             if (rr < S.tol_sq) {  // damage_solver.py:3496-3506

@@ -1064,7 +1064,7 @@ __global__ __launch_bounds__(NT) void q4_vec_kernel(VecArgs A) {
 template <int OP>
 __global__ __launch_bounds__(256) void q4_scalar_kernel(const Q4Inst *inst, Q4CG *cg, const double *part,
                                                         int32_t n_tiles, int32_t check_every) {
-    __shared__ double red[4];
+    __shared__ double red[8];
     Q4CG &S = cg[blockIdx.x];
     if (OP != S_INIT && OP != S_PC_INIT && S.done) return;
     {
@@ -1143,7 +1143,7 @@ __device__ __forceinline__ void minv_strip(const Q4Inst *inst, const int32_t *bl
 template <int NT, int METHOD>
 __global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
     __shared__ double sm[kDmgLds<NT>];
-    __shared__ double red[4];
+    __shared__ double red[8];
     __shared__ Q4CG S;
     const int k = uniform_at(A.pb_inst, blockIdx.x), rank = uniform_at(A.pb_rank, blockIdx.x),
               nb = uniform_at(A.nblk, k);

@@ -753,7 +753,7 @@ __global__ __launch_bounds__(256) void t3_vec_kernel(int64_t nn, T3Cg A) {
 template <int OP>
 __global__ __launch_bounds__(256) void t3_scalar_kernel(DamageCG *cg, const double *part, int32_t nblk,
                                                         int32_t check_every) {
-    __shared__ double red[4];
+    __shared__ double red[8];
     DamageCG &S = cg[0];
     if (OP != S_INIT && OP != S_PC_INIT && OP != S_AJ_INIT && S.done) return;
     scalar_step<OP, 256>(0, nblk, S, part, nblk, check_every, red);
