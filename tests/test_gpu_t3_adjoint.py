@@ -270,3 +270,20 @@ def test_backward_edge_states(dev):
     (d2 * T(w, dev)).sum().backward()
     assert exact(dp2.grad, np.where(sat, 0.0, w))
     assert float(H2.grad.abs().max()) == 0.0
+
+
+def test_adjoint_pcg_stopping_edges(dev):
+    """The device PCG's edges match the oracle's: max_iter = 0 returns the start (pins at their
+    values, 0 elsewhere) after no iteration; a zero right-hand side stops on p.Ap <= 0 in the
+    first iteration with x = 0."""
+    nodes, els = scrambled_mesh(9, 7, seed=45)
+    P, fem, om, omat = setup(nodes, els)
+    fixed, vals = pins(nodes)
+    dmg = P.PhaseFieldDamageSolver(fem, max_iter=0)
+    H = T(np.full(len(els), 1.0), dev)
+    rhs = T(np.random.default_rng(46).uniform(-1.0, 1.0, len(nodes)), dev)
+    x = dmg.adjoint_pcg(H, rhs, torch.as_tensor(fixed, device=dev), T(vals, dev), 1e-22)
+    assert dmg.last_adjoint_iter == 0 and exact(x, np.where(fixed, vals, 0.0))
+    dmg.max_iter = 50
+    x = dmg.adjoint_pcg(H, torch.zeros_like(rhs), None, None, 1e-22)
+    assert dmg.last_adjoint_iter == 1 and exact(x, np.zeros(len(nodes)))

@@ -197,3 +197,20 @@ def test_scalar_override_resets_the_at1_threshold():
     assert mat.at1_source == 0.4
     o = A.override_material(mat, 3.0, 0.25)
     assert o.at1_source == pytest.approx(3.0 * 3.0 / (8.0 * 0.25), rel=1e-15)
+
+
+def test_adjoint_pcg_stopping_edges():
+    """max_iter = 0 returns the start (pins at their values, 0 elsewhere); a zero right-hand
+    side converges at once; a zero operator stops on p.Ap <= 0 before any update."""
+    m, mat = plate(), material("AT2")
+    op = O.T3Damage(m, mat)
+    H = np.full(m.n_elems, 1.0)
+    Minv = 1.0 / (op.diagonal(H) + 1e-30)
+    fixed, vals = pin_set(m)
+    rhs = np.random.default_rng(9).uniform(-1.0, 1.0, m.n_nodes)
+    x, it = A.pcg_adjoint(lambda v: op.apply(v, H), rhs, Minv, fixed, vals, None, 1e-22, 0)
+    assert it == 0 and np.array_equal(x, np.where(fixed, vals, 0.0))
+    x, it = A.pcg_adjoint(lambda v: op.apply(v, H), np.zeros(m.n_nodes), Minv, None, None, None, 1e-22, 50)
+    assert it == 1 and np.all(x == 0.0)
+    x, it = A.pcg_adjoint(lambda v: np.zeros_like(v), rhs, Minv, None, None, None, 1e-22, 50)
+    assert it == 1 and np.all(x == 0.0)
