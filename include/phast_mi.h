@@ -389,8 +389,25 @@ int phast_hex8_mech_solve_mg(const phast_hex8_plan *plan, phast_hex8_mg *mg, con
                              const double *diag, double rtol, double atol, int32_t maxiter, int32_t check_every,
                              int64_t n_mech, double *x_out, double *info, hipStream_t stream);
 
-/* Frees the process-wide solver scratch of the 3D solves. */
-int phast_release_solver_scratch(void);
+/* Solver scratch of a plan (3n+6 doubles per vector: the Krylov vectors, the constrained-solve
+ * vectors and the flag state).  The first solve on a plan makes it unless reserve was called;
+ * it is freed with the plan.  Solves on different plans share nothing, so they may run on
+ * different streams at once; two solves on one plan must not overlap (as in the reference,
+ * where an operator owns its tensors). */
+int phast_hex8_solver_reserve(const phast_hex8_plan *plan);
+int phast_hex8_solver_release(const phast_hex8_plan *plan);
+
+/* linalg.pcg (solid3d/linalg.py:99-220) for an operator evaluated by the caller: the Jacobi
+ * PCG with the flag semantics above, whose apply writes the vector into in_buf (n doubles,
+ * device), calls apply(ctx) -- which must enqueue out_buf = A(in_buf) on `stream` and return
+ * 0 -- and reads out_buf; entries flagged in `fixed` (uint8 (n,), or null) are zeroed after
+ * each apply.  b, diag (n,), x0 (n,) or null; x (n,) out; info[5] as phast_hex8_mech_solve.
+ * Scratch is allocated for the call and freed before it returns. */
+typedef int (*phast_apply_callback)(void *ctx);
+int phast_pcg_callback(phast_apply_callback apply, void *ctx, int64_t n, double *in_buf, const double *out_buf,
+                       const uint8_t *fixed, const double *b, const double *diag, const double *x0, double *x,
+                       double rtol, double atol, int32_t maxiter, int32_t check_every, double *info,
+                       hipStream_t stream);
 
 
 /* ---- adjoint pieces of the converged staggered voxel state ----------------------------

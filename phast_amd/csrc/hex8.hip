@@ -1143,6 +1143,7 @@ extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
     (void)hipFree(pl->mu);
     (void)hipFree(pl->part);
     (void)hipFree(pl->seam);
+    release_solver_work(pl);
     delete pl;
     return PHAST_OK;
 }

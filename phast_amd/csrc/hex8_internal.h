@@ -33,6 +33,9 @@ struct phast_hex8_plan {
     int64_t seam_cap = 0;
     int32_t max_blocks = 0;
     int32_t tx = 16, ty = 8, kz = 8;  // launch shape of the apply kernels (phast_hex8_set_tiles)
+    // scratch of the solves on this plan (linalg.hip): vectors of 3n+6, the flag state and the
+    // pinned readback; made by phast_hex8_solver_reserve or the first solve, freed with the plan
+    mutable void *solver_work = nullptr;
 };
 
 namespace phast {
@@ -47,4 +50,6 @@ int hex8_apply_jacobi(const phast_hex8_plan *pl, double *x, const double *g, con
                       const double *b, const double *diag, const uint8_t *fixed, double omega, hipStream_t st);
 // Grow the seam buffer of the voxel-parallel apply for slabs of kz node layers (set-up time).
 int hex8_ensure_seam(phast_hex8_plan *pl, int32_t kz);
+// Free the solver scratch of a plan (linalg.hip; called by phast_hex8_plan_destroy).
+void release_solver_work(const phast_hex8_plan *pl);
 }  // namespace phast

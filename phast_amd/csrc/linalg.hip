@@ -10,11 +10,10 @@
 // (per-block wave64 butterflies into partials, then one block), so runs are bitwise
 // reproducible.  Operators are supplied as host callbacks that enqueue an apply kernel.
 #include "common.h"
+#include "hex8_internal.h"
 
 #include <cstring>
 #include <vector>
-
-struct phast_hex8_plan;  // defined in hex8.hip
 
 namespace phast {
 
@@ -300,12 +299,17 @@ __global__ __launch_bounds__(256) void k_box_update(int64_t n, const double *Aca
 typedef int (*ApplyFn)(void *ctx, const double *in, double *out, hipStream_t st);
 typedef ApplyFn PrecondFn;  // z = M^-1 r (zero on the fixed set)
 
+// Scratch of the solves on one plan: five Krylov vectors, the constrained-solve vectors
+// (x_c, masked rhs and diagonal, x0, result, free mask), the flag state and its pinned
+// readback.  Sized for n_mech = 3n + 6 (the damage solves use the first n entries).
 struct Work {
     double *part = nullptr;
     PcgState *S = nullptr;
     PcgState *hS = nullptr;
     double *h3 = nullptr, *d3 = nullptr;
     double *r = nullptr, *p = nullptr, *z = nullptr, *Ap = nullptr, *dg = nullptr;
+    double *xc = nullptr, *bf = nullptr, *df = nullptr, *x0f = nullptr, *y = nullptr;
+    uint8_t *freem = nullptr;
     int64_t cap = 0;
 
     int ensure(int64_t n) {
@@ -317,16 +321,16 @@ struct Work {
             PHAST_HIP(hipMalloc(&d3, sizeof(double) * 8));
         }
         if (n > cap) {
-            (void)hipFree(r);
-            (void)hipFree(p);
-            (void)hipFree(z);
-            (void)hipFree(Ap);
-            (void)hipFree(dg);
-            PHAST_HIP(hipMalloc(&r, sizeof(double) * n));
-            PHAST_HIP(hipMalloc(&p, sizeof(double) * n));
-            PHAST_HIP(hipMalloc(&z, sizeof(double) * n));
-            PHAST_HIP(hipMalloc(&Ap, sizeof(double) * n));
-            PHAST_HIP(hipMalloc(&dg, sizeof(double) * n));
+            double **vec[] = {&r, &p, &z, &Ap, &dg, &xc, &bf, &df, &x0f, &y};
+            for (double **v : vec) {
+                (void)hipFree(*v);
+                *v = nullptr;
+            }
+            (void)hipFree(freem);
+            freem = nullptr;
+            cap = 0;
+            for (double **v : vec) PHAST_HIP(hipMalloc(v, sizeof(double) * n));
+            PHAST_HIP(hipMalloc(&freem, (size_t)n));
             cap = n;
         }
         return PHAST_OK;
@@ -337,16 +341,29 @@ struct Work {
         if (hS) (void)hipHostFree(hS);
         if (h3) (void)hipHostFree(h3);
         (void)hipFree(d3);
-        (void)hipFree(r);
-        (void)hipFree(p);
-        (void)hipFree(z);
-        (void)hipFree(Ap);
-        (void)hipFree(dg);
+        double *vec[] = {r, p, z, Ap, dg, xc, bf, df, x0f, y};
+        for (double *v : vec) (void)hipFree(v);
+        (void)hipFree(freem);
         *this = Work();
     }
 };
 
-static Work g_work;  // solver scratch shared by the 3D solves of one process
+// The plan's scratch, created on first use and grown to n doubles per vector.
+static int plan_work(const phast_hex8_plan *pl, int64_t n, Work **out) {
+    if (!pl->solver_work) pl->solver_work = new Work();
+    Work *W = static_cast<Work *>(pl->solver_work);
+    if (int rc = W->ensure(n)) return rc;
+    *out = W;
+    return PHAST_OK;
+}
+
+void release_solver_work(const phast_hex8_plan *pl) {
+    if (!pl->solver_work) return;
+    Work *W = static_cast<Work *>(pl->solver_work);
+    W->release();
+    delete W;
+    pl->solver_work = nullptr;
+}
 
 static inline int nblk(int64_t n) { return (int)std::min<int64_t>(kVecBlocks, ceil_div(std::max<int64_t>(n, 1), 256)); }
 
@@ -370,11 +387,10 @@ static int zero_parts(Work &W, hipStream_t st) {
 // linalg.pcg on the free set: `free_apply` must return the masked action.
 // info: [iterations, converged, rhs_norm, residual_norm, relative_residual]; returns
 // PHAST_ERR_NONFINITE / PHAST_ERR_NONPOSITIVE on breakdown (messages as in the reference).
-static int pcg_flags(ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, const double *b, const double *diag,
-                     const double *x0, double *x, double rtol, double atol, int maxiter, int check_every,
-                     double *info, hipStream_t st, PrecondFn precond = nullptr, void *pctx = nullptr) {
-    Work &W = g_work;
-    if (int rc = W.ensure(n)) return rc;
+static int pcg_flags(Work &W, ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, const double *b,
+                     const double *diag, const double *x0, double *x, double rtol, double atol, int maxiter,
+                     int check_every, double *info, hipStream_t st, PrecondFn precond = nullptr,
+                     void *pctx = nullptr) {
     const int nb = nblk(n);
     // stats: ||b||, finite(b), diag > 0, finite(diag)   (linalg.py:139-155)
     PHAST_TRY0(zero_parts(W, st));
@@ -499,10 +515,9 @@ static int pcg_flags(ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, 
 
 // box_kkt_h8._pcg on the free set (vectors zero on `fixed`).  rhs must already be masked.
 // info: [iterations, converged, initial_norm, residual_norm, relative_residual]
-static int pcg_stop(ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, const double *rhs, const double *diag,
-                    const double *x0, double *x, double tol, int max_iter, int poll, double *info, hipStream_t st) {
-    Work &W = g_work;
-    if (int rc = W.ensure(n)) return rc;
+static int pcg_stop(Work &W, ApplyFn apply, void *ctx, int64_t n, const uint8_t *fixed, const double *rhs,
+                    const double *diag, const double *x0, double *x, double tol, int max_iter, int poll, double *info,
+                    hipStream_t st) {
     const int nb = nblk(n);
     // diag: the reduced diagonal; on fixed entries use 1 (vectors are zero there anyway)
     hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, 1.0, diag, 0.0, (const double *)nullptr, fixed,
@@ -676,18 +691,13 @@ static int mech_solve_impl(const phast_hex8_plan *plan, phast_hex8_mg *mg, const
     PHAST_REQUIRE(plan && fixed && fixed_values && diag && x_out && info, "phast_hex8_mech_solve: null argument");
     PHAST_REQUIRE(maxiter >= 1 && check_every >= 1, "maxiter and check_every must be >= 1");
     PHAST_REQUIRE(!(rtol == 0.0 && atol == 0.0) && rtol >= 0.0 && atol >= 0.0, "rtol and atol must not both be zero");
-    Work &W = g_work;
-    if (int rc = W.ensure(n_mech)) return rc;
+    Work *Wp = nullptr;
+    if (int rc = plan_work(plan, n_mech, &Wp)) return rc;
+    Work &W = *Wp;
     const int64_t n = n_mech;
     const int nb = nblk(n);
-    double *xc = nullptr, *bf = nullptr, *df = nullptr, *x0f = nullptr, *y = nullptr;
-    PHAST_HIP(hipMallocAsync((void **)&xc, sizeof(double) * n, st));
-    PHAST_HIP(hipMallocAsync((void **)&bf, sizeof(double) * n, st));
-    PHAST_HIP(hipMallocAsync((void **)&df, sizeof(double) * n, st));
-    PHAST_HIP(hipMallocAsync((void **)&y, sizeof(double) * n, st));
-    if (x0) PHAST_HIP(hipMallocAsync((void **)&x0f, sizeof(double) * n, st));
-    uint8_t *freem = nullptr;
-    PHAST_HIP(hipMallocAsync((void **)&freem, (size_t)n, st));
+    double *xc = W.xc, *bf = W.bf, *df = W.df, *x0f = x0 ? W.x0f : nullptr, *y = W.y;
+    uint8_t *freem = W.freem;
     hipLaunchKernelGGL(k_not_mask, dim3(nb), dim3(256), 0, st, n, fixed, freem);
     // x_c = where(fixed, values, 0); image = A x_c; b_f = where(free, rhs - image, 0)
     hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, fixed, fixed_values, (const double *)nullptr, 0.0, xc);
@@ -700,18 +710,12 @@ static int mech_solve_impl(const phast_hex8_plan *plan, phast_hex8_mg *mg, const
         hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, diag, (const double *)nullptr, 1.0, df);
         if (x0) hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, x0, (const double *)nullptr, 0.0, x0f);
         MechCtx ctx{plan, g, d, fixed, n};
-        rc = pcg_flags(mech_apply_cb, &ctx, n, fixed, bf, df, x0f, y, rtol, atol, maxiter, check_every, info, st,
+        rc = pcg_flags(W, mech_apply_cb, &ctx, n, fixed, bf, df, x0f, y, rtol, atol, maxiter, check_every, info, st,
                        mg ? mg_precond_cb : nullptr, mg);
         // x = where(free, y, x_c)
         if (rc == PHAST_OK)
             hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, y, xc, 0.0, x_out);
     }
-    (void)hipFreeAsync(xc, st);
-    (void)hipFreeAsync(bf, st);
-    (void)hipFreeAsync(df, st);
-    (void)hipFreeAsync(y, st);
-    if (x0f) (void)hipFreeAsync(x0f, st);
-    (void)hipFreeAsync(freem, st);
     return rc;
 }
 
@@ -724,13 +728,11 @@ extern "C" int phast_hex8_damage_dirichlet_pcg(const phast_hex8_plan *plan, cons
                                                hipStream_t st) {
     PHAST_REQUIRE(plan && b && diag && fixed && values && out && info, "phast_hex8_damage_dirichlet_pcg: null argument");
     PHAST_REQUIRE(tol > 0.0 && max_iter >= 1, "tolerance must be positive and max_iterations >= 1");
-    Work &W = g_work;
-    if (int rc = W.ensure(n)) return rc;
+    Work *Wp = nullptr;
+    if (int rc = plan_work(plan, 3 * n + 6, &Wp)) return rc;
+    Work &W = *Wp;
     const int nb = nblk(n);
-    double *known = nullptr, *rr = nullptr, *y = nullptr;
-    PHAST_HIP(hipMallocAsync((void **)&known, sizeof(double) * n, st));
-    PHAST_HIP(hipMallocAsync((void **)&rr, sizeof(double) * n, st));
-    PHAST_HIP(hipMallocAsync((void **)&y, sizeof(double) * n, st));
+    double *known = W.xc, *rr = W.bf, *y = W.y;
     hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, fixed, values, (const double *)nullptr, 0.0, known);
     DmgCtx ctx{plan, H, scale};
     int rc = dmg_apply_cb(&ctx, known, W.Ap, st);
@@ -750,14 +752,11 @@ extern "C" int phast_hex8_damage_dirichlet_pcg(const phast_hex8_plan *plan, cons
             PHAST_LAUNCHED();
             return PHAST_OK;
         };
-        rc = pcg_stop(masked, &mctx, n, fixed, rr, diag, initial, y, tol, max_iter, 4, info, st);
+        rc = pcg_stop(W, masked, &mctx, n, fixed, rr, diag, initial, y, tol, max_iter, 4, info, st);
         // solution = known with free entries from y
         if (rc == PHAST_OK)
             hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, fixed, known, y, 0.0, out);
     }
-    (void)hipFreeAsync(known, st);
-    (void)hipFreeAsync(rr, st);
-    (void)hipFreeAsync(y, st);
     return rc;
 }
 
@@ -771,8 +770,9 @@ extern "C" int phast_hex8_box_update(const phast_hex8_plan *plan, const double *
                                      hipStream_t st) {
     PHAST_REQUIRE(plan && b && cand && lower && upper && lower_active && upper_active && prescribed && grad && counts,
                   "phast_hex8_box_update: null argument");
-    Work &W = g_work;
-    if (int rc = W.ensure(n)) return rc;
+    Work *Wp = nullptr;
+    if (int rc = plan_work(plan, 3 * n + 6, &Wp)) return rc;
+    Work &W = *Wp;
     DmgCtx ctx{plan, H, scale};
     if (int rc = dmg_apply_cb(&ctx, cand, W.Ap, st)) return rc;
     PHAST_TRY0(zero_parts(W, st));
@@ -785,7 +785,57 @@ extern "C" int phast_hex8_box_update(const phast_hex8_plan *plan, const double *
     return PHAST_OK;
 }
 
-extern "C" int phast_release_solver_scratch(void) {
-    g_work.release();
+// Allocate the plan's solver scratch now (set-up time), so that no later solve on the plan
+// allocates.
+extern "C" int phast_hex8_solver_reserve(const phast_hex8_plan *plan) {
+    PHAST_REQUIRE(plan, "phast_hex8_solver_reserve: null plan");
+    Work *W = nullptr;
+    return plan_work(plan, 3 * plan->n + 6, &W);
+}
+
+extern "C" int phast_hex8_solver_release(const phast_hex8_plan *plan) {
+    PHAST_REQUIRE(plan, "phast_hex8_solver_release: null plan");
+    release_solver_work(plan);
     return PHAST_OK;
+}
+
+// linalg.pcg (solid3d/linalg.py:99-220) of an operator the caller evaluates: before each
+// apply the library writes the vector into `in_buf`, calls apply(ctx), which must enqueue
+// out_buf = A(in_buf) on `st` and return 0, and reads `out_buf`; entries flagged in `fixed`
+// (optional) are zeroed after each apply.  Scratch is allocated for the call and freed.
+extern "C" int phast_pcg_callback(phast_apply_callback apply, void *ctx, int64_t n, double *in_buf,
+                                  const double *out_buf, const uint8_t *fixed, const double *b, const double *diag,
+                                  const double *x0, double *x, double rtol, double atol, int32_t maxiter,
+                                  int32_t check_every, double *info, hipStream_t st) {
+    PHAST_REQUIRE(apply && in_buf && out_buf && b && diag && x && info && n >= 1, "phast_pcg_callback: null argument");
+    PHAST_REQUIRE(maxiter >= 1 && check_every >= 1, "maxiter and check_every must be >= 1");
+    PHAST_REQUIRE(!(rtol == 0.0 && atol == 0.0) && rtol >= 0.0 && atol >= 0.0, "rtol and atol must not both be zero");
+    struct Tramp {
+        phast_apply_callback fn;
+        void *ctx;
+        double *in;
+        const double *out;
+        const uint8_t *fixed;
+        int64_t n;
+    } t{apply, ctx, in_buf, out_buf, fixed, n};
+    auto cb = [](void *c, const double *in, double *out, hipStream_t s) -> int {
+        auto *T = static_cast<Tramp *>(c);
+        PHAST_HIP(hipMemcpyAsync(T->in, in, sizeof(double) * T->n, hipMemcpyDeviceToDevice, s));
+        if (int rc = T->fn(T->ctx)) {
+            set_error("phast_pcg_callback: the apply callback failed (%d)", rc);
+            return PHAST_ERR_ARG;
+        }
+        PHAST_HIP(hipMemcpyAsync(out, T->out, sizeof(double) * T->n, hipMemcpyDeviceToDevice, s));
+        if (T->fixed) {
+            hipLaunchKernelGGL(k_zero_fixed, dim3(nblk(T->n)), dim3(256), 0, s, T->n, T->fixed, out);
+            PHAST_LAUNCHED();
+        }
+        return PHAST_OK;
+    };
+    Work W;
+    int rc = W.ensure(n);
+    if (rc == PHAST_OK) rc = pcg_flags(W, cb, &t, n, fixed, b, diag, x0, x, rtol, atol, maxiter, check_every, info, st);
+    (void)hipStreamSynchronize(st);
+    W.release();
+    return rc;
 }

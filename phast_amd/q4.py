@@ -567,7 +567,7 @@ class StaggeredSolver:
     """
 
     def __init__(self, fem: FEMOperators, dyn: ExplicitDynamics, damage: PhaseFieldDamageSolver,
-                 f_ext=None, bc_mask=None, bc_vals=None, damage_every: int = 1, enable_damage: bool = True):
+                 f_ext=None, bc_mask=None, bc_vals=None, damage_every: int = 3, enable_damage: bool = True):
         self.fem, self.dyn, self.damage_solver = fem, dyn, damage
         n, E, dev = fem.mesh.n_nodes, fem.mesh.n_elems, fem.device
         self.u = torch.zeros((n, 2), dtype=F64, device=dev)

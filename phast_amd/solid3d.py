@@ -39,6 +39,8 @@ for _name, _args in {
                                         C.c_void_p],
     "phast_hex8_box_update": [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7 +
                              [C.c_double, C.c_int32, C.c_int64, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
+    "phast_hex8_solver_reserve": [C.c_void_p],
+    "phast_hex8_solver_release": [C.c_void_p],
 }.items():
     _fn = getattr(_L, _name)
     _fn.argtypes = _args
@@ -307,6 +309,7 @@ class PeriodicVoxelPhaseFieldOperator:
                                         mu.ctypes.data, drv.ctypes.data, self.kappa, self.ell, C.byref(h)),
               "phast_hex8_plan_create")
         self._plan = h
+        check(_L.phast_hex8_solver_reserve(h), "phast_hex8_solver_reserve")  # solves then allocate nothing
         self.set_tiles(*tiles)
         self.Gc = torch.as_tensor(Gc, device=self.device)
 
@@ -600,103 +603,79 @@ def _raise_scalar(rc: int) -> None:
 # linalg.pcg / solve_constrained
 # ----------------------------------------------------------------------------------------
 
-def pcg(action, rhs, *, diag, x0=None, rtol=1e-8, atol=0.0, maxiter=5000, check_every=10):
-    """Jacobi PCG (solid3d/linalg.py:99-220) for an arbitrary device action.
+_APPLY_CB = C.CFUNCTYPE(C.c_int, C.c_void_p)
+_L.phast_pcg_callback.argtypes = [_APPLY_CB, C.c_void_p, C.c_int64] + [C.c_void_p] * 7 + [
+    C.c_double, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.c_void_p]
+_L.phast_pcg_callback.restype = C.c_int
 
-    Operator-bound solves should use :meth:`PeriodicVoxelPhaseFieldOperator.solve_mechanics`
-    (device-resident loop).  This generic form keeps the reference's flag semantics with
-    torch device vectors and reads the flags every ``check_every`` iterations.
-    """
+
+def pcg(action, rhs, *, diag, x0=None, rtol=1e-8, atol=0.0, maxiter=5000, check_every=10, fixed=None):
+    """Jacobi PCG of ``linalg.pcg`` (solid3d/linalg.py:99-220) for any device action.
+
+    The Krylov loop is the library's (``phast_pcg_callback``): vector updates, the
+    flag-masked steps, the fixed-order dot products and the flag reads every
+    ``check_every`` iterations all run there.  Python is entered once per iteration, to
+    evaluate ``action`` on the direction the library places in a device buffer.
+    ``fixed`` (bool, optional) zeroes those entries of every image.  Operator-bound solves
+    are faster through :meth:`PeriodicVoxelPhaseFieldOperator.solve_mechanics`."""
     rtol, atol = float(rtol), float(atol)
     if rtol < 0 or atol < 0 or (rtol == 0 and atol == 0):
         raise ValueError("rtol and atol must be non-negative and not both zero")
-    stats = torch.stack((torch.linalg.vector_norm(rhs), torch.isfinite(rhs).all().to(rhs.dtype),
-                         (diag > 0.0).all().to(rhs.dtype), torch.isfinite(diag).all().to(rhs.dtype))).cpu()
-    b_norm = float(stats[0])
-    if stats[1] < 1.0 or not math.isfinite(b_norm):
-        raise PCGBreakdownError("PCG right-hand side contains a non-finite value")
-    if stats[3] < 1.0:
-        raise PCGBreakdownError("PCG diagonal contains a non-finite value")
-    if stats[2] < 1.0:
-        raise PCGBreakdownError("PCG diagonal has a non-positive entry (non-positive curvature)")
-    if b_norm == 0.0:
-        return torch.zeros_like(rhs), PCGInfo(0, True, 0.0, 0.0, 0.0)
-    threshold = max(rtol * b_norm, atol)
-    if x0 is None:
-        x, r, r_norm = torch.zeros_like(rhs), rhs.clone(), b_norm
-    else:
-        x = x0.clone()
-        r = rhs - action(x)
-        r_norm = float(torch.linalg.vector_norm(r))
-        if not math.isfinite(r_norm):
-            raise PCGBreakdownError("PCG initial residual is not finite")
-    if r_norm <= threshold:
-        return x, PCGInfo(0, True, b_norm, r_norm, r_norm / b_norm)
-    inv = 1.0 / diag
-    z = r * inv
-    p = z.clone()
-    rz = torch.dot(r, z)
-    thr2 = torch.tensor(threshold * threshold, dtype=rhs.dtype, device=rhs.device)
-    zero = torch.zeros((), dtype=rhs.dtype, device=rhs.device)
-    active = torch.ones((), dtype=torch.bool, device=rhs.device)
-    breakdown = torch.zeros_like(active)
-    nonfinite = torch.zeros_like(active)
-    done = torch.zeros((), dtype=torch.long, device=rhs.device)
-    rr = torch.dot(r, r)
-    for it in range(1, maxiter + 1):
-        Ap = action(p)
-        pAp = torch.dot(p, Ap)
-        positive = pAp > 0.0
-        breakdown = breakdown | (active & ~positive)
-        nonfinite = nonfinite | ~torch.isfinite(pAp)
-        step = active & positive
-        alpha = torch.where(step, rz / torch.where(positive, pAp, torch.ones_like(pAp)), zero)
-        x.add_(alpha * p)
-        r.sub_(alpha * Ap)
-        done = done + step.to(torch.long)
-        z = r * inv
-        rz_new = torch.dot(r, z)
-        rr = torch.dot(r, r)
-        nonfinite = nonfinite | ~torch.isfinite(rz_new) | ~torch.isfinite(rr)
-        active = active & (rr > thr2)
-        beta = torch.where(active & (rz > 0.0), rz_new / torch.where(rz > 0.0, rz, torch.ones_like(rz)), zero)
-        p = z + beta * p
-        rz = rz_new
-        if it % check_every == 0 or it == maxiter:
-            flags = torch.stack((nonfinite.to(rhs.dtype), breakdown.to(rhs.dtype), active.to(rhs.dtype), rr,
-                                 done.to(rhs.dtype))).cpu()
-            if flags[0] > 0:
-                raise PCGBreakdownError(f"PCG produced a non-finite value by iteration {it}")
-            if flags[1] > 0:
-                raise PCGBreakdownError(f"PCG met non-positive curvature p.A p <= 0 by iteration {it}; "
-                                        "the operator is not positive definite on the free set")
-            r_norm = math.sqrt(float(flags[3]))
-            if flags[2] < 1.0:
-                return x, PCGInfo(int(flags[4]), True, b_norm, r_norm, r_norm / b_norm)
-    return x, PCGInfo(maxiter, False, b_norm, r_norm, r_norm / b_norm)
+    if not isinstance(rhs, torch.Tensor) or rhs.dim() != 1 or rhs.dtype != F64 or rhs.device.type != "cuda":
+        raise TypeError("rhs must be a float64 vector on a ROCm device")
+    n, dev = rhs.shape[0], rhs.device
+    for name, t in (("diag", diag), ("x0", x0)):
+        if t is not None and (not isinstance(t, torch.Tensor) or t.shape != rhs.shape or t.dtype != F64
+                              or t.device != dev):
+            raise TypeError(f"{name} must be a float64 tensor of shape ({n},) on {dev}")
+    rhs, diag = rhs.contiguous(), diag.contiguous()
+    x0 = None if x0 is None else x0.contiguous()
+    fx = None if fixed is None else fixed.to(device=dev, dtype=torch.uint8).contiguous()
+    inbuf, outbuf = torch.empty_like(rhs), torch.empty_like(rhs)
+    failure = []
+
+    def evaluate(_ctx):
+        try:
+            y = action(inbuf)
+            if not isinstance(y, torch.Tensor) or y.shape != inbuf.shape:
+                raise ValueError(f"PCG action must return a tensor of shape ({n},)")
+            outbuf.copy_(y)
+            return 0
+        except BaseException as exc:  # re-raised below, after the library has unwound
+            failure.append(exc)
+            return 1
+
+    cb = _APPLY_CB(evaluate)
+    x = torch.empty_like(rhs)
+    info = (C.c_double * 5)()
+    rc = _L.phast_pcg_callback(cb, None, n, ptr(inbuf), ptr(outbuf), ptr(fx), ptr(rhs), ptr(diag), ptr(x0), ptr(x),
+                               rtol, atol, int(maxiter), int(check_every), info, stream())
+    if failure:
+        raise failure[0]
+    _raise_pcg(rc)
+    return x, PCGInfo(int(info[0]), bool(info[1]), info[2], info[3], info[4])
 
 
 def solve_constrained(action, *, diag, fixed_mask, fixed_values, rhs=None, x0=None, rtol=1e-8, atol=0.0,
                       maxiter=5000, check_every=10):
-    """Dirichlet-projected PCG (solid3d/linalg.py:223-287).
+    """Dirichlet-eliminated PCG of ``linalg.solve_constrained`` (solid3d/linalg.py:223-287):
+    the prescribed entries are moved to the right-hand side and the PCG runs on the rest.
 
-    ``action`` bound by :func:`mech_action_of` runs entirely on the device."""
+    An action made by :func:`mech_action_of` runs the whole solve in the library; any
+    other callable goes through :func:`pcg` (library loop, Python apply)."""
     bound = getattr(action, "_phast_mech", None)
     if bound is not None:
         op, g, d = bound
         return op.solve_mechanics(g=g, d=d, fixed_mask=fixed_mask, fixed_values=fixed_values, diag=diag, rhs=rhs,
                                   x0=x0, rtol=rtol, atol=atol, maxiter=maxiter, check_every=check_every)
-    free = ~fixed_mask
-    zero = torch.zeros_like(fixed_values)
-    x_c = torch.where(fixed_mask, fixed_values, zero)
-    image = action(x_c)
-    b = -image if rhs is None else rhs - image
-    b_f = torch.where(free, b, zero)
-    diag_f = torch.where(free, diag, torch.ones_like(diag))
-    x0_f = None if x0 is None else torch.where(free, x0, zero)
-    y, info = pcg(lambda v: torch.where(free, action(v), zero), b_f, diag=diag_f, x0=x0_f, rtol=rtol, atol=atol,
-                  maxiter=maxiter, check_every=check_every)
-    return torch.where(free, y, x_c), info
+    zero = torch.zeros((), dtype=fixed_values.dtype, device=fixed_values.device)
+    prescribed = torch.where(fixed_mask, fixed_values, zero)   # known part of the solution
+    lifted = -action(prescribed) if rhs is None else rhs - action(prescribed)
+    y, info = pcg(action, torch.where(fixed_mask, zero, lifted),
+                  diag=torch.where(fixed_mask, torch.ones_like(diag), diag),
+                  x0=None if x0 is None else torch.where(fixed_mask, zero, x0),
+                  rtol=rtol, atol=atol, maxiter=maxiter, check_every=check_every, fixed=fixed_mask)
+    return torch.where(fixed_mask, prescribed, y), info
 
 
 def mech_action_of(op: PeriodicVoxelPhaseFieldOperator, g=None, d=None):

@@ -319,7 +319,15 @@ class PhaseFieldDamageSolver:
         self._gamma_correction = bool(getattr(self.material, "gamma_correction", False))
         if self._gamma_correction:
             self.Gc_field = _gamma_Gc_field(self, Gc_field)
-        # the preconditioner actually used (damage_solver.py:1446-1480 'jacobi'); no multigrid
+        # the preconditioner actually used (damage_solver.py:1446-1480 'jacobi'); no multigrid.
+        # The reference resolves use_multigrid=True with no preconditioner to 'auto' for AT2
+        # (aggregation multigrid, check_every 5), so its CG iterates differ from these.
+        if use_multigrid and preconditioner is None and getattr(self.material, "pf_model", "AT2") == "AT2":
+            warnings.warn("T3 damage solve: the reference's default preconditioner for AT2 ('auto', aggregation "
+                          "multigrid) is not on the HIP path; using Jacobi (pass preconditioner='jacobi' or "
+                          "use_multigrid=False to select it explicitly).  The converged damage agrees to the "
+                          "solver tolerance; CG iteration counts and iterates differ from the reference's.",
+                          stacklevel=2)
         self._preconditioner = "jacobi"
         self._use_multigrid = False
         self.last_iter = 0

@@ -201,7 +201,7 @@ def test_explicit_staggered_steps_track_oracle(dev):
     dt = 0.5 * fem.dt_cfl
     dyn = P.ExplicitDynamics(fem, dt=dt)
     dmg = P.PhaseFieldDamageSolver(fem, tol=1e-12, max_iter=2000, bounds_method="projected_cg")
-    st = P.StaggeredSolver(fem, dyn, dmg, f_ext=T(f_ext, dev))
+    st = P.StaggeredSolver(fem, dyn, dmg, f_ext=T(f_ext, dev), damage_every=1)
     d0 = np.zeros(n)
     d0[(fem.mesh.nodes[:, 1] == 2.0) & (fem.mesh.nodes[:, 0] <= 5.0)] = 1.0
     st.d = T(d0, dev)
@@ -211,8 +211,10 @@ def test_explicit_staggered_steps_track_oracle(dev):
     for _ in range(30):
         st.step_full()
         O.step_full_explicit(odyn, odmg, ost, f_ext)
-    assert rel(st.u, ost.u) < 1e-9 and rel(st.v, ost.v) < 1e-9 and rel(st.d, ost.d) < 1e-9
-    assert rel(st.H_elem, ost.H) < 1e-9
+    print(f"30 steps: u {rel(st.u, ost.u):.1e} v {rel(st.v, ost.v):.1e} d {rel(st.d, ost.d):.1e} "
+          f"H {rel(st.H_elem, ost.H):.1e}")
+    assert rel(st.u, ost.u) < RTOL and rel(st.v, ost.v) < RTOL and rel(st.d, ost.d) < RTOL
+    assert rel(st.H_elem, ost.H) < RTOL
 
 
 def test_batched_plan_is_bitwise_equal_to_single_plans(dev):
@@ -232,7 +234,7 @@ def test_batched_plan_is_bitwise_equal_to_single_plans(dev):
         fem = P.FEMOperators(m, mat)
         dyn = P.ExplicitDynamics(fem, dt=bat.dt[k])
         dmg = P.PhaseFieldDamageSolver(fem, tol=bat.tol, max_iter=bat.max_iter, bounds_method="projected_cg")
-        st = P.StaggeredSolver(fem, dyn, dmg)
+        st = P.StaggeredSolver(fem, dyn, dmg, damage_every=1)
         u, v, a, d, H = (T(x, dev) for x in states[k])
         st.u, st.v, st.a, st.d, st.H_elem = u, v, a, d, H
         for _ in range(3):

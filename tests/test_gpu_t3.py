@@ -332,7 +332,7 @@ def test_explicit_staggered_steps_track_oracle(dev):
     dt = 0.5 * fem.dt_cfl
     dyn = P3.ExplicitDynamics(fem, dt=dt)
     dmg = P3.PhaseFieldDamageSolver(fem, tol=1e-12, max_iter=2000, bounds_method="projected_cg")
-    st = P3.StaggeredSolver(fem, dyn, dmg, f_ext=T(f_ext, dev))
+    st = P3.StaggeredSolver(fem, dyn, dmg, f_ext=T(f_ext, dev), damage_every=1)
     d0 = np.zeros(n)
     d0[mesh.node_sets["crack"]] = 1.0  # preseeded notch
     st.d = T(d0, dev)
@@ -342,11 +342,13 @@ def test_explicit_staggered_steps_track_oracle(dev):
     for k in range(30):
         psi = st.step_full()
         opsi = OQ.step_full_explicit(odyn, odmg, ost, f_ext)
-    assert rel(psi, opsi) < 1e-9
-    assert rel(st.u, ost.u) < 1e-9 and rel(st.v, ost.v) < 1e-9
-    assert rel(st.d, ost.d) < 1e-9 and rel(st.H_elem, ost.H) < 1e-9
+    print(f"30 steps: psi {rel(psi, opsi):.1e} u {rel(st.u, ost.u):.1e} v {rel(st.v, ost.v):.1e} "
+          f"d {rel(st.d, ost.d):.1e} H {rel(st.H_elem, ost.H):.1e}")
+    assert rel(psi, opsi) < 1e-10
+    assert rel(st.u, ost.u) < 1e-10 and rel(st.v, ost.v) < 1e-10
+    assert rel(st.d, ost.d) < 1e-10 and rel(st.H_elem, ost.H) < 1e-10
     assert float(st.d[torch.as_tensor(d0 == 0, device=dev)].max()) > 0.0
-    assert rel(st.H_nodal, ofem.mesh.elem_to_node(ost.H)) < 1e-9
+    assert rel(st.H_nodal, ofem.mesh.elem_to_node(ost.H)) < 1e-10
 
 
 @pytest.mark.parametrize("which", ["scrambled", "split_quad", "one_triangle"])
