@@ -155,6 +155,18 @@ int phast_q4_damage_solve(phast_q4_plan *plan, int32_t method, const double *H, 
                           double *d_out, uint8_t *active_out, int32_t *iters, int32_t *status,
                           hipStream_t stream);
 
+/* No counterpart in the reference.  phast_q4_damage_solve for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments, kernels and results (bit for bit); the CG scratch (vectors,
+ * strip partials, active mask, CG states) is a file-scope static device array of 36 MB that
+ * each GPU the library runs on holds until the process ends, so no call allocates.  Limits:
+ * plans of at most 524288 nodes, 65536 strips and 64 instances (PHAST_ERR_ARG beyond).
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU race. */
+int phast_q4_damage_solve_noalloc(phast_q4_plan *plan, int32_t method, const double *H, const double *Gc_cell,
+                                  const double *d_prev, const double *d_init, const uint8_t *fixed,
+                                  const double *fixed_vals, double tol, int32_t max_iter, double *d_out,
+                                  uint8_t *active_out, int32_t *iters, int32_t *status, hipStream_t stream);
+
 /* Persistent batched explicit step (no single-call counterpart in the reference: it fuses
  * the loop body of solvers/staggered_solver.py:1115-1165 with damage_every = 1, i.e.
  * ExplicitDynamics.step (mechanics_solver.py:204-239), the history max

@@ -64,6 +64,7 @@ _SIGNATURES = {
     "phast_q4_damage_rhs_diag": ([_P] * 6, C.c_int),
     "phast_q4_damage_apply": ([_P] * 6, C.c_int),
     "phast_q4_damage_solve": ([_P, _I32, _P, _P, _P, _P, _P, _P, _D, _I32, _P, _P, _P, _P, _P], C.c_int),
+    "phast_q4_damage_solve_noalloc": ([_P, _I32, _P, _P, _P, _P, _P, _P, _D, _I32, _P, _P, _P, _P, _P], C.c_int),
     "phast_q4_batch_prepare": ([_P, _P], C.c_int),
     "phast_q4_batch_step": ([_P, _I32] + [_P] * 12 + [_D, _I32] + [_P] * 9, C.c_int),
     "phast_q4_batch_error": ([_P, C.POINTER(_I32), _I32], C.c_int),

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace phast {
@@ -1297,9 +1298,9 @@ static int launch_vec(const phast_q4_plan *pl, const VecArgs &A, hipStream_t st)
     return PHAST_OK;
 }
 template <int OP>
-static int launch_scalar(const phast_q4_plan *pl, int check_every, hipStream_t st) {
-    hipLaunchKernelGGL((q4_scalar_kernel<OP>), dim3(pl->n_inst), dim3(256), 0, st, pl->d_inst, pl->cg,
-                       pl->part, pl->n_tiles, check_every);
+static int launch_scalar(const phast_q4_plan *pl, Q4CG *cg, double *part, int check_every, hipStream_t st) {
+    hipLaunchKernelGGL((q4_scalar_kernel<OP>), dim3(pl->n_inst), dim3(256), 0, st, pl->d_inst, cg, part,
+                       pl->n_tiles, check_every);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
@@ -1636,6 +1637,19 @@ static int ensure_work(phast_q4_plan *pl) {
     return PHAST_OK;
 }
 
+// Scratch of one damage solve: the CG vectors, the strip partials, the active mask, the
+// per-instance CG state on the device and its host copy.
+struct Q4SolveScratch {
+    double *r, *p, *p2, *z, *Ap, *b, *minv, *lb, *part;
+    uint8_t *active;
+    Q4CG *cg, *h_cg;
+};
+
+static int damage_solve_impl(phast_q4_plan *pl, const Q4SolveScratch &W, int32_t method, const double *H,
+                             const double *Gc_cell, const double *d_prev, const double *d_init, const uint8_t *fixed,
+                             const double *fixed_vals, double tol, int32_t max_iter, double *d_out,
+                             uint8_t *active_out, int32_t *iters, int32_t *status, hipStream_t st);
+
 extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const double *H, const double *Gc_cell,
                                      const double *d_prev, const double *d_init, const uint8_t *fixed,
                                      const double *fixed_vals, double tol, int32_t max_iter, double *d_out,
@@ -1645,51 +1659,116 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
     PHAST_REQUIRE(!fixed == !fixed_vals, "fixed and fixed_vals must be given together");
     PHAST_REQUIRE(tol >= 0.0 && max_iter >= 0, "tol and max_iter must be non-negative");
     PHAST_TRY(ensure_work(pl));
+    const Q4SolveScratch W{pl->r, pl->p, pl->p2, pl->z, pl->Ap, pl->b, pl->minv, pl->lb, pl->part,
+                           pl->active, pl->cg, pl->h_cg};
+    return damage_solve_impl(pl, W, method, H, Gc_cell, d_prev, d_init, fixed, fixed_vals, tol, max_iter, d_out,
+                             active_out, iters, status, st);
+}
+
+// ---------------------------------------------------------------------------------------
+// the damage solve that allocates nothing: its scratch is the file-scope array `scratch`
+// ---------------------------------------------------------------------------------------
+
+constexpr int64_t kNoAllocNodes = int64_t(1) << 19;  // nodes of all instances of the plan
+constexpr int64_t kNoAllocTiles = int64_t(1) << 16;  // strips of the plan
+constexpr int kNoAllocInst = 64;                     // instances of the plan
+constexpr int64_t kNoAllocCgWords = (int64_t)((sizeof(Q4CG) + 7) / 8);
+constexpr int64_t kNoAllocWords =
+    8 * kNoAllocNodes + kParts * kNoAllocTiles + kNoAllocNodes / 8 + kNoAllocInst * kNoAllocCgWords;
+constexpr int kNoAllocDevices = 64;
+
+// 8 node vectors | kParts x strips partials | active mask (1 byte per node) | CG states
+static __device__ __align__(256) double scratch[kNoAllocWords];
+
+static std::mutex g_scratch_mutex;
+static double *g_scratch_ptr[kNoAllocDevices];           // device address of `scratch`, per device
+static Q4CG g_scratch_host_cg[kNoAllocDevices][kNoAllocInst];  // host copy of the CG states, per device
+
+extern "C" int phast_q4_damage_solve_noalloc(phast_q4_plan *pl, int32_t method, const double *H,
+                                             const double *Gc_cell, const double *d_prev, const double *d_init,
+                                             const uint8_t *fixed, const double *fixed_vals, double tol,
+                                             int32_t max_iter, double *d_out, uint8_t *active_out, int32_t *iters,
+                                             int32_t *status, hipStream_t st) {
+    PHAST_REQUIRE(pl && H && d_prev && d_out, "phast_q4_damage_solve_noalloc: null argument");
+    PHAST_REQUIRE(method == 0 || method == 1, "method must be 0 (projected) or 1 (post-clamp)");
+    PHAST_REQUIRE(!fixed == !fixed_vals, "fixed and fixed_vals must be given together");
+    PHAST_REQUIRE(tol >= 0.0 && max_iter >= 0, "tol and max_iter must be non-negative");
+    PHAST_REQUIRE(pl->n_nodes <= kNoAllocNodes && pl->n_tiles <= kNoAllocTiles && pl->n_inst <= kNoAllocInst,
+                  "phast_q4_damage_solve_noalloc: plan too large for the static scratch (%lld nodes, %d strips, "
+                  "%d instances; at most %lld, %lld, %d)", (long long)pl->n_nodes, pl->n_tiles, pl->n_inst,
+                  (long long)kNoAllocNodes, (long long)kNoAllocTiles, kNoAllocInst);
+    int dev = 0;
+    PHAST_HIP(hipGetDevice(&dev));
+    PHAST_REQUIRE(dev >= 0 && dev < kNoAllocDevices, "phast_q4_damage_solve_noalloc: device %d out of range", dev);
+    double *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        if (!g_scratch_ptr[dev]) {  // first call on this device (it is current): resolve its copy
+            void *ptr = nullptr;
+            PHAST_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            g_scratch_ptr[dev] = static_cast<double *>(ptr);
+        }
+        base = g_scratch_ptr[dev];
+    }
+    const int64_t N = kNoAllocNodes;
+    double *parts = base + 8 * N;
+    uint8_t *active = reinterpret_cast<uint8_t *>(parts + kParts * kNoAllocTiles);
+    Q4CG *cg = reinterpret_cast<Q4CG *>(parts + kParts * kNoAllocTiles + N / 8);
+    const Q4SolveScratch W{base, base + N, base + 2 * N, base + 3 * N, base + 4 * N, base + 5 * N, base + 6 * N,
+                           base + 7 * N, parts, active, cg, g_scratch_host_cg[dev]};
+    return damage_solve_impl(pl, W, method, H, Gc_cell, d_prev, d_init, fixed, fixed_vals, tol, max_iter, d_out,
+                             active_out, iters, status, st);
+}
+
+static int damage_solve_impl(phast_q4_plan *pl, const Q4SolveScratch &W, int32_t method, const double *H,
+                             const double *Gc_cell, const double *d_prev, const double *d_init, const uint8_t *fixed,
+                             const double *fixed_vals, double tol, int32_t max_iter, double *d_out,
+                             uint8_t *active_out, int32_t *iters, int32_t *status, hipStream_t st) {
     // per-instance initial state: tolerance and iteration cap
     for (int k = 0; k < pl->n_inst; ++k) {
-        Q4CG &S = pl->h_cg[k];
+        Q4CG &S = W.h_cg[k];
         std::memset(&S, 0, sizeof(S));
         S.tol = tol;
         S.max_iter = max_iter;
     }
-    PHAST_HIP(hipMemcpyAsync(pl->cg, pl->h_cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyHostToDevice, st));
+    PHAST_HIP(hipMemcpyAsync(W.cg, W.h_cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyHostToDevice, st));
 
     DmgArgs D = dmg_base(pl);
     D.H = H;
     D.Gc = Gc_cell;
     D.d = d_out;
-    D.r = pl->r;
-    D.z = pl->z;
-    D.p = pl->p;
-    D.Ap = pl->Ap;
-    D.b = pl->b;
-    D.minv = pl->minv;
-    D.lb = pl->lb;
+    D.r = W.r;
+    D.z = W.z;
+    D.p = W.p;
+    D.Ap = W.Ap;
+    D.b = W.b;
+    D.minv = W.minv;
+    D.lb = W.lb;
     D.fixed = fixed;
-    D.active = pl->active;
-    D.cg = pl->cg;
-    D.part = pl->part;
+    D.active = W.active;
+    D.cg = W.cg;
+    D.part = W.part;
 
     VecArgs V;
     std::memset(&V, 0, sizeof(V));
     V.inst = pl->d_inst;
     V.block_inst = pl->d_block_inst;
     V.d = d_out;
-    V.r = pl->r;
-    V.z = pl->z;
-    V.p = pl->p;
-    V.lb = pl->lb;
-    V.Ap = pl->Ap;
-    V.minv = pl->minv;
+    V.r = W.r;
+    V.z = W.z;
+    V.p = W.p;
+    V.lb = W.lb;
+    V.Ap = W.Ap;
+    V.minv = W.minv;
     V.init = d_init;
     V.dprev = d_prev;
     V.vals = fixed_vals;
     V.fixed = fixed;
-    V.active = pl->active;
+    V.active = W.active;
     V.active_out = active_out;
     V.d_out = d_out;
-    V.cg = pl->cg;
-    V.part = pl->part;
+    V.cg = W.cg;
+    V.part = W.part;
     V.n_tiles = pl->n_tiles;
 
     // b, diag -> M_inv; d = init, lb = d_prev (with pf_dirichlet pins)
@@ -1697,11 +1776,11 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
         DmgArgs R = dmg_base(pl);
         R.H = H;
         R.Gc = Gc_cell;
-        R.out = pl->b;
-        R.out2 = pl->minv;
+        R.out = W.b;
+        R.out2 = W.minv;
         PHAST_TRY(launch_dmg<OP_RHSDIAG>(pl, R, st));
         hipLaunchKernelGGL(minv_kernel, dim3((unsigned)ceil_div(pl->n_nodes, 256)), dim3(256), 0, st,
-                           pl->n_nodes, pl->minv);
+                           pl->n_nodes, W.minv);
         PHAST_LAUNCHED();
     }
     PHAST_TRY(launch_vec<V_SETUP>(pl, V, st));
@@ -1710,12 +1789,12 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
     if (method == 0) {
         D.x = d_out;
         PHAST_TRY(launch_dmg<OP_CG_INIT>(pl, D, st));
-        PHAST_TRY(launch_scalar<S_INIT>(pl, 10, st));
+        PHAST_TRY(launch_scalar<S_INIT>(pl, W.cg, W.part, 10, st));
         DmgArgs Dr = D;
         Dr.x = d_out;
         // the direction update of iteration k runs inside the A p pass of iteration k+1
         // (OP_CG_APF), alternating between the two direction buffers
-        double *pc = pl->p, *pn = pl->p2;
+        double *pc = W.p, *pn = W.p2;
         while (it < max_iter) {
             D.x = pc;
             if (it == 0) {
@@ -1725,19 +1804,19 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
                 PHAST_TRY(launch_dmg<OP_CG_APF>(pl, D, st));
                 std::swap(pc, pn);
             }
-            PHAST_TRY(launch_scalar<S_A>(pl, 10, st));
+            PHAST_TRY(launch_scalar<S_A>(pl, W.cg, W.part, 10, st));
             V.p = pc;
             PHAST_TRY(launch_vec<V_UPDATE_D_PRE>(pl, V, st));
             PHAST_TRY(launch_dmg<OP_CG_RESID>(pl, Dr, st));
-            PHAST_TRY(launch_scalar<S_B>(pl, 10, st));
+            PHAST_TRY(launch_scalar<S_B>(pl, W.cg, W.part, 10, st));
             PHAST_TRY(launch_vec<V_PRECOND_CHK>(pl, V, st));
-            PHAST_TRY(launch_scalar<S_C>(pl, 10, st));
+            PHAST_TRY(launch_scalar<S_C>(pl, W.cg, W.part, 10, st));
             ++it;
             if (it % poll == 0 || it == max_iter) {
-                PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
+                PHAST_HIP(hipMemcpyAsync(W.h_cg, W.cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
                 PHAST_HIP(hipStreamSynchronize(st));
                 bool all = true;
-                for (int k = 0; k < pl->n_inst; ++k) all = all && pl->h_cg[k].done;
+                for (int k = 0; k < pl->n_inst; ++k) all = all && W.h_cg[k].done;
                 if (all) break;
             }
         }
@@ -1745,10 +1824,10 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
     } else {
         D.x = d_out;
         PHAST_TRY(launch_dmg<OP_PC_INIT>(pl, D, st));
-        PHAST_TRY(launch_scalar<S_PC_INIT>(pl, 50, st));
+        PHAST_TRY(launch_scalar<S_PC_INIT>(pl, W.cg, W.part, 50, st));
         DmgArgs Dr = D;
         Dr.x = d_out;
-        double *pc = pl->p, *pn = pl->p2;
+        double *pc = W.p, *pn = W.p2;
         while (it < max_iter) {
             D.x = pc;
             if (it == 0) {
@@ -1758,27 +1837,27 @@ extern "C" int phast_q4_damage_solve(phast_q4_plan *pl, int32_t method, const do
                 PHAST_TRY(launch_dmg<OP_PC_APF>(pl, D, st));
                 std::swap(pc, pn);
             }
-            PHAST_TRY(launch_scalar<S_PC_A>(pl, 50, st));
+            PHAST_TRY(launch_scalar<S_PC_A>(pl, W.cg, W.part, 50, st));
             V.p = pc;
             PHAST_TRY(launch_vec<V_PC_UPDATE_PRE>(pl, V, st));
             PHAST_TRY(launch_dmg<OP_PC_RESID>(pl, Dr, st));
             PHAST_TRY(launch_vec<V_PC_PRECOND_CHK>(pl, V, st));
-            PHAST_TRY(launch_scalar<S_PC_C>(pl, 50, st));
+            PHAST_TRY(launch_scalar<S_PC_C>(pl, W.cg, W.part, 50, st));
             ++it;
             if (it % poll == 0 || it == max_iter) {
-                PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
+                PHAST_HIP(hipMemcpyAsync(W.h_cg, W.cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
                 PHAST_HIP(hipStreamSynchronize(st));
                 bool all = true;
-                for (int k = 0; k < pl->n_inst; ++k) all = all && pl->h_cg[k].done;
+                for (int k = 0; k < pl->n_inst; ++k) all = all && W.h_cg[k].done;
                 if (all) break;
             }
         }
         PHAST_TRY(launch_vec<V_PC_FINISH>(pl, V, st));
     }
-    PHAST_HIP(hipMemcpyAsync(pl->h_cg, pl->cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
+    PHAST_HIP(hipMemcpyAsync(W.h_cg, W.cg, sizeof(Q4CG) * pl->n_inst, hipMemcpyDeviceToHost, st));
     PHAST_HIP(hipStreamSynchronize(st));
     for (int k = 0; k < pl->n_inst; ++k) {
-        const Q4CG &S = pl->h_cg[k];
+        const Q4CG &S = W.h_cg[k];
         int32_t its = S.iter, stt = S.status;
         if (!S.done) {  // loop ended by the cap without the flag (max_iter == 0 handled in init)
             stt = PHAST_CG_MAXITER;
