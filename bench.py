@@ -30,11 +30,10 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6         # MI355X FP64 (vector and matrix peaks coincide)
 HEX8_BYTES_PER_NODE = 57.0      # u 24 + d 8 + phase 1 + r 24
 HEX8_FLOPS_PER_VOXEL = 940.0    # sum-factorised apply
-# HBM bytes per 64^3 apply (the apply kernel plus its seam/finalize kernel) measured with
-# rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, copy-calibrated), per launch shape;
-# see profiles/hex8_apply_64cube_pore.md
-HEX8_64_TRAFFIC = {(32, 14, 3): ((12528.61 + 2147.25) * 2.0 + 8269.75 + 2112.06) * 1024.0,
-                   (16, 14, 6): ((15001.75 + 1091.25) * 2.0 + 7213.75 + 1056.06) * 1024.0}
+# HBM bytes per 128^3 apply (the apply kernel plus its seam/finalize kernel) measured with
+# rocprofv3 (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel
+# factors), per launch shape; see profiles/hex8_apply_128cube_particles.md
+HEX8_128_TRAFFIC = {(16, 14, 8): ((108249.46 + 6204.88) * 2.0 + 55376.02 + 6144.06) * 1024.0}
 Q4_APPLY_BYTES_PER_NODE = 40.0  # u 16 + d 8 + f 16
 Q4_STEP_BYTES_PER_NODE = 192.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, H r+w 64
 # one projected damage-CG iteration without a bound hit or residual check (per node): A p with
@@ -73,7 +72,13 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the voxel reaction-curve gradient run")
     ap.add_argument("--adjoint-size", type=int, default=96, help="voxels per edge of the gradient run")
     ap.add_argument("--adjoint-steps", type=int, default=30, help="load steps of the gradient run")
-    ap.add_argument("--adjoint-fd", type=int, default=2, help="single-voxel central differences in the gradient run")
+    ap.add_argument("--adjoint-fd", type=int, default=2, help="single-voxel central differences in the gradient run "
+                    "on the most sensitive voxels")
+    ap.add_argument("--adjoint-fd-random", type=int, default=0, help="also central differences on this many seeded "
+                    "random voxels")
+    ap.add_argument("--checkpoint-every", type=int, default=5, help="load-step checkpoint spacing of the gradient run")
+    ap.add_argument("--sequential-window", type=int, default=100, help="steps of the batched and sequential runs "
+                    "timed against each other")
     ap.add_argument("--batched-steps", type=int, default=2000, help="steps of the 64-specimen batched run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 5 device copies of 256 MiB (the reference kernel for HBM counter scaling)")
@@ -126,10 +131,11 @@ def main():
     rng = np.random.default_rng(0)
 
     # ------------------------------------------------------------------ headline
-    n = 64
-    op, mats, phase = pore_cube(n, 12, dev)
+    # 128^3 particle composite: the largest single-GPU voxel workload
+    n = 128
+    op, mats, phase_info = particle_composite_op(n, dev)
     x = torch.as_tensor(rng.uniform(-1e-6, 1e-6, op.n_mech), dtype=torch.float64, device=dev)
-    x[-6:] = torch.tensor([5e-4, -1e-4, -1e-4, 0, 0, 0], dtype=torch.float64)
+    x[-6:] = torch.tensor([2.5e-4, -7.5e-5, -7.5e-5, 0, 0, 0], dtype=torch.float64)
     d = torch.as_tensor(rng.uniform(0, 1, op.n_red), dtype=torch.float64, device=dev)
     if a.tiles:
         op.set_tiles(*(int(v) for v in a.tiles.split(",")))
@@ -151,31 +157,10 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # ------------------------------------------------------------------ CPU baseline (oracle port)
+    # ------------------------------------------------------------------ CPU baseline
     cpu = None
     if not a.no_cpu:
-        from oracle import hex8 as OH
-        try:
-            from threadpoolctl import threadpool_info
-            threads = max([p.get("num_threads", 1) for p in threadpool_info()] + [1])
-        except Exception:
-            threads = 1
-        oop = OH.VoxelOperator((n, n, n), (1e-3,) * 3, mats.stiffness_bank, mats.stiffness_index, mats.Gc, mats.lam,
-                               mats.mu, mats.driver.astype(np.int64), kappa=1e-6, ell=2e-3)
-        xc, dc = x.cpu().numpy(), d.cpu().numpy()
-        reps, times = 0, []
-        t_end = time.perf_counter() + 15.0
-        while (time.perf_counter() < t_end and reps < 25) or reps < 3:
-            t0 = time.perf_counter()
-            yc = oop.mech_action(xc, oop.degradation(dc))
-            times.append(time.perf_counter() - t0)
-            reps += 1
-        t_cpu = float(np.median(times[1:] if len(times) > 1 else times))
-        err = float(np.abs(y.copy_(op.mech_action_d(x, d)).cpu().numpy() - yc).max() / np.abs(yc).max())
-        cpu = {"value": op.n_mech / t_cpu / 1e9, "unit": "GDOF/s", "cores": int(threads), "kind": "port",
-               "sample": f"{reps} numpy applies (degradation + B-matrix stress pass + scatter-add) of the same 64^3 "
-                         f"state, median of the last {max(reps - 1, 1)}; max rel. diff to GPU {err:.1e}",
-               "cpu_model": _cpu_model()}
+        cpu = cpu_baseline(op, mats, n, x, d, y)
 
     # ------------------------------------------------------------------ secondary measurements
     sec = {}
@@ -184,17 +169,18 @@ def main():
     if not a.no_batched:
         try:
             sec["c3_batched_64_specimens"] = batched_specimens(dev, a.batched_steps, a.batch_tile,
-                                                               not a.no_sequential)
+                                                               not a.no_sequential, a.sequential_window)
         except Exception as exc:  # reported, so that the headline line is still printed
             sec["c3_batched_64_specimens"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not a.no_adjoint:
         try:
-            sec["c4_adjoint_gradient"] = adjoint_gradient(dev, a.adjoint_size, a.adjoint_steps, a.adjoint_fd)
+            sec["c4_adjoint_gradient"] = adjoint_gradient(dev, a.adjoint_size, a.adjoint_steps, a.adjoint_fd,
+                                                          a.adjoint_fd_random, a.checkpoint_every)
         except Exception as exc:
             sec["c4_adjoint_gradient"] = {"error": f"{type(exc).__name__}: {exc}"}
 
     out = {
-        "metric": "element-apply GDOF/s (periodic hex8 voxel mech_action, 64^3 pore cube)",
+        "metric": "element-apply GDOF/s (periodic hex8 voxel mech_action, 128^3 particle composite)",
         "value": gdofs,
         "unit": "GDOF/s",
         "n_gpus": 1,
@@ -205,23 +191,71 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random displacement fluctuation, macro strain 5e-4, damage U(0,1))",
-        "config": {"workload": "hex8 periodic voxel element apply, 64^3 cube, centred pore r=12 voxels",
-                   "n_nodes": op.n_red, "n_dofs": op.n_mech,
+        "data": "synthetic (seeded particle placement, random displacement fluctuation, macro strain 2.5e-4, "
+                "damage U(0,1))",
+        "config": {"workload": "hex8 periodic voxel element apply, 128^3 particle composite: seeded non-overlapping "
+                               "spheres, radii U[6,14] voxels, 30% volume fraction, particles 70 GPa in a 3 GPa "
+                               "matrix, one-voxel interface shell with Gc x 0.1",
+                   "n_nodes": op.n_red, "n_dofs": op.n_mech, "particles": phase_info["particles"],
                    "tiles": list(op.tiles), "autotune": tune},
-        "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / FP64_PEAK_TFLOPS, "traffic": HEX8_64_TRAFFIC.get(tuple(op.tiles)),
+        "roofline": {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / FP64_PEAK_TFLOPS, "traffic": HEX8_128_TRAFFIC.get(tuple(op.tiles)),
                      "flops_per_voxel": HEX8_FLOPS_PER_VOXEL, "hbm_algorithmic_GB_s": hbm_gbs,
                      "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
-                     "note": "FP64-bound (16.5 FLOP/B > ridge 9.8): peak is the FP64 figure, computed on the "
-                             "vector FP64 units; one launch = apply kernel + seam/finalize kernel, timed "
-                             "together; traffic (bytes/launch) from rocprofv3 counters for the profiled "
-                             "launch shape, null for another"},
+                     "note": "FP64 vector arithmetic, no MFMA: 940 FLOP per voxel (sum-factorised) against "
+                             "57 B per node is 16.5 FLOP/B, above the FP64 ridge (78.6 TFLOP/s / 8 TB/s = "
+                             "9.8), so the FP64 vector peak is the roofline; one launch = apply kernel + "
+                             "seam/finalize kernel, timed together with HIP events on the launch stream; "
+                             "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE for the "
+                             "profiled launch shape (null for another)"},
         "cpu_baseline": cpu,
         "secondary": sec,
         "wall_s_timed": wall,
     }
     print(json.dumps(out), flush=True)
+
+
+def particle_composite_op(n, device):
+    from phast_amd import solid3d as S
+    from phast_amd import workloads as W
+    phase, phases, h, centres, radii = W.particle_composite(n)
+    mats = S.build_element_arrays(phase, phases)
+    load = S.LoadCase("xx", 0, tuple(2.5e-4 * k for k in range(1, 61)), (1, 2, 3, 4, 5), ())
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid((n, n, n), h), mats, load, kappa=1e-6, ell=2e-3,
+                                           device=device)
+    return op, mats, {"particles": int(len(radii))}
+
+
+def cpu_baseline(op, mats, n, x, d, y):
+    """The torch-CPU restatement of the reference's batched apply (gather, (B,24)x(24,48)
+    product, stiffness product, transposed product, index_add_) of the same state on the
+    host cores this process may use, median of repeated applies over a bounded sample."""
+    import numpy as np
+    import torch
+    from oracle import hex8 as OH
+    from oracle import torch_cpu as TC
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    torch.set_num_threads(threads)
+    oop = OH.VoxelOperator((n, n, n), op.grid.h, mats.stiffness_bank, mats.stiffness_index, mats.Gc, mats.lam,
+                           mats.mu, mats.driver.astype(np.int64), kappa=1e-6, ell=2e-3)
+    ref = TC.TorchCpuVoxelApply(oop)
+    xc, dc = x.cpu(), d.cpu()
+    del oop
+    times = []
+    t_end = time.perf_counter() + 20.0
+    for k in range(40):
+        t0 = time.perf_counter()
+        yc = ref.apply(xc, dc)
+        times.append(time.perf_counter() - t0)
+        if k >= 5 and time.perf_counter() > t_end:
+            break
+    t_cpu = float(np.median(times[3:] if len(times) > 3 else times))  # 3 warm-ups
+    err = float((y.copy_(op.mech_action_d(x, d)).cpu() - yc).abs().max() / yc.abs().max())
+    return {"value": op.n_mech / t_cpu / 1e9, "unit": "GDOF/s", "cores": int(torch.get_num_threads()),
+            "kind": "port", "sample": f"{len(times)} torch-CPU applies of the same 128^3 state (3 warm-ups, median "
+                                      f"of {max(len(times) - 3, 1)}; batches of 65536 voxels: gather, matmul, bmm, "
+                                      f"index_add_); max rel. diff to the GPU apply {err:.1e}",
+            "ms_per_apply": t_cpu * 1e3, "cpu_model": _cpu_model()}
 
 
 def _cpu_model():
@@ -279,7 +313,7 @@ def secondary(rng, dev, a):
         del fem, dyn, u, d, v, acc, fext, Hq
     if "t3" in parts:
         res["t3_split_quad_4096x2048"] = t3_secondary(rng, dev, a)
-    for nv in ((128, 256) if "hex8" in parts else ()):
+    for nv in ((64, 256) if "hex8" in parts else ()):
         op, mats, _ = pore_cube(nv, 12 * nv // 64, dev)
         x = T(rng.uniform(-1e-6, 1e-6, op.n_mech))
         d = T(rng.uniform(0, 1, op.n_red))
@@ -288,10 +322,13 @@ def secondary(rng, dev, a):
             op.set_tiles(*best)
         else:
             best, _ = autotune.tune_hex8(op, x, d, reps=3)
-        t, _ = events_time(lambda: op.mech_action_d(x, d), 10, 2)
-        res[f"hex8_{nv}^3"] = {"apply_GDOF_s": op.n_mech / t / 1e9,
-                               "apply_GB_s": HEX8_BYTES_PER_NODE * op.n_red / t / 1e9,
-                               "apply_ms": t * 1e3, "tiles": list(best)}
+        t, _ = events_time(lambda: op.mech_action_d(x, d), 20 if nv <= 64 else 10, 3)
+        res[f"hex8_{nv}^3_pore_cube"] = {"apply_GDOF_s": op.n_mech / t / 1e9,
+                                         "apply_GB_s": HEX8_BYTES_PER_NODE * op.n_red / t / 1e9,
+                                         "apply_TFLOP_s": HEX8_FLOPS_PER_VOXEL * op.n_elements / t / 1e12,
+                                         "apply_frac_fp64_peak": HEX8_FLOPS_PER_VOXEL * op.n_elements / t / 1e12
+                                         / FP64_PEAK_TFLOPS,
+                                         "apply_ms": t * 1e3, "tiles": list(best)}
         del op, x, d
         torch.cuda.empty_cache()
     return res
@@ -386,51 +423,20 @@ def t3_secondary(rng, dev, a):
     return entry
 
 
-def lognormal_field(rng, ny, nx, mean, cov, corr):
-    """Lognormal per-cell field of the given mean and coefficient of variation whose log is a
-    Gaussian random field with a Gaussian correlation of ``corr`` cells (white noise filtered
-    in Fourier space)."""
-    import numpy as np
-    z = rng.standard_normal((ny, nx))
-    ky = np.fft.fftfreq(ny)[:, None]
-    kx = np.fft.fftfreq(nx)[None, :]
-    z = np.real(np.fft.ifft2(np.fft.fft2(z) * np.exp(-2.0 * np.pi ** 2 * corr ** 2 * (kx ** 2 + ky ** 2))))
-    z = (z - z.mean()) / max(z.std(), 1e-300)
-    s2 = np.log1p(cov ** 2)
-    return mean * np.exp(np.sqrt(s2) * z - 0.5 * s2)
-
-
 def c3_batch_inputs(n_spec=64, seed=2024):
-    """64 SENT-like plates: nx ~ U{64..512}, ny = nx/2, 0.5 mm cells, lognormal Gc (mean 3e-3,
-    CoV 0.2, correlation 4 cells), a notch of length U[0.2, 0.5] W held at d = 1 along the
-    mid-height node row, traction +-1 MPa on the top/bottom edges."""
-    import numpy as np
+    """The 64 notched plates of phast_amd.workloads.specimen_batch as batch inputs."""
     from phast_amd import q4 as P
-    rng = np.random.default_rng(seed)
-    mat = P.Material(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5)
-    items, gcs, fext, fixed = [], [], [], []
-    for nx in rng.integers(64, 513, n_spec):
-        nx = int(nx)
-        ny = nx // 2
-        h = 0.5
-        mesh = P.FEMMesh.rectangle(h * nx, h * ny, nx, ny)
-        items.append((mesh, mat))
-        gcs.append(lognormal_field(rng, ny, nx, 3e-3, 0.2, 4.0).ravel())
-        f = np.zeros((mesh.n_nodes, 2))
-        edge = np.full(nx + 1, h)
-        edge[[0, -1]] = 0.5 * h
-        f[ny * (nx + 1):, 1] = 1.0 * edge
-        f[:nx + 1, 1] = -1.0 * edge
-        fext.append(f)
-        a_len = rng.uniform(0.2, 0.5) * h * nx
-        ids = (ny // 2) * (nx + 1) + np.arange(int(np.floor(a_len / h)) + 1)
-        fixed.append((ids, 1.0))
-    return items, gcs, fext, fixed
+    from phast_amd import workloads as W
+    mat = P.Material(**W.Q4_MATERIAL)
+    specs = W.specimen_batch(n_spec, seed)
+    items = [(P.FEMMesh.rectangle(sp.h * sp.nx, sp.h * sp.ny, sp.nx, sp.ny), mat) for sp in specs]
+    return items, [sp.Gc_field for sp in specs], [sp.f_ext for sp in specs], [(sp.notch, 1.0) for sp in specs]
 
 
-def batched_specimens(dev, steps, fixed_tile="", sequential=True):
-    """c3: specimen-steps/s of the persistent batched step replayed from a captured graph,
-    against the per-kernel batched path and 64 sequential single-specimen runs."""
+def batched_specimens(dev, steps, fixed_tile="", sequential=True, window=100):
+    """Specimen-steps/s of the 64-plate batch: the persistent batched step replayed from a
+    captured graph over the whole run; the per-kernel batched path; and, over the same first
+    ``window`` steps, the batched run against 64 sequential single-specimen runs."""
     import numpy as np
     import torch
     from phast_amd import batch as B
@@ -501,49 +507,60 @@ def batched_specimens(dev, steps, fixed_tile="", sequential=True):
     del bat
     if not sequential:
         return res
-    # 64 sequential single-specimen runs of the per-kernel path, and parity with the batch
-    seq_steps = 25
-    ref = mk(items, gcs, fext, fixed)
-    ref.step_persistent(seq_steps)
-    ref.check_persistent()
+    # the same window of steps, batched (persistent kernel, graph) and as 64 sequential runs of
+    # the per-kernel single-specimen path; agreement at the window's last step
+    window = max((window // chunk) * chunk, chunk)
+    bat = mk(items, gcs, fext, fixed, best)
+    bat.prepare_persistent()
+    g = bat.capture(chunk)
+    g.replay()  # warm the graph, then restart from the initial state
+    del g, bat
+    bat = mk(items, gcs, fext, fixed, best)
+    bat.prepare_persistent()
+    g = bat.capture(chunk)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.replay(window // chunk)
+    torch.cuda.synchronize()
+    wall_bat = time.perf_counter() - t0
+    bat.check_persistent(g.status[-1])
     singles = [mk([items[k]], [gcs[k]], [fext[k]], [fixed[k]]) for k in range(n_spec)]
     singles[0].step(1)  # warm the kernels, then rebuild the first specimen
     singles[0] = mk([items[0]], [gcs[0]], [fext[0]], [fixed[0]])
-    log("sequential single-specimen runs")
+    log(f"sequential single-specimen runs over {window} steps")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for one in singles:
-        one.step(seq_steps)
+        one.step(window)
     torch.cuda.synchronize()
     wall_seq = time.perf_counter() - t0
     worst = 0.0
     for k, one in enumerate(singles):
-        for x, y in zip(ref.instance_state(k), one.instance_state(0)):
-            worst = max(worst, float((x - y).abs().max().item()) / max(float(y.abs().max().item()), 1e-300))
-    res.update({"sequential_steps_per_specimen": seq_steps,
-                "sequential_specimen_steps_s": n_spec * seq_steps / wall_seq,
-                "max_rel_diff_batched_vs_sequential": worst})
-    res["speedup_persistent_graph_vs_sequential"] = (res["persistent_graph_specimen_steps_s"]
-                                                     / res["sequential_specimen_steps_s"])
-    del singles, ref
+        for xb, ys in zip(bat.instance_state(k), one.instance_state(0)):
+            worst = max(worst, float((xb - ys).abs().max().item()) / max(float(ys.abs().max().item()), 1e-300))
+    res.update({"window_steps": window,
+                "window_batched_specimen_steps_s": n_spec * window / wall_bat,
+                "window_sequential_specimen_steps_s": n_spec * window / wall_seq,
+                "speedup_batched_vs_sequential_same_window": wall_seq / wall_bat,
+                "max_rel_diff_batched_vs_sequential_at_window_end": worst})
+    del singles, bat, g
     torch.cuda.empty_cache()
     return res
 
 
-def adjoint_gradient(dev, n, steps, n_fd):
-    """c4: gradient of L = sum_k (R_k(Gc) - R_k(Gc_target))^2 over a load path of the n^3 cell
-    with 20 seeded periodic pores (radii U[4, 9] voxels), Gc = 5e-4 (1 + 0.1 N(0,1)) per voxel;
-    reverse-sweep wall against forward wall, checked by central differences on the ``n_fd``
-    voxels of largest sensitivity."""
+def adjoint_gradient(dev, n, steps, n_fd, n_fd_random=0, checkpoint_every=5):
+    """Gradient of L = sum_k (R_k(Gc) - R_k(Gc_target))^2 over a load path of the n^3 cell
+    with 20 seeded periodic pores (radii U[4, 9] voxels), Gc = 5e-4 (1 + 0.1 N(0,1)) per voxel
+    (phast_amd.workloads.porous_gc_cube): reverse-sweep wall (load-step checkpoints every
+    ``checkpoint_every`` steps, the rest recomputed) against forward wall; central
+    differences (step 1e-6 N/mm) on the ``n_fd`` most sensitive voxels and on ``n_fd_random``
+    seeded random voxels."""
     import numpy as np
     import torch
     from phast_amd import adjoint3d as A
-    from phast_amd import geometry as G
     from phast_amd import solid3d as S
-    h = (1e-3,) * 3
-    phase = G.random_pores((n, n, n), h, 20, (4.0, 9.0), seed=96)
-    phases = {"mi_matrix": S.PhaseMaterial("mi_matrix", 3000.0, 0.35, 5e-4),
-              "pore": S.PhaseMaterial("pore", 3000.0 * 1e-6, 0.35, 5e-4)}
+    from phast_amd import workloads as W
+    phase, phases, h, Gc_np, Gc_t_np = W.porous_gc_cube(n, 20, (4.0, 9.0), seed=96, gc_seed=4)
     mats = S.build_element_arrays(phase, phases)
     targets = tuple(1e-4 * k for k in range(1, steps + 1))  # up to 3e-3: before the peak
     load = S.LoadCase("xx", 0, targets, (1, 2, 3, 4, 5), ())
@@ -551,12 +568,10 @@ def adjoint_gradient(dev, n, steps, n_fd):
     mg = S.VoxelMultigrid(op)
     t_start = time.perf_counter()
     log = lambda msg: print(f"[adjoint {time.perf_counter() - t_start:7.1f}s] {msg}", file=sys.stderr, flush=True)
-    curve = A.ReactionCurve(op, preconditioner=mg, progress=log)
-    rng = np.random.default_rng(4)
+    curve = A.ReactionCurve(op, preconditioner=mg, progress=log, checkpoint_every=checkpoint_every)
     E = op.n_elements
-    Gc = torch.as_tensor(np.maximum(5e-4 * (1 + 0.1 * rng.standard_normal(E)), 1e-5), dtype=torch.float64,
-                         device=dev)
-    Gc_t = Gc * torch.as_tensor(1 + 0.05 * rng.standard_normal(E), dtype=torch.float64, device=dev)
+    Gc = torch.as_tensor(Gc_np, dtype=torch.float64, device=dev)
+    Gc_t = torch.as_tensor(Gc_t_np, dtype=torch.float64, device=dev)
     R_hat, _ = curve.forward(Gc_t)
     R_hat = R_hat.detach().clone()
     torch.cuda.synchronize()
@@ -569,30 +584,43 @@ def adjoint_gradient(dev, n, steps, n_fd):
     grad = curve.backward(tape, 2.0 * (R - R_hat))
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
-    d_max = float(tape["records"][steps - 1].d.max())
+    stats = curve.stats
     del tape
 
     def loss(g):
         Rg, _ = curve.forward(g)
         return float(((Rg - R_hat) ** 2).sum())
 
-    # central differences with a Gc step of 1e-6 N/mm on the voxels of largest sensitivity
-    errs = []
-    for e in torch.argsort(grad.abs(), descending=True)[:n_fd].tolist():
+    def fd(e):
         step = 1e-6
         gp, gm = Gc.clone(), Gc.clone()
         gp[e] += step
         gm[e] -= step
-        f1 = (loss(gp) - loss(gm)) / (2 * step)
+        return (loss(gp) - loss(gm)) / (2 * step)
+
+    gmax = float(grad.abs().max())
+    top = torch.argsort(grad.abs(), descending=True)[:n_fd].tolist()
+    errs = []
+    for e in top:
+        f1 = fd(e)
         errs.append(abs(float(grad[e]) - f1) / max(abs(f1), 1e-300))
-    return {"cells": n, "voxels": E, "load_steps": steps, "pores": 20, "forward_wall_s": t_fwd,
-            "gradient_wall_s": t_bwd, "gradient_over_forward": t_bwd / t_fwd, "loss": L,
-            "d_max_final": d_max, "voxel_fd_rel_errs": errs, "fd_step": 1e-6,
-            "voxel_fd_rel_err_max": max(errs) if errs else None, "fd_voxels": n_fd,
-            "adjoint_block_iterations": curve.stats.block_iterations[::-1],
-            "adjoint_mech_pcg_iterations": curve.stats.mech_pcg_iterations,
-            "adjoint_damage_pcg_iterations": curve.stats.damage_pcg_iterations,
-            "preconditioner": "voxel multigrid V-cycle"}
+    rnd = np.random.default_rng(32).choice(E, n_fd_random, replace=False).tolist() if n_fd_random else []
+    rnd_errs = []
+    for e in rnd:  # |adjoint - FD| / max(|FD|, 1e-2 max|grad|)
+        f1 = fd(e)
+        rnd_errs.append(abs(float(grad[e]) - f1) / max(abs(f1), 1e-2 * gmax))
+    out = {"cells": n, "voxels": E, "load_steps": steps, "pores": 20, "checkpoint_every": checkpoint_every,
+           "forward_wall_s": t_fwd, "gradient_wall_s": t_bwd, "gradient_over_forward": t_bwd / t_fwd, "loss": L,
+           "fd_step": 1e-6, "fd_sensitive_voxels": n_fd, "voxel_fd_rel_errs": errs,
+           "voxel_fd_rel_err_max": max(errs) if errs else None,
+           "adjoint_block_iterations": stats.block_iterations[::-1],
+           "adjoint_mech_pcg_iterations": stats.mech_pcg_iterations,
+           "adjoint_damage_pcg_iterations": stats.damage_pcg_iterations,
+           "preconditioner": "voxel multigrid V-cycle"}
+    if rnd:
+        out.update({"fd_random_voxels": len(rnd), "fd_random_err_max": max(rnd_errs),
+                    "fd_random_criterion": "|adjoint - FD| / max(|FD|, 1e-2 max|grad|)"})
+    return out
 
 
 if __name__ == "__main__":
