@@ -62,7 +62,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary measurements")
-    ap.add_argument("--secondary", type=str, default="q4,hex8,t3", help="secondary parts to run: q4, hex8, t3")
+    ap.add_argument("--secondary", type=str, default="q4,hex8,t3,sent", help="secondary parts to run: q4, hex8, t3, "
+                    "sent (the 200 x 80 SENT plate run end to end)")
     ap.add_argument("--q4-tile", type=str, default="", help="fixed Q4 strip threads,rows (skips the autotuner)")
     ap.add_argument("--hex8-big-tiles", type=str, default="", help="fixed tx,ty,kz of the 128^3/256^3 runs")
     ap.add_argument("--batch-tile", type=str, default="", help="fixed strip of the batched run (skips its autotuner)")
@@ -313,6 +314,8 @@ def secondary(rng, dev, a):
         del fem, dyn, u, d, v, acc, fext, Hq
     if "t3" in parts:
         res["t3_split_quad_4096x2048"] = t3_secondary(rng, dev, a)
+    if "sent" in parts:
+        res["c0_sent_plate_1000_steps"] = sent_plate_run(dev)
     for nv in ((64, 256) if "hex8" in parts else ()):
         op, mats, _ = pore_cube(nv, 12 * nv // 64, dev)
         x = T(rng.uniform(-1e-6, 1e-6, op.n_mech))
@@ -332,6 +335,80 @@ def secondary(rng, dev, a):
         del op, x, d
         torch.cuda.empty_cache()
     return res
+
+
+def sent_plate_run(dev, steps=1000, chunk=20):
+    """The 200 x 80 SENT plate (preseeded notch, +-1 MPa traction, dt = 0.05 us, projected
+    damage CG at tol 1e-5 every step) for 1000 steps, two ways from the same state: the
+    per-kernel loop (StaggeredSolver.step_full: a fused explicit-step launch, then the damage
+    CG's passes with a host read of its flags every 4 iterations) and the persistent kernel
+    with one instance (one launch per step), replayed from a captured graph of ``chunk``
+    steps.  Steps/s, mean damage CG iterations, and the largest relative difference between
+    the two end states (the two organisations compute the same bits)."""
+    import numpy as np
+    import torch
+    from phast_amd import batch as B
+    from phast_amd import q4 as P
+    from phast_amd import workloads as W
+    p = W.sent_plate(steps=steps)
+    mesh = P.FEMMesh(p.nodes, p.elements)
+    mat = P.Material(**p.material)
+    fem = P.FEMOperators(mesh, mat)
+    T = lambda z: torch.as_tensor(np.ascontiguousarray(z), dtype=torch.float64, device=dev)
+    dmg = P.PhaseFieldDamageSolver(fem, tol=1e-5, max_iter=5000, bounds_method="projected_cg")
+    H0 = T(p.H0)
+    d0 = dmg.solve(H0, torch.zeros(mesh.n_nodes, dtype=torch.float64, device=dev))
+    fx = T(p.f_ext)
+
+    def per_kernel():
+        st = P.StaggeredSolver(fem, P.ExplicitDynamics(fem, dt=p.dt), dmg, f_ext=fx, damage_every=1)
+        st.d, st.H_elem = d0.clone(), H0.clone()
+        return st
+
+    st = per_kernel()
+    for _ in range(5):
+        st.step_full()
+    st = per_kernel()
+    its = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.step_full()
+        its += dmg.last_iter
+    torch.cuda.synchronize()
+    wall_pk = time.perf_counter() - t0
+    z = torch.zeros((mesh.n_nodes, 2), dtype=torch.float64, device=dev)
+
+    def persistent():
+        bat = B.BatchedExplicit([(mesh, mat)], dts=[p.dt], f_ext=[p.f_ext], tol=1e-5, max_iter=5000,
+                                bounds_method="projected_cg")
+        bat.load_states([(z, z, z, d0, H0)])
+        bat.prepare_persistent()
+        return bat, bat.capture(chunk)
+
+    bat, g = persistent()
+    g.replay()
+    del g, bat
+    bat, g = persistent()
+    its_p = torch.zeros((), dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // chunk):
+        g.replay()
+        its_p += g.iters.sum()
+    torch.cuda.synchronize()
+    wall_p = time.perf_counter() - t0
+    bat.check_persistent(g.status[-1])
+    n_p = (steps // chunk) * chunk
+    diff = 0.0
+    for a_, b_ in zip(bat.instance_state(0), (st.u, st.v, st.a, st.d, st.H_elem)):
+        diff = max(diff, float((a_ - b_).abs().max() / b_.abs().max().clamp_min(1e-300)))
+    return {"nodes": mesh.n_nodes, "steps": steps, "per_kernel_steps_s": steps / wall_pk,
+            "per_kernel_ms_per_step": wall_pk / steps * 1e3, "persistent_graph_steps_s": n_p / wall_p,
+            "persistent_graph_ms_per_step": wall_p / n_p * 1e3, "speedup_persistent_vs_per_kernel": (n_p / wall_p)
+            / (steps / wall_pk), "damage_cg_iters_mean": its / steps,
+            "damage_cg_iters_mean_persistent": float(its_p.item()) / n_p,
+            "max_rel_diff_persistent_vs_per_kernel": diff}
 
 
 def q4_cg_iteration(P, fem, rng, T):

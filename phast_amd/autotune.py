@@ -4,11 +4,12 @@ from __future__ import annotations
 
 import torch
 
-# (16,16) and (32,8): voxel-parallel kernel with z slabs of kz layers; (16,8) and (8,8):
+# (16,16), (32,8), (16,14), (32,14), (32,6) and (64,6): voxel-parallel kernel with z slabs of kz layers; (16,8) and (8,8):
 # node-column marching kernel walking kz node layers.
 HEX8_TILES = [(16, 16, 8), (16, 16, 16), (32, 8, 2), (32, 8, 3), (32, 8, 4), (32, 8, 8), (32, 8, 16), (16, 14, 2),
               (16, 14, 3), (16, 14, 4), (16, 14, 6), (16, 14, 8), (16, 14, 12), (16, 14, 16), (32, 14, 2), (32, 14, 3),
-              (32, 14, 4), (32, 14, 6), (32, 14, 8), (32, 14, 16), (16, 8, 8)]
+              (32, 14, 4), (32, 14, 6), (32, 14, 8), (32, 14, 16), (16, 8, 8), (32, 6, 4), (32, 6, 8), (32, 6, 16),
+              (64, 6, 4), (64, 6, 8), (64, 6, 16)]
 Q4_TILES = [(128, 8), (128, 32), (256, 8), (256, 16), (256, 32), (256, 64), (64, 32), (256, 128)]
 BATCH_TILES = [(64, 4), (64, 8), (128, 2), (128, 4), (128, 8), (256, 2), (256, 4), (256, 8), (256, 16)]
 

@@ -214,7 +214,8 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
     constexpr int NF = (GMODE == 2) ? 4 : 3;
     constexpr int NS = (NN + NT - 1) / NT;  // node slots loaded per thread
     static_assert(NV <= NT, "voxel tile larger than the block");
-    __shared__ double su[2][NF][NS * NT];  // node layers; slots past NN are padding
+    // node layers: NN slots and one dummy slot that lanes past the tile's nodes write to
+    __shared__ double su[2][NF][NN + 1];
     __shared__ double st[18][NV];  // corners (1,0), (0,1), (1,1) of both faces, 3 components each
     __shared__ double red[7 * (NT / 64)];
 
@@ -257,11 +258,14 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
             if (GMODE == 2) dst[m][NF - 1] = A.d[node];
         }
     };
+    int slot_lds[NS];  // LDS slot of each node slot (NN for the padding lanes)
+#pragma unroll
+    for (int m = 0; m < NS; ++m) slot_lds[m] = min(t + m * NT, NN);
     auto stage = [&](int buf, const double (&src)[NS][NF]) {
 #pragma unroll
         for (int m = 0; m < NS; ++m)
 #pragma unroll
-            for (int f = 0; f < NF; ++f) su[buf][f][t + m * NT] = src[m][f];
+            for (int f = 0; f < NF; ++f) su[buf][f][slot_lds[m]] = src[m][f];
     };
 
     double sE[6] = {0, 0, 0, 0, 0, 0}, en = 0.0;
@@ -848,6 +852,8 @@ static int hex8_apply_any(const phast_hex8_plan *pl, const double *x, const doub
     else if (pl->tx == 32 && pl->ty == 8) rc = launch_apply_t<32, 8>(pl, A, gmode, energy, st, jac);
     else if (pl->tx == 16 && pl->ty == 14) rc = launch_apply_t<16, 14>(pl, A, gmode, energy, st, jac);
     else if (pl->tx == 32 && pl->ty == 14) rc = launch_apply_t<32, 14>(pl, A, gmode, energy, st, jac);
+    else if (pl->tx == 32 && pl->ty == 6) rc = launch_apply_t<32, 6>(pl, A, gmode, energy, st, jac);
+    else if (pl->tx == 64 && pl->ty == 6) rc = launch_apply_t<64, 6>(pl, A, gmode, energy, st, jac);
     else if (pl->tx == 16 && pl->ty == 8) rc = launch_mech_t<16, 8>(pl, A, gmode, energy, st, &nb, jac);
     else if (pl->tx == 8 && pl->ty == 8) rc = launch_mech_t<8, 8>(pl, A, gmode, energy, st, &nb, jac);
     else {
@@ -1105,7 +1111,7 @@ extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double
             }
     }
     pl->iso = iso;
-    pl->max_blocks = (int32_t)(ceil_div(nx, 8) * ceil_div(ny, 8) * nz);
+    pl->max_blocks = (int32_t)(ceil_div(nx, 8) * ceil_div(ny, 6) * nz);  // the narrowest tile: 8 x 6
     const size_t nb = sizeof(double) * (size_t)pl->n;
     if (hipMalloc(&pl->bank, sizeof(double) * 36 * n_bank) != hipSuccess ||
         hipMalloc(&pl->bank_index, sizeof(int32_t) * pl->n) != hipSuccess || hipMalloc(&pl->Gc, nb) != hipSuccess ||
@@ -1164,7 +1170,8 @@ int phast::hex8_ensure_seam(phast_hex8_plan *pl, int32_t kz) {
 extern "C" int phast_hex8_set_tiles(phast_hex8_plan *pl, int32_t tx, int32_t ty, int32_t kz) {
     PHAST_REQUIRE(pl && kz >= 1, "phast_hex8_set_tiles: bad argument");
     PHAST_REQUIRE((tx == 16 && ty == 16) || (tx == 16 && ty == 8) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8) ||
-                      (tx == 16 && ty == 14) || (tx == 32 && ty == 14),
+                      (tx == 16 && ty == 14) || (tx == 32 && ty == 14) || (tx == 32 && ty == 6) ||
+                      (tx == 64 && ty == 6),
                   "unsupported hex8 tile %dx%d", tx, ty);
     if (int rc = hex8_ensure_seam(pl, kz)) return rc;
     pl->tx = tx;
@@ -1240,6 +1247,8 @@ extern "C" int phast_hex8_damage_action(const phast_hex8_plan *pl, const double 
     if (pl->tx == 32 && pl->ty == 8) return launch_damage_vp_t<32, 8>(pl, A, st);
     if (pl->tx == 16 && pl->ty == 14) return launch_damage_vp_t<16, 14>(pl, A, st);
     if (pl->tx == 32 && pl->ty == 14) return launch_damage_vp_t<32, 14>(pl, A, st);
+    if (pl->tx == 32 && pl->ty == 6) return launch_damage_vp_t<32, 6>(pl, A, st);
+    if (pl->tx == 64 && pl->ty == 6) return launch_damage_vp_t<64, 6>(pl, A, st);
     set_error("unsupported hex8 tile");
     return PHAST_ERR_UNSUPPORTED;
 }

@@ -351,7 +351,7 @@ extern "C" int phast_hex8_mg_create(const phast_hex8_plan *fine, int64_t pinned,
         // voxel-parallel apply with slab seams; thin slabs so that small levels still
         // spread over many blocks
         pl->tx = 16; pl->ty = 14; pl->kz = nz >= 32 ? 3 : (nz >= 16 ? 2 : 1);
-        pl->max_blocks = (int32_t)(ceil_div(nx, 8) * ceil_div(ny, 8) * nz);
+        pl->max_blocks = (int32_t)(ceil_div(nx, 8) * ceil_div(ny, 6) * nz);  // the narrowest tile: 8 x 6
         mg->lv[l] = pl;
         mg->levels = l + 1;
         const size_t nb = sizeof(double) * (size_t)pl->n;

@@ -135,6 +135,29 @@ def test_sent_plate_1000_steps(dev):
     assert np.all(np.isfinite(R)) and float(st.d.max()) == 1.0
 
 
+def test_sent_plate_persistent_graph_equals_the_per_kernel_loop(dev):
+    """The SENT run through the persistent kernel with one instance, replayed from a captured
+    graph of 20 steps, gives the per-kernel loop's bits for 200 steps."""
+    from phast_amd import batch as B
+    p, fem, ofem, dmg, odmg, st = _sent(dev, 1e-5)
+    n = p.nodes.shape[0]
+    d0 = dmg.solve(T(p.H0, dev), T(np.zeros(n), dev))
+    st.d, st.H_elem = d0.clone(), T(p.H0, dev)
+    for _ in range(200):
+        st.step_full()
+    from phast_amd import q4 as P
+    bat = B.BatchedExplicit([(fem.mesh, fem.material)], dts=[p.dt], f_ext=[p.f_ext], tol=1e-5, max_iter=5000,
+                            bounds_method="projected_cg")
+    z = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+    bat.load_states([(z, z, z, d0, T(p.H0, dev))])
+    g = bat.capture(20)
+    g.replay(10)
+    bat.check_persistent(g.status[-1])
+    for a, b in zip(bat.instance_state(0), (st.u, st.v, st.a, st.d, st.H_elem)):
+        assert torch.equal(a, b)
+    assert isinstance(fem.material, P.Material)
+
+
 # ----------------------------------------------------------------------------------------
 # 64^3 pore cube
 # ----------------------------------------------------------------------------------------

@@ -46,7 +46,7 @@ def build(shape, h=(1e-3, 1.1e-3, 0.9e-3), seed=0, pores=True, tiles=(32, 8, 8),
 
 @pytest.mark.parametrize("shape", [(6, 5, 7), (16, 16, 16), (17, 9, 10)])
 @pytest.mark.parametrize("tiles", [(16, 8, 8), (8, 8, 4), (16, 16, 3), (32, 8, 16), (16, 16, 1), (32, 8, 2),
-                                   (16, 16, 100), (16, 14, 5)])
+                                   (16, 16, 100), (16, 14, 5), (32, 14, 4), (32, 6, 3), (64, 6, 7)])
 def test_mech_action(dev, shape, tiles):
     S, op, oop = build(shape, tiles=tiles)
     rng = np.random.default_rng(1)
@@ -61,7 +61,8 @@ def test_mech_action(dev, shape, tiles):
     assert rel(op.mech_action(T(x, dev)), oop.mech_action(x)) < RTOL
 
 
-@pytest.mark.parametrize("tiles", [(16, 16, 2), (32, 8, 3), (16, 16, 7), (32, 8, 1), (16, 14, 6), (32, 14, 5)])
+@pytest.mark.parametrize("tiles", [(16, 16, 2), (32, 8, 3), (16, 16, 7), (32, 8, 1), (16, 14, 6), (32, 14, 5),
+                                   (32, 6, 4), (64, 6, 3)])
 def test_voxel_parallel_apply_matches_marching_apply(dev, tiles):
     """Same per-voxel arithmetic and node summation order as the marching kernel: the
     nodal rows agree bitwise; the macro rows and energy (block partials in another order)
@@ -86,7 +87,7 @@ def test_voxel_parallel_apply_matches_marching_apply(dev, tiles):
     assert rel(op.mech_action(x, g), oop.mech_action(xo, go)) < RTOL
 
 
-@pytest.mark.parametrize("tiles", [(16, 8, 8), (32, 14, 3), (16, 14, 6)])
+@pytest.mark.parametrize("tiles", [(16, 8, 8), (32, 14, 3), (16, 14, 6), (64, 6, 5)])
 def test_general_stiffness_bank(dev, tiles):
     """Bank entries that are not isotropic take the kernels' general 6x6 stiffness path
     (per-voxel bank index instead of lam/mu): apply, fused apply, stress sums, diagonal."""
@@ -144,8 +145,9 @@ def test_full_driver(dev):
     assert rel(op.driving_force(T(x, dev)), oop.driving_force(x)) < RTOL
 
 
-def test_damage_operator(dev):
-    S, op, oop = build((10, 6, 8))
+@pytest.mark.parametrize("tiles", [(32, 8, 8), (32, 6, 3), (64, 6, 2)])
+def test_damage_operator(dev, tiles):
+    S, op, oop = build((10, 6, 8), tiles=tiles)
     rng = np.random.default_rng(4)
     H = rng.uniform(0, 1e-3, (op.n_elements, 8))
     x = rng.uniform(0, 1, op.n_red)
