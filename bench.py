@@ -78,8 +78,8 @@ def parse():
     ap.add_argument("--adjoint-fd-random", type=int, default=0, help="also central differences on this many seeded "
                     "random voxels")
     ap.add_argument("--checkpoint-every", type=int, default=5, help="load-step checkpoint spacing of the gradient run")
-    ap.add_argument("--sequential-window", type=int, default=100, help="steps of the batched and sequential runs "
-                    "timed against each other")
+    ap.add_argument("--sequential-window", type=int, default=0, help="steps of the batched and sequential runs "
+                    "timed against each other (0: the whole batched run)")
     ap.add_argument("--batched-steps", type=int, default=2000, help="steps of the 64-specimen batched run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 5 device copies of 256 MiB (the reference kernel for HBM counter scaling)")
@@ -510,10 +510,12 @@ def c3_batch_inputs(n_spec=64, seed=2024):
     return items, [sp.Gc_field for sp in specs], [sp.f_ext for sp in specs], [(sp.notch, 1.0) for sp in specs]
 
 
-def batched_specimens(dev, steps, fixed_tile="", sequential=True, window=100):
+def batched_specimens(dev, steps, fixed_tile="", sequential=True, window=0):
     """Specimen-steps/s of the 64-plate batch: the persistent batched step replayed from a
-    captured graph over the whole run; the per-kernel batched path; and, over the same first
-    ``window`` steps, the batched run against 64 sequential single-specimen runs."""
+    captured graph over the whole run; the per-kernel batched path; and, over the same
+    steps (the whole run, or the first ``window`` steps when ``window`` is given), the
+    batched run against 64 sequential single-specimen runs, with their agreement at the
+    last of those steps."""
     import numpy as np
     import torch
     from phast_amd import batch as B
@@ -571,7 +573,7 @@ def batched_specimens(dev, steps, fixed_tile="", sequential=True, window=100):
                 "damage_cg_iters_max": int(its_max.item()),
                 "ms_per_step": wall / n_steps * 1e3,
                 "d_max_end": float(bat.d.max().item())})
-    del g, bat
+    full_run = (bat, g, wall, n_steps)
     # per-kernel batched path (one launch per kernel over all specimens, host polls the CG)
     short = max(min(100, steps // 10), 10)
     bat = mk(items, gcs, fext, fixed)
@@ -584,31 +586,37 @@ def batched_specimens(dev, steps, fixed_tile="", sequential=True, window=100):
     del bat
     if not sequential:
         return res
-    # the same window of steps, batched (persistent kernel, graph) and as 64 sequential runs of
-    # the per-kernel single-specimen path; agreement at the window's last step
-    window = max((window // chunk) * chunk, chunk)
-    bat = mk(items, gcs, fext, fixed, best)
-    bat.prepare_persistent()
-    g = bat.capture(chunk)
-    g.replay()  # warm the graph, then restart from the initial state
-    del g, bat
-    bat = mk(items, gcs, fext, fixed, best)
-    bat.prepare_persistent()
-    g = bat.capture(chunk)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    g.replay(window // chunk)
-    torch.cuda.synchronize()
-    wall_bat = time.perf_counter() - t0
-    bat.check_persistent(g.status[-1])
-    singles = [mk([items[k]], [gcs[k]], [fext[k]], [fixed[k]]) for k in range(n_spec)]
+    # the same steps, batched (persistent kernel, graph) and as 64 sequential runs of the
+    # per-kernel single-specimen path; agreement at the last of those steps
+    if not window or window >= n_steps:
+        bat, g, wall_bat, window = full_run
+    else:
+        del full_run
+        window = max((window // chunk) * chunk, chunk)
+        bat = mk(items, gcs, fext, fixed, best)
+        bat.prepare_persistent()
+        g = bat.capture(chunk)
+        g.replay()  # warm the graph, then restart from the initial state
+        del g, bat
+        bat = mk(items, gcs, fext, fixed, best)
+        bat.prepare_persistent()
+        g = bat.capture(chunk)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.replay(window // chunk)
+        torch.cuda.synchronize()
+        wall_bat = time.perf_counter() - t0
+        bat.check_persistent(g.status[-1])
+    singles = [mk([items[k]], [gcs[k]], [fext[k]], [fixed[k]], best) for k in range(n_spec)]
     singles[0].step(1)  # warm the kernels, then rebuild the first specimen
-    singles[0] = mk([items[0]], [gcs[0]], [fext[0]], [fixed[0]])
+    singles[0] = mk([items[0]], [gcs[0]], [fext[0]], [fixed[0]], best)
     log(f"sequential single-specimen runs over {window} steps")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for one in singles:
+    for k, one in enumerate(singles):
         one.step(window)
+        if k % 16 == 15:
+            log(f"{k + 1} sequential specimens done")
     torch.cuda.synchronize()
     wall_seq = time.perf_counter() - t0
     worst = 0.0
