@@ -454,6 +454,76 @@ int phast_hex8_adj_damage_history(const phast_hex8_plan *plan, const double *d, 
 int phast_hex8_adj_damage_gc(const phast_hex8_plan *plan, const double *d, const double *l, double scale,
                              double *out_e, hipStream_t stream);
 
+/* ---- PF-CZM damage law and crack closure of the voxel cell -----------------------------
+ * The default law of the reference's public solid3d API (solid3d/api.py:178-183).
+ *
+ * pfczm_setup: Wu's rational degradation g = (1-kappa) A/(A+Q) + kappa, A = (1-d)^p,
+ *   Q = a1 d (1 + a2 d + a2 a3 d^2), a1 per voxel (host (E,)), p >= 2
+ *   (physics/pfczm.py:17-130; operator.py:300-383 _build_pfczm).
+ * pfczm_degradation: g, g', g'' (E,8) at the Gauss points of nodal d (operator.py:939-946,
+ *   436-439 _pfczm_derivatives); any of the outputs may be null.
+ * pfczm_energy: out3 (device) = [sum Gc (2 m.d - d.M d)/(pi ell), sum Gc ell/pi d.L d,
+ *   sum w H g(d_q)]; the damage energy is their sum, the fracture energy the first two
+ *   (operator.py:461-488 pfczm_damage_energy, 611-631 _pfczm_fracture_energy); H may be null.
+ * pfczm_gradient: scale x the gradient of that energy (operator.py:490-512).
+ * pfczm_tangent_weights: W (E,8) = max(H g'' - 2 Gc/(pi ell), 0.25 * 2 Gc/(pi ell))
+ *   (operator.py:514-538); the floor everywhere when H is null.
+ * pfczm_tangent_action / _diagonal: scale (Gc (2 ell/pi) L + sum_q w W_q N_q N_q^T) and its
+ *   Jacobi diagonal (operator.py:540-564).
+ * pfczm_dirichlet_pcg / pfczm_box_update: phast_hex8_damage_dirichlet_pcg and
+ *   phast_hex8_box_update on that tangent, for the box-KKT subproblems of the projected Newton
+ *   solve (operator.py:566-609 pfczm_newton_system; solid3d/staggered.py:843-1096).
+ *
+ * closure_setup: crack closure 'axes' of the isotropic phases (operator.py:631-690): cl_on
+ *   (host uint8 (E,)) is 1 where a voxel adds (1 - g) sigma_c back (isotropic, with a damage
+ *   driver); the plan must be isotropic.  Patterns are uint32 (E,): bit 4 q + k is
+ *   [v_k . eps_q < 0] for Gauss point q and rows v = (trace, e_x, e_y, e_z) (splits.py:417-420).
+ * closure_pattern: newp = the pattern of state x (operator.py:792-806); with old (and g),
+ *   out2 (device) = [max k |v . eps| over every bit, max (1 - g) k |v . eps| over the bits that
+ *   differ from old, or -1 when none] -- the test of operator.py:808-841 is
+ *   out2[1] > tol * out2[0].
+ * mech_action_cl / stress_sums_cl / mech_diagonal_cl: phast_hex8_mech_action,
+ *   phast_hex8_stress_sums and phast_hex8_mech_diagonal with the closure (operator.py:1039-1105,
+ *   1196-1300); mode 1: each point's own pattern (the diagonal: every bit), 2: `pat`; g (E,8)
+ *   required.
+ * mech_solve_cl: phast_hex8_mech_solve with the closure at the fixed pattern `pat`
+ *   (solid3d/staggered.py:550-587 _linear_mechanics with a ClosurePattern). */
+int phast_hex8_pfczm_setup(phast_hex8_plan *plan, const double *a1, double a2, double a3, double p);
+int phast_hex8_pfczm_degradation(const phast_hex8_plan *plan, const double *d, double *g, double *gp, double *gpp,
+                                 hipStream_t stream);
+int phast_hex8_pfczm_energy(const phast_hex8_plan *plan, const double *d, const double *H, double *out3,
+                            hipStream_t stream);
+int phast_hex8_pfczm_gradient(const phast_hex8_plan *plan, const double *d, const double *H, double scale,
+                              double *out, hipStream_t stream);
+int phast_hex8_pfczm_tangent_weights(const phast_hex8_plan *plan, const double *d, const double *H, double *W,
+                                     hipStream_t stream);
+int phast_hex8_pfczm_tangent_action(const phast_hex8_plan *plan, const double *f, const double *W, double scale,
+                                    double *y, hipStream_t stream);
+int phast_hex8_pfczm_tangent_diagonal(const phast_hex8_plan *plan, const double *W, double scale, double *diag,
+                                      hipStream_t stream);
+int phast_hex8_pfczm_dirichlet_pcg(const phast_hex8_plan *plan, const double *W, double scale, const double *b,
+                                   const double *diag, const uint8_t *fixed, const double *values,
+                                   const double *initial, double tol, int32_t max_iter, int64_t n, double *out,
+                                   double *info, hipStream_t stream);
+int phast_hex8_pfczm_box_update(const phast_hex8_plan *plan, const double *W, double scale, const double *b,
+                                const double *cand, const double *lower, const double *upper,
+                                uint8_t *lower_active, uint8_t *upper_active, const uint8_t *prescribed,
+                                double kkt_tol, int32_t apply, int64_t n, double *grad, int64_t *counts,
+                                hipStream_t stream);
+int phast_hex8_closure_setup(phast_hex8_plan *plan, const uint8_t *cl_on);
+int phast_hex8_closure_pattern(const phast_hex8_plan *plan, const double *x, const double *g, const uint32_t *old,
+                               uint32_t *newp, double *out2, hipStream_t stream);
+int phast_hex8_mech_action_cl(const phast_hex8_plan *plan, const double *x, const double *g, const uint32_t *pat,
+                              int32_t mode, double *y, hipStream_t stream);
+int phast_hex8_stress_sums_cl(const phast_hex8_plan *plan, const double *x, const double *g, const uint32_t *pat,
+                              int32_t mode, double *out7, hipStream_t stream);
+int phast_hex8_mech_diagonal_cl(const phast_hex8_plan *plan, const double *g, const uint32_t *pat, int32_t mode,
+                                double *diag, hipStream_t stream);
+int phast_hex8_mech_solve_cl(const phast_hex8_plan *plan, const double *g, const uint32_t *pat,
+                             const uint8_t *fixed, const double *fixed_values, const double *rhs, const double *x0,
+                             const double *diag, double rtol, double atol, int32_t maxiter, int32_t check_every,
+                             int64_t n_mech, double *x_out, double *info, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

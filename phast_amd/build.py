@@ -14,7 +14,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libphast_mi.so"
-SOURCES = ["capi.cpp", "q4.hip", "hex8.hip", "linalg.hip", "mg.hip", "adjoint.hip", "t3.hip"]
+SOURCES = ["capi.cpp", "q4.hip", "hex8.hip", "linalg.hip", "mg.hip", "adjoint.hip", "t3.hip", "pfczm.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -40,6 +40,8 @@ struct MechArgs3 {
     double *y6;            // macro rows destination (y + 3n) or null
     double *out7;          // stress sums destination or null
     double *seam;          // voxel-parallel apply: top-face sums of each slab's last voxel layer
+    const uint8_t *clm;    // crack closure: 1 where the voxel adds (1 - g) sigma_c back
+    const uint32_t *pat;   // crack closure: fixed sign pattern, bit 4 q + k per voxel
     int32_t kz, tiles_x, tiles_y, nvals;
 };
 
@@ -199,6 +201,7 @@ __device__ __forceinline__ void settle(double &a, double &b, int32_t &c) {
 }
 __device__ __forceinline__ void settle2(double &a, double &b) { asm volatile("" : "+v"(a), "+v"(b)); }
 __device__ __forceinline__ void settle1(double &a) { asm volatile("" : "+v"(a)); }
+__device__ __forceinline__ void settle_u(uint32_t &a) { asm volatile("" : "+v"(a)); }
 
 // threads of the voxel-parallel block: the voxel tile rounded up to whole waves
 template <int OX, int OY>
@@ -206,7 +209,8 @@ constexpr int apply_threads() {
     return ((OX + 1) * (OY + 1) + 63) / 64 * 64;
 }
 
-template <int OX, int OY, int GMODE, bool ISO, bool ENERGY>
+// CL: crack closure off (0), at each point's own sign pattern (1) or at A.pat (2)
+template <int OX, int OY, int GMODE, bool ISO, bool ENERGY, int CL = 0>
 __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(const Hex8Geo G, MechArgs3 A) {
     constexpr int NT = apply_threads<OX, OY>();
     constexpr int NVX = OX + 1, NVY = OY + 1, NV = NVX * NVY;
@@ -271,19 +275,23 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
     double sE[6] = {0, 0, 0, 0, 0, 0}, en = 0.0;
     double acc[3] = {0.0, 0.0, 0.0};
     const int s00 = t - NVX - 1, s10 = t - NVX, s01 = t - 1;  // voxels left-below, below, left
-    // voxel material of layer k (lam, mu or the bank index), fetched one layer ahead
+    // voxel material of layer k (lam, mu or the bank index; with the closure, the closure
+    // flag in place of the bank index and the pattern word), fetched one layer ahead
     double ml0 = 0.0, ml1 = 0.0;
     int32_t mb = 0;
-    auto fetch_mat = [&](int k, double &l0, double &l1, int32_t &b) {
+    uint32_t mp = 0u;
+    auto fetch_mat = [&](int k, double &l0, double &l1, int32_t &b, uint32_t &pw) {
         const int64_t e = e_ij + nxy * wrap(k, nz);
         if (ISO) {
             l0 = A.lam[e];
             l1 = A.mu[e];
+            if (CL != 0) b = A.clm[e];
+            if (CL == 2) pw = A.pat[e];
         } else {
             b = A.bank_index[e];
         }
     };
-    fetch_mat(k0, ml0, ml1, mb);
+    fetch_mat(k0, ml0, ml1, mb, mp);
     {  // the first two node layers, both in flight before either is staged
         double pf1[NS][NF];
         fetch(k0, pf);
@@ -292,6 +300,7 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
         stage(1, pf1);
     }
     settle(ml0, ml1, mb);
+    if (CL == 2) settle_u(mp);
 #pragma unroll
     for (int k = 0; k < 6; k += 2) settle2(E[k], E[k + 1]);
     int cur = 0;
@@ -300,7 +309,8 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
         if (kv + 2 <= k1) fetch(kv + 2, pf);  // lands in su[cur] once this layer's voxels are done
         double nl0 = 0.0, nl1 = 0.0;
         int32_t nb = 0;
-        if (kv + 1 < k1) fetch_mat(kv + 1, nl0, nl1, nb);
+        uint32_t np = 0u;
+        if (kv + 1 < k1) fetch_mat(kv + 1, nl0, nl1, nb, np);
         double r[3][8];
         if (has_voxel) {
             const int64_t e = e_ij + nxy * wrap(kv, nz);
@@ -335,7 +345,8 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
                 for (int q = 0; q < 8; ++q) g[q] = 1.0;
             }
             if (ISO) {
-                voxel_mech<true, ENERGY>(G, u, E, g, nullptr, ml0, ml1, owner, r, sE, en);
+                const double lc = (CL != 0 && mb) ? ml0 : 0.0, mc = (CL != 0 && mb) ? ml1 : 0.0;
+                voxel_mech<true, ENERGY, CL>(G, u, E, g, nullptr, ml0, ml1, owner, r, sE, en, lc, mc, mp);
             } else {
                 voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)mb, 0.0, 0.0, owner, r, sE, en);
             }
@@ -355,7 +366,9 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
         ml0 = nl0;
         ml1 = nl1;
         mb = nb;
+        mp = np;
         settle(ml0, ml1, mb);
+        if (CL == 2) settle_u(mp);
         if (owner) {
             const int64_t node = iv + (int64_t)nx * (jv + (int64_t)ny * kv);
 #pragma unroll
@@ -492,10 +505,15 @@ __global__ __launch_bounds__(256) void hex8_degradation_kernel(const Hex8Geo G, 
 }
 
 // Jacobi diagonal: one thread per node gathers its 8 voxels (operator.py:1196-1254); the
-// thread of node n also forms voxel n's contribution to the macro block.
+// thread of node n also forms voxel n's contribution to the macro block.  With the crack
+// closure (clmode 1: every bit, 2: the bits of `pat`; isotropic voxels) each closure bit k of
+// Gauss point q adds w (1 - g_q) k_k (B_q^T v_k)^2 (operator.py:1255-1300): for dof c of a
+// node, (d N / d x_c)^2 (s_0 lam + s_{1+c} 2 mu), and s_0 lam + s_{1+i} 2 mu to macro row i < 3.
 __global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const double *g, const double *bank,
                                                         const int32_t *bank_index, const double *lam,
-                                                        const double *mu, int iso, double *diag, double *part) {
+                                                        const double *mu, int iso, double *diag, double *part,
+                                                        const uint8_t *clm = nullptr, const uint32_t *pat = nullptr,
+                                                        int clmode = 0) {
     __shared__ double red[4];
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
     const int64_t node = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -532,6 +550,14 @@ __global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const d
                 acc[0] += gw * tx;
                 acc[1] += gw * ty;
                 acc[2] += gw * tz;
+                if (clmode != 0 && clm[e]) {
+                    const uint32_t bits = clmode == 2 ? (pat[e] >> (4 * q)) & 15u : 15u;
+                    const double cw = (1.0 - g[e * 8 + q]) * G.w;
+                    const double l0 = (bits & 1u) ? lam[e] : 0.0, m2 = 2.0 * mu[e];
+                    acc[0] += cw * (bx * bx) * (l0 + ((bits & 2u) ? m2 : 0.0));
+                    acc[1] += cw * (by * by) * (l0 + ((bits & 4u) ? m2 : 0.0));
+                    acc[2] += cw * (bz * bz) * (l0 + ((bits & 8u) ? m2 : 0.0));
+                }
             }
         }
         diag[3 * node] = acc[0];
@@ -544,6 +570,16 @@ __global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const d
         voxel_C(iso, bank, bank_index, lam, mu, node, C);
 #pragma unroll
         for (int k2 = 0; k2 < 6; ++k2) dE[k2] = gws * C[k2 * 7];
+        if (clmode != 0 && clm[node]) {
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t bits = clmode == 2 ? (pat[node] >> (4 * q)) & 15u : 15u;
+                const double cw = (1.0 - g[node * 8 + q]) * G.w;
+                const double l0 = (bits & 1u) ? lam[node] : 0.0, m2 = 2.0 * mu[node];
+                dE[0] += cw * (l0 + ((bits & 2u) ? m2 : 0.0));
+                dE[1] += cw * (l0 + ((bits & 4u) ? m2 : 0.0));
+                dE[2] += cw * (l0 + ((bits & 8u) ? m2 : 0.0));
+            }
+        }
     }
 #pragma unroll
     for (int k2 = 0; k2 < 6; ++k2) {
@@ -774,7 +810,7 @@ static int launch_mech_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool
 
 template <int OX, int OY>
 static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool energy, hipStream_t st,
-                          const JacobiSpec *jac) {
+                          const JacobiSpec *jac, int cl = 0) {
     A.tiles_x = (int)ceil_div(pl->geo.nx, OX);
     A.tiles_y = (int)ceil_div(pl->geo.ny, OY);
     const int nbz = (int)ceil_div(pl->geo.nz, A.kz);
@@ -790,7 +826,16 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
     A.seam = pl->seam;
     dim3 grid(nb), block(apply_threads<OX, OY>());
 #define PHAST_L(GM, ISO, EN) hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, GM, ISO, EN>), grid, block, 0, st, pl->geo, A)
-    if (pl->iso) {
+#define PHAST_LC(EN, CL) hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, 1, true, EN, CL>), grid, block, 0, st, pl->geo, A)
+    if (cl != 0) {  // crack closure: g array, isotropic voxels (checked by the caller)
+        if (energy) {
+            if (cl == 1) PHAST_LC(true, 1);
+            else PHAST_LC(true, 2);
+        } else {
+            if (cl == 1) PHAST_LC(false, 1);
+            else PHAST_LC(false, 2);
+        }
+    } else if (pl->iso) {
         if (energy) {
             if (gmode == 0) PHAST_L(0, true, true);
             else if (gmode == 1) PHAST_L(1, true, true);
@@ -812,6 +857,7 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
         }
     }
 #undef PHAST_L
+#undef PHAST_LC
     PHAST_LAUNCHED();
     const int64_t nseam = 3 * (int64_t)pl->geo.nx * pl->geo.ny * nbz;
     if (jac)
@@ -826,7 +872,8 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
 }
 
 static int hex8_apply_any(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
-                          double *out7, bool energy, hipStream_t st, const JacobiSpec *jac) {
+                          double *out7, bool energy, hipStream_t st, const JacobiSpec *jac, int cl = 0,
+                          const uint32_t *pat = nullptr) {
     MechArgs3 A;
     std::memset(&A, 0, sizeof(A));
     A.x = x;
@@ -842,18 +889,33 @@ static int hex8_apply_any(const phast_hex8_plan *pl, const double *x, const doub
     A.out7 = out7;
     A.nvals = energy ? 7 : 6;
     A.kz = pl->kz;
+    A.clm = pl->cl_on;
+    A.pat = pat;
     const int gmode = d ? 2 : (g ? 1 : 0);
     int nb = 0, rc;
     if (!y) {  // stress sums only: still needs a write target for the nodal rows
         set_error("internal: hex8 apply without output");
         return PHAST_ERR_ARG;
     }
-    if (pl->tx == 16 && pl->ty == 16) rc = launch_apply_t<16, 16>(pl, A, gmode, energy, st, jac);
-    else if (pl->tx == 32 && pl->ty == 8) rc = launch_apply_t<32, 8>(pl, A, gmode, energy, st, jac);
-    else if (pl->tx == 16 && pl->ty == 14) rc = launch_apply_t<16, 14>(pl, A, gmode, energy, st, jac);
-    else if (pl->tx == 32 && pl->ty == 14) rc = launch_apply_t<32, 14>(pl, A, gmode, energy, st, jac);
-    else if (pl->tx == 32 && pl->ty == 6) rc = launch_apply_t<32, 6>(pl, A, gmode, energy, st, jac);
-    else if (pl->tx == 64 && pl->ty == 6) rc = launch_apply_t<64, 6>(pl, A, gmode, energy, st, jac);
+    if (cl != 0) {
+        if (!pl->cl_on || gmode != 1 || !pl->iso || (cl == 2 && !pat) || jac) {
+            set_error("internal: the closure apply needs the closure set-up, g (E,8), isotropic voxels and a "
+                      "pattern when fixed");
+            return PHAST_ERR_ARG;
+        }
+        if (!((pl->tx == 16 && (pl->ty == 16 || pl->ty == 14)) || (pl->tx == 32 && (pl->ty == 8 || pl->ty == 14 ||
+                                                                                     pl->ty == 6)) ||
+              (pl->tx == 64 && pl->ty == 6))) {
+            set_error("the crack-closure apply runs on the voxel-parallel tiles; the plan has %dx%d", pl->tx, pl->ty);
+            return PHAST_ERR_UNSUPPORTED;
+        }
+    }
+    if (pl->tx == 16 && pl->ty == 16) rc = launch_apply_t<16, 16>(pl, A, gmode, energy, st, jac, cl);
+    else if (pl->tx == 32 && pl->ty == 8) rc = launch_apply_t<32, 8>(pl, A, gmode, energy, st, jac, cl);
+    else if (pl->tx == 16 && pl->ty == 14) rc = launch_apply_t<16, 14>(pl, A, gmode, energy, st, jac, cl);
+    else if (pl->tx == 32 && pl->ty == 14) rc = launch_apply_t<32, 14>(pl, A, gmode, energy, st, jac, cl);
+    else if (pl->tx == 32 && pl->ty == 6) rc = launch_apply_t<32, 6>(pl, A, gmode, energy, st, jac, cl);
+    else if (pl->tx == 64 && pl->ty == 6) rc = launch_apply_t<64, 6>(pl, A, gmode, energy, st, jac, cl);
     else if (pl->tx == 16 && pl->ty == 8) rc = launch_mech_t<16, 8>(pl, A, gmode, energy, st, &nb, jac);
     else if (pl->tx == 8 && pl->ty == 8) rc = launch_mech_t<8, 8>(pl, A, gmode, energy, st, &nb, jac);
     else {
@@ -866,6 +928,11 @@ static int hex8_apply_any(const phast_hex8_plan *pl, const double *x, const doub
 int hex8_apply_impl(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,
                     double *out7, bool energy, hipStream_t st) {
     return hex8_apply_any(pl, x, g, d, y, out7, energy, st, nullptr);
+}
+
+int hex8_apply_closure(const phast_hex8_plan *pl, const double *x, const double *g, const uint32_t *pat, int mode,
+                       double *y, double *out7, bool energy, hipStream_t st) {
+    return hex8_apply_any(pl, x, g, nullptr, y, out7, energy, st, nullptr, mode, pat);
 }
 
 int hex8_apply_jacobi(const phast_hex8_plan *pl, double *x, const double *g, const double *d, double *Ax,
@@ -997,25 +1064,25 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_damage_vp_kern
 }
 
 template <int OX, int OY>
-static int launch_damage_vp_t(const phast_hex8_plan *pl, DmgArgs3 A, hipStream_t st) {
+static int launch_damage_vp_t(const phast_hex8_plan *pl, const Hex8Geo &G, DmgArgs3 A, hipStream_t st) {
     A.tiles_x = (int)ceil_div(pl->geo.nx, OX);
     A.tiles_y = (int)ceil_div(pl->geo.ny, OY);
     const int tz = (int)ceil_div(pl->geo.nz, A.kz);
     dim3 grid(A.tiles_x * A.tiles_y * tz), block(apply_threads<OX, OY>());
-    if (A.H) hipLaunchKernelGGL((hex8_damage_vp_kernel<OX, OY, true>), grid, block, 0, st, pl->geo, A);
-    else hipLaunchKernelGGL((hex8_damage_vp_kernel<OX, OY, false>), grid, block, 0, st, pl->geo, A);
+    if (A.H) hipLaunchKernelGGL((hex8_damage_vp_kernel<OX, OY, true>), grid, block, 0, st, G, A);
+    else hipLaunchKernelGGL((hex8_damage_vp_kernel<OX, OY, false>), grid, block, 0, st, G, A);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
 
 template <int TX, int TY>
-static int launch_damage_t(const phast_hex8_plan *pl, DmgArgs3 A, hipStream_t st) {
+static int launch_damage_t(const phast_hex8_plan *pl, const Hex8Geo &G, DmgArgs3 A, hipStream_t st) {
     A.tiles_x = (int)ceil_div(pl->geo.nx, TX);
     A.tiles_y = (int)ceil_div(pl->geo.ny, TY);
     const int tz = (int)ceil_div(pl->geo.nz, A.kz);
     dim3 grid(A.tiles_x * A.tiles_y * tz), block(TX * TY);
-    if (A.H) hipLaunchKernelGGL((hex8_damage_kernel<TX, TY, true>), grid, block, 0, st, pl->geo, A);
-    else hipLaunchKernelGGL((hex8_damage_kernel<TX, TY, false>), grid, block, 0, st, pl->geo, A);
+    if (A.H) hipLaunchKernelGGL((hex8_damage_kernel<TX, TY, true>), grid, block, 0, st, G, A);
+    else hipLaunchKernelGGL((hex8_damage_kernel<TX, TY, false>), grid, block, 0, st, G, A);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
@@ -1028,6 +1095,10 @@ using namespace phast;
 // operator.py:242-247): N, gradN at the 2x2x2 Gauss points, M = sum_q w N N^T,
 // L = sum_q w gradN gradN^T, both symmetrised, K = M/ell + ell L.
 void phast::build_voxel_constants(Hex8Geo &G, double hx, double hy, double hz, double ell) {
+    voxel_constants(G, hx, hy, hz, ell, nullptr, nullptr);
+}
+
+void phast::voxel_constants(Hex8Geo &G, double hx, double hy, double hz, double ell, double *Mo, double *Lo) {
     const double a = 1.0 / std::sqrt(3.0);
     const double pt[2] = {-a, a};
     double N[8][8], dN[8][8][3];
@@ -1053,6 +1124,8 @@ void phast::build_voxel_constants(Hex8Geo &G, double hx, double hy, double hz, d
         for (int j = 0; j < 8; ++j) {
             const double Ms = 0.5 * (M[i][j] + M[j][i]), Ls = 0.5 * (L[i][j] + L[j][i]);
             G.K[i * 8 + j] = Ms / ell + ell * Ls;
+            if (Mo) Mo[i * 8 + j] = Ms;
+            if (Lo) Lo[i * 8 + j] = Ls;
         }
     for (int q = 0; q < 8; ++q)
         for (int c = 0; c < 8; ++c) G.Nq[q * 8 + c] = N[q][c];
@@ -1087,6 +1160,10 @@ extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double
     pl->geo.omk = 1.0 - kappa;
     pl->geo.ch = 2.0 * (1.0 - kappa);
     build_voxel_constants(pl->geo, hx, hy, hz, ell);
+    pl->h[0] = hx;
+    pl->h[1] = hy;
+    pl->h[2] = hz;
+    pl->ell = ell;
     pl->n = (int64_t)nx * ny * nz;
     pl->n_bank = n_bank;
     for (int64_t e = 0; e < pl->n; ++e) {
@@ -1149,6 +1226,8 @@ extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
     (void)hipFree(pl->mu);
     (void)hipFree(pl->part);
     (void)hipFree(pl->seam);
+    (void)hipFree(pl->a1);
+    (void)hipFree(pl->cl_on);
     release_solver_work(pl);
     delete pl;
     return PHAST_OK;
@@ -1214,7 +1293,23 @@ extern "C" int phast_hex8_mech_diagonal(const phast_hex8_plan *pl, const double 
     PHAST_REQUIRE(pl && diag, "phast_hex8_mech_diagonal: null argument");
     const int nb = (int)ceil_div(pl->n, 256);
     hipLaunchKernelGGL(hex8_diag_kernel, dim3(nb), dim3(256), 0, st, pl->geo, g, pl->bank, pl->bank_index, pl->lam,
-                       pl->mu, pl->iso ? 1 : 0, diag, pl->part);
+                       pl->mu, pl->iso ? 1 : 0, diag, pl->part, (const uint8_t *)nullptr, (const uint32_t *)nullptr, 0);
+    PHAST_LAUNCHED();
+    hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, pl->part, nb, 6, diag + 3 * pl->n,
+                       (double *)nullptr);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+// The Jacobi diagonal with the crack closure: every closure bit (mode 1) or those of `pat`
+// (mode 2) adds its (1 - g) term (operator.py:1196-1300).
+extern "C" int phast_hex8_mech_diagonal_cl(const phast_hex8_plan *pl, const double *g, const uint32_t *pat,
+                                           int32_t mode, double *diag, hipStream_t st) {
+    PHAST_REQUIRE(pl && g && diag && (mode == 1 || (mode == 2 && pat)), "phast_hex8_mech_diagonal_cl: bad argument");
+    PHAST_REQUIRE(pl->cl_on && pl->iso, "phast_hex8_mech_diagonal_cl: the plan has no crack closure");
+    const int nb = (int)ceil_div(pl->n, 256);
+    hipLaunchKernelGGL(hex8_diag_kernel, dim3(nb), dim3(256), 0, st, pl->geo, g, pl->bank, pl->bank_index, pl->lam,
+                       pl->mu, 1, diag, pl->part, (const uint8_t *)pl->cl_on, pat, (int)mode);
     PHAST_LAUNCHED();
     hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, pl->part, nb, 6, diag + 3 * pl->n,
                        (double *)nullptr);
@@ -1230,9 +1325,10 @@ extern "C" int phast_hex8_driving_force(const phast_hex8_plan *pl, const double 
     return PHAST_OK;
 }
 
-extern "C" int phast_hex8_damage_action(const phast_hex8_plan *pl, const double *d, const double *H, double scale,
-                                        double *y, hipStream_t st) {
-    PHAST_REQUIRE(pl && d && y && d != y, "phast_hex8_damage_action: bad argument");
+// y = scale (Gc_e K d_e + ch sum_q w H_q N_q N_q^T d_e) with the constants of G: the AT2
+// damage operator with the plan's own constants, the PF-CZM Newton tangent with geo_t.
+int phast::hex8_damage_launch(const phast_hex8_plan *pl, const Hex8Geo &G, const double *d, const double *H,
+                              double scale, double *y, hipStream_t st) {
     DmgArgs3 A;
     std::memset(&A, 0, sizeof(A));
     A.d = d;
@@ -1241,34 +1337,42 @@ extern "C" int phast_hex8_damage_action(const phast_hex8_plan *pl, const double 
     A.y = y;
     A.scale = scale;
     A.kz = pl->kz;
-    if (pl->tx == 16 && pl->ty == 16) return launch_damage_vp_t<16, 16>(pl, A, st);
-    if (pl->tx == 16 && pl->ty == 8) return launch_damage_t<16, 8>(pl, A, st);
-    if (pl->tx == 8 && pl->ty == 8) return launch_damage_t<8, 8>(pl, A, st);
-    if (pl->tx == 32 && pl->ty == 8) return launch_damage_vp_t<32, 8>(pl, A, st);
-    if (pl->tx == 16 && pl->ty == 14) return launch_damage_vp_t<16, 14>(pl, A, st);
-    if (pl->tx == 32 && pl->ty == 14) return launch_damage_vp_t<32, 14>(pl, A, st);
-    if (pl->tx == 32 && pl->ty == 6) return launch_damage_vp_t<32, 6>(pl, A, st);
-    if (pl->tx == 64 && pl->ty == 6) return launch_damage_vp_t<64, 6>(pl, A, st);
+    if (pl->tx == 16 && pl->ty == 16) return launch_damage_vp_t<16, 16>(pl, G, A, st);
+    if (pl->tx == 16 && pl->ty == 8) return launch_damage_t<16, 8>(pl, G, A, st);
+    if (pl->tx == 8 && pl->ty == 8) return launch_damage_t<8, 8>(pl, G, A, st);
+    if (pl->tx == 32 && pl->ty == 8) return launch_damage_vp_t<32, 8>(pl, G, A, st);
+    if (pl->tx == 16 && pl->ty == 14) return launch_damage_vp_t<16, 14>(pl, G, A, st);
+    if (pl->tx == 32 && pl->ty == 14) return launch_damage_vp_t<32, 14>(pl, G, A, st);
+    if (pl->tx == 32 && pl->ty == 6) return launch_damage_vp_t<32, 6>(pl, G, A, st);
+    if (pl->tx == 64 && pl->ty == 6) return launch_damage_vp_t<64, 6>(pl, G, A, st);
     set_error("unsupported hex8 tile");
     return PHAST_ERR_UNSUPPORTED;
+}
+
+int phast::hex8_damage_rd_launch(const phast_hex8_plan *pl, const Hex8Geo &G, const double *H, double scale,
+                                 double *b, double *diag, hipStream_t st) {
+    hipLaunchKernelGGL(hex8_damage_rd_kernel, dim3((unsigned)ceil_div(pl->n, 256)), dim3(256), 0, st, G, H, pl->Gc,
+                       scale, b, diag);
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_damage_action(const phast_hex8_plan *pl, const double *d, const double *H, double scale,
+                                        double *y, hipStream_t st) {
+    PHAST_REQUIRE(pl && d && y && d != y, "phast_hex8_damage_action: bad argument");
+    return hex8_damage_launch(pl, pl->geo, d, H, scale, y, st);
 }
 
 extern "C" int phast_hex8_damage_rhs(const phast_hex8_plan *pl, const double *H, double scale, double *b,
                                      hipStream_t st) {
     PHAST_REQUIRE(pl && b, "phast_hex8_damage_rhs: null argument");
-    hipLaunchKernelGGL(hex8_damage_rd_kernel, dim3((unsigned)ceil_div(pl->n, 256)), dim3(256), 0, st, pl->geo, H,
-                       pl->Gc, scale, b, (double *)nullptr);
-    PHAST_LAUNCHED();
-    return PHAST_OK;
+    return hex8_damage_rd_launch(pl, pl->geo, H, scale, b, nullptr, st);
 }
 
 extern "C" int phast_hex8_damage_diagonal(const phast_hex8_plan *pl, const double *H, double scale, double *diag,
                                           hipStream_t st) {
     PHAST_REQUIRE(pl && diag, "phast_hex8_damage_diagonal: null argument");
-    hipLaunchKernelGGL(hex8_damage_rd_kernel, dim3((unsigned)ceil_div(pl->n, 256)), dim3(256), 0, st, pl->geo, H,
-                       pl->Gc, scale, (double *)nullptr, diag);
-    PHAST_LAUNCHED();
-    return PHAST_OK;
+    return hex8_damage_rd_launch(pl, pl->geo, H, scale, nullptr, diag, st);
 }
 
 int phast::hex8_apply(const phast_hex8_plan *pl, const double *x, const double *g, const double *d, double *y,

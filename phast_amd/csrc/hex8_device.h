@@ -169,10 +169,18 @@ __device__ __forceinline__ void eig3(double a11, double a22, double a33, double 
 // Internal force of one voxel: r[c][corner] = sum_q w B_q^T (g_q C (B_q u + E))
 // (operator.py:1108-1169).  With OWN, sE += sum_q w sigma_q and (ENERGY) en += 0.5 sum_q
 // w sigma.eps.  sum-factorised gradients; g w folded into the moduli.
-template <bool ISO, bool ENERGY>
+//
+// Crack closure (CL != 0, isotropic voxels only; operator.py:1039-1105, splits.py:424-560):
+// sigma += (1 - g) sum_k s_k k_k (v_k . eps) v_k over the rows v = (1, e_x, e_y, e_z) with
+// moduli (lamc, 2 muc, 2 muc, 2 muc) -- zero where the voxel adds nothing back.  The bits
+// s_k are the signs [v_k . eps < 0] of the voxel's own strain (CL == 1) or bit 4 q + k of
+// `pat` (CL == 2).
+template <bool ISO, bool ENERGY, int CL = 0>
 __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8], const double E[6],
                                            const double g[8], const double *__restrict__ C, double lam,
-                                           double mu, bool own, double r[3][8], double sE[6], double &en) {
+                                           double mu, bool own, double r[3][8], double sE[6], double &en,
+                                           double lamc = 0.0, double muc = 0.0, uint32_t pat = 0u) {
+    static_assert(CL == 0 || ISO, "the closure add-back is built for isotropic voxels");
     double gx[3][2][2], gy[3][2][2], gz[3][2][2];
     voxel_grad(G, u, gx, gy, gz);
     fold_macro(E, gx, gy, gz);
@@ -198,6 +206,28 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
                     s[3] = mg * e[3];
                     s[4] = mg * e[4];
                     s[5] = mg * e[5];
+                    if (CL != 0) {
+                        const int q = ix + 2 * iy + 4 * iz;
+                        const double cw = (1.0 - g[q]) * G.w;
+                        const double tr = (e[0] + e[1]) + e[2];
+                        bool b0, b1, b2, b3;
+                        if (CL == 1) {
+                            b0 = tr < 0.0;
+                            b1 = e[0] < 0.0;
+                            b2 = e[1] < 0.0;
+                            b3 = e[2] < 0.0;
+                        } else {
+                            const uint32_t bits = pat >> (4 * q);
+                            b0 = bits & 1u;
+                            b1 = bits & 2u;
+                            b2 = bits & 4u;
+                            b3 = bits & 8u;
+                        }
+                        const double ct = b0 ? lamc * tr : 0.0, m2 = 2.0 * muc;
+                        s[0] += cw * (ct + (b1 ? m2 * e[0] : 0.0));
+                        s[1] += cw * (ct + (b2 ? m2 * e[1] : 0.0));
+                        s[2] += cw * (ct + (b3 ? m2 * e[2] : 0.0));
+                    }
                 } else {
 #pragma unroll
                     for (int k = 0; k < 6; ++k) {

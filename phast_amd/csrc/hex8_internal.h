@@ -16,6 +16,19 @@ struct Hex8Geo {
     double Nq[64];          // N[q*8 + a], lexicographic corners, xi-fastest Gauss points
 };
 
+// Wu's PF-CZM law on the voxel cell (physics/pfczm.py; solid3d/operator.py:272-583):
+// g = (1 - eta) A/(A + Q) + eta, A = (1 - d)^p, Q = a1 d (1 + a2 d + a2 a3 d^2) with a1 per
+// voxel; the damage energy Gc/(pi ell)(2 m.d - d.M d) + Gc ell/pi d.L d + sum_q w H g(d_q).
+struct PfczmLaw {
+    double a2, a3, p, eta;
+    int32_t p_int;          // p == 2: the integer-power forms of the reference
+    int32_t pad;
+    double c_local, c_grad; // 1/(pi ell), ell/pi
+    double concave, floor;  // 2/(pi ell) (times Gc per voxel) and the tangent floor fraction 0.25
+    double M[64], L[64];    // consistent mass and stiffness of the voxel, symmetrised [a*8+b]
+    double m[8];            // row sums of M
+};
+
 }  // namespace phast
 
 struct phast_hex8_plan {
@@ -36,6 +49,15 @@ struct phast_hex8_plan {
     // scratch of the solves on this plan (linalg.hip): vectors of 3n+6, the flag state and the
     // pinned readback; made by phast_hex8_solver_reserve or the first solve, freed with the plan
     mutable void *solver_work = nullptr;
+    double h[3] = {0.0, 0.0, 0.0};
+    double ell = 0.0;
+    // PF-CZM damage law (phast_hex8_pfczm_setup): a1 per voxel, the law and the Newton
+    // tangent's constants (K = (2 ell/pi) L, ch = 1) for the damage operator kernels
+    double *a1 = nullptr;
+    phast::PfczmLaw law{};
+    phast::Hex8Geo geo_t{};
+    // crack closure (phast_hex8_closure_setup): 1 where a voxel adds (1 - g) sigma_c back
+    uint8_t *cl_on = nullptr;
 };
 
 namespace phast {
@@ -52,4 +74,15 @@ int hex8_apply_jacobi(const phast_hex8_plan *pl, double *x, const double *g, con
 int hex8_ensure_seam(phast_hex8_plan *pl, int32_t kz);
 // Free the solver scratch of a plan (linalg.hip; called by phast_hex8_plan_destroy).
 void release_solver_work(const phast_hex8_plan *pl);
+// y = A(g) x with the crack closure at each point's own pattern (mode 1) or at `pat` (mode 2).
+int hex8_apply_closure(const phast_hex8_plan *pl, const double *x, const double *g, const uint32_t *pat, int mode,
+                       double *y, double *out7, bool energy, hipStream_t st);
+// y = scale (Gc K d + ch sum_q w H N N^T d) and its rhs / diagonal with the constants of G.
+int hex8_damage_launch(const phast_hex8_plan *pl, const Hex8Geo &G, const double *d, const double *H, double scale,
+                       double *y, hipStream_t st);
+int hex8_damage_rd_launch(const phast_hex8_plan *pl, const Hex8Geo &G, const double *H, double scale, double *b,
+                          double *diag, hipStream_t st);
+// voxel N, M, L (symmetrised), K = M/ell + ell L and the interpolation constants into G;
+// M and L also into Mo, Lo when given
+void voxel_constants(Hex8Geo &G, double hx, double hy, double hz, double ell, double *Mo, double *Lo);
 }  // namespace phast

@@ -621,11 +621,14 @@ struct MechCtx {
     const double *g, *d;
     const uint8_t *fixed;
     int64_t n;
+    const uint32_t *pat;  // crack closure: the fixed sign pattern (null: no closure)
 };
 
 static int mech_apply_cb(void *c, const double *in, double *out, hipStream_t st) {
     auto *m = static_cast<MechCtx *>(c);
-    int rc = m->d ? phast_hex8_mech_action_d(m->plan, in, m->d, out, st) : phast_hex8_mech_action(m->plan, in, m->g, out, st);
+    int rc = m->pat  ? phast_hex8_mech_action_cl(m->plan, in, m->g, m->pat, 2, out, st)
+             : m->d ? phast_hex8_mech_action_d(m->plan, in, m->d, out, st)
+                    : phast_hex8_mech_action(m->plan, in, m->g, out, st);
     if (rc) return rc;
     if (m->fixed) {
         hipLaunchKernelGGL(k_zero_fixed, dim3(nblk(m->n)), dim3(256), 0, st, m->n, m->fixed, out);
@@ -634,14 +637,18 @@ static int mech_apply_cb(void *c, const double *in, double *out, hipStream_t st)
     return PHAST_OK;
 }
 
+// The scalar damage systems: form 0 the AT2 operator with history H, form 1 the PF-CZM
+// Newton tangent with the curvatures W in H's place.
 struct DmgCtx {
     const phast_hex8_plan *plan;
     const double *H;
     double scale;
+    int form;
 };
 
 static int dmg_apply_cb(void *c, const double *in, double *out, hipStream_t st) {
     auto *m = static_cast<DmgCtx *>(c);
+    if (m->form == 1) return phast_hex8_pfczm_tangent_action(m->plan, in, m->H, m->scale, out, st);
     return phast_hex8_damage_action(m->plan, in, m->H, m->scale, out, st);
 }
 
@@ -661,7 +668,8 @@ static int mg_precond_cb(void *c, const double *in, double *out, hipStream_t st)
 static int mech_solve_impl(const phast_hex8_plan *plan, phast_hex8_mg *mg, const double *g, const double *d,
                            const uint8_t *fixed, const double *fixed_values, const double *rhs, const double *x0,
                            const double *diag, double rtol, double atol, int32_t maxiter, int32_t check_every,
-                           int64_t n_mech, double *x_out, double *info, hipStream_t st);
+                           int64_t n_mech, double *x_out, double *info, hipStream_t st,
+                           const uint32_t *pat = nullptr);
 
 extern "C" int phast_hex8_mech_solve(const phast_hex8_plan *plan, const double *g, const double *d,
                                      const uint8_t *fixed, const double *fixed_values, const double *rhs,
@@ -670,6 +678,18 @@ extern "C" int phast_hex8_mech_solve(const phast_hex8_plan *plan, const double *
                                      hipStream_t st) {
     return mech_solve_impl(plan, nullptr, g, d, fixed, fixed_values, rhs, x0, diag, rtol, atol, maxiter, check_every,
                            n_mech, x_out, info, st);
+}
+
+// The same solve with the crack closure at the fixed sign pattern `pat` (A_s(g) of
+// operator.py:1039-1105; g required).
+extern "C" int phast_hex8_mech_solve_cl(const phast_hex8_plan *plan, const double *g, const uint32_t *pat,
+                                        const uint8_t *fixed, const double *fixed_values, const double *rhs,
+                                        const double *x0, const double *diag, double rtol, double atol,
+                                        int32_t maxiter, int32_t check_every, int64_t n_mech, double *x_out,
+                                        double *info, hipStream_t st) {
+    PHAST_REQUIRE(g && pat, "phast_hex8_mech_solve_cl: g and the pattern are required");
+    return mech_solve_impl(plan, nullptr, g, nullptr, fixed, fixed_values, rhs, x0, diag, rtol, atol, maxiter,
+                           check_every, n_mech, x_out, info, st, pat);
 }
 
 // The same solve preconditioned by the voxel multigrid V-cycle (phast_hex8_mg_update must
@@ -687,7 +707,7 @@ extern "C" int phast_hex8_mech_solve_mg(const phast_hex8_plan *plan, phast_hex8_
 static int mech_solve_impl(const phast_hex8_plan *plan, phast_hex8_mg *mg, const double *g, const double *d,
                            const uint8_t *fixed, const double *fixed_values, const double *rhs, const double *x0,
                            const double *diag, double rtol, double atol, int32_t maxiter, int32_t check_every,
-                           int64_t n_mech, double *x_out, double *info, hipStream_t st) {
+                           int64_t n_mech, double *x_out, double *info, hipStream_t st, const uint32_t *pat) {
     PHAST_REQUIRE(plan && fixed && fixed_values && diag && x_out && info, "phast_hex8_mech_solve: null argument");
     PHAST_REQUIRE(maxiter >= 1 && check_every >= 1, "maxiter and check_every must be >= 1");
     PHAST_REQUIRE(!(rtol == 0.0 && atol == 0.0) && rtol >= 0.0 && atol >= 0.0, "rtol and atol must not both be zero");
@@ -702,14 +722,14 @@ static int mech_solve_impl(const phast_hex8_plan *plan, phast_hex8_mg *mg, const
     // x_c = where(fixed, values, 0); image = A x_c; b_f = where(free, rhs - image, 0)
     hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, fixed, fixed_values, (const double *)nullptr, 0.0, xc);
     PHAST_LAUNCHED();
-    MechCtx raw{plan, g, d, nullptr, n};
+    MechCtx raw{plan, g, d, nullptr, n, pat};
     int rc = mech_apply_cb(&raw, xc, W.Ap, st);
     if (rc == PHAST_OK) {
         hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, rhs ? 1.0 : 0.0, rhs ? rhs : W.Ap, -1.0, W.Ap,
                            fixed, 0.0, bf);
         hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, diag, (const double *)nullptr, 1.0, df);
         if (x0) hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, x0, (const double *)nullptr, 0.0, x0f);
-        MechCtx ctx{plan, g, d, fixed, n};
+        MechCtx ctx{plan, g, d, fixed, n, pat};
         rc = pcg_flags(W, mech_apply_cb, &ctx, n, fixed, bf, df, x0f, y, rtol, atol, maxiter, check_every, info, st,
                        mg ? mg_precond_cb : nullptr, mg);
         // x = where(free, y, x_c)
@@ -721,11 +741,31 @@ static int mech_solve_impl(const phast_hex8_plan *plan, phast_hex8_mg *mg, const
 
 // box_kkt_h8.py:350-432 solve_h8_scalar_dirichlet_pcg for the scaled AT2 damage system
 // (operator.py:1496-1509).  b and diag are the scaled rhs / diagonal.  info[5] as pcg_stop.
+static int dirichlet_pcg_impl(const phast_hex8_plan *plan, int form, const double *H, double scale, const double *b,
+                              const double *diag, const uint8_t *fixed, const double *values, const double *initial,
+                              double tol, int32_t max_iter, int64_t n, double *out, double *info, hipStream_t st);
+
 extern "C" int phast_hex8_damage_dirichlet_pcg(const phast_hex8_plan *plan, const double *H, double scale,
                                                const double *b, const double *diag, const uint8_t *fixed,
                                                const double *values, const double *initial, double tol,
                                                int32_t max_iter, int64_t n, double *out, double *info,
                                                hipStream_t st) {
+    return dirichlet_pcg_impl(plan, 0, H, scale, b, diag, fixed, values, initial, tol, max_iter, n, out, info, st);
+}
+
+// The same solve on the PF-CZM Newton tangent with curvatures W (operator.py:566-609).
+extern "C" int phast_hex8_pfczm_dirichlet_pcg(const phast_hex8_plan *plan, const double *W, double scale,
+                                              const double *b, const double *diag, const uint8_t *fixed,
+                                              const double *values, const double *initial, double tol,
+                                              int32_t max_iter, int64_t n, double *out, double *info,
+                                              hipStream_t st) {
+    PHAST_REQUIRE(W, "phast_hex8_pfczm_dirichlet_pcg: null curvatures");
+    return dirichlet_pcg_impl(plan, 1, W, scale, b, diag, fixed, values, initial, tol, max_iter, n, out, info, st);
+}
+
+static int dirichlet_pcg_impl(const phast_hex8_plan *plan, int form, const double *H, double scale, const double *b,
+                              const double *diag, const uint8_t *fixed, const double *values, const double *initial,
+                              double tol, int32_t max_iter, int64_t n, double *out, double *info, hipStream_t st) {
     PHAST_REQUIRE(plan && b && diag && fixed && values && out && info, "phast_hex8_damage_dirichlet_pcg: null argument");
     PHAST_REQUIRE(tol > 0.0 && max_iter >= 1, "tolerance must be positive and max_iterations >= 1");
     Work *Wp = nullptr;
@@ -734,7 +774,7 @@ extern "C" int phast_hex8_damage_dirichlet_pcg(const phast_hex8_plan *plan, cons
     const int nb = nblk(n);
     double *known = W.xc, *rr = W.bf, *y = W.y;
     hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, fixed, values, (const double *)nullptr, 0.0, known);
-    DmgCtx ctx{plan, H, scale};
+    DmgCtx ctx{plan, H, scale, form};
     int rc = dmg_apply_cb(&ctx, known, W.Ap, st);
     if (rc == PHAST_OK) {
         // reduced rhs = (b - A known)[free]
@@ -763,17 +803,41 @@ extern "C" int phast_hex8_damage_dirichlet_pcg(const phast_hex8_plan *plan, cons
 // One active-set bookkeeping pass: grad = A(cand) - b (scaled system), add/release counts
 // and, with apply != 0, the updated masks.  counts: [added_lower, added_upper,
 // released_lower, released_upper, free]; grad (n) is written.
+static int box_update_impl(const phast_hex8_plan *plan, int form, const double *H, double scale, const double *b,
+                           const double *cand, const double *lower, const double *upper, uint8_t *lower_active,
+                           uint8_t *upper_active, const uint8_t *prescribed, double kkt_tol, int32_t apply, int64_t n,
+                           double *grad, int64_t *counts, hipStream_t st);
+
 extern "C" int phast_hex8_box_update(const phast_hex8_plan *plan, const double *H, double scale, const double *b,
                                      const double *cand, const double *lower, const double *upper,
                                      uint8_t *lower_active, uint8_t *upper_active, const uint8_t *prescribed,
                                      double kkt_tol, int32_t apply, int64_t n, double *grad, int64_t *counts,
                                      hipStream_t st) {
+    return box_update_impl(plan, 0, H, scale, b, cand, lower, upper, lower_active, upper_active, prescribed, kkt_tol,
+                           apply, n, grad, counts, st);
+}
+
+// The same pass on the PF-CZM Newton tangent with curvatures W.
+extern "C" int phast_hex8_pfczm_box_update(const phast_hex8_plan *plan, const double *W, double scale,
+                                           const double *b, const double *cand, const double *lower,
+                                           const double *upper, uint8_t *lower_active, uint8_t *upper_active,
+                                           const uint8_t *prescribed, double kkt_tol, int32_t apply, int64_t n,
+                                           double *grad, int64_t *counts, hipStream_t st) {
+    PHAST_REQUIRE(W, "phast_hex8_pfczm_box_update: null curvatures");
+    return box_update_impl(plan, 1, W, scale, b, cand, lower, upper, lower_active, upper_active, prescribed, kkt_tol,
+                           apply, n, grad, counts, st);
+}
+
+static int box_update_impl(const phast_hex8_plan *plan, int form, const double *H, double scale, const double *b,
+                           const double *cand, const double *lower, const double *upper, uint8_t *lower_active,
+                           uint8_t *upper_active, const uint8_t *prescribed, double kkt_tol, int32_t apply, int64_t n,
+                           double *grad, int64_t *counts, hipStream_t st) {
     PHAST_REQUIRE(plan && b && cand && lower && upper && lower_active && upper_active && prescribed && grad && counts,
                   "phast_hex8_box_update: null argument");
     Work *Wp = nullptr;
     if (int rc = plan_work(plan, 3 * n + 6, &Wp)) return rc;
     Work &W = *Wp;
-    DmgCtx ctx{plan, H, scale};
+    DmgCtx ctx{plan, H, scale, form};
     if (int rc = dmg_apply_cb(&ctx, cand, W.Ap, st)) return rc;
     PHAST_TRY0(zero_parts(W, st));
     hipLaunchKernelGGL(k_box_update, dim3(nblk(n)), dim3(256), 0, st, n, W.Ap, b, cand, lower, upper, lower_active,

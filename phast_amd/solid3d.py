@@ -41,6 +41,25 @@ for _name, _args in {
                              [C.c_double, C.c_int32, C.c_int64, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
     "phast_hex8_solver_reserve": [C.c_void_p],
     "phast_hex8_solver_release": [C.c_void_p],
+    "phast_hex8_pfczm_setup": [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double],
+    "phast_hex8_pfczm_degradation": [C.c_void_p] * 6,
+    "phast_hex8_pfczm_energy": [C.c_void_p] * 5,
+    "phast_hex8_pfczm_gradient": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p],
+    "phast_hex8_pfczm_tangent_weights": [C.c_void_p] * 5,
+    "phast_hex8_pfczm_tangent_action": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p],
+    "phast_hex8_pfczm_tangent_diagonal": [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p],
+    "phast_hex8_pfczm_dirichlet_pcg": [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 5 +
+                                      [C.c_double, C.c_int32, C.c_int64, C.c_void_p, C.POINTER(C.c_double),
+                                       C.c_void_p],
+    "phast_hex8_pfczm_box_update": [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7 +
+                                   [C.c_double, C.c_int32, C.c_int64, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
+    "phast_hex8_closure_setup": [C.c_void_p, C.c_void_p],
+    "phast_hex8_closure_pattern": [C.c_void_p] * 7,
+    "phast_hex8_mech_action_cl": [C.c_void_p] * 4 + [C.c_int32, C.c_void_p, C.c_void_p],
+    "phast_hex8_stress_sums_cl": [C.c_void_p] * 4 + [C.c_int32, C.c_void_p, C.c_void_p],
+    "phast_hex8_mech_diagonal_cl": [C.c_void_p] * 3 + [C.c_int32, C.c_void_p, C.c_void_p],
+    "phast_hex8_mech_solve_cl": [C.c_void_p] * 8 + [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int64,
+                                                     C.c_void_p, C.POINTER(C.c_double), C.c_void_p],
 }.items():
     _fn = getattr(_L, _name)
     _fn.argtypes = _args
@@ -135,12 +154,14 @@ class VoxelGrid:
 
 @dataclass(frozen=True)
 class PhaseMaterial:
-    """Isotropic phase (solid3d/model.py PhaseMaterial, isotropic kind)."""
+    """Isotropic phase (solid3d/model.py PhaseMaterial, isotropic kind); ``tensile_strength``
+    f_t is needed by the PF-CZM law only."""
     name: str
     E: float
     nu: float
     Gc: float
     driver: str = "spectral"
+    tensile_strength: float | None = None
 
 
 def isotropic_stiffness_3d(E: float, nu: float) -> np.ndarray:
@@ -166,6 +187,8 @@ class ElementMaterialArrays:
     mu: np.ndarray
     driver: np.ndarray           # (E,) int8
     phase: np.ndarray            # (E,) int
+    E: np.ndarray | None = None     # (E,) Young's modulus of the voxel's phase (PF-CZM E_ref)
+    f_t: np.ndarray | None = None   # (E,) tensile strength (NaN where the phase has none)
 
 
 def build_element_arrays(phase_id: np.ndarray, phases: Mapping[str, PhaseMaterial],
@@ -183,6 +206,7 @@ def build_element_arrays(phase_id: np.ndarray, phases: Mapping[str, PhaseMateria
     bank = []
     n_ids = len(PHASE_NAMES)
     Gc_id, lam_id, mu_id = np.zeros(n_ids), np.zeros(n_ids), np.zeros(n_ids)
+    E_id, ft_id = np.zeros(n_ids), np.full(n_ids, np.nan)
     drv_id = np.zeros(n_ids, dtype=np.int8)
     for slot, name in enumerate(present):
         m = phases[name]
@@ -192,13 +216,16 @@ def build_element_arrays(phase_id: np.ndarray, phases: Mapping[str, PhaseMateria
         Gc_id[pid] = m.Gc
         lam_id[pid] = m.E * m.nu / ((1.0 + m.nu) * (1.0 - 2.0 * m.nu))
         mu_id[pid] = m.E / (2.0 * (1.0 + m.nu))
+        E_id[pid] = m.E
+        if m.tensile_strength is not None:
+            ft_id[pid] = float(m.tensile_strength)
         if name == "pore" or m.driver == "none":  # materials.py:450-471
             drv_id[pid] = DRIVER_NONE
         else:
             drv_id[pid] = DRIVER_SPECTRAL if energy_split_isotropic == "spectral" else DRIVER_FULL
     Gc = Gc_id[phase_flat] if Gc_field is None else np.asarray(Gc_field, dtype=np.float64).reshape(-1)
     return ElementMaterialArrays(np.stack(bank), index, Gc, lam_id[phase_flat], mu_id[phase_flat],
-                                 drv_id[phase_flat], phase_flat)
+                                 drv_id[phase_flat], phase_flat, E_id[phase_flat], ft_id[phase_flat])
 
 
 @dataclass(frozen=True)
@@ -241,15 +268,50 @@ def choose_pinned_node(shape, phase_flat: np.ndarray, pore_id: int | None = PHAS
 # the operator
 # ----------------------------------------------------------------------------------------
 
+_PATTERN_SHIFTS = torch.arange(32, dtype=torch.int64).view(8, 4)   # bit 4 q + k
+
+
+@dataclass(eq=False)
+class ClosurePattern:
+    """Sign pattern of the isotropic crack closure (solid3d/model.py:1200-1223): ``iso``
+    (E, 8, 4) bool holds ``[tr eps < 0], [eps_xx < 0], [eps_yy < 0], [eps_zz < 0]`` per Gauss
+    point.  Kept packed on the device, one 32-bit word per voxel (bit 4 q + k), which is the
+    form the kernels read; ``iso`` unpacks it.  There is no laminate band group here."""
+    bits: torch.Tensor            # (E,) int32
+    band: None = None
+
+    @classmethod
+    def from_iso(cls, iso: torch.Tensor) -> "ClosurePattern":
+        if not isinstance(iso, torch.Tensor) or iso.dtype != torch.bool or iso.dim() != 3 or iso.shape[1:] != (8, 4):
+            raise TypeError("iso must be a torch.bool tensor of shape (E, 8, 4)")
+        words = (iso.to(torch.int64) << _PATTERN_SHIFTS.to(iso.device)).sum(dim=(1, 2))
+        words = torch.where(words >= 2 ** 31, words - 2 ** 32, words)
+        return cls(words.to(torch.int32).contiguous())
+
+    @property
+    def iso(self) -> torch.Tensor:
+        words = self.bits.to(torch.int64) & 0xFFFFFFFF
+        return ((words.view(-1, 1, 1) >> _PATTERN_SHIFTS.to(words.device)) & 1).to(torch.bool)
+
+    def groups(self):
+        return (self.iso, None)
+
+    def equals(self, other: "ClosurePattern") -> bool:
+        return torch.equal(self.bits, other.bits)
+
+
 @dataclass(frozen=True)
 class H8ScalarSPDSystem:
-    """Matrix-free SPD data of one scalar subproblem (box_kkt_h8.py:63-122)."""
+    """Matrix-free SPD data of one scalar subproblem (box_kkt_h8.py:63-122).  ``_form`` 0: the
+    AT2 damage system with history ``_H``; 1: the PF-CZM Newton tangent with the curvatures
+    ``_H``."""
     action: Callable[[torch.Tensor], torch.Tensor]
     rhs: torch.Tensor
     diagonal: torch.Tensor
     _op: object = None
     _H: torch.Tensor | None = None
     _scale: float = 1.0
+    _form: int = 0
 
     def __post_init__(self):
         if not callable(self.action):
@@ -269,15 +331,31 @@ class H8ScalarSPDSystem:
         return self.action(field)
 
 
-class PeriodicVoxelPhaseFieldOperator:
-    """Matrix-free mechanics and AT2 damage operators of one triply periodic voxel cell.
+PFCZM_C_ALPHA = math.pi          # physics/pfczm.py:6
+PFCZM_TANGENT_FLOOR = 0.25       # solid3d/operator.py:124
 
-    Mirrors solid3d/operator.py:133-1544 for the AT2 law without crack closure or
-    interface drivers.  x = [u~ (3 n), E (6)] with node-major u~ (``3 node + c``).
+
+def pfczm_softening_parameters(softening: str):
+    """``(a2, a3)`` of Wu's softening laws (physics/pfczm.py:55-63)."""
+    if softening == "linear":
+        return -0.5, 0.0
+    if softening == "exponential":
+        return 2.0 ** (5.0 / 3.0) - 3.0, 0.0
+    raise ValueError(f"PF-CZM softening must be one of ('linear', 'exponential'), got {softening!r}")
+
+
+class PeriodicVoxelPhaseFieldOperator:
+    """Matrix-free mechanics and damage operators of one triply periodic voxel cell.
+
+    Mirrors solid3d/operator.py:133-1544: the AT2 law or (``damage_law='pfczm'``) Wu's PF-CZM
+    law, with or without the isotropic crack closure (``crack_closure='axes'``); no interface
+    drivers.  x = [u~ (3 n), E (6)] with node-major u~ (``3 node + c``).
     """
 
     def __init__(self, grid: VoxelGrid, materials: ElementMaterialArrays, load: LoadCase, *, kappa: float = 1e-6,
-                 ell: float, pinned_node: int | None = None, device=None, tiles=(16, 14, 8)):
+                 ell: float, pinned_node: int | None = None, device=None, tiles=(16, 14, 8),
+                 damage_law: str = "at2", pfczm_softening: str = "linear", pfczm_p: float = 2,
+                 crack_closure: str = "none"):
         self.device = _lib.require_device() if device is None else torch.device(device)
         self.grid, self.mat, self.load = grid, materials, load
         self.dtype = F64
@@ -312,6 +390,71 @@ class PeriodicVoxelPhaseFieldOperator:
         check(_L.phast_hex8_solver_reserve(h), "phast_hex8_solver_reserve")  # solves then allocate nothing
         self.set_tiles(*tiles)
         self.Gc = torch.as_tensor(Gc, device=self.device)
+        if damage_law not in ("at2", "pfczm"):
+            raise ValueError(f"damage_law must be 'at2' or 'pfczm', got {damage_law!r}")
+        if crack_closure not in ("none", "axes"):
+            raise ValueError(f"crack_closure must be 'none' or 'axes', got {crack_closure!r}")
+        self.damage_law = damage_law
+        self._pfczm = damage_law == "pfczm"
+        self._closure = crack_closure == "axes"
+        self.a1 = None
+        self.pfczm_bound = None
+        if self._pfczm:
+            self._build_pfczm(materials, pfczm_softening, pfczm_p, drv)
+        if self._closure:
+            self._build_closure(materials, drv, lam)
+
+    # -- PF-CZM law and crack closure set-up (operator.py:300-383, 631-690) --------------
+    def _build_pfczm(self, materials, softening, p, drv):
+        """Per-voxel a1 = 4 E Gc/(pi ell f_t^2 (1 - kappa)) (physics/pfczm.py:17-32) and the
+        onset check a1 >= p + a2 on voxels with a damage driver (operator.py:300-383)."""
+        a2, a3 = pfczm_softening_parameters(softening)
+        if float(p) < 2.0:
+            raise ValueError(f"pfczm_p must be >= 2, got {p!r}")
+        self._pfczm_a2, self._pfczm_a3, self._pfczm_p = a2, a3, p
+        if materials.E is None or materials.f_t is None:
+            raise ValueError("damage_law 'pfczm' needs the element E and tensile strength arrays")
+        f_t = np.asarray(materials.f_t, dtype=np.float64)
+        E_ref = np.asarray(materials.E, dtype=np.float64)
+        missing = sorted({PHASE_NAMES[int(i)] for i in np.unique(np.asarray(materials.phase)[~np.isfinite(f_t)])})
+        if missing:
+            raise ValueError("damage_law 'pfczm' needs a tensile strength for every phase of the cell "
+                             f"(material.tensile_strength); missing: {', '.join(missing)}")
+        if not np.all(f_t > 0.0):
+            raise ValueError("PFCZM a1 requires f_t > 0")
+        Gc = np.asarray(materials.Gc, dtype=np.float64)
+        a1 = 4.0 * E_ref * Gc / (math.pi * self.ell * f_t ** 2 * max(1.0 - self.kappa, 1.0e-30))
+        bound = float(p) + a2
+        self.pfczm_bound = bound
+        active = drv != DRIVER_NONE
+        if active.any() and float(a1[active].min()) < bound:
+            raise ValueError(f"damage_law 'pfczm': the onset condition a1 >= p + a2 = {bound:g} (g''(0) >= 0) fails "
+                             f"at ell = {self.ell:g} mm: smallest a1 {float(a1[active].min()):.4g}")
+        a1 = np.ascontiguousarray(a1)
+        check(_L.phast_hex8_pfczm_setup(self._plan, a1.ctypes.data, a2, a3, float(p)), "pfczm_setup")
+        self.a1 = torch.as_tensor(a1, device=self.device)
+
+    def _build_closure(self, materials, drv, lam):
+        """Isotropic closure group: every voxel with a damage driver (operator.py:631-666; all
+        phases here are isotropic), lam >= 0 there."""
+        on = (drv != DRIVER_NONE).astype(np.uint8)
+        if on.any() and float(lam[on.astype(bool)].min()) < 0.0:
+            raise ValueError("crack_closure 'axes' needs lam >= 0 in every isotropic phase")
+        on = np.ascontiguousarray(on)
+        check(_L.phast_hex8_closure_setup(self._plan, on.ctypes.data), "closure_setup")
+        self.cl_on = torch.as_tensor(on.astype(bool), device=self.device)
+
+    @property
+    def is_pfczm(self) -> bool:
+        return self._pfczm
+
+    @property
+    def has_closure(self) -> bool:
+        return self._closure
+
+    @property
+    def has_interface_stress(self) -> bool:
+        return False
 
     def __del__(self):
         h = getattr(self, "_plan", None)
@@ -368,51 +511,137 @@ class PeriodicVoxelPhaseFieldOperator:
 
     # -- mechanics -----------------------------------------------------------------------
     def degradation(self, d: torch.Tensor) -> torch.Tensor:
-        """g (E,8) = (1-kappa)(1-d_q)^2 + kappa (operator.py:939-946)."""
+        """g (E,8) at d_q = d_e N^T (operator.py:939-946): (1-kappa)(1-d_q)^2 + kappa, or Wu's
+        rational PF-CZM degradation with the voxel's a1."""
         d = self._check("d", d, (self.n_red,))
         g = torch.empty((self.n_elements, 8), dtype=F64, device=self.device)
-        check(_L.phast_hex8_degradation(self._plan, ptr(d), ptr(g), stream()), "degradation")
+        if self._pfczm:
+            check(_L.phast_hex8_pfczm_degradation(self._plan, ptr(d), ptr(g), None, None, stream()), "degradation")
+        else:
+            check(_L.phast_hex8_degradation(self._plan, ptr(d), ptr(g), stream()), "degradation")
         return g
 
+    def _check_closure(self, closure):
+        """operator.py:735-753."""
+        if closure is None:
+            return None
+        if not self._closure:
+            raise ValueError("closure applies with crack_closure 'axes' only; this operator has none")
+        if not isinstance(closure, ClosurePattern):
+            raise TypeError("closure must be a ClosurePattern")
+        b = closure.bits
+        if b.dtype != torch.int32 or b.shape != (self.n_elements,) or b.device != self.device:
+            raise ValueError(f"closure must hold ({self.n_elements},) int32 words on {self.device}")
+        return closure
+
+    def _closure_mode(self, g, closure):
+        """0: no add-back (no closure, or undamaged g); 1: own pattern; 2: `closure`."""
+        closure = self._check_closure(closure)
+        if not self._closure or g is None:
+            return 0, None
+        return (1, None) if closure is None else (2, closure.bits.contiguous())
+
     def mech_action(self, x, g=None, *, opening=None, closure=None) -> torch.Tensor:
-        """A(g) x (operator.py:1171-1194); g=None means undamaged."""
-        if opening is not None or closure is not None:
-            raise NotImplementedError("interface openings / crack closure are not on this path")
+        """A(g) x (operator.py:1171-1194); g=None means undamaged.  With the crack closure the
+        action is that of the pattern ``closure`` (linear), or the internal force at each
+        point's own pattern when it is None."""
+        if opening is not None:
+            raise ValueError("opening applies to the variational band stress only; this cell has none")
         x = self._check("x", x, (self.n_mech,))
         g = self._check_qp("g", g)
+        mode, bits = self._closure_mode(g, closure)
         y = torch.empty_like(x)
-        check(_L.phast_hex8_mech_action(self._plan, ptr(x), ptr(g), ptr(y), stream()), "mech_action")
+        if mode:
+            check(_L.phast_hex8_mech_action_cl(self._plan, ptr(x), ptr(g), ptr(bits), mode, ptr(y), stream()),
+                  "mech_action")
+        else:
+            check(_L.phast_hex8_mech_action(self._plan, ptr(x), ptr(g), ptr(y), stream()), "mech_action")
         return y
 
     def mech_action_d(self, x, d) -> torch.Tensor:
-        """A(g(d)) x with g formed in-kernel from nodal damage (fused degradation)."""
+        """A(g(d)) x; for the AT2 law without closure g is formed in-kernel from nodal damage
+        (fused degradation)."""
         x = self._check("x", x, (self.n_mech,))
         d = self._check("d", d, (self.n_red,))
+        if self._pfczm or self._closure:
+            return self.mech_action(x, self.degradation(d))
         y = torch.empty_like(x)
         check(_L.phast_hex8_mech_action_d(self._plan, ptr(x), ptr(d), ptr(y), stream()), "mech_action_d")
         return y
 
     def mech_diagonal(self, g=None, *, opening=None, closure=None) -> torch.Tensor:
-        """Jacobi diagonal (operator.py:1196-1254)."""
+        """Jacobi diagonal (operator.py:1196-1300); with the crack closure every closure bit
+        (all of them when ``closure`` is None) adds its (1 - g) term."""
+        if opening is not None:
+            raise ValueError("opening applies to the variational band stress only; this cell has none")
         g = self._check_qp("g", g)
+        mode, bits = self._closure_mode(g, closure)
         out = torch.empty(self.n_mech, dtype=F64, device=self.device)
-        check(_L.phast_hex8_mech_diagonal(self._plan, ptr(g), ptr(out), stream()), "mech_diagonal")
+        if mode:
+            check(_L.phast_hex8_mech_diagonal_cl(self._plan, ptr(g), ptr(bits), mode, ptr(out), stream()),
+                  "mech_diagonal")
+        else:
+            check(_L.phast_hex8_mech_diagonal(self._plan, ptr(g), ptr(out), stream()), "mech_diagonal")
         return out
 
-    def _stress_sums(self, x, g):
+    def _stress_sums(self, x, g, closure=None):
         x = self._check("x", x, (self.n_mech,))
         g = self._check_qp("g", g)
+        mode, bits = self._closure_mode(g, closure)
         out = torch.empty(7, dtype=F64, device=self.device)
-        check(_L.phast_hex8_stress_sums(self._plan, ptr(x), ptr(g), ptr(out), stream()), "stress_sums")
+        if mode:
+            check(_L.phast_hex8_stress_sums_cl(self._plan, ptr(x), ptr(g), ptr(bits), mode, ptr(out), stream()),
+                  "stress_sums")
+        else:
+            check(_L.phast_hex8_stress_sums(self._plan, ptr(x), ptr(g), ptr(out), stream()), "stress_sums")
         return out
 
     def macro_stress(self, x, g=None, *, opening=None, closure=None) -> torch.Tensor:
         """Sigma = r_E / V (operator.py:1303-1316)."""
-        return self._stress_sums(x, g)[:6] / self.volume
+        return self._stress_sums(x, g, closure)[:6] / self.volume
 
     def elastic_energy(self, x, g=None, *, opening=None, closure=None) -> torch.Tensor:
-        """0.5 x.A(g) x (operator.py:1318-1340)."""
-        return self._stress_sums(x, g)[6]
+        """0.5 x.A(g) x (operator.py:1318-1340); with the closure the degraded energy at the
+        pattern of x (or ``closure``)."""
+        return self._stress_sums(x, g, closure)[6]
+
+    def hill_mandel(self, x, g=None, *, closure=None) -> dict:
+        """<sigma.eps> against Sigma.E (operator.py:1390-1413)."""
+        sums = self._stress_sums(x, g, closure)
+        E = self._check("x", x, (self.n_mech,))[3 * self.n_red:]
+        mean_work = float(2.0 * sums[6] / self.volume)
+        macro_work = float(torch.dot(sums[:6] / self.volume, E))
+        scale = max(abs(mean_work), abs(macro_work))
+        return {"mean_work": mean_work, "macro_work": macro_work,
+                "relative": abs(mean_work - macro_work) / scale if scale > 0.0 else 0.0}
+
+    # -- crack closure patterns (operator.py:786-841) --------------------------------------
+    def closure_pattern(self, x) -> ClosurePattern:
+        """Sign pattern v . eps < 0 of every closure term at state x (operator.py:792-806)."""
+        if not self._closure:
+            raise ValueError("closure_pattern needs crack_closure 'axes'")
+        x = self._check("x", x, (self.n_mech,))
+        bits = torch.empty(self.n_elements, dtype=torch.int32, device=self.device)
+        check(_L.phast_hex8_closure_pattern(self._plan, ptr(x), None, None, ptr(bits), None, stream()),
+              "closure_pattern")
+        return ClosurePattern(bits)
+
+    def closure_pattern_change(self, x, pattern: ClosurePattern, g, *, tolerance: float = 1.0e-8):
+        """New pattern of x and whether a flipped bit moves the stress by more than
+        ``tolerance`` times the group's largest k |v . eps| (operator.py:808-841)."""
+        pattern = self._check_closure(pattern)
+        if pattern is None:
+            raise ValueError("closure_pattern_change needs a pattern")
+        x = self._check("x", x, (self.n_mech,))
+        g = self._check_qp("g", g)
+        if g is None:
+            raise ValueError("closure_pattern_change needs g")
+        bits = torch.empty(self.n_elements, dtype=torch.int32, device=self.device)
+        out = torch.empty(2, dtype=F64, device=self.device)
+        check(_L.phast_hex8_closure_pattern(self._plan, ptr(x), ptr(g), ptr(pattern.bits.contiguous()), ptr(bits),
+                                            ptr(out), stream()), "closure_pattern_change")
+        strength, jump = (float(v) for v in out.cpu())
+        return ClosurePattern(bits), bool(jump >= 0.0 and jump > tolerance * strength)
 
     def driving_force(self, x) -> torch.Tensor:
         """Undamaged driving energy Y (E,8) (operator.py:1415-1444)."""
@@ -456,6 +685,85 @@ class PeriodicVoxelPhaseFieldOperator:
         return H8ScalarSPDSystem(action=lambda f: self._damage_action(f, H, scale),
                                  rhs=self.damage_rhs(H, scale), diagonal=self.damage_diagonal(H, scale),
                                  _op=self, _H=H, _scale=scale)
+
+    # -- PF-CZM damage (operator.py:461-631) ----------------------------------------------
+    def _require_pfczm(self):
+        if not self._pfczm:
+            raise ValueError("the PF-CZM damage operators need damage_law 'pfczm'")
+
+    def _pfczm_parts(self, d, H):
+        self._require_pfczm()
+        d = self._check("d", d, (self.n_red,))
+        H = self._check_qp("H", H)
+        out = torch.empty(3, dtype=F64, device=self.device)
+        check(_L.phast_hex8_pfczm_energy(self._plan, ptr(d), ptr(H), ptr(out), stream()), "pfczm_energy")
+        return out
+
+    def pfczm_damage_energy(self, d, H) -> torch.Tensor:
+        """sum_e [sum_q w H_q g(d_q) + Gc/(pi ell)(2 m.d_e - d_e.M d_e) + Gc ell/pi d_e.L d_e]
+        in mJ, a 0-dim tensor (operator.py:461-488); H None drops the history term."""
+        return self._pfczm_parts(d, H).sum()
+
+    def pfczm_damage_gradient(self, d, H, scale: float = 1.0) -> torch.Tensor:
+        """Gradient (n_red,) of :meth:`pfczm_damage_energy` times ``scale`` (operator.py:490-512)."""
+        self._require_pfczm()
+        d = self._check("d", d, (self.n_red,))
+        H = self._check_qp("H", H)
+        out = torch.empty_like(d)
+        check(_L.phast_hex8_pfczm_gradient(self._plan, ptr(d), ptr(H), float(scale), ptr(out), stream()),
+              "pfczm_gradient")
+        return out
+
+    def pfczm_tangent_weights(self, d, H) -> torch.Tensor:
+        """Gauss-point curvatures max(H g''(d_q) - 2 Gc/(pi ell), 0.25 * 2 Gc/(pi ell)) (E,8)
+        (operator.py:514-538)."""
+        self._require_pfczm()
+        d = self._check("d", d, (self.n_red,))
+        H = self._check_qp("H", H)
+        W = torch.empty((self.n_elements, 8), dtype=F64, device=self.device)
+        check(_L.phast_hex8_pfczm_tangent_weights(self._plan, ptr(d), ptr(H), ptr(W), stream()), "pfczm_weights")
+        return W
+
+    def pfczm_tangent_action(self, field, weights, scale: float = 1.0) -> torch.Tensor:
+        """scale (Gc (2 ell/pi) L f_e + sum_q w c_q N_q N_q^T f_e) (operator.py:540-552)."""
+        self._require_pfczm()
+        field = self._check("field", field, (self.n_red,))
+        weights = self._check("weights", weights, (self.n_elements, 8))
+        y = torch.empty_like(field)
+        check(_L.phast_hex8_pfczm_tangent_action(self._plan, ptr(field), ptr(weights), float(scale), ptr(y),
+                                                 stream()), "pfczm_tangent_action")
+        return y
+
+    def pfczm_tangent_diagonal(self, weights, scale: float = 1.0) -> torch.Tensor:
+        """Jacobi diagonal of :meth:`pfczm_tangent_action` (operator.py:554-564)."""
+        self._require_pfczm()
+        weights = self._check("weights", weights, (self.n_elements, 8))
+        out = torch.empty(self.n_red, dtype=F64, device=self.device)
+        check(_L.phast_hex8_pfczm_tangent_diagonal(self._plan, ptr(weights), float(scale), ptr(out), stream()),
+              "pfczm_tangent_diagonal")
+        return out
+
+    def pfczm_newton_system(self, d, H, *, scaled: bool = True, correction: bool = False):
+        """SPD Newton subproblem at d and the gradient there, both times ``damage_scale`` when
+        scaled (operator.py:566-609).  rhs = K d - gradient (``correction``: -gradient, the
+        unknown being the step)."""
+        scale = self.damage_scale if scaled else 1.0
+        W = self.pfczm_tangent_weights(d, H)
+        gradient = self.pfczm_damage_gradient(d, H, scale)
+        action = lambda f: self.pfczm_tangent_action(f, W, scale)   # noqa: E731
+        diagonal = self.pfczm_tangent_diagonal(W, scale)
+        rhs = -gradient if correction else action(d) - gradient
+        return H8ScalarSPDSystem(action=action, rhs=rhs, diagonal=diagonal, _op=self, _H=W, _scale=scale,
+                                 _form=1), gradient
+
+    def fracture_energy(self, d) -> dict:
+        """Phase-field surface functional in mJ (operator.py:1511-1544; PF-CZM: 611-631):
+        ``total``, ``local``, ``gradient``."""
+        d = self._check("d", d, (self.n_red,))
+        if self._pfczm:
+            local, grad, _ = (float(v) for v in self._pfczm_parts(d, None).cpu())
+            return {"total": local + grad, "local": local, "gradient": grad}
+        return {"total": 0.5 * float(torch.dot(d, self.damage_action(d)))}
 
     # -- adjoint pieces (transposed derivatives of the residuals) --------------------------
     def set_gc(self, Gc: torch.Tensor) -> None:
@@ -551,10 +859,16 @@ class PeriodicVoxelPhaseFieldOperator:
 
     # -- device-resident solves ----------------------------------------------------------
     def solve_mechanics(self, *, g=None, d=None, fixed_mask, fixed_values, diag, rhs=None, x0=None,
-                        rtol=1e-8, atol=0.0, maxiter=5000, check_every=10):
-        """solve_constrained(mech_action(., g), ...) on the device (linalg.py:223-287)."""
+                        rtol=1e-8, atol=0.0, maxiter=5000, check_every=10, closure=None):
+        """solve_constrained(mech_action(., g), ...) on the device (linalg.py:223-287); with
+        ``closure`` (a ClosurePattern, g required) the action of that fixed pattern."""
         if g is not None and d is not None:
             raise ValueError("pass g or d, not both")
+        if d is not None and (self._pfczm or self._closure):
+            g, d = self.degradation(d), None
+        mode, bits = self._closure_mode(g, closure)
+        if self._closure and g is not None and mode != 2:
+            raise ValueError("a mechanics solve with the crack closure needs a fixed pattern (closure=...)")
         g = self._check_qp("g", g)
         d = None if d is None else self._check("d", d, (self.n_red,))
         fm = self._check_mask(fixed_mask, self.n_mech)
@@ -566,9 +880,14 @@ class PeriodicVoxelPhaseFieldOperator:
             raise ValueError("rtol and atol must be non-negative and not both zero")
         x = torch.empty(self.n_mech, dtype=F64, device=self.device)
         info = (C.c_double * 5)()
-        rc = _L.phast_hex8_mech_solve(self._plan, ptr(g), ptr(d), ptr(fm), ptr(fv), ptr(rhs), ptr(x0), ptr(dg),
-                                      float(rtol), float(atol), int(maxiter), int(check_every), self.n_mech, ptr(x),
-                                      info, stream())
+        if mode == 2:
+            rc = _L.phast_hex8_mech_solve_cl(self._plan, ptr(g), ptr(bits), ptr(fm), ptr(fv), ptr(rhs), ptr(x0),
+                                             ptr(dg), float(rtol), float(atol), int(maxiter), int(check_every),
+                                             self.n_mech, ptr(x), info, stream())
+        else:
+            rc = _L.phast_hex8_mech_solve(self._plan, ptr(g), ptr(d), ptr(fm), ptr(fv), ptr(rhs), ptr(x0), ptr(dg),
+                                          float(rtol), float(atol), int(maxiter), int(check_every), self.n_mech,
+                                          ptr(x), info, stream())
         _raise_pcg(rc)
         return x, PCGInfo(int(info[0]), bool(info[1]), info[2], info[3], info[4])
 
@@ -665,9 +984,10 @@ def solve_constrained(action, *, diag, fixed_mask, fixed_values, rhs=None, x0=No
     other callable goes through :func:`pcg` (library loop, Python apply)."""
     bound = getattr(action, "_phast_mech", None)
     if bound is not None:
-        op, g, d = bound
+        op, g, d, closure = bound
         return op.solve_mechanics(g=g, d=d, fixed_mask=fixed_mask, fixed_values=fixed_values, diag=diag, rhs=rhs,
-                                  x0=x0, rtol=rtol, atol=atol, maxiter=maxiter, check_every=check_every)
+                                  x0=x0, rtol=rtol, atol=atol, maxiter=maxiter, check_every=check_every,
+                                  closure=closure)
     zero = torch.zeros((), dtype=fixed_values.dtype, device=fixed_values.device)
     prescribed = torch.where(fixed_mask, fixed_values, zero)   # known part of the solution
     lifted = -action(prescribed) if rhs is None else rhs - action(prescribed)
@@ -678,10 +998,14 @@ def solve_constrained(action, *, diag, fixed_mask, fixed_values, rhs=None, x0=No
     return torch.where(fixed_mask, prescribed, y), info
 
 
-def mech_action_of(op: PeriodicVoxelPhaseFieldOperator, g=None, d=None):
-    """``lambda v: op.mech_action(v, g)`` tagged so that solve_constrained runs on the device."""
-    fn = (lambda v: op.mech_action(v, g)) if d is None else (lambda v: op.mech_action_d(v, d))
-    fn._phast_mech = (op, g, d)
+def mech_action_of(op: PeriodicVoxelPhaseFieldOperator, g=None, d=None, closure=None):
+    """``lambda v: op.mech_action(v, g, closure=closure)`` tagged so that solve_constrained
+    runs on the device."""
+    if d is None:
+        fn = lambda v: op.mech_action(v, g, closure=closure)   # noqa: E731
+    else:
+        fn = lambda v: op.mech_action_d(v, d)   # noqa: E731
+    fn._phast_mech = (op, g, d, closure)
     return fn
 
 
@@ -781,9 +1105,9 @@ def solve_h8_scalar_dirichlet_pcg(system: H8ScalarSPDSystem, *, fixed, values, i
         raise ValueError(f"tolerance must be finite and positive, got {tolerance!r}")
     out = torch.empty(n, dtype=F64, device=op.device)
     info = (C.c_double * 5)()
-    rc = _L.phast_hex8_damage_dirichlet_pcg(op._plan, ptr(system._H), float(system._scale), ptr(system.rhs),
-                                            ptr(system.diagonal), ptr(fx), ptr(values), ptr(init), float(tolerance),
-                                            int(max_iterations), n, ptr(out), info, stream())
+    fn = _L.phast_hex8_pfczm_dirichlet_pcg if system._form == 1 else _L.phast_hex8_damage_dirichlet_pcg
+    rc = fn(op._plan, ptr(system._H), float(system._scale), ptr(system.rhs), ptr(system.diagonal), ptr(fx),
+            ptr(values), ptr(init), float(tolerance), int(max_iterations), n, ptr(out), info, stream())
     _raise_scalar(rc)
     krylov = ScalarPCGResult(out, int(info[0]), bool(info[1]), info[2], info[3], info[4])
     return ScalarDirichletResult(out, krylov, n_free)
@@ -853,9 +1177,10 @@ def solve_h8_scalar_box_kkt(system: H8ScalarSPDSystem, *, lower_bound=0.0, upper
         n_la, n_ua, n_pr = (int(v) for v in torch.stack([lact.sum(), uact.sum(), presc.sum()]).cpu())
         apply = 1 if it < max_active_set_iterations else 0
         la8, ua8 = lact.view(torch.uint8).clone(), uact.view(torch.uint8).clone()
-        rc = _L.phast_hex8_box_update(op._plan, ptr(system._H), float(system._scale), ptr(system.rhs), ptr(cand),
-                                      ptr(lower), ptr(upper), ptr(la8), ptr(ua8), ptr(presc.view(torch.uint8)),
-                                      float(kkt_tolerance), apply, n, ptr(grad), counts, stream())
+        upd = _L.phast_hex8_pfczm_box_update if system._form == 1 else _L.phast_hex8_box_update
+        rc = upd(op._plan, ptr(system._H), float(system._scale), ptr(system.rhs), ptr(cand), ptr(lower), ptr(upper),
+                 ptr(la8), ptr(ua8), ptr(presc.view(torch.uint8)), float(kkt_tolerance), apply, n, ptr(grad), counts,
+                 stream())
         check(rc, "box_update")
         add_l, add_u, rel_l, rel_u, n_free = (int(c) for c in counts)
         trace.append(BoxKKTIteration(it, n_la, n_ua, n_pr, n_free, last.krylov.iterations, last.krylov.converged,

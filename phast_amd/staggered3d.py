@@ -2,10 +2,12 @@
 
 Mirrors ``solid3d/staggered.py`` (StaggeredSolver: ``_attempt`` 1259-1517,
 ``_damage`` 753-802, ``_change`` 1097-1121, ``_predictor`` 1207-1213, ``advance`` 1215-1257,
-``advance_increment`` 1523-1582) for the AT2 law without crack closure, interface
-stress or Anderson mixing.  The mechanics solve is the device-resident constrained PCG
-with the degradation formed in-kernel from the nodal damage; the damage solve is the
-device box-KKT active set.
+``advance_increment`` 1523-1582) without interface stress or Anderson mixing, for the AT2
+law and for the PF-CZM law (``_damage_pfczm`` 843-1096, the projected Newton solve), with or
+without the crack closure (``_closure_mechanics`` 698-751 with ``_energy_line_search``
+589-626).  The linear mechanics solves are the device-resident constrained PCG (for AT2
+without closure with the degradation formed in-kernel from the nodal damage); the damage
+subproblems are the device box-KKT active set.
 """
 from __future__ import annotations
 
@@ -21,6 +23,17 @@ from .solid3d import (F64, H8ScalarBoxKKTConvergenceError, LoadCase, PCGBreakdow
                       PeriodicVoxelPhaseFieldOperator, solve_h8_scalar_box_kkt)
 
 _MIN_LINEAR_TOLERANCE = 4.0 * float(np.finfo(np.float64).eps)  # staggered.py:97
+_TINY = float(np.finfo(np.float64).tiny)                        # staggered.py:96
+INTERFACE_MAX_SOLVES = 50           # staggered.py:99
+CLOSURE_PATTERN_TOLERANCE = 1.0e-8  # staggered.py:105
+LINE_SEARCH_ARMIJO = 1.0e-4         # staggered.py:109
+LINE_SEARCH_ENERGY_RTOL = 1.0e-12   # staggered.py:112
+LINE_SEARCH_MAX_HALVINGS = 30       # staggered.py:115
+PFCZM_ARMIJO = 1.0e-4               # staggered.py:118
+PFCZM_ENERGY_RTOL = 1.0e-12         # staggered.py:121
+PFCZM_MAX_HALVINGS = 30             # staggered.py:124
+PFCZM_MAX_DOUBLINGS = 6             # staggered.py:127
+PFCZM_INNER_KKT_FRACTION = 0.1      # staggered.py:130
 
 
 @dataclass(frozen=True)
@@ -41,6 +54,7 @@ class SolverConfig:
     final_mechanics_resolve: bool = True
     damaged_threshold: float = 0.95
     preconditioner: str = "jacobi"
+    pfczm_max_newton: int = 100
 
 
 @dataclass(eq=False)
@@ -90,6 +104,8 @@ class IncrementRecord:
     mech_s: float = 0.0
     damage_s: float = 0.0
     history_s: float = 0.0
+    newton_iters_pf: int | None = None   # PF-CZM: projected Newton iterations of the increment
+    closure_solves: int | None = None    # crack closure: linear mechanics solves of the increment
 
 
 class StaggerConvergenceError(RuntimeError):
@@ -100,7 +116,11 @@ class StaggerConvergenceError(RuntimeError):
 
 
 class _AttemptFailure(RuntimeError):
-    pass
+    """staggered.py:273-285: a failed attempt and the iterations its solves had spent."""
+
+    def __init__(self, message, *, pcg_mech: int = 0, pcg_pf: int = 0, active_set: int = 0):
+        super().__init__(message)
+        self.pcg_mech, self.pcg_pf, self.active_set = int(pcg_mech), int(pcg_pf), int(active_set)
 
 
 class StaggeredSolver:
@@ -137,9 +157,81 @@ class StaggeredSolver:
         x[self._n_u + self._controlled] = target
         return x
 
-    def _mechanics(self, d, values, x0):
-        """staggered.py:550-587 (linear mechanics at fixed damage)."""
+    def _linear_mechanics(self, g, values, x0, closure=None):
+        """staggered.py:550-587: one constrained PCG solve at fixed g (and closure pattern)."""
         op, cfg = self.op, self.cfg
+        diag = op.mech_diagonal(g, closure=closure)
+        try:
+            x, info = op.solve_mechanics(g=g, closure=closure, fixed_mask=self._mask, fixed_values=values, diag=diag,
+                                         x0=x0, rtol=cfg.pcg_rtol_mech, maxiter=cfg.pcg_maxit_mech,
+                                         check_every=cfg.check_every)
+        except PCGBreakdownError as exc:
+            raise _AttemptFailure(f"mechanics PCG breakdown: {exc}") from exc
+        if not info.converged:
+            raise _AttemptFailure(f"mechanics PCG did not converge in {cfg.pcg_maxit_mech} iterations "
+                                  f"(relative residual {info.relative_residual:.3e} > {cfg.pcg_rtol_mech:.3e})",
+                                  pcg_mech=info.iterations)
+        return x, info
+
+    def _energy_line_search(self, x, y, g):
+        """staggered.py:589-626: halve the step from x towards the pattern solution y until the
+        degraded energy (each point at its own pattern) passes the Armijo test with the
+        round-off allowance; alpha = 1 returns y itself."""
+        op = self.op
+        p = y - x
+        energy0 = float(op.elastic_energy(x, g))
+        force = op.mech_action(x, g)
+        free = ~self._mask
+        slope = min(float(torch.dot(force[free], p[free])), 0.0)
+        allowance = LINE_SEARCH_ENERGY_RTOL * abs(energy0)
+        alpha = 1.0
+        for _ in range(LINE_SEARCH_MAX_HALVINGS + 1):
+            trial = y if alpha == 1.0 else x + alpha * p
+            energy = float(op.elastic_energy(trial, g))
+            if not math.isfinite(energy):
+                raise _AttemptFailure("mechanics energy is not finite in the line search")
+            if energy <= energy0 + LINE_SEARCH_ARMIJO * alpha * slope + allowance:
+                return trial, alpha
+            alpha *= 0.5
+        raise _AttemptFailure(f"the energy line search of the closure mechanics solve found no decrease in "
+                              f"{LINE_SEARCH_MAX_HALVINGS} halvings")
+
+    def _closure_mechanics(self, g, values, x0):
+        """staggered.py:698-751: linear solves at the fixed closure pattern, each followed by
+        the energy line search, until no bit flips materially (CLOSURE_PATTERN_TOLERANCE)
+        after a full step."""
+        op = self.op
+        closure = op.closure_pattern(x0)
+        iterations = damped = 0
+        for solve in range(1, INTERFACE_MAX_SOLVES + 1):
+            try:
+                y, info = self._linear_mechanics(g, values, x0, closure)
+                iterations += info.iterations
+                x, alpha = self._energy_line_search(x0, y, g)
+                damped += int(alpha < 1.0)
+            except _AttemptFailure as exc:
+                exc.pcg_mech += iterations
+                raise
+            closure, changed = op.closure_pattern_change(x, closure, g, tolerance=CLOSURE_PATTERN_TOLERANCE)
+            if not changed and alpha == 1.0:
+                self.last_interface_solves = solve
+                self.last_line_search_steps = damped
+                return x, dataclasses.replace(info, iterations=iterations)
+            x0 = x
+        raise _AttemptFailure(f"the closure pattern (crack_closure 'axes') did not settle in {INTERFACE_MAX_SOLVES} "
+                              f"mechanics solves", pcg_mech=iterations)
+
+    def _mechanics(self, d, values, x0):
+        """staggered.py:628-696 (mechanics at fixed damage): one linear solve, or the closure
+        pattern iteration."""
+        op, cfg = self.op, self.cfg
+        if op.has_closure or op.is_pfczm:
+            g = op.degradation(d)
+            if op.has_closure:
+                x, info = self._closure_mechanics(g, values, x0)
+            else:
+                x, info = self._linear_mechanics(g, values, x0)
+            return x, info, g
         g = op.degradation(d)
         diag = op.mech_diagonal(g)
         try:
@@ -160,6 +252,8 @@ class StaggeredSolver:
 
     def _damage(self, H, lower, initial):
         """staggered.py:753-802 (box_kkt irreversibility)."""
+        if self.op.is_pfczm:
+            return self._damage_pfczm(H, lower, initial)
         cfg = self.cfg
         system = self.op.damage_system(H)
         r0 = float(torch.linalg.vector_norm(system.rhs - system.apply(initial)))
@@ -178,6 +272,97 @@ class StaggeredSolver:
             raise _AttemptFailure(f"damage PCG breakdown: {exc}") from exc
         self._last_active = (res.lower_active, res.upper_active)
         return res.solution, res.total_pcg_iterations, res.active_set_iterations, res.kkt.satisfies_kkt
+
+    def _damage_pfczm(self, H, lower, initial):
+        """staggered.py:843-1096 (absolute-field QP): projected Newton on the PF-CZM damage
+        energy over lower <= d <= 1.  Each iteration: stop when the scaled projected gradient
+        is at most damage_kkt_tol in the maximum norm; else solve the box-QP of the Newton
+        tangent holding the bound nodes whose gradient points outward, take the Armijo step
+        with halving, and after a full step that is not yet converged double it along the
+        projected path while the energy decreases."""
+        op, cfg = self.op, self.cfg
+        tol = cfg.damage_kkt_tol
+        inner_tol = PFCZM_INNER_KKT_FRACTION * tol
+        scale = op.damage_scale
+
+        def projected_residual(field):
+            grad = op.pfczm_damage_gradient(field, H, scale)
+            pg = torch.where(field <= lower, grad.clamp(max=0.0), grad)
+            pg = torch.where(field >= 1.0, pg.clamp(min=0.0), pg)
+            return float(pg.abs().max())
+
+        d = initial
+        energy = float(op.pfczm_damage_energy(d, H))
+        if not math.isfinite(energy):
+            raise _AttemptFailure("damage PF-CZM energy is not finite")
+        pcg = sets = 0
+        ones = torch.ones_like(d)
+        residual = math.inf
+        iteration = 0
+        for iteration in range(cfg.pfczm_max_newton + 1):
+            system, gradient = op.pfczm_newton_system(d, H)
+            at_lower = d <= lower
+            at_upper = d >= 1.0
+            projected = torch.where(at_lower, gradient.clamp(max=0.0), gradient)
+            projected = torch.where(at_upper, projected.clamp(min=0.0), projected)
+            residual = float(projected.abs().max())
+            if not math.isfinite(residual):
+                raise _AttemptFailure("damage PF-CZM gradient is not finite", pcg_pf=pcg, active_set=sets)
+            if residual <= tol:
+                self.last_newton_iters = iteration
+                self._newton_count += iteration
+                self._last_active = None
+                return d, pcg, sets, True
+            if iteration == cfg.pfczm_max_newton:
+                break
+            hold_lower = at_lower & (gradient > tol)
+            hold_upper = at_upper & (gradient < -tol)
+            upper_k = torch.where(hold_lower, lower, ones)
+            lower_k = torch.where(hold_upper, ones, lower)
+            r0 = float(torch.linalg.vector_norm(gradient))
+            tolerance = max(min(cfg.damage_linear_tol, 0.1 * inner_tol / max(r0, 1.0)), _MIN_LINEAR_TOLERANCE)
+            try:
+                result = solve_h8_scalar_box_kkt(system, lower_bound=lower_k, upper_bound=upper_k, initial=d,
+                                                 linear_tolerance=tolerance, kkt_tolerance=inner_tol,
+                                                 max_linear_iterations=cfg.damage_maxit,
+                                                 max_active_set_iterations=cfg.damage_max_active_set)
+            except H8ScalarBoxKKTConvergenceError as exc:
+                raise _AttemptFailure(f"damage PF-CZM Newton subproblem failed at Newton iteration "
+                                      f"{iteration + 1}: {exc}", pcg_pf=pcg, active_set=sets) from exc
+            except (RuntimeError, ValueError) as exc:
+                raise _AttemptFailure(f"damage PF-CZM Newton subproblem PCG breakdown: {exc}") from exc
+            pcg += int(result.total_pcg_iterations)
+            sets += int(result.active_set_iterations)
+            target = result.solution
+            step = target - d
+            slope = float((gradient * step).sum()) / scale
+            t = 1.0
+            for _ in range(PFCZM_MAX_HALVINGS + 1):
+                trial = target if t == 1.0 else torch.minimum(torch.maximum(d + t * step, lower), ones)
+                trial_energy = float(op.pfczm_damage_energy(trial, H))
+                if math.isfinite(trial_energy) and trial_energy <= (
+                        energy + PFCZM_ARMIJO * t * slope + PFCZM_ENERGY_RTOL * abs(energy)):
+                    break
+                t *= 0.5
+            else:
+                raise _AttemptFailure(f"damage PF-CZM line search failed at Newton iteration {iteration + 1} "
+                                      f"(no decrease after {PFCZM_MAX_HALVINGS} halvings)", pcg_pf=pcg,
+                                      active_set=sets)
+            if t == 1.0 and not projected_residual(trial) <= tol:
+                for _ in range(PFCZM_MAX_DOUBLINGS):
+                    t *= 2.0
+                    further = torch.minimum(torch.maximum(d + t * step, lower), ones)
+                    if torch.equal(further, trial):
+                        break
+                    further_energy = float(op.pfczm_damage_energy(further, H))
+                    if not (math.isfinite(further_energy) and further_energy < trial_energy):
+                        break
+                    trial, trial_energy = further, further_energy
+            d = trial
+            energy = trial_energy
+        raise _AttemptFailure(f"damage PF-CZM Newton did not converge in {cfg.pfczm_max_newton} iterations "
+                              f"(projected gradient {residual:.3e} > damage_kkt_tol {tol:.3e})", pcg_pf=pcg,
+                              active_set=sets)
 
     def _change(self, x_new, x_old, d_new, d_old, step_size):
         """staggered.py:1097-1121."""
@@ -201,8 +386,9 @@ class StaggeredSolver:
                                       state.H)
 
     def fracture_energy(self, d) -> float:
-        """Gamma(d) = 1/2 sum_e Gc d_e.(M/ell + ell L) d_e (operator.py:1511-1544, total)."""
-        return 0.5 * float(torch.dot(d, self.op.damage_action(d)))
+        """Gamma(d) (operator.py:1511-1544, total): 1/2 sum_e Gc d_e.(M/ell + ell L) d_e for AT2,
+        the PF-CZM surface functional for PF-CZM (operator.py:611-631)."""
+        return self.op.fracture_energy(d)["total"]
 
     # -- one attempt -----------------------------------------------------------------
     def _attempt(self, state, target, previous, increment, cutbacks, start):
@@ -215,11 +401,15 @@ class StaggeredSolver:
         mech_s = damage_s = history_s = 0.0
         pcg_mech = pcg_pf = active_set = 0
         converged, change, H, info = False, math.inf, state.H, None
+        closure_solves = 0
+        self._newton_count = 0
         k = 0
         for k in range(1, cfg.max_stagger + 1):
             t0 = perf_counter()
             x_new, info, g = self._mechanics(d_old, values, x_old)
             pcg_mech += info.iterations
+            if op.has_closure:
+                closure_solves += self.last_interface_solves
             t1 = perf_counter()
             Y = op.driving_force(x_new)
             H = torch.maximum(state.H, Y)
@@ -245,6 +435,8 @@ class StaggeredSolver:
         if cfg.final_mechanics_resolve:
             x, info, g = self._mechanics(d, values, x_old)
             pcg_mech += info.iterations
+            if op.has_closure:
+                closure_solves += self.last_interface_solves
         else:
             x, g = x_old, op.degradation(d)
         mech_s += perf_counter() - t0
@@ -275,12 +467,14 @@ class StaggeredSolver:
                               macro_stress=macro_stress, reaction_force=macro_stress[self._load_axis] * fa * fb,
                               applied_disp=macro_strain[self._load_axis] * self._load_length, elastic=elastic,
                               fracture=fracture, external=external,
-                              balance_residual_rel=residual / max(abs(external), 1e-300),
+                              balance_residual_rel=residual / max(abs(external), _TINY),
                               max_damage=float(stats[0]), mean_damage=float(stats[1]), max_H=float(stats[2]),
                               stagger_iters=k, pcg_iters_mech=pcg_mech, pcg_iters_pf=pcg_pf,
                               active_set_iters=active_set, cutbacks=cutbacks, stagger_change=change,
                               hill_mandel_rel=hill_rel, wall_s=perf_counter() - start, mech_s=mech_s,
-                              damage_s=damage_s, history_s=history_s)
+                              damage_s=damage_s, history_s=history_s,
+                              newton_iters_pf=self._newton_count if op.is_pfczm else None,
+                              closure_solves=closure_solves if op.has_closure else None)
         return new_state, rec
 
     def advance(self, state, target, *, increment=None, cutbacks=0):

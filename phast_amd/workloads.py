@@ -135,6 +135,25 @@ def porous_gc_cube(n: int = 96, n_pores: int = 20, r_range=(4.0, 9.0), seed: int
 # batched plates
 # ----------------------------------------------------------------------------------------
 
+def pore_in_cube_pfczm(n: int = 48, increments: int = 150, to: float = 3.0e-3):
+    """The reference's PF-CZM pore example (examples/three_dimensional/pore_in_cube/run.py:19-27):
+    a periodic cube of n^3 voxels of 0.5 um with a centred spherical pore of diameter 0.4 L,
+    matrix E = 345 GPa, nu = 0.2, tensile strength 600 MPa, Gc = 25 J/m^2; pore stiffness
+    E x 1e-6 (solid3d/api.py:34, 167-172); PF-CZM (linear softening, p = 2) with ell = 1.5 um;
+    E_xx raised to ``to`` in ``increments`` equal steps, the other macro stresses zero.
+    Returns (phase (n,n,n) uint8, phases, h, ell, load case)."""
+    from phast_amd import geometry as G
+    from phast_amd import solid3d as S
+    h = (0.5e-3,) * 3
+    phase = G.sphere_pore((n, n, n), h, 0.4)
+    E, nu, f_t, Gc = 345e3, 0.2, 600.0, 25e-3
+    phases = {"mi_matrix": S.PhaseMaterial("mi_matrix", E, nu, Gc, tensile_strength=f_t),
+              "pore": S.PhaseMaterial("pore", E * 1.0e-6, nu, Gc, tensile_strength=f_t)}
+    load = S.LoadCase.from_schedule("macro_xx", controlled=0, schedule=[(to, to / increments)], free=(1, 2, 3, 4, 5),
+                                    fixed_zero=())
+    return phase, phases, h, 1.5e-3, load
+
+
 def lognormal_field(rng, ny, nx, mean, cov, corr):
     """Lognormal per-cell field of the given mean and coefficient of variation whose log is a
     Gaussian random field with a Gaussian correlation of ``corr`` cells (white noise filtered

@@ -64,3 +64,35 @@ def test_t3_plan_validates_connectivity_before_any_device_call():
         rc = L.phast_t3_plan_create(3, 1, nodes.ctypes.data, els.ctypes.data, 1.0, 0.2, 1.0, 1.0, 1e-7, 0, split,
                                     split, 0, 0.0, ctypes.byref(h))
         assert rc == _lib.PHAST_ERR_ARG and not h.value
+
+
+def _declared_arity():
+    """Parameter count of every ``int phast_*(...)`` declaration in include/*.h."""
+    out = {}
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:int|const char \*)\s*(phast_\w+)\s*\(([^;]*?)\)\s*;", text, flags=re.M | re.S):
+            params = m.group(2).strip()
+            out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_bound_argtypes_match_the_header():
+    """Every ctypes binding the host modules set up passes as many arguments as the C
+    declaration takes (a wrong count only shows at the first call, on the GPU)."""
+    from phast_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built (run python -m phast_amd.build)")
+    import phast_amd.solid3d, phast_amd.q4, phast_amd.t3, phast_amd.batch, phast_amd.adjoint3d  # noqa: F401,E401
+    arity = _declared_arity()
+    lib = _lib.lib()
+    checked, wrong = 0, []
+    for name, n in arity.items():
+        fn = getattr(lib, name, None)
+        if fn is None or fn.argtypes is None:
+            continue
+        checked += 1
+        if len(fn.argtypes) != n:
+            wrong.append((name, len(fn.argtypes), n))
+    assert not wrong, wrong
+    assert checked >= 60

@@ -1,0 +1,167 @@
+"""Parity of the PF-CZM damage law and the crack closure on the voxel cell with the oracle
+(GPU): the operators at 1e-12, patterns bit for bit, and the staggered increments of a
+pore cell through damage onset, peak and softening."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hex8 as O
+from oracle import pfczm as P
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+T = lambda x, dev: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
+
+
+def build(n=None, shape=None, tiles=(16, 14, 8), closure=True, load=None, pore_voxels=None):
+    """The pore-in-cube PF-CZM material on an n^3 cell (or `shape` with chosen pore voxels)."""
+    from phast_amd import solid3d as S
+    from phast_amd import workloads as W
+    m = 10 if n is None else n
+    phase, phases, h, ell, load0 = W.pore_in_cube_pfczm(m)
+    if shape is not None:
+        nx, ny, nz = shape
+        phase = np.full((nz, ny, nx), 3, dtype=np.uint8)
+        phase.reshape(-1)[list(pore_voxels or ())] = 4
+    else:
+        shape = (m, m, m)
+    mats = S.build_element_arrays(phase, phases)
+    load = load0 if load is None else load
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid(shape, h), mats, load, kappa=1e-6, ell=ell, tiles=tiles,
+                                           damage_law="pfczm", crack_closure="axes" if closure else "none")
+    base = O.VoxelOperator(shape, h, mats.stiffness_bank, mats.stiffness_index, mats.Gc, mats.lam, mats.mu,
+                           mats.driver.astype(np.int64), kappa=1e-6, ell=ell, pinned_node=op.pinned_node,
+                           controlled=load.controlled)
+    oop = P.PfczmOperator(base, mats.E, mats.f_t, closure=closure)
+    return S, op, oop
+
+
+def _state(op, seed=0):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-2e-7, 2e-7, op.n_mech)
+    x[-6:] = [1.5e-3, -4e-4, 2e-4, 3e-4, -2e-4, 1e-4]
+    d = np.clip(rng.uniform(-0.2, 0.9, op.n_red), 0.0, None)
+    H = rng.uniform(0.0, 3.0, (op.n_elements, 8))
+    return x, d, H
+
+
+@pytest.mark.parametrize("tiles", [(16, 14, 8), (32, 6, 3), (16, 16, 2), (64, 6, 5)])
+def test_pfczm_damage_operators(dev, tiles):
+    S, op, oop = build(shape=(7, 6, 5), pore_voxels=(3, 40, 41, 100), tiles=tiles)
+    x, d, H = _state(op)
+    td, tH = T(d, dev), T(H, dev)
+    assert rel(op.degradation(td), oop.degradation(d)) < 1e-13
+    assert rel(op.pfczm_damage_energy(td, tH), oop.damage_energy(d, H)) < 1e-12
+    assert rel(op.pfczm_damage_energy(td, None), oop.damage_energy(d, None)) < 1e-12
+    assert math.isclose(op.fracture_energy(td)["total"], oop.fracture_energy(d), rel_tol=1e-12)
+    assert rel(op.pfczm_damage_gradient(td, tH), oop.damage_gradient(d, H)) < 1e-12
+    assert rel(op.pfczm_damage_gradient(td, None, 3.0), 3.0 * oop.damage_gradient(d, None)) < 1e-12
+    W = oop.tangent_weights(d, H)
+    assert rel(op.pfczm_tangent_weights(td, tH), W) < 1e-12
+    assert rel(op.pfczm_tangent_weights(td, None), oop.tangent_weights(d, None)) < 1e-15
+    f = np.random.default_rng(3).standard_normal(op.n_red)
+    assert rel(op.pfczm_tangent_action(T(f, dev), T(W, dev)), oop.tangent_action(f, W)) < 1e-12
+    assert rel(op.pfczm_tangent_diagonal(T(W, dev), 2.0), 2.0 * oop.tangent_diagonal(W)) < 1e-12
+
+
+@pytest.mark.parametrize("tiles", [(16, 14, 8), (32, 6, 3), (32, 8, 2), (16, 16, 100)])
+def test_closure_operators(dev, tiles):
+    S, op, oop = build(shape=(7, 6, 5), pore_voxels=(3, 40, 41, 100), tiles=tiles)
+    x, d, H = _state(op, seed=1)
+    tx = T(x, dev)
+    g = oop.degradation(d)
+    tg = T(g, dev)
+    # own pattern: the internal force, the energy, the macro stress
+    assert rel(op.mech_action(tx, tg), oop.mech_action(x, g)) < 1e-11
+    _, r_E, energy = oop.stress_pass(x, g, assemble=False)
+    assert math.isclose(float(op.elastic_energy(tx, tg)), energy, rel_tol=1e-12)
+    assert rel(op.macro_stress(tx, tg), r_E / oop.b.volume) < 1e-12
+    # the pattern of x, bit for bit, and the action of a fixed (random) pattern
+    own = op.closure_pattern(tx)
+    np.testing.assert_array_equal(own.iso.cpu().numpy(), oop.closure_pattern(x))
+    rnd = (np.random.default_rng(2).random((op.n_elements, 8, 4)) < 0.5) & oop.cl_on[:, None, None]
+    pat = S.ClosurePattern.from_iso(torch.as_tensor(rnd, device=dev))
+    np.testing.assert_array_equal(pat.iso.cpu().numpy(), rnd)
+    assert rel(op.mech_action(tx, tg, closure=pat), oop.mech_action(x, g, rnd)) < 1e-11
+    assert rel(op.mech_diagonal(tg, closure=pat), oop.mech_diagonal(g, rnd)) < 1e-12
+    assert rel(op.mech_diagonal(tg), oop.mech_diagonal(g)) < 1e-12
+    assert rel(op.mech_diagonal(None), oop.mech_diagonal(None)) < 1e-12
+    # undamaged: no add-back
+    assert rel(op.mech_action(tx), oop.mech_action(x)) < 1e-11
+    # the pattern-change rule: the own pattern is settled, a flipped bit of real strength is not
+    new, changed = op.closure_pattern_change(tx, own, tg)
+    assert not changed and new.equals(own)
+    flipped = own.iso.clone()
+    e, q, k = np.argwhere(oop.closure_pattern(x))[0]
+    flipped[e, q, k] = False
+    _, changed = op.closure_pattern_change(tx, S.ClosurePattern.from_iso(flipped), tg)
+    _, expect = oop.closure_pattern_change(x, flipped.cpu().numpy(), g)
+    assert changed == expect
+
+
+def test_closure_refuses_a_marching_tile_and_a_missing_pattern(dev):
+    S, op, oop = build(shape=(7, 6, 5), pore_voxels=(3,), tiles=(16, 8, 8))
+    x, d, H = _state(op)
+    with pytest.raises(NotImplementedError, match="voxel-parallel"):
+        op.mech_action(T(x, dev), T(oop.degradation(d), dev))
+    S, op, oop = build(shape=(7, 6, 5), pore_voxels=(3,))
+    g = T(oop.degradation(d), dev)
+    with pytest.raises(ValueError, match="fixed pattern"):
+        op.solve_mechanics(g=g, fixed_mask=op.mech_fixed_mask(), fixed_values=op.mech_fixed_values(1e-3),
+                           diag=op.mech_diagonal(g))
+
+
+def test_projected_newton_damage_solve(dev):
+    from phast_amd import staggered3d as G3
+    S, op, oop = build(shape=(8, 7, 6), pore_voxels=(5, 6, 60))
+    rng = np.random.default_rng(4)
+    lower = np.clip(rng.uniform(-0.3, 0.3, op.n_red), 0.0, None)
+    H = rng.uniform(0.0, 6.0, (op.n_elements, 8))
+    d_ref, nit, _, _ = P.damage_pfczm(oop, H, lower, lower.copy())
+    solver = G3.StaggeredSolver(op)
+    solver._newton_count = 0
+    d, pcg, sets, ok = solver._damage(T(H, dev), T(lower, dev), T(lower, dev))
+    print(f"newton {solver.last_newton_iters} (oracle {nit}), max |d - d_oracle| "
+          f"{float(np.abs(d.cpu().numpy() - d_ref).max()):.3e}", flush=True)
+    assert ok and solver.last_newton_iters == nit
+    assert float(np.abs(d.cpu().numpy() - d_ref).max()) < 1e-10
+    assert bool((d >= T(lower, dev)).all()) and bool((d <= 1.0).all())
+
+
+def test_pore_cell_increments_through_the_peak(dev):
+    """10^3 pore cell, E_xx in 30 steps of 1e-4: elastic, onset at step 16, peak near step 20,
+    softening.  Every increment's macro stress, damage and counts against the oracle."""
+    from phast_amd import solid3d as S
+    from phast_amd import staggered3d as G3
+    load = S.LoadCase.from_schedule("macro_xx", controlled=0, schedule=[(3e-3, 1e-4)], free=(1, 2, 3, 4, 5),
+                                    fixed_zero=())
+    _, op, oop = build(n=10, load=load)
+    solver = G3.StaggeredSolver(op, load, G3.SolverConfig(max_cutbacks=5))
+    st = solver.initial_state()
+    ost = P.initial_state(oop)
+    worst_s = worst_d = 0.0
+    for k, target in enumerate(load.targets, start=1):
+        st, rec = solver.advance(st, target)
+        ost, orec = P.staggered_increment(oop, ost, target)
+        ds = abs(rec.macro_stress[0] - orec["sigma"][0]) / abs(orec["sigma"][0])
+        dd = float(np.abs(st.d.cpu().numpy() - ost["d"]).max())
+        worst_s, worst_d = max(worst_s, ds), max(worst_d, dd)
+        print(f"step {k}: sigma {rec.macro_stress[0]:.9f} (oracle {orec['sigma'][0]:.9f}) d max "
+              f"{float(st.d.max()):.5f} |dd| {dd:.2e} stagger {rec.stagger_iters}/{orec['stagger_iters']} newton "
+              f"{rec.newton_iters_pf}/{orec['newton']} closure {rec.closure_solves}/{orec['closure_solves']}",
+              flush=True)
+        assert rec.cutbacks == 0
+        assert rec.stagger_iters == orec["stagger_iters"]
+        assert math.isclose(rec.balance_residual_rel, orec["balance_residual_rel"], rel_tol=1e-5, abs_tol=1e-12)
+    print(f"worst relative stress {worst_s:.3e}, worst |d - d_oracle| {worst_d:.3e}", flush=True)
+    assert worst_s < 1e-8 and worst_d < 1e-8
+    assert float(st.d.max()) > 0.05
