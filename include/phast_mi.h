@@ -192,6 +192,23 @@ int phast_q4_batch_step(phast_q4_plan *plan, int32_t method, const double *u, co
                         hipStream_t stream);
 int phast_q4_batch_error(phast_q4_plan *plan, int32_t *err, int32_t clear);
 
+/* n_steps persistent batched steps in ONE launch (the same loop body as phast_q4_batch_step,
+ * staggered_solver.py:1115-1165 with damage_every = 1, repeated).  Each instance advances on
+ * its own blocks, joined only by its own barriers, so instances whose damage solves finish
+ * early run ahead instead of waiting for the slowest one at every step.  The state is kept
+ * in two buffer sets that the steps alternate between: step s reads (u, v, a, d, H) and
+ * writes (u1, v1, a1, d1, H1) when s is even, the other way round when s is odd, so after an
+ * even n_steps the result is in (u, v, a, d, H) and after an odd one in (u1, ...).  psi
+ * holds the last step's psi+.  iters/status are DEVICE int32 arrays [n_steps][n_inst].
+ * Results are bitwise those of n_steps calls of phast_q4_batch_step.  Needs
+ * phast_q4_batch_prepare; allocates nothing, never synchronises, may be captured.  Refuses
+ * an n_steps x max_iter that would overflow the per-instance barrier counters. */
+int phast_q4_batch_run(phast_q4_plan *plan, int32_t method, int32_t n_steps, double *u, double *v, double *a,
+                       double *d, const double *f_ext, const double *M_inv, const uint8_t *bc_mask,
+                       const double *bc_vals, double *H, const double *Gc_cell, const uint8_t *fixed,
+                       const double *fixed_vals, double tol, int32_t max_iter, double *u1, double *v1, double *a1,
+                       double *H1, double *psi, double *d1, int32_t *iters, int32_t *status, hipStream_t stream);
+
 /* =====================================================================================
  * 2D linear triangles (T3) on an unstructured mesh
  * ===================================================================================== */

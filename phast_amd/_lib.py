@@ -68,6 +68,7 @@ _SIGNATURES = {
     "phast_q4_batch_prepare": ([_P, _P], C.c_int),
     "phast_q4_batch_step": ([_P, _I32] + [_P] * 12 + [_D, _I32] + [_P] * 9, C.c_int),
     "phast_q4_batch_error": ([_P, C.POINTER(_I32), _I32], C.c_int),
+    "phast_q4_batch_run": ([_P, _I32, _I32] + [_P] * 12 + [_D, _I32] + [_P] * 9, C.c_int),
     "phast_t3_plan_create": ([_I64, _I64, _P, _P, _D, _D, _D, _D, _D, _I32, _I32, _I32, _I32, _D, C.POINTER(_P)],
                              C.c_int),
     "phast_t3_plan_destroy": ([_P], C.c_int),

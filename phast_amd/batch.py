@@ -9,7 +9,9 @@ Two ways to advance them:
 * ``step_persistent`` / ``capture``: ONE persistent kernel per step runs the mechanics
   and the whole damage solve of every instance, with barriers among the blocks of each
   instance instead of kernel boundaries and no host round trip, so a sequence of steps
-  can be captured into a graph and replayed.
+  can be captured into a graph and replayed.  With ``steps_per_launch > 1`` one launch
+  runs several steps, and each instance advances through them on its own blocks without
+  waiting for the other instances between steps.
 
 Each instance keeps its own CG scalars and its own partial sums, so the result of an
 instance is bitwise the result of running it alone, on either path.
@@ -163,19 +165,52 @@ class BatchedExplicit:
         self.H, self.H1 = self.H1, self.H
         self.d, self.d1 = self.d1, self.d
 
-    def step_persistent(self, n_steps: int = 1) -> None:
-        """Advance every instance ``n_steps`` explicit steps, one kernel launch per step."""
+    def _launch_run(self, n_steps: int, iters: torch.Tensor, status: torch.Tensor) -> None:
+        check(_lib.lib().phast_q4_batch_run(
+            self.plan.handle, self.method, int(n_steps), ptr(self.u), ptr(self.v), ptr(self.a), ptr(self.d),
+            ptr(self.f_ext), ptr(self.Minv), None, None, ptr(self.H), ptr(self.Gc), ptr(self.fixed),
+            ptr(self.fixed_vals), self.tol, self.max_iter, ptr(self.u1), ptr(self.v1), ptr(self.a1), ptr(self.H1),
+            ptr(self.psi), ptr(self.d1), ptr(iters), ptr(status), stream()), "persistent batched run")
+        if n_steps % 2:  # an odd number of steps leaves the state in the second buffer set
+            self.u, self.u1 = self.u1, self.u
+            self.v, self.v1 = self.v1, self.v
+            self.a, self.a1 = self.a1, self.a
+            self.H, self.H1 = self.H1, self.H
+            self.d, self.d1 = self.d1, self.d
+
+    def step_persistent(self, n_steps: int = 1, steps_per_launch: int = 1) -> None:
+        """Advance every instance ``n_steps`` explicit steps.  ``steps_per_launch = 1``: one
+        persistent kernel per step, every instance in lockstep.  Larger: that many steps per
+        launch, each instance running ahead on its own blocks within a launch."""
         if self.grid is None:
             self.prepare_persistent()
-        for _ in range(n_steps):
-            self._launch_persistent(self.iters_dev, self.status_dev)
-            self.steps += 1
+        spl = max(1, int(steps_per_launch))
+        done = 0
+        while done < n_steps:
+            k = min(spl, n_steps - done)
+            if spl == 1:
+                self._launch_persistent(self.iters_dev, self.status_dev)
+            else:
+                iters = torch.zeros((k, self.n_inst), dtype=torch.int32, device=self.device)
+                status = torch.zeros_like(iters)
+                self._launch_run(k, iters, status)
+                # per instance: the last step's iterations; the first failed step's status, if any
+                bad = status != _lib.CG_CONVERGED
+                first_bad = status.gather(0, bad.to(torch.int32).argmax(dim=0, keepdim=True))[0]
+                self.iters_dev.copy_(iters[-1])
+                self.status_dev.copy_(torch.where(bad.any(dim=0), first_bad, status[-1]))
+            done += k
+            self.steps += k
 
-    def capture(self, n_steps: int = 2) -> "BatchGraph":
+    def capture(self, n_steps: int = 2, steps_per_launch: int = 1) -> "BatchGraph":
         """Capture ``n_steps`` (even, so the ping-pong buffers come back) persistent steps into
-        a graph; each replay advances the batch by ``n_steps``."""
+        a graph, ``steps_per_launch`` (a divisor of ``n_steps``) steps per kernel launch; each
+        replay advances the batch by ``n_steps``."""
+        spl = int(steps_per_launch)
         if n_steps <= 0 or n_steps % 2:
             raise ValueError("n_steps must be a positive even number")
+        if spl <= 0 or n_steps % spl:
+            raise ValueError("steps_per_launch must be a positive divisor of n_steps")
         if self.grid is None:
             self.prepare_persistent()
         iters = torch.zeros((n_steps, self.n_inst), dtype=torch.int32, device=self.device)
@@ -183,8 +218,11 @@ class BatchedExplicit:
         graph = torch.cuda.CUDAGraph()
         torch.cuda.synchronize()
         with torch.cuda.graph(graph):
-            for k in range(n_steps):
-                self._launch_persistent(iters[k], status[k])
+            for k in range(0, n_steps, spl):
+                if spl == 1:
+                    self._launch_persistent(iters[k], status[k])
+                else:
+                    self._launch_run(spl, iters[k:k + spl], status[k:k + spl])
         return BatchGraph(self, graph, iters, status, n_steps)
 
     def check_persistent(self, status: torch.Tensor | None = None) -> np.ndarray:

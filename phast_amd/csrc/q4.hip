@@ -75,8 +75,11 @@ struct phast_q4_plan {
     phast::Q4CG *h_cg = nullptr;  // pinned mirror
     int32_t last_iters = 0;
     // persistent batched step (set up by phast_q4_batch_prepare)
-    int32_t b_grid[2] = {0, 0};  // grid per damage method
-    int32_t *b_pb_inst[2] = {nullptr, nullptr}, *b_pb_rank[2] = {nullptr, nullptr}, *b_nblk[2] = {nullptr, nullptr};
+    // grid and block tables per kernel: [method] for the one-step kernel, [2 + method] for
+    // the several-step kernel
+    int32_t b_grid[4] = {0, 0, 0, 0};
+    int32_t *b_pb_inst[4] = {}, *b_pb_rank[4] = {}, *b_nblk[4] = {};
+    int32_t b_nb_max[4] = {0, 0, 0, 0};  // most blocks of one instance
     uint32_t *b_bar = nullptr;
     int32_t *b_err = nullptr;
     uint64_t b_spin_ticks = 0;
@@ -1141,22 +1144,15 @@ __device__ __forceinline__ void minv_strip(const Q4Inst *inst, const int32_t *bl
     }
 }
 
-template <int NT, int METHOD>
-__global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
-    __shared__ double sm[kDmgLds<NT>];
-    __shared__ double red[8];
-    __shared__ Q4CG S;
-    const int k = uniform_at(A.pb_inst, blockIdx.x), rank = uniform_at(A.pb_rank, blockIdx.x),
-              nb = uniform_at(A.nblk, k);
+// The damage solve of instance k on its nb blocks (this block has rank `rank`): the strip
+// passes and scalar steps of phast_q4_damage_solve in the same order, kernel boundaries
+// replaced by `barrier`.  Ends with the solve's final pass; no barrier after it.
+template <int NT, int METHOD, class BAR>
+__device__ __forceinline__ void batch_damage(const BatchArgs &A, int k, int rank, int nb, double *sm, double *red,
+                                             Q4CG &S, BAR &&barrier) {
     QI &P = inst_at(A.M.inst, k);
     const int n_tiles = A.V.n_tiles;
     const int check_every = METHOD == 0 ? 10 : 50;
-    uint32_t *arrive = A.bar + 2 * k;
-    uint32_t nbar = 0;
-    auto barrier = [&]() {
-        ++nbar;
-        inst_barrier(arrive, nbar * (uint32_t)nb, A.err, A.spin_ticks);
-    };
 #define PHAST_STRIPS for (int s_ = rank; s_ < P.n_tiles; s_ += nb)
     const int base = P.tile_off;
 
@@ -1233,19 +1229,86 @@ __global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
         PHAST_STRIPS vec_strip<NT, V_PC_FINISH>(A.V, base + s_, S, red);
     }
 #undef PHAST_STRIPS
+}
+
+// Leaving a persistent launch: the last block of an instance out resets its counters, so
+// the next launch (or graph replay) starts from zero.
+__device__ __forceinline__ void batch_depart(uint32_t *arrive, int nb) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (rank == 0) {
-            A.iters[k] = S.iter;
-            A.status[k] = S.status;
-        }
         __threadfence();
         if (atomicAdd(arrive + 1, 1u) == (uint32_t)nb - 1) {  // last block of the instance out
             atomicExch(arrive, 0u);
             atomicExch(arrive + 1, 0u);
         }
     }
+}
+
+template <int NT, int METHOD>
+__global__ __launch_bounds__(NT) void q4_batch_kernel(BatchArgs A) {
+    __shared__ double sm[kDmgLds<NT>];
+    __shared__ double red[8];
+    __shared__ Q4CG S;
+    const int k = uniform_at(A.pb_inst, blockIdx.x), rank = uniform_at(A.pb_rank, blockIdx.x),
+              nb = uniform_at(A.nblk, k);
+    uint32_t *arrive = A.bar + 2 * k;
+    uint32_t nbar = 0;
+    auto barrier = [&]() {
+        ++nbar;
+        inst_barrier(arrive, nbar * (uint32_t)nb, A.err, A.spin_ticks);
+    };
+    batch_damage<NT, METHOD>(A, k, rank, nb, sm, red, S, barrier);
+    __syncthreads();
+    if (threadIdx.x == 0 && rank == 0) {
+        A.iters[k] = S.iter;
+        A.status[k] = S.status;
+    }
+    batch_depart(arrive, nb);
+}
+
+// Several explicit steps in ONE launch.  Each instance runs its steps on its own blocks,
+// joined only by the instance's barriers, so an instance whose damage solve needs few
+// iterations goes on to its next step instead of waiting for the slowest instance; the
+// launch ends when every instance has done n_steps.  Step s reads the buffers of A[s & 1]
+// and writes the other set (A[1] is A[0] with inputs and outputs swapped), and every step
+// is the explicit-step pass followed by the damage solve, exactly as one launch of
+// q4_mech_kernel<STEP, HIST> and one of q4_batch_kernel: the same strips, partial sums and
+// scalar steps, so the bits are those of n_steps separate steps.
+struct BatchRunArgs {
+    BatchArgs A[2];
+    int32_t n_steps, n_inst;
+    int32_t *iters, *status;  // [n_steps][n_inst]
+};
+
+template <int NT, int METHOD>
+__global__ __launch_bounds__(NT) void q4_batch_run_kernel(BatchRunArgs R) {
+    __shared__ double sm[kMechLds<NT> > kDmgLds<NT> ? kMechLds<NT> : kDmgLds<NT>];
+    __shared__ double red[8];
+    __shared__ Q4CG S;
+    const BatchArgs &A0 = R.A[0];
+    const int k = uniform_at(A0.pb_inst, blockIdx.x), rank = uniform_at(A0.pb_rank, blockIdx.x),
+              nb = uniform_at(A0.nblk, k);
+    QI &P = inst_at(A0.M.inst, k);
+    uint32_t *arrive = A0.bar + 2 * k;
+    uint32_t nbar = 0;
+    auto barrier = [&]() {
+        ++nbar;
+        inst_barrier(arrive, nbar * (uint32_t)nb, A0.err, A0.spin_ticks);
+    };
+    for (int step = 0; step < R.n_steps; ++step) {
+        const BatchArgs &A = R.A[step & 1];
+        for (int s_ = rank; s_ < P.n_tiles; s_ += nb) mech_strip<NT, true, true>(A.M, P.tile_off + s_, sm);
+        barrier();  // u1, v1, a1 and H_new of every strip before the damage passes read them
+        batch_damage<NT, METHOD>(A, k, rank, nb, sm, red, S, barrier);
+        __syncthreads();
+        if (threadIdx.x == 0 && rank == 0) {
+            R.iters[(int64_t)step * R.n_inst + k] = S.iter;
+            R.status[(int64_t)step * R.n_inst + k] = S.status;
+        }
+        barrier();  // d_out of every strip before the next step's rows are staged
+    }
+    batch_depart(arrive, nb);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1431,7 +1494,7 @@ extern "C" int phast_q4_plan_destroy(phast_q4_plan *pl) {
     (void)hipFree(pl->active);
     (void)hipFree(pl->cg);
     if (pl->h_cg) (void)hipHostFree(pl->h_cg);
-    for (int m = 0; m < 2; ++m) {
+    for (int m = 0; m < 4; ++m) {
         (void)hipFree(pl->b_pb_inst[m]);
         (void)hipFree(pl->b_pb_rank[m]);
         (void)hipFree(pl->b_nblk[m]);
@@ -1877,19 +1940,29 @@ namespace phast {
 
 template <int NT, int M>
 static int batch_occupancy(int *blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, q4_batch_kernel<NT, M>, NT, 0) == hipSuccess
-               ? PHAST_OK
-               : PHAST_ERR_HIP;
+    const hipError_t e = M < 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, q4_batch_kernel<NT, M & 1>,
+                                                                              NT, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                     blocks_per_cu, q4_batch_run_kernel<NT, M & 1>, NT, 0);
+    return e == hipSuccess ? PHAST_OK : PHAST_ERR_HIP;
+}
+
+template <int NT>
+static int batch_occupancy_nt(int m, int *bpc) {
+    switch (m) {
+        case 0: return batch_occupancy<NT, 0>(bpc);
+        case 1: return batch_occupancy<NT, 1>(bpc);
+        case 2: return batch_occupancy<NT, 2>(bpc);
+        case 3: return batch_occupancy<NT, 3>(bpc);
+    }
+    return PHAST_ERR_UNSUPPORTED;
 }
 
 static int batch_occupancy_of(int nt, int m, int *bpc) {
-    switch (nt * 2 + m) {
-        case 256 * 2 + 0: return batch_occupancy<256, 0>(bpc);
-        case 256 * 2 + 1: return batch_occupancy<256, 1>(bpc);
-        case 128 * 2 + 0: return batch_occupancy<128, 0>(bpc);
-        case 128 * 2 + 1: return batch_occupancy<128, 1>(bpc);
-        case 64 * 2 + 0: return batch_occupancy<64, 0>(bpc);
-        case 64 * 2 + 1: return batch_occupancy<64, 1>(bpc);
+    switch (nt) {
+        case 256: return batch_occupancy_nt<256>(m, bpc);
+        case 128: return batch_occupancy_nt<128>(m, bpc);
+        case 64: return batch_occupancy_nt<64>(m, bpc);
     }
     return PHAST_ERR_UNSUPPORTED;
 }
@@ -1898,6 +1971,12 @@ template <int NT>
 static void batch_launch(const phast_q4_plan *pl, int m, const BatchArgs &A, hipStream_t st) {
     if (m == 0) hipLaunchKernelGGL((q4_batch_kernel<NT, 0>), dim3(pl->b_grid[0]), dim3(NT), 0, st, A);
     else hipLaunchKernelGGL((q4_batch_kernel<NT, 1>), dim3(pl->b_grid[1]), dim3(NT), 0, st, A);
+}
+
+template <int NT>
+static void batch_run_launch(const phast_q4_plan *pl, int m, const BatchRunArgs &R, hipStream_t st) {
+    if (m == 0) hipLaunchKernelGGL((q4_batch_run_kernel<NT, 0>), dim3(pl->b_grid[2]), dim3(NT), 0, st, R);
+    else hipLaunchKernelGGL((q4_batch_run_kernel<NT, 1>), dim3(pl->b_grid[3]), dim3(NT), 0, st, R);
 }
 
 }  // namespace phast
@@ -1912,7 +1991,7 @@ extern "C" int phast_q4_batch_prepare(phast_q4_plan *pl, int32_t *grid_out) {
     PHAST_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     PHAST_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
     pl->b_spin_ticks = (uint64_t)(rate_khz > 0 ? rate_khz : 100000) * 1000ull * 2ull;  // 2 s per wait
-    for (int m = 0; m < 2; ++m) {
+    for (int m = 0; m < 4; ++m) {
         int bpc = 0;
         const int rc = batch_occupancy_of(pl->tx, m, &bpc);
         if (rc != PHAST_OK) {
@@ -1956,6 +2035,7 @@ extern "C" int phast_q4_batch_prepare(phast_q4_plan *pl, int32_t *grid_out) {
         PHAST_HIP(hipMemcpy(pl->b_pb_rank[m], pb_rank.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
         PHAST_HIP(hipMemcpy(pl->b_nblk[m], nb.data(), sizeof(int32_t) * pl->n_inst, hipMemcpyHostToDevice));
         pl->b_grid[m] = (int32_t)total;
+        pl->b_nb_max[m] = *std::max_element(nb.begin(), nb.end());
     }
     if (!pl->b_bar) {
         PHAST_HIP(hipMalloc(&pl->b_bar, sizeof(uint32_t) * 2 * pl->n_inst));
@@ -1971,26 +2051,28 @@ extern "C" int phast_q4_batch_prepare(phast_q4_plan *pl, int32_t *grid_out) {
     return PHAST_OK;
 }
 
-extern "C" int phast_q4_batch_step(phast_q4_plan *pl, int32_t method, const double *u, const double *v,
-                                   const double *a, const double *d, const double *f_ext, const double *M_inv,
-                                   const uint8_t *bc_mask, const double *bc_vals, const double *H_old,
-                                   const double *Gc_cell, const uint8_t *fixed, const double *fixed_vals, double tol,
-                                   int32_t max_iter, double *u1, double *v1, double *a1, double *H_new, double *psi,
-                                   double *d_out, int32_t *iters, int32_t *status, hipStream_t st) {
-    PHAST_REQUIRE(pl && u && v && a && d && M_inv && H_old && u1 && v1 && a1 && H_new && d_out && iters && status,
-                  "phast_q4_batch_step: null argument");
+namespace phast {
+
+// The arguments of one batched step: inputs u, v, a, d (= d_prev), H_old; outputs u1, v1,
+// a1, H_new, psi, d_out; the plan's CG work and the block tables of kernel `m`.
+static int batch_args(phast_q4_plan *pl, int32_t method, int m, const double *u, const double *v, const double *a,
+                      const double *d, const double *f_ext, const double *M_inv, const uint8_t *bc_mask,
+                      const double *bc_vals, const double *H_old, const double *Gc_cell, const uint8_t *fixed,
+                      const double *fixed_vals, double tol, int32_t max_iter, double *u1, double *v1, double *a1,
+                      double *H_new, double *psi, double *d_out, BatchArgs &A) {
+    PHAST_REQUIRE(pl && u && v && a && d && M_inv && H_old && u1 && v1 && a1 && H_new && d_out,
+                  "batched step: null argument");
     PHAST_REQUIRE(method == 0 || method == 1, "method must be 0 (projected) or 1 (post-clamp)");
-    PHAST_REQUIRE(pl->b_grid[method] > 0, "phast_q4_batch_step: call phast_q4_batch_prepare first");
+    PHAST_REQUIRE(pl->b_grid[m] > 0, "batched step: call phast_q4_batch_prepare first");
     PHAST_REQUIRE(!bc_mask == !bc_vals, "bc_mask and bc_vals must be given together");
     PHAST_REQUIRE(!fixed == !fixed_vals, "fixed and fixed_vals must be given together");
     PHAST_REQUIRE(tol >= 0.0 && max_iter >= 0, "tol and max_iter must be non-negative");
     PHAST_REQUIRE(u1 != u && v1 != v && a1 != a && H_new != H_old && d_out != d,
-                  "phast_q4_batch_step: outputs must not alias inputs");
+                  "batched step: outputs must not alias inputs");
     for (const void *q : {(const void *)u, (const void *)v, (const void *)a, (const void *)f_ext,
                           (const void *)u1, (const void *)v1, (const void *)a1, (const void *)H_old,
                           (const void *)H_new, (const void *)psi, (const void *)bc_vals})
         PHAST_REQUIRE(!q || aligned16(q), "vector arguments must be 16-byte aligned");
-    BatchArgs A;
     std::memset(&A, 0, sizeof(A));
     A.M.inst = pl->d_inst;
     A.M.block_inst = pl->d_block_inst;
@@ -2054,23 +2136,74 @@ extern "C" int phast_q4_batch_step(phast_q4_plan *pl, int32_t method, const doub
     V.part = pl->part;
     V.n_tiles = pl->n_tiles;
 
-    A.pb_inst = pl->b_pb_inst[method];
-    A.pb_rank = pl->b_pb_rank[method];
-    A.nblk = pl->b_nblk[method];
+    A.pb_inst = pl->b_pb_inst[m];
+    A.pb_rank = pl->b_pb_rank[m];
+    A.nblk = pl->b_nblk[m];
     A.bar = pl->b_bar;
     A.err = pl->b_err;
-    A.iters = iters;
-    A.status = status;
     A.minv = pl->minv;
     A.tol = tol;
     A.max_iter = max_iter;
     A.method = method;
     A.spin_ticks = pl->b_spin_ticks;
+    return PHAST_OK;
+}
+
+}  // namespace phast
+
+extern "C" int phast_q4_batch_step(phast_q4_plan *pl, int32_t method, const double *u, const double *v,
+                                   const double *a, const double *d, const double *f_ext, const double *M_inv,
+                                   const uint8_t *bc_mask, const double *bc_vals, const double *H_old,
+                                   const double *Gc_cell, const uint8_t *fixed, const double *fixed_vals, double tol,
+                                   int32_t max_iter, double *u1, double *v1, double *a1, double *H_new, double *psi,
+                                   double *d_out, int32_t *iters, int32_t *status, hipStream_t st) {
+    PHAST_REQUIRE(pl && iters && status, "phast_q4_batch_step: null argument");
+    BatchArgs A;
+    PHAST_TRY(batch_args(pl, method, method, u, v, a, d, f_ext, M_inv, bc_mask, bc_vals, H_old, Gc_cell, fixed,
+                         fixed_vals, tol, max_iter, u1, v1, a1, H_new, psi, d_out, A));
+    A.iters = iters;
+    A.status = status;
     PHAST_TRY(mech_dispatch(pl, A.M, true, true, st));
     switch (pl->tx) {
         case 256: batch_launch<256>(pl, method, A, st); break;
         case 128: batch_launch<128>(pl, method, A, st); break;
         case 64: batch_launch<64>(pl, method, A, st); break;
+        default: set_error("unsupported strip width %d", pl->tx); return PHAST_ERR_UNSUPPORTED;
+    }
+    PHAST_LAUNCHED();
+    return PHAST_OK;
+}
+
+
+extern "C" int phast_q4_batch_run(phast_q4_plan *pl, int32_t method, int32_t n_steps, double *u, double *v,
+                                  double *a, double *d, const double *f_ext, const double *M_inv,
+                                  const uint8_t *bc_mask, const double *bc_vals, double *H, const double *Gc_cell,
+                                  const uint8_t *fixed, const double *fixed_vals, double tol, int32_t max_iter,
+                                  double *u1, double *v1, double *a1, double *H1, double *psi, double *d1,
+                                  int32_t *iters, int32_t *status, hipStream_t st) {
+    PHAST_REQUIRE(pl && iters && status, "phast_q4_batch_run: null argument");
+    PHAST_REQUIRE(n_steps >= 1, "phast_q4_batch_run: n_steps must be at least 1");
+    PHAST_REQUIRE(method == 0 || method == 1, "method must be 0 (projected) or 1 (post-clamp)");
+    // the arrival counter of an instance counts (barriers passed) x (its blocks) within one
+    // launch: at most 5 barriers per CG iteration plus 6 per step
+    const int32_t nb_max = std::max(pl->b_nb_max[2 + method], 1);
+    const double arrivals = (double)n_steps * (5.0 * ((double)max_iter + 1.0) + 6.0) * (double)nb_max;
+    PHAST_REQUIRE(arrivals < 4.0e9, "phast_q4_batch_run: %d steps of up to %d CG iterations overflow the barrier "
+                  "counters; use fewer steps per launch", n_steps, max_iter);
+    BatchRunArgs R;
+    std::memset(&R, 0, sizeof(R));
+    PHAST_TRY(batch_args(pl, method, 2 + method, u, v, a, d, f_ext, M_inv, bc_mask, bc_vals, H, Gc_cell, fixed,
+                         fixed_vals, tol, max_iter, u1, v1, a1, H1, psi, d1, R.A[0]));
+    PHAST_TRY(batch_args(pl, method, 2 + method, u1, v1, a1, d1, f_ext, M_inv, bc_mask, bc_vals, H1, Gc_cell, fixed,
+                         fixed_vals, tol, max_iter, u, v, a, H, psi, d, R.A[1]));
+    R.n_steps = n_steps;
+    R.n_inst = pl->n_inst;
+    R.iters = iters;
+    R.status = status;
+    switch (pl->tx) {
+        case 256: batch_run_launch<256>(pl, method, R, st); break;
+        case 128: batch_run_launch<128>(pl, method, R, st); break;
+        case 64: batch_run_launch<64>(pl, method, R, st); break;
         default: set_error("unsupported strip width %d", pl->tx); return PHAST_ERR_UNSUPPORTED;
     }
     PHAST_LAUNCHED();

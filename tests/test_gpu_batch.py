@@ -104,3 +104,48 @@ def test_graph_replay_matches_eager_persistent_steps(dev):
     assert cap.steps == 8
     for k in range(cap.n_inst):
         assert same(cap.instance_state(k), eager.instance_state(k)), k
+
+
+@pytest.mark.parametrize("method", ["projected_cg", "post_clamp"])
+def test_several_steps_per_launch_are_bitwise_one_step_launches(dev, method):
+    """Instances running ahead of each other inside one launch give the bits (and CG counts)
+    of one launch per step: 7 steps as launches of 5 and 2 steps, and as 7 single launches."""
+    from phast_amd import _lib
+    data = specimens((40, 67, 9, 150), seed=11)
+    one = make(*data, method, dev)
+    per_step = []
+    for _ in range(7):
+        one.step_persistent(1)
+        one.check_persistent()
+        per_step.append(one.iters_dev.cpu().numpy().astype(np.int64))
+    run = make(*data, method, dev)
+    run.prepare_persistent()
+    iters = torch.zeros((5, run.n_inst), dtype=torch.int32, device=dev)
+    status = torch.zeros_like(iters)
+    run._launch_run(5, iters, status)  # odd: the state moves to the second buffer set
+    torch.cuda.synchronize()
+    assert bool((status == _lib.CG_CONVERGED).all())
+    assert np.array_equal(iters.cpu().numpy(), np.array(per_step[:5]))
+    run.step_persistent(2, steps_per_launch=2)
+    run.check_persistent()
+    assert np.array_equal(run.iters_dev.cpu().numpy(), per_step[-1])
+    for k in range(run.n_inst):
+        assert same(run.instance_state(k), one.instance_state(k)), k
+    assert max(int(i.max()) for i in per_step) > 0
+
+
+def test_graph_of_several_step_launches_matches_eager_steps(dev):
+    data = specimens((48, 21, 90, 130), seed=13)
+    eager = make(*data, "projected_cg", dev)
+    eager.step_persistent(12)
+    eager.check_persistent()
+    cap = make(*data, "projected_cg", dev)
+    g = cap.capture(6, steps_per_launch=3)
+    for _ in range(2):  # replays with eager work and synchronisation between them
+        g.replay(1)
+        torch.cuda.synchronize()
+        cap.check_persistent(g.status)
+        _ = cap.u.sum().item()
+    assert cap.steps == 12
+    for k in range(cap.n_inst):
+        assert same(cap.instance_state(k), eager.instance_state(k)), k
