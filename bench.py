@@ -2,17 +2,18 @@
 """Benchmark of the matrix-free phase-field hot path on one MI355X.
 
 Headline (one JSON line on stdout): element-apply GDOF/s of the periodic hex8 voxel
-operator on the 64^3 pore cube (a centred spherical pore of radius 12 voxels, void
-stiffness 1e-6, E = 3 GPa, nu = 0.35, Gc = 0.5 J/m^2, l0 = 2 voxels, voxel 1 um).  One
-step = one apply y = A(g(d)) x with the degradation formed in-kernel from nodal damage,
-inputs resident in HBM.  GDOF = 3 n + 6 output dofs per apply.  ``roofline`` prices the
-apply kernel's algorithmic FLOPs (940 per voxel, sum-factorised) against the 78.6 TFLOP/s
-FP64 peak -- its intensity (940 FLOP / 57 B) is above the FP64 ridge -- and also reports the
-algorithmic bytes (57 B per node: u 24 + d 8 + phase 1 + r 24) against 8 TB/s;
-``cpu_baseline`` times the oracle's numpy restatement of the same apply on the host cores.
-``secondary`` holds the Q4 apply and fused explicit step, the T3 force / explicit step /
-damage apply on an 8.4 M-node split-quad plate, the larger voxel sizes, the 64-specimen
-batched run and the reaction-curve gradient, measured in the same run.
+operator on the 128^3 particle composite (seeded non-overlapping spheres, radii U[6,14]
+voxels, 30 % volume fraction, 70 GPa particles in a 3 GPa matrix, a one-voxel interface
+shell with Gc x 0.1).  One step = one apply y = A(g(d)) x with the degradation formed
+in-kernel from nodal damage, inputs resident in HBM.  GDOF = 3 n + 6 output dofs per apply.
+``roofline`` prices the apply's algorithmic FLOPs (940 per voxel, sum-factorised) against
+the 78.6 TFLOP/s FP64 peak -- its intensity (940 FLOP / 57 B) is above the FP64 ridge -- and
+also reports the algorithmic bytes (57 B per node: u 24 + d 8 + phase 1 + r 24) against
+8 TB/s; ``cpu_baseline`` times a torch-CPU restatement of the reference's batched apply on
+the host cores.  ``secondary`` holds the Q4 apply and fused explicit step, the T3 force /
+explicit step / damage apply on an 8.4 M-node split-quad plate, the other voxel sizes, the
+200 x 80 SENT run, the reference's 48^3 PF-CZM pore example against its recorded history,
+the 64-specimen batched run and the reaction-curve gradient, measured in the same run.
 """
 from __future__ import annotations
 
@@ -70,6 +71,7 @@ def parse():
     ap.add_argument("--no-sequential", action="store_true", help="skip the 64 single-specimen runs of the batched case")
     ap.add_argument("--tiles", type=str, default="", help="fixed hex8 tile tx,ty,kz (skips the autotuner)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-specimen batched run")
+    ap.add_argument("--no-pfczm", action="store_true", help="skip the 48^3 PF-CZM pore example")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the voxel reaction-curve gradient run")
     ap.add_argument("--adjoint-size", type=int, default=96, help="voxels per edge of the gradient run")
     ap.add_argument("--adjoint-steps", type=int, default=30, help="load steps of the gradient run")
@@ -81,6 +83,8 @@ def parse():
     ap.add_argument("--sequential-window", type=int, default=0, help="steps of the batched and sequential runs "
                     "timed against each other (0: the whole batched run)")
     ap.add_argument("--batched-steps", type=int, default=2000, help="steps of the 64-specimen batched run")
+    ap.add_argument("--batch-steps-per-launch", type=int, default=20, help="explicit steps per persistent launch "
+                    "of the batched run (1: one launch per step, every specimen in lockstep)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 5 device copies of 256 MiB (the reference kernel for HBM counter scaling)")
     return ap.parse_args()
@@ -167,10 +171,16 @@ def main():
     sec = {}
     if not a.no_secondary:
         sec = secondary(rng, dev, a)
+    if not a.no_pfczm:
+        try:
+            sec["pfczm_pore48_recorded_history"] = pfczm_pore_history(dev)
+        except Exception as exc:
+            sec["pfczm_pore48_recorded_history"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not a.no_batched:
         try:
             sec["c3_batched_64_specimens"] = batched_specimens(dev, a.batched_steps, a.batch_tile,
-                                                               not a.no_sequential, a.sequential_window)
+                                                               not a.no_sequential, a.sequential_window,
+                                                               a.batch_steps_per_launch)
         except Exception as exc:  # reported, so that the headline line is still printed
             sec["c3_batched_64_specimens"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not a.no_adjoint:
@@ -510,7 +520,50 @@ def c3_batch_inputs(n_spec=64, seed=2024):
     return items, [sp.Gc_field for sp in specs], [sp.f_ext for sp in specs], [(sp.notch, 1.0) for sp in specs]
 
 
-def batched_specimens(dev, steps, fixed_tile="", sequential=True, window=0):
+def pfczm_pore_history(dev):
+    """The reference's PF-CZM pore example (48^3 voxels of 0.5 um, pore diameter 0.4 L, E_xx
+    to 3e-3 in 150 increments, spectral driver, crack closure, stagger tol 1e-4, PCG rtol
+    1e-8, up to 5 cutbacks): wall time of the whole load path and the largest relative
+    difference of the macro stress from the reference's recorded history."""
+    import torch
+    from phast_amd import solid3d as S
+    from phast_amd import staggered3d as G3
+    from phast_amd import workloads as W
+    gold = json.loads((ROOT / "tests" / "golden" / "pore_in_cube_pfczm_history.json").read_text())
+    c = gold["cell"]
+    n = c["shape"][0]
+    phase, phases, h, ell, load = W.pore_in_cube_pfczm(n, c["increments"], c["to"])
+    mats = S.build_element_arrays(phase, phases)
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid((n, n, n), h), mats, load, kappa=1e-6, ell=ell,
+                                           damage_law="pfczm", crack_closure="axes")
+    cfg = G3.SolverConfig(stagger_tol=c["stagger_tol"], pcg_rtol_mech=c["pcg_rtol_mech"],
+                          damage_kkt_tol=c["damage_kkt_tol"], max_cutbacks=c["max_cutbacks"])
+    solver = G3.StaggeredSolver(op, load, cfg)
+    st = solver.initial_state()
+    worst, stagger, pcg, newton = 0.0, 0, 0, 0
+    peak = (0.0, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for row in gold["rows"]:
+        k = int(row["step"])
+        st, recs = solver.advance_increment(st, load.targets[k - 1])
+        rec = recs[-1]
+        worst = max(worst, abs(rec.macro_stress[0] - row["macro_stress_xx"]) / abs(row["macro_stress_xx"]))
+        stagger += sum(r.stagger_iters for r in recs)
+        pcg += sum(r.pcg_iters_mech for r in recs)
+        newton += sum(r.newton_iters_pf or 0 for r in recs)
+        if rec.macro_stress[0] > peak[0]:
+            peak = (rec.macro_stress[0], rec.macro_strain[0])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"voxels": n ** 3, "increments": len(gold["rows"]), "wall_s": wall,
+            "increments_per_s": len(gold["rows"]) / wall,
+            "max_rel_diff_macro_stress_vs_recorded": worst, "peak_stress_MPa": peak[0],
+            "peak_strain": peak[1], "d_max_end": float(st.d.max().item()), "stagger_iterations": stagger,
+            "mech_pcg_iterations": pcg, "newton_iterations": newton}
+
+
+def batched_specimens(dev, steps, fixed_tile="", sequential=True, window=0, steps_per_launch=20):
     """Specimen-steps/s of the 64-plate batch: the persistent batched step replayed from a
     captured graph over the whole run; the per-kernel batched path; and, over the same
     steps (the whole run, or the first ``window`` steps when ``window`` is given), the
@@ -544,36 +597,53 @@ def batched_specimens(dev, steps, fixed_tile="", sequential=True, window=0):
         res["autotune_ms_per_step"] = {f"{k[0]},{k[1]}": round(v, 3) for k, v in timings.items()}
         del state
     log(f"strip {best}")
-    # persistent kernel + graph over the full run
-    bat = mk(items, gcs, fext, fixed, best)
-    res["grid_blocks"] = list(bat.prepare_persistent())
+    # persistent kernel + graph over the full run: `steps_per_launch` steps per launch (each
+    # specimen runs ahead on its own blocks within a launch), and one launch per step (every
+    # specimen in lockstep)
     chunk = 20
-    g = bat.capture(chunk)
     reps = max(steps // chunk, 1)
-    bad = torch.zeros((), dtype=torch.int64, device=dev)
-    its = torch.zeros(n_spec, dtype=torch.int64, device=dev)
-    its_max = torch.zeros((), dtype=torch.int64, device=dev)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for rep_i in range(reps):
-        g.replay()
-        if rep_i % 20 == 19:
-            log(f"replayed {(rep_i + 1) * chunk} steps")
-        bad += (g.status != _lib.CG_CONVERGED).sum()
-        its += g.iters.sum(0)
-        its_max = torch.maximum(its_max, g.iters.max().to(torch.int64))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     n_steps = reps * chunk
-    bat.check_persistent(g.status[-1])
-    res.update({"steps": n_steps, "graph_chunk": chunk, "persistent_graph_wall_s": wall,
+
+    def full(spl):
+        bat = mk(items, gcs, fext, fixed, best)
+        grid = list(bat.prepare_persistent())
+        g = bat.capture(chunk, steps_per_launch=spl)
+        bad = torch.zeros((), dtype=torch.int64, device=dev)
+        its = torch.zeros(n_spec, dtype=torch.int64, device=dev)
+        its_max = torch.zeros((), dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for rep_i in range(reps):
+            g.replay()
+            if rep_i % 20 == 19:
+                log(f"{spl} steps per launch: replayed {(rep_i + 1) * chunk} steps")
+            bad += (g.status != _lib.CG_CONVERGED).sum()
+            its += g.iters.sum(0)
+            its_max = torch.maximum(its_max, g.iters.max().to(torch.int64))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        bat.check_persistent(g.status)
+        return bat, g, wall, grid, int(bad.item()), its, int(its_max.item())
+
+    bat, g, wall, grid, bad, its, its_max = full(steps_per_launch)
+    res.update({"steps": n_steps, "graph_chunk": chunk, "steps_per_launch": steps_per_launch, "grid_blocks": grid,
+                "persistent_graph_wall_s": wall,
                 "persistent_graph_specimen_steps_s": n_spec * n_steps / wall,
-                "unconverged_solves": int(bad.item()),
+                "unconverged_solves": bad,
                 "damage_cg_iters_per_step_mean": float(its.sum().item()) / (n_spec * n_steps),
-                "damage_cg_iters_max": int(its_max.item()),
+                "damage_cg_iters_max": its_max,
                 "ms_per_step": wall / n_steps * 1e3,
                 "d_max_end": float(bat.d.max().item())})
     full_run = (bat, g, wall, n_steps)
+    if steps_per_launch != 1:
+        lock, g1, wall1, grid1, bad1, _, _ = full(1)
+        same = all(torch.equal(x, y) for x, y in zip(lock.packed_state(), bat.packed_state()))
+        res.update({"lockstep_grid_blocks": grid1, "lockstep_wall_s": wall1,
+                    "lockstep_specimen_steps_s": n_spec * n_steps / wall1,
+                    "lockstep_unconverged_solves": bad1,
+                    "speedup_run_ahead_vs_lockstep": wall1 / wall,
+                    "lockstep_bitwise_equal_at_end": bool(same)})
+        del lock, g1
     # per-kernel batched path (one launch per kernel over all specimens, host polls the CG)
     short = max(min(100, steps // 10), 10)
     bat = mk(items, gcs, fext, fixed)
@@ -595,18 +665,18 @@ def batched_specimens(dev, steps, fixed_tile="", sequential=True, window=0):
         window = max((window // chunk) * chunk, chunk)
         bat = mk(items, gcs, fext, fixed, best)
         bat.prepare_persistent()
-        g = bat.capture(chunk)
+        g = bat.capture(chunk, steps_per_launch=steps_per_launch)
         g.replay()  # warm the graph, then restart from the initial state
         del g, bat
         bat = mk(items, gcs, fext, fixed, best)
         bat.prepare_persistent()
-        g = bat.capture(chunk)
+        g = bat.capture(chunk, steps_per_launch=steps_per_launch)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         g.replay(window // chunk)
         torch.cuda.synchronize()
         wall_bat = time.perf_counter() - t0
-        bat.check_persistent(g.status[-1])
+        bat.check_persistent(g.status)
     singles = [mk([items[k]], [gcs[k]], [fext[k]], [fixed[k]], best) for k in range(n_spec)]
     singles[0].step(1)  # warm the kernels, then rebuild the first specimen
     singles[0] = mk([items[0]], [gcs[0]], [fext[0]], [fixed[0]], best)

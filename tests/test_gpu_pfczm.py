@@ -1,7 +1,9 @@
 """Parity of the PF-CZM damage law and the crack closure on the voxel cell with the oracle
 (GPU): the operators at 1e-12, patterns bit for bit, and the staggered increments of a
 pore cell through damage onset, peak and softening."""
+import json
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -11,6 +13,8 @@ from oracle import hex8 as O
 from oracle import pfczm as P
 
 pytestmark = pytest.mark.gpu
+
+HISTORY = Path(__file__).with_name("golden") / "pore_in_cube_pfczm_history.json"
 
 
 def rel(a, b):
@@ -165,3 +169,41 @@ def test_pore_cell_increments_through_the_peak(dev):
     print(f"worst relative stress {worst_s:.3e}, worst |d - d_oracle| {worst_d:.3e}", flush=True)
     assert worst_s < 1e-8 and worst_d < 1e-8
     assert float(st.d.max()) > 0.05
+
+
+def test_pore_in_cube_recorded_pfczm_history(dev):
+    """The reference's recorded run of its PF-CZM pore example (48^3 voxels of 0.5 um, pore
+    diameter 0.4 L, E_xx to 3e-3 in 150 increments): every increment's macro stress and
+    balance residual against the recorded history, through onset (~increment 56), the
+    peak (~90) and softening to d_max 0.71."""
+    from phast_amd import solid3d as S
+    from phast_amd import staggered3d as G3
+    from phast_amd import workloads as W
+    gold = json.loads(HISTORY.read_text())
+    c = gold["cell"]
+    n = c["shape"][0]
+    phase, phases, h, ell, load = W.pore_in_cube_pfczm(n, c["increments"], c["to"])
+    assert h[0] == c["voxel_mm"] and ell == c["ell_mm"]
+    mats = S.build_element_arrays(phase, phases)
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid((n, n, n), h), mats, load, kappa=1e-6, ell=ell,
+                                           damage_law="pfczm", crack_closure="axes")
+    cfg = G3.SolverConfig(stagger_tol=c["stagger_tol"], pcg_rtol_mech=c["pcg_rtol_mech"],
+                          damage_kkt_tol=c["damage_kkt_tol"], max_cutbacks=c["max_cutbacks"])
+    solver = G3.StaggeredSolver(op, load, cfg)
+    st = solver.initial_state()
+    worst = 0.0
+    for row in gold["rows"]:
+        k = int(row["step"])
+        st, recs = solver.advance_increment(st, load.targets[k - 1])
+        rec = recs[-1]
+        assert abs(rec.macro_strain[0] - row["macro_strain_xx"]) <= 1e-15
+        err = abs(rec.macro_stress[0] - row["macro_stress_xx"]) / abs(row["macro_stress_xx"])
+        worst = max(worst, err)
+        if k % 10 == 0:
+            print(f"increment {k}: sigma_xx {rec.macro_stress[0]:.9f} (recorded {row['macro_stress_xx']:.9f}, "
+                  f"rel {err:.2e}) balance {rec.balance_residual_rel:.3e} (recorded "
+                  f"{row['balance_residual_rel']:.3e}) d_max {rec.max_damage:.4f}", flush=True)
+        assert err < 1e-9, (k, rec.macro_stress[0], row["macro_stress_xx"])
+        assert math.isclose(rec.balance_residual_rel, row["balance_residual_rel"], rel_tol=1e-5, abs_tol=1e-12), k
+    print(f"worst relative macro stress difference over 150 increments: {worst:.2e}", flush=True)
+    assert float(st.d.max()) > 0.7
