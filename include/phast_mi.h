@@ -495,10 +495,18 @@ int phast_hex8_adj_damage_gc(const phast_hex8_plan *plan, const double *d, const
  *   (host uint8 (E,)) is 1 where a voxel adds (1 - g) sigma_c back (isotropic, with a damage
  *   driver); the plan must be isotropic.  Patterns are uint32 (E,): bit 4 q + k is
  *   [v_k . eps_q < 0] for Gauss point q and rows v = (trace, e_x, e_y, e_z) (splits.py:417-420).
+ * closure_setup_rows: the closure of cells with a general stiffness bank, including the
+ *   smeared-laminate interface band (operator.py:631-690, splits.py:424-742): cl_group (host
+ *   uint8 (E,)) is 0 where a voxel adds nothing back, 1 for the isotropic group, 2 for the
+ *   band group; rows (host, 28 doubles per bank entry) hold the entry's four closure rows
+ *   v_k (4 x 6, engineering-strain contractions) and moduli k_k (4): (trace, e_x, e_y, e_z)
+ *   with (lam, 2 mu, 2 mu, 2 mu) for an isotropic phase, (C m1, q2 + q3, q2, q3) with
+ *   (1/a11, lam_S, 2 mu_S, 2 mu_S) for a band voxel (splits.py:603-670).
  * closure_pattern: newp = the pattern of state x (operator.py:792-806); with old (and g),
  *   out2 (device) = [max k |v . eps| over every bit, max (1 - g) k |v . eps| over the bits that
- *   differ from old, or -1 when none] -- the test of operator.py:808-841 is
- *   out2[1] > tol * out2[0].
+ *   differ from old, or -1 when none] for the isotropic group, followed by the same two
+ *   values for the band group when the plan has one (out2 then holds 4 doubles) -- the test of
+ *   operator.py:808-841 is out2[1] > tol * out2[0] or out2[3] > tol * out2[2].
  * mech_action_cl / stress_sums_cl / mech_diagonal_cl: phast_hex8_mech_action,
  *   phast_hex8_stress_sums and phast_hex8_mech_diagonal with the closure (operator.py:1039-1105,
  *   1196-1300); mode 1: each point's own pattern (the diagonal: every bit), 2: `pat`; g (E,8)
@@ -528,6 +536,7 @@ int phast_hex8_pfczm_box_update(const phast_hex8_plan *plan, const double *W, do
                                 double kkt_tol, int32_t apply, int64_t n, double *grad, int64_t *counts,
                                 hipStream_t stream);
 int phast_hex8_closure_setup(phast_hex8_plan *plan, const uint8_t *cl_on);
+int phast_hex8_closure_setup_rows(phast_hex8_plan *plan, const uint8_t *cl_group, const double *rows);
 int phast_hex8_closure_pattern(const phast_hex8_plan *plan, const double *x, const double *g, const uint32_t *old,
                                uint32_t *newp, double *out2, hipStream_t stream);
 int phast_hex8_mech_action_cl(const phast_hex8_plan *plan, const double *x, const double *g, const uint32_t *pat,

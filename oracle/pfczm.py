@@ -85,15 +85,16 @@ def degradation_derivatives(d, a1, a2, a3, p=2, eta=0.0):
 
 
 class PfczmOperator:
-    """The voxel operator with the PF-CZM law and (optionally) the isotropic crack closure.
+    """The voxel operator with the PF-CZM law and (optionally) the crack closure.
 
     ``base`` carries the geometry, the stiffness bank, ``lam``/``mu``/``Gc``/``driver`` per
     voxel and ``kappa``, ``ell``.  ``E_ref`` and ``f_t`` per voxel give ``a1``
-    (operator.py:317-383).  Every phase is isotropic here, so the closure group is every
-    voxel with a damage driver (operator.py:631-666: pores and driver 'none' add nothing
-    back)."""
+    (operator.py:317-383).  The closure has two groups (operator.py:631-690): the isotropic
+    voxels with a damage driver (rows trace, e_x, e_y, e_z; moduli lam, 2 mu, 2 mu, 2 mu) and,
+    when ``band = (voxel ids, rows (nb, 4, 6), moduli (nb, 4))`` is given, the smeared-laminate
+    voxels with their own rows and moduli; pores and driver 'none' add nothing back."""
 
-    def __init__(self, base: VoxelOperator, E_ref, f_t, *, softening="linear", p=2, closure=True):
+    def __init__(self, base: VoxelOperator, E_ref, f_t, *, softening="linear", p=2, closure=True, band=None):
         self.b = base
         self.kappa, self.ell = base.kappa, base.ell
         self.a2, self.a3 = softening_parameters(softening)
@@ -105,15 +106,28 @@ class PfczmOperator:
         if active.any() and float(self.a1[active].min()) < bound:
             raise ValueError("onset condition a1 >= p + a2 fails")
         _, _, _, B, M, L = reference_voxel(base.h)
+        self.B = B
         self.M, self.L = M, L
         self.m = M.sum(axis=1)                                          # operator.py:321
         self.K_t = (2.0 / PFCZM_C_ALPHA) * self.ell * L                 # operator.py:322
         self.closure = bool(closure)
-        self.cl_on = active.copy()
+        E = base.E
+        group = np.where(active, 1, 0).astype(np.int8)
+        rows = np.broadcast_to(CLOSURE_ROWS, (E, 4, 6)).copy()
         lam = np.where(active, base.lam, 0.0)
         mu = np.where(active, base.mu, 0.0)
-        self.cl_k = np.stack((lam, 2.0 * mu, 2.0 * mu, 2.0 * mu), axis=-1)    # splits.py:498-503
-        self.BtR = np.einsum("qia,ki->qka", B, CLOSURE_ROWS)                   # operator.py:666
+        k = np.stack((lam, 2.0 * mu, 2.0 * mu, 2.0 * mu), axis=-1)     # splits.py:498-503
+        if band is not None:
+            ids, b_rows, b_k = band
+            ids = np.asarray(ids, dtype=np.int64)
+            on = active[ids]
+            group[ids] = np.where(on, 2, 0)
+            rows[ids] = b_rows
+            k[ids] = np.where(on[:, None], b_k, 0.0)
+        self.cl_group = group
+        self.cl_on = group != 0
+        self.cl_rows = rows
+        self.cl_k = k
 
     # -- law ------------------------------------------------------------------------
     def derivatives(self, d):
@@ -126,8 +140,8 @@ class PfczmOperator:
 
     # -- mechanics with the closure add-back ----------------------------------------
     def closure_values(self, x):
-        """``v . eps`` (E, 8, 4) (operator.py:786-790)."""
-        return self.b.qp_strain(x) @ CLOSURE_ROWS.T
+        """``v . eps`` (E, 8, 4) with each voxel's rows (operator.py:786-816)."""
+        return np.einsum("eqm,ekm->eqk", self.b.qp_strain(x), self.cl_rows)
 
     def stress_pass(self, x, g=None, pattern=None, assemble=True):
         """operator.py:1039-1105: ``g C eps + (1 - g) sigma_c`` (own pattern when ``pattern``
@@ -138,10 +152,10 @@ class PfczmOperator:
         if g is not None:
             sigma = g[..., None] * sigma
             if self.closure:
-                values = eps @ CLOSURE_ROWS.T
+                values = np.einsum("eqm,ekm->eqk", eps, self.cl_rows)
                 active = values < 0.0 if pattern is None else pattern
                 coef = np.where(active, values, 0.0) * self.cl_k[:, None, :]
-                sigma = sigma + (1.0 - g)[..., None] * (coef @ CLOSURE_ROWS)
+                sigma = sigma + (1.0 - g)[..., None] * np.einsum("eqk,ekm->eqm", coef, self.cl_rows)
         weighted = sigma * b.w[None, :, None]
         r_E = weighted.sum(axis=(0, 1))
         energy = 0.5 * (weighted * eps).sum()
@@ -160,7 +174,8 @@ class PfczmOperator:
 
     def mech_diagonal(self, g=None, pattern=None):
         """operator.py:1196-1300: the plain Jacobi diagonal plus, for every closure bit
-        (all bits when ``pattern`` is None), ``sum_q w (1 - g) k (B_q^T v)^2``."""
+        (all bits when ``pattern`` is None), ``sum_q w (1 - g) k (B_q^T v)^2`` and, in the
+        macro rows, ``sum_q w (1 - g) k v_i^2``."""
         diag = self.b.mech_diagonal(g)
         if g is None or not self.closure:
             return diag
@@ -168,9 +183,13 @@ class PfczmOperator:
         coef = (1.0 - g)[..., None] * self.cl_k[:, None, :] * self.b.w[None, :, None]
         if pattern is not None:
             coef = coef * pattern
-        local = np.einsum("eqk,qka->ea", coef, self.BtR ** 2)
+        local = np.empty((self.b.E, 24))
+        for s0 in range(0, self.b.E, 4096):
+            sl = slice(s0, s0 + 4096)
+            BtR = np.einsum("qia,eki->eqka", self.B, self.cl_rows[sl])
+            local[sl] = np.einsum("eqk,eqka->ea", coef[sl], BtR ** 2)
         diag[:n3] += scatter(self.b.dofs, local, n3)
-        diag[n3:] += np.einsum("ek,ki->i", coef.sum(axis=1), CLOSURE_ROWS ** 2)
+        diag[n3:] += np.einsum("eqk,eki->i", coef, self.cl_rows ** 2)
         return diag
 
     def closure_pattern(self, x):
@@ -178,16 +197,23 @@ class PfczmOperator:
         return (self.closure_values(x) < 0.0) & self.cl_on[:, None, None]
 
     def closure_pattern_change(self, x, pattern, g, tolerance=PATTERN_TOLERANCE):
-        """operator.py:808-841: the new pattern and whether a flipped bit moves the stress by
-        more than ``tolerance`` times the group's largest ``k |v . eps|``."""
+        """operator.py:808-841: the new pattern and whether, in either group, a flipped bit
+        moves the stress by more than ``tolerance`` times that group's largest
+        ``k |v . eps|``."""
         values = self.closure_values(x)
         new = (values < 0.0) & self.cl_on[:, None, None]
         strength = self.cl_k[:, None, :] * np.abs(values)
-        flipped = new != pattern
         changed = False
-        if flipped.any():
-            jump = (1.0 - g)[..., None] * strength
-            changed = float(jump[flipped].max()) > tolerance * float(strength.max())
+        for grp in (1, 2):
+            sel = self.cl_group == grp
+            if grp == 1:
+                sel = np.ones_like(sel) if not (self.cl_group == 2).any() else self.cl_group != 2
+            if not sel.any():
+                continue
+            flipped = new[sel] != pattern[sel]
+            if not changed and flipped.any():
+                jump = (1.0 - g[sel])[..., None] * strength[sel]
+                changed = bool(float(jump[flipped].max()) > tolerance * float(strength[sel].max()))
         return new, changed
 
     # -- PF-CZM damage --------------------------------------------------------------

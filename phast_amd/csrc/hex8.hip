@@ -42,6 +42,7 @@ struct MechArgs3 {
     double *seam;          // voxel-parallel apply: top-face sums of each slab's last voxel layer
     const uint8_t *clm;    // crack closure: 1 where the voxel adds (1 - g) sigma_c back
     const uint32_t *pat;   // crack closure: fixed sign pattern, bit 4 q + k per voxel
+    const double *cltab;   // crack closure, general banks: rows and moduli per bank entry (28)
     int32_t kz, tiles_x, tiles_y, nvals;
 };
 
@@ -289,6 +290,8 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
             if (CL == 2) pw = A.pat[e];
         } else {
             b = A.bank_index[e];
+            if (CL != 0) l0 = A.clm[e] != 0 ? 1.0 : 0.0;
+            if (CL == 2) pw = A.pat[e];
         }
     };
     fetch_mat(k0, ml0, ml1, mb, mp);
@@ -348,7 +351,9 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(c
                 const double lc = (CL != 0 && mb) ? ml0 : 0.0, mc = (CL != 0 && mb) ? ml1 : 0.0;
                 voxel_mech<true, ENERGY, CL>(G, u, E, g, nullptr, ml0, ml1, owner, r, sE, en, lc, mc, mp);
             } else {
-                voxel_mech<false, ENERGY>(G, u, E, g, A.bank + 36 * (int64_t)mb, 0.0, 0.0, owner, r, sE, en);
+                voxel_mech<false, ENERGY, CL>(G, u, E, g, A.bank + 36 * (int64_t)mb, 0.0, 0.0, owner, r, sE, en,
+                                              0.0, 0.0, mp, CL != 0 ? A.cltab + 28 * (int64_t)mb : nullptr,
+                                              ml0 != 0.0);
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -509,11 +514,15 @@ __global__ __launch_bounds__(256) void hex8_degradation_kernel(const Hex8Geo G, 
 // closure (clmode 1: every bit, 2: the bits of `pat`; isotropic voxels) each closure bit k of
 // Gauss point q adds w (1 - g_q) k_k (B_q^T v_k)^2 (operator.py:1255-1300): for dof c of a
 // node, (d N / d x_c)^2 (s_0 lam + s_{1+c} 2 mu), and s_0 lam + s_{1+i} 2 mu to macro row i < 3.
+//
+// General banks (cltab): the rows v_k and moduli k_k of the voxel's bank entry, with
+// (B_q^T v)_x = bx v_0 + bz v_4 + by v_5, (B_q^T v)_y = by v_1 + bz v_3 + bx v_5,
+// (B_q^T v)_z = bz v_2 + by v_3 + bx v_4, and k_k v_i^2 for each macro row i.
 __global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const double *g, const double *bank,
                                                         const int32_t *bank_index, const double *lam,
                                                         const double *mu, int iso, double *diag, double *part,
                                                         const uint8_t *clm = nullptr, const uint32_t *pat = nullptr,
-                                                        int clmode = 0) {
+                                                        int clmode = 0, const double *cltab = nullptr) {
     __shared__ double red[4];
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
     const int64_t node = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -553,10 +562,25 @@ __global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const d
                 if (clmode != 0 && clm[e]) {
                     const uint32_t bits = clmode == 2 ? (pat[e] >> (4 * q)) & 15u : 15u;
                     const double cw = (1.0 - g[e * 8 + q]) * G.w;
-                    const double l0 = (bits & 1u) ? lam[e] : 0.0, m2 = 2.0 * mu[e];
-                    acc[0] += cw * (bx * bx) * (l0 + ((bits & 2u) ? m2 : 0.0));
-                    acc[1] += cw * (by * by) * (l0 + ((bits & 4u) ? m2 : 0.0));
-                    acc[2] += cw * (bz * bz) * (l0 + ((bits & 8u) ? m2 : 0.0));
+                    if (cltab) {
+                        const double *T = cltab + 28 * (int64_t)bank_index[e];
+                        for (int b = 0; b < 4; ++b) {
+                            if (!((bits >> b) & 1u)) continue;
+                            const double *v = T + 6 * b;
+                            const double px = (bx * v[0] + bz * v[4]) + by * v[5];
+                            const double py = (by * v[1] + bz * v[3]) + bx * v[5];
+                            const double pz = (bz * v[2] + by * v[3]) + bx * v[4];
+                            const double ck = cw * T[24 + b];
+                            acc[0] += ck * (px * px);
+                            acc[1] += ck * (py * py);
+                            acc[2] += ck * (pz * pz);
+                        }
+                    } else {
+                        const double l0 = (bits & 1u) ? lam[e] : 0.0, m2 = 2.0 * mu[e];
+                        acc[0] += cw * (bx * bx) * (l0 + ((bits & 2u) ? m2 : 0.0));
+                        acc[1] += cw * (by * by) * (l0 + ((bits & 4u) ? m2 : 0.0));
+                        acc[2] += cw * (bz * bz) * (l0 + ((bits & 8u) ? m2 : 0.0));
+                    }
                 }
             }
         }
@@ -574,10 +598,20 @@ __global__ __launch_bounds__(256) void hex8_diag_kernel(const Hex8Geo G, const d
             for (int q = 0; q < 8; ++q) {
                 const uint32_t bits = clmode == 2 ? (pat[node] >> (4 * q)) & 15u : 15u;
                 const double cw = (1.0 - g[node * 8 + q]) * G.w;
-                const double l0 = (bits & 1u) ? lam[node] : 0.0, m2 = 2.0 * mu[node];
-                dE[0] += cw * (l0 + ((bits & 2u) ? m2 : 0.0));
-                dE[1] += cw * (l0 + ((bits & 4u) ? m2 : 0.0));
-                dE[2] += cw * (l0 + ((bits & 8u) ? m2 : 0.0));
+                if (cltab) {
+                    const double *T = cltab + 28 * (int64_t)bank_index[node];
+                    for (int b = 0; b < 4; ++b) {
+                        if (!((bits >> b) & 1u)) continue;
+                        const double ck = cw * T[24 + b];
+#pragma unroll
+                        for (int k2 = 0; k2 < 6; ++k2) dE[k2] += ck * (T[6 * b + k2] * T[6 * b + k2]);
+                    }
+                } else {
+                    const double l0 = (bits & 1u) ? lam[node] : 0.0, m2 = 2.0 * mu[node];
+                    dE[0] += cw * (l0 + ((bits & 2u) ? m2 : 0.0));
+                    dE[1] += cw * (l0 + ((bits & 4u) ? m2 : 0.0));
+                    dE[2] += cw * (l0 + ((bits & 8u) ? m2 : 0.0));
+                }
             }
         }
     }
@@ -826,14 +860,23 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
     A.seam = pl->seam;
     dim3 grid(nb), block(apply_threads<OX, OY>());
 #define PHAST_L(GM, ISO, EN) hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, GM, ISO, EN>), grid, block, 0, st, pl->geo, A)
-#define PHAST_LC(EN, CL) hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, 1, true, EN, CL>), grid, block, 0, st, pl->geo, A)
-    if (cl != 0) {  // crack closure: g array, isotropic voxels (checked by the caller)
+#define PHAST_LC(ISO, EN, CL) \
+    hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, 1, ISO, EN, CL>), grid, block, 0, st, pl->geo, A)
+    if (cl != 0 && pl->iso) {  // crack closure: g array (checked by the caller)
         if (energy) {
-            if (cl == 1) PHAST_LC(true, 1);
-            else PHAST_LC(true, 2);
+            if (cl == 1) PHAST_LC(true, true, 1);
+            else PHAST_LC(true, true, 2);
         } else {
-            if (cl == 1) PHAST_LC(false, 1);
-            else PHAST_LC(false, 2);
+            if (cl == 1) PHAST_LC(true, false, 1);
+            else PHAST_LC(true, false, 2);
+        }
+    } else if (cl != 0) {  // general bank: the closure rows of each bank entry
+        if (energy) {
+            if (cl == 1) PHAST_LC(false, true, 1);
+            else PHAST_LC(false, true, 2);
+        } else {
+            if (cl == 1) PHAST_LC(false, false, 1);
+            else PHAST_LC(false, false, 2);
         }
     } else if (pl->iso) {
         if (energy) {
@@ -891,6 +934,7 @@ static int hex8_apply_any(const phast_hex8_plan *pl, const double *x, const doub
     A.kz = pl->kz;
     A.clm = pl->cl_on;
     A.pat = pat;
+    A.cltab = pl->cl_tab;
     const int gmode = d ? 2 : (g ? 1 : 0);
     int nb = 0, rc;
     if (!y) {  // stress sums only: still needs a write target for the nodal rows
@@ -898,9 +942,9 @@ static int hex8_apply_any(const phast_hex8_plan *pl, const double *x, const doub
         return PHAST_ERR_ARG;
     }
     if (cl != 0) {
-        if (!pl->cl_on || gmode != 1 || !pl->iso || (cl == 2 && !pat) || jac) {
-            set_error("internal: the closure apply needs the closure set-up, g (E,8), isotropic voxels and a "
-                      "pattern when fixed");
+        if (!pl->cl_on || gmode != 1 || (!pl->iso && !pl->cl_tab) || (cl == 2 && !pat) || jac) {
+            set_error("internal: the closure apply needs the closure set-up (with the rows of a general bank), "
+                      "g (E,8) and a pattern when fixed");
             return PHAST_ERR_ARG;
         }
         if (!((pl->tx == 16 && (pl->ty == 16 || pl->ty == 14)) || (pl->tx == 32 && (pl->ty == 8 || pl->ty == 14 ||
@@ -1228,6 +1272,7 @@ extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
     (void)hipFree(pl->seam);
     (void)hipFree(pl->a1);
     (void)hipFree(pl->cl_on);
+    (void)hipFree(pl->cl_tab);
     release_solver_work(pl);
     delete pl;
     return PHAST_OK;
@@ -1306,10 +1351,11 @@ extern "C" int phast_hex8_mech_diagonal(const phast_hex8_plan *pl, const double 
 extern "C" int phast_hex8_mech_diagonal_cl(const phast_hex8_plan *pl, const double *g, const uint32_t *pat,
                                            int32_t mode, double *diag, hipStream_t st) {
     PHAST_REQUIRE(pl && g && diag && (mode == 1 || (mode == 2 && pat)), "phast_hex8_mech_diagonal_cl: bad argument");
-    PHAST_REQUIRE(pl->cl_on && pl->iso, "phast_hex8_mech_diagonal_cl: the plan has no crack closure");
+    PHAST_REQUIRE(pl->cl_on && (pl->iso || pl->cl_tab), "phast_hex8_mech_diagonal_cl: the plan has no crack closure");
     const int nb = (int)ceil_div(pl->n, 256);
     hipLaunchKernelGGL(hex8_diag_kernel, dim3(nb), dim3(256), 0, st, pl->geo, g, pl->bank, pl->bank_index, pl->lam,
-                       pl->mu, 1, diag, pl->part, (const uint8_t *)pl->cl_on, pat, (int)mode);
+                       pl->mu, pl->iso ? 1 : 0, diag, pl->part, (const uint8_t *)pl->cl_on, pat, (int)mode,
+                       pl->iso ? (const double *)nullptr : (const double *)pl->cl_tab);
     PHAST_LAUNCHED();
     hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, pl->part, nb, 6, diag + 3 * pl->n,
                        (double *)nullptr);

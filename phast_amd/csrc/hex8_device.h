@@ -170,17 +170,19 @@ __device__ __forceinline__ void eig3(double a11, double a22, double a33, double 
 // (operator.py:1108-1169).  With OWN, sE += sum_q w sigma_q and (ENERGY) en += 0.5 sum_q
 // w sigma.eps.  sum-factorised gradients; g w folded into the moduli.
 //
-// Crack closure (CL != 0, isotropic voxels only; operator.py:1039-1105, splits.py:424-560):
-// sigma += (1 - g) sum_k s_k k_k (v_k . eps) v_k over the rows v = (1, e_x, e_y, e_z) with
-// moduli (lamc, 2 muc, 2 muc, 2 muc) -- zero where the voxel adds nothing back.  The bits
-// s_k are the signs [v_k . eps < 0] of the voxel's own strain (CL == 1) or bit 4 q + k of
-// `pat` (CL == 2).
+// Crack closure (CL != 0; operator.py:1039-1105, splits.py:424-742):
+// sigma += (1 - g) sum_k s_k k_k (v_k . eps) v_k.  Isotropic cells (ISO): the rows
+// v = (1, e_x, e_y, e_z) with moduli (lamc, 2 muc, 2 muc, 2 muc) -- zero where the voxel adds
+// nothing back.  General banks: the four rows and moduli of the voxel's bank entry in
+// T[0..23], T[24..27] (the cell axes of an isotropic phase or the frame of a laminate band),
+// where `clon`.  The bits s_k are the signs [v_k . eps < 0] of the voxel's own strain
+// (CL == 1) or bit 4 q + k of `pat` (CL == 2).
 template <bool ISO, bool ENERGY, int CL = 0>
 __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8], const double E[6],
                                            const double g[8], const double *__restrict__ C, double lam,
                                            double mu, bool own, double r[3][8], double sE[6], double &en,
-                                           double lamc = 0.0, double muc = 0.0, uint32_t pat = 0u) {
-    static_assert(CL == 0 || ISO, "the closure add-back is built for isotropic voxels");
+                                           double lamc = 0.0, double muc = 0.0, uint32_t pat = 0u,
+                                           const double *__restrict__ T = nullptr, bool clon = false) {
     double gx[3][2][2], gy[3][2][2], gz[3][2][2];
     voxel_grad(G, u, gx, gy, gz);
     fold_macro(E, gx, gy, gz);
@@ -235,6 +237,22 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
 #pragma unroll
                         for (int m = 0; m < 6; ++m) t += C[k * 6 + m] * e[m];
                         s[k] = gw * t;
+                    }
+                    if (CL != 0 && clon) {
+                        const int q = ix + 2 * iy + 4 * iz;
+                        const double cw = (1.0 - g[q]) * G.w;
+                        double c4[4];
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            double v = 0.0;
+#pragma unroll
+                            for (int m = 0; m < 6; ++m) v += T[6 * b + m] * e[m];
+                            const bool act = CL == 1 ? v < 0.0 : ((pat >> (4 * q + b)) & 1u) != 0u;
+                            c4[b] = act ? T[24 + b] * v : 0.0;
+                        }
+#pragma unroll
+                        for (int m = 0; m < 6; ++m)
+                            s[m] += cw * (((c4[0] * T[m] + c4[1] * T[6 + m]) + c4[2] * T[12 + m]) + c4[3] * T[18 + m]);
                     }
                 }
                 if (ENERGY) {

@@ -56,8 +56,13 @@ struct phast_hex8_plan {
     double *a1 = nullptr;
     phast::PfczmLaw law{};
     phast::Hex8Geo geo_t{};
-    // crack closure (phast_hex8_closure_setup): 1 where a voxel adds (1 - g) sigma_c back
+    // crack closure (phast_hex8_closure_setup / _rows): the group of each voxel (0: adds
+    // nothing back, 1: isotropic group, 2: laminate band group) and, for cells with a general
+    // stiffness bank, the closure rows of each bank entry (28 doubles: rows v_k (4 x 6),
+    // moduli k_k (4))
     uint8_t *cl_on = nullptr;
+    double *cl_tab = nullptr;
+    bool cl_band = false;  // some voxel is in the band group
 };
 
 namespace phast {

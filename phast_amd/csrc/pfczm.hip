@@ -198,16 +198,20 @@ __global__ __launch_bounds__(256) void pfczm_weights_kernel(const Hex8Geo G, con
 }
 
 // Closure sign pattern of state x (operator.py:786-841): bit 4 q + k of newp[e] is
-// [v_k . eps_q < 0] on closure voxels (rows: trace, e_x, e_y, e_z).  With `old` (and g):
-// block maxima of k_k |v_k . eps| (all bits) and of (1 - g) k_k |v_k . eps| over the bits
-// that differ from `old` (-1 when none) to part[8 b + 0..1].
+// [v_k . eps_q < 0] on closure voxels -- rows trace, e_x, e_y, e_z with moduli (lam, 2 mu,
+// 2 mu, 2 mu) on isotropic cells, or the rows and moduli of the voxel's bank entry in
+// `tab` (28 doubles each) on cells with a general bank.  With `old` (and g): per closure
+// group (clm 1: isotropic, 2: laminate band), block maxima of k_k |v_k . eps| (all bits)
+// and of (1 - g) k_k |v_k . eps| over the bits that differ from `old` (-1 when none), to
+// part[8 b + 2 (group - 1) + 0..1].
 __global__ __launch_bounds__(256) void closure_scan_kernel(const Hex8Geo G, const double *x, const double *g,
                                                            const double *lam, const double *mu, const uint8_t *clm,
+                                                           const double *tab, const int32_t *bank_index,
                                                            const uint32_t *old, uint32_t *newp, double *part) {
-    __shared__ double red[2][4];
+    __shared__ double red[4][4];
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    double smax = 0.0, jmax = -1.0;
+    double smax[2] = {0.0, 0.0}, jmax[2] = {-1.0, -1.0};
     if (e < n) {
         const int i = (int)(e % G.nx), j = (int)((e / G.nx) % G.ny), k = (int)(e / ((int64_t)G.nx * G.ny));
         double u[3][8], E[6];
@@ -224,41 +228,69 @@ __global__ __launch_bounds__(256) void closure_scan_kernel(const Hex8Geo G, cons
         double gx[3][2][2], gy[3][2][2], gz[3][2][2];
         voxel_grad(G, u, gx, gy, gz);
         fold_macro(E, gx, gy, gz);
-        const bool on = clm[e] != 0;
-        const double kl = on ? lam[e] : 0.0, km = on ? 2.0 * mu[e] : 0.0;
+        const int grp = clm[e];
+        const bool on = grp != 0;
+        const int gi = grp == 2 ? 1 : 0;
+        const double *T = tab ? tab + 28 * (int64_t)bank_index[e] : nullptr;
+        double kk[4];
+        if (T) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) kk[b] = on ? T[24 + b] : 0.0;
+        } else {
+            const double kl = on ? lam[e] : 0.0, km = on ? 2.0 * mu[e] : 0.0;
+            kk[0] = kl;
+            kk[1] = kk[2] = kk[3] = km;
+        }
         const uint32_t ob = old ? old[e] : 0u;
         uint32_t bits = 0u;
+        double sm = 0.0, jm = -1.0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int ix = q & 1, iy = (q >> 1) & 1, iz = q >> 2;
             double s[6];
             qp_strain(gx, gy, gz, ix, iy, iz, s);
-            const double v[4] = {(s[0] + s[1]) + s[2], s[0], s[1], s[2]};
-            const double kk[4] = {kl, km, km, km};
+            double v[4];
+            if (T) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 6; ++m) t += T[6 * b + m] * s[m];
+                    v[b] = t;
+                }
+            } else {
+                v[0] = (s[0] + s[1]) + s[2];
+                v[1] = s[0];
+                v[2] = s[1];
+                v[3] = s[2];
+            }
             const double omg = old ? 1.0 - g[e * 8 + q] : 0.0;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const uint32_t bit = (on && v[b] < 0.0) ? 1u : 0u;
                 bits |= bit << (4 * q + b);
                 const double strength = kk[b] * fabs(v[b]);
-                smax = fmax(smax, strength);
-                if (old && (((ob >> (4 * q + b)) & 1u) != bit)) jmax = fmax(jmax, omg * strength);
+                sm = fmax(sm, strength);
+                if (old && (((ob >> (4 * q + b)) & 1u) != bit)) jm = fmax(jm, omg * strength);
             }
         }
+        smax[gi] = sm;
+        jmax[gi] = jm;
         newp[e] = bits;
     }
-    smax = wave_max(smax);
-    jmax = wave_max(jmax);
     const int lane = threadIdx.x & 63, w = threadIdx.x / 64;
-    if (lane == 0) {
-        red[0][w] = smax;
-        red[1][w] = jmax;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const double a = wave_max(smax[c]), b = wave_max(jmax[c]);
+        if (lane == 0) {
+            red[2 * c][w] = a;
+            red[2 * c + 1][w] = b;
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        part[8 * blockIdx.x] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
-        part[8 * blockIdx.x + 1] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
-    }
+    if (threadIdx.x < 4)
+        part[8 * blockIdx.x + threadIdx.x] = fmax(fmax(red[threadIdx.x][0], red[threadIdx.x][1]),
+                                                  fmax(red[threadIdx.x][2], red[threadIdx.x][3]));
 }
 
 // One block: out[k] = sum (op 0) or max (op 1) of part[8 b + k] over b < nb, in a fixed order.
@@ -334,13 +366,41 @@ extern "C" int phast_hex8_pfczm_setup(phast_hex8_plan *pl, const double *a1, dou
 
 extern "C" int phast_hex8_closure_setup(phast_hex8_plan *pl, const uint8_t *cl_on) {
     PHAST_REQUIRE(pl && cl_on, "phast_hex8_closure_setup: null argument");
-    PHAST_REQUIRE(pl->iso, "crack closure 'axes' is built for cells whose every stiffness is isotropic");
+    PHAST_REQUIRE(pl->iso, "phast_hex8_closure_setup is for cells whose every stiffness is isotropic; a general "
+                  "bank needs phast_hex8_closure_setup_rows");
     for (int64_t e = 0; e < pl->n; ++e)
         PHAST_REQUIRE(cl_on[e] <= 1, "phast_hex8_closure_setup: cl_on[%lld] must be 0 or 1", (long long)e);
     uint8_t *dev = pl->cl_on;
     if (!dev) PHAST_HIP(hipMalloc(&dev, (size_t)pl->n));
     pl->cl_on = dev;
     PHAST_HIP(hipMemcpy(dev, cl_on, (size_t)pl->n, hipMemcpyHostToDevice));
+    pl->cl_band = false;
+    return PHAST_OK;
+}
+
+extern "C" int phast_hex8_closure_setup_rows(phast_hex8_plan *pl, const uint8_t *cl_group, const double *rows) {
+    PHAST_REQUIRE(pl && cl_group && rows, "phast_hex8_closure_setup_rows: null argument");
+    bool band = false;
+    for (int64_t e = 0; e < pl->n; ++e) {
+        PHAST_REQUIRE(cl_group[e] <= 2, "phast_hex8_closure_setup_rows: cl_group[%lld] must be 0, 1 or 2",
+                      (long long)e);
+        band = band || cl_group[e] == 2;
+    }
+    for (int64_t k = 0; k < 28 * (int64_t)pl->n_bank; ++k)
+        PHAST_REQUIRE(std::isfinite(rows[k]), "phast_hex8_closure_setup_rows: rows[%lld] is not finite", (long long)k);
+    for (int32_t b = 0; b < pl->n_bank; ++b)
+        for (int k = 0; k < 4; ++k)
+            PHAST_REQUIRE(rows[28 * (int64_t)b + 24 + k] >= 0.0,
+                          "phast_hex8_closure_setup_rows: modulus %d of bank entry %d is negative", k, b);
+    uint8_t *dev = pl->cl_on;
+    if (!dev) PHAST_HIP(hipMalloc(&dev, (size_t)pl->n));
+    pl->cl_on = dev;
+    PHAST_HIP(hipMemcpy(dev, cl_group, (size_t)pl->n, hipMemcpyHostToDevice));
+    double *tab = pl->cl_tab;
+    if (!tab) PHAST_HIP(hipMalloc(&tab, sizeof(double) * 28 * (size_t)pl->n_bank));
+    pl->cl_tab = tab;
+    PHAST_HIP(hipMemcpy(tab, rows, sizeof(double) * 28 * (size_t)pl->n_bank, hipMemcpyHostToDevice));
+    pl->cl_band = band;
     return PHAST_OK;
 }
 
@@ -407,11 +467,13 @@ extern "C" int phast_hex8_closure_pattern(const phast_hex8_plan *pl, const doubl
     PHAST_REQUIRE(pl->cl_on, "phast_hex8_closure_pattern: the plan has no crack closure (phast_hex8_closure_setup)");
     PHAST_REQUIRE(!old || (g && out2), "phast_hex8_closure_pattern: a pattern change needs g and out2");
     const int nb = (int)ceil_div(pl->n, 256);
+    const double *tab = pl->iso ? nullptr : pl->cl_tab;
+    PHAST_REQUIRE(pl->iso || tab, "phast_hex8_closure_pattern: a general bank needs phast_hex8_closure_setup_rows");
     hipLaunchKernelGGL(closure_scan_kernel, dim3(nb), dim3(256), 0, st, pl->geo, x, g, pl->lam, pl->mu, pl->cl_on,
-                       old, newp, pl->part);
+                       tab, pl->bank_index, old, newp, pl->part);
     PHAST_LAUNCHED();
     if (out2) {
-        hipLaunchKernelGGL(partials_kernel, dim3(1), dim3(256), 0, st, pl->part, nb, 2, 1, out2);
+        hipLaunchKernelGGL(partials_kernel, dim3(1), dim3(256), 0, st, pl->part, nb, pl->cl_band ? 4 : 2, 1, out2);
         PHAST_LAUNCHED();
     }
     return PHAST_OK;

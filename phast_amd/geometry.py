@@ -100,3 +100,72 @@ def polydisperse_particles(shape, h, r_range, max_fraction: float, seed: int, sh
         phase[(d < r + 0.5 * shell) & (phase == MATRIX)] = INTERFACE
         phase[d < r - 0.5 * shell] = INCLUSION
     return phase, np.array(centres), np.array(radii)
+
+
+def place_particles(lengths, n: int, radius: float, min_gap: float, seed: int, max_tries: int = 200000):
+    """Random sequential addition of ``n`` spheres of ``radius`` in a periodic box: draws
+    ``default_rng(seed).random(3) * lengths`` and keeps a centre whose minimum-image distance
+    to every kept centre is at least ``2 radius + min_gap`` (geometry.py:30-42).  Returns the
+    (n, 3) centres."""
+    rng = np.random.default_rng(seed)
+    box = np.asarray(lengths, dtype=np.float64)
+    d_min = 2.0 * radius + min_gap
+    kept = []
+    for _ in range(max_tries):
+        c = rng.random(3) * box
+        ok = True
+        for o in kept:
+            gap = np.abs(c - o)
+            if np.sqrt((np.minimum(gap, box - gap) ** 2).sum()) < d_min:
+                ok = False
+                break
+        if ok:
+            kept.append(c)
+            if len(kept) == n:
+                return np.array(kept)
+    raise RuntimeError(f"could not place {n} non-overlapping particles (placed {len(kept)}); lower n or the radius")
+
+
+def nearest_centre(shape, h, centres):
+    """Minimum-image distance from every voxel centre to the nearest of ``centres`` and the
+    unit vector pointing away from it (0 at a centre), (nz, ny, nx) and (nz, ny, nx, 3)
+    (geometry.py:45-63; the first centre wins a tie)."""
+    nx, ny, nz = shape
+    L = (nx * h[0], ny * h[1], nz * h[2])
+    x, y, z = voxel_centres(shape, h)
+    best = np.full((nz, ny, nx), np.inf)
+    vec = np.zeros((nz, ny, nx, 3))
+    for c in centres:
+        comps = []
+        for v, cc, LL, sl in ((x, c[0], L[0], (None, None, slice(None))), (y, c[1], L[1], (None, slice(None), None)),
+                              (z, c[2], L[2], (slice(None), None, None))):
+            dv = v - cc
+            dv = dv - LL * np.round(dv / LL)
+            comps.append(np.broadcast_to(dv[sl], (nz, ny, nx)))
+        d = np.sqrt(comps[0] ** 2 + comps[1] ** 2 + comps[2] ** 2)
+        closer = d < best
+        best = np.where(closer, d, best)
+        for k in range(3):
+            vec[..., k] = np.where(closer, comps[k], vec[..., k])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        unit = np.where(best[..., None] > 0, vec / best[..., None], 0.0)
+    return best, unit
+
+
+def particles_with_shell(shape, h, n: int, radius: float, shell: float, gap: float, seed: int):
+    """``n`` spherical particles of radius ``radius`` wrapped in an interface shell of
+    thickness ``shell`` straddling the surface, placed by :func:`place_particles` with the
+    outer radius ``radius + shell / 2`` and the minimum gap ``gap`` (lengths in mm;
+    geometry.py:67-81).  Returns (phase (nz, ny, nx) uint8, tangent (nz, ny, nx, 3): the outward
+    radial direction on the shell, (0, 0, 1) elsewhere, centres (n, 3))."""
+    nx, ny, nz = shape
+    centres = place_particles((nx * h[0], ny * h[1], nz * h[2]), n, radius + 0.5 * shell, gap, seed)
+    dist, unit = nearest_centre(shape, h, centres)
+    phase = np.full((nz, ny, nx), MATRIX, dtype=np.uint8)
+    phase[dist < radius + 0.5 * shell] = INTERFACE
+    phase[dist < radius - 0.5 * shell] = INCLUSION
+    tangent = np.zeros((nz, ny, nx, 3))
+    tangent[..., 2] = 1.0
+    on = phase == INTERFACE
+    tangent[on] = unit[on]
+    return phase, tangent, centres

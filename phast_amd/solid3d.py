@@ -54,6 +54,7 @@ for _name, _args in {
     "phast_hex8_pfczm_box_update": [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7 +
                                    [C.c_double, C.c_int32, C.c_int64, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
     "phast_hex8_closure_setup": [C.c_void_p, C.c_void_p],
+    "phast_hex8_closure_setup_rows": [C.c_void_p] * 3,
     "phast_hex8_closure_pattern": [C.c_void_p] * 7,
     "phast_hex8_mech_action_cl": [C.c_void_p] * 4 + [C.c_int32, C.c_void_p, C.c_void_p],
     "phast_hex8_stress_sums_cl": [C.c_void_p] * 4 + [C.c_int32, C.c_void_p, C.c_void_p],
@@ -154,14 +155,30 @@ class VoxelGrid:
 
 @dataclass(frozen=True)
 class PhaseMaterial:
-    """Isotropic phase (solid3d/model.py PhaseMaterial, isotropic kind); ``tensile_strength``
-    f_t is needed by the PF-CZM law only."""
+    """One phase (solid3d/model.py:340-420 PhaseMaterial): ``kind`` 'isotropic', or
+    'smeared_laminate' -- isotropic ``layers`` ``(fraction, E, nu)`` stacked along the voxel's
+    tangent, with ``E``/``nu`` the isotropic equivalent that only the damage driver and the
+    PF-CZM ``E_ref`` use.  ``tensile_strength`` f_t is needed by the PF-CZM law only."""
     name: str
     E: float
     nu: float
     Gc: float
     driver: str = "spectral"
     tensile_strength: float | None = None
+    kind: str = "isotropic"
+    layers: tuple = ()
+
+    def __post_init__(self):
+        if self.kind not in ("isotropic", "smeared_laminate"):
+            raise ValueError(f"{self.name}: kind must be 'isotropic' or 'smeared_laminate', got {self.kind!r}")
+        if self.kind == "smeared_laminate" and not self.layers:
+            raise ValueError(f"{self.name}: smeared_laminate phases need layers")
+        if self.kind == "isotropic" and self.layers:
+            raise ValueError(f"{self.name}: layers apply to smeared_laminate phases only")
+
+    @property
+    def is_laminate(self) -> bool:
+        return self.kind == "smeared_laminate"
 
 
 def isotropic_stiffness_3d(E: float, nu: float) -> np.ndarray:
@@ -177,6 +194,104 @@ def isotropic_stiffness_3d(E: float, nu: float) -> np.ndarray:
     return C6
 
 
+# Voigt order (xx, yy, zz, yz, xz, xy); engineering strains carry doubled shears.
+_VOIGT_PAIRS = ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))
+LAMINATE_NORMAL = (0, 4, 5)     # stress components continuous across layers stacked along local axis 1
+LAMINATE_IN_PLANE = (1, 2, 3)   # strain components continuous along the layers
+
+
+def _eng_strain(T: np.ndarray) -> np.ndarray:
+    """Engineering strain (..., 6) of symmetric tensors (..., 3, 3): shears doubled."""
+    return np.stack([T[..., i, j] * (1.0 if i == j else 2.0) for i, j in _VOIGT_PAIRS], axis=-1)
+
+
+def _eng_stress(T: np.ndarray) -> np.ndarray:
+    """Voigt stress (..., 6) of symmetric tensors (..., 3, 3)."""
+    return np.stack([T[..., i, j] for i, j in _VOIGT_PAIRS], axis=-1)
+
+
+def _tensor_of(v: np.ndarray, shear: float) -> np.ndarray:
+    """Symmetric tensor (..., 3, 3) of a Voigt vector (..., 6) whose shears are scaled by ``shear``."""
+    T = np.zeros(v.shape[:-1] + (3, 3))
+    for k, (i, j) in enumerate(_VOIGT_PAIRS):
+        T[..., i, j] = v[..., k] * (1.0 if i == j else shear)
+        T[..., j, i] = T[..., i, j]
+    return T
+
+
+def mandel(C: np.ndarray) -> np.ndarray:
+    """Mandel form of engineering-Voigt stiffnesses (..., 6, 6): the shear rows and columns
+    scaled by sqrt 2, so that the eigenvalues are those of the stiffness tensor."""
+    w = np.array([1.0, 1.0, 1.0, math.sqrt(2.0), math.sqrt(2.0), math.sqrt(2.0)])
+    return C * w[:, None] * w[None, :]
+
+
+def laminate_stiffness(layers) -> np.ndarray:
+    """Exact stiffness of isotropic layers ``(fraction, E, nu)`` stacked along local axis 1
+    (solid3d/materials.py:99-144).  Normal components (stress xx, xz, xy) are uniform across
+    the layers and in-plane components (strain yy, zz, yz) uniform along them:
+    C_nn = <C_nn^-1>^-1, C_np = C_nn <C_nn^-1 C_np>, C_pn = <C_pn C_nn^-1> C_nn and
+    C_pp = <C_pp - C_pn C_nn^-1 C_np> + <C_pn C_nn^-1> C_nn <C_nn^-1 C_np>."""
+    rows = [tuple(float(v) for v in layer) for layer in layers]
+    if not rows or any(len(r) != 3 for r in rows):
+        raise ValueError("layers must be a non-empty sequence of (fraction, E, nu)")
+    f = np.array([r[0] for r in rows])
+    if np.any(f <= 0.0) or abs(float(f.sum()) - 1.0) > 1.0e-12:
+        raise ValueError("layer fractions must be positive and sum to 1")
+    Cl = np.stack([isotropic_stiffness_3d(E, nu) for _, E, nu in rows])
+    n, pl = np.array(LAMINATE_NORMAL), np.array(LAMINATE_IN_PLANE)
+    Cnn, Cnp = Cl[:, n][:, :, n], Cl[:, n][:, :, pl]
+    Cpn, Cpp = Cl[:, pl][:, :, n], Cl[:, pl][:, :, pl]
+    Cnn_inv = np.linalg.inv(Cnn)
+    avg = lambda v: np.einsum("l,lij->ij", f, v)  # noqa: E731
+    A = np.linalg.inv(avg(Cnn_inv))
+    Bm = avg(Cnn_inv @ Cnp)
+    Pm = avg(Cpn @ Cnn_inv)
+    out = np.zeros((6, 6))
+    out[np.ix_(n, n)] = A
+    out[np.ix_(n, pl)] = A @ Bm
+    out[np.ix_(pl, n)] = Pm @ A
+    out[np.ix_(pl, pl)] = avg(Cpp - Cpn @ Cnn_inv @ Cnp) + Pm @ A @ Bm
+    out = 0.5 * (out + out.T)
+    if float(np.linalg.eigvalsh(mandel(out)).min()) <= 0.0:
+        raise ValueError("laminate stiffness is not symmetric positive definite")
+    return out
+
+
+def frame_from_tangent(t) -> np.ndarray:
+    """Right-handed frames (..., 3, 3) with columns e1, e2, e3 (solid3d/materials.py:175-203):
+    e1 = t/|t| (a zero tangent counts as (1, 0, 0)); e2 the normalised cross product of the
+    global axis least aligned with e1 (lowest index on ties) with e1; e3 = e1 x e2."""
+    t = np.asarray(t, dtype=np.float64)
+    if t.shape[-1] != 3:
+        raise ValueError(f"tangent must end in 3 entries, got {t.shape}")
+    norm = np.linalg.norm(t, axis=-1, keepdims=True)
+    valid = norm > 0.0
+    e1 = np.where(valid, t / np.where(valid, norm, 1.0), np.array([1.0, 0.0, 0.0]))
+    axis = np.eye(3)[np.argmin(np.abs(e1), axis=-1)]
+    e2 = np.cross(axis, e1)
+    e2 = e2 / np.linalg.norm(e2, axis=-1, keepdims=True)
+    e3 = np.cross(e1, e2)
+    return np.stack((e1, e2, e3), axis=-1)
+
+
+def rotate_stiffness(C_local: np.ndarray, R: np.ndarray) -> np.ndarray:
+    """Engineering stiffnesses (n, 6, 6) in global axes of ``C_local`` (6, 6) or (n, 6, 6)
+    given in frames ``R`` (n, 3, 3) whose columns are the material axes
+    (solid3d/materials.py:211-245): C = T_s C_local T_e with T_e the map of global to local
+    engineering strain (eps_l = R^T eps R) and T_s that of local to global stress
+    (sigma = R sigma_l R^T)."""
+    R = np.asarray(R, dtype=np.float64)
+    if R.ndim != 3 or R.shape[1:] != (3, 3):
+        raise ValueError(f"R must have shape (n, 3, 3), got {R.shape}")
+    if R.shape[0] and float(np.abs(np.einsum("nji,njk->nik", R, R) - np.eye(3)).max()) > 1.0e-10:
+        raise ValueError("every R must be orthogonal")
+    I6 = np.eye(6)
+    Te = _eng_strain(np.einsum("nji,cjk,nkl->ncil", R, _tensor_of(I6, 0.5), R)).transpose(0, 2, 1)
+    Ts = _eng_stress(np.einsum("nij,cjk,nlk->ncil", R, _tensor_of(I6, 1.0), R)).transpose(0, 2, 1)
+    return Ts @ C_local @ Te
+
+
 @dataclass
 class ElementMaterialArrays:
     """Per-voxel material data (solid3d/model.py:1137-1196; built as materials.py:474-548)."""
@@ -189,29 +304,43 @@ class ElementMaterialArrays:
     phase: np.ndarray            # (E,) int
     E: np.ndarray | None = None     # (E,) Young's modulus of the voxel's phase (PF-CZM E_ref)
     f_t: np.ndarray | None = None   # (E,) tensile strength (NaN where the phase has none)
+    tangent: np.ndarray | None = None  # (E,3) interface normal of each voxel (laminate frames)
+    laminate: np.ndarray | None = None  # (E,) bool: voxel of a smeared-laminate phase
+
+    @property
+    def all_isotropic(self) -> bool:
+        return self.laminate is None or not bool(np.any(self.laminate))
 
 
 def build_element_arrays(phase_id: np.ndarray, phases: Mapping[str, PhaseMaterial],
                          energy_split_isotropic: str = "spectral",
-                         Gc_field: np.ndarray | None = None) -> ElementMaterialArrays:
-    """Bank of the present isotropic phases in phase-id order (materials.py:474-548).
+                         Gc_field: np.ndarray | None = None, tangent: np.ndarray | None = None) -> ElementMaterialArrays:
+    """Per-voxel material arrays (materials.py:474-548): a stiffness bank of the present
+    isotropic phases in phase-id order, then one entry per smeared-laminate voxel in voxel
+    order, its laminate stiffness rotated into the frame of its ``tangent`` (nz,ny,nx,3) or
+    (E,3); lam/mu of every phase (the isotropic equivalent of a laminate).
 
     ``Gc_field`` (E,) optionally replaces the per-phase Gc (per-voxel fracture energy)."""
     phase_flat = np.asarray(phase_id).reshape(-1).astype(np.int64)
+    n_el = phase_flat.shape[0]
     present = [n for n in PHASE_NAMES if PHASE_ID[n] in set(np.unique(phase_flat).tolist())]
     for n in present:
         if n not in phases:
             raise ValueError(f"no material for phase {n!r}")
-    index = np.empty(phase_flat.shape[0], dtype=np.int64)
+    index = np.empty(n_el, dtype=np.int64)
     bank = []
     n_ids = len(PHASE_NAMES)
     Gc_id, lam_id, mu_id = np.zeros(n_ids), np.zeros(n_ids), np.zeros(n_ids)
     E_id, ft_id = np.zeros(n_ids), np.full(n_ids, np.nan)
     drv_id = np.zeros(n_ids, dtype=np.int8)
-    for slot, name in enumerate(present):
+    iso_names = [n for n in present if not phases[n].is_laminate]
+    lam_names = [n for n in present if phases[n].is_laminate]
+    for slot, name in enumerate(iso_names):
         m = phases[name]
-        bank.append(isotropic_stiffness_3d(m.E, m.nu))
+        bank.append(isotropic_stiffness_3d(m.E, m.nu)[None])
         index[phase_flat == PHASE_ID[name]] = slot
+    for name in present:
+        m = phases[name]
         pid = PHASE_ID[name]
         Gc_id[pid] = m.Gc
         lam_id[pid] = m.E * m.nu / ((1.0 + m.nu) * (1.0 - 2.0 * m.nu))
@@ -223,9 +352,113 @@ def build_element_arrays(phase_id: np.ndarray, phases: Mapping[str, PhaseMateria
             drv_id[pid] = DRIVER_NONE
         else:
             drv_id[pid] = DRIVER_SPECTRAL if energy_split_isotropic == "spectral" else DRIVER_FULL
+    tan = None
+    laminate = np.zeros(n_el, dtype=bool)
+    if tangent is not None:
+        tan = np.asarray(tangent, dtype=np.float64).reshape(-1, 3)
+        if tan.shape[0] != n_el:
+            raise ValueError(f"tangent must hold one vector per voxel ({n_el}), got {tan.shape[0]}")
+    if lam_names:
+        if tan is None:
+            raise ValueError("smeared_laminate phases need the voxel tangent field")
+        ids = np.array([PHASE_ID[n] for n in lam_names])
+        framed = np.nonzero(np.isin(phase_flat, ids))[0]
+        local = np.stack([laminate_stiffness(phases[n].layers) for n in lam_names])
+        slot_of = np.full(n_ids, -1, dtype=np.int64)
+        slot_of[ids] = np.arange(len(lam_names))
+        bank.append(rotate_stiffness(local[slot_of[phase_flat[framed]]], frame_from_tangent(tan[framed])))
+        index[framed] = len(iso_names) + np.arange(framed.shape[0])
+        laminate[framed] = True
     Gc = Gc_id[phase_flat] if Gc_field is None else np.asarray(Gc_field, dtype=np.float64).reshape(-1)
-    return ElementMaterialArrays(np.stack(bank), index, Gc, lam_id[phase_flat], mu_id[phase_flat],
-                                 drv_id[phase_flat], phase_flat, E_id[phase_flat], ft_id[phase_flat])
+    return ElementMaterialArrays(np.concatenate(bank, axis=0), index, Gc, lam_id[phase_flat], mu_id[phase_flat],
+                                 drv_id[phase_flat], phase_flat, E_id[phase_flat], ft_id[phase_flat], tan,
+                                 laminate)
+
+
+# -- crack closure rows of the two groups (solid3d/splits.py:424-742) --------------------
+CLOSURE_MODULI_TOLERANCE = 1.0e-10      # splits.py:411
+CLOSURE_STIFFENING_TOLERANCE = 1.0e-12  # splits.py:414
+INTERFACE_DIAGONAL_TOLERANCE = 1.0e-10  # splits.py:183
+CLOSURE_ISO_ROWS = np.array([[1.0, 1.0, 1.0, 0.0, 0.0, 0.0],
+                             [1.0, 0.0, 0.0, 0.0, 0.0, 0.0],
+                             [0.0, 1.0, 0.0, 0.0, 0.0, 0.0],
+                             [0.0, 0.0, 1.0, 0.0, 0.0, 0.0]])  # splits.py:489-495: trace, e_x, e_y, e_z
+
+
+def _strain_of_dyad(a: np.ndarray) -> np.ndarray:
+    """Engineering strain of a (x) a: [a1^2, a2^2, a3^2, 2 a2 a3, 2 a1 a3, 2 a1 a2]."""
+    a1, a2, a3 = a[..., 0], a[..., 1], a[..., 2]
+    return np.stack((a1 * a1, a2 * a2, a3 * a3, 2.0 * a2 * a3, 2.0 * a1 * a3, 2.0 * a1 * a2), axis=-1)
+
+
+def _normal_strain_vector(a: np.ndarray) -> np.ndarray:
+    """q with q . eps = a.eps.a for engineering eps: [a1^2, a2^2, a3^2, a2 a3, a1 a3, a1 a2]
+    (splits.py:547-559)."""
+    a1, a2, a3 = a[..., 0], a[..., 1], a[..., 2]
+    return np.stack((a1 * a1, a2 * a2, a3 * a3, a2 * a3, a1 * a3, a1 * a2), axis=-1)
+
+
+def _sym_outer(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    return 0.5 * (a[..., :, None] * b[..., None, :] + b[..., :, None] * a[..., None, :])
+
+
+def interface_traction_rows(tangent: np.ndarray) -> np.ndarray:
+    """Rows m_k (..., 3, 6) with t_k = m_k . sigma for the local traction components
+    (11, 13, 12) of the frame of ``tangent`` (splits.py:190-209): the engineering strain of
+    sym(a n^T) for a = n, e3, e2."""
+    F = frame_from_tangent(tangent)
+    n, e2, e3 = F[..., :, 0], F[..., :, 1], F[..., :, 2]
+    return np.stack([_eng_strain(_sym_outer(a, n)) for a in (n, e3, e2)], axis=-2)
+
+
+def band_closure_rows(C: np.ndarray, tangent: np.ndarray):
+    """Closure rows (nb, 4, 6) and moduli (nb, 4) of smeared-laminate voxels
+    (operator.py:669-687; splits.py:212-258, 603-670): rows (C m1, q2 + q3, q2, q3) with m1
+    the normal-traction row and q_k the normal-strain vectors of the in-plane axes e2, e3;
+    moduli (1/a11, lam_S, 2 mu_S, 2 mu_S) with a11 the normal entry of the (diagonal)
+    normal block M C M^T and (lam_S, mu_S) = (S23, (S22 - S23)/2) of the in-plane Schur
+    block S = C_pp - C_pn C_nn^-1 C_np, which must be two-dimensionally isotropic."""
+    F = frame_from_tangent(tangent)
+    M = interface_traction_rows(tangent)
+    Cnn = M @ C @ M.transpose(0, 2, 1)
+    diag = np.diagonal(Cnn, axis1=-2, axis2=-1)
+    if diag.size:
+        if not np.all(diag > 0.0):
+            raise ValueError("the interface normal block C_nn must have a positive diagonal")
+        off = np.abs(Cnn - diag[..., :, None] * np.eye(3)).max(axis=(-2, -1)) / np.abs(diag).max(axis=-1)
+        if not float(off.max()) <= INTERFACE_DIAGONAL_TOLERANCE:
+            raise ValueError(f"crack_closure 'axes' needs a band frame with a diagonal C_nn (relative "
+                             f"off-diagonal {float(off.max()):.3e})")
+    e2, e3 = F[..., :, 1], F[..., :, 2]
+    P = np.stack((_strain_of_dyad(e2), _strain_of_dyad(e3), _eng_strain(_sym_outer(e2, e3))), axis=-2)
+    Cnp = M @ C @ P.transpose(0, 2, 1)
+    Cpp = P @ C @ P.transpose(0, 2, 1)
+    S = Cpp - Cnp.transpose(0, 2, 1) @ np.linalg.solve(Cnn, Cnp)
+    s22, s33, s23 = S[:, 0, 0], S[:, 1, 1], S[:, 0, 1]
+    if s22.size:
+        scale = np.maximum(np.abs(s22), np.abs(s33))
+        defect = float((np.abs(s22 - s33) / scale).max())
+        coupling = float((np.maximum(np.abs(S[:, 0, 2]), np.abs(S[:, 1, 2])) / scale).max())
+        if not (defect <= CLOSURE_MODULI_TOLERANCE and coupling <= CLOSURE_MODULI_TOLERANCE):
+            raise ValueError("crack_closure 'axes' needs a band whose in-plane Schur block is two-dimensionally "
+                             f"isotropic (relative S22 - S33 {defect:.3e}, normal-shear coupling {coupling:.3e})")
+        if np.any(s23 < 0.0):
+            raise ValueError("crack_closure 'axes' needs a band Schur block with S23 >= 0 (lam_S >= 0)")
+    lam_S, mu_S = s23, 0.5 * (s22 - s23)
+    q2, q3 = _normal_strain_vector(e2), _normal_strain_vector(e3)
+    Cm = np.einsum("nij,nj->ni", C, M[:, 0, :])
+    rows = np.stack((Cm, q2 + q3, q2, q3), axis=-2)
+    moduli = np.stack((1.0 / diag[:, 0], lam_S, 2.0 * mu_S, 2.0 * mu_S), axis=-1)
+    return rows, moduli
+
+
+def closure_stiffening_margin(C: np.ndarray, rows: np.ndarray, moduli: np.ndarray) -> np.ndarray:
+    """lambda_min(C - D) / max C_ii in Mandel form, D = sum_k k_k v_k v_k^T with every bit
+    active (splits.py:456-486)."""
+    D = np.einsum("nki,nk,nkj->nij", rows, moduli, rows)
+    diff = mandel(C - D)
+    diff = 0.5 * (diff + diff.transpose(0, 2, 1))
+    return np.linalg.eigvalsh(diff)[:, 0] / np.diagonal(C, axis1=-2, axis2=-1).max(axis=-1)
 
 
 @dataclass(frozen=True)
@@ -273,10 +506,12 @@ _PATTERN_SHIFTS = torch.arange(32, dtype=torch.int64).view(8, 4)   # bit 4 q + k
 
 @dataclass(eq=False)
 class ClosurePattern:
-    """Sign pattern of the isotropic crack closure (solid3d/model.py:1200-1223): ``iso``
-    (E, 8, 4) bool holds ``[tr eps < 0], [eps_xx < 0], [eps_yy < 0], [eps_zz < 0]`` per Gauss
-    point.  Kept packed on the device, one 32-bit word per voxel (bit 4 q + k), which is the
-    form the kernels read; ``iso`` unpacks it.  There is no laminate band group here."""
+    """Sign pattern of the crack closure (solid3d/model.py:1200-1223): ``iso`` (E, 8, 4) bool
+    holds the bits ``[v_k . eps < 0]`` of each voxel's four closure rows per Gauss point
+    (trace, eps_xx, eps_yy, eps_zz in the isotropic group; t11, p2 + p3, p2, p3 of the band
+    frame for a laminate band voxel, whose bits the reference keeps in a separate ``band``
+    array).  Kept packed on the device, one 32-bit word per voxel (bit 4 q + k), which is the
+    form the kernels read; ``iso`` unpacks it."""
     bits: torch.Tensor            # (E,) int32
     band: None = None
 
@@ -348,7 +583,8 @@ class PeriodicVoxelPhaseFieldOperator:
     """Matrix-free mechanics and damage operators of one triply periodic voxel cell.
 
     Mirrors solid3d/operator.py:133-1544: the AT2 law or (``damage_law='pfczm'``) Wu's PF-CZM
-    law, with or without the isotropic crack closure (``crack_closure='axes'``); no interface
+    law, with or without the crack closure (``crack_closure='axes'``: the isotropic phases on the
+    cell axes, smeared-laminate voxels on their band frame); no interface
     drivers.  x = [u~ (3 n), E (6)] with node-major u~ (``3 node + c``).
     """
 
@@ -397,6 +633,8 @@ class PeriodicVoxelPhaseFieldOperator:
         self.damage_law = damage_law
         self._pfczm = damage_law == "pfczm"
         self._closure = crack_closure == "axes"
+        self.n_closure_band = 0
+        self.band_closure = None   # (voxel ids, rows (nb,4,6), moduli (nb,4)) of the laminate band
         self.a1 = None
         self.pfczm_bound = None
         if self._pfczm:
@@ -435,14 +673,56 @@ class PeriodicVoxelPhaseFieldOperator:
         self.a1 = torch.as_tensor(a1, device=self.device)
 
     def _build_closure(self, materials, drv, lam):
-        """Isotropic closure group: every voxel with a damage driver (operator.py:631-666; all
-        phases here are isotropic), lam >= 0 there."""
-        on = (drv != DRIVER_NONE).astype(np.uint8)
-        if on.any() and float(lam[on.astype(bool)].min()) < 0.0:
+        """The closure groups (operator.py:631-690): the isotropic phases' voxels with a damage
+        driver (cell axes, lam >= 0 there) and the smeared-laminate voxels with a driver (the
+        band frame: rows and moduli of ``band_closure_rows``); neither group may stiffen a
+        voxel beyond its intact stiffness."""
+        on = drv != DRIVER_NONE
+        laminate = np.zeros_like(on) if materials.laminate is None else np.asarray(materials.laminate, dtype=bool)
+        iso = on & ~laminate
+        band = on & laminate
+        if iso.any() and float(lam[iso].min()) < 0.0:
             raise ValueError("crack_closure 'axes' needs lam >= 0 in every isotropic phase")
-        on = np.ascontiguousarray(on)
-        check(_L.phast_hex8_closure_setup(self._plan, on.ctypes.data), "closure_setup")
-        self.cl_on = torch.as_tensor(on.astype(bool), device=self.device)
+        self.cl_on = torch.as_tensor(on, device=self.device)
+        self.n_closure_band = int(band.sum())
+        if materials.all_isotropic:
+            group = np.ascontiguousarray(iso.astype(np.uint8))
+            check(_L.phast_hex8_closure_setup(self._plan, group.ctypes.data), "closure_setup")
+            return
+        bank = np.asarray(materials.stiffness_bank, dtype=np.float64)
+        index = np.asarray(materials.stiffness_index, dtype=np.int64)
+        mu = np.asarray(materials.mu, dtype=np.float64)
+        tab = np.zeros((bank.shape[0], 28))
+        entries = np.unique(index[iso])
+        for slot in entries:  # isotropic phases: one bank entry each, shared lam and mu
+            members = iso & (index == slot)
+            lam_s, mu_s = lam[members], mu[members]
+            if np.ptp(lam_s) > 0.0 or np.ptp(mu_s) > 0.0:
+                raise ValueError("isotropic elements sharing a stiffness must share lam and mu")
+            tab[slot, :24] = CLOSURE_ISO_ROWS.reshape(-1)
+            tab[slot, 24:] = (lam_s[0], 2.0 * mu_s[0], 2.0 * mu_s[0], 2.0 * mu_s[0])
+        if entries.size:
+            margin = closure_stiffening_margin(bank[entries], tab[entries, :24].reshape(-1, 4, 6), tab[entries, 24:])
+            if float(margin.min()) < -CLOSURE_STIFFENING_TOLERANCE:
+                raise ValueError(f"the isotropic closure stiffens a phase (margin {float(margin.min()):.3e})")
+        if band.any():
+            if materials.tangent is None:
+                raise ValueError("the band closure needs the voxel tangent field")
+            ids = np.nonzero(band)[0]
+            slots = index[ids]
+            if np.unique(slots).size != slots.size:
+                raise ValueError("every band voxel needs its own bank entry")
+            rows, moduli = band_closure_rows(bank[slots], np.asarray(materials.tangent)[ids])
+            margin = closure_stiffening_margin(bank[slots], rows, moduli)
+            if float(margin.min()) < -CLOSURE_STIFFENING_TOLERANCE:
+                raise ValueError("the band closure stiffens an element beyond its intact stiffness "
+                                 f"(lambda_min(C - D) / max C_ii = {float(margin.min()):.3e})")
+            tab[slots, :24] = rows.reshape(-1, 24)
+            tab[slots, 24:] = moduli
+            self.band_closure = (ids, rows, moduli)
+        group = np.ascontiguousarray((iso.astype(np.uint8) + 2 * band.astype(np.uint8)))
+        tab = np.ascontiguousarray(tab)
+        check(_L.phast_hex8_closure_setup_rows(self._plan, group.ctypes.data, tab.ctypes.data), "closure_setup_rows")
 
     @property
     def is_pfczm(self) -> bool:
@@ -637,11 +917,12 @@ class PeriodicVoxelPhaseFieldOperator:
         if g is None:
             raise ValueError("closure_pattern_change needs g")
         bits = torch.empty(self.n_elements, dtype=torch.int32, device=self.device)
-        out = torch.empty(2, dtype=F64, device=self.device)
+        out = torch.empty(4 if self.n_closure_band else 2, dtype=F64, device=self.device)
         check(_L.phast_hex8_closure_pattern(self._plan, ptr(x), ptr(g), ptr(pattern.bits.contiguous()), ptr(bits),
                                             ptr(out), stream()), "closure_pattern_change")
-        strength, jump = (float(v) for v in out.cpu())
-        return ClosurePattern(bits), bool(jump >= 0.0 and jump > tolerance * strength)
+        vals = [float(v) for v in out.cpu()]
+        changed = any(jump >= 0.0 and jump > tolerance * strength for strength, jump in zip(vals[0::2], vals[1::2]))
+        return ClosurePattern(bits), changed
 
     def driving_force(self, x) -> torch.Tensor:
         """Undamaged driving energy Y (E,8) (operator.py:1415-1444)."""

@@ -154,6 +154,57 @@ def pore_in_cube_pfczm(n: int = 48, increments: int = 150, to: float = 3.0e-3):
     return phase, phases, h, 1.5e-3, load
 
 
+INTERFACE_LAYERS_PARTICLES = ((0.10397175686880507, 20000.0, 0.2), (0.44801412156559745, 270000.0, 0.2),
+                              (0.44801412156559745, 345000.0, 0.2))
+
+
+def particles_in_cube_pfczm(n: int = 48, increments: int = 150, to: float = 3.0e-3):
+    """The reference's PF-CZM particle example (examples/three_dimensional/particles_in_cube/
+    run.py:19-32): a periodic cube of n^3 voxels of 0.5 um with four spherical particles of
+    radius 5 um placed by random sequential addition (seed 20261023, gap 1 um), each wrapped
+    in a 1.92 um interface shell, a smeared laminate of (fraction, E, nu) layers across the
+    shell (its isotropic equivalent E = 122.6 GPa, nu = 0.2 drives the damage); matrix
+    E = 345 GPa, f_t = 600 MPa, Gc = 25 J/m^2; particles E = 270 GPa, f_t = 3136 MPa,
+    Gc = 518 J/m^2; interface f_t = 63 MPa, Gc = 0.646 J/m^2; PF-CZM with ell = 1.5 um;
+    E_xx raised to ``to`` in ``increments`` equal steps, the other macro stresses zero.
+    Returns (phase (n,n,n) uint8, tangent (n,n,n,3), phases, h, ell, load case)."""
+    from phast_amd import geometry as G
+    from phast_amd import solid3d as S
+    h = (0.5e-3,) * 3
+    phase, tangent, _ = G.particles_with_shell((n, n, n), h, 4, 5.0e-3, 1.9235993121898142e-3, 1.0e-3, 20261023)
+    phases = {"mi_matrix": S.PhaseMaterial("mi_matrix", 345e3, 0.2, 25e-3, tensile_strength=600.0),
+              "fibre": S.PhaseMaterial("fibre", 270e3, 0.2, 0.5180310650205762, tensile_strength=3136.0),
+              "bn_band": S.PhaseMaterial("bn_band", 122601.74043706217, 0.2, 0.00064633998586781,
+                                         tensile_strength=63.0, kind="smeared_laminate",
+                                         layers=INTERFACE_LAYERS_PARTICLES)}
+    load = S.LoadCase.from_schedule("macro_xx", controlled=0, schedule=[(to, to / increments)], free=(1, 2, 3, 4, 5),
+                                    fixed_zero=())
+    return phase, tangent, phases, h, 1.5e-3, load
+
+
+INTERFACE_LAYERS_FIBRES = ((0.10823084088511756, 20000.0, 0.2), (0.4458845795574412, 270000.0, 0.2),
+                           (0.4458845795574412, 345000.0, 0.2))
+
+
+def fibre_unit_cell_pfczm(increments: int = 150, to: float = 3.0e-3):
+    """Materials and load of the reference's nine-fibre unit cell (examples/three_dimensional/
+    fibre_unit_cell/run.py:19-32), whose microstructure (phase ids and interface normals of
+    103 x 103 x 2 voxels of 0.5 um) is an imported image: matrix E = 345 GPa, f_t = 600 MPa,
+    Gc = 25 J/m^2; fibres E = 270 GPa, f_t = 3136 MPa, Gc = 518 J/m^2; a smeared-laminate
+    interface band (isotropic equivalent E = 119.7 GPa, nu = 0.2, f_t = 63 MPa,
+    Gc = 0.646 J/m^2); PF-CZM with ell = 1.5 um; E_xx raised to ``to`` in ``increments``
+    equal steps, the other macro stresses zero.  Returns (phases, h, ell, load case)."""
+    from phast_amd import solid3d as S
+    phases = {"mi_matrix": S.PhaseMaterial("mi_matrix", 345e3, 0.2, 25e-3, tensile_strength=600.0),
+              "fibre": S.PhaseMaterial("fibre", 270e3, 0.2, 0.5180310650205762, tensile_strength=3136.0),
+              "bn_band": S.PhaseMaterial("bn_band", 119683.28713197137, 0.2, 0.0006463399858678095,
+                                         tensile_strength=63.0, kind="smeared_laminate",
+                                         layers=INTERFACE_LAYERS_FIBRES)}
+    load = S.LoadCase.from_schedule("macro_xx", controlled=0, schedule=[(to, to / increments)], free=(1, 2, 3, 4, 5),
+                                    fixed_zero=())
+    return phases, (0.5e-3,) * 3, 1.5e-3, load
+
+
 def lognormal_field(rng, ny, nx, mean, cov, corr):
     """Lognormal per-cell field of the given mean and coefficient of variation whose log is a
     Gaussian random field with a Gaussian correlation of ``corr`` cells (white noise filtered

@@ -260,3 +260,102 @@ def test_homogeneous_cell_stagger_reaches_the_independent_fixed_point():
         assert math.isclose(rec["sigma"][0], sigma, rel_tol=1e-9)
         assert abs(rec["sigma"][1]) < 1e-7 * sigma
     assert seen_damage
+
+
+# -- the laminate band group -----------------------------------------------------------------
+
+LAYERS = ((0.10397175686880507, 20000.0, 0.2), (0.44801412156559745, 270000.0, 0.2),
+          (0.44801412156559745, 345000.0, 0.2))
+
+
+def _band_op(shape=(3, 2, 2), band_voxels=(1, 4, 7, 10), seed=0):
+    """The pore-in-cube matrix with some voxels replaced by a laminate band of random normals,
+    with the band rows and moduli of the host set-up."""
+    from phast_amd import solid3d as S
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(seed)
+    ids = np.array(band_voxels)
+    t = rng.standard_normal((ids.size, 3))
+    Cb = S.rotate_stiffness(S.laminate_stiffness(LAYERS), S.frame_from_tangent(t))
+    bank = np.concatenate([O.isotropic_stiffness_3d(E_M, NU)[None], Cb])
+    index = np.zeros(n, dtype=np.int64)
+    index[ids] = 1 + np.arange(ids.size)
+    Eb = 122601.74
+    Ep = np.full(n, E_M)
+    Ep[ids] = Eb
+    lam = Ep * NU / ((1 + NU) * (1 - 2 * NU))
+    mu = Ep / (2 * (1 + NU))
+    f_t = np.full(n, F_T)
+    f_t[ids] = 63.0
+    Gc = np.full(n, GC)
+    Gc[ids] = 6.46e-4
+    base = O.VoxelOperator(shape, (0.5e-3,) * 3, bank, index, Gc, lam, mu, np.full(n, 2), kappa=KAPPA, ell=ELL,
+                           pinned_node=0)
+    rows, k = S.band_closure_rows(Cb, t)
+    return P.PfczmOperator(base, Ep, f_t, closure=True, band=(ids, rows, k))
+
+
+def test_band_internal_force_is_the_derivative_of_the_degraded_energy():
+    op = _band_op()
+    assert (op.cl_group == 2).sum() == 4 and (op.cl_group == 1).sum() == op.b.E - 4
+    rng = np.random.default_rng(5)
+    x = np.concatenate([1e-6 * rng.standard_normal(3 * op.b.n_red), [1e-3, -2e-3, 5e-4, 3e-4, -1e-4, 2e-4]])
+    g = 0.2 + 0.7 * rng.random((op.b.E, 8))
+    f = op.mech_action(x, g)
+    assert op.closure_pattern(x)[op.cl_group == 2].any()
+    v = rng.standard_normal(op.b.n_mech)
+    v[-6:] *= 1e-3
+    v[:-6] *= 1e-6
+    hstep = 1e-4
+    fd = (op.elastic_energy(x + hstep * v, g) - op.elastic_energy(x - hstep * v, g)) / (2 * hstep)
+    assert math.isclose(fd, float(f @ v), rel_tol=1e-8)
+    np.testing.assert_allclose(op.mech_action(x, g, op.closure_pattern(x)), f, rtol=1e-14, atol=1e-14 * np.abs(f).max())
+
+
+def test_band_pattern_action_and_diagonal_from_dense_assembly():
+    op = _band_op(shape=(2, 2, 2), band_voxels=(0, 5, 6))
+    rng = np.random.default_rng(7)
+    g = 0.1 + 0.8 * rng.random((op.b.E, 8))
+    pattern = (rng.random((op.b.E, 8, 4)) < 0.5) & op.cl_on[:, None, None]
+    n = op.b.n_mech
+    K = np.stack([op.mech_action(np.eye(n)[i], g, pattern) for i in range(n)], axis=1)
+    np.testing.assert_allclose(K, K.T, rtol=1e-12, atol=1e-12 * np.abs(K).max())
+    expect = np.zeros(n)
+    Kd = np.zeros((n, n))
+    N, grad, w, B, _, _ = O.reference_voxel(op.b.h)
+    for e in range(op.b.E):
+        C = op.b.bank[op.b.index[e]]
+        dofs = list(op.b.dofs[e]) + list(range(n - 6, n))
+        R = op.cl_rows[e]
+        for q in range(8):
+            Bq = np.hstack([B[q], np.eye(6)])
+            D = (R.T * (op.cl_k[e] * pattern[e, q])) @ R
+            Kq = w[q] * Bq.T @ (g[e, q] * C + (1 - g[e, q]) * D) @ Bq
+            expect[dofs] += np.diag(Kq)
+            Kd[np.ix_(dofs, dofs)] += Kq
+    np.testing.assert_allclose(op.mech_diagonal(g, pattern), expect, rtol=1e-12)
+    # the action is the dense assembly (duplicate periodic dofs add up)
+    np.testing.assert_allclose(K, Kd, rtol=1e-11, atol=1e-12 * np.abs(Kd).max())
+
+
+def test_pattern_change_is_judged_per_group():
+    """A flipped bit counts against its own group's largest strength (operator.py:826-840):
+    flip the strongest bit of the group with the smaller maximum, with a tolerance between
+    the two groups' scales -- a change for the per-group rule, none for a global one."""
+    op = _band_op(shape=(2, 2, 2), band_voxels=(3,))
+    x = _homogeneous_x(op, np.array([1e-3, -1e-3, 5e-4, 0, 0, 0]))
+    g = np.full((op.b.E, 8), 0.5)
+    own = op.closure_pattern(x)
+    assert op.closure_pattern_change(x, own, g)[1] is False
+    strength = op.cl_k[:, None, :] * np.abs(op.closure_values(x))
+    smax = {grp: strength[op.cl_group == grp].max() for grp in (1, 2)}
+    small, big = sorted((1, 2), key=lambda grp: smax[grp])
+    assert 0.0 < smax[small] < smax[big]
+    cand = np.where(own & (op.cl_group == small)[:, None, None], strength, -1.0)
+    e, q, k = np.unravel_index(np.argmax(cand), cand.shape)
+    s_bit = strength[e, q, k]
+    old = own.copy()
+    old[e, q, k] = False
+    tol = 0.5 * s_bit / math.sqrt(smax[small] * smax[big])
+    assert op.closure_pattern_change(x, old, g, tolerance=tol)[1] is True
+    assert op.closure_pattern_change(x, old, g, tolerance=0.5 * s_bit / smax[small] * 1.01)[1] is False
