@@ -176,6 +176,11 @@ def test_particles_in_cube_recorded_pfczm_history(dev):
 
 
 FIBRE = Path(__file__).with_name("golden") / "fibre_unit_cell.npz"
+# macro stress within 1e-9 of the recording through increment 44 (peak near 15, d_max 1 by
+# 40); in the fully softened tail the two runs differ by 1e-9 to 6e-8 without growing (the
+# recording's PCG tolerance is 1e-8)
+FIBRE_RTOL_EARLY, FIBRE_RTOL_TAIL = 1e-9, 1e-6
+FIBRE_INCREMENTS = 150 if os.environ.get("PHAST_FULL_HISTORY") else 30
 FIBRE_HISTORY = Path(__file__).with_name("golden") / "fibre_unit_cell_pfczm_history.json"
 
 
@@ -210,16 +215,17 @@ def test_fibre_unit_cell_recorded_pfczm_history(dev):
     solver = G3.StaggeredSolver(op, load, cfg)
     st = solver.initial_state()
     worst = 0.0
-    for row in gold["rows"][:HISTORY_INCREMENTS]:
+    for row in gold["rows"][:FIBRE_INCREMENTS]:
         k = int(row["step"])
         st, recs = solver.advance_increment(st, load.targets[k - 1])
         rec = recs[-1]
         err = abs(rec.macro_stress[0] - row["macro_stress_xx"]) / abs(row["macro_stress_xx"])
         worst = max(worst, err)
-        if k % 10 == 0 or err > 1e-8:
+        if k % 10 == 0 or err > 1e-9:
             print(f"increment {k}: sigma_xx {rec.macro_stress[0]:.9f} (recorded {row['macro_stress_xx']:.9f}, "
                   f"rel {err:.2e}) balance {rec.balance_residual_rel:.3e} (recorded "
                   f"{row['balance_residual_rel']:.3e}) d_max {rec.max_damage:.4f}", flush=True)
-        assert err < 1e-8, (k, rec.macro_stress[0], row["macro_stress_xx"])
+        assert err < (FIBRE_RTOL_EARLY if k <= 44 else FIBRE_RTOL_TAIL), (k, rec.macro_stress[0],
+                                                                          row["macro_stress_xx"])
         assert math.isclose(rec.balance_residual_rel, row["balance_residual_rel"], rel_tol=1e-4, abs_tol=1e-12), k
-    print(f"worst relative macro stress difference over {HISTORY_INCREMENTS} increments: {worst:.2e}", flush=True)
+    print(f"worst relative macro stress difference over {FIBRE_INCREMENTS} increments: {worst:.2e}", flush=True)
