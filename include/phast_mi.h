@@ -311,6 +311,28 @@ int phast_t3_adjoint_solve(phast_t3_plan *plan, const double *H, const double *G
 int phast_t3_adjoint_terms(const phast_t3_plan *plan, const double *lam, const double *d_pull, const double *d_unc,
                            double *gH, double *lKd, double *lMd, double *lM1, hipStream_t stream);
 
+/* ---- general quadrilateral meshes on the same plan type ------------------------------
+ * The reference's Q4 branch on any quadrilateral mesh (core/mesh.py:305-352 _precompute_q4,
+ * core/quad_elements.py:315-349 q4_quadrature_geometry), where the structured-plate plans
+ * above need congruent axis-aligned cells.  phast_q4u_plan_create: nodes host (N,2),
+ * elements host (E,4) int32 in counter-clockwise corner order; every Gauss-point Jacobian
+ * must be positive (else PHAST_ERR_ARG, as the reference raises); split / stress_split
+ * PHAST_SPLIT_AMOR or _ISOTROPIC; AT2.  The incidence list is in e*4+a order (the order of
+ * scatter_add over elements.flatten()).  The plan then works with phast_t3_plan_destroy,
+ * _mesh_info (M_scalar = scatter of sum_q N_a wdetJ, h_min = the shortest cell edge, the
+ * total area), _internal_force, _damping_force, _force_history, _explicit_step,
+ * _element_fields (eps (3,E,4) and psi (E,4) only), _elem_to_node, _damage_apply,
+ * _damage_rhs_diag and _damage_solve, with Gauss-point arrays: H, H_old, H_new, psi and the
+ * projected field are (E,4) (q = xi fastest: (-a,-a), (a,-a), (-a,a), (a,a)), g is
+ * evaluated at the Gauss points from N d (fem_operators.py:626-630), the damage operator is
+ * _Ax_q4 / _prepare_cg_q4 (damage_solver.py:1797-1838, 2270-2301).  Each pass is an element
+ * pass (one thread per quadrilateral; geometry recomputed from the corner coordinates) that
+ * writes two values per corner to plan-owned memory, then a node pass that sums them; passes
+ * on one plan must therefore not overlap.  No adjoint, no patches. */
+int phast_q4u_plan_create(int64_t n_nodes, int64_t n_elems, const double *nodes, const int32_t *elements, double E,
+                          double nu, double Gc, double l0, double eta_residual, int32_t plane_stress, int32_t split,
+                          int32_t stress_split, phast_t3_plan **out);
+
 /* =====================================================================================
  * 3D periodic hex8 voxel cells
  * ===================================================================================== */

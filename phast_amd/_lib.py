@@ -72,6 +72,7 @@ _SIGNATURES = {
     "phast_t3_plan_create": ([_I64, _I64, _P, _P, _D, _D, _D, _D, _D, _I32, _I32, _I32, _I32, _D, C.POINTER(_P)],
                              C.c_int),
     "phast_t3_plan_destroy": ([_P], C.c_int),
+    "phast_q4u_plan_create": ([_I64, _I64, _P, _P, _D, _D, _D, _D, _D, _I32, _I32, _I32, C.POINTER(_P)], C.c_int),
     "phast_t3_mesh_info": ([_P, _P, _P, _P], C.c_int),
     "phast_t3_set_path": ([_P, _I32, C.POINTER(_I32), C.POINTER(_I32)], C.c_int),
     "phast_t3_internal_force": ([_P, _P, _P, _P, _P], C.c_int),

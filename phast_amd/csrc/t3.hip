@@ -50,11 +50,18 @@ struct T3Mat {
 
 struct T3Mesh {
     const double2 *xy;
-    const int32_t *el;   // (E,3)
+    const int32_t *el;   // (E,3) triangles, (E,4) quadrilaterals
     const int32_t *ptr;  // (N+1)
-    const int32_t *inc;  // incidences e*3 + a
+    const int32_t *inc;  // incidences e*K + a (K corners per element)
     int64_t n_nodes, n_elems;
+    double *ce = nullptr;  // quadrilaterals: two values per corner (E,4,2) from the element pass
 };
+
+// Where a node pass takes its element corner values: SRC_RECOMPUTE evaluates the triangle of
+// each incidence, SRC_PATCH reads the patch's triangles from LDS, SRC_BUFFER reads the corner
+// values an element pass wrote to m.ce (quadrilateral meshes; the incidence e*4 + a indexes
+// them directly).
+enum : int { SRC_RECOMPUTE = 0, SRC_PATCH = 1, SRC_BUFFER = 2 };
 
 // Area and shape-function gradients of one triangle (core/mesh.py:236-268).
 __device__ __forceinline__ void t3_geometry(double2 p0, double2 p1, double2 p2, double &area, double gp[3][2]) {
@@ -218,15 +225,20 @@ __device__ __forceinline__ void t3_patch_fill(const T3Patch &P, double *lds, F c
 
 // Sums over node n's incidences, in e*3+a order, of the two corner values.  __dadd_rn keeps
 // each addition a separate rounding in both organisations, so they agree bitwise.
-template <bool PATCH, class F>
+template <int SRC, class F>
 __device__ __forceinline__ void t3_node_sum(const T3Mesh &m, const T3Patch &P, const double *lds, int64_t slot,
                                             int64_t n, F corners, double &s0, double &s1) {
     s0 = 0.0;
     s1 = 0.0;
-    const int32_t kb = PATCH ? P.sptr[slot] : m.ptr[n], ke = PATCH ? P.sptr[slot + 1] : m.ptr[n + 1];
+    const int32_t kb = SRC == SRC_PATCH ? P.sptr[slot] : m.ptr[n];
+    const int32_t ke = SRC == SRC_PATCH ? P.sptr[slot + 1] : m.ptr[n + 1];
     for (int32_t k = kb; k < ke; ++k) {
         double v0, v1;
-        if (PATCH) {
+        if (SRC == SRC_BUFFER) {
+            const int64_t ic = m.inc[k];
+            v0 = m.ce[2 * ic];
+            v1 = m.ce[2 * ic + 1];
+        } else if (SRC == SRC_PATCH) {
             const int sl = P.sloc[k];
             v0 = lds[2 * sl];
             v1 = lds[2 * sl + 1];
@@ -301,7 +313,7 @@ __global__ __launch_bounds__(256) void t3_predictor_kernel(int64_t nn, const dou
 
 // MODE 0: f = internal force; 1: + psi+ and H = max(H_old, psi+) per element;
 // 2: explicit-step corrector on top of 1 (u is the predicted u1; writes v1, a1).
-template <int MODE, bool PATCH>
+template <int MODE, int SRC>
 __global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3Patch P, const T3Mat M, const double2 *u,
                                                        const double *d, double2 *f, const double *H_old, double *H_new,
                                                        double *psi, const T3Step S) {
@@ -310,12 +322,12 @@ __global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3P
     auto corners = [&](int4 t, Corners &c, bool owner) {
         t3_force_corners<(MODE >= 1)>(m, M, ux, d, H_old, H_new, psi, t, c, owner);
     };
-    if (PATCH) t3_patch_fill(P, lds, corners);
+    if (SRC == SRC_PATCH) t3_patch_fill(P, lds, corners);
     const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (slot >= m.n_nodes) return;
-    const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
+    const int64_t n = SRC == SRC_PATCH ? (int64_t)P.pnode[slot] : slot;
     double fx, fy;
-    t3_node_sum<PATCH>(m, P, lds, slot, n, corners, fx, fy);
+    t3_node_sum<SRC>(m, P, lds, slot, n, corners, fx, fy);
     if (MODE == 0) {
         f[n] = make_double2(S.fscale * fx, S.fscale * fy);  // x 1 is exact
         return;
@@ -386,17 +398,17 @@ __device__ __forceinline__ void t3_damage_corners(const T3Mesh &m, const T3Mat &
 
 // OP 0: y = A x; OP 1: y = b, y2 = diag; OP 2: y = b, y2 = M_inv = 1/(diag + 1e-30)
 // (damage_solver.py:2265)
-template <int OP, bool PATCH>
+template <int OP, int SRC>
 __global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3Patch P, const T3Mat M, const double *H,
                                                         const double *Gc, const double *x, double *y, double *y2) {
     extern __shared__ double lds[];
     auto corners = [&](int4 t, Corners &c, bool) { t3_damage_corners<(OP == 0 ? 0 : 1)>(m, M, H, Gc, x, t, c); };
-    if (PATCH) t3_patch_fill(P, lds, corners);
+    if (SRC == SRC_PATCH) t3_patch_fill(P, lds, corners);
     const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (slot >= m.n_nodes) return;
-    const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
+    const int64_t n = SRC == SRC_PATCH ? (int64_t)P.pnode[slot] : slot;
     double s0, s1;
-    t3_node_sum<PATCH>(m, P, lds, slot, n, corners, s0, s1);
+    t3_node_sum<SRC>(m, P, lds, slot, n, corners, s0, s1);
     if (OP == 0) {
         y[n] = Gc ? s0 + s1 : M.Gc_l0 * s0 + s1;
     } else {
@@ -518,6 +530,250 @@ enum T3VecOp : int {
     TA_FINISH = 16,    // x = fixed ? vals : x
 };
 
+
+// ---------------------------------------------------------------------------------------
+// General quadrilateral meshes (the Q4 branch of the reference on any quad mesh)
+// ---------------------------------------------------------------------------------------
+// An element pass (one thread per quadrilateral) evaluates the element at its 2x2 Gauss
+// points and writes two values per corner to m.ce; the node passes above (SRC_BUFFER) sum
+// them in each node's e*4 + a order, the order of the reference's scatter_add over
+// elements.flatten().  Geometry is recomputed from the corner coordinates as
+// core/quad_elements.py:315-349 does once: Gauss points (xi, eta) = (-a,-a), (a,-a), (-a,a),
+// (a,a), a = 1/sqrt3, weights 1; J_ij = sum_a dN_a/dxi_i x_aj, grad N_a = J^-T dN_a/dxi,
+// wdetJ = det J.  History and psi+ are per Gauss point (E,4).
+
+struct Q4uElem {
+    int32_t n[4];
+    double N[4][4];      // [q][a]
+    double gp[4][4][2];  // [q][a][x|y]
+    double wd[4];        // [q]
+};
+
+__device__ __forceinline__ void q4u_geometry(const T3Mesh &m, int64_t e, Q4uElem &Q) {
+    double2 X[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        Q.n[a] = m.el[4 * e + a];
+        X[a] = m.xy[Q.n[a]];
+    }
+    const double g = 1.0 / sqrt(3.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double xi = (q & 1) ? g : -g, eta = (q & 2) ? g : -g;
+        Q.N[q][0] = 0.25 * ((1.0 - xi) * (1.0 - eta));
+        Q.N[q][1] = 0.25 * ((1.0 + xi) * (1.0 - eta));
+        Q.N[q][2] = 0.25 * ((1.0 + xi) * (1.0 + eta));
+        Q.N[q][3] = 0.25 * ((1.0 - xi) * (1.0 + eta));
+        const double dxi[4] = {0.25 * -(1.0 - eta), 0.25 * (1.0 - eta), 0.25 * (1.0 + eta), 0.25 * -(1.0 + eta)};
+        const double det_[4] = {0.25 * -(1.0 - xi), 0.25 * -(1.0 + xi), 0.25 * (1.0 + xi), 0.25 * (1.0 - xi)};
+        double J00 = 0.0, J01 = 0.0, J10 = 0.0, J11 = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            J00 += dxi[a] * X[a].x;
+            J01 += dxi[a] * X[a].y;
+            J10 += det_[a] * X[a].x;
+            J11 += det_[a] * X[a].y;
+        }
+        const double det = J00 * J11 - J01 * J10;
+        const double i00 = J11 / det, i01 = -J01 / det, i10 = -J10 / det, i11 = J00 / det;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            Q.gp[q][a][0] = dxi[a] * i00 + det_[a] * i01;
+            Q.gp[q][a][1] = dxi[a] * i10 + det_[a] * i11;
+        }
+        Q.wd[q] = det;
+    }
+}
+
+// Forces of one quadrilateral at its corners (fem_operators.py:244-254 strain, 614-653
+// stress with g at the Gauss points from N d, 681-699 assembly); with HIST also psi+ and
+// H = max(H_old, psi+) at each Gauss point (fem_operators.py:951-1006, staggered_solver.py:
+// 804-845).  MODE 2 reads the predicted displacement u1.
+template <bool HIST>
+__global__ __launch_bounds__(256) void q4u_force_elem_kernel(const T3Mesh m, const T3Mat M, const double2 *u,
+                                                             const double *d, const double *H_old, double *H_new,
+                                                             double *psi, double *ce) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m.n_elems) return;
+    Q4uElem Q;
+    q4u_geometry(m, e, Q);
+    double2 U[4];
+    double D[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        U[a] = u[Q.n[a]];
+        D[a] = d ? d[Q.n[a]] : 0.0;
+    }
+    double fx[4] = {0.0, 0.0, 0.0, 0.0}, fy[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double exx = 0.0, eyy = 0.0, gx = 0.0, gy = 0.0, dq = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            exx += Q.gp[q][a][0] * U[a].x;
+            eyy += Q.gp[q][a][1] * U[a].y;
+            gx += Q.gp[q][a][1] * U[a].x;
+            gy += Q.gp[q][a][0] * U[a].y;
+            dq += Q.N[q][a] * D[a];
+        }
+        const double gxy = gx + gy;
+        double g = 1.0;
+        if (d) {
+            const double omd = 1.0 - dq;
+            g = (omd * omd) * (1.0 - M.eta) + M.eta;  // physics/material.py:529
+        }
+        double sxx, syy, sxy;
+        t3_stress(M, exx, eyy, gxy, g, sxx, syy, sxy);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            fx[a] += (Q.gp[q][a][0] * sxx + Q.gp[q][a][1] * sxy) * Q.wd[q];
+            fy[a] += (Q.gp[q][a][1] * syy + Q.gp[q][a][0] * sxy) * Q.wd[q];
+        }
+        if (HIST) {
+            const double p = t3_psi_plus(M, exx, eyy, gxy);
+            if (psi) psi[4 * e + q] = p;
+            const double h = H_old[4 * e + q];
+            H_new[4 * e + q] = h >= p ? h : p;  // max returns one operand bit-exactly
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double *c = m.ce + 8 * e + 2 * a;
+        c[0] = fx[a];
+        c[1] = fy[a];
+    }
+}
+
+// Gauss-point strain (3,E,4) and psi+ (E,4) of one quadrilateral (fem_operators.py:244-254,
+// 951-1006); each output optional.
+__global__ __launch_bounds__(256) void q4u_fields_kernel(const T3Mesh m, const T3Mat M, const double2 *u, double *eps,
+                                                         double *psi) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m.n_elems) return;
+    Q4uElem Q;
+    q4u_geometry(m, e, Q);
+    double2 U[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) U[a] = u[Q.n[a]];
+    const int64_t ne = m.n_elems;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double exx = 0.0, eyy = 0.0, gx = 0.0, gy = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            exx += Q.gp[q][a][0] * U[a].x;
+            eyy += Q.gp[q][a][1] * U[a].y;
+            gx += Q.gp[q][a][1] * U[a].x;
+            gy += Q.gp[q][a][0] * U[a].y;
+        }
+        const double gxy = gx + gy;
+        if (eps) {
+            eps[4 * e + q] = exx;
+            eps[4 * (ne + e) + q] = eyy;
+            eps[4 * (2 * ne + e) + q] = gxy;
+        }
+        if (psi) psi[4 * e + q] = t3_psi_plus(M, exx, eyy, gxy);
+    }
+}
+
+// Damage operator of one quadrilateral with Gauss-point history H (E,4) (damage_solver.py:
+// 1797-1838 _Ax_q4, 2270-2301 _prepare_cg_q4), corner values:
+// OP 0: (sum_q wdet grad N_a . grad x_q [x Gc_e l0], sum_q N_a (2 H_q + Gc/l0) x_q wdet);
+//       the node pass forms Gc l0 s0 + s1 (s0 + s1 with a per-element Gc field);
+// OP 1: (sum_q N_a 2 H_q wdet, Gc l0 sum_q wdet |grad N_a|^2 + sum_q N_a^2 (2 H_q + Gc/l0) wdet);
+// OP 2: x |-> the Gauss-point projection weights sum_q N_a f_q wdet of a field f (E,4)
+//       (core/mesh.py:407-428), one value per corner (the second is 0).
+// CGOP (a T_* pass of the damage CG, or -1): the pass is skipped as the node pass is.
+template <int OP, int CGOP>
+__global__ __launch_bounds__(256) void q4u_damage_elem_kernel(const T3Mesh m, const T3Mat M, const double *H,
+                                                              const double *Gc, const double *x, const DamageCG *cg,
+                                                              double *ce) {
+    if (CGOP >= 0) {
+        const DamageCG &S = cg[0];
+        if (CGOP != T_CG_INIT && CGOP != T_PC_INIT && S.done) return;
+        if (CGOP == T_CG_RESID && !(S.limited || S.check)) return;
+        if (CGOP == T_PC_RESID && !S.check) return;
+    }
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m.n_elems) return;
+    Q4uElem Q;
+    q4u_geometry(m, e, Q);
+    double c0[4] = {0.0, 0.0, 0.0, 0.0}, c1[4] = {0.0, 0.0, 0.0, 0.0};
+    if (OP == 2) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double fw = H[4 * e + q] * Q.wd[q];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) c0[a] += Q.N[q][a] * fw;
+        }
+    } else {
+        const double ge = Gc ? Gc[e] : 0.0;
+        const double ratio = Gc ? ge / M.l0 : M.Gc_over_l0;  // damage_solver.py:2274-2279
+        const double gl = Gc ? ge * M.l0 : M.Gc_l0;
+        double X[4];
+        if (OP == 0) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) X[a] = x[Q.n[a]];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double rd = 2.0 * H[4 * e + q] + ratio;
+            if (OP == 0) {
+                double gdx = 0.0, gdy = 0.0, xq = 0.0;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    gdx += Q.gp[q][a][0] * X[a];
+                    gdy += Q.gp[q][a][1] * X[a];
+                    xq += Q.N[q][a] * X[a];
+                }
+                const double mq = (rd * xq) * Q.wd[q];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    c0[a] += Q.wd[q] * (Q.gp[q][a][0] * gdx + Q.gp[q][a][1] * gdy);
+                    c1[a] += Q.N[q][a] * mq;
+                }
+            } else {
+                const double bw = (2.0 * H[4 * e + q]) * Q.wd[q], rw = rd * Q.wd[q];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    c0[a] += Q.N[q][a] * bw;
+                    c1[a] += (Q.N[q][a] * Q.N[q][a]) * rw;
+                }
+            }
+        }
+        if (OP == 0 && Gc) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) c0[a] = gl * c0[a];
+        }
+        if (OP == 1) {  // the Laplacian diagonal first, then the mass diagonal added
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                double lap = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    lap += Q.wd[q] * (Q.gp[q][a][0] * Q.gp[q][a][0] + Q.gp[q][a][1] * Q.gp[q][a][1]);
+                c1[a] = gl * lap + c1[a];
+            }
+        }
+    }
+    double *c = m.ce + 8 * e;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        c[2 * a] = c0[a];
+        c[2 * a + 1] = c1[a];
+    }
+}
+
+// Nodal projection of a Gauss-point field on quadrilaterals: sum of the corner weights over
+// the node's incidences / (M + 1e-30) (core/mesh.py:407-428).
+__global__ __launch_bounds__(256) void q4u_e2n_kernel(const T3Mesh m, const double *Mn, double *out) {
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n >= m.n_nodes) return;
+    double s = 0.0;
+    for (int32_t k = m.ptr[n]; k < m.ptr[n + 1]; ++k) s = __dadd_rn(s, m.ce[2 * (int64_t)m.inc[k]]);
+    out[n] = s / (Mn[n] + 1e-30);
+}
+
 struct T3Cg {
     const double *H, *Gc;
     double *d, *r, *z, *p, *Ap, *lb;
@@ -530,7 +786,7 @@ struct T3Cg {
     int32_t nblk;
 };
 
-template <int OP, bool PATCH>
+template <int OP, int SRC>
 __global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Patch P, const T3Mat M, T3Cg A) {
     extern __shared__ double lds[];
     __shared__ double red[12];
@@ -541,13 +797,13 @@ __global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Patc
     constexpr double INF = __builtin_huge_val();
     const double *x = (OP == T_CG_AP || OP == T_PC_AP) ? A.p : A.d;
     auto corners = [&](int4 t, Corners &c, bool) { t3_damage_corners<0>(m, M, A.H, A.Gc, x, t, c); };
-    if (PATCH) t3_patch_fill(P, lds, corners);
+    if (SRC == SRC_PATCH) t3_patch_fill(P, lds, corners);
     const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double p0 = 0.0, p1 = (OP == T_CG_AP) ? INF : 0.0, p2 = (OP == T_CG_AP) ? INF : 0.0;
     if (slot < m.n_nodes) {
-        const int64_t n = PATCH ? (int64_t)P.pnode[slot] : slot;
+        const int64_t n = SRC == SRC_PATCH ? (int64_t)P.pnode[slot] : slot;
         double s0, s1;
-        t3_node_sum<PATCH>(m, P, lds, slot, n, corners, s0, s1);
+        t3_node_sum<SRC>(m, P, lds, slot, n, corners, s0, s1);
         const double g0 = A.Gc ? s0 + s1 : M.Gc_l0 * s0 + s1;
         const double xn = x[n];
         if (OP == T_CG_INIT || OP == T_CG_RESID) {
@@ -784,6 +1040,8 @@ struct phast_t3_plan {
     bool has_patch = false, use_patch = false;
     int32_t max_list = 0;
     size_t lds_bytes = 0;
+    int32_t K = 3;          // corners per element: 3 triangles, 4 quadrilaterals
+    double *ce = nullptr;   // quadrilaterals: corner values of the element passes (E,4,2)
 };
 
 namespace {
@@ -956,6 +1214,104 @@ extern "C" int phast_t3_plan_create(int64_t n_nodes, int64_t n_elems, const doub
     return PHAST_OK;
 }
 
+// Plan of a general quadrilateral mesh (the reference's Q4 branch on any quad mesh): the same
+// plan type, K = 4 corners per element, no patches (every pass is an element pass writing
+// corner values and a node pass summing them).  Host set-up mirrors core/mesh.py:305-352
+// _precompute_q4: Gauss-point Jacobians (every one must be positive, quad_elements.py:
+// 331-333), the lumped mass sum_q N_a wdetJ scattered per node, h_min = the smallest edge.
+extern "C" int phast_q4u_plan_create(int64_t n_nodes, int64_t n_elems, const double *nodes, const int32_t *elements,
+                                     double E, double nu, double Gc, double l0, double eta_residual,
+                                     int32_t plane_stress, int32_t split, int32_t stress_split, phast_t3_plan **out) {
+    PHAST_REQUIRE(out && nodes && elements && n_nodes >= 4 && n_elems >= 1, "phast_q4u_plan_create: bad argument");
+    PHAST_REQUIRE(4 * n_elems < ((int64_t)1 << 31) && n_nodes < ((int64_t)1 << 31),
+                  "phast_q4u_plan_create: mesh too large for 32-bit incidence indices");
+    PHAST_REQUIRE(split == T3_ISO || split == T3_AMOR, "the quadrilateral path implements the amor and isotropic "
+                  "splits, got %d", split);
+    PHAST_REQUIRE(stress_split == T3_ISO || stress_split == T3_AMOR, "unknown stress split %d", stress_split);
+    for (int64_t k = 0; k < 4 * n_elems; ++k)
+        PHAST_REQUIRE(elements[k] >= 0 && elements[k] < n_nodes, "element %lld refers to node %d outside the mesh",
+                      (long long)(k / 4), elements[k]);
+    const double g = 1.0 / std::sqrt(3.0);
+    std::vector<double> M_scalar((size_t)n_nodes, 0.0);
+    double hmin = INFINITY, tot = 0.0;
+    std::vector<int32_t> cnt((size_t)n_nodes + 1, 0);
+    for (int64_t e = 0; e < n_elems; ++e) {
+        const int32_t *q4 = elements + 4 * e;
+        double wd[4], Nq[4][4];
+        for (int q = 0; q < 4; ++q) {
+            const double xi = (q & 1) ? g : -g, eta = (q & 2) ? g : -g;
+            const double dxi[4] = {0.25 * -(1.0 - eta), 0.25 * (1.0 - eta), 0.25 * (1.0 + eta), 0.25 * -(1.0 + eta)};
+            const double det_[4] = {0.25 * -(1.0 - xi), 0.25 * -(1.0 + xi), 0.25 * (1.0 + xi), 0.25 * (1.0 - xi)};
+            Nq[q][0] = 0.25 * ((1.0 - xi) * (1.0 - eta));
+            Nq[q][1] = 0.25 * ((1.0 + xi) * (1.0 - eta));
+            Nq[q][2] = 0.25 * ((1.0 + xi) * (1.0 + eta));
+            Nq[q][3] = 0.25 * ((1.0 - xi) * (1.0 + eta));
+            double J00 = 0.0, J01 = 0.0, J10 = 0.0, J11 = 0.0;
+            for (int a = 0; a < 4; ++a) {
+                const double *X = nodes + 2 * q4[a];
+                J00 += dxi[a] * X[0];
+                J01 += dxi[a] * X[1];
+                J10 += det_[a] * X[0];
+                J11 += det_[a] * X[1];
+            }
+            wd[q] = J00 * J11 - J01 * J10;
+            PHAST_REQUIRE(wd[q] > 0.0, "Q4 element %lld has a non-positive Jacobian at Gauss point %d (check the "
+                          "counter-clockwise corner order)", (long long)e, q);
+        }
+        double hmin_e = INFINITY;
+        for (int k = 0; k < 4; ++k) {
+            const double *A = nodes + 2 * q4[k], *B = nodes + 2 * q4[(k + 1) % 4];
+            hmin_e = std::min(hmin_e, std::sqrt((B[0] - A[0]) * (B[0] - A[0]) + (B[1] - A[1]) * (B[1] - A[1])));
+        }
+        hmin = std::min(hmin, hmin_e);
+        for (int a = 0; a < 4; ++a) {
+            double m = 0.0;
+            for (int q = 0; q < 4; ++q) m += Nq[q][a] * wd[q];
+            M_scalar[(size_t)q4[a]] += m;
+            ++cnt[(size_t)q4[a] + 1];
+        }
+        tot += ((wd[0] + wd[1]) + wd[2]) + wd[3];
+    }
+    auto *pl = new phast_t3_plan();
+    pl->K = 4;
+    T3Mat &M = pl->mat;
+    M.lam = plane_stress ? E * nu / (1.0 - nu * nu) : E * nu / ((1 + nu) * (1 - 2 * nu));  // material.py:379-388
+    M.mu = E / (2 * (1 + nu));
+    M.kappa = E / (3.0 * (1.0 - 2.0 * nu));
+    M.ezz_coef = -nu / (1.0 - nu);
+    M.eta = eta_residual;
+    M.at1 = 0;
+    M.Gc_l0 = Gc * l0;
+    M.Gc_over_l0 = Gc / l0;
+    M.src = 0.0;
+    M.l0 = l0;
+    M.split = split;
+    M.ssplit = stress_split;
+    M.plane_stress = plane_stress;
+    pl->M_scalar = std::move(M_scalar);
+    pl->h_min = hmin;
+    pl->total_area = tot;
+    for (int64_t i = 0; i < n_nodes; ++i) cnt[(size_t)i + 1] += cnt[(size_t)i];
+    std::vector<int32_t> inc((size_t)(4 * n_elems)), fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t k = 0; k < 4 * n_elems; ++k) inc[(size_t)fill[(size_t)elements[k]]++] = (int32_t)k;
+    if (hipMalloc(&pl->xy, sizeof(double2) * (size_t)n_nodes) != hipSuccess ||
+        hipMalloc(&pl->el, sizeof(int32_t) * 4 * (size_t)n_elems) != hipSuccess ||
+        hipMalloc(&pl->ptr, sizeof(int32_t) * (size_t)(n_nodes + 1)) != hipSuccess ||
+        hipMalloc(&pl->inc, sizeof(int32_t) * 4 * (size_t)n_elems) != hipSuccess ||
+        hipMalloc(&pl->ce, sizeof(double) * 8 * (size_t)n_elems) != hipSuccess ||
+        hipMemcpy(pl->xy, nodes, sizeof(double2) * (size_t)n_nodes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->el, elements, sizeof(int32_t) * 4 * (size_t)n_elems, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->ptr, cnt.data(), sizeof(int32_t) * (size_t)(n_nodes + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->inc, inc.data(), sizeof(int32_t) * 4 * (size_t)n_elems, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("phast_q4u_plan_create: device allocation/copy failed");
+        phast_t3_plan_destroy(pl);
+        return PHAST_ERR_HIP;
+    }
+    pl->mesh = T3Mesh{pl->xy, pl->el, pl->ptr, pl->inc, n_nodes, n_elems, pl->ce};
+    *out = pl;
+    return PHAST_OK;
+}
+
 extern "C" int phast_t3_plan_destroy(phast_t3_plan *pl) {
     if (!pl) return PHAST_OK;
     (void)hipFree(pl->xy);
@@ -967,6 +1323,7 @@ extern "C" int phast_t3_plan_destroy(phast_t3_plan *pl) {
     (void)hipFree(pl->pel);
     (void)hipFree(pl->sptr);
     (void)hipFree(pl->sloc);
+    (void)hipFree(pl->ce);
     for (double *q : {pl->r, pl->p, pl->z, pl->Ap, pl->b, pl->minv, pl->lb, pl->part}) (void)hipFree(q);
     (void)hipFree(pl->active);
     (void)hipFree(pl->cg);
@@ -993,15 +1350,28 @@ extern "C" int phast_t3_set_path(phast_t3_plan *pl, int32_t path, int32_t *activ
     return PHAST_OK;
 }
 
+static inline dim3 elem_grid(const phast_t3_plan *pl) { return dim3((unsigned)ceil_div(pl->mesh.n_elems, 256)); }
+
 template <int MODE>
 static int launch_force(const phast_t3_plan *pl, const double *u, const double *d, double *f, const double *H_old,
                         double *H_new, double *psi, const T3Step &S, hipStream_t st) {
-    if (pl->use_patch)
-        hipLaunchKernelGGL((t3_force_kernel<MODE, true>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh,
-                           pl->patch, pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, S);
-    else
-        hipLaunchKernelGGL((t3_force_kernel<MODE, false>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
+    if (pl->K == 4) {  // element pass, then the node pass over the corner values
+        if (MODE >= 1)
+            hipLaunchKernelGGL(q4u_force_elem_kernel<true>, elem_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat,
+                               (const double2 *)u, d, H_old, H_new, psi, pl->ce);
+        else
+            hipLaunchKernelGGL(q4u_force_elem_kernel<false>, elem_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat,
+                               (const double2 *)u, d, H_old, H_new, psi, pl->ce);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL((t3_force_kernel<MODE, SRC_BUFFER>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
                            pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, S);
+    } else if (pl->use_patch) {
+        hipLaunchKernelGGL((t3_force_kernel<MODE, SRC_PATCH>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh,
+                           pl->patch, pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, S);
+    } else {
+        hipLaunchKernelGGL((t3_force_kernel<MODE, SRC_RECOMPUTE>), node_grid(pl), dim3(256), 0, st, pl->mesh,
+                           pl->patch, pl->mat, (const double2 *)u, d, (double2 *)f, H_old, H_new, psi, S);
+    }
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
@@ -1009,12 +1379,19 @@ static int launch_force(const phast_t3_plan *pl, const double *u, const double *
 template <int OP>
 static int launch_damage(const phast_t3_plan *pl, const double *H, const double *Gc, const double *x, double *y,
                          double *y2, hipStream_t st) {
-    if (pl->use_patch)
-        hipLaunchKernelGGL((t3_damage_kernel<OP, true>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh,
-                           pl->patch, pl->mat, H, Gc, x, y, y2);
-    else
-        hipLaunchKernelGGL((t3_damage_kernel<OP, false>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
+    if (pl->K == 4) {
+        hipLaunchKernelGGL((q4u_damage_elem_kernel<(OP == 0 ? 0 : 1), -1>), elem_grid(pl), dim3(256), 0, st, pl->mesh,
+                           pl->mat, H, Gc, x, (const DamageCG *)nullptr, pl->ce);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL((t3_damage_kernel<OP, SRC_BUFFER>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
                            pl->mat, H, Gc, x, y, y2);
+    } else if (pl->use_patch) {
+        hipLaunchKernelGGL((t3_damage_kernel<OP, SRC_PATCH>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh,
+                           pl->patch, pl->mat, H, Gc, x, y, y2);
+    } else {
+        hipLaunchKernelGGL((t3_damage_kernel<OP, SRC_RECOMPUTE>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
+                           pl->mat, H, Gc, x, y, y2);
+    }
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
@@ -1073,6 +1450,13 @@ extern "C" int phast_t3_element_fields(const phast_t3_plan *pl, const double *u,
                                        double *sig, double *psi, hipStream_t st) {
     PHAST_REQUIRE(pl && u && (eps || sig || psi), "phast_t3_element_fields: null argument");
     PHAST_REQUIRE(d || !sig, "phast_t3_element_fields: the stress needs the damage d");
+    if (pl->K == 4) {
+        PHAST_REQUIRE(!sig, "phast_t3_element_fields: quadrilateral plans give strain and psi+ only");
+        hipLaunchKernelGGL(q4u_fields_kernel, elem_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, (const double2 *)u,
+                           eps, psi);
+        PHAST_LAUNCHED();
+        return PHAST_OK;
+    }
     hipLaunchKernelGGL(t3_elem_fields_kernel, dim3((unsigned)ceil_div(pl->mesh.n_elems, 256)), dim3(256), 0, st,
                        pl->mesh, pl->mat, (const double2 *)u, d, eps, sig, psi);
     PHAST_LAUNCHED();
@@ -1082,6 +1466,15 @@ extern "C" int phast_t3_element_fields(const phast_t3_plan *pl, const double *u,
 extern "C" int phast_t3_elem_to_node(const phast_t3_plan *pl, const double *field_e, const double *M, double *out,
                                      hipStream_t st) {
     PHAST_REQUIRE(pl && field_e && M && out, "phast_t3_elem_to_node: null argument");
+    if (pl->K == 4) {  // Gauss-point field (E,4)
+        hipLaunchKernelGGL((q4u_damage_elem_kernel<2, -1>), elem_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat,
+                           field_e, (const double *)nullptr, (const double *)nullptr, (const DamageCG *)nullptr,
+                           pl->ce);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL(q4u_e2n_kernel, node_grid(pl), dim3(256), 0, st, pl->mesh, M, out);
+        PHAST_LAUNCHED();
+        return PHAST_OK;
+    }
     hipLaunchKernelGGL(t3_e2n_kernel, node_grid(pl), dim3(256), 0, st, pl->mesh, field_e, M, out);
     PHAST_LAUNCHED();
     return PHAST_OK;
@@ -1119,12 +1512,20 @@ static int t3_ensure_work(phast_t3_plan *pl) {
 
 template <int OP>
 static int t3_cg(const phast_t3_plan *pl, const T3Cg &A, hipStream_t st) {
-    if (pl->use_patch)
-        hipLaunchKernelGGL((t3_cg_kernel<OP, true>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh, pl->patch,
+    if (pl->K == 4) {  // the operator's element pass on the pass's field, skipped as the node pass is
+        const double *x = (OP == T_CG_AP || OP == T_PC_AP) ? A.p : A.d;
+        hipLaunchKernelGGL((q4u_damage_elem_kernel<0, OP>), elem_grid(pl), dim3(256), 0, st, pl->mesh, pl->mat, A.H,
+                           A.Gc, x, (const DamageCG *)A.cg, pl->ce);
+        PHAST_LAUNCHED();
+        hipLaunchKernelGGL((t3_cg_kernel<OP, SRC_BUFFER>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
                            pl->mat, A);
-    else
-        hipLaunchKernelGGL((t3_cg_kernel<OP, false>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch, pl->mat,
-                           A);
+    } else if (pl->use_patch) {
+        hipLaunchKernelGGL((t3_cg_kernel<OP, SRC_PATCH>), node_grid(pl), dim3(256), pl->lds_bytes, st, pl->mesh,
+                           pl->patch, pl->mat, A);
+    } else {
+        hipLaunchKernelGGL((t3_cg_kernel<OP, SRC_RECOMPUTE>), node_grid(pl), dim3(256), 0, st, pl->mesh, pl->patch,
+                           pl->mat, A);
+    }
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
@@ -1252,6 +1653,7 @@ extern "C" int phast_t3_adjoint_solve(phast_t3_plan *pl, const double *H, const 
                                       double rel_tol, int32_t max_iter, double *x_out, int32_t *iters,
                                       int32_t *status, hipStream_t st) {
     PHAST_REQUIRE(pl && H && x_out, "phast_t3_adjoint_solve: null argument");
+    PHAST_REQUIRE(pl->K == 3, "phast_t3_adjoint_solve: the damage adjoint is built for triangle meshes");
     PHAST_REQUIRE(rel_tol >= 0.0 && max_iter >= 0, "rel_tol and max_iter must be non-negative");
     PHAST_REQUIRE(fixed || !fixed_vals, "fixed_vals needs fixed");
     PHAST_REQUIRE(x_out != rhs && x_out != H && x_out != Gc_e, "x_out must not alias rhs, H or Gc_e");
@@ -1316,6 +1718,7 @@ extern "C" int phast_t3_adjoint_terms(const phast_t3_plan *pl, const double *lam
                                       const double *d_unc, double *gH, double *lKd, double *lMd, double *lM1,
                                       hipStream_t st) {
     PHAST_REQUIRE(pl && lam, "phast_t3_adjoint_terms: null argument");
+    PHAST_REQUIRE(pl->K == 3, "phast_t3_adjoint_terms: the damage adjoint is built for triangle meshes");
     PHAST_REQUIRE(!gH || d_pull, "gH needs d_pull");
     PHAST_REQUIRE(!(lKd || lMd) || d_unc, "lKd and lMd need d_unc");
     const int64_t ne = pl->mesh.n_elems;

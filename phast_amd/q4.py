@@ -122,13 +122,40 @@ def _q4_reference_geometry(coords: np.ndarray):
     return N, grad, wdet
 
 
+def _structured_layout(nodes: np.ndarray, elements: np.ndarray):
+    """(nx, ny, x, y) when the mesh is the structured plate of ``rectangular_sent_q4_structured``
+    (congruent axis-aligned cells on a uniform tensor grid, nodes j*(nx+1)+i, cells j*nx+i with
+    corners (i,j),(i+1,j),(i+1,j+1),(i,j+1)); None otherwise."""
+    x = np.unique(nodes[:, 0])
+    y = np.unique(nodes[:, 1])
+    nx, ny = len(x) - 1, len(y) - 1
+    if nx < 1 or ny < 1 or (nx + 1) * (ny + 1) != nodes.shape[0] or nx * ny != elements.shape[0]:
+        return None
+    jj, ii = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    if not (np.array_equal(nodes[:, 0], x[ii.ravel()]) and np.array_equal(nodes[:, 1], y[jj.ravel()])):
+        return None
+    je, ie = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    n0 = (je * (nx + 1) + ie).ravel()
+    expect = np.stack([n0, n0 + 1, n0 + nx + 2, n0 + nx + 1], axis=1)
+    if not np.array_equal(elements, expect):
+        return None
+    hx, hy = np.diff(x), np.diff(y)
+    if np.ptp(hx) > 1e-12 * hx.max() or np.ptp(hy) > 1e-12 * hy.max():
+        return None
+    return nx, ny, x, y
+
+
 class FEMMesh:
-    """Structured Q4 plate: the mesh of ``core/mesh_generator.py:1695-1818``.
+    """Q4 mesh (core/mesh.py:305-360 precompute).
 
     ``nodes`` (N,2) and ``elements`` (E,4) are kept on the CPU for host-side set-up
-    (boundary sets, Neumann loads); all cells must be congruent axis-aligned rectangles
-    numbered as the generator numbers them (checked), which is what the tile kernels
-    exploit.  ``device`` is the ROCm device the fields live on.
+    (boundary sets, Neumann loads).  A structured plate (the mesh of
+    ``core/mesh_generator.py:1695-1818``: congruent axis-aligned cells numbered as the
+    generator numbers them) runs on the strip kernels, which exploit the one congruent cell;
+    any other quadrilateral mesh (``structured`` False) runs on the general-quadrilateral plan,
+    whose passes recompute each cell's Gauss-point geometry from its corners.  Every
+    Gauss-point Jacobian must be positive (counter-clockwise corners).  ``device`` is the ROCm
+    device the fields live on.
     """
 
     element_type = "Q4"
@@ -138,43 +165,42 @@ class FEMMesh:
         elements = np.asarray(elements, dtype=np.int64)
         if nodes.ndim != 2 or nodes.shape[1] != 2 or elements.ndim != 2 or elements.shape[1] != 4:
             raise ValueError("Q4 mesh needs nodes (N,2) and elements (E,4)")
-        x = np.unique(nodes[:, 0])
-        y = np.unique(nodes[:, 1])
-        nx, ny = len(x) - 1, len(y) - 1
-        if nx < 1 or ny < 1 or (nx + 1) * (ny + 1) != nodes.shape[0] or nx * ny != elements.shape[0]:
-            raise NotImplementedError("the Q4 HIP path needs a structured rectangular plate mesh")
-        jj, ii = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
-        if not (np.array_equal(nodes[:, 0], x[ii.ravel()]) and np.array_equal(nodes[:, 1], y[jj.ravel()])):
-            raise NotImplementedError("Q4 nodes must be numbered j*(nx+1)+i on a tensor grid")
-        je, ie = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
-        n0 = (je * (nx + 1) + ie).ravel()
-        expect = np.stack([n0, n0 + 1, n0 + nx + 2, n0 + nx + 1], axis=1)
-        if not np.array_equal(elements, expect):
-            raise NotImplementedError("Q4 cells must be numbered j*nx+i with corners (i,j),(i+1,j),(i+1,j+1),(i,j+1)")
-        hx, hy = np.diff(x), np.diff(y)
-        if np.ptp(hx) > 1e-12 * hx.max() or np.ptp(hy) > 1e-12 * hy.max():
-            raise NotImplementedError("the Q4 HIP path needs a uniform grid spacing")
+        if elements.size and (elements.min() < 0 or elements.max() >= nodes.shape[0]):
+            raise ValueError("Q4 elements refer to nodes outside the mesh")
+        layout = _structured_layout(nodes, elements)
+        self.structured = layout is not None
         self.nodes, self.elements = nodes, elements
-        self.nx, self.ny = nx, ny
         self.n_nodes, self.n_elems = nodes.shape[0], elements.shape[0]
-        self.quad_N, self.quad_grad_phi_cell, self.quad_wdetJ_cell = _q4_reference_geometry(nodes[elements[0]])
-        edges = [np.linalg.norm(nodes[elements[0], :][k] - nodes[elements[0], :][(k + 1) % 4]) for k in range(4)]
-        self.h_min = float(min(edges))  # mesh.py:336-345 (all cells congruent)
-        self.hx, self.hy = float(x[1] - x[0]), float(y[1] - y[0])
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         self.dtype = F64
-        self.node_sets = {
-            "bottom": np.nonzero(nodes[:, 1] == y[0])[0], "top": np.nonzero(nodes[:, 1] == y[-1])[0],
-            "left": np.nonzero(nodes[:, 0] == x[0])[0], "right": np.nonzero(nodes[:, 0] == x[-1])[0],
-        }
+        c = nodes[elements]
+        self._elem_h = np.min(np.stack([np.linalg.norm(c[:, (k + 1) % 4] - c[:, k], axis=1) for k in range(4)],
+                                       axis=1), axis=1)
+        self.h_min = float(self._elem_h.min())  # mesh.py:331-339
+        if self.structured:
+            nx, ny, x, y = layout
+            self.nx, self.ny = nx, ny
+            self.quad_N, self.quad_grad_phi_cell, self.quad_wdetJ_cell = _q4_reference_geometry(nodes[elements[0]])
+            self.hx, self.hy = float(x[1] - x[0]), float(y[1] - y[0])
+            self.node_sets = {
+                "bottom": np.nonzero(nodes[:, 1] == y[0])[0], "top": np.nonzero(nodes[:, 1] == y[-1])[0],
+                "left": np.nonzero(nodes[:, 0] == x[0])[0], "right": np.nonzero(nodes[:, 0] == x[-1])[0],
+            }
+        else:
+            self.nx = self.ny = None
+            # boundary sets by the bounding box (mesh.py:380-405 identify_boundaries)
+            lo, hi = nodes.min(axis=0), nodes.max(axis=0)
+            tol = 1e-6 * max(float((hi - lo).max()), 1.0)
+            self.node_sets = {
+                "bottom": np.nonzero(np.abs(nodes[:, 1] - lo[1]) < tol)[0],
+                "top": np.nonzero(np.abs(nodes[:, 1] - hi[1]) < tol)[0],
+                "left": np.nonzero(np.abs(nodes[:, 0] - lo[0]) < tol)[0],
+                "right": np.nonzero(np.abs(nodes[:, 0] - hi[0]) < tol)[0],
+            }
 
     @property
     def elem_h_np(self) -> np.ndarray:
-        """Per-cell minimum edge length (mesh.py:331-338), computed on first use."""
-        if getattr(self, "_elem_h", None) is None:
-            c = self.nodes[self.elements]
-            self._elem_h = np.min(np.stack([np.linalg.norm(c[:, (k + 1) % 4] - c[:, k], axis=1)
-                                            for k in range(4)], axis=1), axis=1)
+        """Per-cell minimum edge length (mesh.py:331-338)."""
         return self._elem_h
 
     @classmethod
@@ -192,6 +218,9 @@ class FEMMesh:
 
 def q4_desc(mesh: FEMMesh, mat: Material, dt: float = 0.0) -> Q4Desc:
     """C descriptor of one plate."""
+    if not mesh.structured:
+        raise NotImplementedError("the strip kernels (and the batched plans) need structured Q4 plates; a general "
+                                  "quadrilateral mesh runs through FEMOperators on the general-quadrilateral plan")
     d = Q4Desc()
     d.nx, d.ny = mesh.nx, mesh.ny
     for i, v in enumerate(mesh.quad_grad_phi_cell.ravel()):
@@ -202,16 +231,47 @@ def q4_desc(mesh: FEMMesh, mat: Material, dt: float = 0.0) -> Q4Desc:
         d.wdetJ[i] = float(v)
     d.lam, d.mu, d.kappa, d.nu = mat.lam, mat.mu, mat.kappa, mat.nu
     d.plane_stress = int(bool(mat.plane_stress))
-    if mat.energy_split not in ("amor", "isotropic"):
-        raise NotImplementedError(f"energy_split={mat.energy_split!r}: the Q4 HIP path implements 'amor' and "
-                                  "'isotropic'")
-    if mat.pf_model != "AT2":  # damage_solver.py:1256-1263: the native Q4 damage is AT2 only
-        raise NotImplementedError("native Q4 damage supports only pf_model='AT2'")
+    _check_q4_material(mat)
     d.split = 0 if mat.energy_split == "amor" else 1
     d.stress_split = 0 if mat.stress_split == "amor" else 1
     d.eta_residual, d.Gc, d.l0, d.dt = mat.eta_residual, mat.Gc, mat.l0, float(dt)
     d.hx, d.hy = mesh.hx, mesh.hy
     return d
+
+
+def _check_q4_material(mat: Material) -> None:
+    if mat.energy_split not in ("amor", "isotropic"):
+        raise NotImplementedError(f"energy_split={mat.energy_split!r}: the Q4 HIP path implements 'amor' and "
+                                  "'isotropic'")
+    if mat.pf_model != "AT2":  # damage_solver.py:1256-1263: the native Q4 damage is AT2 only
+        raise NotImplementedError("native Q4 damage supports only pf_model='AT2'")
+
+
+class GeneralQ4Plan:
+    """Owner of a general-quadrilateral plan (``phast_q4u_plan_create``): any Q4 mesh with
+    positive Gauss-point Jacobians; element passes recompute each cell's geometry."""
+
+    def __init__(self, mesh: FEMMesh, mat: Material):
+        _lib.require_device()
+        _check_q4_material(mat)
+        nodes = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
+        els = np.ascontiguousarray(mesh.elements, dtype=np.int32)
+        h = C.c_void_p()
+        check(_lib.lib().phast_q4u_plan_create(
+            mesh.n_nodes, mesh.n_elems, nodes.ctypes.data, els.ctypes.data, mat.E, mat.nu, mat.Gc, mat.l0,
+            mat.eta_residual, int(bool(mat.plane_stress)), 0 if mat.energy_split == "amor" else 1,
+            0 if mat.stress_split == "amor" else 1, C.byref(h)), "phast_q4u_plan_create")
+        self.handle = h
+        M = np.empty(mesh.n_nodes)
+        hmin, area = C.c_double(), C.c_double()
+        check(_lib.lib().phast_t3_mesh_info(h, M.ctypes.data, C.byref(hmin), C.byref(area)), "mesh_info")
+        self.M_scalar_host, self.h_min, self.total_area = M, hmin.value, area.value
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib.lib().phast_t3_plan_destroy(h)
+            self.handle = None
 
 
 class Q4Plan:
@@ -287,9 +347,12 @@ class FEMOperators:
         self.device, self.dtype = mesh.device, F64
         self._tile = tile
         self._plans: dict[float, Q4Plan] = {}
-        plan = self.plan(0.0)
-        M = torch.empty(mesh.n_nodes, dtype=F64, device=self.device)
-        check(_lib.lib().phast_q4_lumped_mass(plan.handle, ptr(M), stream()), "lumped mass")
+        self.general = None if mesh.structured else GeneralQ4Plan(mesh, material)
+        if self.general is not None:
+            M = torch.as_tensor(self.general.M_scalar_host, dtype=F64, device=self.device)
+        else:
+            M = torch.empty(mesh.n_nodes, dtype=F64, device=self.device)
+            check(_lib.lib().phast_q4_lumped_mass(self.plan(0.0).handle, ptr(M), stream()), "lumped mass")
         self.M_scalar = M  # mesh.py:346-352
         # fem_operators.py:83-85 (interleaved vector mass and its guarded inverse)
         self.M_vec = material.rho * M.unsqueeze(1).expand(-1, 2).flatten()
@@ -314,6 +377,10 @@ class FEMOperators:
         """``f_int (N,2)`` (fem_operators.py:598-612)."""
         u, d = self._u(u), self._d(d)
         f = torch.empty_like(u)
+        if self.general is not None:
+            check(_lib.lib().phast_t3_internal_force(self.general.handle, ptr(u), ptr(d), ptr(f), stream()),
+                  "internal_force")
+            return f
         check(_lib.lib().phast_q4_internal_force(self.plan(0.0).handle, ptr(u), ptr(d), ptr(f), stream()),
               "internal_force")
         return f
@@ -322,6 +389,10 @@ class FEMOperators:
         """Gauss-point strains ``(eps_xx, eps_yy, gam_xy)``, each (E,4) (fem_operators.py:244-254)."""
         u = self._u(u)
         eps = torch.empty((3, self.mesh.n_elems, 4), dtype=F64, device=self.device)
+        if self.general is not None:
+            check(_lib.lib().phast_t3_element_fields(self.general.handle, ptr(u), None, ptr(eps), None, None,
+                                                     stream()), "compute_strain")
+            return eps[0], eps[1], eps[2]
         check(_lib.lib().phast_q4_strain(self.plan(0.0).handle, ptr(u), ptr(eps), stream()), "compute_strain")
         return eps[0], eps[1], eps[2]
 
@@ -329,6 +400,10 @@ class FEMOperators:
         """psi+ (E,4) (fem_operators.py:951-1006); ``strain`` is recomputed from ``u``."""
         u = self._u(u)
         psi = torch.empty((self.mesh.n_elems, 4), dtype=F64, device=self.device)
+        if self.general is not None:
+            check(_lib.lib().phast_t3_element_fields(self.general.handle, ptr(u), None, None, None, ptr(psi),
+                                                     stream()), "compute_psi_plus")
+            return psi
         check(_lib.lib().phast_q4_driving_force(self.plan(0.0).handle, ptr(u), ptr(psi), None, None, stream()),
               "compute_psi_plus")
         return psi
@@ -343,6 +418,9 @@ class FEMOperators:
         H_old = _dev_check(H_old, (self.mesh.n_elems, 4), "H_old", self.device)
         psi = torch.empty_like(H_old)
         H_new = torch.empty_like(H_old)
+        if self.general is not None:
+            psi = self.compute_psi_plus(u)
+            return psi, history_update(H_old, psi)
         check(_lib.lib().phast_q4_driving_force(self.plan(0.0).handle, ptr(u), ptr(psi), ptr(H_old), ptr(H_new),
                                                 stream()), "driving_force")
         return psi, H_new
@@ -353,6 +431,10 @@ class FEMOperators:
             field_q = field_q.unsqueeze(1).expand(-1, 4)
         field_q = _dev_check(field_q.contiguous(), (self.mesh.n_elems, 4), "field", self.device)
         out = torch.empty(self.mesh.n_nodes, dtype=F64, device=self.device)
+        if self.general is not None:
+            check(_lib.lib().phast_t3_elem_to_node(self.general.handle, ptr(field_q), ptr(self.M_scalar), ptr(out),
+                                                   stream()), "elem_to_node")
+            return out
         check(_lib.lib().phast_q4_elem_to_node(self.plan(0.0).handle, ptr(field_q), ptr(self.M_scalar), ptr(out),
                                                stream()), "elem_to_node")
         return out
@@ -429,8 +511,23 @@ class ExplicitDynamics:
             H_old = _dev_check(H_old, (fem.mesh.n_elems, 4), "H_old", fem.device)
             H_new = torch.empty_like(H_old)
             psi = torch.empty_like(H_old) if want_psi else None
-        plan = fem.plan(self.dt)
         fd = None
+        if fem.general is not None:  # general quadrilaterals: history array always written
+            g = fem.general.handle
+            if H_old is None:
+                H_old = torch.zeros((fem.mesh.n_elems, 4), dtype=F64, device=fem.device)
+                H_scr = torch.empty_like(H_old)
+            else:
+                H_scr = H_new
+            if self._damping_beta > 0.0:
+                fd = torch.empty_like(v)
+                check(_lib.lib().phast_t3_damping_force(g, ptr(v), ptr(d), self._damping_beta, ptr(fd), stream()),
+                      "damping_force")
+            check(_lib.lib().phast_t3_explicit_step(
+                g, float(self.dt), ptr(u), ptr(v), ptr(a), ptr(d), ptr(fe), ptr(fd), ptr(fem.M_inv_node), ptr(m),
+                ptr(bv), ptr(H_old), ptr(u1), ptr(v1), ptr(a1), ptr(H_scr), ptr(psi), stream()), "explicit_step")
+            return u1, v1, a1, H_new, psi
+        plan = fem.plan(self.dt)
         if self._damping_beta > 0.0:  # mechanics_solver.py:192-195
             fd = torch.empty_like(v)
             check(_lib.lib().phast_q4_damping_force(plan.handle, ptr(v), ptr(d), self._damping_beta, ptr(fd),
@@ -501,6 +598,10 @@ class PhaseFieldDamageSolver:
         n = self.mesh.n_nodes
         b = torch.empty(n, dtype=F64, device=self.fem.device)
         diag = torch.empty_like(b)
+        if self.fem.general is not None:
+            check(_lib.lib().phast_t3_damage_rhs_diag(self.fem.general.handle, ptr(H), ptr(self.Gc_field), ptr(b),
+                                                      ptr(diag), stream()), "damage_rhs_diag")
+            return b, diag
         check(_lib.lib().phast_q4_damage_rhs_diag(self.fem.plan(0.0).handle, ptr(H), ptr(self.Gc_field), ptr(b),
                                                   ptr(diag), stream()), "damage_rhs_diag")
         return b, diag
@@ -510,6 +611,10 @@ class PhaseFieldDamageSolver:
         H = self._normalize_H(H_input)
         x = self.fem._d(x, "x")
         y = torch.empty_like(x)
+        if self.fem.general is not None:
+            check(_lib.lib().phast_t3_damage_apply(self.fem.general.handle, ptr(H), ptr(self.Gc_field), ptr(x),
+                                                   ptr(y), stream()), "damage_apply")
+            return y
         check(_lib.lib().phast_q4_damage_apply(self.fem.plan(0.0).handle, ptr(H), ptr(self.Gc_field), ptr(x),
                                                ptr(y), stream()), "damage_apply")
         return y
@@ -542,9 +647,14 @@ class PhaseFieldDamageSolver:
         active = torch.empty(self.mesh.n_nodes, dtype=torch.uint8, device=self.fem.device) if method == 0 else None
         iters = (C.c_int32 * 1)()
         status = (C.c_int32 * 1)()
-        check(_lib.lib().phast_q4_damage_solve(
-            self.fem.plan(0.0).handle, method, ptr(H), ptr(gcf), ptr(d_prev), ptr(init), ptr(fixed), ptr(vals),
-            self.tol, self.max_iter, ptr(d_out), ptr(active), iters, status, stream()), "damage_solve")
+        if self.fem.general is not None:
+            check(_lib.lib().phast_t3_damage_solve(
+                self.fem.general.handle, method, ptr(H), ptr(gcf), ptr(d_prev), ptr(init), ptr(fixed), ptr(vals),
+                self.tol, self.max_iter, ptr(d_out), ptr(active), iters, status, stream()), "damage_solve")
+        else:
+            check(_lib.lib().phast_q4_damage_solve(
+                self.fem.plan(0.0).handle, method, ptr(H), ptr(gcf), ptr(d_prev), ptr(init), ptr(fixed), ptr(vals),
+                self.tol, self.max_iter, ptr(d_out), ptr(active), iters, status, stream()), "damage_solve")
         self.last_iter = int(iters[0])
         self.last_status = _lib.CG_STATUS_NAMES.get(int(status[0]), str(status[0]))
         self.last_converged = int(status[0]) == _lib.CG_CONVERGED

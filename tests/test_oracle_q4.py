@@ -257,3 +257,89 @@ def test_projected_cg_matches_bruteforce_box_qp():
     dmg = O.PhaseFieldDamageSolver(fem, tol=1e-14, max_iter=2000, bounds_method="projected_cg")
     d = dmg.solve(H, lb)
     np.testing.assert_allclose(d, best, atol=1e-9)
+
+
+# -- general (distorted, renumbered) quadrilateral meshes ---------------------------------
+
+def distorted_q4_mesh(nx=4, ny=3, W=4.0, H=3.0, amp=0.22, seed=0, shuffle=True):
+    """A structured grid whose interior nodes are moved by up to ``amp`` cells and whose
+    nodes and cells are renumbered at random (corners stay counter-clockwise)."""
+    nodes, quads = O.structured_q4_mesh(W, H, nx, ny)
+    rng = np.random.default_rng(seed)
+    hx, hy = W / nx, H / ny
+    x, y = nodes[:, 0], nodes[:, 1]
+    inner = (x > 0) & (x < W) & (y > 0) & (y < H)
+    nodes = nodes.copy()
+    nodes[inner, 0] += amp * hx * rng.uniform(-1, 1, inner.sum())
+    nodes[inner, 1] += amp * hy * rng.uniform(-1, 1, inner.sum())
+    if shuffle:
+        perm = rng.permutation(nodes.shape[0])       # new id of old node i: inv[i]
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(perm.size)
+        nodes = nodes[perm]
+        quads = inv[quads]
+        quads = quads[rng.permutation(quads.shape[0])]
+        quads = np.stack([np.roll(q, int(r)) for q, r in zip(quads, rng.integers(0, 4, quads.shape[0]))])
+    return nodes, quads
+
+
+def _shoelace(p):
+    x, y = p[:, 0], p[:, 1]
+    return 0.5 * float(np.dot(x, np.roll(y, -1)) - np.dot(np.roll(x, -1), y))
+
+
+def test_distorted_mesh_geometry_and_patch_test():
+    nodes, quads = distorted_q4_mesh()
+    mesh = O.Q4Mesh(nodes, quads)
+    for e in range(mesh.n_elems):   # sum_q wdetJ is the cell's polygon area (bilinear map, exact)
+        assert math.isclose(mesh.wdet[e].sum(), _shoelace(nodes[quads[e]]), rel_tol=1e-13)
+    assert (mesh.wdet > 0).all()
+    np.testing.assert_allclose(mesh.grad.sum(axis=2), 0.0, atol=1e-12)
+    assert math.isclose(mesh.M_scalar.sum(), 12.0, rel_tol=1e-13)
+    # a linear field has the exact constant gradient at every Gauss point of every cell
+    G = np.array([[1.3e-3, -0.4e-3], [0.7e-3, 2.1e-3]])   # du_i/dx_j
+    u = nodes @ G.T + np.array([0.1, -0.2])
+    mat = O.Material(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5, energy_split="isotropic")
+    fem = O.FEMOperators(mesh, mat)
+    exx, eyy, gxy = fem.compute_strain(u)
+    np.testing.assert_allclose(exx, G[0, 0], rtol=1e-11)
+    np.testing.assert_allclose(eyy, G[1, 1], rtol=1e-11)
+    np.testing.assert_allclose(gxy, G[0, 1] + G[1, 0], rtol=1e-11)
+    x, y = nodes[:, 0], nodes[:, 1]
+    interior = (x > 1e-9) & (x < 4 - 1e-9) & (y > 1e-9) & (y < 3 - 1e-9)
+    f = fem.internal_force(u, np.full(mesh.n_nodes, 0.2))
+    assert np.abs(f[interior]).max() < 1e-11 * np.abs(f).max()
+
+
+def test_distorted_amor_force_is_energy_gradient_and_damage_operator_is_the_assembly():
+    nodes, quads = distorted_q4_mesh(seed=3)
+    mesh = O.Q4Mesh(nodes, quads)
+    mat = O.Material(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.5, energy_split="amor")
+    fem = O.FEMOperators(mesh, mat)
+    rng = np.random.default_rng(4)
+    u = rng.uniform(-1e-3, 1e-3, (mesh.n_nodes, 2))
+    d = rng.uniform(0, 1, mesh.n_nodes)
+    f = fem.internal_force(u, d)
+    h = 1e-7
+    fd = np.zeros_like(u)
+    for i in range(mesh.n_nodes):
+        for c in range(2):
+            up, um = u.copy(), u.copy()
+            up[i, c] += h
+            um[i, c] -= h
+            fd[i, c] = (_energy(fem, up, d) - _energy(fem, um, d)) / (2 * h)
+    np.testing.assert_allclose(f, fd, rtol=1e-6, atol=1e-6 * np.abs(f).max())
+    dmg = O.PhaseFieldDamageSolver(fem, tol=1e-12)
+    H = rng.uniform(0, 1e-3, (mesh.n_elems, 4))
+    A, b_ref = _independent_damage_matrix(mesh, mat.Gc, mat.l0, H)
+    rho, b, M_inv = dmg.prepare(H, np.zeros(mesh.n_nodes))
+    x = rng.uniform(0, 1, mesh.n_nodes)
+    np.testing.assert_allclose(dmg.Ax(x, rho), A @ x, rtol=1e-13)
+    np.testing.assert_allclose(b, b_ref, rtol=1e-13)
+    # the Laplacian part annihilates constants and, away from the boundary, linear fields
+    lin = 0.3 * nodes[:, 0] - 0.2 * nodes[:, 1] + 1.0
+    lap = dmg.Ax(lin, np.zeros_like(rho))
+    xs, ys = nodes[:, 0], nodes[:, 1]
+    interior = (xs > 1e-9) & (xs < 4 - 1e-9) & (ys > 1e-9) & (ys < 3 - 1e-9)
+    assert np.abs(lap[interior]).max() < 1e-12 * max(np.abs(lap).max(), 1e-30) + 1e-15
+    assert np.abs(dmg.Ax(np.ones(mesh.n_nodes), np.zeros_like(rho))).max() < 1e-13
