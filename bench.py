@@ -54,6 +54,12 @@ T3_CG_BYTES_PER_NODE = 88.0 + 65.0 + 25.0 + 9.0
 # one iteration of the adjoint PCG of the backward: A p (p 8, H 16, xy 16, conn 24; Ap 8
 # written), x/r/z update (x, p, r, Ap, Minv read 40; x, r, z written 24), direction (z, p 16; p 8)
 T3_ADJ_BYTES_PER_NODE = 72.0 + 64.0 + 24.0
+# general quadrilateral meshes (one cell per node): the force reads u 16, d 8, the corner
+# coordinates 16 and the connectivity 16 and writes f 16 (72 B/node algorithmic); the element
+# pass also writes 64 B of corner values per cell that the node pass reads back with the
+# incidence list (20 B), 148 B/node more
+Q4G_APPLY_BYTES_PER_NODE = 72.0
+Q4G_APPLY_COUNTED_PER_NODE = 72.0 + 148.0
 
 
 def parse():
@@ -63,8 +69,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary measurements")
-    ap.add_argument("--secondary", type=str, default="q4,hex8,t3,sent", help="secondary parts to run: q4, hex8, t3, "
-                    "sent (the 200 x 80 SENT plate run end to end)")
+    ap.add_argument("--secondary", type=str, default="q4,q4g,hex8,t3,sent",
+                    help="secondary parts to run: q4, q4g (a general quadrilateral mesh), hex8, t3, sent (the 200 x 80 "
+                    "SENT plate run end to end)")
     ap.add_argument("--q4-tile", type=str, default="", help="fixed Q4 strip threads,rows (skips the autotuner)")
     ap.add_argument("--hex8-big-tiles", type=str, default="", help="fixed tx,ty,kz of the 128^3/256^3 runs")
     ap.add_argument("--batch-tile", type=str, default="", help="fixed strip of the batched run (skips its autotuner)")
@@ -322,6 +329,8 @@ def secondary(rng, dev, a):
             entry.update(q4_cg_iteration(P, fem, rng, T))
         res[name] = entry
         del fem, dyn, u, d, v, acc, fext, Hq
+    if "q4g" in parts:
+        res["q4_general_distorted_4096x2048"] = q4_general_secondary(rng, dev, a)
     if "t3" in parts:
         res["t3_split_quad_4096x2048"] = t3_secondary(rng, dev, a)
     if "sent" in parts:
@@ -445,6 +454,57 @@ def q4_cg_iteration(P, fem, rng, T):
         out = {"cg_iteration_ms": t_it * 1e3, "cg_iteration_GB_s": Q4_CG_BYTES_PER_NODE * n / t_it / 1e9,
                "cg_iterations_timed": [i1, i2], "cg_status": st2}
     return out
+
+
+def q4_general_secondary(rng, dev, a):
+    """Q4 on a general quadrilateral mesh: a 4096 x 2048-cell plate whose interior nodes are
+    moved by up to 20 % of a cell (8.4 M nodes, 8.4 M cells; it no longer qualifies for the
+    strip kernels): internal force, explicit step with psi+ and history, damage apply and one
+    projected damage-CG iteration on the general-quadrilateral plan."""
+    import time
+    import numpy as np
+    import torch
+    from phast_amd import q4 as P
+    from phast_amd import workloads as W
+    T = lambda z: torch.as_tensor(z, dtype=torch.float64, device=dev)
+    nx, ny, L = 4096, 2048, 0.125
+    nodes, cells = W.structured_plate(nx * L, ny * L, nx, ny)
+    inner = (nodes[:, 0] > 0) & (nodes[:, 0] < nx * L) & (nodes[:, 1] > 0) & (nodes[:, 1] < ny * L)
+    nodes = nodes.copy()
+    nodes[inner] += 0.2 * L * rng.uniform(-1, 1, (int(inner.sum()), 2))
+    mesh = P.FEMMesh(nodes, cells, device=dev)
+    fem = P.FEMOperators(mesh, P.Material(E=32000.0, nu=0.2, rho=2.45e-9, Gc=3e-3, l0=0.25))
+    n, E = mesh.n_nodes, mesh.n_elems
+    u = T(rng.uniform(-1e-4, 1e-4, (n, 2)))
+    d = T(rng.uniform(0, 1, n))
+    steps = max(10, a.steps // 10)
+    t, _ = events_time(lambda: fem.internal_force(u, d), steps, 3)
+    entry = {"nodes": n, "cells": E, "structured": mesh.structured, "apply_GDOF_s": 2 * n / t / 1e9,
+             "apply_GB_s": Q4G_APPLY_BYTES_PER_NODE * n / t / 1e9,
+             "apply_design_GB_s": Q4G_APPLY_COUNTED_PER_NODE * n / t / 1e9, "apply_ms": t * 1e3}
+    dyn = P.ExplicitDynamics(fem, dt=0.5 * fem.dt_cfl)
+    v, acc, fext = T(rng.uniform(-1e-2, 1e-2, (n, 2))), T(np.zeros((n, 2))), T(np.zeros((n, 2)))
+    Hq = T(np.zeros((E, 4)))
+    t2, _ = events_time(lambda: dyn._launch(u, v, acc, d, fext, None, None, H_old=Hq, want_psi=True), steps, 3)
+    dmg = P.PhaseFieldDamageSolver(fem)
+    Hd = T(rng.uniform(0, 1e-3, (E, 4)))
+    t3, _ = events_time(lambda: dmg.damage_apply(Hd, d), steps, 3)
+    entry.update({"explicit_step_ms": t2 * 1e3, "damage_apply_ms": t3 * 1e3})
+    walls = {}
+    for k in (20, 60):
+        solver = P.PhaseFieldDamageSolver(fem, tol=0.0, max_iter=k, bounds_method="projected_cg")
+        d0 = torch.zeros(n, dtype=torch.float64, device=dev)
+        solver.solve(Hd, d0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.solve(Hd, d0)
+        torch.cuda.synchronize()
+        walls[k] = (time.perf_counter() - t0, solver.last_iter)
+    if walls[60][1] > walls[20][1]:
+        entry["cg_iteration_ms"] = (walls[60][0] - walls[20][0]) / (walls[60][1] - walls[20][1]) * 1e3
+    del fem, dyn, dmg, u, d, v, acc, fext, Hq, Hd
+    torch.cuda.empty_cache()
+    return entry
 
 
 def t3_secondary(rng, dev, a):
