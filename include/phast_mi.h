@@ -343,7 +343,8 @@ typedef struct phast_hex8_plan phast_hex8_plan;
  * (solid3d/periodic.py:61-108, 193-221).  bank: host (n_bank,6,6) engineering-Voigt
  * stiffnesses; bank_index: host (E,) int32 voxel -> bank entry; Gc, lam, mu: host (E,);
  * driver: host (E,) int8 (0 none, 1 full, 2 spectral; solid3d/operator.py:1415-1444);
- * kappa: residual stiffness; ell: length scale (solid3d/operator.py:229-251). */
+ * kappa: residual stiffness; ell: length scale (solid3d/operator.py:229-251).
+ * nx, ny, nz >= 2; nz <= 65535. */
 int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,
                            const double *bank, int32_t n_bank, const int32_t *bank_index,
                            const double *Gc, const double *lam, const double *mu, const int8_t *driver,

@@ -210,9 +210,19 @@ constexpr int apply_threads() {
     return ((OX + 1) * (OY + 1) + 63) / 64 * 64;
 }
 
+// Waves per SIMD the register budget must allow: a block of four waves puts one wave on each
+// SIMD, so two resident blocks per CU need the registers of two waves per SIMD (at most 256
+// per lane); larger blocks place two waves on a SIMD by themselves.  Only the isotropic
+// variants without the energy sum fit that budget without spilling; the others keep one wave.
+template <int OX, int OY, bool ISO, bool ENERGY>
+constexpr int apply_min_waves() {
+    return (apply_threads<OX, OY>() <= 256 && ISO && !ENERGY) ? 2 : 1;
+}
+
 // CL: crack closure off (0), at each point's own sign pattern (1) or at A.pat (2)
 template <int OX, int OY, int GMODE, bool ISO, bool ENERGY, int CL = 0>
-__global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_apply_kernel(const Hex8Geo G, MechArgs3 A) {
+__global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY, ISO, ENERGY>())) void hex8_apply_kernel(
+    const Hex8Geo G, MechArgs3 A) {
     constexpr int NT = apply_threads<OX, OY>();
     constexpr int NVX = OX + 1, NVY = OY + 1, NV = NVX * NVY;
     constexpr int NNX = OX + 2, NNY = OY + 2, NN = NNX * NNY;
@@ -412,14 +422,16 @@ __global__ __launch_bounds__(256) void hex8_finalize_kernel(const double *part, 
         }
 }
 
-// After the voxel-parallel apply: block 0 sums the macro-row partials (as
-// hex8_finalize_kernel); the other blocks add each slab's seam (top-face sums of the voxel
-// layer below its first node layer, computed by the slab below) to that node layer:
-// y = seam + bottom, the same sum the marching kernel forms.
+// After the voxel-parallel apply: block (0, 0) sums the macro-row partials (as
+// hex8_finalize_kernel); the blocks (x > 0, slab) add that slab's seam (top-face sums of the
+// voxel layer below its first node layer, computed by the slab below) to that node layer:
+// y = seam + bottom, the same sum the marching kernel forms.  Grid: (1 + ceil(3 nx ny / 256),
+// slabs); the slab is the block row, so no thread divides an index.
 __global__ __launch_bounds__(256) void hex8_seam_finalize_kernel(const Hex8Geo G, const double *seam, int kz,
-                                                                 int nslabs, double *y, const double *part,
-                                                                 int nblocks, int nvals, double *y6, double *out7) {
+                                                                 double *y, const double *part, int nblocks,
+                                                                 int nvals, double *y6, double *out7) {
     if (blockIdx.x == 0) {
+        if (blockIdx.y != 0) return;
         __shared__ double red[7 * 4];
         double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int b = threadIdx.x; b < nblocks; b += 256)
@@ -434,23 +446,24 @@ __global__ __launch_bounds__(256) void hex8_seam_finalize_kernel(const Hex8Geo G
             }
         return;
     }
-    const int64_t nxy = (int64_t)G.nx * G.ny;
-    const int64_t i = (int64_t)(blockIdx.x - 1) * 256 + threadIdx.x;  // (slab, node in layer, component)
-    if (i >= 3 * nxy * nslabs) return;
-    const int64_t q = i / 3, sl = q / nxy;
-    const int64_t at = 3 * (q - sl * nxy + nxy * (sl * kz)) + (i - 3 * q);
-    y[at] = seam[i] + y[at];
+    const int64_t m = 3 * (int64_t)G.nx * G.ny;  // doubles per node layer
+    const int64_t i = (int64_t)(blockIdx.x - 1) * 256 + threadIdx.x;
+    if (i >= m) return;
+    const int64_t sl = blockIdx.y, at = m * (sl * kz) + i;
+    y[at] = seam[m * sl + i] + y[at];
 }
 
-// A multigrid smoothing step's finalize: the macro rows (block 0) and, after the
+// A multigrid smoothing step's finalize: the macro rows (block (0, 0)) and, after the
 // voxel-parallel apply, the seams, fused with the damped Jacobi update
 // x += omega (b - A x) / diag on free dofs (fixed dofs stay 0) -- k_jacobi's arithmetic.
+// Grid: (1 + ceil(3 nx ny / 256), nz), one block row per node layer.
 __global__ __launch_bounds__(256) void hex8_finalize_jacobi_kernel(const Hex8Geo G, const double *seam, int kz,
                                                                    double *Ax, const double *part, int nblocks,
                                                                    const double *b, const double *diag,
                                                                    const uint8_t *fixed, double omega, double *x) {
-    const int64_t nxy = (int64_t)G.nx * G.ny, n3 = 3 * nxy * G.nz;
+    const int64_t m = 3 * (int64_t)G.nx * G.ny, n3 = m * G.nz;
     if (blockIdx.x == 0) {
+        if (blockIdx.y != 0) return;
         __shared__ double red[6 * 4];
         double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int bb = threadIdx.x; bb < nblocks; bb += 256)
@@ -471,15 +484,14 @@ __global__ __launch_bounds__(256) void hex8_finalize_jacobi_kernel(const Hex8Geo
         }
         return;
     }
-    const int64_t i = (int64_t)(blockIdx.x - 1) * 256 + threadIdx.x;
-    if (i >= n3) return;
+    const int64_t il = (int64_t)(blockIdx.x - 1) * 256 + threadIdx.x;  // within the node layer
+    if (il >= m) return;
+    const int layer = blockIdx.y;
+    const int64_t i = m * layer + il;
     double v = Ax[i];
-    if (seam) {
-        const int64_t q = i / 3, layer = q / nxy;
-        if (layer % kz == 0) {
-            v = seam[3 * ((layer / kz) * nxy + (q - layer * nxy)) + (i - 3 * q)] + v;
-            Ax[i] = v;
-        }
+    if (seam && layer % kz == 0) {
+        v = seam[m * (layer / kz) + il] + v;
+        Ax[i] = v;
     }
     if (fixed[i]) {
         x[i] = 0.0;
@@ -833,9 +845,10 @@ static int launch_mech_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, bool
 #undef PHAST_L
     PHAST_LAUNCHED();
     if (jac)
-        hipLaunchKernelGGL(hex8_finalize_jacobi_kernel, dim3((unsigned)(1 + ceil_div(3 * pl->n, 256))), dim3(256), 0,
-                           st, pl->geo, (const double *)nullptr, 1, A.y, A.part, nb, jac->b, jac->diag, jac->fixed,
-                           jac->omega, jac->x);
+        hipLaunchKernelGGL(hex8_finalize_jacobi_kernel,
+                           dim3((unsigned)(1 + ceil_div(3 * (int64_t)pl->geo.nx * pl->geo.ny, 256)), pl->geo.nz),
+                           dim3(256), 0, st, pl->geo, (const double *)nullptr, 1, A.y, A.part, nb, jac->b, jac->diag,
+                           jac->fixed, jac->omega, jac->x);
     else
         hipLaunchKernelGGL(hex8_finalize_kernel, dim3(1), dim3(256), 0, st, A.part, nb, A.nvals, A.y6, A.out7);
     PHAST_LAUNCHED();
@@ -902,14 +915,13 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
 #undef PHAST_L
 #undef PHAST_LC
     PHAST_LAUNCHED();
-    const int64_t nseam = 3 * (int64_t)pl->geo.nx * pl->geo.ny * nbz;
+    const unsigned layer_blocks = (unsigned)(1 + ceil_div(3 * (int64_t)pl->geo.nx * pl->geo.ny, 256));
     if (jac)
-        hipLaunchKernelGGL(hex8_finalize_jacobi_kernel, dim3((unsigned)(1 + ceil_div(3 * pl->n, 256))), dim3(256), 0,
-                           st, pl->geo, A.seam, A.kz, A.y, A.part, nb, jac->b, jac->diag, jac->fixed, jac->omega,
-                           jac->x);
+        hipLaunchKernelGGL(hex8_finalize_jacobi_kernel, dim3(layer_blocks, pl->geo.nz), dim3(256), 0, st, pl->geo,
+                           A.seam, A.kz, A.y, A.part, nb, jac->b, jac->diag, jac->fixed, jac->omega, jac->x);
     else
-        hipLaunchKernelGGL(hex8_seam_finalize_kernel, dim3((unsigned)(1 + ceil_div(nseam, 256))), dim3(256), 0, st,
-                           pl->geo, A.seam, A.kz, nbz, A.y, A.part, nb, A.nvals, A.y6, A.out7);
+        hipLaunchKernelGGL(hex8_seam_finalize_kernel, dim3(layer_blocks, nbz), dim3(256), 0, st, pl->geo, A.seam,
+                           A.kz, A.y, A.part, nb, A.nvals, A.y6, A.out7);
     PHAST_LAUNCHED();
     return PHAST_OK;
 }
@@ -1185,6 +1197,13 @@ void phast::voxel_constants(Hex8Geo &G, double hx, double hy, double hz, double 
     G.bey = G.be * G.ihy;
     G.alz = G.al * G.ihz;
     G.bez = G.be * G.ihz;
+    G.c3 = a;
+    G.kx = 0.25 / hx;
+    G.ky = 0.25 / hy;
+    G.kz = 0.25 / hz;
+    G.kcx = G.kx * a;
+    G.kcy = G.ky * a;
+    G.kcz = G.kz * a;
 }
 
 
@@ -1194,6 +1213,7 @@ extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double
                                       double kappa, double ell, phast_hex8_plan **out) {
     PHAST_REQUIRE(out && bank && bank_index && Gc && lam && mu && driver, "phast_hex8_plan_create: null argument");
     PHAST_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "periodic voxel cell needs at least 2 voxels per axis");
+    PHAST_REQUIRE(nz <= 65535, "periodic voxel cell: at most 65535 voxel layers along z (one grid row per layer)");
     PHAST_REQUIRE(hx > 0 && hy > 0 && hz > 0 && ell > 0 && n_bank >= 1, "bad geometry / bank size");
     auto *pl = new phast_hex8_plan();
     std::memset(&pl->geo, 0, sizeof(pl->geo));

@@ -74,6 +74,106 @@ __device__ __forceinline__ void voxel_grad(const Hex8Geo &G, const double u[3][8
     }
 }
 
+// Walsh-Hadamard modes of the trilinear field with corner values u: W_S = sum_a s_S(a) u[a]
+// with s_x(a) = 2 lx - 1 etc., so that u = (1/8) sum_S W_S x^Sx y^Sy z^Sz on the centred
+// voxel [-1,1]^3.  W = {x, y, z, xy, xz, yz, xyz}; the constant mode is not formed.
+__device__ __forceinline__ void voxel_modes(const double u[8], double W[7]) {
+    double s[4], d[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {  // j = ly + 2 lz
+        s[j] = u[2 * j + 1] + u[2 * j];
+        d[j] = u[2 * j + 1] - u[2 * j];
+    }
+    const double ss0 = s[1] + s[0], ss1 = s[3] + s[2], sd0 = s[1] - s[0], sd1 = s[3] - s[2];
+    const double ds0 = d[1] + d[0], ds1 = d[3] + d[2], dd0 = d[1] - d[0], dd1 = d[3] - d[2];
+    W[0] = ds1 + ds0;
+    W[1] = sd1 + sd0;
+    W[2] = ss1 - ss0;
+    W[3] = dd1 + dd0;
+    W[4] = ds1 - ds0;
+    W[5] = sd1 - sd0;
+    W[6] = dd1 - dd0;
+}
+
+// voxel_grad followed by fold_macro, from the modes: at the Gauss point (sx, sy, sz) c3
+// du/dX = kx (W_x + c3 sy W_xy + c3 sz W_xz + c3^2 sy sz W_xyz), kx = 1/(4 hx), and alike for
+// y and z; the y and z derivatives share W_yz + c3 sx W_xyz.  The macro strain entries are
+// added to the constant terms kx W_x, ky W_y, kz W_z once instead of at four points each.
+__device__ __forceinline__ void voxel_grad_macro(const Hex8Geo &G, const double u[3][8], const double E[6],
+                                                 double gx[3][2][2], double gy[3][2][2], double gz[3][2][2]) {
+    const double c = G.c3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double W[7];
+        voxel_modes(u[k], W);
+        double bx = G.kx * W[0], by = G.ky * W[1], bz = G.kz * W[2];
+        if (k == 0) {  // gx[0] += E0, gy[0] += E5, gz[0] += E4
+            bx += E[0];
+            by += E[5];
+            bz += E[4];
+        } else if (k == 1) {  // gy[1] += E1, gz[1] += E3
+            by += E[1];
+            bz += E[3];
+        } else {  // gz[2] += E2
+            bz += E[2];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const double si = i ? 1.0 : -1.0;
+            const double X = fma(si * G.kcx, W[3], bx);  // x derivative at sy = si
+            const double Y = fma(si * c, W[6], W[4]);
+            gx[k][i][0] = fma(-G.kcx, Y, X);
+            gx[k][i][1] = fma(G.kcx, Y, X);
+            const double V = fma(si * c, W[6], W[5]);  // shared by y and z at sx = si
+            const double Xy = fma(si * G.kcy, W[3], by);
+            gy[k][i][0] = fma(-G.kcy, V, Xy);
+            gy[k][i][1] = fma(G.kcy, V, Xy);
+            const double Xz = fma(si * G.kcz, W[4], bz);
+            gz[k][i][0] = fma(-G.kcz, V, Xz);
+            gz[k][i][1] = fma(G.kcz, V, Xz);
+        }
+    }
+}
+
+// Transpose of voxel_grad_macro without the macro terms: corner forces
+// r[a] = sum_q (dN_a/dX Sx + dN_a/dY Sy + dN_a/dZ Sz)(q) for the Gauss-point sums
+// Sx[iy][iz] (already summed over ix), Sy[ix][iz], Sz[ix][iy] of one component.  The sums
+// T of Sx, Sy, Sz over their four points (the macro stress rows) are returned in Tsum.
+__device__ __forceinline__ void voxel_gradT(const Hex8Geo &G, const double Sx[2][2], const double Sy[2][2],
+                                            const double Sz[2][2], double r[8], double Tsum[3]) {
+    const double c = G.c3;
+    // 2x2 butterflies: (sum, second-index sign, first-index sign, both) of each direction
+    const double ax0 = Sx[1][0] + Sx[0][0], ax1 = Sx[1][1] + Sx[0][1];
+    const double bx0 = Sx[1][0] - Sx[0][0], bx1 = Sx[1][1] - Sx[0][1];
+    const double ay0 = Sy[1][0] + Sy[0][0], ay1 = Sy[1][1] + Sy[0][1];
+    const double by0 = Sy[1][0] - Sy[0][0], by1 = Sy[1][1] - Sy[0][1];
+    const double az0 = Sz[1][0] + Sz[0][0], az1 = Sz[1][1] + Sz[0][1];
+    const double bz0 = Sz[1][0] - Sz[0][0], bz1 = Sz[1][1] - Sz[0][1];
+    Tsum[0] = ax1 + ax0;
+    Tsum[1] = ay1 + ay0;
+    Tsum[2] = az1 + az0;
+    // force modes: F_x = kx T_x ...; the mixed ones before their c3 factors
+    const double Fx = G.kx * Tsum[0], Fy = G.ky * Tsum[1], Fz = G.kz * Tsum[2];
+    const double Gxy = fma(G.kx, bx1 + bx0, G.ky * (by1 + by0));
+    const double Gxz = fma(G.kx, ax1 - ax0, G.kz * (bz1 + bz0));
+    const double Gyz = fma(G.ky, ay1 - ay0, G.kz * (az1 - az0));
+    const double Gxyz = fma(G.kx, bx1 - bx0, fma(G.ky, by1 - by0, G.kz * (bz1 - bz0)));
+    // r(sx,sy,sz) = sx Fx + sy Fy + sz Fz + c3 (sx sy Gxy + sx sz Gxz + sy sz Gyz) + c3^2 sx sy sz Gxyz
+#pragma unroll
+    for (int ly = 0; ly < 2; ++ly) {
+        const double sy = ly ? 1.0 : -1.0;
+        const double U = fma(sy * c, Gxy, Fx);
+        const double Z = fma(sy * c, Gyz, Fz);
+        const double V = fma(sy * c, Gxyz, Gxz);
+        const double P0 = sy * Fy - U, P1 = sy * Fy + U;  // lx = 0, 1
+        const double Q0 = fma(-c, V, Z), Q1 = fma(c, V, Z);
+        r[2 * ly] = P0 - Q0;
+        r[2 * ly + 1] = P1 - Q1;
+        r[2 * ly + 4] = P0 + Q0;
+        r[2 * ly + 5] = P1 + Q1;
+    }
+}
+
 // engineering strain (xx,yy,zz,yz,xz,xy) at Gauss point (ix,iy,iz); the macro strain has
 // been added into the gradient sets beforehand (fold_macro).
 __device__ __forceinline__ void qp_strain(const double gx[3][2][2], const double gy[3][2][2],
@@ -184,11 +284,11 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
                                            double lamc = 0.0, double muc = 0.0, uint32_t pat = 0u,
                                            const double *__restrict__ T = nullptr, bool clon = false) {
     double gx[3][2][2], gy[3][2][2], gz[3][2][2];
-    voxel_grad(G, u, gx, gy, gz);
-    fold_macro(E, gx, gy, gz);
+    voxel_grad_macro(G, u, E, gx, gy, gz);
     // each partial sum takes two Gauss points: the first assigns, the second adds
     double Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
     double ew = 0.0;
+    const double lw = lam * G.w, mw = mu * G.w, mw2 = 2.0 * mw;  // (ISO) moduli times the weight
 #pragma unroll
     for (int iz = 0; iz < 2; ++iz)
 #pragma unroll
@@ -197,10 +297,10 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
             for (int ix = 0; ix < 2; ++ix) {
                 double e[6];
                 qp_strain(gx, gy, gz, ix, iy, iz, e);
-                const double gw = g[ix + 2 * iy + 4 * iz] * G.w;
                 double s[6];
                 if (ISO) {
-                    const double lg = lam * gw, mg = mu * gw, mg2 = 2.0 * mg;
+                    const double gq = g[ix + 2 * iy + 4 * iz];
+                    const double lg = lw * gq, mg = mw * gq, mg2 = mw2 * gq;
                     const double lt = lg * ((e[0] + e[1]) + e[2]);
                     s[0] = lt + mg2 * e[0];
                     s[1] = lt + mg2 * e[1];
@@ -231,6 +331,7 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
                         s[2] += cw * (ct + (b3 ? m2 * e[2] : 0.0));
                     }
                 } else {
+                    const double gw = g[ix + 2 * iy + 4 * iz] * G.w;
 #pragma unroll
                     for (int k = 0; k < 6; ++k) {
                         double t = 0.0;
@@ -289,42 +390,17 @@ __device__ __forceinline__ void voxel_mech(const Hex8Geo &G, const double u[3][8
                     Sz[2][ix][iy] += s[2];
                 }
             }
-    if (own) {  // sum_q w sigma_q from the partial sums (each component appears once per channel)
-        sE[0] += (Sx[0][0][0] + Sx[0][1][0]) + (Sx[0][0][1] + Sx[0][1][1]);
-        sE[1] += (Sy[1][0][0] + Sy[1][1][0]) + (Sy[1][0][1] + Sy[1][1][1]);
-        sE[2] += (Sz[2][0][0] + Sz[2][1][0]) + (Sz[2][0][1] + Sz[2][1][1]);
-        sE[3] += (Sy[2][0][0] + Sy[2][1][0]) + (Sy[2][0][1] + Sy[2][1][1]);
-        sE[4] += (Sx[2][0][0] + Sx[2][1][0]) + (Sx[2][0][1] + Sx[2][1][1]);
-        sE[5] += (Sx[1][0][0] + Sx[1][1][0]) + (Sx[1][0][1] + Sx[1][1][1]);
+    double Ts[3][3];  // Ts[k][dir]: sum over the four points of the direction's sums
+#pragma unroll
+    for (int k = 0; k < 3; ++k) voxel_gradT(G, Sx[k], Sy[k], Sz[k], r[k], Ts[k]);
+    if (own) {  // sum_q w sigma_q: each component appears once among the sums
+        sE[0] += Ts[0][0];
+        sE[1] += Ts[1][1];
+        sE[2] += Ts[2][2];
+        sE[3] += Ts[2][1];
+        sE[4] += Ts[2][0];
+        sE[5] += Ts[1][0];
         if (ENERGY) en += 0.5 * ew;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double R[2][2];
-        interp2Ts(G.al, G.be, G.alx, G.bex, Sx[c], R);  // R[ly][lz]
-#pragma unroll
-        for (int ly = 0; ly < 2; ++ly)
-#pragma unroll
-            for (int lz = 0; lz < 2; ++lz) {
-                r[c][1 + 2 * ly + 4 * lz] = R[ly][lz];
-                r[c][2 * ly + 4 * lz] = -R[ly][lz];
-            }
-        interp2Ts(G.al, G.be, G.aly, G.bey, Sy[c], R);  // R[lx][lz]
-#pragma unroll
-        for (int lx = 0; lx < 2; ++lx)
-#pragma unroll
-            for (int lz = 0; lz < 2; ++lz) {
-                r[c][lx + 2 + 4 * lz] += R[lx][lz];
-                r[c][lx + 4 * lz] -= R[lx][lz];
-            }
-        interp2Ts(G.al, G.be, G.alz, G.bez, Sz[c], R);  // R[lx][ly]
-#pragma unroll
-        for (int lx = 0; lx < 2; ++lx)
-#pragma unroll
-            for (int ly = 0; ly < 2; ++ly) {
-                r[c][lx + 2 * ly + 4] += R[lx][ly];
-                r[c][lx + 2 * ly] -= R[lx][ly];
-            }
     }
 }
 

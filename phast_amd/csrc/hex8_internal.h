@@ -11,6 +11,9 @@ struct Hex8Geo {
     double w;               // Gauss weight * det J = hx hy hz / 8
     double al, be;          // linear interpolation weights (1 +- 1/sqrt3)/2
     double alx, bex, aly, bey, alz, bez;  // al/h, be/h per axis (interpolation + derivative scale)
+    double c3;              // Gauss point in centred coordinates, 1/sqrt3
+    double kx, ky, kz;      // 1/(4h): derivative of the Walsh-Hadamard modes per axis
+    double kcx, kcy, kcz;   // k c3 per axis
     double kappa, omk, ch;  // residual stiffness, 1-kappa, 2(1-kappa)
     double K[64];           // AT2 voxel matrix M/ell + ell L, lexicographic corners [a*8+b]
     double Nq[64];          // N[q*8 + a], lexicographic corners, xi-fastest Gauss points

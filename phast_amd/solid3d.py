@@ -579,6 +579,16 @@ def pfczm_softening_parameters(softening: str):
     raise ValueError(f"PF-CZM softening must be one of ('linear', 'exponential'), got {softening!r}")
 
 
+def default_tiles(shape) -> tuple[int, int, int]:
+    """Launch shape of the voxel kernels when the caller names none: 16x14 node tiles in z
+    slabs of kz layers, with kz as large as 8 while the grid still has about 512 blocks (two
+    resident blocks on each of 256 CUs); small cells get thin slabs so that they fill the
+    device.  Any kz gives the same nodal rows, bit for bit."""
+    nx, ny, nz = (int(v) for v in shape)
+    tiles_xy = -(-nx // 16) * -(-ny // 14)
+    return 16, 14, max(1, min(8, tiles_xy * nz // 512))
+
+
 class PeriodicVoxelPhaseFieldOperator:
     """Matrix-free mechanics and damage operators of one triply periodic voxel cell.
 
@@ -589,7 +599,7 @@ class PeriodicVoxelPhaseFieldOperator:
     """
 
     def __init__(self, grid: VoxelGrid, materials: ElementMaterialArrays, load: LoadCase, *, kappa: float = 1e-6,
-                 ell: float, pinned_node: int | None = None, device=None, tiles=(16, 14, 8),
+                 ell: float, pinned_node: int | None = None, device=None, tiles=None,
                  damage_law: str = "at2", pfczm_softening: str = "linear", pfczm_p: float = 2,
                  crack_closure: str = "none"):
         self.device = _lib.require_device() if device is None else torch.device(device)
@@ -624,7 +634,7 @@ class PeriodicVoxelPhaseFieldOperator:
               "phast_hex8_plan_create")
         self._plan = h
         check(_L.phast_hex8_solver_reserve(h), "phast_hex8_solver_reserve")  # solves then allocate nothing
-        self.set_tiles(*tiles)
+        self.set_tiles(*(default_tiles(grid.shape) if tiles is None else tiles))
         self.Gc = torch.as_tensor(Gc, device=self.device)
         if damage_law not in ("at2", "pfczm"):
             raise ValueError(f"damage_law must be 'at2' or 'pfczm', got {damage_law!r}")
