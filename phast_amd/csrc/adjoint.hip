@@ -263,8 +263,7 @@ __global__ __launch_bounds__(256) void qp_stress_apply_kernel(const Hex8Geo G, c
 __global__ __launch_bounds__(256) void sum_parts6_kernel(const double *part, int nblocks, double *out6) {
     __shared__ double red[24];
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nblocks; b += 256)
-        for (int c = 0; c < 6; ++c) s[c] += part[8 * b + c];
+    sum_block_partials<256, 6>(part, nblocks, 6, s);
     block_sum_n<256, 6>(s, red);
     if (threadIdx.x == 0)
         for (int c = 0; c < 6; ++c) out6[c] = s[c];

@@ -97,6 +97,28 @@ __device__ __forceinline__ void block_sum_n(double v[K], double *sh) {
         v[k] = s;
     }
 }
+// Per-thread part of a fixed-order sum of block partials: s[k] += part[8 b + k] for
+// b = threadIdx.x, threadIdx.x + NT, ... < nblocks and k < nvals, in that order.  Four rows
+// of loads are in flight before their adds (lanes past the end load the last row and skip
+// its adds), so the sum waits for one memory latency per four rows instead of one per row.
+template <int NT, int K>
+__device__ __forceinline__ void sum_block_partials(const double *part, int nblocks, int nvals, double s[K]) {
+    for (int b0 = threadIdx.x; b0 < nblocks; b0 += 4 * NT) {
+        double v[4][K];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = min(b0 + u * NT, nblocks - 1);
+#pragma unroll
+            for (int k = 0; k < K; ++k) v[u][k] = part[8 * b + k];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (b0 + u * NT < nblocks)
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    if (k < nvals) s[k] += v[u][k];
+    }
+}
 // One block sum (block_sum's arithmetic, same bits) and two block minima sharing one pair of
 // barriers.  `sh` holds 3 * NT/64 doubles; results are valid in every thread.
 template <int NT>

@@ -410,10 +410,7 @@ __global__ __launch_bounds__(256) void hex8_finalize_kernel(const double *part, 
                                                             double *y6, double *out7) {
     __shared__ double red[7 * 4];
     double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nblocks; b += 256)
-#pragma unroll
-        for (int k = 0; k < 7; ++k)
-            if (k < nvals) s[k] += part[8 * b + k];
+    sum_block_partials<256, 7>(part, nblocks, nvals, s);
     block_sum_n<256, 7>(s, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < nvals; ++k) {
@@ -434,10 +431,7 @@ __global__ __launch_bounds__(256) void hex8_seam_finalize_kernel(const Hex8Geo G
         if (blockIdx.y != 0) return;
         __shared__ double red[7 * 4];
         double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int b = threadIdx.x; b < nblocks; b += 256)
-#pragma unroll
-            for (int k = 0; k < 7; ++k)
-                if (k < nvals) s[k] += part[8 * b + k];
+        sum_block_partials<256, 7>(part, nblocks, nvals, s);
         block_sum_n<256, 7>(s, red);
         if (threadIdx.x == 0)
             for (int k = 0; k < nvals; ++k) {
@@ -466,9 +460,7 @@ __global__ __launch_bounds__(256) void hex8_finalize_jacobi_kernel(const Hex8Geo
         if (blockIdx.y != 0) return;
         __shared__ double red[6 * 4];
         double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int bb = threadIdx.x; bb < nblocks; bb += 256)
-#pragma unroll
-            for (int k = 0; k < 6; ++k) s[k] += part[8 * bb + k];
+        sum_block_partials<256, 6>(part, nblocks, 6, s);
         block_sum_n<256, 6>(s, red);
         const int k = threadIdx.x;
         if (k < 6) {

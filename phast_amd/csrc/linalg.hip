@@ -51,11 +51,19 @@ __device__ __forceinline__ void write_parts5(double *part, const double v[5], do
 // every row's loads in flight together.
 template <int K>
 __device__ __forceinline__ void sum_parts_n(const double *part, int k0, double out[K], double *red) {
+    static_assert(kVecBlocks % 256 == 0, "whole rows of partials per thread");
+    constexpr int R = kVecBlocks / 256;
+    double v[R][K];  // every load issued before the first add
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[j][k] = part[(k0 + k) * kVecBlocks + threadIdx.x + 256 * j];
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = 0.0;
-    for (int i = threadIdx.x; i < kVecBlocks; i += 256)
 #pragma unroll
-        for (int k = 0; k < K; ++k) out[k] += part[(k0 + k) * kVecBlocks + i];
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] += v[j][k];
     block_sum_n<256, K>(out, red);
 }
 
