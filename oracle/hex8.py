@@ -468,12 +468,47 @@ def box_kkt(action, rhs, diag, lower, upper, initial=None, lin_tol=1e-10, kkt_to
     raise RuntimeError("H8 scalar box-KKT active set did not stabilise")
 
 
+def fracture_energy_at2(op: VoxelOperator, d):
+    """AT2 crack-surface energy (solid3d/operator.py:1511-1544): 1/2 sum_e Gc_e d_e.K d_e with
+    K = M/ell + ell L, i.e. half of d . (damage operator without the history term) d."""
+    return 0.5 * float(d @ op.damage_action(d))
+
+
+def anderson_mix(pairs, lower, upper=1.0, regularisation=1e-10):
+    """Projected type II Anderson mix of the stored pairs [(d_i, G(d_i)), ...]
+    (solid3d/staggered.py:336-374): residuals f_i = G_i - d_i; least squares
+    min ||f_k - dF gamma|| over unit-scaled columns of the residual differences with the
+    normal equations regularised by ``regularisation``; G_k - dG gamma clamped to the box.
+    None with fewer than two pairs or when no column differs."""
+    if len(pairs) < 2:
+        return None
+    G = np.stack([o for _, o in pairs], axis=1)
+    F = np.stack([o - i for i, o in pairs], axis=1)
+    dF, dG = np.diff(F, axis=1), np.diff(G, axis=1)
+    nrm = np.linalg.norm(dF, axis=0)
+    keep = nrm > 0.0
+    if not keep.any():
+        return None
+    dF, dG, nrm = dF[:, keep], dG[:, keep], nrm[keep]
+    A = dF / nrm
+    gamma = np.linalg.solve(A.T @ A + regularisation * np.eye(A.shape[1]), A.T @ F[:, -1]) / nrm
+    mixed = G[:, -1] - dG @ gamma
+    if not np.isfinite(mixed).all():
+        return None
+    return np.minimum(np.maximum(mixed, lower), upper)
+
+
 def staggered_increment(op: VoxelOperator, state: dict, target: float, *, stagger_tol=1e-4, min_stagger=2,
                         max_stagger=2000, rtol=1e-8, damage_linear_tol=1e-10, damage_kkt_tol=1e-8,
-                        check_every=10):
-    """One accepted increment of solid3d/staggered.py:_attempt (1259-1517), AT2 box-KKT route,
-    no Anderson mixing, final mechanics re-solve.  ``state`` holds x, d, H, E (macro strain).
-    Returns the new state and (stagger_iters, macro_stress)."""
+                        check_every=10, history="max", irreversibility="box_kkt", anderson_depth=0,
+                        anderson_start=2):
+    """One accepted increment of solid3d/staggered.py:_attempt (1259-1517), AT2 route, final
+    mechanics re-solve.  ``history`` 'max' (H = max(H_prev, Y)) or 'current' (H = Y);
+    ``irreversibility`` 'box_kkt' (d_prev <= d <= 1) or 'history_only' (one unconstrained
+    solve, staggered.py:804-839); ``anderson_depth`` > 0 mixes the damage fixed point from
+    iteration ``anderson_start`` on, each mixed input judged by the end energy and fixed-point
+    residual of its own iteration (staggered.py:1283-1401).  ``state`` holds x, d, H, E.
+    Returns the new state and (stagger_iters, macro_stress, anderson (accepted, rejected))."""
     c = op.controlled
     n = 3 * op.n_red
     previous = float(state["x"][n + c])
@@ -494,15 +529,23 @@ def staggered_increment(op: VoxelOperator, state: dict, target: float, *, stagge
         assert info[1]
         return x
 
+    pairs, fallback, best, accepted, rejected_n = [], None, math.inf, 0, 0
     for k in range(1, max_stagger + 1):
         x_new = mech(d_old, x_old)
-        H = np.maximum(state["H"], op.driving_force(x_new))
+        Y = op.driving_force(x_new)
+        H = np.maximum(state["H"], Y) if history == "max" else Y
         s = op.damage_scale
         act = lambda v: s * op.damage_action(v, H)
         rhs, dg = s * op.damage_rhs(H), s * op.damage_diagonal(H)
         r0 = float(np.linalg.norm(rhs - act(d_old)))
         tol = max(min(damage_linear_tol, 0.1 * damage_kkt_tol / max(r0, 1.0)), 4.0 * np.finfo(np.float64).eps)
-        d_new = box_kkt(act, rhs, dg, lower, np.ones(op.n_red), initial=d_old, lin_tol=tol, kkt_tol=damage_kkt_tol)[0]
+        if irreversibility == "box_kkt":
+            d_new = box_kkt(act, rhs, dg, lower, np.ones(op.n_red), initial=d_old, lin_tol=tol,
+                            kkt_tol=damage_kkt_tol)[0]
+        else:
+            d_new, _, conv = dirichlet_pcg(act, rhs, dg, np.zeros(op.n_red, dtype=bool), np.zeros(op.n_red),
+                                           d_old, tol, 4000)
+            assert conv
         du = np.linalg.norm(x_new[:n] - x_old[:n])
         u = np.linalg.norm(x_new[:n])
         dE = np.abs(x_new[n:] - x_old[n:]).max()
@@ -511,10 +554,31 @@ def staggered_increment(op: VoxelOperator, state: dict, target: float, *, stagge
         scale_E = max(E_inf, step)
         u_ref = max(u, scale_E * u_floor)
         change = max(du / u_ref if u_ref > 0 else 0.0, dE / scale_E if scale_E > 0 else 0.0, dd)
-        x_old, d_old = x_new, d_new
+        if anderson_depth > 0:
+            residual = float(np.linalg.norm(d_new - d_old))
+            energy = op.elastic_energy(x_new, op.degradation(d_new)) + fracture_energy_at2(op, d_new)
+            bad = False
+            if fallback is not None:
+                bad = bool(energy > best + 1e-12 * abs(best) or residual > fallback[2])
+                accepted, rejected_n = accepted + int(not bad), rejected_n + int(bad)
+            if not bad:
+                best = min(best, energy)
+            if bad:  # continue from the plain iterate the mix had replaced
+                d_old, x_old, fallback, pairs = fallback[0], fallback[1], None, []
+                continue
+            fallback = None
         if change <= stagger_tol and k >= min_stagger:
+            x_old, d_old = x_new, d_new
             break
+        if anderson_depth > 0:
+            pairs = (pairs + [(d_old.copy(), d_new.copy())])[-(anderson_depth + 1):]
+            mixed = anderson_mix(pairs, lower) if k >= anderson_start else None
+            if mixed is not None and not np.array_equal(mixed, d_new):
+                fallback = (d_new, x_new, residual)
+                x_old, d_old = x_new, mixed
+                continue
+        x_old, d_old = x_new, d_new
     x = mech(d_old, x_old)
     g = op.degradation(d_old)
     sigma = op.macro_stress(x, g)
-    return {"x": x, "d": d_old, "H": H}, (k, sigma)
+    return {"x": x, "d": d_old, "H": H}, (k, sigma, (accepted, rejected_n))

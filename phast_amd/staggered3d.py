@@ -2,8 +2,8 @@
 
 Mirrors ``solid3d/staggered.py`` (StaggeredSolver: ``_attempt`` 1259-1517,
 ``_damage`` 753-802, ``_change`` 1097-1121, ``_predictor`` 1207-1213, ``advance`` 1215-1257,
-``advance_increment`` 1523-1582) without interface stress or Anderson mixing, for the AT2
-law and for the PF-CZM law (``_damage_pfczm`` 843-1096, the projected Newton solve), with or
+``advance_increment`` 1523-1582, projected Anderson mixing 290-374) without interface
+stress, for the AT2 law and for the PF-CZM law (``_damage_pfczm`` 843-1096, the projected Newton solve), with or
 without the crack closure (``_closure_mechanics`` 698-751 with ``_energy_line_search``
 589-626).  The linear mechanics solves are the device-resident constrained PCG (for AT2
 without closure with the degradation formed in-kernel from the nodal damage); the damage
@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+from collections import deque
 from dataclasses import dataclass, field
 from time import perf_counter
 
@@ -20,7 +21,7 @@ import numpy as np
 import torch
 
 from .solid3d import (F64, H8ScalarBoxKKTConvergenceError, LoadCase, PCGBreakdownError,
-                      PeriodicVoxelPhaseFieldOperator, solve_h8_scalar_box_kkt)
+                      PeriodicVoxelPhaseFieldOperator, solve_h8_scalar_box_kkt, solve_h8_scalar_dirichlet_pcg)
 
 _MIN_LINEAR_TOLERANCE = 4.0 * float(np.finfo(np.float64).eps)  # staggered.py:97
 _TINY = float(np.finfo(np.float64).tiny)                        # staggered.py:96
@@ -34,6 +35,8 @@ PFCZM_ENERGY_RTOL = 1.0e-12         # staggered.py:121
 PFCZM_MAX_HALVINGS = 30             # staggered.py:124
 PFCZM_MAX_DOUBLINGS = 6             # staggered.py:127
 PFCZM_INNER_KKT_FRACTION = 0.1      # staggered.py:130
+ANDERSON_REGULARISATION = 1.0e-10   # staggered.py:133
+ANDERSON_ENERGY_RTOL = 1.0e-12      # staggered.py:136
 
 
 @dataclass(frozen=True)
@@ -55,6 +58,25 @@ class SolverConfig:
     damaged_threshold: float = 0.95
     preconditioner: str = "jacobi"
     pfczm_max_newton: int = 100
+    irreversibility: str = "box_kkt"  # or "history_only": unconstrained damage solve, bound violations counted
+    history: str = "max"              # H = max(H_prev_step, Y), or "current": H = Y
+    anderson_depth: int = 0           # projected Anderson mixing of the damage fixed point when > 0
+    anderson_start: int = 2           # plain staggered iterations before the first mix
+
+    def __post_init__(self):
+        """The checks of model.py:830-857 for the settings read here."""
+        for key, choices in (("irreversibility", ("box_kkt", "history_only")), ("history", ("max", "current")),
+                             ("on_stagger_failure", ("cutback", "raise"))):
+            if getattr(self, key) not in choices:
+                raise ValueError(f"{key} must be one of {choices}, got {getattr(self, key)!r}")
+        depth, start = self.anderson_depth, self.anderson_start
+        if isinstance(depth, bool) or int(depth) != depth or int(depth) < 0:
+            raise ValueError(f"anderson_depth must be an integer >= 0, got {depth!r}")
+        if isinstance(start, bool) or int(start) != start or int(start) < 2:
+            raise ValueError(f"anderson_start must be an integer >= 2 (a mix needs two staggered residuals), "
+                             f"got {start!r}")
+        object.__setattr__(self, "anderson_depth", int(depth))
+        object.__setattr__(self, "anderson_start", int(start))
 
 
 @dataclass(eq=False)
@@ -106,6 +128,9 @@ class IncrementRecord:
     history_s: float = 0.0
     newton_iters_pf: int | None = None   # PF-CZM: projected Newton iterations of the increment
     closure_solves: int | None = None    # crack closure: linear mechanics solves of the increment
+    kkt_ok: bool = True                  # the last damage solve met its bounds (history_only: no violation)
+    anderson_accepted: int | None = None  # Anderson mixing: mixed iterates kept / discarded
+    anderson_rejected: int | None = None
 
 
 class StaggerConvergenceError(RuntimeError):
@@ -123,8 +148,70 @@ class _AttemptFailure(RuntimeError):
         self.pcg_mech, self.pcg_pf, self.active_set = int(pcg_mech), int(pcg_pf), int(active_set)
 
 
+class ProjectedAnderson:
+    """Type II Anderson mixing of the damage fixed point d = G(d), projected onto the box of
+    the damage solve (staggered.py:290-374; Walker and Ni 2011).
+
+    The last depth + 1 pairs (d_i, G(d_i)) are kept.  ``mix`` takes the residuals
+    f_i = G(d_i) - d_i, the differences dF (of residuals) and dG (of outputs) of neighbouring
+    pairs, scales each column of dF to unit length (columns of zero length are dropped),
+    solves (A^T A + rho I) beta = A^T f_k for the scaled A with rho = ANDERSON_REGULARISATION,
+    and returns clamp(G(d_k) - dG gamma, lower, upper) with gamma = beta / scale.  None when
+    fewer than two pairs are stored, when no column remains, or when the solve fails or gives
+    a non-finite iterate."""
+
+    def __init__(self, depth: int, *, regularisation: float = ANDERSON_REGULARISATION):
+        if isinstance(depth, bool) or int(depth) != depth or int(depth) < 1:
+            raise ValueError(f"depth must be an integer >= 1, got {depth!r}")
+        if not (math.isfinite(regularisation) and regularisation >= 0.0):
+            raise ValueError(f"regularisation must be finite and >= 0, got {regularisation!r}")
+        self.depth, self.regularisation = int(depth), float(regularisation)
+        self._pairs: deque = deque(maxlen=self.depth + 1)
+
+    def __len__(self):
+        return len(self._pairs)
+
+    def reset(self):
+        self._pairs.clear()
+
+    def push(self, d_in: torch.Tensor, d_out: torch.Tensor):
+        if d_in.shape != d_out.shape or d_in.dim() != 1:
+            raise ValueError("d_in and d_out must be vectors of the same length")
+        self._pairs.append((d_in.detach().clone(), d_out.detach().clone()))
+
+    def mix(self, lower, upper=1.0):
+        if len(self._pairs) < 2:
+            return None
+        outs = [o for _, o in self._pairs]
+        res = [o - i for i, o in self._pairs]
+        dF = torch.stack([b - a for a, b in zip(res[:-1], res[1:])], dim=1)
+        dG = torch.stack([b - a for a, b in zip(outs[:-1], outs[1:])], dim=1)
+        scale = torch.linalg.vector_norm(dF, dim=0)
+        keep = scale > 0.0
+        if not bool(keep.any()):
+            return None
+        dF, dG, scale = dF[:, keep], dG[:, keep], scale[keep]
+        A = dF / scale
+        normal = A.T @ A
+        normal.diagonal().add_(self.regularisation)
+        try:
+            beta = torch.linalg.solve(normal, A.T @ res[-1])
+        except RuntimeError:
+            return None
+        mixed = outs[-1] - dG @ (beta / scale)
+        if not bool(torch.isfinite(mixed).all()):
+            return None
+        mixed = torch.maximum(mixed, lower) if isinstance(lower, torch.Tensor) else mixed.clamp(min=float(lower))
+        mixed = torch.minimum(mixed, upper) if isinstance(upper, torch.Tensor) else mixed.clamp(max=float(upper))
+        return mixed
+
+
 class StaggeredSolver:
-    """AT2 staggered solver of one voxel cell under mixed macro control."""
+    """AT2 / PF-CZM staggered solver of one voxel cell under mixed macro control.
+
+    ``iterate_observer`` (a callable, or None) receives one dict per staggered iterate of
+    the attempts whose step is in ``observe_steps`` (staggered.py:1180-1195): step, attempt,
+    increment, cutbacks, target, iteration, x, g, d, change, input_mixed, discarded."""
 
     def __init__(self, op: PeriodicVoxelPhaseFieldOperator, load: LoadCase | None = None,
                  cfg: SolverConfig | None = None, preconditioner=None):
@@ -141,9 +228,14 @@ class StaggeredSolver:
         axis = self._controlled if self._controlled < 3 else 0
         others = tuple(lengths[a] for a in range(3) if a != axis)
         self._load_axis, self._load_length, self._face_lengths = axis, lengths[axis], others
+        if op.has_closure and self.cfg.anderson_depth > 0:  # staggered.py:445-446
+            raise ValueError("Anderson mixing (anderson_depth > 0) is not available with crack_closure 'axes'")
         self.preconditioner = preconditioner  # optional callable building an MG preconditioner
         self.last_state: SolverState | None = None
         self.n_attempts = 0
+        self.iterate_observer = None
+        self.observe_steps: frozenset = frozenset()
+        self.last_bound_violations: dict | None = None  # history_only: the last solve's violations
 
     def initial_state(self) -> SolverState:
         op = self.op
@@ -261,6 +353,8 @@ class StaggeredSolver:
             raise _AttemptFailure("damage residual is not finite")
         tol = min(cfg.damage_linear_tol, 0.1 * cfg.damage_kkt_tol / max(r0, 1.0))
         tol = max(tol, _MIN_LINEAR_TOLERANCE)
+        if cfg.irreversibility == "history_only":
+            return self._damage_unconstrained(system, lower, initial, tol)
         try:
             res = solve_h8_scalar_box_kkt(system, lower_bound=lower, upper_bound=1.0, initial=initial,
                                           linear_tolerance=tol, kkt_tolerance=cfg.damage_kkt_tol,
@@ -272,6 +366,29 @@ class StaggeredSolver:
             raise _AttemptFailure(f"damage PCG breakdown: {exc}") from exc
         self._last_active = (res.lower_active, res.upper_active)
         return res.solution, res.total_pcg_iterations, res.active_set_iterations, res.kkt.satisfies_kkt
+
+    def _damage_unconstrained(self, system, lower, initial, tol):
+        """irreversibility='history_only' (staggered.py:804-839): one PCG solve with no bound;
+        the violations of d_prev <= d <= 1 are counted, not enforced."""
+        fixed = torch.zeros(self.op.n_red, dtype=torch.bool, device=self.device)
+        try:
+            solve = solve_h8_scalar_dirichlet_pcg(system, fixed=fixed, values=torch.zeros_like(initial),
+                                                  initial=initial, tolerance=tol,
+                                                  max_iterations=self.cfg.damage_maxit)
+        except (RuntimeError, ValueError) as exc:
+            raise _AttemptFailure(f"damage PCG breakdown: {exc}") from exc
+        if not solve.krylov.converged:
+            raise _AttemptFailure(f"damage PCG did not converge in {self.cfg.damage_maxit} iterations",
+                                  pcg_pf=solve.krylov.iterations)
+        d = solve.solution
+        stats = torch.stack(((d < lower).sum().to(F64), (d < 0.0).sum().to(F64), (d > 1.0).sum().to(F64),
+                             (lower - d).clamp_min(0.0).max(), (d - 1.0).clamp_min(0.0).max())).cpu().tolist()
+        self.last_bound_violations = {"below_previous": int(stats[0]), "below_zero": int(stats[1]),
+                                      "above_one": int(stats[2]), "max_decrease": stats[3],
+                                      "max_excess_over_one": stats[4]}
+        self._last_active = None
+        ok = int(stats[0]) == 0 and int(stats[1]) == 0 and int(stats[2]) == 0
+        return d, int(solve.krylov.iterations), 0, ok
 
     def _damage_pfczm(self, H, lower, initial):
         """staggered.py:843-1096 (absolute-field QP): projected Newton on the PF-CZM damage
@@ -385,6 +502,18 @@ class StaggeredSolver:
         return self.op.field_snapshot(state.step, float(state.macro_strain[self._controlled]), state.x, state.d,
                                       state.H)
 
+    def _observe(self, state, **row):
+        """Hand one staggered iterate to ``iterate_observer`` when its step is observed."""
+        step = int(state.step) + 1
+        if self.iterate_observer is None or step not in self.observe_steps:
+            return
+        self.iterate_observer({"step": step, "attempt": int(self.n_attempts), **row})
+
+    def _total_energy(self, x, d, g=None):
+        """(E_el(x, g(d)), Gamma(d)) (staggered.py:1201-1205)."""
+        g = self.op.degradation(d) if g is None else g
+        return float(self.op._stress_sums(x, g)[6]), self.fracture_energy(d)
+
     def fracture_energy(self, d) -> float:
         """Gamma(d) (operator.py:1511-1544, total): 1/2 sum_e Gc d_e.(M/ell + ell L) d_e for AT2,
         the PF-CZM surface functional for PF-CZM (operator.py:611-631)."""
@@ -403,29 +532,85 @@ class StaggeredSolver:
         converged, change, H, info = False, math.inf, state.H, None
         closure_solves = 0
         self._newton_count = 0
+        kkt_ok = True
+        # Anderson mixing: a fresh history for every attempt (staggered.py:1283-1289)
+        anderson = ProjectedAnderson(cfg.anderson_depth) if cfg.anderson_depth > 0 else None
+        fallback = None  # (d, x, residual) of the plain iterate a pending mixed input replaced
+        best_energy = math.inf
+        n_accepted = n_rejected = 0
+        where = dict(increment=increment, cutbacks=cutbacks, target=target)
         k = 0
         for k in range(1, cfg.max_stagger + 1):
             t0 = perf_counter()
-            x_new, info, g = self._mechanics(d_old, values, x_old)
-            pcg_mech += info.iterations
-            if op.has_closure:
-                closure_solves += self.last_interface_solves
-            t1 = perf_counter()
-            Y = op.driving_force(x_new)
-            H = torch.maximum(state.H, Y)
-            t2 = perf_counter()
-            d_new, iters, sets, kkt_ok = self._damage(H, lower, d_old)
-            pcg_pf += iters
-            active_set += sets
-            t3 = perf_counter()
-            change = self._change(x_new, x_old, d_new, d_old, step_size)
+            try:
+                x_new, info, g = self._mechanics(d_old, values, x_old)
+                pcg_mech += info.iterations
+                if op.has_closure:
+                    closure_solves += self.last_interface_solves
+                t1 = perf_counter()
+                Y = op.driving_force(x_new)
+                H = torch.maximum(state.H, Y) if cfg.history == "max" else Y
+                t2 = perf_counter()
+                d_new, iters, sets, kkt_ok = self._damage(H, lower, d_old)
+                pcg_pf += iters
+                active_set += sets
+                t3 = perf_counter()
+                change = self._change(x_new, x_old, d_new, d_old, step_size)
+            except _AttemptFailure as exc:
+                if fallback is None:
+                    raise
+                # the iteration from a mixed input failed: back to the plain iterate; the
+                # iterations its solves had spent count (staggered.py:1311-1341)
+                n_rejected += 1
+                pcg_mech += exc.pcg_mech
+                pcg_pf += exc.pcg_pf
+                active_set += exc.active_set
+                mech_s += perf_counter() - t0
+                self._observe(state, **where, iteration=k, x=None, g=None, d=d_old, change=float("nan"),
+                              input_mixed=True, discarded=True)
+                d_old, x_old, _ = fallback
+                fallback = None
+                anderson.reset()
+                continue
             mech_s += t1 - t0
             history_s += t2 - t1
             damage_s += t3 - t2
+            rejected = False
+            if anderson is not None:
+                residual = float(torch.linalg.vector_norm(d_new - d_old))
+                elastic_k, fracture_k = self._total_energy(x_new, d_new)
+                energy = elastic_k + fracture_k
+                if fallback is not None:
+                    # the mixed input is judged by its own iteration: its end energy must not
+                    # exceed the lowest of the attempt, nor its residual that of the iteration
+                    # that produced the replaced plain iterate (staggered.py:1354-1378)
+                    rejected = (energy > best_energy + ANDERSON_ENERGY_RTOL * abs(best_energy)
+                                or residual > fallback[2])
+                    if rejected:
+                        n_rejected += 1
+                    else:
+                        n_accepted += 1
+                if not rejected:
+                    best_energy = min(best_energy, energy)
+            self._observe(state, **where, iteration=k, x=x_new, g=g, d=d_new, change=float(change),
+                          input_mixed=fallback is not None, discarded=bool(rejected))
+            if rejected:
+                d_old, x_old, _ = fallback
+                fallback = None
+                anderson.reset()
+                continue
+            fallback = None
             if change <= cfg.stagger_tol and k >= cfg.min_stagger:
                 x_old, d_old = x_new, d_new
                 converged = True
                 break
+            if anderson is not None:
+                anderson.push(d_old, d_new)
+                mixed = anderson.mix(lower) if k >= cfg.anderson_start else None
+                if mixed is not None and not torch.equal(mixed, d_new):
+                    fallback = (d_new, x_new, residual)
+                    x_old, d_old = x_new, mixed
+                    continue
             x_old, d_old = x_new, d_new
         if not converged:
             raise _AttemptFailure(f"staggered iteration did not converge in {cfg.max_stagger} iterations "
@@ -474,7 +659,9 @@ class StaggeredSolver:
                               hill_mandel_rel=hill_rel, wall_s=perf_counter() - start, mech_s=mech_s,
                               damage_s=damage_s, history_s=history_s,
                               newton_iters_pf=self._newton_count if op.is_pfczm else None,
-                              closure_solves=closure_solves if op.has_closure else None)
+                              closure_solves=closure_solves if op.has_closure else None, kkt_ok=bool(kkt_ok),
+                              anderson_accepted=n_accepted if anderson is not None else None,
+                              anderson_rejected=n_rejected if anderson is not None else None)
         return new_state, rec
 
     def advance(self, state, target, *, increment=None, cutbacks=0):

@@ -233,7 +233,7 @@ def test_staggered_increments_track_oracle(dev):
     ost = {"x": np.zeros(op.n_mech), "d": np.zeros(op.n_red), "H": np.zeros((op.n_elements, 8))}
     for target in (4e-3, 8e-3, 1.2e-2):
         st, rec = solver.advance(st, target)
-        ost, (k, sigma) = O.staggered_increment(oop, ost, target, rtol=1e-12, damage_kkt_tol=1e-9)
+        ost, (k, sigma, _) = O.staggered_increment(oop, ost, target, rtol=1e-12, damage_kkt_tol=1e-9)
         assert rec.stagger_iters == k
         assert rel(st.d, ost["d"]) < RTOL and rel(st.x, ost["x"]) < RTOL
         assert rel(st.macro_stress, sigma) < RTOL and rel(st.H, ost["H"]) < RTOL
@@ -242,6 +242,40 @@ def test_staggered_increments_track_oracle(dev):
     assert snap.time_value == 1.2e-2 and snap.step == st.step
     np.testing.assert_allclose(snap.macro_stress, st.macro_stress.cpu().numpy(), rtol=1e-12)
     assert rel(snap.history_max, st.H.amax(dim=1)) == 0.0
+
+
+@pytest.mark.parametrize("opts", [dict(history="current", irreversibility="history_only"),
+                                  dict(history="current"), dict(anderson_depth=3)])
+def test_staggered_options_track_oracle(dev, opts):
+    """The stagger-loop options of solid3d/model.py:SolverConfig on the small damaged cell:
+    history 'current', irreversibility 'history_only' and Anderson mixing (depth 3), loaded
+    and then unloaded, against the oracle's increment with the same options; the fracture
+    energy the Anderson safeguard uses against the oracle's."""
+    from phast_amd import staggered3d as ST
+    S, op, oop = build((8, 8, 8), h=(1e-3,) * 3, seed=11, pores=False)
+    cfg = ST.SolverConfig(pcg_rtol_mech=1e-12, damage_kkt_tol=1e-9, **opts)
+    solver = ST.StaggeredSolver(op, cfg=cfg)
+    seen = []
+    solver.iterate_observer, solver.observe_steps = seen.append, frozenset({2})
+    st = solver.initial_state()
+    ost = {"x": np.zeros(op.n_mech), "d": np.zeros(op.n_red), "H": np.zeros((op.n_elements, 8))}
+    for target in (4e-3, 8e-3, 5e-3):
+        st, rec = solver.advance(st, target)
+        ost, (k, sigma, acc) = O.staggered_increment(oop, ost, target, rtol=1e-12, damage_kkt_tol=1e-9, **opts)
+        print(target, rec.stagger_iters, k, rec.anderson_accepted, rec.anderson_rejected, acc)
+        assert rec.stagger_iters == k
+        if "anderson_depth" in opts:
+            assert (rec.anderson_accepted, rec.anderson_rejected) == acc
+        assert rel(st.d, ost["d"]) < RTOL and rel(st.x, ost["x"]) < RTOL
+        assert rel(st.macro_stress, sigma) < RTOL and rel(st.H, ost["H"]) < RTOL
+    assert [r["iteration"] for r in seen] == list(range(1, len(seen) + 1)) and len(seen) >= 2
+    assert all(r["step"] == 2 and r["x"] is not None for r in seen)
+    assert float(st.d.max()) > 0.05
+    d = st.d
+    assert abs(solver.fracture_energy(d) - O.fracture_energy_at2(oop, d.cpu().numpy())) \
+        <= 1e-12 * solver.fracture_energy(d)
+    if opts.get("irreversibility") == "history_only":
+        assert solver.last_bound_violations is not None
 
 
 @pytest.mark.parametrize("shape", [(16, 16, 16), (24, 16, 32)])

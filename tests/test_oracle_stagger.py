@@ -217,7 +217,7 @@ def test_staggered_increment_homogeneous_cell_fixed_point():
     st = _state(op)
     H_max = 0.0
     for t in (2e-4, 4e-4, 6e-4, 3e-4):
-        st, (k, sigma) = OH.staggered_increment(op, st, t, rtol=1e-13)
+        st, (k, sigma, _) = OH.staggered_increment(op, st, t, rtol=1e-13)
         Y = 0.5 * LAM(E, nu) * ((1 - 2 * nu) * t) ** 2 + MU(E, nu) * t ** 2
         H_max = max(H_max, Y)
         c = 2 * (1 - kap)
@@ -253,7 +253,7 @@ def test_staggered_increment_elastic_laminate():
     s = t + Eyy
     ezz = -lam * s / (lam + 2 * mu)
     sxx = f @ (a * s + 2 * mu * t)
-    st, (k, sigma) = OH.staggered_increment(op, _state(op), t, rtol=1e-13)
+    st, (k, sigma, _) = OH.staggered_increment(op, _state(op), t, rtol=1e-13)
     n = 3 * op.n_red
     assert k == 2
     assert st["d"].max() < 1e-12
@@ -265,3 +265,126 @@ def test_staggered_increment_elastic_laminate():
     u = st["x"][:n].reshape(nz, ny, nx, 3)
     assert np.abs(u[..., :2]).max() < 1e-12 * t
     assert np.ptp(u[..., 2], axis=(1, 2)).max() < 1e-9 * np.abs(u[..., 2]).max()
+
+
+# ----------------------------------------------------------------------------------------
+# stagger-loop options: history 'current', irreversibility 'history_only', Anderson mixing
+# ----------------------------------------------------------------------------------------
+
+def _uniform_d(H, Gc, ell, kap=1e-6):
+    c = 2 * (1 - kap)
+    return c * H / (Gc / ell + c * H)
+
+
+@pytest.mark.parametrize("history,irreversibility", [("max", "box_kkt"), ("max", "history_only"),
+                                                     ("current", "box_kkt"), ("current", "history_only")])
+def test_staggered_increment_history_and_irreversibility_options(history, irreversibility):
+    """Homogeneous cell, loaded to 6e-4 and unloaded to 3e-4.  The damage of the uniform
+    problem is d(H) = c H / (Gc/l + c H).  On unloading H is the loading maximum
+    (history 'max') or the current Y ('current'); the box solve then keeps d >= d_prev,
+    'history_only' does not: only history 'current' with 'history_only' lets the damage
+    heal, to d(Y(3e-4))."""
+    E, nu, Gc, ell = 3000.0, 0.35, 5e-4, 2.0
+    op = _cell((4, 3, 5), np.zeros(60, dtype=np.int64), [(E, nu, Gc)], h=(1.0, 1.1, 0.9), ell=ell)
+    Y = lambda t: 0.5 * LAM(E, nu) * ((1 - 2 * nu) * t) ** 2 + MU(E, nu) * t ** 2
+    st = _state(op)
+    for t in (2e-4, 6e-4):
+        st, _ = OH.staggered_increment(op, st, t, rtol=1e-13, history=history, irreversibility=irreversibility)
+        np.testing.assert_allclose(st["d"], _uniform_d(Y(t), Gc, ell), rtol=1e-10)
+    d_load = _uniform_d(Y(6e-4), Gc, ell)
+    st, (k, sigma, _) = OH.staggered_increment(op, st, 3e-4, rtol=1e-13, history=history,
+                                               irreversibility=irreversibility)
+    H_expect = Y(3e-4) if history == "current" else Y(6e-4)
+    np.testing.assert_allclose(st["H"], H_expect, rtol=1e-10)
+    d_expect = _uniform_d(Y(3e-4), Gc, ell) if (history, irreversibility) == ("current", "history_only") \
+        else d_load
+    np.testing.assert_allclose(st["d"], d_expect, rtol=1e-10)
+    g = (1 - 1e-6) * (1 - d_expect) ** 2 + 1e-6
+    assert math.isclose(sigma[0], g * E * 3e-4, rel_tol=1e-10)
+
+
+@pytest.mark.parametrize("reg", [0.0, 1e-10])
+def test_anderson_mix_solves_an_affine_fixed_point(reg):
+    """For d -> M d + c (a contraction), n + 1 plain iterates give n residual differences
+    that span R^n, so the type II mix is the fixed point (I - M)^-1 c itself (exactly
+    without regularisation); far inside the box the projection does nothing."""
+    rng = np.random.default_rng(3)
+    n = 5
+    M = rng.uniform(-1, 1, (n, n))
+    M *= 0.8 / np.abs(np.linalg.eigvals(M)).max()
+    c = rng.uniform(-1, 1, n)
+    fixed = np.linalg.solve(np.eye(n) - M, c)
+    d, pairs = rng.uniform(-1, 1, n), []
+    for _ in range(n + 1):
+        out = M @ d + c
+        pairs.append((d.copy(), out))
+        d = out
+    mixed = OH.anderson_mix(pairs, -1e9, 1e9, regularisation=reg)
+    np.testing.assert_allclose(mixed, fixed, rtol=1e-9 if reg else 1e-11, atol=1e-9 if reg else 1e-11)
+    # fewer pairs than the dimension: a least-squares mix, not the fixed point, but projected
+    lo = fixed - 1e-3
+    part = OH.anderson_mix(pairs[:3], lo, 1e9, regularisation=reg)
+    assert part is not None and np.all(part >= lo)
+    assert OH.anderson_mix(pairs[:1], -1e9) is None
+    same = [(pairs[0][0], pairs[0][1]), (pairs[0][0], pairs[0][1])]
+    assert OH.anderson_mix(same, -1e9) is None  # every residual difference vanishes
+
+
+def test_anderson_mix_matches_the_product_host_class():
+    """The host class ProjectedAnderson (torch) against the oracle's mix on random pairs,
+    with box clamping active on both sides; depth 3 keeps the last four pairs."""
+    import torch
+    from phast_amd.staggered3d import ProjectedAnderson
+    rng = np.random.default_rng(7)
+    n = 40
+    lower = rng.uniform(0.0, 0.3, n)
+    acc = ProjectedAnderson(3)
+    pairs = []
+    for i in range(6):
+        a, b = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+        pairs.append((a, b))
+        acc.push(torch.tensor(a), torch.tensor(b))
+        got = acc.mix(torch.tensor(lower))
+        want = OH.anderson_mix(pairs[-4:], lower)
+        if want is None:
+            assert got is None
+        else:
+            np.testing.assert_allclose(got.numpy(), want, rtol=1e-9, atol=1e-12)
+            assert (got.numpy() >= lower).all() and (got.numpy() <= 1.0).all()
+    acc.reset()
+    assert len(acc) == 0 and acc.mix(torch.tensor(lower)) is None
+
+
+def test_staggered_increment_anderson_reaches_the_plain_fixed_point():
+    """A two-phase cell past the damage peak, where the plain staggered iteration needs many
+    iterations: with Anderson mixing (depth 3) the increment converges to the same fixed
+    point within the stagger tolerance, and every mixed input is either kept or discarded."""
+    rng = np.random.default_rng(5)
+    shape = (5, 4, 4)
+    index = (rng.uniform(0, 1, 80) < 0.3).astype(np.int64)
+    op = _cell(shape, index, [(3000.0, 0.35, 5e-4), (70000.0, 0.3, 5e-3)], ell=2.0)
+    plain, mixed = _state(op), _state(op)
+    totals = [0, 0]
+    for t in (4e-3, 8e-3, 1.2e-2):
+        plain, (kp, sp, ap) = OH.staggered_increment(op, plain, t, rtol=1e-13, stagger_tol=1e-8)
+        mixed, (km, sm, am) = OH.staggered_increment(op, mixed, t, rtol=1e-13, stagger_tol=1e-8, anderson_depth=3)
+        assert ap == (0, 0)
+        totals[0] += kp
+        totals[1] += km
+        np.testing.assert_allclose(mixed["d"], plain["d"], atol=1e-6)
+        np.testing.assert_allclose(sm, sp, rtol=1e-5, atol=1e-6 * abs(sp[0]))
+    assert plain["d"].max() > 0.3
+    assert totals[0] > 12, totals  # the plain iteration is slow here
+
+
+def test_fracture_energy_at2_uniform_and_quadratic():
+    """Gamma(c 1) = Gc c^2 V / (2 ell) (the stiffness part vanishes on constants, the mass
+    sums to the volume); Gamma is a quadratic form: Gamma(a u) = a^2 Gamma(u)."""
+    Gc, ell = 5e-4, 2.0
+    op = _cell((4, 3, 5), np.zeros(60, dtype=np.int64), [(3000.0, 0.35, Gc)], h=(1.0, 1.1, 0.9), ell=ell)
+    V = 4 * 3 * 5 * 1.0 * 1.1 * 0.9
+    for c in (0.3, 1.0):
+        assert math.isclose(OH.fracture_energy_at2(op, np.full(op.n_red, c)), Gc * c * c * V / (2 * ell),
+                            rel_tol=1e-12)
+    u = np.random.default_rng(1).uniform(0, 1, op.n_red)
+    assert math.isclose(OH.fracture_energy_at2(op, 0.5 * u), 0.25 * OH.fracture_energy_at2(op, u), rel_tol=1e-12)
