@@ -134,8 +134,7 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
                     g[q] = G.omk * (omd * omd) + G.kappa;  // splits.py:56-59
                 }
             } else if (GMODE == 1) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) g[q] = A.g[e * 8 + q];
+                load_qp8(A.g, e, g);
             } else {
 #pragma unroll
                 for (int q = 0; q < 8; ++q) g[q] = 1.0;
@@ -351,8 +350,7 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
                     g[q] = G.omk * (omd * omd) + G.kappa;  // splits.py:56-59
                 }
             } else if (GMODE == 1) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) g[q] = A.g[e * 8 + q];
+                load_qp8(A.g, e, g);
             } else {
 #pragma unroll
                 for (int q = 0; q < 8; ++q) g[q] = 1.0;
@@ -691,6 +689,19 @@ struct DmgArgs3 {
     int32_t kz, tiles_x, tiles_y;
 };
 
+// The voxel matrix K and the Gauss-point shape values Nq (64 doubles each) staged in LDS at
+// block start for the voxel-parallel damage apply: read from the kernel argument inside the
+// unrolled 8x8 products, they need 256 scalar registers, and the compiler spills those to
+// vector lanes (v_readlane / v_writelane on every use).  Same values in the same order, so
+// the same bits as the marching kernel, which keeps the kernel-argument reads (staged there,
+// the compiler holds both tables in vector registers).
+__device__ __forceinline__ void stage_voxel_tables(const Hex8Geo &G, double *sK, double *sN) {
+    for (int i = threadIdx.x; i < 128; i += blockDim.x) {
+        if (i < 64) sK[i] = G.K[i];
+        else sN[i - 64] = G.Nq[i - 64];
+    }
+}
+
 template <int TX, int TY, bool HIST>
 __global__ __launch_bounds__(TX *TY) void hex8_damage_kernel(const Hex8Geo G, DmgArgs3 A) {
     constexpr int NT = TX * TY;
@@ -1003,6 +1014,8 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_damage_vp_kern
     constexpr int NS = (NN + NT - 1) / NT;
     __shared__ double su[2][NS * NT];  // node layers of d; slots past NN are padding
     __shared__ double st[6][NV];       // corners 1, 2, 3, 5, 6, 7 of each voxel
+    __shared__ double sK[64], sN[64];
+    stage_voxel_tables(G, sK, sN);
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const int tiles = A.tiles_x * A.tiles_y;
     const int bxy = blockIdx.x % tiles, bz = blockIdx.x / tiles;
@@ -1034,9 +1047,7 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_damage_vp_kern
     auto fetch_vox = [&](int k, double &gc, double (&h)[8]) {
         const int64_t e = e_ij + nxy * wrap(k, nz);
         gc = A.Gc[e];
-        if (HIST)
-#pragma unroll
-            for (int q = 0; q < 8; ++q) h[q] = A.H[e * 8 + q];
+        if (HIST) load_qp8(A.H, e, h);
     };
     double gc0 = 0.0, h0[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     fetch_vox(k0 - 1, gc0, h0);
@@ -1068,7 +1079,7 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_damage_vp_kern
             for (int a = 0; a < 8; ++a) {
                 double tt = 0.0;
 #pragma unroll
-                for (int b = 0; b < 8; ++b) tt += dc[b] * G.K[b * 8 + a];
+                for (int b = 0; b < 8; ++b) tt += dc[b] * sK[b * 8 + a];
                 y[a] = gc0 * tt;
             }
             if (HIST) {
@@ -1081,7 +1092,7 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_damage_vp_kern
                 for (int a = 0; a < 8; ++a) {
                     double tt = 0.0;
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) tt += hq[q] * G.Nq[q * 8 + a];
+                    for (int q = 0; q < 8; ++q) tt += hq[q] * sN[q * 8 + a];
                     y[a] += G.ch * tt;
                 }
             }

@@ -202,6 +202,24 @@ __device__ __forceinline__ void fold_macro(const double E[6], double gx[3][2][2]
         }
 }
 
+// The 8 Gauss-point values of voxel e from an (E, 8) array: four 16-byte loads when the
+// array is 16-byte aligned (a wave's loads then touch each 64-byte row once per pair of
+// values instead of once per value), else eight 8-byte loads.  The branch is uniform.
+__device__ __forceinline__ void load_qp8(const double *__restrict__ a, int64_t e, double v[8]) {
+    if ((reinterpret_cast<uintptr_t>(a) & 15) == 0) {
+        const double2 *p = reinterpret_cast<const double2 *>(a + e * 8);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double2 t = p[k];
+            v[2 * k] = t.x;
+            v[2 * k + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = a[e * 8 + q];
+    }
+}
+
 // trilinear interpolation of corner scalars to the 8 Gauss points (xi fastest)
 __device__ __forceinline__ void voxel_interp(const Hex8Geo &G, const double d[8], double dq[8]) {
     double ax[2][2][2];  // [ly][lz][ix]

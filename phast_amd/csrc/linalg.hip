@@ -737,7 +737,9 @@ static int mech_solve_impl(const phast_hex8_plan *plan, phast_hex8_mg *mg, const
                            fixed, 0.0, bf);
         hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, diag, (const double *)nullptr, 1.0, df);
         if (x0) hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, freem, x0, (const double *)nullptr, 0.0, x0f);
-        MechCtx ctx{plan, g, d, fixed, n, pat};
+        // the apply leaves the fixed entries of A p as they are: k_mask_dot zeroes them before
+        // anything reads them
+        MechCtx ctx{plan, g, d, nullptr, n, pat};
         rc = pcg_flags(W, mech_apply_cb, &ctx, n, fixed, bf, df, x0f, y, rtol, atol, maxiter, check_every, info, st,
                        mg ? mg_precond_cb : nullptr, mg);
         // x = where(free, y, x_c)
@@ -787,20 +789,9 @@ static int dirichlet_pcg_impl(const phast_hex8_plan *plan, int form, const doubl
     if (rc == PHAST_OK) {
         // reduced rhs = (b - A known)[free]
         hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, 1.0, b, -1.0, W.Ap, fixed, 0.0, rr);
-        struct Masked {
-            DmgCtx *c;
-            const uint8_t *fixed;
-            int64_t n;
-        } mctx{&ctx, fixed, n};
-        auto masked = [](void *c, const double *in, double *out, hipStream_t s) -> int {
-            auto *m = static_cast<Masked *>(c);
-            int r = dmg_apply_cb(m->c, in, out, s);
-            if (r) return r;
-            hipLaunchKernelGGL(k_zero_fixed, dim3(nblk(m->n)), dim3(256), 0, s, m->n, m->fixed, out);
-            PHAST_LAUNCHED();
-            return PHAST_OK;
-        };
-        rc = pcg_stop(W, masked, &mctx, n, fixed, rr, diag, initial, y, tol, max_iter, 4, info, st);
+        // pcg_stop masks the fixed entries of every image it reads (k_axpby_mask, k_mask_dot),
+        // so the plain action serves as the masked one
+        rc = pcg_stop(W, dmg_apply_cb, &ctx, n, fixed, rr, diag, initial, y, tol, max_iter, 4, info, st);
         // solution = known with free entries from y
         if (rc == PHAST_OK)
             hipLaunchKernelGGL(k_where, dim3(nb), dim3(256), 0, st, n, fixed, known, y, 0.0, out);

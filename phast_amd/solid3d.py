@@ -581,12 +581,13 @@ def pfczm_softening_parameters(softening: str):
 
 def default_tiles(shape) -> tuple[int, int, int]:
     """Launch shape of the voxel kernels when the caller names none: 16x14 node tiles in z
-    slabs of kz layers, with kz as large as 8 while the grid still has about 512 blocks (two
-    resident blocks on each of 256 CUs); small cells get thin slabs so that they fill the
-    device.  Any kz gives the same nodal rows, bit for bit."""
+    slabs of 8 layers when that already gives 1024 blocks (four per CU of a 256-CU device),
+    else slabs of 3 layers, the measured best of the mechanics and damage applies at 48^3 and
+    64^3 (thinner slabs add launch-wide parallelism but evaluate more halo and seam layers).
+    Any kz gives the same nodal rows, bit for bit."""
     nx, ny, nz = (int(v) for v in shape)
     tiles_xy = -(-nx // 16) * -(-ny // 14)
-    return 16, 14, max(1, min(8, tiles_xy * nz // 512))
+    return 16, 14, max(1, min(nz, 8 if tiles_xy * nz >= 1024 * 8 else 3))
 
 
 class PeriodicVoxelPhaseFieldOperator:
