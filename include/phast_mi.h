@@ -350,9 +350,11 @@ int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double
                            const double *Gc, const double *lam, const double *mu, const int8_t *driver,
                            double kappa, double ell, phast_hex8_plan **out);
 int phast_hex8_plan_destroy(phast_hex8_plan *plan);
-/* Launch shape of the element apply.  (16,16), (32,8), (16,14), (32,14): voxel-parallel
- * kernel, one thread per voxel of a (tile_x+1) x (tile_y+1) voxel tile (blocks of whole
- * waves: 320, 320, 256, 512 threads), walking kz node layers.
+/* Launch shape of the element apply.  (16,16), (32,8), (16,14), (32,14), (32,6), (64,6):
+ * voxel-parallel kernel, one thread per voxel of a (tile_x+1) x (tile_y+1) voxel tile
+ * (blocks of whole waves: 320, 320, 256, 512, 256, 512 threads), walking kz node layers.
+ * The tiles of more than 256 threads run the isotropic apply without energy sums or crack
+ * closure; the other variants run on the 16x14 tile with the same kz.
  * (16,8) and (8,8): node-column marching kernel
  * walking kz node layers.  Default (16,8,8). */
 int phast_hex8_set_tiles(phast_hex8_plan *plan, int32_t tile_x, int32_t tile_y, int32_t kz);

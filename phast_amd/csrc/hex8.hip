@@ -878,7 +878,17 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
 #define PHAST_L(GM, ISO, EN) hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, GM, ISO, EN>), grid, block, 0, st, pl->geo, A)
 #define PHAST_LC(ISO, EN, CL) \
     hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, 1, ISO, EN, CL>), grid, block, 0, st, pl->geo, A)
-    if (cl != 0 && pl->iso) {  // crack closure: g array (checked by the caller)
+    if constexpr (apply_threads<OX, OY>() > 256) {
+        // large blocks carry only the isotropic apply without energy or closure (the caller
+        // sends the other variants to the 16x14 tile)
+        if (!(pl->iso && !energy && cl == 0)) {
+            set_error("internal: the %dx%d tile runs only the isotropic apply", OX, OY);
+            return PHAST_ERR_ARG;
+        }
+        if (gmode == 0) PHAST_L(0, true, false);
+        else if (gmode == 1) PHAST_L(1, true, false);
+        else PHAST_L(2, true, false);
+    } else if (cl != 0 && pl->iso) {  // crack closure: g array (checked by the caller)
         if (energy) {
             if (cl == 1) PHAST_LC(true, true, 1);
             else PHAST_LC(true, true, 2);
@@ -969,7 +979,16 @@ static int hex8_apply_any(const phast_hex8_plan *pl, const double *x, const doub
             return PHAST_ERR_UNSUPPORTED;
         }
     }
-    if (pl->tx == 16 && pl->ty == 16) rc = launch_apply_t<16, 16>(pl, A, gmode, energy, st, jac, cl);
+    // Blocks of more than 256 threads (16x16, 32x8: 320; 32x14, 64x6: 512) put two waves on a
+    // SIMD, so their kernels get at most 256 registers; only the isotropic apply without the
+    // energy sum and the closure fits that (the others spill 240-670 B per lane).  Those run
+    // on the 16x14 tile with the same slab depth: the nodal rows are the same bits on every
+    // tile, the macro rows differ in rounding only.
+    const bool lean = pl->iso && !energy && cl == 0;
+    const bool big = (pl->tx == 16 && pl->ty == 16) || (pl->tx == 32 && (pl->ty == 8 || pl->ty == 14)) ||
+                     (pl->tx == 64 && pl->ty == 6);
+    if (big && !lean) rc = launch_apply_t<16, 14>(pl, A, gmode, energy, st, jac, cl);
+    else if (pl->tx == 16 && pl->ty == 16) rc = launch_apply_t<16, 16>(pl, A, gmode, energy, st, jac, cl);
     else if (pl->tx == 32 && pl->ty == 8) rc = launch_apply_t<32, 8>(pl, A, gmode, energy, st, jac, cl);
     else if (pl->tx == 16 && pl->ty == 14) rc = launch_apply_t<16, 14>(pl, A, gmode, energy, st, jac, cl);
     else if (pl->tx == 32 && pl->ty == 14) rc = launch_apply_t<32, 14>(pl, A, gmode, energy, st, jac, cl);
