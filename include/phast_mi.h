@@ -410,8 +410,9 @@ int phast_hex8_damage_dirichlet_pcg(const phast_hex8_plan *plan, const double *H
                                     double *info, hipStream_t stream);
 
 /* One primal active-set bookkeeping pass of solve_h8_scalar_box_kkt (box_kkt_h8.py:672-733):
- * grad = scale A(H) cand - b; counts[5] = added lower/upper, released lower/upper, free;
- * with apply != 0 the masks are updated in place. */
+ * grad = scale A(H) cand - b; counts[8] = added lower/upper, released lower/upper, free,
+ * active lower, active upper (of the masks as passed), free after the update; with
+ * apply != 0 the masks are updated in place. */
 int phast_hex8_box_update(const phast_hex8_plan *plan, const double *H, double scale, const double *b,
                           const double *cand, const double *lower, const double *upper, uint8_t *lower_active,
                           uint8_t *upper_active, const uint8_t *prescribed, double kkt_tol, int32_t apply, int64_t n,

@@ -106,6 +106,23 @@ __global__ __launch_bounds__(256) void k_dots(int64_t n, const double *x, const 
     write_parts(part, a, b, bad, red);
 }
 
+// The box-KKT PCG's start: z = r / diag; partials r.z, r.r (k_update<1>'s sums with a step
+// of zero) and the count of non-finite residual entries (k_dots'), in one pass.
+__global__ __launch_bounds__(256) void k_start_stop(int64_t n, const double *r, const double *dg, double *z,
+                                                    double *part) {
+    __shared__ double red[kRed];
+    double rz = 0.0, rr = 0.0, bad = 0.0;
+    GRID_STRIDE(i, n) {
+        const double ri = r[i];
+        const double zi = ri / dg[i];
+        z[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+        if (!isfinite(ri)) bad += 1.0;
+    }
+    write_parts(part, rz, rr, bad, red);
+}
+
 // linalg.py:194-203: x += a p, r -= a Ap, z = r * inv_diag; partials r.z, r.r
 // (mode 1, box-KKT: z = r / diag; partials r.r, r.z).
 template <int MODE>
@@ -269,35 +286,56 @@ __global__ __launch_bounds__(256) void k_final_parts(const double *part, double 
     if (threadIdx.x == 0)
         for (int k = 0; k < nv; ++k) out3[k] = t[k];
 }
+__global__ __launch_bounds__(256) void k_final_parts8(const double *part, double *out8) {
+    __shared__ double red[8 * 4];
+    double t[8];
+    sum_parts_n<8>(part, 0, t, red);
+    if (threadIdx.x < 8) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k == (int)threadIdx.x) v = t[k];
+        out8[threadIdx.x] = v;
+    }
+}
 
 // box-KKT active-set bookkeeping (box_kkt_h8.py:672-685, 731-733):
 //   grad = A cand - b ; add/release masks ; counts of each.
 // states: lower_active, upper_active, prescribed (lb == ub).
+// counts: added lower, added upper, released lower, released upper, free, active lower,
+// active upper (the last three of the masks the pass reads), free after the update.
 __global__ __launch_bounds__(256) void k_box_update(int64_t n, const double *Acand, const double *b, const double *cand,
                                                     const double *lo, const double *up, uint8_t *lact, uint8_t *uact,
                                                     const uint8_t *presc, double kkt_tol, int apply, double *grad,
                                                     double *part) {
-    __shared__ double red[kRed];
-    double cnt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    __shared__ double red[8 * 4];
+    double cnt[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     GRID_STRIDE(i, n) {
         const double g = Acand[i] - b[i];
         grad[i] = g;
-        const bool la = lact[i], ua = uact[i], fixed = la || ua || presc[i];
+        const bool la = lact[i], ua = uact[i], pr = presc[i], fixed = la || ua || pr;
         const bool add_l = !fixed && cand[i] < lo[i];
         const bool add_u = !fixed && cand[i] > up[i];
         const bool rel_l = la && g < -kkt_tol;
         const bool rel_u = ua && g > kkt_tol;
+        const bool la1 = (la || add_l) && !rel_l, ua1 = (ua || add_u) && !rel_u;
         cnt[0] += add_l ? 1.0 : 0.0;
         cnt[1] += add_u ? 1.0 : 0.0;
         cnt[2] += rel_l ? 1.0 : 0.0;
         cnt[3] += rel_u ? 1.0 : 0.0;
         cnt[4] += fixed ? 0.0 : 1.0;
+        cnt[5] += la ? 1.0 : 0.0;
+        cnt[6] += ua ? 1.0 : 0.0;
+        cnt[7] += (apply ? (la1 || ua1 || pr) : fixed) ? 0.0 : 1.0;
         if (apply) {
-            lact[i] = (la || add_l) && !rel_l;
-            uact[i] = (ua || add_u) && !rel_u;
+            lact[i] = la1;
+            uact[i] = ua1;
         }
     }
-    write_parts5(part, cnt, red);
+    block_sum_n<256, 8>(cnt, red);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) part[k * kVecBlocks + blockIdx.x] = cnt[k];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -541,36 +579,34 @@ static int pcg_stop(Work &W, ApplyFn apply, void *ctx, int64_t n, const uint8_t 
     if (int rc = apply(ctx, x, W.Ap, st)) return rc;
     hipLaunchKernelGGL(k_axpby_mask, dim3(nb), dim3(256), 0, st, n, 1.0, rhs, -1.0, W.Ap, fixed, 0.0, W.r);
     PHAST_LAUNCHED();
+    // residual norm, its finiteness and the preconditioned residual (z = r / diag, r.z) in one
+    // pass and one read
     PHAST_TRY0(zero_parts(W, st));
-    hipLaunchKernelGGL(k_dots, dim3(nb), dim3(256), 0, st, n, W.r, (const double *)nullptr, W.part);
+    hipLaunchKernelGGL(k_start_stop, dim3(nb), dim3(256), 0, st, n, W.r, W.dg, W.z, W.part);
     PHAST_LAUNCHED();
     double s3[3];
-    if (int rc = read3(W, st, s3)) return rc;
+    if (int rc = read3(W, st, s3)) return rc;  // [r.z, r.r, non-finite count]
     if (s3[2] > 0.0) {
         set_error("PCG action returned a non-finite vector");
         return PHAST_ERR_NONFINITE;
     }
-    const double initial = std::sqrt(s3[0]);
+    const double initial = std::sqrt(s3[1]);
     const double ref = std::max(initial, 1.0);
     if (initial <= tol * ref) {
         info[0] = 0; info[1] = 1; info[2] = initial; info[3] = initial; info[4] = initial / ref;
         return PHAST_OK;
     }
-    // preconditioned = r / diag ; direction = preconditioned ; rz = r.preconditioned
+    // direction = preconditioned residual; rz = r.preconditioned
     {
-        PcgState h;
-        std::memset(&h, 0, sizeof(h));
-        h.thr2 = tol * ref;
-        *W.hS = h;
-        PHAST_HIP(hipMemcpyAsync(W.S, W.hS, sizeof(PcgState), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL((k_update<1>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.r, W.r, W.dg, W.z, W.S, W.part);
-        PHAST_LAUNCHED();
-        if (int rc = read3(W, st, s3)) return rc;  // [r.z, r.r, -]
         if (!std::isfinite(s3[0]) || s3[0] <= 0.0) {
             set_error("PCG encountered a non-positive initial residual product");
             return PHAST_ERR_NONPOSITIVE;
         }
-        W.hS->rz = s3[0];
+        PcgState h;
+        std::memset(&h, 0, sizeof(h));
+        h.thr2 = tol * ref;
+        h.rz = s3[0];
+        *W.hS = h;
         PHAST_HIP(hipMemcpyAsync(W.S, W.hS, sizeof(PcgState), hipMemcpyHostToDevice, st));
         PHAST_HIP(hipMemcpyAsync(W.p, W.z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     }
@@ -842,9 +878,11 @@ static int box_update_impl(const phast_hex8_plan *plan, int form, const double *
     hipLaunchKernelGGL(k_box_update, dim3(nblk(n)), dim3(256), 0, st, n, W.Ap, b, cand, lower, upper, lower_active,
                        upper_active, prescribed, kkt_tol, apply, grad, W.part);
     PHAST_LAUNCHED();
-    double s5[5];
-    if (int rc = readn(W, st, s5, 5)) return rc;
-    for (int k = 0; k < 5; ++k) counts[k] = (int64_t)s5[k];
+    hipLaunchKernelGGL(k_final_parts8, dim3(1), dim3(256), 0, st, W.part, W.d3);
+    PHAST_LAUNCHED();
+    PHAST_HIP(hipMemcpyAsync(W.h3, W.d3, sizeof(double) * 8, hipMemcpyDeviceToHost, st));
+    PHAST_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 8; ++k) counts[k] = (int64_t)W.h3[k];
     return PHAST_OK;
 }
 

@@ -239,6 +239,29 @@ __global__ void k_scale_copy(int64_t n, const double *w, double s, double *v) {
     if (i < n) v[i] = s * w[i];
 }
 
+// One power-iteration normalisation on the device: |w|^2 and |v|^2 summed over the gb
+// block partials in block order by one thread; out[0] = 1/|w| (the next v = out[0] w),
+// out[1] = |w|/|v| (the estimate); out[2] counts breakdowns (a sum that is not positive or
+// not finite; the next v is then zero and every later step breaks down too), restarted at
+// the first step of a level.
+__global__ void k_power_norm(unsigned gb, const double *part, int first, double *out) {
+    if (threadIdx.x != 0) return;
+    double sw = 0.0, sv = 0.0;
+    for (unsigned k = 0; k < gb; ++k) {
+        sw += part[2 * k];
+        sv += part[2 * k + 1];
+    }
+    const bool bad = !(sw > 0.0) || !(sv > 0.0) || !isfinite(sw);
+    out[0] = bad ? 0.0 : 1.0 / sqrt(sw);
+    out[1] = bad ? 0.0 : sqrt(sw / sv);
+    out[2] = (first ? 0.0 : out[2]) + (bad ? 1.0 : 0.0);
+}
+
+__global__ void k_scale_copy_dev(int64_t n, const double *w, const double *s, double *v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = s[0] * w[i];
+}
+
 // deterministic start vector for the power iteration (zero on the fixed set)
 __global__ void k_power_start(int64_t n, const uint8_t *fixed, double *v) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -277,6 +300,7 @@ struct phast_hex8_mg {
     double omega_l[kMaxLevels] = {};
     double lam_max[kMaxLevels] = {};
     double *hpart = nullptr;             // pinned host partials
+    double *pw = nullptr;                // device power-iteration state: 3 doubles per level
 };
 
 static inline unsigned nb256(int64_t n) { return (unsigned)ceil_div(std::max<int64_t>(n, 1), 256); }
@@ -301,6 +325,7 @@ extern "C" int phast_hex8_mg_destroy(phast_hex8_mg *mg) {
         (void)hipFree(mg->fixed[l]);
     }
     (void)hipFree(mg->macro6);
+    (void)hipFree(mg->pw);
     if (mg->hpart) (void)hipHostFree(mg->hpart);
     delete mg;
     return PHAST_OK;
@@ -330,7 +355,7 @@ extern "C" int phast_hex8_mg_create(const phast_hex8_plan *fine, int64_t pinned,
         return PHAST_ERR_HIP;
     };
     if (hipHostMalloc(&mg->hpart, sizeof(double) * 2 * 512, hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&mg->macro6, 6) != hipSuccess ||
+        hipMalloc(&mg->pw, sizeof(double) * 3 * kMaxLevels) != hipSuccess || hipMalloc(&mg->macro6, 6) != hipSuccess ||
         hipMemcpy(mg->macro6, macro_fixed6, 6, hipMemcpyHostToDevice) != hipSuccess)
         return fail("alloc");
     int64_t pin = pinned;
@@ -409,34 +434,34 @@ extern "C" int phast_hex8_mg_update(phast_hex8_mg *mg, const double *g, const do
         rc = phast_hex8_mech_diagonal(c, nullptr, mg->diag[l], st);
     }
     if (gtmp) (void)hipFreeAsync(gtmp, st);
-    // lambda_max(D^-1 A) per level by 12 power iterations -> Jacobi weights
+    // lambda_max(D^-1 A) per level by 12 power iterations -> Jacobi weights; the
+    // normalisations run on the device and the estimates are read once at the end
     for (int l = 0; l < mg->levels && rc == PHAST_OK; ++l) {
         const int64_t nm = 3 * mg->n[l] + 6;
         const unsigned gb = (unsigned)std::min<int64_t>(512, nb256(nm));
         const double *g = l == 0 ? mg->g : nullptr;
         const double *d = l == 0 ? mg->d : nullptr;
+        double *pw = mg->pw + 3 * l;
         hipLaunchKernelGGL(k_power_start, dim3(nb256(nm)), dim3(256), 0, st, nm, mg->fixed[l], mg->x[l]);
-        double lam = 0.0;
         for (int it = 0; it < 12 && rc == PHAST_OK; ++it) {
             rc = hex8_apply(mg->lv[l], mg->x[l], g, d, mg->Ax[l], nullptr, false, st);
             if (rc) break;
             hipLaunchKernelGGL(k_power_step, dim3(gb), dim3(256), 0, st, nm, mg->Ax[l], mg->diag[l], mg->fixed[l],
                                mg->x[l], mg->r[l], mg->b[l]);
-            PHAST_HIP(hipMemcpyAsync(mg->hpart, mg->b[l], sizeof(double) * 2 * gb, hipMemcpyDeviceToHost, st));
-            PHAST_HIP(hipStreamSynchronize(st));
-            double sw = 0.0, sv = 0.0;
-            for (unsigned k = 0; k < gb; ++k) {
-                sw += mg->hpart[2 * k];
-                sv += mg->hpart[2 * k + 1];
-            }
-            if (!(sw > 0.0) || !(sv > 0.0) || !std::isfinite(sw)) {
-                set_error("multigrid power iteration broke down on level %d", l);
-                return PHAST_ERR_NONFINITE;
-            }
-            lam = std::sqrt(sw / sv);
-            hipLaunchKernelGGL(k_scale_copy, dim3(nb256(nm)), dim3(256), 0, st, nm, mg->r[l], 1.0 / std::sqrt(sw),
-                               mg->x[l]);
+            hipLaunchKernelGGL(k_power_norm, dim3(1), dim3(64), 0, st, gb, mg->b[l], it == 0 ? 1 : 0, pw);
+            hipLaunchKernelGGL(k_scale_copy_dev, dim3(nb256(nm)), dim3(256), 0, st, nm, mg->r[l], pw, mg->x[l]);
+            PHAST_LAUNCHED();
         }
+    }
+    if (rc != PHAST_OK) return rc;
+    PHAST_HIP(hipMemcpyAsync(mg->hpart, mg->pw, sizeof(double) * 3 * mg->levels, hipMemcpyDeviceToHost, st));
+    PHAST_HIP(hipStreamSynchronize(st));
+    for (int l = 0; l < mg->levels; ++l) {
+        if (mg->hpart[3 * l + 2] != 0.0) {
+            set_error("multigrid power iteration broke down on level %d", l);
+            return PHAST_ERR_NONFINITE;
+        }
+        const double lam = mg->hpart[3 * l + 1];
         mg->lam_max[l] = lam;
         mg->omega_l[l] = mg->omega / (1.05 * lam);
     }

@@ -1379,8 +1379,10 @@ class H8ScalarBoxKKTConvergenceError(RuntimeError):
 
 
 def solve_h8_scalar_dirichlet_pcg(system: H8ScalarSPDSystem, *, fixed, values, initial=None,
-                                  tolerance=DEFAULT_LINEAR_TOLERANCE, max_iterations=4000) -> ScalarDirichletResult:
-    """Exact-Dirichlet PCG of the damage system on the device (box_kkt_h8.py:350-432)."""
+                                  tolerance=DEFAULT_LINEAR_TOLERANCE, max_iterations=4000,
+                                  _n_free: int | None = None) -> ScalarDirichletResult:
+    """Exact-Dirichlet PCG of the damage system on the device (box_kkt_h8.py:350-432).
+    ``_n_free`` (internal): the number of free entries when the caller already counted them."""
     op = system._op
     if op is None:
         raise NotImplementedError("the device solve needs a system built by PeriodicVoxelPhaseFieldOperator")
@@ -1388,7 +1390,7 @@ def solve_h8_scalar_dirichlet_pcg(system: H8ScalarSPDSystem, *, fixed, values, i
     fx = op._check_mask(fixed, n)
     values = op._check("values", values, (n,))
     init = None if initial is None else op._check("initial", initial, (n,))
-    n_free = int(n - fixed.sum().item())
+    n_free = int(n - fixed.sum().item()) if _n_free is None else int(_n_free)
     if n_free == 0:
         known = torch.where(fixed, values, torch.zeros_like(values))
         z = torch.empty(0, dtype=F64, device=op.device)
@@ -1456,17 +1458,19 @@ def solve_h8_scalar_box_kkt(system: H8ScalarSPDSystem, *, lower_bound=0.0, upper
     cand_init = initial
     trace, total = [], 0
     grad = torch.empty(n, dtype=F64, device=dev)
-    counts = (C.c_int64 * 5)()
+    counts = (C.c_int64 * 8)()
     last = None
     cand = None
+    n_pr = int(presc.sum())
+    free_next = n - n_pr  # free entries of the next solve (the update pass counts them)
     for it in range(1, max_active_set_iterations + 1):
         fixed = lact | uact | presc
         values = torch.where(uact, upper, lower)
         last = solve_h8_scalar_dirichlet_pcg(system, fixed=fixed, values=values, initial=cand_init,
-                                             tolerance=linear_tolerance, max_iterations=max_linear_iterations)
+                                             tolerance=linear_tolerance, max_iterations=max_linear_iterations,
+                                             _n_free=free_next)
         cand = last.solution
         total += last.krylov.iterations
-        n_la, n_ua, n_pr = (int(v) for v in torch.stack([lact.sum(), uact.sum(), presc.sum()]).cpu())
         apply = 1 if it < max_active_set_iterations else 0
         la8, ua8 = lact.view(torch.uint8).clone(), uact.view(torch.uint8).clone()
         upd = _L.phast_hex8_pfczm_box_update if system._form == 1 else _L.phast_hex8_box_update
@@ -1474,7 +1478,7 @@ def solve_h8_scalar_box_kkt(system: H8ScalarSPDSystem, *, lower_bound=0.0, upper
                  ptr(la8), ptr(ua8), ptr(presc.view(torch.uint8)), float(kkt_tolerance), apply, n, ptr(grad), counts,
                  stream())
         check(rc, "box_update")
-        add_l, add_u, rel_l, rel_u, n_free = (int(c) for c in counts)
+        add_l, add_u, rel_l, rel_u, n_free, n_la, n_ua, free_next = (int(c) for c in counts)
         trace.append(BoxKKTIteration(it, n_la, n_ua, n_pr, n_free, last.krylov.iterations, last.krylov.converged,
                                      add_l, add_u, rel_l, rel_u))
         if not last.krylov.converged:
