@@ -229,6 +229,93 @@ __global__ __launch_bounds__(256) void k_scalar_stop_b(PcgState *S, const double
     }
 }
 
+// The box-KKT PCG's scalar steps fused into its two vector passes.  Every block sums the
+// partials of the previous pass in the scalar kernels' order (so it takes the same decision
+// from the same bits) and block 0 records the state; a field block 0 writes in a pass is
+// not one the other blocks read there (S->done aside: a block that already sees it set has
+// the same decision to follow).  The latest r.z alternates between S->rz and S->rz_new:
+// iteration k (k = S->steps in the update pass, S->iter - 1 in the direction pass) reads
+// slot k & 1 and the direction pass writes slot (k + 1) & 1.  The update pass writes its
+// partials to rows 4, 5 of `part`, since rows 0, 1 are still being read.
+__global__ __launch_bounds__(256) void k_update_stop(int64_t n, double *x, double *r, const double *p,
+                                                     const double *Ap, const double *dg, double *z, PcgState *S,
+                                                     double *part) {
+    __shared__ double red[kRed];
+    __shared__ int sdone;  // one reading per block, so that the whole block takes one branch
+    if (threadIdx.x == 0) sdone = S->done;
+    __syncthreads();
+    if (sdone) return;
+    const double rz = (S->steps & 1) ? S->rz_new : S->rz;
+    double t[2];
+    sum_parts_n<2>(part, 0, t, red);  // k_scalar_stop_a's sums
+    const double den = t[0], bad = t[1];
+    const bool fail_bad = bad > 0.0, fail_pd = !isfinite(den) || den <= 0.0;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        S->iter += 1;
+        if (fail_bad) {
+            S->done = 1;
+            S->status = PS_NONFINITE;
+        } else if (fail_pd) {
+            S->done = 1;
+            S->status = PS_NOT_PD;
+        } else {
+            S->pAp = den;
+            S->alpha = rz / den;
+        }
+    }
+    if (fail_bad || fail_pd) return;
+    const double a = rz / den;
+    double rzn = 0.0, rr = 0.0;
+    GRID_STRIDE(i, n) {
+        x[i] = x[i] + a * p[i];
+        const double ri = r[i] - a * Ap[i];
+        r[i] = ri;
+        const double zi = ri / dg[i];
+        z[i] = zi;
+        rzn += ri * zi;
+        rr += ri * ri;
+    }
+    write_parts(part + 4 * kVecBlocks, rzn, rr, 0.0, red);
+}
+__global__ __launch_bounds__(256) void k_pupdate_stop(int64_t n, double *p, const double *z, PcgState *S,
+                                                      const double *part, int32_t max_iter) {
+    __shared__ double red[kRed];
+    __shared__ int sdone;
+    if (threadIdx.x == 0) sdone = S->done;
+    __syncthreads();
+    if (sdone) return;
+    const int k = S->iter - 1;
+    const double rz_old = (k & 1) ? S->rz_new : S->rz;
+    double t[2];
+    sum_parts_n<2>(part, 4, t, red);  // k_scalar_stop_b's sums
+    const double rz_new = t[0], rr = t[1];
+    const bool conv = sqrt(rr) <= S->thr2;  // thr2 holds tol * reference_norm here
+    const bool nonpos = !conv && (!isfinite(rz_new) || rz_new <= 0.0);
+    const double b = rz_new / rz_old;
+    const bool maxed = !conv && !nonpos && S->iter >= max_iter;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        S->rr = rr;
+        if (conv) {
+            S->done = 1;
+            S->status = PS_CONVERGED;
+        } else if (nonpos) {
+            S->done = 1;
+            S->status = PS_NONPOS_RZ;
+        } else {
+            S->beta = b;
+            if (k & 1) S->rz = rz_new;
+            else S->rz_new = rz_new;
+            S->steps = k + 1;
+            if (maxed) {
+                S->done = 1;
+                S->status = PS_MAXITER;
+            }
+        }
+    }
+    if (conv || nonpos || maxed) return;
+    GRID_STRIDE(i, n) p[i] = z[i] + b * p[i];
+}
+
 // partials r.z, r.r, count(!finite(z)) after an external preconditioner
 __global__ __launch_bounds__(256) void k_rz(int64_t n, const double *r, const double *z, const PcgState *S,
                                             double *part) {
@@ -615,13 +702,9 @@ static int pcg_stop(Work &W, ApplyFn apply, void *ctx, int64_t n, const uint8_t 
         if (int rc = apply(ctx, W.p, W.Ap, st)) return rc;
         hipLaunchKernelGGL(k_mask_dot, dim3(nb), dim3(256), 0, st, n, W.p, W.Ap, fixed, W.S, W.part);
         PHAST_LAUNCHED();
-        hipLaunchKernelGGL(k_scalar_stop_a, dim3(1), dim3(256), 0, st, W.S, W.part);
+        hipLaunchKernelGGL(k_update_stop, dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S, W.part);
         PHAST_LAUNCHED();
-        hipLaunchKernelGGL((k_update<1>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S, W.part);
-        PHAST_LAUNCHED();
-        hipLaunchKernelGGL(k_scalar_stop_b, dim3(1), dim3(256), 0, st, W.S, W.part, max_iter);
-        PHAST_LAUNCHED();
-        hipLaunchKernelGGL(k_pupdate, dim3(nb), dim3(256), 0, st, n, W.p, W.z, W.S);
+        hipLaunchKernelGGL(k_pupdate_stop, dim3(nb), dim3(256), 0, st, n, W.p, W.z, W.S, W.part, max_iter);
         PHAST_LAUNCHED();
         ++it;
         if (it % poll == 0 || it == max_iter) {
