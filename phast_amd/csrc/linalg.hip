@@ -21,7 +21,8 @@ constexpr int kVecBlocks = 1024;  // grid of the vector kernels (grid-stride)
 
 struct PcgState {
     double rz, rz_new, pAp, alpha, beta, rr, thr2, b_norm;
-    int32_t iter, done, active, breakdown, nonfinite, steps, status, pad;
+    int32_t iter, done, active, breakdown, nonfinite, steps, status;
+    int32_t phase;  // fused linalg.pcg passes: the latest r.z is rz (phase even) or rz_new (odd)
 };
 
 enum : int { PS_RUNNING = 0, PS_CONVERGED = 1, PS_NOT_PD = 2, PS_NONPOS_RZ = 3, PS_NONFINITE = 4, PS_MAXITER = 5 };
@@ -179,6 +180,74 @@ __global__ __launch_bounds__(256) void k_scalar_flags_b(PcgState *S, const doubl
     S->active = S->active && (rr > S->thr2);
     S->beta = (S->active && S->rz > 0.0) ? rz_new / S->rz : 0.0;
     S->rz = rz_new;
+}
+
+// linalg.pcg's scalar steps fused into its vector passes, as k_update_stop / k_pupdate_stop
+// do for the box-KKT PCG: every block sums the partials in k_scalar_flags_a/b's order and
+// block 0 records the flags.  The flags block 0 writes are not read by the other blocks of
+// the same pass, except `active`, which only ever falls to 0 and then falls for every
+// block's own reading too.  S->phase counts the direction passes and equals S->iter at the
+// start of an iteration.  MODE 0: z = r * inv_diag with partials r.z, r.r to rows 4, 5;
+// MODE 2: r only (an external preconditioner and k_rz follow).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_update_flags(int64_t n, double *x, double *r, const double *p,
+                                                      const double *Ap, const double *dg, double *z, PcgState *S,
+                                                      double *part) {
+    __shared__ double red[kRed];
+    double t[2];
+    sum_parts_n<2>(part, 0, t, red);  // k_scalar_flags_a's sums
+    const double pAp = t[0], bad = t[1];
+    const int active = S->active;
+    const double rz = (S->phase & 1) ? S->rz_new : S->rz;
+    const bool positive = pAp > 0.0;
+    const bool step = active && positive;
+    const double a = step ? rz / pAp : 0.0;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (active && !positive) S->breakdown = 1;
+        if (!isfinite(pAp) || bad > 0.0) S->nonfinite = 1;
+        S->pAp = pAp;
+        S->alpha = a;
+        S->steps += step ? 1 : 0;
+        S->iter += 1;
+    }
+    double rzn = 0.0, rr = 0.0;
+    GRID_STRIDE(i, n) {
+        x[i] = x[i] + a * p[i];
+        const double ri = r[i] - a * Ap[i];
+        r[i] = ri;
+        if (MODE == 0) {
+            const double zi = ri * dg[i];
+            z[i] = zi;
+            rzn += ri * zi;
+        }
+        rr += ri * ri;
+    }
+    if (MODE == 0) write_parts(part + 4 * kVecBlocks, rzn, rr, 0.0, red);
+}
+// k_scalar_flags_b fused into the direction update p = z + beta p; the r.z, r.r partials
+// are in rows `row`, `row + 1`.
+__global__ __launch_bounds__(256) void k_pupdate_flags(int64_t n, double *p, const double *z, PcgState *S,
+                                                       const double *part, int row) {
+    __shared__ double red[kRed];
+    double t[2];
+    sum_parts_n<2>(part, row, t, red);  // k_scalar_flags_b's sums
+    const double rz_new = t[0], rr = t[1];
+    // this iteration's phase, from iter (which only the update pass writes): block 0 writes
+    // S->phase here, so no block may read it in this pass
+    const int ph = S->iter - 1;
+    const double rz_old = (ph & 1) ? S->rz_new : S->rz;
+    const bool active = S->active && (rr > S->thr2);
+    const double b = (active && rz_old > 0.0) ? rz_new / rz_old : 0.0;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (!isfinite(rz_new) || !isfinite(rr)) S->nonfinite = 1;
+        S->rr = rr;
+        S->active = active;
+        S->beta = b;
+        if (ph & 1) S->rz = rz_new;
+        else S->rz_new = rz_new;
+        S->phase = ph + 1;
+    }
+    GRID_STRIDE(i, n) p[i] = z[i] + b * p[i];
 }
 
 // box_kkt_h8._pcg scalar steps (stop at first convergence; breakdowns end the loop)
@@ -606,21 +675,22 @@ static int pcg_flags(Work &W, ApplyFn apply, void *ctx, int64_t n, const uint8_t
         hipLaunchKernelGGL(k_mask_dot, dim3(nb), dim3(256), 0, st, n, W.p, W.Ap, fixed, (const PcgState *)nullptr,
                            W.part);
         PHAST_LAUNCHED();
-        hipLaunchKernelGGL(k_scalar_flags_a, dim3(1), dim3(256), 0, st, W.S, W.part);
-        PHAST_LAUNCHED();
-        // x, r, z update (alpha = 0 for inactive iterations leaves x and r unchanged)
+        // x, r, z update with the step the partials give (alpha = 0 for inactive iterations
+        // leaves x and r unchanged), then the direction
         if (precond) {
-            hipLaunchKernelGGL((k_update<2>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S, W.part);
+            hipLaunchKernelGGL((k_update_flags<2>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S,
+                               W.part);
             PHAST_LAUNCHED();
             if (int rc = precond(pctx, W.r, W.z, st)) return rc;
             hipLaunchKernelGGL(k_rz, dim3(nb), dim3(256), 0, st, n, W.r, W.z, (const PcgState *)nullptr, W.part);
+            PHAST_LAUNCHED();
+            hipLaunchKernelGGL(k_pupdate_flags, dim3(nb), dim3(256), 0, st, n, W.p, W.z, W.S, W.part, 0);
         } else {
-            hipLaunchKernelGGL((k_update<0>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S, W.part);
+            hipLaunchKernelGGL((k_update_flags<0>), dim3(nb), dim3(256), 0, st, n, x, W.r, W.p, W.Ap, W.dg, W.z, W.S,
+                               W.part);
+            PHAST_LAUNCHED();
+            hipLaunchKernelGGL(k_pupdate_flags, dim3(nb), dim3(256), 0, st, n, W.p, W.z, W.S, W.part, 4);
         }
-        PHAST_LAUNCHED();
-        hipLaunchKernelGGL(k_scalar_flags_b, dim3(1), dim3(256), 0, st, W.S, W.part);
-        PHAST_LAUNCHED();
-        hipLaunchKernelGGL(k_pupdate, dim3(nb), dim3(256), 0, st, n, W.p, W.z, W.S);
         PHAST_LAUNCHED();
         if (it % check_every == 0 || it == maxiter) {
             PHAST_HIP(hipMemcpyAsync(W.hS, W.S, sizeof(PcgState), hipMemcpyDeviceToHost, st));
