@@ -417,6 +417,13 @@ int phast_hex8_box_update(const phast_hex8_plan *plan, const double *H, double s
                           const double *cand, const double *lower, const double *upper, uint8_t *lower_active,
                           uint8_t *upper_active, const uint8_t *prescribed, double kkt_tol, int32_t apply, int64_t n,
                           double *grad, int64_t *counts, hipStream_t stream);
+/* The KKT audit of a box-constrained solution (box_kkt_h8.py:459-558) in one pass: out8
+ * (host) = max |grad| on free entries, max |projected gradient|, max complementarity
+ * |mult (bound gap)|, counts of lower / upper sign violations, of entries below lower and
+ * above upper, and of bounds with lower > upper.  grad = A sol - b (device). */
+int phast_hex8_kkt_stats(const phast_hex8_plan *plan, const double *grad, const double *sol, const double *lower,
+                         const double *upper, const uint8_t *lower_active, const uint8_t *upper_active,
+                         const uint8_t *prescribed, double kkt_tol, int64_t n, double *out8, hipStream_t stream);
 
 /* Geometric multigrid on the periodic voxel hierarchy (no reference counterpart; the
  * reference's 3D solves are Jacobi-PCG only, solid3d/model.py:782).  Coarse levels halve

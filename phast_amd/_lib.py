@@ -163,8 +163,13 @@ def ptr(t: torch.Tensor | None) -> int | None:
 
 
 def stream(device: torch.device | None = None) -> int:
-    """Raw hipStream_t of torch's current stream."""
-    return torch.cuda.current_stream(device).cuda_stream
+    """Raw hipStream_t of torch's current stream (on ``device``, default the current device).
+    Read straight from torch's stream registry: this runs before every library call, and
+    ``torch.cuda.current_stream`` builds a Stream object after several device queries."""
+    idx = torch._C._cuda_getDevice() if device is None else torch.device(device).index
+    if idx is None:
+        idx = torch._C._cuda_getDevice()
+    return torch._C._cuda_getCurrentRawStream(idx)
 
 
 def require_device() -> torch.device:

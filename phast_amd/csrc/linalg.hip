@@ -494,6 +494,65 @@ __global__ __launch_bounds__(256) void k_box_update(int64_t n, const double *Aca
         for (int k = 0; k < 8; ++k) part[k * kVecBlocks + blockIdx.x] = cnt[k];
 }
 
+// KKT audit of a box solution in one pass (box_kkt_h8.py:459-558): per block the maxima of
+// |grad| on free entries, of the projected gradient and of the two complementarity products,
+// and the counts of sign violations and bound violations.  The maxima propagate NaN, as
+// torch's max does.
+__device__ __forceinline__ double nan_max(double a, double b) { return (a != a || b != b) ? NAN : fmax(a, b); }
+
+__global__ __launch_bounds__(256) void k_kkt_stats(int64_t n, const double *grad, const double *sol,
+                                                   const double *lo, const double *up, const uint8_t *lact,
+                                                   const uint8_t *uact, const uint8_t *presc, double kkt_tol,
+                                                   double *part) {
+    __shared__ double red[8 * 4];
+    double mx[3] = {0.0, 0.0, 0.0}, cnt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    GRID_STRIDE(i, n) {
+        const double g = grad[i], x = sol[i], l = lo[i], u = up[i];
+        const bool la = lact[i], ua = uact[i], fr = !(la || ua || presc[i]);
+        double proj = fr ? g : 0.0;
+        if (la) proj = fmin(g, 0.0);
+        if (ua) proj = fmax(g, 0.0);
+        if (la && g != g) proj = g;  // keep a NaN gradient visible
+        if (ua && g != g) proj = g;
+        const double lm = la ? (g != g ? g : fmax(g, 0.0)) : 0.0;
+        const double um = ua ? (g != g ? g : fmax(-g, 0.0)) : 0.0;
+        mx[0] = nan_max(mx[0], fr ? fabs(g) : 0.0);
+        mx[1] = nan_max(mx[1], fabs(proj));
+        mx[2] = nan_max(mx[2], nan_max(fabs(lm * (x - l)), fabs(um * (u - x))));
+        cnt[0] += (la && g < -kkt_tol) ? 1.0 : 0.0;
+        cnt[1] += (ua && -g < -kkt_tol) ? 1.0 : 0.0;
+        cnt[2] += x < l ? 1.0 : 0.0;
+        cnt[3] += x > u ? 1.0 : 0.0;
+        cnt[4] += l > u ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) mx[k] = nan_max(mx[k], __shfl_xor(mx[k], o, kWave));
+    block_sum_n<256, 5>(cnt, red);  // its barriers also order the LDS use below
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) red[k * 4 + w] = mx[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            part[k * kVecBlocks + blockIdx.x] = nan_max(nan_max(red[k * 4], red[k * 4 + 1]),
+                                                        nan_max(red[k * 4 + 2], red[k * 4 + 3]));
+#pragma unroll
+        for (int k = 0; k < 5; ++k) part[(3 + k) * kVecBlocks + blockIdx.x] = cnt[k];
+    }
+}
+__global__ __launch_bounds__(256) void k_kkt_final(const double *part, int nb, double *out8) {
+    if (threadIdx.x >= 8) return;
+    const int k = threadIdx.x;
+    double v = 0.0;
+    for (int b = 0; b < nb; ++b) v = k < 3 ? nan_max(v, part[k * kVecBlocks + b]) : v + part[k * kVecBlocks + b];
+    out8[k] = v;
+}
+
 // ---------------------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------------------
@@ -1036,6 +1095,30 @@ static int box_update_impl(const phast_hex8_plan *plan, int form, const double *
     PHAST_HIP(hipMemcpyAsync(W.h3, W.d3, sizeof(double) * 8, hipMemcpyDeviceToHost, st));
     PHAST_HIP(hipStreamSynchronize(st));
     for (int k = 0; k < 8; ++k) counts[k] = (int64_t)W.h3[k];
+    return PHAST_OK;
+}
+
+// The KKT audit of a box solution: out8 (host) = max |grad| on free entries, max projected
+// gradient, max complementarity, lower / upper sign violations, entries below lower, above
+// upper, bounds with lower > upper (box_kkt_h8.py:459-558).
+extern "C" int phast_hex8_kkt_stats(const phast_hex8_plan *plan, const double *grad, const double *sol,
+                                    const double *lower, const double *upper, const uint8_t *lower_active,
+                                    const uint8_t *upper_active, const uint8_t *prescribed, double kkt_tol, int64_t n,
+                                    double *out8, hipStream_t st) {
+    PHAST_REQUIRE(plan && grad && sol && lower && upper && lower_active && upper_active && prescribed && out8 && n >= 1,
+                  "phast_hex8_kkt_stats: null argument");
+    Work *Wp = nullptr;
+    if (int rc = plan_work(plan, 3 * n + 6, &Wp)) return rc;
+    Work &W = *Wp;
+    const int nb = nblk(n);
+    hipLaunchKernelGGL(k_kkt_stats, dim3(nb), dim3(256), 0, st, n, grad, sol, lower, upper, lower_active, upper_active,
+                       prescribed, kkt_tol, W.part);
+    PHAST_LAUNCHED();
+    hipLaunchKernelGGL(k_kkt_final, dim3(1), dim3(64), 0, st, W.part, nb, W.d3);
+    PHAST_LAUNCHED();
+    PHAST_HIP(hipMemcpyAsync(W.h3, W.d3, sizeof(double) * 8, hipMemcpyDeviceToHost, st));
+    PHAST_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 8; ++k) out8[k] = W.h3[k];
     return PHAST_OK;
 }
 

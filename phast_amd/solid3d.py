@@ -39,6 +39,8 @@ for _name, _args in {
                                         C.c_void_p],
     "phast_hex8_box_update": [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7 +
                              [C.c_double, C.c_int32, C.c_int64, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
+    "phast_hex8_kkt_stats": [C.c_void_p] + [C.c_void_p] * 7 + [C.c_double, C.c_int64, C.POINTER(C.c_double),
+                                                              C.c_void_p],
     "phast_hex8_solver_reserve": [C.c_void_p],
     "phast_hex8_solver_release": [C.c_void_p],
     "phast_hex8_pfczm_setup": [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double],
@@ -553,9 +555,13 @@ class H8ScalarSPDSystem:
             raise TypeError("action must be callable")
         if self.rhs.ndim != 1 or self.rhs.numel() == 0:
             raise ValueError("rhs must be a non-empty one-dimensional tensor")
-        if not bool(torch.isfinite(self.rhs).all()):
+        if self.diagonal.shape != self.rhs.shape:
+            raise ValueError("diagonal must be strictly positive")
+        finite, positive = (bool(v) for v in torch.stack((torch.isfinite(self.rhs).all(),
+                                                          (self.diagonal > 0.0).all())).cpu())
+        if not finite:
             raise ValueError("rhs contains a non-finite value")
-        if self.diagonal.shape != self.rhs.shape or not bool((self.diagonal > 0.0).all()):
+        if not positive:
             raise ValueError("diagonal must be strictly positive")
 
     @property
@@ -1407,9 +1413,24 @@ def solve_h8_scalar_dirichlet_pcg(system: H8ScalarSPDSystem, *, fixed, values, i
     return ScalarDirichletResult(out, krylov, n_free)
 
 
-def _kkt_diagnostics(system, solution, lower, upper, lact, uact, presc, kkt_tol) -> BoxKKTDiagnostics:
-    """Audit of box_kkt_h8.py:459-558 (the quantities its ``satisfies_kkt`` depends on)."""
-    grad = system.apply(solution) - system.rhs
+def _kkt_diagnostics(system, solution, lower, upper, lact, uact, presc, kkt_tol, grad=None) -> BoxKKTDiagnostics:
+    """Audit of box_kkt_h8.py:459-558 (the quantities its ``satisfies_kkt`` depends on), in
+    one device pass (``phast_hex8_kkt_stats``) for a system built by the voxel operator.
+    ``grad`` (optional) is A solution - b when the caller already has it."""
+    if grad is None:
+        grad = system.apply(solution) - system.rhs
+    op = system._op
+    if op is not None:
+        n = system.n_dofs
+        out = (C.c_double * 8)()
+        u8 = lambda m: m.contiguous().view(torch.uint8)  # noqa: E731
+        check(_L.phast_hex8_kkt_stats(op._plan, ptr(grad.contiguous()), ptr(solution.contiguous()),
+                                      ptr(lower.contiguous()), ptr(upper.contiguous()), ptr(u8(lact)), ptr(u8(uact)),
+                                      ptr(u8(presc)), float(kkt_tol), n, out, stream()), "kkt_stats")
+        stats = list(out)
+        ok = (stats[5] == 0 and stats[6] == 0 and stats[7] == 0 and stats[3] == 0 and stats[4] == 0
+              and stats[0] <= kkt_tol and stats[1] <= kkt_tol and stats[2] <= kkt_tol)
+        return BoxKKTDiagnostics(kkt_tol, stats[0], stats[1], stats[2], int(stats[3]), int(stats[4]), bool(ok))
     fixed = lact | uact | presc
     free = ~fixed
     zero = torch.zeros_like(grad)
@@ -1484,8 +1505,8 @@ def solve_h8_scalar_box_kkt(system: H8ScalarSPDSystem, *, lower_bound=0.0, upper
         if not last.krylov.converged:
             raise H8ScalarBoxKKTConvergenceError("H8 scalar box-KKT PCG did not converge", last_solution=cand,
                                                  last_linear_solve=last, evidence=(it, total, tuple(trace)))
-        if add_l + add_u + rel_l + rel_u == 0:
-            kkt = _kkt_diagnostics(system, cand, lower, upper, lact, uact, presc, kkt_tolerance)
+        if add_l + add_u + rel_l + rel_u == 0:  # the update pass left grad = A cand - b for these masks
+            kkt = _kkt_diagnostics(system, cand, lower, upper, lact, uact, presc, kkt_tolerance, grad=grad)
             if not kkt.satisfies_kkt:
                 raise H8ScalarBoxKKTConvergenceError("H8 scalar box-KKT active set reached a non-KKT state",
                                                      last_solution=cand, last_linear_solve=last,

@@ -1,6 +1,7 @@
 """Parity of the periodic hex8 HIP path with the oracle, and the reference's recorded
 pore-in-cube elastic rows (GPU).  Fields within 1e-10 relative (max-norm); masks exact."""
 import json
+import math
 from pathlib import Path
 
 import numpy as np
@@ -196,6 +197,55 @@ def test_box_kkt(dev):
     assert np.array_equal(res.lower_active.cpu().numpy(), ola)
     assert np.array_equal(res.upper_active.cpu().numpy(), oua)
     assert res.active_set_iterations == its
+
+
+def _kkt_stats_numpy(grad, sol, lo, up, la, ua, pr, tol):
+    """box_kkt_h8.py:459-558's audit quantities, restated with numpy (NaN-propagating max)."""
+    free = ~(la | ua | pr)
+    proj = np.where(free, grad, 0.0)
+    proj = np.where(la, np.minimum(grad, 0.0), proj)
+    proj = np.where(ua, np.maximum(grad, 0.0), proj)
+    lm = np.where(la, np.maximum(grad, 0.0), 0.0)
+    um = np.where(ua, np.maximum(-grad, 0.0), 0.0)
+    comp = float(np.maximum(np.max(np.abs(lm * (sol - lo))), np.max(np.abs(um * (up - sol)))))
+    return (np.max(np.abs(np.where(free, grad, 0.0))), np.max(np.abs(proj)), comp,
+            int((la & (grad < -tol)).sum()), int((ua & (-grad < -tol)).sum()))
+
+
+def test_kkt_audit_one_pass(dev):
+    """The one-pass device audit (phast_hex8_kkt_stats) against the numpy restatement on
+    random states: every mask combination, sign violations, bound violations, and a NaN
+    gradient entry, which must fail the audit as torch's max would."""
+    S, op, oop = build((8, 6, 7))
+    rng = np.random.default_rng(21)
+    n = op.n_red
+    H = T(rng.uniform(0, 4e-3, (op.n_elements, 8)), dev)
+    sysm = op.damage_system(H)
+    for trial in range(4):
+        sol = rng.uniform(-0.1, 1.1, n)
+        lo, up = np.clip(rng.uniform(-0.2, 0.5, n), 0, 1), np.ones(n)
+        la, ua = rng.uniform(0, 1, n) < 0.3, rng.uniform(0, 1, n) < 0.2
+        ua &= ~la
+        pr = rng.uniform(0, 1, n) < 0.05
+        grad = rng.normal(0, 1e-8 if trial % 2 else 1e-3, n)
+        if trial == 3:
+            grad[17] = np.nan
+        tt = lambda a: T(a, dev)  # noqa: E731
+        tb = lambda a: torch.as_tensor(a, device=dev)  # noqa: E731
+        d = S._kkt_diagnostics(sysm, tt(sol), tt(lo), tt(up), tb(la), tb(ua), tb(pr), 1e-9, grad=tt(grad))
+        fg, pm, cm, lsv, usv = _kkt_stats_numpy(grad, sol, lo, up, la, ua, pr, 1e-9)
+        same = lambda a, b: a == b or (math.isnan(a) and math.isnan(b))  # noqa: E731
+        assert same(d.free_stationarity_maximum, fg) and same(d.projected_gradient_maximum, pm)
+        assert same(d.complementarity_maximum, cm)
+        if trial == 3:  # the NaN reaches at least one maximum, and the audit fails
+            assert math.isnan(fg) or math.isnan(pm) or math.isnan(cm)
+        assert (d.lower_sign_violation_dofs, d.upper_sign_violation_dofs) == (lsv, usv)
+        assert d.satisfies_kkt is False  # random states violate the bounds
+    x = T(np.full(n, 0.5), dev)  # an interior point with zero gradient satisfies the audit
+    z = torch.zeros(n, dtype=torch.bool, device=dev)
+    d = S._kkt_diagnostics(sysm, x, T(np.zeros(n), dev), T(np.ones(n), dev), z, z, z, 1e-9,
+                           grad=torch.zeros(n, dtype=torch.float64, device=dev))
+    assert d.satisfies_kkt and d.free_stationarity_maximum == 0.0
 
 
 def test_pore_in_cube_recorded_elastic_rows(dev):
