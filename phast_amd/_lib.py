@@ -162,14 +162,19 @@ def ptr(t: torch.Tensor | None) -> int | None:
     return t.data_ptr()
 
 
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_CUR_DEVICE = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def stream(device: torch.device | None = None) -> int:
     """Raw hipStream_t of torch's current stream (on ``device``, default the current device).
-    Read straight from torch's stream registry: this runs before every library call, and
-    ``torch.cuda.current_stream`` builds a Stream object after several device queries."""
-    idx = torch._C._cuda_getDevice() if device is None else torch.device(device).index
-    if idx is None:
-        idx = torch._C._cuda_getDevice()
-    return torch._C._cuda_getCurrentRawStream(idx)
+    Read straight from torch's stream registry where torch offers it: this runs before every
+    library call, and ``torch.cuda.current_stream`` builds a Stream object after several
+    device queries."""
+    if _RAW_STREAM is None or _CUR_DEVICE is None:
+        return torch.cuda.current_stream(device).cuda_stream
+    idx = None if device is None else torch.device(device).index
+    return _RAW_STREAM(_CUR_DEVICE() if idx is None else idx)
 
 
 def require_device() -> torch.device:
