@@ -253,23 +253,28 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
     // in-layer offsets of this thread's node slots; a slot past the tile's NN nodes loads
     // node (0,0) of the layer into LDS padding, so every lane loads and stores without
     // branching (which keeps the compiler's wait counts exact across the loop)
-    int64_t slot_ij[NS];
+    // Addresses are a uniform layer base (scalar registers) plus a 32-bit byte offset per
+    // lane within the layer (24 nx ny < 2^32, checked at plan creation), so the loads and
+    // stores need no 64-bit vector address arithmetic per layer.
+    uint32_t slot_ij[NS];
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
         const int s = t + m * NT;
-        slot_ij[m] = s < NN ? wrapm(i0 - 1 + s % NNX, nx) + (int64_t)nx * wrapm(j0 - 1 + s / NNX, ny) : 0;
+        slot_ij[m] = s < NN ? (uint32_t)(wrapm(i0 - 1 + s % NNX, nx) + nx * wrapm(j0 - 1 + s / NNX, ny)) : 0u;
     }
     // node layer k of this thread's slots, fetched into registers (issued one layer ahead)
     double pf[NS][NF];
     auto fetch = [&](int k, double (&dst)[NS][NF]) {
         const int64_t base = nxy * wrap(k, nz);
+        const char *xb = reinterpret_cast<const char *>(A.x + 3 * base);
+        const char *db = reinterpret_cast<const char *>(A.d + base);
 #pragma unroll
         for (int m = 0; m < NS; ++m) {
-            const int64_t node = base + slot_ij[m];
-            dst[m][0] = A.x[3 * node];
-            dst[m][1] = A.x[3 * node + 1];
-            dst[m][2] = A.x[3 * node + 2];
-            if (GMODE == 2) dst[m][NF - 1] = A.d[node];
+            const double *px = reinterpret_cast<const double *>(xb + 24u * slot_ij[m]);
+            dst[m][0] = px[0];
+            dst[m][1] = px[1];
+            dst[m][2] = px[2];
+            if (GMODE == 2) dst[m][NF - 1] = *reinterpret_cast<const double *>(db + 8u * slot_ij[m]);
         }
     };
     int slot_lds[NS];  // LDS slot of each node slot (NN for the padding lanes)
@@ -285,22 +290,24 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
     double sE[6] = {0, 0, 0, 0, 0, 0}, en = 0.0;
     double acc[3] = {0.0, 0.0, 0.0};
     const int s00 = t - NVX - 1, s10 = t - NVX, s01 = t - 1;  // voxels left-below, below, left
+    const uint32_t y_off = owner ? (uint32_t)(iv + nx * jv) : 0u;  // owned node within its layer
     // voxel material of layer k (lam, mu or the bank index; with the closure, the closure
     // flag in place of the bank index and the pattern word), fetched one layer ahead
     double ml0 = 0.0, ml1 = 0.0;
     int32_t mb = 0;
     uint32_t mp = 0u;
+    const uint32_t e_off = (uint32_t)e_ij;
     auto fetch_mat = [&](int k, double &l0, double &l1, int32_t &b, uint32_t &pw) {
-        const int64_t e = e_ij + nxy * wrap(k, nz);
+        const int64_t base = nxy * wrap(k, nz);
         if (ISO) {
-            l0 = A.lam[e];
-            l1 = A.mu[e];
-            if (CL != 0) b = A.clm[e];
-            if (CL == 2) pw = A.pat[e];
+            l0 = (A.lam + base)[e_off];
+            l1 = (A.mu + base)[e_off];
+            if (CL != 0) b = (A.clm + base)[e_off];
+            if (CL == 2) pw = (A.pat + base)[e_off];
         } else {
-            b = A.bank_index[e];
-            if (CL != 0) l0 = A.clm[e] != 0 ? 1.0 : 0.0;
-            if (CL == 2) pw = A.pat[e];
+            b = (A.bank_index + base)[e_off];
+            if (CL != 0) l0 = (A.clm + base)[e_off] != 0 ? 1.0 : 0.0;
+            if (CL == 2) pw = (A.pat + base)[e_off];
         }
     };
     fetch_mat(k0, ml0, ml1, mb, mp);
@@ -383,11 +390,11 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
         settle(ml0, ml1, mb);
         if (CL == 2) settle_u(mp);
         if (owner) {
-            const int64_t node = iv + (int64_t)nx * (jv + (int64_t)ny * kv);
+            double *yl = A.y + 3 * (nxy * kv) + 3u * y_off;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const double bot = ((st[6 + c][s00] + st[3 + c][s10]) + st[c][s01]) + r[c][0];
-                A.y[3 * node + c] = kv == k0 ? bot : acc[c] + bot;  // first layer: seam added later
+                yl[c] = kv == k0 ? bot : acc[c] + bot;  // first layer: seam added later
                 acc[c] = ((st[15 + c][s00] + st[12 + c][s10]) + st[9 + c][s01]) + r[c][4];
             }
         }
@@ -1236,6 +1243,8 @@ extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double
     PHAST_REQUIRE(out && bank && bank_index && Gc && lam && mu && driver, "phast_hex8_plan_create: null argument");
     PHAST_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "periodic voxel cell needs at least 2 voxels per axis");
     PHAST_REQUIRE(nz <= 65535, "periodic voxel cell: at most 65535 voxel layers along z (one grid row per layer)");
+    PHAST_REQUIRE(24 * (int64_t)nx * ny < ((int64_t)1 << 32),
+                  "periodic voxel cell: 24 nx ny bytes must fit 32-bit offsets within a node layer");
     PHAST_REQUIRE(hx > 0 && hy > 0 && hz > 0 && ell > 0 && n_bank >= 1, "bad geometry / bank size");
     auto *pl = new phast_hex8_plan();
     std::memset(&pl->geo, 0, sizeof(pl->geo));
