@@ -119,6 +119,16 @@ __device__ __forceinline__ void sum_block_partials(const double *part, int nbloc
                     if (k < nvals) s[k] += v[u][k];
     }
 }
+// Tile worked by block b of an nblocks-block launch, in an XCD-aware order.  The dispatcher
+// hands consecutive workgroups to the 8 XCDs in turn (block b on XCD b % 8), each XCD with
+// its own L2.  Block b takes tile (b % 8) q + b / 8 with q = nblocks / 8, so each XCD works
+// through one contiguous run of tiles and the halo lines neighbouring tiles share are
+// fetched into one L2; the last nblocks % 8 blocks keep their own index.  The map is a
+// bijection whatever the dispatch order: that order only sets the speed.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+    const int q = nblocks >> 3;
+    return b < (q << 3) ? (b & 7) * q + (b >> 3) : b;
+}
 // One block sum (block_sum's arithmetic, same bits) and two block minima sharing one pair of
 // barriers.  `sh` holds 3 * NT/64 doubles; results are valid in every thread.
 template <int NT>

@@ -52,7 +52,8 @@ struct MechArgs3 {
 // The kernel boundary orders the partials before that sum: no fence, counter or atomic.
 // `red` holds 7 * NT/64 doubles.
 template <int NT, bool ENERGY>
-__device__ __forceinline__ void macro_epilogue(const MechArgs3 &A, const double sE[6], double en, double *red) {
+__device__ __forceinline__ void macro_epilogue(const MechArgs3 &A, const double sE[6], double en, double *red,
+                                               int blk) {
     constexpr int K = ENERGY ? 7 : 6;
     double v[K];
 #pragma unroll
@@ -61,7 +62,7 @@ __device__ __forceinline__ void macro_epilogue(const MechArgs3 &A, const double 
     block_sum_n<NT, K>(v, red);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < K; ++k) A.part[8 * blockIdx.x + k] = v[k];
+        for (int k = 0; k < K; ++k) A.part[8 * blk + k] = v[k];
 }
 
 template <int TX, int TY, int GMODE, bool ISO, bool ENERGY>  // GMODE 0: g=1, 1: g array, 2: g from d
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(TX *TY) void hex8_mech_kernel(const Hex8Geo G, Mech
         }
         cur ^= 1;
     }
-    macro_epilogue<NT, ENERGY>(A, sE, en, red);
+    macro_epilogue<NT, ENERGY>(A, sE, en, red, blockIdx.x);
 }
 
 
@@ -235,7 +236,8 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
 
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const int tiles = A.tiles_x * A.tiles_y;
-    const int bxy = blockIdx.x % tiles, bz = blockIdx.x / tiles;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);  // tile index; partials go to part[8 blk + k]
+    const int bxy = blk % tiles, bz = blk / tiles;
     const int i0 = (bxy % A.tiles_x) * OX, j0 = (bxy / A.tiles_x) * OY;
     const int k0 = bz * A.kz, k1 = min(k0 + A.kz, nz);
     const int t = threadIdx.x;
@@ -406,7 +408,7 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
 #pragma unroll
         for (int c = 0; c < 3; ++c) A.seam[at + c] = acc[c];
     }
-    macro_epilogue<NT, ENERGY>(A, sE, en, red);
+    macro_epilogue<NT, ENERGY>(A, sE, en, red, blk);
 }
 
 // Sum the block partials of the macro rows in a fixed order (block order per row, all
@@ -1044,7 +1046,8 @@ __global__ __launch_bounds__((apply_threads<OX, OY>())) void hex8_damage_vp_kern
     stage_voxel_tables(G, sK, sN);
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const int tiles = A.tiles_x * A.tiles_y;
-    const int bxy = blockIdx.x % tiles, bz = blockIdx.x / tiles;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);  // tile index of this block
+    const int bxy = blk % tiles, bz = blk / tiles;
     const int i0 = (bxy % A.tiles_x) * OX, j0 = (bxy / A.tiles_x) * OY;
     const int k0 = bz * A.kz, k1 = min(k0 + A.kz, nz);
     const int t = threadIdx.x;

@@ -205,8 +205,8 @@ struct Corners {
 // Fill the block's LDS with the corner values of its patch's triangles; corners(t, c, owner)
 // fills the Corners of triangle t.  Every thread of the block must call it.
 template <class F>
-__device__ __forceinline__ void t3_patch_fill(const T3Patch &P, double *lds, F corners) {
-    const int32_t j0 = P.pel_ptr[blockIdx.x], nel = P.pel_ptr[blockIdx.x + 1] - j0;
+__device__ __forceinline__ void t3_patch_fill(const T3Patch &P, double *lds, F corners, int blk) {
+    const int32_t j0 = P.pel_ptr[blk], nel = P.pel_ptr[blk + 1] - j0;
     for (int32_t j = threadIdx.x; j < nel; j += 256) {
         int4 t = P.pel[j0 + j];
         const bool owner = t.x < 0;
@@ -322,8 +322,9 @@ __global__ __launch_bounds__(256) void t3_force_kernel(const T3Mesh m, const T3P
     auto corners = [&](int4 t, Corners &c, bool owner) {
         t3_force_corners<(MODE >= 1)>(m, M, ux, d, H_old, H_new, psi, t, c, owner);
     };
-    if (SRC == SRC_PATCH) t3_patch_fill(P, lds, corners);
-    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);  // patch of this block
+    if (SRC == SRC_PATCH) t3_patch_fill(P, lds, corners, blk);
+    const int64_t slot = (int64_t)blk * 256 + threadIdx.x;
     if (slot >= m.n_nodes) return;
     const int64_t n = SRC == SRC_PATCH ? (int64_t)P.pnode[slot] : slot;
     double fx, fy;
@@ -403,8 +404,9 @@ __global__ __launch_bounds__(256) void t3_damage_kernel(const T3Mesh m, const T3
                                                         const double *Gc, const double *x, double *y, double *y2) {
     extern __shared__ double lds[];
     auto corners = [&](int4 t, Corners &c, bool) { t3_damage_corners<(OP == 0 ? 0 : 1)>(m, M, H, Gc, x, t, c); };
-    if (SRC == SRC_PATCH) t3_patch_fill(P, lds, corners);
-    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);  // patch of this block
+    if (SRC == SRC_PATCH) t3_patch_fill(P, lds, corners, blk);
+    const int64_t slot = (int64_t)blk * 256 + threadIdx.x;
     if (slot >= m.n_nodes) return;
     const int64_t n = SRC == SRC_PATCH ? (int64_t)P.pnode[slot] : slot;
     double s0, s1;
@@ -797,8 +799,9 @@ __global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Patc
     constexpr double INF = __builtin_huge_val();
     const double *x = (OP == T_CG_AP || OP == T_PC_AP) ? A.p : A.d;
     auto corners = [&](int4 t, Corners &c, bool) { t3_damage_corners<0>(m, M, A.H, A.Gc, x, t, c); };
-    if (SRC == SRC_PATCH) t3_patch_fill(P, lds, corners);
-    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);  // patch of this block
+    if (SRC == SRC_PATCH) t3_patch_fill(P, lds, corners, blk);
+    const int64_t slot = (int64_t)blk * 256 + threadIdx.x;
     double p0 = 0.0, p1 = (OP == T_CG_AP) ? INF : 0.0, p2 = (OP == T_CG_AP) ? INF : 0.0;
     if (slot < m.n_nodes) {
         const int64_t n = SRC == SRC_PATCH ? (int64_t)P.pnode[slot] : slot;
@@ -858,8 +861,8 @@ __global__ __launch_bounds__(256) void t3_cg_kernel(const T3Mesh m, const T3Patc
     }
     if (OP == T_PC_RESID) return;
     // per-block partials; the reductions of one pass share their barriers (same bits as one
-    // block_sum / block_min each)
-    const int b = blockIdx.x;
+    // block_sum / block_min each), indexed by patch
+    const int b = blk;
     if (OP == T_CG_AP) {
         block_sum_min2<256>(p0, p1, p2, red);
         if (threadIdx.x == 0) {
