@@ -43,7 +43,9 @@ struct MechArgs3 {
     const uint8_t *clm;    // crack closure: 1 where the voxel adds (1 - g) sigma_c back
     const uint32_t *pat;   // crack closure: fixed sign pattern, bit 4 q + k per voxel
     const double *cltab;   // crack closure, general banks: rows and moduli per bank entry (28)
-    int32_t kz, tiles_x, tiles_y, nvals;
+    const uint8_t *phase;  // isotropic cells of few phases: phase of each voxel, or null
+    const double *ptab;    // (lam, mu) of each phase
+    int32_t kz, tiles_x, tiles_y, nvals, n_phase;
 };
 
 // End of a mechanics apply: the block's macro-row (and energy) partials, all reduced
@@ -219,8 +221,10 @@ constexpr int apply_min_waves() {
     return (apply_threads<OX, OY>() <= 256 && ISO && !ENERGY) ? 2 : 1;
 }
 
-// CL: crack closure off (0), at each point's own sign pattern (1) or at A.pat (2)
-template <int OX, int OY, int GMODE, bool ISO, bool ENERGY, int CL = 0>
+// CL: crack closure off (0), at each point's own sign pattern (1) or at A.pat (2).
+// PH: the voxel's (lam, mu) from its 1-byte phase and the plan's phase table (held in LDS)
+// instead of the two per-voxel arrays (isotropic cells without closure).
+template <int OX, int OY, int GMODE, bool ISO, bool ENERGY, int CL = 0, bool PH = false>
 __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY, ISO, ENERGY>())) void hex8_apply_kernel(
     const Hex8Geo G, MechArgs3 A) {
     constexpr int NT = apply_threads<OX, OY>();
@@ -229,8 +233,10 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
     constexpr int NF = (GMODE == 2) ? 4 : 3;
     constexpr int NS = (NN + NT - 1) / NT;  // node slots loaded per thread
     static_assert(NV <= NT, "voxel tile larger than the block");
+    static_assert(!PH || (ISO && CL == 0), "the phase table serves the isotropic apply without closure");
     // node layers: NN slots and one dummy slot that lanes past the tile's nodes write to
     __shared__ double su[2][NF][NN + 1];
+    __shared__ double sph[PH ? 2 * kHex8MaxPhases : 1];  // (lam, mu) per phase
     __shared__ double st[18][NV];  // corners (1,0), (0,1), (1,1) of both faces, 3 components each
     __shared__ double red[7 * (NT / 64)];
 
@@ -301,7 +307,9 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
     const uint32_t e_off = (uint32_t)e_ij;
     auto fetch_mat = [&](int k, double &l0, double &l1, int32_t &b, uint32_t &pw) {
         const int64_t base = nxy * wrap(k, nz);
-        if (ISO) {
+        if (PH) {
+            b = (A.phase + base)[e_off];
+        } else if (ISO) {
             l0 = (A.lam + base)[e_off];
             l1 = (A.mu + base)[e_off];
             if (CL != 0) b = (A.clm + base)[e_off];
@@ -312,6 +320,8 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
             if (CL == 2) pw = (A.pat + base)[e_off];
         }
     };
+    if (PH)  // visible after the first barrier of the layer loop
+        for (int i = t; i < 2 * A.n_phase; i += NT) sph[i] = A.ptab[i];
     fetch_mat(k0, ml0, ml1, mb, mp);
     {  // the first two node layers, both in flight before either is staged
         double pf1[NS][NF];
@@ -333,6 +343,10 @@ __global__ __launch_bounds__((apply_threads<OX, OY>()), (apply_min_waves<OX, OY,
         uint32_t np = 0u;
         if (kv + 1 < k1) fetch_mat(kv + 1, nl0, nl1, nb, np);
         double r[3][8];
+        if (PH) {
+            ml0 = sph[2 * mb];
+            ml1 = sph[2 * mb + 1];
+        }
         if (has_voxel) {
             const int64_t e = e_ij + nxy * wrap(kv, nz);
             double u[3][8], g[8];
@@ -885,6 +899,9 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
     A.seam = pl->seam;
     dim3 grid(nb), block(apply_threads<OX, OY>());
 #define PHAST_L(GM, ISO, EN) hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, GM, ISO, EN>), grid, block, 0, st, pl->geo, A)
+#define PHAST_LP(GM) \
+    hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, GM, true, false, 0, true>), grid, block, 0, st, pl->geo, A)
+    const bool ph = A.phase != nullptr && !energy && cl == 0;
 #define PHAST_LC(ISO, EN, CL) \
     hipLaunchKernelGGL((hex8_apply_kernel<OX, OY, 1, ISO, EN, CL>), grid, block, 0, st, pl->geo, A)
     if constexpr (apply_threads<OX, OY>() > 256) {
@@ -894,7 +911,11 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
             set_error("internal: the %dx%d tile runs only the isotropic apply", OX, OY);
             return PHAST_ERR_ARG;
         }
-        if (gmode == 0) PHAST_L(0, true, false);
+        if (ph) {
+            if (gmode == 0) PHAST_LP(0);
+            else if (gmode == 1) PHAST_LP(1);
+            else PHAST_LP(2);
+        } else if (gmode == 0) PHAST_L(0, true, false);
         else if (gmode == 1) PHAST_L(1, true, false);
         else PHAST_L(2, true, false);
     } else if (cl != 0 && pl->iso) {  // crack closure: g array (checked by the caller)
@@ -918,6 +939,10 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
             if (gmode == 0) PHAST_L(0, true, true);
             else if (gmode == 1) PHAST_L(1, true, true);
             else PHAST_L(2, true, true);
+        } else if (ph) {
+            if (gmode == 0) PHAST_LP(0);
+            else if (gmode == 1) PHAST_LP(1);
+            else PHAST_LP(2);
         } else {
             if (gmode == 0) PHAST_L(0, true, false);
             else if (gmode == 1) PHAST_L(1, true, false);
@@ -935,6 +960,7 @@ static int launch_apply_t(const phast_hex8_plan *pl, MechArgs3 A, int gmode, boo
         }
     }
 #undef PHAST_L
+#undef PHAST_LP
 #undef PHAST_LC
     PHAST_LAUNCHED();
     const unsigned layer_blocks = (unsigned)(1 + ceil_div(3 * (int64_t)pl->geo.nx * pl->geo.ny, 256));
@@ -960,6 +986,9 @@ static int hex8_apply_any(const phast_hex8_plan *pl, const double *x, const doub
     A.bank_index = pl->bank_index;
     A.lam = pl->lam;
     A.mu = pl->mu;
+    A.phase = pl->phase;
+    A.ptab = pl->ptab;
+    A.n_phase = pl->n_phase;
     A.y = y;
     A.part = pl->part;
     A.y6 = out7 ? nullptr : y + 3 * pl->n;
@@ -1239,6 +1268,14 @@ void phast::voxel_constants(Hex8Geo &G, double hx, double hy, double hz, double 
 }
 
 
+// equal bit patterns (a phase groups voxels whose moduli are the same doubles)
+static bool same_bits(double a, double b) {
+    uint64_t x, y;
+    std::memcpy(&x, &a, sizeof x);
+    std::memcpy(&y, &b, sizeof y);
+    return x == y;
+}
+
 extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,
                                       const double *bank, int32_t n_bank, const int32_t *bank_index,
                                       const double *Gc, const double *lam, const double *mu, const int8_t *driver,
@@ -1308,6 +1345,40 @@ extern "C" int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double
         phast_hex8_plan_destroy(pl);
         return rc;
     }
+    if (iso) {  // few distinct (lam, mu) pairs (bitwise): a phase index per voxel and a table
+        std::vector<uint8_t> ph((size_t)pl->n);
+        std::vector<double> tab;
+        int last = -1;
+        for (int64_t e = 0; e < pl->n; ++e) {
+            const double l = lam[e], m = mu[e];
+            int p = -1;
+            if (last >= 0 && same_bits(tab[2 * last], l) && same_bits(tab[2 * last + 1], m)) p = last;
+            for (int q = 0; p < 0 && q < (int)tab.size() / 2; ++q)
+                if (same_bits(tab[2 * q], l) && same_bits(tab[2 * q + 1], m)) p = q;
+            if (p < 0) {
+                if ((int)tab.size() / 2 == kHex8MaxPhases) {
+                    tab.clear();
+                    break;
+                }
+                p = (int)tab.size() / 2;
+                tab.push_back(l);
+                tab.push_back(m);
+            }
+            ph[(size_t)e] = (uint8_t)p;
+            last = p;
+        }
+        if (!tab.empty()) {
+            if (hipMalloc(&pl->phase, (size_t)pl->n) != hipSuccess ||
+                hipMalloc(&pl->ptab, sizeof(double) * tab.size()) != hipSuccess ||
+                hipMemcpy(pl->phase, ph.data(), (size_t)pl->n, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(pl->ptab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                set_error("phast_hex8_plan_create: device allocation/copy failed");
+                phast_hex8_plan_destroy(pl);
+                return PHAST_ERR_HIP;
+            }
+            pl->n_phase = (int32_t)(tab.size() / 2);
+        }
+    }
     *out = pl;
     return PHAST_OK;
 }
@@ -1322,6 +1393,8 @@ extern "C" int phast_hex8_plan_destroy(phast_hex8_plan *pl) {
     }
     (void)hipFree(pl->lam);
     (void)hipFree(pl->mu);
+    (void)hipFree(pl->phase);
+    (void)hipFree(pl->ptab);
     (void)hipFree(pl->part);
     (void)hipFree(pl->seam);
     (void)hipFree(pl->a1);

@@ -5,6 +5,9 @@
 
 namespace phast {
 
+// Most distinct (lam, mu) pairs the voxel-parallel apply reads through a 1-byte phase index.
+constexpr int kHex8MaxPhases = 256;
+
 struct Hex8Geo {
     int32_t nx, ny, nz, pad;
     double ihx, ihy, ihz;   // 1/h
@@ -43,6 +46,11 @@ struct phast_hex8_plan {
     int8_t *driver = nullptr;
     int32_t n_bank = 0;
     bool iso = false;   // every bank entry isotropic: stress from (lam, mu) per voxel
+    // isotropic cells with at most kHex8MaxPhases distinct (lam, mu) pairs: the phase of each
+    // voxel (1 byte) and the pairs, read by the voxel-parallel apply instead of lam and mu
+    uint8_t *phase = nullptr;
+    double *ptab = nullptr;
+    int32_t n_phase = 0;
     bool owns_material = true;
     double *part = nullptr;  // block partials (8 per block)
     double *seam = nullptr;  // voxel-parallel apply: slab seams (seam_cap doubles)

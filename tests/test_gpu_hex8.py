@@ -88,6 +88,46 @@ def test_voxel_parallel_apply_matches_marching_apply(dev, tiles):
     assert rel(op.mech_action(x, g), oop.mech_action(xo, go)) < RTOL
 
 
+@pytest.mark.parametrize("n_moduli", [256, 257, 0])
+def test_isotropic_moduli_per_voxel(dev, n_moduli):
+    """Isotropic cells with n_moduli distinct (E, nu) pairs drawn per voxel (0: every voxel
+    its own): up to 256 pairs the voxel-parallel apply reads a 1-byte phase and a table,
+    beyond that the per-voxel lam/mu arrays.  Either way the nodal rows equal the marching
+    kernel's bitwise (same doubles, same arithmetic) and the oracle's to RTOL."""
+    import dataclasses
+    from phast_amd import solid3d as S
+    from phast_amd import geometry as G
+    shape, h = (20, 19, 14), (1e-3, 1.1e-3, 0.9e-3)
+    n = shape[0] * shape[1] * shape[2]
+    rng = np.random.default_rng(8)
+    k = n if n_moduli == 0 else n_moduli
+    E, nu = rng.uniform(2e3, 8e4, k), rng.uniform(0.2, 0.4, k)
+    idx = np.arange(n) if n_moduli == 0 else np.concatenate([np.arange(k), rng.integers(0, k, n - k)])
+    bank = np.stack([S.isotropic_stiffness_3d(a, b) for a, b in zip(E, nu)])
+    base = S.build_element_arrays(np.full((shape[2], shape[1], shape[0]), G.MATRIX, np.uint8),
+                                  {"mi_matrix": S.PhaseMaterial("mi_matrix", 3000.0, 0.35, 5e-4)})
+    mats = dataclasses.replace(base, stiffness_bank=bank, stiffness_index=idx.astype(np.int64),
+                               lam=bank[idx, 0, 1].copy(), mu=bank[idx, 3, 3].copy())
+    assert len({(a, b) for a, b in zip(mats.lam, mats.mu)}) == k
+    load = S.LoadCase("uni", 0, (1e-3,), (1, 2, 3, 4, 5), ())
+    op = S.PeriodicVoxelPhaseFieldOperator(S.VoxelGrid(shape, h), mats, load, kappa=1e-6, ell=2e-3, tiles=(16, 8, 8))
+    oop = O.VoxelOperator(shape, h, mats.stiffness_bank, mats.stiffness_index, mats.Gc, mats.lam, mats.mu,
+                          mats.driver.astype(np.int64), kappa=1e-6, ell=2e-3, pinned_node=op.pinned_node,
+                          controlled=0)
+    x = T(rng.uniform(-1e-6, 1e-6, op.n_mech), dev)
+    x[-6:] = T(rng.uniform(-1e-3, 1e-3, 6), dev)
+    d = T(rng.uniform(0, 1, op.n_red), dev)
+    g = op.degradation(d)
+    ref, ref_d, ref_1 = op.mech_action(x, g), op.mech_action_d(x, d), op.mech_action(x)
+    m = 3 * n
+    for tiles in [(16, 14, 8), (16, 14, 3), (32, 8, 5)]:
+        op.set_tiles(*tiles)
+        y, y_d, y_1 = op.mech_action(x, g), op.mech_action_d(x, d), op.mech_action(x)
+        assert torch.equal(y[:m], ref[:m]) and torch.equal(y_d[:m], ref_d[:m]) and torch.equal(y_1[:m], ref_1[:m])
+        assert rel(y[m:], ref[m:]) < 1e-13
+    assert rel(op.mech_action(x, g), oop.mech_action(x.cpu().numpy(), g.cpu().numpy())) < RTOL
+
+
 @pytest.mark.parametrize("tiles", [(16, 8, 8), (32, 14, 3), (16, 14, 6), (64, 6, 5)])
 def test_general_stiffness_bank(dev, tiles):
     """Bank entries that are not isotropic take the kernels' general 6x6 stiffness path
