@@ -34,7 +34,7 @@ HEX8_FLOPS_PER_VOXEL = 940.0    # sum-factorised apply
 # HBM bytes per 128^3 apply (the apply kernel plus its seam/finalize kernel) measured with
 # rocprofv3 (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel
 # factors), per launch shape; see profiles/hex8_apply_128cube_particles.md
-HEX8_128_TRAFFIC = {(16, 14, 8): ((108138.8 + 6204.6) * 2.0 + 55376.0 + 6144.1) * 1024.0}
+HEX8_128_TRAFFIC = {(16, 14, 8): ((40650.1 + 6200.7) * 2.0 + 55376.0 + 6144.1) * 1024.0}
 Q4_APPLY_BYTES_PER_NODE = 40.0  # u 16 + d 8 + f 16
 Q4_STEP_BYTES_PER_NODE = 192.0  # u,v,a r+w 96, f_ext 16, Minv 8, d 8, H r+w 64
 # one projected damage-CG iteration without a bound hit or residual check (per node): A p with
