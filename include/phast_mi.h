@@ -344,7 +344,10 @@ typedef struct phast_hex8_plan phast_hex8_plan;
  * stiffnesses; bank_index: host (E,) int32 voxel -> bank entry; Gc, lam, mu: host (E,);
  * driver: host (E,) int8 (0 none, 1 full, 2 spectral; solid3d/operator.py:1415-1444);
  * kappa: residual stiffness; ell: length scale (solid3d/operator.py:229-251).
- * nx, ny, nz >= 2; nz <= 65535. */
+ * nx, ny, nz >= 2; nz <= 65535; 24 nx ny < 2^32 (a node layer is addressed with 32-bit
+ * offsets).  When every bank entry is isotropic and (lam, mu) take at most 256 distinct
+ * pairs (bitwise), the plan also keeps a 1-byte phase per voxel and the pairs, which the
+ * voxel-parallel apply reads instead of lam and mu (same doubles, same results). */
 int phast_hex8_plan_create(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,
                            const double *bank, int32_t n_bank, const int32_t *bank_index,
                            const double *Gc, const double *lam, const double *mu, const int8_t *driver,
